@@ -1,0 +1,156 @@
+"""Benchmark: per-frame foam step at N=10,000, JFA 128^3 (one MI355X).
+
+A step is one frame of the foam (hash grid, PBD, pressure, noise, JFA power
+diagram, FSC extraction, controller) on the uniform-box workload: N=10,000,
+radii U[0.012, 0.024], JFA 128^3 int32 labels, 8 passes, JFA every frame.
+Inputs are resident in HBM before the timed region.
+
+The JSON line also carries:
+  roofline      the JFA pass kernel: algorithmic 8 B per voxel-pass (read and
+                write one int32 label) times 128^3 voxels per launch, over the
+                average launch duration from HIP events on its stream
+  cpu_baseline  the CPU restatement (C, OpenMP over the host cores) timed on
+                a bounded sample of the same frames
+  cadence       the same workload with JFA on a 1/5 cadence after 20 frames
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "fabric-of-space-taichi-jfa_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "frames/s at N=10k,128³ JFA; Gvox-pass/s + %HBM; surrogate vs plain @FSC≤2%"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-frames", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cadence", action="store_true")
+    return ap.parse_args()
+
+
+def run_frames(sim, frames):
+    for _ in range(frames):
+        sim.step()
+
+
+def timed(cfg, pos, rad, steps, warmup, with_events):
+    import torch
+
+    import fos_lib as fl
+    from foam import FoamSim
+    sim = FoamSim(cfg, pos, rad)
+    run_frames(sim, warmup)
+    torch.cuda.synchronize()
+    P = cfg.passes()
+    ev_arrays = []
+    if with_events:
+        for _ in range(steps):
+            arr = (ctypes.c_void_p * (P + 1))()
+            fl.call("fos_events_create", P + 1, arr)
+            ev_arrays.append(arr)
+        sim.pass_events = list(ev_arrays)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_frames(sim, steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    pass_ms = []
+    for arr in ev_arrays:
+        ms = (ctypes.c_float * P)()
+        fl.call("fos_events_elapsed", P + 1, arr, ms)
+        pass_ms.append(list(ms))
+        fl.call("fos_events_destroy", P + 1, arr)
+    return t1 - t0, pass_ms, sim
+
+
+def cpu_baseline(cfg, pos, rad, frames):
+    from oracle import oracle as O
+    from oracle.frame import OracleFoam
+    threads = len(os.sched_getaffinity(0))
+    O.set_threads(threads)
+    ref = OracleFoam(cfg, pos, rad)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        ref.step()
+    dt = time.perf_counter() - t0
+    return {"value": round(frames / dt, 4), "unit": "frames/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {frames} frames of the same N=10k/128^3 workload (JFA every "
+                      f"frame), C restatement of the reference loops, OpenMP on {threads} "
+                      f"host threads, {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the foam step is single-device in this build)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import foam_config as fc
+    import foam_inputs as fi
+
+    cfg = fc.preset("c2")
+    pos, rad = fi.for_config(cfg)
+    V = cfg.res ** 3
+    P = cfg.passes()
+    elapsed, pass_ms, sim = timed(cfg, pos, rad, args.steps, args.warmup, True)
+    fps = args.steps / elapsed
+    flat = np.array(pass_ms, dtype=np.float64)       # [frames, passes] in ms
+    avg_pass_s = float(flat.mean()) / 1e3
+    achieved = 8.0 * V / avg_pass_s / 1e9            # GB/s
+    gvox = P * V / (float(flat.sum(axis=1).mean()) / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "uniform box N=10000, radii U[0.012,0.024], JFA 128^3 int32, "
+                               "8 passes, JFA every frame, PBD<=12, pressure x20, "
+                               "Gaussian Brownian 1e-4",
+                   "n": cfg.n, "jfa_res": cfg.res, "jfa_passes": P, "hash_res": cfg.hash_res(),
+                   "frames": args.steps},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_jfa_step",
+                     "avg_launch_us": round(avg_pass_s * 1e6, 3),
+                     "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)]},
+        "gvox_pass_per_s": round(gvox, 2),
+        "jfa_ms_per_frame": round(float(flat.sum(axis=1).mean()), 4),
+    }
+    if not args.no_cadence:
+        c2c = fc.preset("c2c")
+        el2, _, _ = timed(c2c, pos, rad, args.steps, args.warmup, False)
+        out["cadence"] = {"workload": "same, JFA every 5th frame after a 20-frame warm start",
+                          "value": round(args.steps / el2, 3), "unit": "frames/s"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, pos, rad, args.cpu_frames)
+        out["speedup_vs_cpu"] = round(fps / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

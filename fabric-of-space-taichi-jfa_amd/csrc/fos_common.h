@@ -1,0 +1,184 @@
+// fos_common.h -- device helpers and launch plumbing shared by the HIP
+// translation units of libfos.so.
+//
+// Numerics contract: every fp32 expression below is evaluated in the order
+// the reference writes it, with contraction disabled (the library is built
+// with -ffp-contract=off) and HIP's correctly rounded fp32 division/sqrt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fos.h"
+
+#define FOS_MAX_NEIGHBORS 32
+#define FOS_BLOCK 256
+
+namespace fos {
+
+// Sets the thread-local error text returned by fos_last_error().
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+#define FOS_CHECK_LAUNCH(name)                                   \
+  do {                                                           \
+    hipError_t _e = hipGetLastError();                           \
+    if (_e != hipSuccess) return ::fos::hip_fail(_e, name);      \
+  } while (0)
+
+#define FOS_HIP(call)                                            \
+  do {                                                           \
+    hipError_t _e = (call);                                      \
+    if (_e != hipSuccess) return ::fos::hip_fail(_e, #call);     \
+  } while (0)
+
+#define FOS_REQUIRE(cond, ...)                                   \
+  do {                                                           \
+    if (!(cond)) {                                               \
+      ::fos::set_error(__VA_ARGS__);                             \
+      return FOS_EINVAL;                                         \
+    }                                                            \
+  } while (0)
+
+static inline unsigned blocks_for(int64_t n, int bs = FOS_BLOCK) {
+  return (unsigned)((n + bs - 1) / bs);
+}
+
+// jfa.py:247-251 / grid.py:91-95: ((i % r) + r) % r
+__device__ __forceinline__ int wrapi(int i, int r) { return ((i % r) + r) % r; }
+
+// Wrap for |off| <= r: one conditional add/sub equals the double modulo.
+__device__ __forceinline__ int wrap1(int i, int r) {
+  return i < 0 ? i + r : (i >= r ? i - r : i);
+}
+
+// jfa.py:303-308 wrapP for the power distance: d - L*round(d/L).  With
+// L == 1 the division and product are exact and round/rint differ only at
+// |d| == 0.5, where both candidates have the same square, so the squared
+// distance is bit-identical with rint (one instruction).
+template <bool UNIT_L>
+__device__ __forceinline__ float jwrap(float d, float L) {
+  if (UNIT_L) return d - rintf(d);
+  return d - L * roundf(d / L);
+}
+
+// jfa.py:285-287 voxel_to_world
+__device__ __forceinline__ float vcenter(int i, float voxel, float half_l) {
+  return ((float)i + 0.5f) * voxel - half_l;
+}
+
+// jfa.py:451-477 power_distance_sq with a precomputed seed (x, y, z,
+// (beta*r)*(beta*r)); the dot product sums left to right.
+template <bool UNIT_L>
+__device__ __forceinline__ float power_dist(float cx, float cy, float cz,
+                                            float4 s, float L) {
+  float dx = jwrap<UNIT_L>(cx - s.x, L);
+  float dy = jwrap<UNIT_L>(cy - s.y, L);
+  float dz = jwrap<UNIT_L>(cz - s.z, L);
+  float d2 = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+  d2 = __fadd_rn(d2, __fmul_rn(dz, dz));
+  return __fsub_rn(d2, s.w);
+}
+
+// grid.py:30-48 wrapP: p - L*floor(p*INV_L + 0.5)
+__device__ __forceinline__ float gwrap(float p, float L, float inv_l) {
+  return p - L * floorf(p * inv_l + 0.5f);
+}
+
+// grid.py:202-219 cell of a wrapped coordinate, with the 1e-7 hash epsilon.
+__device__ __forceinline__ int hash_axis(float pw, int G, float half_l,
+                                         float inv_l) {
+  float q = ((pw + half_l) + 1e-7f) * inv_l;
+  return wrapi((int)floorf(q * (float)G), G);
+}
+
+// dynamics.py:101-103 stencil centre: no epsilon, truncating cast, no wrap.
+__device__ __forceinline__ int centre_axis(float p, int G, float L,
+                                           float half_l, float inv_l) {
+  float pw = gwrap(p, L, inv_l);
+  float q = (pw + half_l) * inv_l;
+  return (int)(q * (float)G);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    uint4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+__device__ __forceinline__ float u01(uint32_t b) {
+  return (float)(b >> 8) * 5.9604644775390625e-8f;
+}
+__device__ __forceinline__ float u01_open(uint32_t b) {
+  return (float)((b >> 8) + 1u) * 5.9604644775390625e-8f;
+}
+
+// Block-wide sum of an int into one atomic per block.
+__device__ __forceinline__ void block_add(int v, int *dst) {
+  __shared__ int s_acc;
+  if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) s_acc = 0;
+  __syncthreads();
+  // wave-level reduction first
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((((threadIdx.z * blockDim.y + threadIdx.y) * blockDim.x + threadIdx.x) & 63) == 0 && v)
+    atomicAdd(&s_acc, v);
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0 && s_acc)
+    atomicAdd(dst, s_acc);
+}
+
+}  // namespace fos
+
+// Scratch owned by a context (fos_ctx_create); sized once, never per call.
+struct fos_ctx {
+  int device;
+  int max_n;
+  int max_res;
+  int max_cells;
+  // seed table (x, y, z, (beta*r)^2)
+  float4 *seeds;
+  // face-pair row tables: capacity tab_cap entries per particle row
+  int tab_cap;
+  int *tab_b;
+  unsigned *tab_k;
+  int *tab_c;
+  int *flags;  // [0] row-table overflow, [1] fixed-point changed, [2..] misc
+  // finalize scratch
+  float *score;           // [max_n*32]
+  unsigned char *cls;     // [max_n*32]
+  unsigned char *dec;     // [max_n*32]
+  int *in_count;          // [max_n+1]
+  int *in_start;          // [max_n+1]
+  int *in_fill;           // [max_n]
+  int *in_src;            // [max_n*32]
+  int *acc_in;            // [max_n]
+  // scan scratch
+  int *scan_blocks;       // block partials
+  int scan_blocks_cap;
+  // grid scratch
+  int *cell_write;        // [max_cells]
+  int *cell_of;           // [max_n]
+  float *pos_tmp;         // [max_n*3]
+  float *fscratch;        // [16] small device floats (reductions)
+  int *iscratch;          // [16] small device ints
+  // host pinned mirror for small reads
+  int64_t *host_i;
+  double *host_f;
+};
+
+namespace fos {
+int exclusive_scan(fos_ctx *ctx, const int *in, int *out, int n,
+                   hipStream_t st);
+}

@@ -1,0 +1,657 @@
+// fos_dynamics.hip -- PBD overlap projection, max overlap, FSC controller,
+// pressure equilibration, Brownian/OU noise, XSPH, damping and the topology
+// candidate builder (reference: dynamics.py, topology.py).
+#include <string.h>
+
+#include "fos_common.h"
+
+using namespace fos;
+
+// Walks the 27 hash cells around particle i in the reference order
+// (dx, dy, dz in -1..1, occupants in cell order) and calls f(j).
+template <typename F>
+__device__ __forceinline__ void for_stencil(int cx, int cy, int cz, int G,
+                                            const int32_t *__restrict__ cs,
+                                            const int32_t *__restrict__ cc,
+                                            const int32_t *__restrict__ ci, F f) {
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int nx = wrapi(cx + dx, G);
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int ny = wrapi(cy + dy, G);
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int nz = wrapi(cz + dz, G);
+        const int id = nx * G * G + ny * G + nz;
+        const int st = cs[id], ct = cc[id];
+        for (int k = st; k < st + ct; ++k) f(ci[k]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float dot3(float x, float y, float z) {
+  float d2 = x * x + y * y;
+  return d2 + z * z;
+}
+
+// dynamics.py:96-171, Jacobi form
+__global__ void k_pbd(const float *__restrict__ pin, float *__restrict__ pout,
+                      const float *__restrict__ rad, int n, int G, float L,
+                      float half_l, float inv_l, float gap, float mdf, float dmul,
+                      const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+                      const int32_t *__restrict__ ci) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = pin[3 * i], py = pin[3 * i + 1], pz = pin[3 * i + 2];
+  const float ri = rad[i];
+  const float maxd = mdf * ri;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  const int cx = centre_axis(px, G, L, half_l, inv_l);
+  const int cy = centre_axis(py, G, L, half_l, inv_l);
+  const int cz = centre_axis(pz, G, L, half_l, inv_l);
+  for_stencil(cx, cy, cz, G, cs, cc, ci, [&](int j) {
+    if (j == i) return;
+    const float dx = gwrap(pin[3 * j] - px, L, inv_l);
+    const float dy = gwrap(pin[3 * j + 1] - py, L, inv_l);
+    const float dz = gwrap(pin[3 * j + 2] - pz, L, inv_l);
+    const float dist = sqrtf(dot3(dx, dy, dz) + 1e-12f);
+    const float ts = (ri + rad[j]) * gap;
+    if (dist < ts) {
+      const float overlap = ts - dist;
+      const float ux = dx / dist, uy = dy / dist, uz = dz / dist;
+      const float mag = fminf(overlap * 0.5f, maxd);
+      const float vx = fmaxf(-maxd, fminf(maxd, -ux * mag));
+      const float vy = fmaxf(-maxd, fminf(maxd, -uy * mag));
+      const float vz = fmaxf(-maxd, fminf(maxd, -uz * mag));
+      ax = ax + vx;
+      ay = ay + vy;
+      az = az + vz;
+    }
+  });
+  const float maxt = maxd * dmul;
+  const float cm = sqrtf(dot3(ax, ay, az));
+  if (cm > maxt) {
+    const float s = maxt / cm;
+    ax = ax * s;
+    ay = ay * s;
+    az = az * s;
+  }
+  pout[3 * i] = gwrap(px + ax, L, inv_l);
+  pout[3 * i + 1] = gwrap(py + ay, L, inv_l);
+  pout[3 * i + 2] = gwrap(pz + az, L, inv_l);
+}
+
+// dynamics.py:313-352 (+ the global max of dynamics.py:356-364, as a true
+// maximum: non-negative floats order like their int bit patterns).
+__global__ void k_max_overlap(const float *__restrict__ pos,
+                              const float *__restrict__ rad, int n, int G, float L,
+                              float half_l, float inv_l,
+                              const int32_t *__restrict__ cs,
+                              const int32_t *__restrict__ cc,
+                              const int32_t *__restrict__ ci,
+                              float *__restrict__ lmd, float *gmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  const float ri = rad[i];
+  float lm = 0.0f;
+  const int cx = centre_axis(px, G, L, half_l, inv_l);
+  const int cy = centre_axis(py, G, L, half_l, inv_l);
+  const int cz = centre_axis(pz, G, L, half_l, inv_l);
+  for_stencil(cx, cy, cz, G, cs, cc, ci, [&](int j) {
+    if (!(i < j)) return;
+    const float dx = gwrap(pos[3 * j] - px, L, inv_l);
+    const float dy = gwrap(pos[3 * j + 1] - py, L, inv_l);
+    const float dz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
+    const float dist = sqrtf(dot3(dx, dy, dz) + 1e-8f);
+    const float td = ri + rad[j];
+    if (dist < td) lm = fmaxf(lm, td - dist);
+  });
+  lmd[i] = lm;
+  if (lm > 0.0f) atomicMax((int *)gmax, __float_as_int(lm));
+}
+
+// dynamics.py:1226-1240
+__device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
+  const float t = fminf(1.0f, fmaxf(0.0f, (x - e0) / (e1 - e0)));
+  return t * t * (3.0f - 2.0f * t);
+}
+
+// dynamics.py:1281-1320
+__global__ void k_fsc_targets(const float *__restrict__ rad, float *__restrict__ rt,
+                              const int32_t *__restrict__ fsc,
+                              const float *__restrict__ ema, int n, float fl,
+                              float fh, float gp, float r_min, float r_max,
+                              float db) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float r0 = rad[i];
+  const float f = ema[i] > 0.0f ? ema[i] : (float)fsc[i];
+  const float gap = 1.0f, idle = 0.003f;
+  float out;
+  if (f < fl - gap) {
+    out = fminf(r0 * (1.0f + gp * 1.0f), r_max);
+  } else if (f < fl) {
+    const float g = 1.0f - smoothstep(fl - gap - db, fl, f);
+    out = fminf(r0 * (1.0f + gp * g), r_max);
+  } else if (f > fh + gap) {
+    out = fmaxf(r0 * (1.0f - gp * 1.0f), r_min);
+  } else if (f > fh) {
+    const float g = smoothstep(fh, fh + gap + db, f);
+    out = fmaxf(r0 * (1.0f - gp * g), r_min);
+  } else {
+    const float center = 0.5f * (fl + fh);
+    if (f < center)
+      out = fminf(r0 * (1.0f + gp * idle), r_max);
+    else
+      out = fmaxf(r0 * (1.0f - gp * idle), r_min);
+  }
+  rt[i] = out;
+}
+
+// dynamics.py:1345-1360: counts, sum of dr (double), max |dr| (float bits)
+__global__ void k_controller_stats(const float *__restrict__ rad,
+                                   const float *__restrict__ rt,
+                                   const float *__restrict__ ema, int n, float fl,
+                                   float fh, unsigned long long *cnt, double *sum,
+                                   int *maxbits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float f = ema[i];
+  const float dr = rt[i] - rad[i];
+  if (f < fl)
+    atomicAdd(&cnt[0], 1ull);
+  else if (f > fh)
+    atomicAdd(&cnt[1], 1ull);
+  else
+    atomicAdd(&cnt[2], 1ull);
+  atomicAdd(sum, (double)dr);
+  const float a = fabsf(dr);
+  if (a > 0.0f) atomicMax(maxbits, __float_as_int(a));
+}
+
+// dynamics.py:1399-1427
+__global__ void k_nudge(float *__restrict__ rad, const float *__restrict__ rt,
+                        const float *__restrict__ lmd, int n, float alpha,
+                        float msp, float r_min, float r_max, float back, int mode,
+                        float tol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float r0 = rad[i];
+  const float t = fminf(r_max, fmaxf(r_min, rt[i]));
+  float d = alpha * (t - r0);
+  float b = 1.0f;
+  if (mode == 1) {
+    b = back;
+  } else if (mode == 2) {
+    b = (tol - lmd[i]) / tol;
+    b = fminf(1.0f, fmaxf(0.25f, b));
+  }
+  d = d * b;
+  const float cap = msp * r0;
+  if (d > cap) d = cap;
+  if (d < -cap) d = -cap;
+  rad[i] = fminf(r_max, fmaxf(r_min, r0 + d));
+}
+
+// dynamics.py:1451, int32 wraparound arithmetic
+__device__ __forceinline__ int lcg_hash(int i, int frame) {
+  const unsigned t = (unsigned)i + 12345u * (unsigned)frame;
+  return (int)((1103515245u * t + 12345u) & 0x7fffffffu);
+}
+
+// fp32 power evaluated in double and rounded once: the correctly rounded fp32
+// result except in ~2^-29 of cases, independent of the fp32 pow library.
+__device__ __forceinline__ float pow_f32(float x, float y) {
+  return (float)pow((double)x, (double)y);
+}
+
+// dynamics.py:1502-1543 (delta for particle i from the current radii)
+__global__ void k_pressure_delta(int n, int frame, const float *__restrict__ rad,
+                                 float *__restrict__ delta_r,
+                                 const int32_t *__restrict__ face_ids,
+                                 const int32_t *__restrict__ fsc, float k,
+                                 float p_exp, float inv_p, float cap, int max_nei) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int fc = fsc[i];
+  float acc = 0.0f;
+  if (fc > 0) {
+    const float ri = rad[i];
+    const float vi = pow_f32(ri, p_exp);
+    const int an = fc < max_nei ? fc : max_nei;
+    const int start = lcg_hash(i, frame) % fc;
+    for (int q = 0; q < an; ++q) {
+      const int idx = (start + q) % fc;
+      const int j = face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + idx];
+      if (j < 0 || j >= n || j == i) continue;
+      const float vj = pow_f32(rad[j], p_exp);
+      const float raw = k * (vi - vj);
+      const float vmin = fminf(vi, vj);
+      const float dv = fmaxf(-cap * vmin, fminf(cap * vmin, raw));
+      acc = acc + (pow_f32(vi - dv, inv_p) - ri);
+    }
+  }
+  delta_r[i] = acc;
+}
+
+// dynamics.py:1546-1557
+__global__ void k_pressure_apply(int n, float *__restrict__ rad,
+                                 const float *__restrict__ delta_r, float r_min,
+                                 float r_max, float *stats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = delta_r[i];
+  const float a = fabsf(d);
+  if (a > 1e-12f) {
+    atomicAdd((int *)&stats[1], 1);
+    atomicMax((int *)&stats[0], __float_as_int(a));
+  }
+  rad[i] = fmaxf(r_min, fminf(r_max, rad[i] + d));
+}
+
+__global__ void k_minmax(const float *__restrict__ rad, int n, int *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // order-preserving int image of a float
+  const int b = __float_as_int(rad[i]);
+  const int key = b >= 0 ? b : (int)(0x80000000u - (unsigned)b);
+  atomicMin(&out[0], key);
+  atomicMax(&out[1], key);
+}
+
+// Box-Muller in double, rounded once to fp32.
+__device__ __forceinline__ float box_muller(uint32_t a, uint32_t b) {
+  const double u1 = (double)u01_open(a), u2 = (double)u01(b);
+  return (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2));
+}
+
+// dynamics.py:1616-1634 with Philox noise keyed (seed, frame, i)
+__global__ void k_brownian(float *__restrict__ pos, int n, float strength, float L,
+                           float half_l, uint32_t k0, uint32_t k1, int frame,
+                           int mode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint4 r0 = philox(make_uint4((unsigned)i, (unsigned)frame, 0u, 0u), k0, k1);
+  float d[3];
+  if (mode == 0) {
+    d[0] = ((u01(r0.x) - 0.5f) * 2.0f) * strength;
+    d[1] = ((u01(r0.y) - 0.5f) * 2.0f) * strength;
+    d[2] = ((u01(r0.z) - 0.5f) * 2.0f) * strength;
+  } else {
+    const uint4 r1 = philox(make_uint4((unsigned)i, (unsigned)frame, 1u, 0u), k0, k1);
+    d[0] = box_muller(r0.x, r0.y) * strength;
+    d[1] = box_muller(r0.z, r0.w) * strength;
+    d[2] = box_muller(r1.x, r1.y) * strength;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float p = pos[3 * i + a] + d[a];
+    if (p < -half_l)
+      p = p + L;
+    else if (p >= half_l)
+      p = p - L;
+    pos[3 * i + a] = p;
+  }
+}
+
+// dynamics.py:620-660
+__global__ void k_ou(float *__restrict__ pos, float *__restrict__ vj,
+                     const float *__restrict__ rad, int n, float tau, float dt,
+                     float sdt, float capf, float L, float inv_l, uint32_t k0,
+                     uint32_t k1, int frame) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint4 r = philox(make_uint4((unsigned)i, (unsigned)frame, 2u, 0u), k0, k1);
+  const unsigned rr[3] = {r.x, r.y, r.z};
+  float v[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float x = vj[3 * i + a];
+    v[a] = x + ((-x / tau) * dt + sdt * (u01(rr[a]) - 0.5f));
+  }
+  const float cap = capf * (rad[i] + rad[i]);
+  const float len = sqrtf(dot3(v[0], v[1], v[2])) * dt;
+  if (len > cap) {
+    const float s = (cap / fmaxf(1e-6f, len)) / dt;
+    v[0] *= s;
+    v[1] *= s;
+    v[2] *= s;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    vj[3 * i + a] = v[a];
+    if (pos) pos[3 * i + a] = gwrap(pos[3 * i + a] + v[a] * dt, L, inv_l);
+  }
+}
+
+// dynamics.py:649-660 integrate_jitter
+__global__ void k_integrate(float *__restrict__ pos, const float *__restrict__ vj, int n,
+                            float dt, float L, float inv_l) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) pos[3 * i + a] = gwrap(pos[3 * i + a] + vj[3 * i + a] * dt, L, inv_l);
+}
+
+// dynamics.py:543-589
+__global__ void k_xsph(const float *__restrict__ pos, const float *__restrict__ vel,
+                       float *__restrict__ vt, int n, int G, float L, float half_l,
+                       float inv_l, float lim, float eps,
+                       const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+                       const int32_t *__restrict__ ci) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  int cnt = 0;
+  const int cx = centre_axis(px, G, L, half_l, inv_l);
+  const int cy = centre_axis(py, G, L, half_l, inv_l);
+  const int cz = centre_axis(pz, G, L, half_l, inv_l);
+  for_stencil(cx, cy, cz, G, cs, cc, ci, [&](int j) {
+    if (j == i) return;
+    const float dx = gwrap(pos[3 * j] - px, L, inv_l);
+    const float dy = gwrap(pos[3 * j + 1] - py, L, inv_l);
+    const float dz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
+    if (dot3(dx, dy, dz) < lim) {
+      sx += vel[3 * j];
+      sy += vel[3 * j + 1];
+      sz += vel[3 * j + 2];
+      cnt += 1;
+    }
+  });
+  float ax = vel[3 * i], ay = vel[3 * i + 1], az = vel[3 * i + 2];
+  if (cnt > 0) {
+    const float fc = (float)cnt;
+    ax = sx / fc;
+    ay = sy / fc;
+    az = sz / fc;
+  }
+  vt[3 * i] = (1.0f - eps) * vel[3 * i] + eps * ax;
+  vt[3 * i + 1] = (1.0f - eps) * vel[3 * i + 1] + eps * ay;
+  vt[3 * i + 2] = (1.0f - eps) * vel[3 * i + 2] + eps * az;
+}
+
+__global__ void k_copy_f32(const float *__restrict__ s, float *__restrict__ d, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) d[i] = s[i];
+}
+
+__global__ void k_scale_f32(float *__restrict__ v, int64_t m, float f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) v[i] *= f;
+}
+
+// topology.py:87-133
+__global__ void k_topo(const float *__restrict__ pos, const float *__restrict__ rad,
+                       int n, int G, float L, float half_l, float inv_l, float mult,
+                       int cap, const int32_t *__restrict__ cs,
+                       const int32_t *__restrict__ cc, const int32_t *__restrict__ ci,
+                       int32_t *__restrict__ pairs, int32_t *__restrict__ count,
+                       int32_t *__restrict__ trunc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  const int cx = wrapi(centre_axis(px, G, L, half_l, inv_l), G);
+  const int cy = wrapi(centre_axis(py, G, L, half_l, inv_l), G);
+  const int cz = wrapi(centre_axis(pz, G, L, half_l, inv_l), G);
+  int c = 0, tr = 0;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int id = wrapi(cx + dx, G) * G * G + wrapi(cy + dy, G) * G + wrapi(cz + dz, G);
+        for (int k = cs[id]; k < cs[id] + cc[id]; ++k) {
+          const int j = ci[k];
+          if (j <= i) continue;
+          const float ddx = gwrap(pos[3 * j] - px, L, inv_l);
+          const float ddy = gwrap(pos[3 * j + 1] - py, L, inv_l);
+          const float ddz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
+          const float th = mult * (rad[i] + rad[j]);
+          if (dot3(ddx, ddy, ddz) <= th * th) {
+            if (c < cap) {
+              pairs[(int64_t)i * cap + c] = j;
+              c += 1;
+            } else {
+              tr = 1;
+              break;
+            }
+          }
+        }
+      }
+  count[i] = c;
+  trunc[i] = tr;
+}
+
+extern "C" {
+
+#define PART_ARGS_OK(n) ((n) >= 0)
+
+int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad, int n,
+                 int G, float L, float gap_factor, float max_disp_frac,
+                 float disp_mult, const int32_t *cell_start,
+                 const int32_t *cell_count, const int32_t *cell_indices,
+                 void *stream) {
+  FOS_REQUIRE(n >= 0 && G >= 3 && L > 0.0f &&
+                  (n == 0 || (pos_in && pos_out && pos_in != pos_out && rad &&
+                              cell_start && cell_count && cell_indices)),
+              "fos_pbd_pass: bad arguments (n=%d G=%d)", n, G);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_pbd, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     pos_in, pos_out, rad, n, G, L, (float)(0.5 * (double)L),
+                     (float)(1.0 / (double)L), gap_factor, max_disp_frac, disp_mult,
+                     cell_start, cell_count, cell_indices);
+  FOS_CHECK_LAUNCH("k_pbd");
+  return 0;
+}
+
+int fos_max_overlap(const float *pos, const float *rad, int n, int G, float L,
+                    const int32_t *cell_start, const int32_t *cell_count,
+                    const int32_t *cell_indices, float *local_max_depth,
+                    float *max_depth, void *stream) {
+  FOS_REQUIRE(n >= 0 && G >= 3 && L > 0.0f && max_depth &&
+                  (n == 0 || (pos && rad && cell_start && cell_count &&
+                              cell_indices && local_max_depth)),
+              "fos_max_overlap: bad arguments (n=%d G=%d)", n, G);
+  int rc = fos_fill_f32(max_depth, 1, 0.0f, stream);
+  if (rc || n == 0) return rc;
+  hipLaunchKernelGGL(k_max_overlap, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, pos, rad, n, G, L, (float)(0.5 * (double)L),
+                     (float)(1.0 / (double)L), cell_start, cell_count, cell_indices,
+                     local_max_depth, max_depth);
+  FOS_CHECK_LAUNCH("k_max_overlap");
+  return 0;
+}
+
+int fos_fsc_targets(const float *rad, float *rad_target, const int32_t *fsc,
+                    const float *fsc_ema, int n, int f_low, int f_high,
+                    float growth_pct, float r_min, float r_max, float deadband,
+                    void *stream) {
+  FOS_REQUIRE(n >= 0 && (n == 0 || (rad && rad_target && fsc && fsc_ema)),
+              "fos_fsc_targets: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_fsc_targets, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, rad, rad_target, fsc, fsc_ema, n,
+                     (float)f_low, (float)f_high, growth_pct, r_min, r_max, deadband);
+  FOS_CHECK_LAUNCH("k_fsc_targets");
+  return 0;
+}
+
+int fos_controller_stats(fos_ctx *ctx, const float *rad, const float *rad_target,
+                         const float *fsc_ema, int n, int f_low, int f_high,
+                         int64_t *out_i, double *out_f, void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && out_i && out_f &&
+                  (n == 0 || (rad && rad_target && fsc_ema)),
+              "fos_controller_stats: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long *cnt = (unsigned long long *)ctx->iscratch;
+  int *mx = ctx->iscratch + 6;
+  double *sum = (double *)ctx->fscratch;
+  int rc = fos_fill_i32(ctx->iscratch, 8, 0, stream);
+  if (rc) return rc;
+  rc = fos_fill_i32((int32_t *)sum, 2, 0, stream);
+  if (rc) return rc;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_controller_stats, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st,
+                       rad, rad_target, fsc_ema, n, (float)f_low, (float)f_high, cnt,
+                       sum, mx);
+    FOS_CHECK_LAUNCH("k_controller_stats");
+  }
+  unsigned long long hc[3];
+  int hm = 0;
+  double hs = 0.0;
+  FOS_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipMemcpyAsync(&hm, mx, sizeof(hm), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipMemcpyAsync(&hs, sum, sizeof(hs), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipStreamSynchronize(st));
+  out_i[0] = (int64_t)hc[0];
+  out_i[1] = (int64_t)hc[1];
+  out_i[2] = (int64_t)hc[2];
+  float m;
+  memcpy(&m, &hm, sizeof(m));
+  out_f[0] = hs / (double)(n > 1 ? n : 1);
+  out_f[1] = (double)m;
+  return 0;
+}
+
+int fos_nudge_radii(float *rad, const float *rad_target,
+                    const float *local_max_depth, int n, float alpha,
+                    float max_step_pct, float r_min, float r_max, float back_global,
+                    int mode, float contact_tol, void *stream) {
+  FOS_REQUIRE(n >= 0 && (n == 0 || (rad && rad_target && local_max_depth)),
+              "fos_nudge_radii: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_nudge, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, rad, rad_target, local_max_depth, n, alpha,
+                     max_step_pct, r_min, r_max, back_global, mode, contact_tol);
+  FOS_CHECK_LAUNCH("k_nudge");
+  return 0;
+}
+
+int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
+                      const int32_t *face_ids, const int32_t *fsc, float k,
+                      float p_exp, float pair_cap, int max_nei, float r_min,
+                      float r_max, float *stats, void *stream) {
+  FOS_REQUIRE(n >= 0 && stats && (n == 0 || (rad && delta_r && face_ids && fsc)),
+              "fos_pressure_iter: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = fos_fill_f32(stats, 2, 0.0f, stream);
+  if (rc || n == 0) return rc;
+  hipLaunchKernelGGL(k_pressure_delta, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+                     frame, rad, delta_r, face_ids, fsc, k, p_exp, 1.0f / p_exp,
+                     pair_cap, max_nei);
+  FOS_CHECK_LAUNCH("k_pressure_delta");
+  hipLaunchKernelGGL(k_pressure_apply, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+                     rad, delta_r, r_min, r_max, stats);
+  FOS_CHECK_LAUNCH("k_pressure_apply");
+  return 0;
+}
+
+int fos_pressure_stats(fos_ctx *ctx, const float *rad, int n, float *out,
+                       void *stream) {
+  FOS_REQUIRE(ctx && n >= 1 && rad && out, "fos_pressure_stats: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  int *mm = ctx->iscratch;
+  int rc = fos_fill_i32(mm, 1, 0x7fffffff, stream);
+  if (rc) return rc;
+  rc = fos_fill_i32(mm + 1, 1, (int32_t)0x80000000, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_minmax, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, rad, n, mm);
+  FOS_CHECK_LAUNCH("k_minmax");
+  int h[2];
+  FOS_HIP(hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipStreamSynchronize(st));
+  for (int q = 0; q < 2; ++q) {
+    const int key = h[q];
+    const int b = key >= 0 ? key : (int)(0x80000000u - (unsigned)key);
+    memcpy(&out[q], &b, sizeof(float));
+  }
+  return 0;
+}
+
+int fos_brownian(float *pos, int n, float strength, float L, uint64_t seed,
+                 int frame, int mode, void *stream) {
+  FOS_REQUIRE(n >= 0 && (mode == 0 || mode == 1) && L > 0.0f && (n == 0 || pos),
+              "fos_brownian: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_brownian, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, pos, n, strength, L, L * 0.5f,
+                     (uint32_t)seed, (uint32_t)(seed >> 32), frame, mode);
+  FOS_CHECK_LAUNCH("k_brownian");
+  return 0;
+}
+
+int fos_ou_jitter(float *pos, float *v_jit, const float *rad, int n, float mean_r,
+                  float dt, float jitter_rms, float jitter_tau,
+                  float drift_cap_frac, float L, uint64_t seed, int frame,
+                  void *stream) {
+  FOS_REQUIRE(n >= 0 && L > 0.0f && dt > 0.0f && (n == 0 || (v_jit && rad)),
+              "fos_ou_jitter: bad arguments");
+  if (n == 0) return 0;
+  const float tau = jitter_tau > 1e-6f ? jitter_tau : 1e-6f;
+  const float sigma = jitter_rms * mean_r * sqrtf(2.0f / tau);
+  const float sdt = sigma * sqrtf(dt);
+  hipLaunchKernelGGL(k_ou, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     pos, v_jit, rad, n, tau, dt, sdt, drift_cap_frac, L,
+                     (float)(1.0 / (double)L), (uint32_t)seed ^ 0x5bd1e995u,
+                     (uint32_t)(seed >> 32), frame);
+  FOS_CHECK_LAUNCH("k_ou");
+  return 0;
+}
+
+int fos_integrate_jitter(float *pos, const float *v_jit, int n, float dt, float L,
+                         void *stream) {
+  FOS_REQUIRE(n >= 0 && L > 0.0f && (n == 0 || (pos && v_jit)),
+              "fos_integrate_jitter: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_integrate, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     pos, v_jit, n, dt, L, (float)(1.0 / (double)L));
+  FOS_CHECK_LAUNCH("k_integrate");
+  return 0;
+}
+
+int fos_xsph(const float *pos, float *vel, float *vel_tmp, int n, int G, float L,
+             float lim_sq, float eps, const int32_t *cell_start,
+             const int32_t *cell_count, const int32_t *cell_indices, void *stream) {
+  FOS_REQUIRE(n >= 0 && G >= 3 && L > 0.0f &&
+                  (n == 0 || (pos && vel && vel_tmp && cell_start && cell_count &&
+                              cell_indices)),
+              "fos_xsph: bad arguments");
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_xsph, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, vel,
+                     vel_tmp, n, G, L, (float)(0.5 * (double)L), (float)(1.0 / (double)L),
+                     lim_sq, eps, cell_start, cell_count, cell_indices);
+  FOS_CHECK_LAUNCH("k_xsph");
+  hipLaunchKernelGGL(k_copy_f32, dim3(blocks_for((int64_t)n * 3)), dim3(FOS_BLOCK), 0, st,
+                     vel_tmp, vel, (int64_t)n * 3);
+  FOS_CHECK_LAUNCH("k_copy_f32");
+  return 0;
+}
+
+int fos_damp(float *vel, int n, float factor, void *stream) {
+  FOS_REQUIRE(n >= 0 && (n == 0 || vel), "fos_damp: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_scale_f32, dim3(blocks_for((int64_t)n * 3)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, vel, (int64_t)n * 3, factor);
+  FOS_CHECK_LAUNCH("k_scale_f32");
+  return 0;
+}
+
+int fos_topo_candidates(const float *pos, const float *rad, int n, int G, float L,
+                        float mult, int cap, const int32_t *cell_start,
+                        const int32_t *cell_count, const int32_t *cell_indices,
+                        int32_t *pairs, int32_t *count, int32_t *truncated,
+                        void *stream) {
+  FOS_REQUIRE(n >= 0 && G >= 3 && cap >= 1 && L > 0.0f &&
+                  (n == 0 || (pos && rad && cell_start && cell_count && cell_indices &&
+                              pairs && count && truncated)),
+              "fos_topo_candidates: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_topo, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     pos, rad, n, G, L, (float)(0.5 * (double)L), (float)(1.0 / (double)L),
+                     mult, cap, cell_start, cell_count, cell_indices, pairs, count,
+                     truncated);
+  FOS_CHECK_LAUNCH("k_topo");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// fos_grid.hip -- periodic spatial hash as a counting sort (reference:
+// grid.py:145-309, rebuild order run.py:476-504).  Cells are G^3, index
+// cx*G*G + cy*G + cz.  Particles inside a cell are listed in ascending index
+// (the serial scatter order), which fixes the PBD accumulation order.
+#include "fos_common.h"
+
+using namespace fos;
+
+__device__ __forceinline__ int particle_cell(const float *__restrict__ pos, int i,
+                                             int G, float L, float half_l,
+                                             float inv_l, float pw[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) pw[d] = gwrap(pos[3 * i + d], L, inv_l);
+  const int c0 = hash_axis(pw[0], G, half_l, inv_l);
+  const int c1 = hash_axis(pw[1], G, half_l, inv_l);
+  const int c2 = hash_axis(pw[2], G, half_l, inv_l);
+  return c0 * G * G + c1 * G + c2;
+}
+
+// grid.py:202-222
+__global__ void k_grid_count(const float *__restrict__ pos, int n, int G, float L,
+                             float half_l, float inv_l, int32_t *__restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float pw[3];
+  const int c = particle_cell(pos, i, G, L, half_l, inv_l, pw);
+  atomicAdd(&count[c], 1);
+}
+
+// grid.py:285-309: wraps pos in place and appends i to its cell.
+__global__ void k_grid_scatter(float *__restrict__ pos, int n, int G, float L,
+                               float half_l, float inv_l, int32_t *__restrict__ wr,
+                               int32_t *__restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float pw[3];
+  const int c = particle_cell(pos, i, G, L, half_l, inv_l, pw);
+  pos[3 * i] = pw[0];
+  pos[3 * i + 1] = pw[1];
+  pos[3 * i + 2] = pw[2];
+  const int slot = atomicAdd(&wr[c], 1);
+  idx[slot] = i;
+}
+
+// Restores the serial (stable) order inside each cell.  After the scatter,
+// wr[c] is the end of cell c and wr[c-1] its start (cells are contiguous).
+__global__ void k_grid_sort_cells(int C, const int32_t *__restrict__ wr,
+                                  int32_t *__restrict__ idx) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const int b = c ? wr[c - 1] : 0, e = wr[c];
+  for (int p = b + 1; p < e; ++p) {
+    const int key = idx[p];
+    int q = p - 1;
+    while (q >= b && idx[q] > key) {
+      idx[q + 1] = idx[q];
+      --q;
+    }
+    idx[q + 1] = key;
+  }
+}
+
+__global__ void k_copy_cells(const int32_t *__restrict__ s, int32_t *__restrict__ d,
+                             int C) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) d[c] = s[c];
+}
+
+extern "C" {
+
+int fos_grid_count(const float *pos, int n, int G, float L, int32_t *cell_count,
+                   void *stream) {
+  FOS_REQUIRE(n >= 0 && G >= 3 && G <= 1290 && L > 0.0f && cell_count &&
+                  (n == 0 || pos),
+              "fos_grid_count: bad arguments (n=%d G=%d)", n, G);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_grid_count, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, pos, n, G, L, (float)(0.5 * (double)L),
+                     (float)(1.0 / (double)L), cell_count);
+  FOS_CHECK_LAUNCH("k_grid_count");
+  return 0;
+}
+
+int fos_grid_scan(fos_ctx *ctx, const int32_t *cell_count, int32_t *cell_start,
+                  int G, void *stream) {
+  FOS_REQUIRE(ctx && cell_count && cell_start && G >= 3 &&
+                  (int64_t)G * G * G <= ctx->max_cells,
+              "fos_grid_scan: bad arguments (G=%d)", G);
+  return exclusive_scan(ctx, cell_count, cell_start, G * G * G, (hipStream_t)stream);
+}
+
+int fos_grid_scatter(float *pos, int n, int G, float L, int32_t *cell_write,
+                     int32_t *cell_indices, void *stream) {
+  FOS_REQUIRE(n >= 0 && G >= 3 && G <= 1290 && L > 0.0f && cell_write &&
+                  (n == 0 || (pos && cell_indices)),
+              "fos_grid_scatter: bad arguments (n=%d G=%d)", n, G);
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos,
+                     n, G, L, (float)(0.5 * (double)L), (float)(1.0 / (double)L),
+                     cell_write, cell_indices);
+  FOS_CHECK_LAUNCH("k_grid_scatter");
+  const int C = G * G * G;
+  hipLaunchKernelGGL(k_grid_sort_cells, dim3(blocks_for(C)), dim3(FOS_BLOCK), 0, st, C,
+                     cell_write, cell_indices);
+  FOS_CHECK_LAUNCH("k_grid_sort_cells");
+  return 0;
+}
+
+int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
+                   int32_t *cell_count, int32_t *cell_start,
+                   int32_t *cell_indices, void *stream) {
+  FOS_REQUIRE(ctx && G >= 3 && (int64_t)G * G * G <= ctx->max_cells && n >= 0 &&
+                  n <= ctx->max_n && cell_count && cell_start,
+              "fos_grid_build: bad arguments (n=%d G=%d)", n, G);
+  const int C = G * G * G;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = fos_fill_i32(cell_count, C, 0, stream);
+  if (rc) return rc;
+  rc = fos_grid_count(pos, n, G, L, cell_count, stream);
+  if (rc) return rc;
+  rc = exclusive_scan(ctx, cell_count, cell_start, C, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_copy_cells, dim3(blocks_for(C)), dim3(FOS_BLOCK), 0, st,
+                     cell_start, ctx->cell_write, C);
+  FOS_CHECK_LAUNCH("k_copy_cells");
+  return fos_grid_scatter(pos, n, G, L, ctx->cell_write, cell_indices, stream);
+}
+
+}  // extern "C"

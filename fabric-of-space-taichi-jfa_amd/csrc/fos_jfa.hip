@@ -1,0 +1,1043 @@
+// fos_jfa.hip -- power-diagram Jump Flood and face-sharing-count extraction
+// (reference: jfa.py).  Labels are int32 owner ids in one dense res^3 grid,
+// index x*res*res + y*res + z (z fastest, as Taichi's ndrange).
+#include "fos_common.h"
+
+namespace fos {
+int fill_u32(unsigned *p, int64_t n, unsigned v, hipStream_t st);
+int alloc_tables(fos_ctx *ctx, int cap);
+}  // namespace fos
+
+using namespace fos;
+
+// ---------------------------------------------------------------------------
+// Seed table and seeding (jfa.py:485-520)
+// ---------------------------------------------------------------------------
+
+__global__ void k_seed_table(const float *__restrict__ pos,
+                             const float *__restrict__ rad, int n, float beta,
+                             float4 *__restrict__ seeds) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float b = beta * rad[i];
+  seeds[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], b * b);
+}
+
+// Serial semantics of the seeding loop: the owner of a voxel is the
+// lexicographic minimum of (power distance to the voxel centre, index) over
+// the particles seeded there.  A CAS loop reaches that minimum under any
+// interleaving because the order is total.
+template <bool UNIT_L>
+__global__ void k_seed(const float *__restrict__ pos,
+                       const float4 *__restrict__ seeds, int n, int res,
+                       float L, float half_l, float voxel,
+                       int32_t *__restrict__ grid) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float nrm = (pos[3 * i + d] + half_l) / L;  // jfa.py:268-270
+    c[d] = wrapi((int)floorf(nrm * (float)res), res);
+  }
+  const int64_t v = ((int64_t)c[0] * res + c[1]) * res + c[2];
+  const float cx = vcenter(c[0], voxel, half_l), cy = vcenter(c[1], voxel, half_l),
+              cz = vcenter(c[2], voxel, half_l);
+  const float pi = power_dist<UNIT_L>(cx, cy, cz, seeds[i], L);
+  int cur = __hip_atomic_load(&grid[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    if (cur != -1) {
+      float pc = power_dist<UNIT_L>(cx, cy, cz, seeds[cur], L);
+      bool better = (pi < pc) || (pi == pc && i < cur);
+      if (!better) return;
+    }
+    int old = atomicCAS(&grid[v], cur, i);
+    if (old == cur) return;
+    cur = old;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One JFA pass (jfa.py:524-623)
+// ---------------------------------------------------------------------------
+
+// Whether pass p runs under the device-side early exit (jfa.py:1104-1112):
+// it runs unless an earlier pass q >= min_passes-1 changed fewer than
+// th * V voxels.  Every thread derives the same answer from the history.
+__device__ __forceinline__ bool pass_active(const int32_t *info, int p,
+                                            int early, double th, int min_passes,
+                                            double V) {
+  if (!early) return true;
+  for (int q = 0; q < p; ++q) {
+    if (q >= min_passes - 1 && (double)info[2 + q] / V < th) return false;
+  }
+  return true;
+}
+
+// Block (64, 4): 64 consecutive z voxels per wave; grid (ceil(res/64),
+// ceil(res/4), res).  Candidates equal to the incumbent or to an earlier
+// candidate are skipped: under the strict '<' they can never win, so the
+// result is identical to evaluating all 27.
+template <bool UNIT_L>
+__global__ void __launch_bounds__(256)
+    k_jfa_step(const float4 *__restrict__ seeds, int res, float L, float half_l,
+               float voxel, int step, const int32_t *__restrict__ in,
+               int32_t *__restrict__ out, int32_t *__restrict__ changes,
+               const int32_t *__restrict__ info, int pass, int early, double th,
+               int min_passes) {
+  if (info) {
+    double V = (double)res * res * res;
+    if (!pass_active(info, pass, early, th, min_passes, V)) return;
+  }
+  const int z = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int x = blockIdx.z;
+  int changed = 0;
+  if (z < res && y < res) {
+    const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
+                cz = vcenter(z, voxel, half_l);
+    const int64_t v = ((int64_t)x * res + y) * res + z;
+    const int old = in[v];
+    int best = old;
+    float bd = 1e10f;
+    if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
+    int xs[3], ys[3], zs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      xs[k] = wrap1(x + (k - 1) * step, res);
+      ys[k] = wrap1(y + (k - 1) * step, res);
+      zs[k] = wrap1(z + (k - 1) * step, res);
+    }
+    int last = best;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const int64_t row = ((int64_t)xs[a] * res + ys[b]) * res;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int cand = in[row + zs[c]];
+          if (cand >= 0 && cand != best && cand != last) {
+            last = cand;
+            const float pd = power_dist<UNIT_L>(cx, cy, cz, seeds[cand], L);
+            if (pd < bd) {
+              bd = pd;
+              best = cand;
+            }
+          }
+        }
+      }
+    out[v] = best;
+    changed = (best != old);
+  }
+  if (changes) block_add(changed, changes);
+}
+
+// Ends the pass loop: records passes executed and the exit flag, and copies
+// temp into grid when an odd number of passes ran (jfa.py:1115-1116).
+__global__ void k_jfa_finish(int res, int num_passes, int early, double th,
+                             int min_passes, int32_t *__restrict__ info,
+                             const int32_t *__restrict__ temp,
+                             int32_t *__restrict__ grid) {
+  const double V = (double)res * res * res;
+  int executed = 0;
+  for (int p = 0; p < num_passes; ++p) {
+    if (!pass_active(info, p, early, th, min_passes, V)) break;
+    executed = p + 1;
+  }
+  const int fired = (executed < num_passes) ? 1 : 0;
+  const int64_t total = (int64_t)res * res * res;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    info[0] = executed;
+    info[1] = fired;
+  }
+  // The last executed pass also decides the exit when it is the final one.
+  if (executed & 1) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = t; i < total; i += stride) grid[i] = temp[i];
+  }
+}
+
+__global__ void k_copy_i32(const int32_t *__restrict__ s, int32_t *__restrict__ d,
+                           int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) d[i] = s[i];
+}
+
+// ---------------------------------------------------------------------------
+// Majority filter (jfa.py:640-699)
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+    k_majority(int res, const int32_t *__restrict__ g, int32_t *__restrict__ f) {
+  const int z = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int x = blockIdx.z;
+  if (z >= res || y >= res) return;
+  const int64_t v = ((int64_t)x * res + y) * res + z;
+  const int my = g[v];
+  if (my < 0) {
+    f[v] = -1;
+    return;
+  }
+  const int64_t rr = (int64_t)res * res;
+  int nb[6];
+  nb[0] = g[(int64_t)wrap1(x - 1, res) * rr + (int64_t)y * res + z];
+  nb[1] = g[(int64_t)wrap1(x + 1, res) * rr + (int64_t)y * res + z];
+  nb[2] = g[(int64_t)x * rr + (int64_t)wrap1(y - 1, res) * res + z];
+  nb[3] = g[(int64_t)x * rr + (int64_t)wrap1(y + 1, res) * res + z];
+  nb[4] = g[(int64_t)x * rr + (int64_t)y * res + wrap1(z - 1, res)];
+  nb[5] = g[(int64_t)x * rr + (int64_t)y * res + wrap1(z + 1, res)];
+  int ids[7], cnt[7], nu = 1;
+  ids[0] = my;
+  cnt[0] = 1;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int id = nb[k];
+    if (id >= 0) {
+      bool found = false;
+      for (int u = 0; u < nu; ++u)
+        if (ids[u] == id) {
+          cnt[u] += 1;
+          found = true;
+          break;
+        }
+      if (!found) {
+        ids[nu] = id;
+        cnt[nu] = 1;
+        nu += 1;
+      }
+    }
+  }
+  int maxc = 0, best = my;
+  for (int u = 0; u < nu; ++u)
+    if (cnt[u] > maxc) {
+      maxc = cnt[u];
+      best = ids[u];
+    }
+  f[v] = best;
+}
+
+// ---------------------------------------------------------------------------
+// Face-pair accumulation (jfa.py:703-762), serial semantics:
+//   row a lists its partners b > a by first-encounter key
+//   k = 6*lin(I) + 2*axis + (dir > 0), first 32 kept, counts complete.
+// Emission inserts (b, min key, count) into a per-row open-addressed table;
+// a compaction pass ranks each row's entries by key into face_ids/counts.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void tab_insert(int *__restrict__ tb,
+                                           unsigned *__restrict__ tk,
+                                           int *__restrict__ tc, int cap,
+                                           int a, int b, unsigned key, int cnt,
+                                           int *overflow) {
+  const int64_t base = (int64_t)a * cap;
+  const unsigned h = (unsigned)b * 2654435761u;
+  for (int p = 0; p < cap; ++p) {
+    const int s = (int)((h + (unsigned)p) & (unsigned)(cap - 1));
+    int cur = __hip_atomic_load(&tb[base + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == -1) {
+      int old = atomicCAS(&tb[base + s], -1, b);
+      cur = (old == -1) ? b : old;
+    }
+    if (cur == b) {
+      atomicMin(&tk[base + s], key);
+      atomicAdd(&tc[base + s], cnt);
+      return;
+    }
+  }
+  atomicOr(overflow, 1);
+}
+
+__global__ void __launch_bounds__(256)
+    k_face_emit(int res, int n, const int32_t *__restrict__ f, int *tb,
+                unsigned *tk, int *tc, int cap, int *overflow) {
+  const int z = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int x = blockIdx.z;
+  if (z >= res || y >= res) return;
+  const int64_t v = ((int64_t)x * res + y) * res + z;
+  const int a = f[v];
+  if (a < 0 || a >= n) return;
+  const int64_t rr = (int64_t)res * res;
+  int nb[6];
+  nb[0] = f[(int64_t)wrap1(x - 1, res) * rr + (int64_t)y * res + z];
+  nb[1] = f[(int64_t)wrap1(x + 1, res) * rr + (int64_t)y * res + z];
+  nb[2] = f[(int64_t)x * rr + (int64_t)wrap1(y - 1, res) * res + z];
+  nb[3] = f[(int64_t)x * rr + (int64_t)wrap1(y + 1, res) * res + z];
+  nb[4] = f[(int64_t)x * rr + (int64_t)y * res + wrap1(z - 1, res)];
+  nb[5] = f[(int64_t)x * rr + (int64_t)y * res + wrap1(z + 1, res)];
+  const unsigned kbase = 6u * (unsigned)v;
+  // Merge repeated partners of this voxel first: one insert per distinct b,
+  // carrying the smallest key (the first face in scan order) and the count.
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int b = nb[k];
+    if (b < 0 || b >= n || a >= b) continue;
+    bool dup = false;
+#pragma unroll
+    for (int q = 0; q < k; ++q)
+      if (nb[q] == b) dup = true;
+    if (dup) continue;
+    int cnt = 1;
+#pragma unroll
+    for (int q = k + 1; q < 6; ++q)
+      if (nb[q] == b) cnt += 1;
+    tab_insert(tb, tk, tc, cap, a, b, kbase + (unsigned)k, cnt, overflow);
+  }
+}
+
+// One wave per row: rank entries by key (keys are unique within a row), write
+// the first 32 into face_ids/face_counts, clear the rest, reset the table.
+__global__ void __launch_bounds__(256)
+    k_face_compact(int n, int *tb, unsigned *tk, int *tc, int cap,
+                   int32_t *__restrict__ face_ids,
+                   int32_t *__restrict__ face_counts) {
+  extern __shared__ unsigned s_keys[];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + wid;
+  unsigned *keys = s_keys + wid * cap;
+  const bool live = row < n;
+  const int64_t base = (int64_t)row * cap;
+  int nvalid = 0;
+  for (int e = lane; e < cap; e += 64) {
+    unsigned k = 0xffffffffu;
+    if (live && tb[base + e] != -1) {
+      k = tk[base + e];
+      nvalid += 1;
+    }
+    keys[e] = k;
+  }
+  for (int o = 32; o > 0; o >>= 1) nvalid += __shfl_xor(nvalid, o, 64);
+  __syncthreads();
+  if (!live) return;
+  for (int e = lane; e < cap; e += 64) {
+    const int b = tb[base + e];
+    if (b == -1) continue;
+    const unsigned k = keys[e];
+    int rank = 0;
+    for (int q = 0; q < cap; ++q) rank += (keys[q] < k) ? 1 : 0;
+    if (rank < FOS_MAX_NEIGHBORS) {
+      face_ids[(int64_t)row * FOS_MAX_NEIGHBORS + rank] = b;
+      face_counts[(int64_t)row * FOS_MAX_NEIGHBORS + rank] = tc[base + e];
+    }
+    tb[base + e] = -1;
+    tk[base + e] = 0xffffffffu;
+    tc[base + e] = 0;
+  }
+  if (lane < FOS_MAX_NEIGHBORS && lane >= nvalid) {
+    face_ids[(int64_t)row * FOS_MAX_NEIGHBORS + lane] = -1;
+    face_counts[(int64_t)row * FOS_MAX_NEIGHBORS + lane] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Finalize (jfa.py:789-857, add_neighbor jfa.py:400-443)
+//
+// Serial order: rows i ascending, slots s ascending; an accepted pair (i, j)
+// appends j to row i and i to row j (while the row holds < 32).  Row i's
+// hysteresis read prev[i][s] >= T_DROP is true when position s was written
+// this frame before the read -- i.e. s < used_i at that moment, where
+// used_i = min(32, L_i + own accepts so far) and L_i = accepted appends from
+// rows i' < i -- or when the previous frame left prev[i][s] >= T_DROP.
+// Accept writes are scores >= T_DROP, reject writes land at the slot's own
+// index (never read again this frame).  Rows depend only on lower rows, so
+// iterating "decide rows given L, recount L" from L = 0 reaches the serial
+// result (monotone, unique fixed point).
+// ---------------------------------------------------------------------------
+
+#define CLS_INVALID 0
+#define CLS_REJECT 1
+#define CLS_ACCEPT 2
+#define CLS_AMBIG 3
+#define CLS_OLD 4
+
+// jfa.py:379-397 expected_area
+__device__ __forceinline__ float expected_area(float ri, float rj, float voxel) {
+  float r_mean = 0.5f * (ri + rj);
+  float t = (2.0f * r_mean) / voxel;
+  return 1.5f * (t * t);
+}
+
+__global__ void k_fin_classify(int n, const float *__restrict__ rad, float voxel,
+                               const int32_t *__restrict__ face_ids,
+                               const int32_t *__restrict__ face_counts,
+                               const float *__restrict__ prev,
+                               float *__restrict__ score,
+                               unsigned char *__restrict__ cls,
+                               int *__restrict__ in_count) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * FOS_MAX_NEIGHBORS) return;
+  const int i = (int)(t / FOS_MAX_NEIGHBORS);
+  const int j = face_ids[t];
+  unsigned char c = CLS_INVALID;
+  float sc = 0.0f;
+  if (j >= 0 && j < n) {
+    const int v = face_counts[t];
+    if (v > 0) {
+      const float e = expected_area(rad[i], rad[j], voxel);
+      if (e > 0.0f) {
+        sc = (float)v / e;
+        if (sc >= 0.15f)
+          c = CLS_ACCEPT;
+        else if (sc < 0.10f)
+          c = CLS_REJECT;
+        else
+          c = CLS_AMBIG;
+        if (prev[t] >= 0.10f) c |= CLS_OLD;
+        atomicAdd(&in_count[j], 1);
+      }
+    }
+  }
+  score[t] = sc;
+  cls[t] = c;
+}
+
+__global__ void k_fin_scatter(int n, const int32_t *__restrict__ face_ids,
+                              const unsigned char *__restrict__ cls,
+                              const int *__restrict__ in_start,
+                              int *__restrict__ in_fill, int *__restrict__ in_src) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * FOS_MAX_NEIGHBORS) return;
+  if ((cls[t] & 3) == CLS_INVALID) return;
+  const int j = face_ids[t];
+  const int p = in_start[j] + atomicAdd(&in_fill[j], 1);
+  in_src[p] = (int)t;
+}
+
+// Sort each incoming list ascending (source row, then slot) so the
+// materialised neighbour rows list lower partners in row order.
+__global__ void k_fin_sort_in(int n, const int *__restrict__ in_start,
+                              int *__restrict__ in_src) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = in_start[j], e = in_start[j + 1];
+  for (int p = b + 1; p < e; ++p) {
+    const int key = in_src[p];
+    int q = p - 1;
+    while (q >= b && in_src[q] > key) {
+      in_src[q + 1] = in_src[q];
+      --q;
+    }
+    in_src[q + 1] = key;
+  }
+}
+
+__global__ void k_fin_decide(int n, const unsigned char *__restrict__ cls,
+                             const int *__restrict__ acc_in,
+                             unsigned char *__restrict__ dec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int used = acc_in[i] < FOS_MAX_NEIGHBORS ? acc_in[i] : FOS_MAX_NEIGHBORS;
+  const int64_t base = (int64_t)i * FOS_MAX_NEIGHBORS;
+  for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
+    const unsigned char c = cls[base + s];
+    const int k = c & 3;
+    bool acc = false;
+    if (k == CLS_ACCEPT) {
+      acc = true;
+    } else if (k == CLS_AMBIG) {
+      acc = (s < used) || (c & CLS_OLD);
+    }
+    dec[base + s] = acc ? 1 : 0;
+    if (acc && used < FOS_MAX_NEIGHBORS) used += 1;
+  }
+}
+
+__global__ void k_fin_count(int n, const int *__restrict__ in_start,
+                            const int *__restrict__ in_src,
+                            const unsigned char *__restrict__ dec,
+                            int *__restrict__ acc_in, int *__restrict__ changed) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int a = 0;
+  for (int p = in_start[j]; p < in_start[j + 1]; ++p) a += dec[in_src[p]];
+  if (a != acc_in[j]) {
+    acc_in[j] = a;
+    atomicOr(changed, 1);
+  }
+}
+
+__global__ void k_fin_materialize(int n, const int32_t *__restrict__ face_ids,
+                                  const float *__restrict__ score,
+                                  const unsigned char *__restrict__ cls,
+                                  const unsigned char *__restrict__ dec,
+                                  const int *__restrict__ in_start,
+                                  const int *__restrict__ in_src,
+                                  float *__restrict__ prev,
+                                  int32_t *__restrict__ nl,
+                                  int32_t *__restrict__ nc,
+                                  int32_t *__restrict__ used_count,
+                                  int32_t *__restrict__ fsc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t base = (int64_t)i * FOS_MAX_NEIGHBORS;
+  float pr[FOS_MAX_NEIGHBORS];
+  int row[FOS_MAX_NEIGHBORS];
+#pragma unroll
+  for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
+    pr[s] = prev[base + s];
+    row[s] = -1;
+  }
+  int used = 0;
+  // appends from lower rows, in row order
+  for (int p = in_start[i]; p < in_start[i + 1] && used < FOS_MAX_NEIGHBORS; ++p) {
+    const int src = in_src[p];
+    if (!dec[src]) continue;
+    row[used] = src / FOS_MAX_NEIGHBORS;
+    pr[used] = score[src];
+    used += 1;
+  }
+  // own slots in order
+  for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
+    const unsigned char c = cls[base + s];
+    if ((c & 3) == CLS_INVALID) continue;
+    if (dec[base + s]) {
+      if (used < FOS_MAX_NEIGHBORS) {
+        row[used] = face_ids[base + s];
+        pr[used] = score[base + s];
+        used += 1;
+      }
+    } else {
+      pr[s] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
+    prev[base + s] = pr[s];
+    nl[base + s] = row[s];
+  }
+  nc[i] = used;
+  used_count[i] = used;
+  fsc[i] = used;
+}
+
+// jfa.py:766-785
+__global__ void k_fsc_ema(int n, float alpha, const int32_t *__restrict__ fsc,
+                          float *__restrict__ ema) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float e = ema[i];
+  const int f = fsc[i];
+  if (e == 0.0f && f > 0)
+    ema[i] = (float)f;
+  else
+    ema[i] = alpha * (float)f + (1.0f - alpha) * e;
+}
+
+// ---------------------------------------------------------------------------
+// Telemetry (jfa.py:1204-1330) and validation (jfa.py:1337-1397)
+// ---------------------------------------------------------------------------
+
+// acc: [0] p_dir, [1] overflow, [2] min, [3] max, [4] sum_fsc (int64 as
+// unsigned long long); dacc: [0] sum_ema, [1] sum_score, [2] count_score
+__global__ void k_stats_a(int n, const float *__restrict__ rad, float voxel,
+                          const int32_t *__restrict__ fsc,
+                          const float *__restrict__ ema,
+                          const int32_t *__restrict__ used,
+                          const int32_t *__restrict__ face_ids,
+                          const int32_t *__restrict__ face_counts,
+                          unsigned long long *acc, int *mm, double *dacc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int u = used[i];
+  atomicAdd(&acc[0], (unsigned long long)u);
+  if (u >= FOS_MAX_NEIGHBORS) atomicAdd(&acc[1], 1ull);
+  atomicAdd(&acc[4], (unsigned long long)(long long)fsc[i]);
+  atomicMin(&mm[0], fsc[i]);
+  atomicMax(&mm[1], fsc[i]);
+  atomicAdd(&dacc[0], (double)ema[i]);
+  double s = 0.0;
+  int c = 0;
+  for (int k = 0; k < u && k < FOS_MAX_NEIGHBORS; ++k) {
+    const int j = face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + k];
+    if (j < 0 || j >= n) continue;
+    const int v = face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + k];
+    if (v <= 0) continue;
+    const float e = expected_area(rad[i], rad[j], voxel);
+    if (e > 0.0f) {
+      s += (double)((float)v / e);
+      c += 1;
+    }
+  }
+  if (c) {
+    atomicAdd(&dacc[1], s);
+    atomicAdd(&dacc[2], (double)c);
+  }
+}
+
+__global__ void k_stats_b(int n, const float *__restrict__ rad, float voxel,
+                          const int32_t *__restrict__ used,
+                          const int32_t *__restrict__ face_ids,
+                          const int32_t *__restrict__ face_counts, double *dacc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double mean = dacc[2] > 0 ? dacc[1] / dacc[2] : 0.0;
+  const int u = used[i];
+  double s = 0.0;
+  for (int k = 0; k < u && k < FOS_MAX_NEIGHBORS; ++k) {
+    const int j = face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + k];
+    if (j < 0 || j >= n) continue;
+    const int v = face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + k];
+    if (v <= 0) continue;
+    const float e = expected_area(rad[i], rad[j], voxel);
+    if (e > 0.0f) {
+      const double d = (double)((float)v / e) - mean;
+      s += d * d;
+    }
+  }
+  if (s != 0.0) atomicAdd(&dacc[3], s);
+}
+
+__global__ void k_validate(int n, const int32_t *__restrict__ nl,
+                           const int32_t *__restrict__ nc,
+                           unsigned long long *out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * FOS_MAX_NEIGHBORS) return;
+  const int i = (int)(t / FOS_MAX_NEIGHBORS), s = (int)(t % FOS_MAX_NEIGHBORS);
+  const int j = nl[t];
+  if (j == i) atomicAdd(&out[2], 1ull);
+  if (j >= 0 && j < n) {
+    bool found = false;
+    for (int k = 0; k < FOS_MAX_NEIGHBORS; ++k)
+      if (nl[(int64_t)j * FOS_MAX_NEIGHBORS + k] == i) {
+        found = true;
+        break;
+      }
+    if (!found) atomicAdd(&out[0], 1ull);
+  }
+  if (s == 0 && nc[i] >= FOS_MAX_NEIGHBORS) atomicAdd(&out[1], 1ull);
+}
+
+// jfa.py:983-1008; voxel -> tile (jfa.py:316-334) and wrapped halo marking
+__global__ void k_dirty_mark(const float *__restrict__ pos,
+                             const float *__restrict__ rad,
+                             const float *__restrict__ pos_prev,
+                             const float *__restrict__ rad_prev, int n, int res,
+                             float L, float half_l, float voxel, int tile,
+                             int halo, float pos_thr, float rad_thr,
+                             uint8_t *__restrict__ dirty) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float dx = pos[3 * i] - pos_prev[3 * i], dy = pos[3 * i + 1] - pos_prev[3 * i + 1],
+              dz = pos[3 * i + 2] - pos_prev[3 * i + 2];
+  float d2 = dx * dx + dy * dy;
+  d2 = d2 + dz * dz;
+  const float pd = sqrtf(d2);
+  const float rd = fabsf(rad[i] - rad_prev[i]);
+  int tc[3], tp[3];
+  for (int d = 0; d < 3; ++d) {
+    tc[d] = wrapi((int)floorf(((pos[3 * i + d] + half_l) / L) * (float)res), res) / tile;
+    tp[d] = wrapi((int)floorf(((pos_prev[3 * i + d] + half_l) / L) * (float)res), res) / tile;
+  }
+  const bool is_dirty = (pd > pos_thr * voxel) || (rd > rad_thr * voxel) ||
+                        tc[0] != tp[0] || tc[1] != tp[1] || tc[2] != tp[2];
+  if (!is_dirty) return;
+  const int tpa = (res + tile - 1) / tile;
+  for (int a = -halo; a <= halo; ++a)
+    for (int b = -halo; b <= halo; ++b)
+      for (int c = -halo; c <= halo; ++c) {
+        const int t0 = wrapi(tc[0] + c, tpa), t1 = wrapi(tc[1] + b, tpa),
+                  t2 = wrapi(tc[2] + a, tpa);
+        dirty[((int64_t)t0 * tpa + t1) * tpa + t2] = 1;
+      }
+}
+
+__global__ void k_dirty_count(const uint8_t *__restrict__ d, int64_t m,
+                              unsigned long long *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int v = (i < m && d[i] == 1) ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, (unsigned long long)v);
+}
+
+__global__ void k_tile_cache(const float *__restrict__ pos,
+                             const float *__restrict__ rad, int n,
+                             float *__restrict__ pp, float *__restrict__ rp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  pp[3 * i] = pos[3 * i];
+  pp[3 * i + 1] = pos[3 * i + 1];
+  pp[3 * i + 2] = pos[3 * i + 2];
+  rp[i] = rad[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static inline dim3 vox_grid(int res) {
+  return dim3((unsigned)((res + 63) / 64), (unsigned)((res + 3) / 4), (unsigned)res);
+}
+
+static inline bool unit_l(float L) { return L == 1.0f; }
+
+extern "C" {
+
+int fos_jfa_seed_table(fos_ctx *ctx, const float *pos, const float *rad, int n,
+                       float beta, void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && n <= ctx->max_n && (n == 0 || (pos && rad)),
+              "fos_jfa_seed_table: bad arguments (n=%d)", n);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_seed_table, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, pos, rad, n, beta, ctx->seeds);
+  FOS_CHECK_LAUNCH("k_seed_table");
+  return 0;
+}
+
+int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
+                 int res, float L, float voxel, float beta, int32_t *grid,
+                 int32_t *temp, void *stream) {
+  FOS_REQUIRE(ctx && grid && res >= 1 && res <= ctx->max_res && n >= 0 &&
+                  n <= ctx->max_n && L > 0.0f && voxel > 0.0f,
+              "fos_jfa_init: bad arguments (n=%d res=%d)", n, res);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t V = (int64_t)res * res * res;
+  int rc = fos_fill_i32(grid, V, -1, stream);
+  if (rc) return rc;
+  if (temp) {
+    rc = fos_fill_i32(temp, V, -1, stream);
+    if (rc) return rc;
+  }
+  if (n == 0) return 0;
+  rc = fos_jfa_seed_table(ctx, pos, rad, n, beta, stream);
+  if (rc) return rc;
+  const float half_l = (float)(0.5 * (double)L);
+  if (unit_l(L))
+    hipLaunchKernelGGL(k_seed<true>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st,
+                       pos, ctx->seeds, n, res, L, half_l, voxel, grid);
+  else
+    hipLaunchKernelGGL(k_seed<false>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st,
+                       pos, ctx->seeds, n, res, L, half_l, voxel, grid);
+  FOS_CHECK_LAUNCH("k_seed");
+  return 0;
+}
+
+static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
+                       const int32_t *in, int32_t *out, int32_t *changes,
+                       const int32_t *info, int pass, int early, double th,
+                       int min_passes, hipStream_t st) {
+  const float half_l = (float)(0.5 * (double)L);
+  if (unit_l(L))
+    hipLaunchKernelGGL(k_jfa_step<true>, vox_grid(res), dim3(64, 4), 0, st,
+                       ctx->seeds, res, L, half_l, voxel, step, in, out, changes,
+                       info, pass, early, th, min_passes);
+  else
+    hipLaunchKernelGGL(k_jfa_step<false>, vox_grid(res), dim3(64, 4), 0, st,
+                       ctx->seeds, res, L, half_l, voxel, step, in, out, changes,
+                       info, pass, early, th, min_passes);
+  FOS_CHECK_LAUNCH("k_jfa_step");
+  return 0;
+}
+
+int fos_jfa_step(fos_ctx *ctx, int res, float L, float voxel, int step_size,
+                 const int32_t *in, int32_t *out, int32_t *changes,
+                 void *stream) {
+  FOS_REQUIRE(ctx && in && out && in != out && res >= 1 && res <= ctx->max_res &&
+                  step_size >= 1 && step_size <= res && L > 0.0f,
+              "fos_jfa_step: bad arguments (res=%d step=%d)", res, step_size);
+  return launch_step(ctx, res, L, voxel, step_size, in, out, changes, nullptr, 0,
+                     0, 0.0, 0, (hipStream_t)stream);
+}
+
+int fos_jfa_copy(int res, const int32_t *src, int32_t *dst, void *stream) {
+  FOS_REQUIRE(src && dst && res >= 1, "fos_jfa_copy: bad arguments");
+  const int64_t V = (int64_t)res * res * res;
+  unsigned b = blocks_for(V);
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_copy_i32, dim3(b), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     src, dst, V);
+  FOS_CHECK_LAUNCH("k_copy_i32");
+  return 0;
+}
+
+int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
+                         int num_passes, int early_exit, double th_pass_delta,
+                         int min_passes, int32_t *grid, int32_t *temp,
+                         int32_t *info, void *const *events, void *stream) {
+  FOS_REQUIRE(ctx && grid && temp && info && grid != temp && res >= 1 &&
+                  res <= ctx->max_res && num_passes >= 0 && num_passes <= 64 &&
+                  L > 0.0f,
+              "fos_jfa_passes: bad arguments (res=%d passes=%d)", res, num_passes);
+  hipStream_t st = (hipStream_t)stream;
+  int rc = fos_fill_i32(info, 2 + num_passes, 0, stream);
+  if (rc) return rc;
+  int step = res / 2;
+  for (int p = 0; p < num_passes; ++p) {
+    if (step < 1) step = 1;
+    const int32_t *in = (p & 1) ? temp : grid;
+    int32_t *out = (p & 1) ? grid : temp;
+    if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[p], st));
+    rc = launch_step(ctx, res, L, voxel, step, in, out, info + 2 + p, info, p,
+                     early_exit, th_pass_delta, min_passes, st);
+    if (rc) return rc;
+    step /= 2;
+  }
+  if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[num_passes], st));
+  const int64_t V = (int64_t)res * res * res;
+  unsigned b = blocks_for(V);
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_jfa_finish, dim3(b), dim3(FOS_BLOCK), 0, st, res, num_passes,
+                     early_exit, th_pass_delta, min_passes, info, temp, grid);
+  FOS_CHECK_LAUNCH("k_jfa_finish");
+  return 0;
+}
+
+int fos_jfa_passes(fos_ctx *ctx, int res, float L, float voxel, int num_passes,
+                   int early_exit, double th_pass_delta, int min_passes,
+                   int32_t *grid, int32_t *temp, int32_t *info, void *stream) {
+  return fos_jfa_passes_timed(ctx, res, L, voxel, num_passes, early_exit, th_pass_delta,
+                              min_passes, grid, temp, info, nullptr, stream);
+}
+
+int fos_events_create(int count, void **events) {
+  FOS_REQUIRE(count >= 1 && events, "fos_events_create: bad arguments");
+  for (int i = 0; i < count; ++i) {
+    hipEvent_t e;
+    FOS_HIP(hipEventCreate(&e));
+    events[i] = (void *)e;
+  }
+  return 0;
+}
+
+int fos_events_destroy(int count, void **events) {
+  FOS_REQUIRE(count >= 0 && (count == 0 || events), "fos_events_destroy: bad arguments");
+  for (int i = 0; i < count; ++i)
+    if (events[i]) (void)hipEventDestroy((hipEvent_t)events[i]);
+  return 0;
+}
+
+int fos_events_elapsed(int count, void *const *events, float *ms) {
+  FOS_REQUIRE(count >= 2 && events && ms, "fos_events_elapsed: bad arguments");
+  FOS_HIP(hipEventSynchronize((hipEvent_t)events[count - 1]));
+  for (int i = 0; i + 1 < count; ++i)
+    FOS_HIP(hipEventElapsedTime(&ms[i], (hipEvent_t)events[i], (hipEvent_t)events[i + 1]));
+  return 0;
+}
+
+int fos_majority(int res, const int32_t *grid, int32_t *filtered, void *stream) {
+  FOS_REQUIRE(grid && filtered && grid != filtered && res >= 1,
+              "fos_majority: bad arguments");
+  hipLaunchKernelGGL(k_majority, vox_grid(res), dim3(64, 4), 0, (hipStream_t)stream,
+                     res, grid, filtered);
+  FOS_CHECK_LAUNCH("k_majority");
+  return 0;
+}
+
+int fos_face_accumulate(fos_ctx *ctx, int res, int n, const int32_t *filtered,
+                        int32_t *face_ids, int32_t *face_counts, void *stream) {
+  FOS_REQUIRE(ctx && filtered && face_ids && face_counts && res >= 1 &&
+                  res <= ctx->max_res && n >= 0 && n <= ctx->max_n,
+              "fos_face_accumulate: bad arguments (n=%d res=%d)", n, res);
+  FOS_REQUIRE((double)res * res * res * 6.0 < 4294967295.0,
+              "fos_face_accumulate: res=%d exceeds the 32-bit face key", res);
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  while (true) {
+    hipLaunchKernelGGL(k_face_emit, vox_grid(res), dim3(64, 4), 0, st, res, n,
+                       filtered, ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap,
+                       ctx->flags);
+    FOS_CHECK_LAUNCH("k_face_emit");
+    FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    FOS_HIP(hipStreamSynchronize(st));
+    const int overflow = *(volatile int *)ctx->host_i;
+    if (!overflow) break;
+    // A row met more distinct partners than the table holds: grow the
+    // tables and emit again from scratch.
+    const int cap = ctx->tab_cap * 4;
+    FOS_REQUIRE(cap <= 4096, "fos_face_accumulate: row table overflow at %d",
+                ctx->tab_cap);
+    int rc = alloc_tables(ctx, cap);
+    if (rc) return rc;
+    rc = fos_fill_i32(ctx->flags, 1, 0, stream);
+    if (rc) return rc;
+  }
+  const int cap = ctx->tab_cap;
+  const size_t shmem = (size_t)4 * cap * sizeof(unsigned);
+  hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st,
+                     n, ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts);
+  FOS_CHECK_LAUNCH("k_face_compact");
+  return 0;
+}
+
+int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
+                       const int32_t *face_ids, const int32_t *face_counts,
+                       float *prev_scores, int32_t *neighbor_list,
+                       int32_t *neighbor_count, int32_t *used_count,
+                       int32_t *fsc, void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && n <= ctx->max_n && voxel > 0.0f &&
+                  (n == 0 || (rad && face_ids && face_counts && prev_scores &&
+                              neighbor_list && neighbor_count && used_count && fsc)),
+              "fos_finalize_faces: bad arguments (n=%d)", n);
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t m = (int64_t)n * FOS_MAX_NEIGHBORS;
+  int rc = fos_fill_i32(ctx->in_count, n + 1, 0, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fin_classify, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
+                     rad, voxel, face_ids, face_counts, prev_scores, ctx->score,
+                     ctx->cls, ctx->in_count);
+  FOS_CHECK_LAUNCH("k_fin_classify");
+  rc = exclusive_scan(ctx, ctx->in_count, ctx->in_start, n + 1, st);
+  if (rc) return rc;
+  rc = fos_fill_i32(ctx->in_fill, n, 0, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fin_scatter, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
+                     face_ids, ctx->cls, ctx->in_start, ctx->in_fill, ctx->in_src);
+  FOS_CHECK_LAUNCH("k_fin_scatter");
+  hipLaunchKernelGGL(k_fin_sort_in, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+                     ctx->in_start, ctx->in_src);
+  FOS_CHECK_LAUNCH("k_fin_sort_in");
+  rc = fos_fill_i32(ctx->acc_in, n, 0, stream);
+  if (rc) return rc;
+  // Fixed point; every round is cheap, the host reads one flag per round.
+  for (int round = 0; round <= n + 1; ++round) {
+    rc = fos_fill_i32(ctx->flags + 1, 1, 0, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fin_decide, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+                       ctx->cls, ctx->acc_in, ctx->dec);
+    FOS_CHECK_LAUNCH("k_fin_decide");
+    hipLaunchKernelGGL(k_fin_count, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+                       ctx->in_start, ctx->in_src, ctx->dec, ctx->acc_in, ctx->flags + 1);
+    FOS_CHECK_LAUNCH("k_fin_count");
+    FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags + 1, sizeof(int),
+                           hipMemcpyDeviceToHost, st));
+    FOS_HIP(hipStreamSynchronize(st));
+    if (!*(volatile int *)ctx->host_i) break;
+  }
+  hipLaunchKernelGGL(k_fin_materialize, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+                     face_ids, ctx->score, ctx->cls, ctx->dec, ctx->in_start,
+                     ctx->in_src, prev_scores, neighbor_list, neighbor_count,
+                     used_count, fsc);
+  FOS_CHECK_LAUNCH("k_fin_materialize");
+  return 0;
+}
+
+int fos_fsc_ema(int n, float alpha, const int32_t *fsc, float *fsc_ema,
+                void *stream) {
+  FOS_REQUIRE(n >= 0 && (n == 0 || (fsc && fsc_ema)), "fos_fsc_ema: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_fsc_ema, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, n, alpha, fsc, fsc_ema);
+  FOS_CHECK_LAUNCH("k_fsc_ema");
+  return 0;
+}
+
+int fos_jfa_stats(fos_ctx *ctx, int n, const float *rad, float voxel,
+                  const int32_t *fsc, const float *fsc_ema,
+                  const int32_t *used_count, const int32_t *face_ids,
+                  const int32_t *face_counts, int64_t *out_i, double *out_f,
+                  void *stream) {
+  FOS_REQUIRE(ctx && n >= 1 && n <= ctx->max_n && rad && fsc && fsc_ema &&
+                  used_count && face_ids && face_counts && out_i && out_f,
+              "fos_jfa_stats: bad arguments (n=%d)", n);
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long *acc = (unsigned long long *)ctx->iscratch;  // 5 x u64
+  int *mm = ctx->iscratch + 10;
+  double *dacc = (double *)ctx->fscratch;  // 4 doubles
+  int rc = fos_fill_i32(ctx->iscratch, 10, 0, stream);
+  if (rc) return rc;
+  rc = fos_fill_i32(mm, 1, 0x7fffffff, stream);
+  if (rc) return rc;
+  rc = fos_fill_i32(mm + 1, 1, (int32_t)0x80000000, stream);
+  if (rc) return rc;
+  rc = fos_fill_i32((int32_t *)dacc, 8, 0, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_stats_a, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n, rad,
+                     voxel, fsc, fsc_ema, used_count, face_ids, face_counts, acc, mm,
+                     dacc);
+  FOS_CHECK_LAUNCH("k_stats_a");
+  hipLaunchKernelGGL(k_stats_b, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n, rad,
+                     voxel, used_count, face_ids, face_counts, dacc);
+  FOS_CHECK_LAUNCH("k_stats_b");
+  unsigned long long h_acc[5];
+  int h_mm[2];
+  double h_d[4];
+  FOS_HIP(hipMemcpyAsync(h_acc, acc, sizeof(h_acc), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipMemcpyAsync(h_mm, mm, sizeof(h_mm), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipMemcpyAsync(h_d, dacc, sizeof(h_d), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipStreamSynchronize(st));
+  out_i[0] = (int64_t)h_acc[0];
+  out_i[1] = (int64_t)h_acc[1];
+  out_i[2] = h_mm[0];
+  out_i[3] = h_mm[1];
+  out_i[4] = (int64_t)h_acc[4];
+  out_f[0] = (double)out_i[4] / (double)n;
+  out_f[1] = h_d[0] / (double)n;
+  out_f[2] = h_d[2] > 0 ? h_d[1] / h_d[2] : 0.0;
+  out_f[3] = h_d[2] > 0 ? sqrt(h_d[3] / h_d[2]) : 0.0;
+  return 0;
+}
+
+int fos_validate(fos_ctx *ctx, int n, const int32_t *neighbor_list,
+                 const int32_t *neighbor_count, int64_t *out, void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && n <= ctx->max_n && out &&
+                  (n == 0 || (neighbor_list && neighbor_count)),
+              "fos_validate: bad arguments (n=%d)", n);
+  out[0] = out[1] = out[2] = 0;
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long *acc = (unsigned long long *)ctx->iscratch;
+  int rc = fos_fill_i32(ctx->iscratch, 6, 0, stream);
+  if (rc) return rc;
+  const int64_t m = (int64_t)n * FOS_MAX_NEIGHBORS;
+  hipLaunchKernelGGL(k_validate, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
+                     neighbor_list, neighbor_count, acc);
+  FOS_CHECK_LAUNCH("k_validate");
+  unsigned long long h[3];
+  FOS_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipStreamSynchronize(st));
+  out[0] = (int64_t)h[0];
+  out[1] = (int64_t)h[1];
+  out[2] = (int64_t)h[2];
+  return 0;
+}
+
+int fos_dirty_mark(const float *pos, const float *rad, const float *pos_prev,
+                   const float *rad_prev, int n, int res, float L, float voxel,
+                   int tile, int halo, float pos_thr, float rad_thr,
+                   uint8_t *tile_dirty, void *stream) {
+  FOS_REQUIRE(n >= 0 && res >= 1 && tile >= 1 && halo >= 0 && tile_dirty &&
+                  (n == 0 || (pos && rad && pos_prev && rad_prev)),
+              "fos_dirty_mark: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_dirty_mark, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, pos, rad, pos_prev, rad_prev, n, res, L,
+                     (float)(0.5 * (double)L), voxel, tile, halo, pos_thr, rad_thr,
+                     tile_dirty);
+  FOS_CHECK_LAUNCH("k_dirty_mark");
+  return 0;
+}
+
+int fos_dirty_count(fos_ctx *ctx, const uint8_t *tile_dirty, int tiles,
+                    int64_t *count, void *stream) {
+  FOS_REQUIRE(ctx && tile_dirty && tiles >= 1 && count, "fos_dirty_count: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t m = (int64_t)tiles * tiles * tiles;
+  unsigned long long *acc = (unsigned long long *)ctx->iscratch;
+  int rc = fos_fill_i32(ctx->iscratch, 2, 0, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dirty_count, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st,
+                     tile_dirty, m, acc);
+  FOS_CHECK_LAUNCH("k_dirty_count");
+  unsigned long long h = 0;
+  FOS_HIP(hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipStreamSynchronize(st));
+  *count = (int64_t)h;
+  return 0;
+}
+
+int fos_tile_cache_update(const float *pos, const float *rad, int n,
+                          float *pos_prev, float *rad_prev, void *stream) {
+  FOS_REQUIRE(n >= 0 && (n == 0 || (pos && rad && pos_prev && rad_prev)),
+              "fos_tile_cache_update: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_tile_cache, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, pos, rad, n, pos_prev, rad_prev);
+  FOS_CHECK_LAUNCH("k_tile_cache");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+"""Headless per-frame foam step on one GPU (the caller contract of run.py).
+
+One ``FoamSim.step()`` reproduces the order of the reference main loop
+(run.py:700-1261) without GUI, render or prints:
+
+  0  rebuild_grid                                   run.py:704
+  1  compute_max_overlap -> adaptive PBD passes     run.py:709-714
+  2  settle: nudge_radii_adaptive_ema               run.py:733-751
+  3  equilibrate_pressure x pressure_iters          run.py:755-771
+  4  apply_brownian_motion                          run.py:807-813
+  5  passes x (rebuild_grid, project_overlaps)      run.py:829-846
+  6  OU jitter (apply_brownian + integrate_jitter)  run.py:857-867
+  7  rebuild_grid                                   run.py:874
+  8  JFA (on cadence frames) -> FSC, EMA            run.py:1074
+  9  warm-start gate, set_fsc_targets, settle reset run.py:1184-1231
+
+XSPH and global damping act on a velocity field that the reference never
+makes non-zero (rescue is off), so they are not run here (the drop-in
+``dynamics`` module still exposes them).  PBD is the Jacobi form: every
+particle reads the pass-start positions.
+
+All work is enqueued on torch's current stream through libfos; the host
+synchronises once per frame to read the overlap maximum (it sets the PBD pass
+count, as in the reference) and once per JFA frame for the warm-start gate.
+"""
+import math
+
+import numpy as np
+import torch
+
+import fos_lib as fl
+import foam_config as fc
+
+MAXN = 32
+
+
+class FoamSim:
+    def __init__(self, cfg, pos, rad, device="cuda", telemetry=False):
+        fl.require_gpu()
+        self.cfg = cfg
+        self.telemetry = telemetry
+        n = int(cfg.n)
+        dev = torch.device(device)
+        self.n = n
+        self.pos = torch.as_tensor(np.asarray(pos, np.float32).reshape(n, 3), device=dev).contiguous()
+        self.rad = torch.as_tensor(np.asarray(rad, np.float32).reshape(n), device=dev).contiguous()
+        self.pos_tmp = torch.empty_like(self.pos)
+        self.rad_target = self.rad.clone()
+        self.lmd = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.delta_r = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.v_jit = torch.zeros(n, 3, dtype=torch.float32, device=dev)
+        self.G = cfg.hash_res()
+        C = self.G ** 3
+        self.cell_count = torch.zeros(C, dtype=torch.int32, device=dev)
+        self.cell_start = torch.zeros(C, dtype=torch.int32, device=dev)
+        self.cell_indices = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.res = int(cfg.res)
+        V = self.res ** 3
+        self.grid = torch.full((V,), -1, dtype=torch.int32, device=dev)
+        self.temp = torch.full((V,), -1, dtype=torch.int32, device=dev)
+        self.filtered = torch.full((V,), -1, dtype=torch.int32, device=dev)
+        self.face_ids = torch.full((n, MAXN), -1, dtype=torch.int32, device=dev)
+        self.face_counts = torch.zeros(n, MAXN, dtype=torch.int32, device=dev)
+        self.fsc = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.fsc_ema = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.neighbor_list = torch.full((n, MAXN), -1, dtype=torch.int32, device=dev)
+        self.neighbor_count = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.used_count = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.prev_scores = torch.zeros(n, MAXN, dtype=torch.float32, device=dev)
+        self.info = torch.zeros(2 + 64, dtype=torch.int32, device=dev)
+        self.pstats = torch.zeros(2, dtype=torch.float32, device=dev)
+        self.maxd = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.ctx = fl.context(n, self.res, C)
+        self.frame = 0
+        self.settle = 0
+        self.growth = True
+        self.last = {}
+        # optional list of ctypes arrays of passes+1 hipEvent_t handles; each
+        # JFA frame consumes one to time its pass kernels on the stream
+        self.pass_events = None
+
+    # -- building blocks ------------------------------------------------------
+    def rebuild_grid(self):
+        c = self.cfg
+        fl.call("fos_grid_build", self.ctx.handle, fl.ptr(self.pos), self.n, self.G, c.L,
+                fl.ptr(self.cell_count), fl.ptr(self.cell_start), fl.ptr(self.cell_indices),
+                fl.stream())
+
+    def max_overlap(self):
+        c = self.cfg
+        fl.call("fos_max_overlap", fl.ptr(self.pos), fl.ptr(self.rad), self.n, self.G, c.L,
+                fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
+                fl.ptr(self.lmd), fl.ptr(self.maxd), fl.stream())
+
+    def pbd_pass(self):
+        c = self.cfg
+        fl.call("fos_pbd_pass", fl.ptr(self.pos), fl.ptr(self.pos_tmp), fl.ptr(self.rad),
+                self.n, self.G, c.L, c.gap_factor(), c.max_disp_frac, c.disp_mult,
+                fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
+                fl.stream())
+        self.pos, self.pos_tmp = self.pos_tmp, self.pos
+
+    def pressure(self, frame):
+        c = self.cfg
+        for _ in range(c.pressure_iters):
+            fl.call("fos_pressure_iter", self.n, frame, fl.ptr(self.rad), fl.ptr(self.delta_r),
+                    fl.ptr(self.face_ids), fl.ptr(self.fsc), c.pressure_k, c.pressure_exp,
+                    c.pressure_cap, c.pressure_max_nei, c.r_min, c.r_max,
+                    fl.ptr(self.pstats), fl.stream())
+
+    def run_jfa(self):
+        """jfa.run_jfa without the host copies (jfa.py:1078-1132)."""
+        c, ctx, st = self.cfg, self.ctx.handle, fl.stream()
+        voxel = c.voxel()
+        fl.call("fos_jfa_init", ctx, fl.ptr(self.pos), fl.ptr(self.rad), self.n, self.res, c.L,
+                voxel, c.beta, fl.ptr(self.grid), None, st)
+        ev = self.pass_events.pop(0) if self.pass_events else None
+        fl.call("fos_jfa_passes_timed", ctx, self.res, c.L, voxel, c.passes(),
+                int(c.early_exit), c.th_pass_delta, c.min_passes, fl.ptr(self.grid),
+                fl.ptr(self.temp), fl.ptr(self.info), ev, st)
+        fl.call("fos_majority", self.res, fl.ptr(self.grid), fl.ptr(self.filtered), st)
+        fl.call("fos_face_accumulate", ctx, self.res, self.n, fl.ptr(self.filtered),
+                fl.ptr(self.face_ids), fl.ptr(self.face_counts), st)
+        fl.call("fos_finalize_faces", ctx, self.n, fl.ptr(self.rad), voxel,
+                fl.ptr(self.face_ids), fl.ptr(self.face_counts), fl.ptr(self.prev_scores),
+                fl.ptr(self.neighbor_list), fl.ptr(self.neighbor_count),
+                fl.ptr(self.used_count), fl.ptr(self.fsc), st)
+        fl.call("fos_fsc_ema", self.n, c.ema_alpha, fl.ptr(self.fsc), fl.ptr(self.fsc_ema), st)
+
+    # -- one frame ------------------------------------------------------------
+    def step(self):
+        c = self.cfg
+        f = self.frame
+        st = fl.stream()
+        # 0-1
+        self.rebuild_grid()
+        self.max_overlap()
+        max_depth = float(self.maxd.item())
+        passes = max(c.pbd_base, min(c.pbd_max,
+                                     int(c.pbd_base + c.pbd_scale * (max_depth / (0.2 * c.r_max)))))
+        # 2 settle
+        if self.settle > 0:
+            back = max(0.25, min(1.0, (c.contact_tol - max_depth) / c.contact_tol))
+            fl.call("fos_nudge_radii", fl.ptr(self.rad), fl.ptr(self.rad_target), fl.ptr(self.lmd),
+                    self.n, c.alpha(), c.max_step_pct, c.r_min, c.r_max, back,
+                    c.backpressure_mode, c.contact_tol, st)
+            self.settle -= 1
+        # 3 pressure
+        if c.pressure:
+            self.pressure(f)
+        # 4 Brownian
+        if c.brownian:
+            fl.call("fos_brownian", fl.ptr(self.pos), self.n, c.brownian_strength, c.L,
+                    c.seed, f, c.brownian_mode, st)
+        # 5 PBD
+        for _ in range(passes):
+            self.rebuild_grid()
+            self.pbd_pass()
+        # 6 OU jitter
+        if c.ou_jitter:
+            mean_r = float(self.rad.mean().item())
+            fl.call("fos_ou_jitter", fl.ptr(self.pos), fl.ptr(self.v_jit), fl.ptr(self.rad),
+                    self.n, mean_r, fc.DT, fc.JITTER_RMS, fc.JITTER_TAU,
+                    fc.f32(fc.MAX_DRIFT_FRACTION * c.gap_fraction), c.L, c.seed, f, st)
+        # 7
+        self.rebuild_grid()
+        # 8 JFA on cadence frames
+        ran = c.jfa_runs(f)
+        if ran:
+            self.run_jfa()
+            f0 = float((self.fsc == 0).sum().item()) / float(max(1, self.n))
+            self.growth = not ((f < c.warmstart_frames) or (f0 > c.fsc_zero_rate))
+        else:
+            self.growth = True
+        # 9 controller
+        if self.growth:
+            fl.call("fos_fsc_targets", fl.ptr(self.rad), fl.ptr(self.rad_target), fl.ptr(self.fsc),
+                    fl.ptr(self.fsc_ema), self.n, c.f_low, c.f_high, c.growth_pct, c.r_min,
+                    c.r_max, c.deadband, st)
+            self.settle = c.adjustment_frames
+        self.last = {"frame": f, "pbd_passes": passes, "max_depth": max_depth,
+                     "jfa": ran, "growth": self.growth}
+        if self.telemetry and ran:
+            self.last.update(self.jfa_telemetry())
+        self.frame += 1
+        return self.last
+
+    def jfa_telemetry(self):
+        """jfa.py:1139-1195 stats and jfa.py:1400-1426 validation."""
+        c = self.cfg
+        oi = (fl.I64 * 5)()
+        of = (fl.D * 4)()
+        fl.call("fos_jfa_stats", self.ctx.handle, self.n, fl.ptr(self.rad), c.voxel(),
+                fl.ptr(self.fsc), fl.ptr(self.fsc_ema), fl.ptr(self.used_count),
+                fl.ptr(self.face_ids), fl.ptr(self.face_counts), oi, of, fl.stream())
+        vo = (fl.I64 * 3)()
+        fl.call("fos_validate", self.ctx.handle, self.n, fl.ptr(self.neighbor_list),
+                fl.ptr(self.neighbor_count), vo, fl.stream())
+        info = self.info.cpu().numpy()
+        return {"num_passes": int(info[0]), "early_exit": bool(info[1]),
+                "p_dir": int(oi[0]), "overflow_count": int(oi[1]), "min_fsc": int(oi[2]),
+                "max_fsc": int(oi[3]), "mean_fsc": of[0], "mean_fsc_ema": of[1],
+                "mean_score": of[2], "std_score": of[3], "asymmetric_pairs": int(vo[0]),
+                "self_loops": int(vo[2])}
+
+    # -- state ------------------------------------------------------------------
+    STATE = ("pos", "rad", "rad_target", "lmd", "v_jit", "face_ids", "face_counts", "fsc",
+             "fsc_ema", "neighbor_list", "neighbor_count", "used_count", "prev_scores")
+
+    def snapshot(self):
+        torch.cuda.synchronize()
+        out = {k: getattr(self, k).detach().cpu().numpy().copy() for k in self.STATE}
+        out["frame"] = np.int64(self.frame)
+        out["settle"] = np.int64(self.settle)
+        return out
+
+    def restore(self, snap):
+        for k in self.STATE:
+            getattr(self, k).copy_(torch.from_numpy(np.ascontiguousarray(snap[k])))
+        self.frame = int(snap["frame"])
+        self.settle = int(snap["settle"])
+
+    def save(self, path):
+        np.savez(path, **self.snapshot())
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.restore({k: z[k] for k in z.files})
+
+    def export_csv(self, path):
+        """run.py:608-629 particle dump: ID,X,Y,Z,Radius,FSC."""
+        p = self.pos.cpu().numpy()
+        r = self.rad.cpu().numpy()
+        f = self.fsc.cpu().numpy()
+        with open(path, "w") as fh:
+            fh.write("ID,X,Y,Z,Radius,FSC\n")
+            for i in range(self.n):
+                fh.write(f"{i},{p[i,0]:.6f},{p[i,1]:.6f},{p[i,2]:.6f},{r[i]:.6f},{int(f[i])}\n")

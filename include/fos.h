@@ -1,0 +1,235 @@
+/*
+ * fos.h -- C ABI of libfos.so, the MI355X (gfx950) implementation of the
+ * per-frame foam step of VirtualOrganics/Fabric-of-Space-Taichi-JFA.
+ *
+ * Every entry point takes raw device pointers (contiguous, caller-owned),
+ * sizes, scalars and a hipStream_t passed as void* (NULL = default stream).
+ * Work is enqueued on that stream; functions that return values on the host
+ * (pointers documented "host") synchronise that stream first.  Nothing is
+ * allocated per call: scratch lives in a context made by fos_ctx_create.
+ *
+ * Return value: 0 on success, FOS_EINVAL (-22) for bad arguments, or
+ * -(1000 + hipError_t) for a HIP runtime error.  fos_last_error() gives the
+ * text of the last failure on the calling thread.
+ *
+ * Layouts: pos float[n][3] (x, y, z), rad float[n], labels int32[res^3]
+ * indexed x*res*res + y*res + z, per-particle rows int32/float[n][32],
+ * hash cells int32[G^3] indexed cx*G*G + cy*G + cz.
+ *
+ * Each declaration cites the reference function (file:line) it replaces.
+ */
+#ifndef FOS_H
+#define FOS_H
+#include <stdint.h>
+
+#define FOS_OK 0
+#define FOS_EINVAL (-22)
+#define FOS_ABI_VERSION 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fos_ctx fos_ctx;
+
+int fos_abi_version(void);
+const char *fos_last_error(void);
+
+/* Context: scratch for up to max_n particles, res <= max_res voxels per axis,
+ * max_cells hash cells.  Replaces the field allocation of jfa.init_jfa
+ * (jfa.py:120-204), grid/dynamics fields (run.py:129-199). */
+int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells);
+int fos_ctx_destroy(fos_ctx *ctx);
+
+/* Device fills (Taichi field clears, e.g. grid.py:146-153 clear_grid). */
+int fos_fill_i32(int32_t *dst, int64_t count, int32_t value, void *stream);
+int fos_fill_f32(float *dst, int64_t count, float value, void *stream);
+
+/* ---- jfa.py ------------------------------------------------------------ */
+
+/* Per-frame seed table (x, y, z, (beta*r)^2) that the JFA kernels gather;
+ * the reference recomputes the same terms per candidate (jfa.py:470-477). */
+int fos_jfa_seed_table(fos_ctx *ctx, const float *pos, const float *rad,
+                       int n, float beta, void *stream);
+/* jfa.py:485-520 jfa_init: clears grid and temp (res^3) to -1, builds the
+ * seed table, seeds every particle's voxel (argmin power distance, lowest
+ * index on ties). */
+int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
+                 int res, float L, float voxel, float beta, int32_t *grid,
+                 int32_t *temp, void *stream);
+/* jfa.py:524-623 jfa_step_ping0/1: one pass in -> out at step_size over the
+ * seed table built by the last fos_jfa_init/fos_jfa_seed_table; adds the
+ * changed-voxel count into *changes (device int32). */
+int fos_jfa_step(fos_ctx *ctx, int res, float L, float voxel, int step_size,
+                 const int32_t *in, int32_t *out, int32_t *changes,
+                 void *stream);
+/* jfa.py:626-632 jfa_copy_temp_to_grid. */
+int fos_jfa_copy(int res, const int32_t *src, int32_t *dst, void *stream);
+/* jfa.py:1078-1116 run_jfa pass loop with the early exit decided on the
+ * device (no host sync).  info (device int32[2 + num_passes]) receives
+ * [0] passes executed, [1] early-exit flag, [2+p] changes of pass p.  The
+ * final labels end in grid. */
+int fos_jfa_passes(fos_ctx *ctx, int res, float L, float voxel,
+                   int num_passes, int early_exit, double th_pass_delta,
+                   int min_passes, int32_t *grid, int32_t *temp,
+                   int32_t *info, void *stream);
+/* fos_jfa_passes that also records events[p] (hipEvent_t) on the stream
+ * before pass p and events[num_passes] after the last pass, so each pass
+ * kernel can be timed on the stream it runs on.  No reference counterpart:
+ * per-pass timing for the benchmark (the reference times JFA on the host,
+ * run.py:1071-1080). */
+int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
+                         int num_passes, int early_exit, double th_pass_delta,
+                         int min_passes, int32_t *grid, int32_t *temp,
+                         int32_t *info, void *const *events, void *stream);
+/* hipEvent_t helpers for fos_jfa_passes_timed: create count events, destroy
+ * them, and read the count-1 elapsed intervals (ms) between consecutive
+ * events after waiting for the last one. */
+int fos_events_create(int count, void **events);
+int fos_events_destroy(int count, void **events);
+int fos_events_elapsed(int count, void *const *events, float *ms);
+/* jfa.py:639-699 majority_filter_labels: grid -> filtered. */
+int fos_majority(int res, const int32_t *grid, int32_t *filtered,
+                 void *stream);
+/* jfa.py:702-762 accumulate_face_area: face_ids/face_counts rows [n][32] in
+ * first-encounter order (serial semantics).  Synchronises the stream once to
+ * check the row tables for overflow. */
+int fos_face_accumulate(fos_ctx *ctx, int res, int n, const int32_t *filtered,
+                        int32_t *face_ids, int32_t *face_counts,
+                        void *stream);
+/* jfa.py:788-857 finalize_faces_kernel (+ add_neighbor jfa.py:400-443) with
+ * the serial-order hysteresis resolved by a fixed point over rows.
+ * prev_scores persists across frames.  Synchronises per fixed-point round. */
+int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
+                       const int32_t *face_ids, const int32_t *face_counts,
+                       float *prev_scores, int32_t *neighbor_list,
+                       int32_t *neighbor_count, int32_t *used_count,
+                       int32_t *fsc, void *stream);
+/* jfa.py:765-785 update_fsc_ema. */
+int fos_fsc_ema(int n, float alpha, const int32_t *fsc, float *fsc_ema,
+                void *stream);
+/* jfa.py:1204-1330 compute_pair_symmetry / compute_overflow_count /
+ * compute_fsc_stats / compute_score_stats.  Host out_i[5] = {p_dir,
+ * overflow, min_fsc, max_fsc, sum_fsc}; host out_f[4] = {mean_fsc,
+ * mean_fsc_ema, mean_score, std_score}. */
+int fos_jfa_stats(fos_ctx *ctx, int n, const float *rad, float voxel,
+                  const int32_t *fsc, const float *fsc_ema,
+                  const int32_t *used_count, const int32_t *face_ids,
+                  const int32_t *face_counts, int64_t *out_i, double *out_f,
+                  void *stream);
+/* jfa.py:1337-1397 check_symmetry / check_overflow / check_self_loops.
+ * Host out[3] = {asymmetric, overflow, self_loops}. */
+int fos_validate(fos_ctx *ctx, int n, const int32_t *neighbor_list,
+                 const int32_t *neighbor_count, int64_t *out, void *stream);
+/* jfa.py:966-1008 mark_dirty_tiles (tile_dirty uint8[tpa^3]). */
+int fos_dirty_mark(const float *pos, const float *rad, const float *pos_prev,
+                   const float *rad_prev, int n, int res, float L,
+                   float voxel, int tile, int halo, float pos_thr,
+                   float rad_thr, uint8_t *tile_dirty, void *stream);
+/* jfa.py:1011-1023 count_dirty_tiles (host *count). */
+int fos_dirty_count(fos_ctx *ctx, const uint8_t *tile_dirty, int tiles,
+                    int64_t *count, void *stream);
+/* jfa.py:1036-1049 update_tile_cache. */
+int fos_tile_cache_update(const float *pos, const float *rad, int n,
+                          float *pos_prev, float *rad_prev, void *stream);
+
+/* ---- grid.py ----------------------------------------------------------- */
+
+/* grid.py:189-222 count_particles_per_cell (cell_count must be cleared). */
+int fos_grid_count(const float *pos, int n, int G, float L,
+                   int32_t *cell_count, void *stream);
+/* grid.py:229-243 prefix_sum: exclusive scan over G^3 cells. */
+int fos_grid_scan(fos_ctx *ctx, const int32_t *cell_count,
+                  int32_t *cell_start, int G, void *stream);
+/* grid.py:268-309 scatter_particles: stores wrapped positions back into pos
+ * and writes cell_indices, ascending particle index within each cell
+ * (stable).  cell_write holds cell_start on entry and the cell ends on exit,
+ * as in the reference. */
+int fos_grid_scatter(float *pos, int n, int G, float L, int32_t *cell_write,
+                     int32_t *cell_indices, void *stream);
+/* run.py:476-504 rebuild_grid: clear, count, scan, copy, scatter. */
+int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
+                   int32_t *cell_count, int32_t *cell_start,
+                   int32_t *cell_indices, void *stream);
+
+/* ---- dynamics.py ------------------------------------------------------- */
+
+/* dynamics.py:77-171 project_overlaps, Jacobi form: reads pos_in, writes
+ * pos_out (distinct buffers).  gap_factor = (float)(1 + GAP_FRACTION). */
+int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad,
+                 int n, int G, float L, float gap_factor, float max_disp_frac,
+                 float disp_mult, const int32_t *cell_start,
+                 const int32_t *cell_count, const int32_t *cell_indices,
+                 void *stream);
+/* dynamics.py:304-372 compute_local_max_overlaps + reduce_max_depth: fills
+ * local_max_depth and the device float *max_depth (a true maximum). */
+int fos_max_overlap(const float *pos, const float *rad, int n, int G, float L,
+                    const int32_t *cell_start, const int32_t *cell_count,
+                    const int32_t *cell_indices, float *local_max_depth,
+                    float *max_depth, void *stream);
+/* dynamics.py:1242-1320 set_fsc_targets. */
+int fos_fsc_targets(const float *rad, float *rad_target, const int32_t *fsc,
+                    const float *fsc_ema, int n, int f_low, int f_high,
+                    float growth_pct, float r_min, float r_max,
+                    float deadband, void *stream);
+/* dynamics.py:1322-1363 compute_controller_stats.  Host out_i[3] = {grow,
+ * shrink, idle}; host out_f[2] = {mean_dr, max_abs_dr}. */
+int fos_controller_stats(fos_ctx *ctx, const float *rad,
+                         const float *rad_target, const float *fsc_ema, int n,
+                         int f_low, int f_high, int64_t *out_i, double *out_f,
+                         void *stream);
+/* dynamics.py:1366-1427 nudge_radii_adaptive_ema. */
+int fos_nudge_radii(float *rad, const float *rad_target,
+                    const float *local_max_depth, int n, float alpha,
+                    float max_step_pct, float r_min, float r_max,
+                    float back_global, int mode, float contact_tol,
+                    void *stream);
+/* dynamics.py:1454-1557 equilibrate_pressure, one Jacobi iteration: delta_r
+ * from the current radii, then the clamped apply.  stats (device float[2])
+ * receives {max|dr|, changed count as float bits of an int32} and is
+ * overwritten by this call. */
+int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
+                      const int32_t *face_ids, const int32_t *fsc, float k,
+                      float p_exp, float pair_cap, int max_nei, float r_min,
+                      float r_max, float *stats, void *stream);
+/* dynamics.py:1560-1589 compute_pressure_stats: host out[2] = {rmin, rmax}. */
+int fos_pressure_stats(fos_ctx *ctx, const float *rad, int n, float *out,
+                       void *stream);
+/* dynamics.py:1596-1634 apply_brownian_motion with Philox4x32-10 noise keyed
+ * (seed, frame, particle).  mode 0: uniform (u-0.5)*2*strength (reference
+ * form); mode 1: Gaussian with sigma = strength. */
+int fos_brownian(float *pos, int n, float strength, float L, uint64_t seed,
+                 int frame, int mode, void *stream);
+/* dynamics.py:607-660 apply_brownian (Ornstein-Uhlenbeck velocity update with
+ * Philox uniforms keyed (seed, frame, particle)) fused with integrate_jitter;
+ * pos == NULL updates v_jit only.  mean_r is the host-computed mean radius
+ * (run.py:860-861); drift_cap_frac = (float)(MAX_DRIFT_FRACTION*GAP_FRACTION). */
+int fos_ou_jitter(float *pos, float *v_jit, const float *rad, int n,
+                  float mean_r, float dt, float jitter_rms, float jitter_tau,
+                  float drift_cap_frac, float L, uint64_t seed, int frame,
+                  void *stream);
+/* dynamics.py:649-660 integrate_jitter: pos = wrapP(pos + v_jit*dt). */
+int fos_integrate_jitter(float *pos, const float *v_jit, int n, float dt,
+                         float L, void *stream);
+/* dynamics.py:532-593 apply_xsph_smoothing. */
+int fos_xsph(const float *pos, float *vel, float *vel_tmp, int n, int G,
+             float L, float lim_sq, float eps, const int32_t *cell_start,
+             const int32_t *cell_count, const int32_t *cell_indices,
+             void *stream);
+/* dynamics.py:473-480 apply_global_damping. */
+int fos_damp(float *vel, int n, float factor, void *stream);
+
+/* ---- topology.py ------------------------------------------------------- */
+
+/* topology.py:68-133 build_topo_candidates: pairs[n][cap], count[n],
+ * truncated[n], in the reference's cell/occupant order. */
+int fos_topo_candidates(const float *pos, const float *rad, int n, int G,
+                        float L, float mult, int cap,
+                        const int32_t *cell_start, const int32_t *cell_count,
+                        const int32_t *cell_indices, int32_t *pairs,
+                        int32_t *count, int32_t *truncated, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,157 @@
+"""Whole-frame parity (run.py:700-1261 order) and the drop-in modules.
+
+The GPU driver and the CPU restatement advance the same state; after each
+frame every state array is compared: integer arrays bit-exact, positions and
+radii within 1e-5 relative (and, in practice, bit-exact).
+"""
+import numpy as np
+import pytest
+import torch
+
+import foam_config as fc
+import foam_inputs as fi
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+INT_STATE = ("face_ids", "face_counts", "fsc", "neighbor_list", "neighbor_count", "used_count")
+FLOAT_STATE = ("pos", "rad", "rad_target", "lmd", "fsc_ema", "prev_scores", "v_jit")
+
+
+def compare(a, b, where):
+    for k in INT_STATE:
+        assert np.array_equal(a[k], b[k]), (where, k)
+    for k in FLOAT_STATE:
+        assert np.allclose(a[k], b[k], rtol=1e-5, atol=1e-9), (where, k)
+
+
+def small_cfg(**kw):
+    cfg = fc.preset("c1")
+    cfg.n = 600
+    cfg.res = 40
+    cfg.frames = 4
+    cfg.warmstart_frames = 1
+    cfg.pbd_base, cfg.pbd_max = 4, 8
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+@pytest.mark.parametrize("variant", ["plain", "noise", "cadence"])
+def test_frames_match_oracle(gpu, variant):
+    from foam import FoamSim
+    from oracle.frame import OracleFoam
+    kw = {}
+    if variant == "noise":
+        kw = dict(brownian=True, brownian_mode=1, brownian_strength=1e-3, ou_jitter=True)
+    if variant == "cadence":
+        kw = dict(jfa_every=2, jfa_warm=1, pressure_iters=3)
+    cfg = small_cfg(**kw)
+    pos, rad = fi.uniform(cfg.n, 0.03, 0.06, seed=21)
+    cfg.r_min, cfg.r_max = 0.02, 0.07
+    sim = FoamSim(cfg, pos, rad)
+    ref = OracleFoam(cfg, pos, rad)
+    for f in range(cfg.frames):
+        print(f"[frame parity] variant={variant} frame={f}", flush=True)
+        a = sim.step()
+        b = ref.step()
+        assert a["pbd_passes"] == b["pbd_passes"] and a["jfa"] == b["jfa"]
+        assert a["growth"] == b["growth"]
+        compare(sim.snapshot(), ref.snapshot(), f)
+
+
+def test_c1_first_frames_bit_exact(gpu):
+    from foam import FoamSim
+    from oracle.frame import OracleFoam
+    cfg = fc.preset("c1")
+    pos, rad = fi.for_config(cfg)
+    sim = FoamSim(cfg, pos, rad)
+    ref = OracleFoam(cfg, pos, rad)
+    for f in range(2):
+        sim.step()
+        ref.step()
+        a, b = sim.snapshot(), ref.snapshot()
+        compare(a, b, f)
+        assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["fsc"], b["fsc"])
+
+
+def test_snapshot_roundtrip(gpu, tmp_path):
+    from foam import FoamSim
+    cfg = small_cfg()
+    pos, rad = fi.uniform(cfg.n, 0.03, 0.06, seed=5)
+    a = FoamSim(cfg, pos, rad)
+    a.step()
+    path = str(tmp_path / "s.npz")
+    a.save(path)
+    b = FoamSim(cfg, pos, rad)
+    b.load(path)
+    a.step()
+    b.step()
+    compare(a.snapshot(), b.snapshot(), "restore")
+    a.export_csv(str(tmp_path / "p.csv"))
+    head = open(tmp_path / "p.csv").readline().strip()
+    assert head == "ID,X,Y,Z,Radius,FSC"
+
+
+def test_dropin_run_jfa_matches_oracle(gpu, orc):
+    import jfa
+    fc.JFA_RES_MIN, fc.JFA_RES_MAX = 16, 320
+    fc.DOMAIN_SIZE = 1.0
+    fc.JFA_EARLY_EXIT_ENABLED = False
+    n, res = 2000, 64
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    jfa.init_jfa(max_particles=n, max_res=res)
+    jfa.set_resolution(res)
+    assert jfa.JFA_NUM_PASSES == 7
+    tp = torch.from_numpy(pos).cuda()
+    tr = torch.from_numpy(rad).cuda()
+    fsc, stats = jfa.run_jfa(tp, tr, n)
+    voxel = fc.f32(1 / 64)
+    lab, passes, _, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, 7)
+    ids, cts = orc.accumulate_faces(orc.majority(lab, res), res, n)
+    prev, nl, nc, used, fs = orc.finalize_faces(n, rad, voxel, ids, cts, np.zeros((n, 32), F))
+    assert np.array_equal(jfa.jfa_grid.to_numpy().ravel(), lab)
+    assert np.array_equal(fsc, fs) and np.array_equal(jfa.face_ids.to_numpy(), ids)
+    assert np.array_equal(jfa.neighbor_list.to_numpy(), nl)
+    assert stats["num_passes"] == 7 and not stats["early_exit"]
+    assert stats["delta_history"] == [h / res ** 3 for h in hist]
+    assert stats["min_fsc"] == fs.min() and stats["max_fsc"] == fs.max()
+    v = jfa.validate_jfa(n)
+    assert v["self_loops"] == 0 and v["asymmetric_pairs"] == int(orc.validate(n, nl, nc)[0])
+    # the per-kernel entry points reproduce the same grid
+    jfa.jfa_init(tp, tr, n)
+    step = res // 2
+    for p in range(7):
+        (jfa.jfa_step_ping0 if p % 2 == 0 else jfa.jfa_step_ping1)(tp, tr, max(step, 1))
+        step //= 2
+    jfa.jfa_copy_temp_to_grid()
+    assert np.array_equal(jfa.jfa_grid.to_numpy().ravel(), lab)
+    assert jfa.label_changes[None] == hist[-1]
+
+
+def test_dropin_grid_and_dynamics(gpu, orc):
+    import dynamics
+    import grid
+    n, G_ = 2000, 13
+    fc.DOMAIN_SIZE, fc.GRID_RES = 1.0, G_
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
+    tp = torch.from_numpy(pos).cuda()
+    tr = torch.from_numpy(rad).cuda()
+    tcc = torch.zeros(G_ ** 3, dtype=torch.int32, device="cuda")
+    tcs = torch.zeros_like(tcc)
+    tcw = torch.zeros_like(tcc)
+    tci = torch.zeros(n, dtype=torch.int32, device="cuda")
+    grid.clear_grid(tcc)
+    grid.count_particles_per_cell(tp, tcc, n)
+    grid.prefix_sum(tcc, tcs)
+    grid.copy_cell_pointers(tcs, tcw)
+    grid.scatter_particles(tp, tcw, tci, n)
+    assert np.array_equal(tci.cpu().numpy(), ci) and np.array_equal(tp.cpu().numpy(), p)
+    lmd = torch.zeros(n, device="cuda")
+    g, want_l = orc.max_overlap(p, rad, n, G_, 1.0, cs, cc, ci)
+    assert dynamics.compute_max_overlap(tp, tr, tcs, tcc, tci, lmd, n) == g
+    dynamics.project_overlaps(tp, tr, tcs, tcc, tci, n)
+    want = orc.pbd_jacobi(p, rad, n, G_, 1.0, fc.f32(1.015), 0.2, 2.0, cs, cc, ci)
+    assert np.array_equal(tp.cpu().numpy(), want)
+    assert dynamics.hash_func(7, 3) == orc.lcg_hash(7, 3)

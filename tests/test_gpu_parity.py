@@ -1,0 +1,451 @@
+"""GPU parity: every libfos kernel against the CPU restatement on the same
+seeded inputs, through the C ABI.
+
+Bar: integer outputs (labels, filtered labels, face rows and counts,
+neighbour lists, FSC, hash CSR) bit-exact; fp32 outputs bit-exact where the
+two sides evaluate the same correctly rounded operations in the same order
+(positions, radii, scores), otherwise within 1e-5 relative as stated per test.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import foam_config as fc
+import foam_inputs as fi
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+class G:
+    """Thin numpy-in / numpy-out wrapper over the C ABI."""
+
+    def __init__(self, fl, n, res, cells=27):
+        self.fl = fl
+        self.ctx = fl.Context(max(1, n), max(1, res), max(27, cells))
+        self.h = self.ctx.handle
+
+    def call(self, name, *a):
+        self.fl.call(name, *a)
+
+    def ptr(self, t):
+        return self.fl.ptr(t)
+
+
+def run_jfa_gpu(fl, pos, rad, n, res, passes, early=False, th=0.006, minp=3):
+    g = G(fl, n, res)
+    tp, tr = dev(pos), dev(rad)
+    V = res ** 3
+    grid = torch.empty(V, dtype=torch.int32, device="cuda")
+    temp = torch.empty(V, dtype=torch.int32, device="cuda")
+    info = torch.zeros(66, dtype=torch.int32, device="cuda")
+    voxel = fc.f32(1.0 / res)
+    st = fl.stream()
+    g.call("fos_jfa_init", g.h, g.ptr(tp), g.ptr(tr), n, res, 1.0, voxel, 1.0, g.ptr(grid),
+           g.ptr(temp), st)
+    g.call("fos_jfa_passes", g.h, res, 1.0, voxel, passes, int(early), th, minp, g.ptr(grid),
+           g.ptr(temp), g.ptr(info), st)
+    return host(grid), host(info)
+
+
+# ---- JFA ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", ["c1_64", "c1_48", "dense_seeds", "single"])
+def test_jfa_init_and_passes_bit_exact(gpu, orc, case):
+    if case == "c1_64":
+        n, res = 2000, 64
+        pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    elif case == "c1_48":                         # non power of two
+        n, res = 2000, 48
+        pos, rad = fi.uniform(n, 0.02, 0.04, seed=1)
+    elif case == "dense_seeds":                   # many seed collisions and ties
+        n, res = 3000, 16
+        pos, rad = fi.uniform(n, 0.01, 0.02, seed=2)
+        pos[1::2] = pos[0::2]
+        rad[1::2] = rad[0::2]
+    else:
+        n, res = 1, 20
+        pos, rad = fi.uniform(1, 0.02, 0.04, seed=3)
+    voxel = fc.f32(1.0 / res)
+    P = int(math.ceil(math.log2(res))) + 1
+    print(f"[jfa parity] case={case} n={n} res={res} passes={P}", flush=True)
+    g0, _ = orc.jfa_init(pos, rad, n, res, 1.0, voxel)
+    # seeding alone
+    gg = G(gpu, n, res)
+    grid = torch.empty(res ** 3, dtype=torch.int32, device="cuda")
+    temp = torch.empty(res ** 3, dtype=torch.int32, device="cuda")
+    gg.call("fos_jfa_init", gg.h, gg.ptr(dev(pos)), gg.ptr(dev(rad)), n, res, 1.0, voxel, 1.0,
+            gg.ptr(grid), gg.ptr(temp), gpu.stream())
+    assert (host(grid) == g0).all()
+    assert (host(temp) == -1).all()
+    lab, passes, early, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, P)
+    glab, info = run_jfa_gpu(gpu, pos, rad, n, res, P)
+    assert (glab == lab).all()
+    assert info[0] == passes and info[1] == 0 and (info[2:2 + passes] == hist).all()
+
+
+def test_jfa_single_steps_match(gpu, orc):
+    n, res = 700, 40
+    pos, rad = fi.uniform(n, 0.02, 0.05, seed=4)
+    voxel = fc.f32(1.0 / res)
+    g = G(gpu, n, res)
+    cur, _ = orc.jfa_init(pos, rad, n, res, 1.0, voxel)
+    tp, tr = dev(pos), dev(rad)
+    g.call("fos_jfa_seed_table", g.h, g.ptr(tp), g.ptr(tr), n, 1.0, gpu.stream())
+    for step in (20, 10, 5, 2, 1, 1, 3):
+        out, ch = orc.jfa_step(pos, rad, res, 1.0, voxel, step, cur)
+        tin = dev(cur)
+        tout = torch.empty_like(tin)
+        chg = torch.zeros(1, dtype=torch.int32, device="cuda")
+        g.call("fos_jfa_step", g.h, res, 1.0, voxel, step, g.ptr(tin), g.ptr(tout), g.ptr(chg),
+               gpu.stream())
+        assert (host(tout) == out).all(), step
+        assert int(host(chg)[0]) == ch
+        cur = out
+
+
+def test_jfa_early_exit_on_device(gpu, orc):
+    n, res = 2000, 64
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    voxel = fc.f32(1.0 / res)
+    _, _, _, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, 7)
+    th = float(hist[4]) / res ** 3 + 1e-9          # fires at pass 4 (or earlier)
+    lab, passes, early, h2 = orc.jfa_run(pos, rad, n, res, 1.0, voxel, 7, True, th, 3)
+    glab, info = run_jfa_gpu(gpu, pos, rad, n, res, 7, True, th, 3)
+    assert early and info[1] == 1 and info[0] == passes
+    assert (glab == lab).all()
+
+
+def test_jfa_general_domain_size(gpu, orc):
+    """L != 1 takes the division form of the minimum image."""
+    n, res, L = 300, 32, 0.189
+    pos, rad = fi.uniform(n, 0.004, 0.008, seed=5, L=L)
+    voxel = fc.f32(L / res)
+    P = 6
+    lab, passes, _, hist = orc.jfa_run(pos, rad, n, res, L, voxel, P)
+    g = G(gpu, n, res)
+    grid = torch.empty(res ** 3, dtype=torch.int32, device="cuda")
+    temp = torch.empty(res ** 3, dtype=torch.int32, device="cuda")
+    info = torch.zeros(66, dtype=torch.int32, device="cuda")
+    g.call("fos_jfa_init", g.h, g.ptr(dev(pos)), g.ptr(dev(rad)), n, res, L, voxel, 1.0,
+           g.ptr(grid), None, gpu.stream())
+    g.call("fos_jfa_passes", g.h, res, L, voxel, P, 0, 0.006, 3, g.ptr(grid), g.ptr(temp),
+           g.ptr(info), gpu.stream())
+    assert (host(grid) == lab).all()
+
+
+# ---- majority + faces -----------------------------------------------------------
+
+def gpu_faces(gpu, lab, res, n):
+    g = G(gpu, n, res)
+    tl = dev(lab)
+    filt = torch.empty_like(tl)
+    ids = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+    cts = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+    g.call("fos_majority", res, g.ptr(tl), g.ptr(filt), gpu.stream())
+    g.call("fos_face_accumulate", g.h, res, n, g.ptr(filt), g.ptr(ids), g.ptr(cts), gpu.stream())
+    return host(filt), host(ids), host(cts)
+
+
+def test_majority_and_faces_c1(gpu, orc):
+    n, res = 2000, 64
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, fc.f32(1 / 64), 7)
+    f = orc.majority(lab, res)
+    ids, cts = orc.accumulate_faces(f, res, n)
+    gf, gids, gcts = gpu_faces(gpu, lab, res, n)
+    assert (gf == f).all()
+    assert (gids == ids).all() and (gcts == cts).all()
+
+
+def test_faces_row_truncation_and_table_growth(gpu, orc):
+    """Row 0 meets 100 distinct higher partners (> 32 kept, > 64 table
+    slots): the first 32 by scan order survive with full counts."""
+    res, n = 112, 101
+    lab = np.zeros((res, res, res), np.int32)
+    for k in range(100):
+        lab[4 + k, :, 7] = 1 + k
+    lab = lab.ravel()
+    ids, cts = orc.accumulate_faces(lab, res, n)
+    g = G(gpu, n, res)
+    tl = dev(lab)
+    gids = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+    gcts = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+    g.call("fos_face_accumulate", g.h, res, n, g.ptr(tl), g.ptr(gids), g.ptr(gcts), gpu.stream())
+    assert (host(gids) == ids).all() and (host(gcts) == cts).all()
+    # the grown tables keep working on the next call
+    g.call("fos_face_accumulate", g.h, res, n, g.ptr(tl), g.ptr(gids), g.ptr(gcts), gpu.stream())
+    assert (host(gids) == ids).all() and (host(gcts) == cts).all()
+
+
+def test_faces_exclude_ids_at_or_above_n(gpu, orc):
+    n, res = 500, 32
+    pos, rad = fi.uniform(n, 0.02, 0.05, seed=7)
+    lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, fc.f32(1 / 32), 6)
+    m = 400                                   # active_n below the labelled ids
+    f = orc.majority(lab, res)
+    ids, cts = orc.accumulate_faces(f, res, m)
+    gf, gids, gcts = gpu_faces(gpu, lab, res, m)
+    assert (gids == ids).all() and (gcts == cts).all()
+
+
+# ---- finalize -----------------------------------------------------------------------
+
+def gpu_finalize(gpu, n, rad, voxel, ids, cts, prev):
+    g = G(gpu, n, 1)
+    tp = dev(prev.astype(F))
+    nl = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+    nc = torch.empty(n, dtype=torch.int32, device="cuda")
+    used = torch.empty(n, dtype=torch.int32, device="cuda")
+    fsc = torch.empty(n, dtype=torch.int32, device="cuda")
+    g.call("fos_finalize_faces", g.h, n, g.ptr(dev(rad)), voxel, g.ptr(dev(ids)),
+           g.ptr(dev(cts)), g.ptr(tp), g.ptr(nl), g.ptr(nc), g.ptr(used), g.ptr(fsc),
+           gpu.stream())
+    return host(tp), host(nl), host(nc), host(used), host(fsc)
+
+
+def test_finalize_multi_frame_hysteresis(gpu, orc):
+    """Several frames of C1-style faces with the hysteresis state carried
+    over; includes many scores in the [0.10, 0.15) band."""
+    n, res = 2000, 64
+    voxel = fc.f32(1 / 64)
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    rng = np.random.default_rng(3)
+    prev_o = np.zeros((n, 32), F)
+    prev_g = prev_o.copy()
+    for frame in range(4):
+        p = fi.wrap(pos + rng.normal(0, 0.004, pos.shape).astype(F))
+        lab, _, _, _ = orc.jfa_run(p, rad, n, res, 1.0, voxel, 7)
+        f = orc.majority(lab, res)
+        ids, cts = orc.accumulate_faces(f, res, n)
+        # push some counts into the hysteresis band
+        band = rng.random(cts.shape) < 0.15
+        cts = np.where(band & (cts > 0), np.maximum(1, (cts * 0.2).astype(np.int32)), cts)
+        o = orc.finalize_faces(n, rad, voxel, ids, cts, prev_o)
+        gout = gpu_finalize(gpu, n, rad, voxel, ids, cts, prev_g)
+        for a, b, name in zip(o, gout, ("prev", "nl", "nc", "used", "fsc")):
+            assert np.array_equal(a, b), (frame, name)
+        prev_o, prev_g = o[0], gout[0]
+
+
+def test_finalize_chains_and_overflow(gpu, orc):
+    """Adversarial rows: ambiguous scores whose decisions depend on appends
+    from lower rows (long dependency chains) and rows that overflow 32."""
+    n = 300
+    voxel = fc.f32(1 / 64)
+    rad = np.full(n, 0.02, F)
+    rng = np.random.default_rng(9)
+    ids = np.full((n, 32), -1, np.int32)
+    cts = np.zeros((n, 32), np.int32)
+    for i in range(n - 1):
+        k = rng.integers(1, 32)
+        partners = np.sort(rng.choice(np.arange(i + 1, n), size=min(k, n - 1 - i), replace=False))
+        rng.shuffle(partners)
+        ids[i, :len(partners)] = partners
+        cts[i, :len(partners)] = rng.choice([1, 1, 1, 2, 3], size=len(partners))
+    ids[0:40, 0] = 250                        # row 250 overflows from lower rows
+    cts[0:40, 0] = 3
+    prev = (rng.random((n, 32)) < 0.3).astype(F) * F(0.12)
+    o = orc.finalize_faces(n, rad, voxel, ids, cts, prev)
+    gout = gpu_finalize(gpu, n, rad, voxel, ids, cts, prev)
+    for a, b, name in zip(o, gout, ("prev", "nl", "nc", "used", "fsc")):
+        assert np.array_equal(a, b), name
+
+
+def test_ema_stats_validate(gpu, orc):
+    n = 2000
+    rng = np.random.default_rng(1)
+    fsc = rng.integers(0, 20, n).astype(np.int32)
+    ema = np.where(rng.random(n) < 0.3, 0.0, rng.random(n) * 15).astype(F)
+    e = orc.fsc_ema(n, fc.f32(0.33), fsc, ema)
+    te = dev(ema)
+    gpu.call("fos_fsc_ema", n, 0.33, gpu.ptr(dev(fsc)), gpu.ptr(te), gpu.stream())
+    assert np.array_equal(host(te), e)
+    # stats / validate on a real C1 JFA frame
+    res = 64
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    voxel = fc.f32(1 / 64)
+    lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, voxel, 7)
+    ids, cts = orc.accumulate_faces(orc.majority(lab, res), res, n)
+    prev, nl, nc, used, fs = orc.finalize_faces(n, rad, voxel, ids, cts, np.zeros((n, 32), F))
+    oi, of = orc.jfa_stats(n, rad, voxel, fs, e, used, ids, cts)
+    g = G(gpu, n, res)
+    goi = (gpu.I64 * 5)()
+    gof = (gpu.D * 4)()
+    g.call("fos_jfa_stats", g.h, n, g.ptr(dev(rad)), voxel, g.ptr(dev(fs)), g.ptr(dev(e)),
+           g.ptr(dev(used)), g.ptr(dev(ids)), g.ptr(dev(cts)), goi, gof, gpu.stream())
+    assert list(goi) == list(oi)
+    assert np.allclose(list(gof), of, rtol=1e-9)
+    vo = (gpu.I64 * 3)()
+    g.call("fos_validate", g.h, n, g.ptr(dev(nl)), g.ptr(dev(nc)), vo, gpu.stream())
+    assert list(vo) == list(orc.validate(n, nl, nc))
+
+
+def test_dirty_tiles(gpu, orc):
+    n, res = 2000, 64
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    rng = np.random.default_rng(2)
+    prev = fi.wrap(pos + rng.normal(0, 0.01, pos.shape).astype(F))
+    rprev = rad * F(1.01)
+    voxel = fc.f32(1 / 64)
+    d = orc.mark_dirty_tiles(pos, rad, prev, rprev, n, res, 1.0, voxel)
+    td = torch.zeros(64, dtype=torch.uint8, device="cuda")
+    gpu.call("fos_dirty_mark", gpu.ptr(dev(pos)), gpu.ptr(dev(rad)), gpu.ptr(dev(prev)),
+             gpu.ptr(dev(rprev)), n, res, 1.0, voxel, 16, 1, 0.5, 0.25, gpu.ptr(td), gpu.stream())
+    assert np.array_equal(host(td), d)
+
+
+# ---- grid + dynamics ---------------------------------------------------------------
+
+def gpu_grid(gpu, pos, n, G_):
+    g = G(gpu, n, 1, G_ ** 3)
+    tp = dev(pos)
+    cc = torch.empty(G_ ** 3, dtype=torch.int32, device="cuda")
+    cs = torch.empty_like(cc)
+    ci = torch.empty(n, dtype=torch.int32, device="cuda")
+    g.call("fos_grid_build", g.h, g.ptr(tp), n, G_, 1.0, g.ptr(cc), g.ptr(cs), g.ptr(ci),
+           gpu.stream())
+    return tp, cc, cs, ci
+
+
+@pytest.mark.parametrize("n,G_,seed", [(2000, 13, 0), (10000, 21, 1), (5000, 3, 2)])
+def test_grid_build_bit_exact(gpu, orc, n, G_, seed):
+    pos, _ = fi.uniform(n, 0.01, 0.02, seed=seed)
+    pos[::5] += F(0.75)                       # unwrapped inputs
+    p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
+    tp, tcc, tcs, tci = gpu_grid(gpu, pos, n, G_)
+    assert np.array_equal(host(tp), p)
+    assert np.array_equal(host(tcc), cc) and np.array_equal(host(tcs), cs)
+    assert np.array_equal(host(tci), ci)
+
+
+def test_grid_clustered_cells(gpu, orc):
+    n, G_ = 20000, 34
+    pos, _ = fi.clustered(n, seed=4)
+    p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
+    tp, tcc, tcs, tci = gpu_grid(gpu, pos, n, G_)
+    assert np.array_equal(host(tci), ci) and np.array_equal(host(tcs), cs)
+
+
+def test_pbd_maxoverlap_bit_exact(gpu, orc):
+    n, G_ = 2000, 13
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
+    gap = fc.f32(1.015)
+    want = orc.pbd_jacobi(p, rad, n, G_, 1.0, gap, 0.2, 2.0, cs, cc, ci)
+    tp, tcc, tcs, tci = gpu_grid(gpu, pos, n, G_)
+    out = torch.empty_like(tp)
+    tr = dev(rad)
+    gpu.call("fos_pbd_pass", gpu.ptr(tp), gpu.ptr(out), gpu.ptr(tr), n, G_, 1.0, gap, 0.2, 2.0,
+             gpu.ptr(tcs), gpu.ptr(tcc), gpu.ptr(tci), gpu.stream())
+    got = host(out)
+    assert np.allclose(got, want, rtol=1e-5, atol=1e-7)
+    assert np.array_equal(got, want)
+    g, lmd = orc.max_overlap(p, rad, n, G_, 1.0, cs, cc, ci)
+    tl = torch.empty(n, dtype=torch.float32, device="cuda")
+    tm = torch.empty(1, dtype=torch.float32, device="cuda")
+    gpu.call("fos_max_overlap", gpu.ptr(tp), gpu.ptr(tr), n, G_, 1.0, gpu.ptr(tcs), gpu.ptr(tcc),
+             gpu.ptr(tci), gpu.ptr(tl), gpu.ptr(tm), gpu.stream())
+    assert np.array_equal(host(tl), lmd) and float(host(tm)[0]) == g
+
+
+def test_controller_kernels(gpu, orc):
+    n = 5000
+    rng = np.random.default_rng(6)
+    rad = rng.uniform(0.01, 0.04, n).astype(F)
+    fsc = rng.integers(0, 25, n).astype(np.int32)
+    ema = np.where(rng.random(n) < 0.2, 0.0, rng.random(n) * 25).astype(F)
+    rt = orc.fsc_targets(rad, fsc, ema, n, 12, 16, fc.f32(0.1), fc.f32(0.006), fc.f32(0.024))
+    trt = torch.empty(n, dtype=torch.float32, device="cuda")
+    gpu.call("fos_fsc_targets", gpu.ptr(dev(rad)), gpu.ptr(trt), gpu.ptr(dev(fsc)),
+             gpu.ptr(dev(ema)), n, 12, 16, 0.1, 0.006, 0.024, 1.0, gpu.stream())
+    assert np.array_equal(host(trt), rt)
+    lmd = (rng.random(n) * 0.05).astype(F)
+    a = fc.f32(1 - 0.05 ** (1 / 15))
+    for mode in (0, 1, 2):
+        want = orc.nudge_radii(rad, rt, lmd, n, a, fc.f32(0.12), fc.f32(0.006), fc.f32(0.024),
+                               fc.f32(0.6), mode, fc.f32(0.035))
+        tr = dev(rad)
+        gpu.call("fos_nudge_radii", gpu.ptr(tr), gpu.ptr(trt), gpu.ptr(dev(lmd)), n, a, 0.12,
+                 0.006, 0.024, 0.6, mode, 0.035, gpu.stream())
+        assert np.array_equal(host(tr), want), mode
+    oi, of = orc.controller_stats(rad, rt, ema, n, 12, 16)
+    g = G(gpu, n, 1)
+    goi = (gpu.I64 * 3)()
+    gof = (gpu.D * 2)()
+    g.call("fos_controller_stats", g.h, gpu.ptr(dev(rad)), gpu.ptr(trt), gpu.ptr(dev(ema)), n, 12,
+           16, goi, gof, gpu.stream())
+    assert list(goi) == list(oi) and gof[1] == of[1]
+    assert gof[0] == pytest.approx(of[0], rel=1e-9, abs=1e-15)
+
+
+def test_pressure_iterations(gpu, orc):
+    n, res = 2000, 64
+    voxel = fc.f32(1 / 64)
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, voxel, 7)
+    ids, cts = orc.accumulate_faces(orc.majority(lab, res), res, n)
+    _, _, _, _, fs = orc.finalize_faces(n, rad, voxel, ids, cts, np.zeros((n, 32), F))
+    r = rad.copy()
+    tr = dev(rad)
+    tdr = torch.empty(n, dtype=torch.float32, device="cuda")
+    st = torch.zeros(2, dtype=torch.float32, device="cuda")
+    tids, tfs = dev(ids), dev(fs)
+    for frame in range(10):
+        r, dr, mx, ch = orc.pressure(n, frame, r, ids, fs, fc.f32(0.1), fc.f32(3.0),
+                                     fc.f32(0.02), 10, fc.f32(0.01), fc.f32(0.04))
+        gpu.call("fos_pressure_iter", n, frame, gpu.ptr(tr), gpu.ptr(tdr), gpu.ptr(tids),
+                 gpu.ptr(tfs), 0.1, 3.0, 0.02, 10, 0.01, 0.04, gpu.ptr(st), gpu.stream())
+        got = host(tr)
+        assert np.allclose(got, r, rtol=1e-5, atol=0)
+        s = host(st)
+        assert s.view(np.int32)[1] == ch
+        assert s[0] == pytest.approx(mx, rel=1e-5)
+    # the double-evaluated power makes the radii bit-identical in practice
+    assert np.array_equal(host(tr), r)
+
+
+def test_noise_kernels(gpu, orc):
+    n = 10000
+    pos, rad = fi.uniform(n, 0.012, 0.024, seed=0)
+    for mode, s in ((0, 2e-4), (1, 1e-4)):
+        want = orc.brownian(pos, n, fc.f32(s), 1.0, 12345, 17, mode)
+        tp = dev(pos)
+        gpu.call("fos_brownian", gpu.ptr(tp), n, s, 1.0, 12345, 17, mode, gpu.stream())
+        got = host(tp)
+        assert np.allclose(got, want, rtol=1e-5, atol=1e-9)
+        assert (got == want).mean() > 0.999
+    v = np.random.default_rng(1).normal(0, 1e-3, (n, 3)).astype(F)
+    wp, wv = orc.ou_jitter(pos, v, rad, n, fc.f32(0.018), fc.f32(0.016), fc.f32(0.1),
+                           fc.f32(1.0), fc.f32(0.003), 1.0, 99, 4)
+    tp, tv = dev(pos), dev(v)
+    gpu.call("fos_ou_jitter", gpu.ptr(tp), gpu.ptr(tv), gpu.ptr(dev(rad)), n, 0.018, 0.016, 0.1,
+             1.0, 0.003, 1.0, 99, 4, gpu.stream())
+    assert np.array_equal(host(tv), wv) and np.array_equal(host(tp), wp)
+
+
+def test_topology_candidates(gpu, orc):
+    n, G_ = 3000, 10
+    pos, rad = fi.uniform(n, 0.01, 0.03, seed=8)
+    p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
+    pairs, cnt, tr = orc.topo_candidates(p, rad, n, G_, 1.0, fc.f32(2.2), 28, cs, cc, ci)
+    tp, tcc, tcs, tci = gpu_grid(gpu, pos, n, G_)
+    gp = torch.full((n, 28), -1, dtype=torch.int32, device="cuda")
+    gc = torch.empty(n, dtype=torch.int32, device="cuda")
+    gt = torch.empty(n, dtype=torch.int32, device="cuda")
+    gpu.call("fos_topo_candidates", gpu.ptr(tp), gpu.ptr(dev(rad)), n, G_, 1.0, 2.2, 28,
+             gpu.ptr(tcs), gpu.ptr(tcc), gpu.ptr(tci), gpu.ptr(gp), gpu.ptr(gc), gpu.ptr(gt),
+             gpu.stream())
+    assert np.array_equal(host(gc), cnt) and np.array_equal(host(gt), tr)
+    assert np.array_equal(host(gp), pairs)
