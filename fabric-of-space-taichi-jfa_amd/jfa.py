@@ -112,6 +112,9 @@ def init_jfa(max_particles=None, max_res=None, device="cuda"):
 def _bind_views():
     global jfa_grid, jfa_temp, label_filtered
     r = JFA_RES
+    if r > _max_res:                 # bound by set_resolution once it fits
+        jfa_grid = jfa_temp = label_filtered = None
+        return
     V = r ** 3
     jfa_grid = Field(_flat["grid"][:V].view(r, r, r))
     jfa_temp = Field(_flat["temp"][:V].view(r, r, r))

@@ -18,9 +18,23 @@ import foam_inputs as fi
 pytestmark = pytest.mark.gpu
 F = np.float32
 
+# Device copies made inline in a call's argument list must outlive the call:
+# a dropped temporary returns its block to torch's caching allocator, which
+# hands it to the next copy, so two arguments would alias.
+_alive = []
+
+
+@pytest.fixture(autouse=True)
+def _keep_device_copies():
+    yield
+    torch.cuda.synchronize()
+    _alive.clear()
+
 
 def dev(a):
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    _alive.append(t)
+    return t
 
 
 def host(t):
@@ -254,8 +268,16 @@ def test_finalize_chains_and_overflow(gpu, orc):
         rng.shuffle(partners)
         ids[i, :len(partners)] = partners
         cts[i, :len(partners)] = rng.choice([1, 1, 1, 2, 3], size=len(partners))
-    ids[0:40, 0] = 250                        # row 250 overflows from lower rows
-    cts[0:40, 0] = 3
+    for i in range(40):                       # row 250 overflows from lower rows
+        row = [250] + [p for p in ids[i] if p not in (-1, 250)]
+        ids[i, :] = -1
+        ids[i, :min(32, len(row))] = row[:32]
+        cts[i, 0] = 3
+        cts[i, len(row):] = 0
+    # rows list distinct partners, as accumulate_face_area produces them
+    for i in range(n):
+        v = ids[i][ids[i] >= 0]
+        assert len(set(v.tolist())) == len(v)
     prev = (rng.random((n, 32)) < 0.3).astype(F) * F(0.12)
     o = orc.finalize_faces(n, rad, voxel, ids, cts, prev)
     gout = gpu_finalize(gpu, n, rad, voxel, ids, cts, prev)
