@@ -125,6 +125,21 @@ __device__ __forceinline__ float u01_open(uint32_t b) {
   return (float)((b >> 8) + 1u) * 5.9604644775390625e-8f;
 }
 
+// Per-pass change counters are striped over FOS_STRIPES cache lines (block id
+// mod FOS_STRIPES): one hot address would serialise every block's atomic.
+#define FOS_STRIPES 64
+#define FOS_STRIPE_INTS 16
+#define FOS_PASS_SLOTS 64
+#define FOS_STRIPE_PASS_INTS (FOS_STRIPES * FOS_STRIPE_INTS)
+
+// Wave-wide total of pass q's striped counter (every lane gets it).
+__device__ __forceinline__ int stripes_total(const int *stripes, int q) {
+  const int lane = (int)(threadIdx.x & 63);
+  int v = stripes[(q * FOS_STRIPES + lane) * FOS_STRIPE_INTS];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // Block-wide sum of an int into one atomic per block.
 __device__ __forceinline__ void block_add(int v, int *dst) {
   __shared__ int s_acc;
@@ -149,6 +164,8 @@ struct fos_ctx {
   int max_cells;
   // seed table (x, y, z, (beta*r)^2)
   float4 *seeds;
+  // striped per-pass changed-voxel counters [FOS_PASS_SLOTS][FOS_STRIPES][16]
+  int *stripes;
   // face-pair row tables: capacity tab_cap entries per particle row
   int tab_cap;
   int *tab_b;
@@ -171,6 +188,7 @@ struct fos_ctx {
   int *cell_write;        // [max_cells]
   int *cell_of;           // [max_n]
   float *pos_tmp;         // [max_n*3]
+  float *rad_tmp;         // [max_n] pressure ping-pong
   float *fscratch;        // [16] small device floats (reductions)
   int *iscratch;          // [16] small device ints
   // host pinned mirror for small reads

@@ -168,11 +168,11 @@ int fos_fill_f32(float *dst, int64_t count, float value, void *stream) {
 
 int fos_ctx_destroy(fos_ctx *ctx) {
   if (!ctx) return 0;
-  void *ptrs[] = {ctx->seeds,    ctx->tab_b,    ctx->tab_k,     ctx->tab_c,
+  void *ptrs[] = {ctx->seeds,    ctx->stripes,  ctx->tab_b,    ctx->tab_k,     ctx->tab_c,
                   ctx->flags,    ctx->score,    ctx->cls,       ctx->dec,
                   ctx->in_count, ctx->in_start, ctx->in_fill,   ctx->in_src,
                   ctx->acc_in,   ctx->scan_blocks, ctx->cell_write, ctx->cell_of,
-                  ctx->pos_tmp,  ctx->fscratch, ctx->iscratch};
+                  ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->host_i) (void)hipHostFree(ctx->host_i);
@@ -235,6 +235,7 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
     }                                                      \
   } while (0)
   ALLOC(c->seeds, (size_t)max_n * sizeof(float4));
+  ALLOC(c->stripes, (size_t)FOS_PASS_SLOTS * FOS_STRIPE_PASS_INTS * sizeof(int));
   ALLOC(c->flags, 64 * sizeof(int));
   ALLOC(c->score, nr * sizeof(float));
   ALLOC(c->cls, nr);
@@ -250,6 +251,7 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
   ALLOC(c->cell_write, (size_t)max_cells * sizeof(int));
   ALLOC(c->cell_of, (size_t)max_n * sizeof(int));
   ALLOC(c->pos_tmp, (size_t)max_n * 3 * sizeof(float));
+  ALLOC(c->rad_tmp, (size_t)max_n * sizeof(float));
   ALLOC(c->fscratch, 64 * sizeof(float));
   ALLOC(c->iscratch, 64 * sizeof(int));
 #undef ALLOC

@@ -33,40 +33,103 @@ __device__ __forceinline__ float dot3(float x, float y, float z) {
   return d2 + z * z;
 }
 
-// dynamics.py:96-171, Jacobi form
-__global__ void k_pbd(const float *__restrict__ pin, float *__restrict__ pout,
-                      const float *__restrict__ rad, int n, int G, float L,
-                      float half_l, float inv_l, float gap, float mdf, float dmul,
-                      const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
-                      const int32_t *__restrict__ ci) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float px = pin[3 * i], py = pin[3 * i + 1], pz = pin[3 * i + 2];
-  const float ri = rad[i];
-  const float maxd = mdf * ri;
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  const int cx = centre_axis(px, G, L, half_l, inv_l);
-  const int cy = centre_axis(py, G, L, half_l, inv_l);
-  const int cz = centre_axis(pz, G, L, half_l, inv_l);
-  for_stencil(cx, cy, cz, G, cs, cc, ci, [&](int j) {
-    if (j == i) return;
-    const float dx = gwrap(pin[3 * j] - px, L, inv_l);
-    const float dy = gwrap(pin[3 * j + 1] - py, L, inv_l);
-    const float dz = gwrap(pin[3 * j + 2] - pz, L, inv_l);
-    const float dist = sqrtf(dot3(dx, dy, dz) + 1e-12f);
-    const float ts = (ri + rad[j]) * gap;
-    if (dist < ts) {
-      const float overlap = ts - dist;
-      const float ux = dx / dist, uy = dy / dist, uz = dz / dist;
-      const float mag = fminf(overlap * 0.5f, maxd);
-      const float vx = fmaxf(-maxd, fminf(maxd, -ux * mag));
-      const float vy = fmaxf(-maxd, fminf(maxd, -uy * mag));
-      const float vz = fmaxf(-maxd, fminf(maxd, -uz * mag));
-      ax = ax + vx;
-      ay = ay + vy;
-      az = az + vz;
+// PBD with 32 lanes per particle (8 particles per 256-thread block): lane c <
+// 27 walks stencil cell c = (dx+1)*9 + (dy+1)*3 + (dz+1) and parks its
+// overlap corrections in LDS in occupant order; lane 0 then adds them in the
+// reference order (cells dx, dy, dz, occupants ascending), so the sum is the
+// serial one bit for bit.  A cell with more than PBD_SLOTS corrections sends
+// its particle through the serial loop instead.
+#define PBD_SLOTS 6
+__device__ __forceinline__ void pbd_pair(int j, int i, float px, float py, float pz,
+                                         float ri, float maxd, float gap, float L,
+                                         float inv_l, const float *__restrict__ pin,
+                                         const float *__restrict__ rad, bool &hit,
+                                         float &vx, float &vy, float &vz) {
+  hit = false;
+  if (j == i) return;
+  const float dx = gwrap(pin[3 * j] - px, L, inv_l);
+  const float dy = gwrap(pin[3 * j + 1] - py, L, inv_l);
+  const float dz = gwrap(pin[3 * j + 2] - pz, L, inv_l);
+  const float dist = sqrtf(dot3(dx, dy, dz) + 1e-12f);
+  const float ts = (ri + rad[j]) * gap;
+  if (dist < ts) {
+    const float overlap = ts - dist;
+    const float ux = dx / dist, uy = dy / dist, uz = dz / dist;
+    const float mag = fminf(overlap * 0.5f, maxd);
+    vx = fmaxf(-maxd, fminf(maxd, -ux * mag));
+    vy = fmaxf(-maxd, fminf(maxd, -uy * mag));
+    vz = fmaxf(-maxd, fminf(maxd, -uz * mag));
+    hit = true;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_pbd_wave(const float *__restrict__ pin, float *__restrict__ pout,
+               const float *__restrict__ rad, int n, int G, float L, float half_l,
+               float inv_l, float gap, float mdf, float dmul,
+               const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+               const int32_t *__restrict__ ci) {
+  __shared__ float s_v[8][27][PBD_SLOTS][3];
+  __shared__ int s_cnt[8][27];
+  const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
+  const int i = blockIdx.x * 8 + grp;
+  const bool live = i < n;
+  float px = 0.f, py = 0.f, pz = 0.f, ri = 0.f, maxd = 0.f;
+  int cx = 0, cy = 0, cz = 0;
+  if (live) {
+    px = pin[3 * i];
+    py = pin[3 * i + 1];
+    pz = pin[3 * i + 2];
+    ri = rad[i];
+    maxd = mdf * ri;
+    cx = centre_axis(px, G, L, half_l, inv_l);
+    cy = centre_axis(py, G, L, half_l, inv_l);
+    cz = centre_axis(pz, G, L, half_l, inv_l);
+  }
+  if (live && l < 27) {
+    const int id = wrapi(cx + l / 9 - 1, G) * G * G + wrapi(cy + (l / 3) % 3 - 1, G) * G +
+                   wrapi(cz + l % 3 - 1, G);
+    const int st = cs[id], ct = cc[id];
+    int m = 0;
+    for (int k = st; k < st + ct; ++k) {
+      bool hit;
+      float vx, vy, vz;
+      pbd_pair(ci[k], i, px, py, pz, ri, maxd, gap, L, inv_l, pin, rad, hit, vx, vy, vz);
+      if (hit) {
+        if (m < PBD_SLOTS) {
+          s_v[grp][l][m][0] = vx;
+          s_v[grp][l][m][1] = vy;
+          s_v[grp][l][m][2] = vz;
+        }
+        m += 1;
+      }
     }
-  });
+    s_cnt[grp][l] = m;
+  }
+  __syncthreads();
+  if (!live || l != 0) return;
+  bool spill = false;
+  for (int c = 0; c < 27; ++c) spill |= s_cnt[grp][c] > PBD_SLOTS;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  if (!spill) {
+    for (int c = 0; c < 27; ++c)
+      for (int q = 0; q < s_cnt[grp][c]; ++q) {
+        ax = ax + s_v[grp][c][q][0];
+        ay = ay + s_v[grp][c][q][1];
+        az = az + s_v[grp][c][q][2];
+      }
+  } else {
+    for_stencil(cx, cy, cz, G, cs, cc, ci, [&](int j) {
+      bool hit;
+      float vx, vy, vz;
+      pbd_pair(j, i, px, py, pz, ri, maxd, gap, L, inv_l, pin, rad, hit, vx, vy, vz);
+      if (hit) {
+        ax = ax + vx;
+        ay = ay + vy;
+        az = az + vz;
+      }
+    });
+  }
   const float maxt = maxd * dmul;
   const float cm = sqrtf(dot3(ax, ay, az));
   if (cm > maxt) {
@@ -80,34 +143,41 @@ __global__ void k_pbd(const float *__restrict__ pin, float *__restrict__ pout,
   pout[3 * i + 2] = gwrap(pz + az, L, inv_l);
 }
 
-// dynamics.py:313-352 (+ the global max of dynamics.py:356-364, as a true
-// maximum: non-negative floats order like their int bit patterns).
-__global__ void k_max_overlap(const float *__restrict__ pos,
-                              const float *__restrict__ rad, int n, int G, float L,
-                              float half_l, float inv_l,
-                              const int32_t *__restrict__ cs,
-                              const int32_t *__restrict__ cc,
-                              const int32_t *__restrict__ ci,
-                              float *__restrict__ lmd, float *gmax) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
-  const float ri = rad[i];
+// Max overlap with 32 lanes per particle: the maximum is order-free, so the
+// lanes' partial maxima combine with a half-wave reduction.
+__global__ void __launch_bounds__(256)
+    k_max_overlap_wave(const float *__restrict__ pos, const float *__restrict__ rad, int n,
+                       int G, float L, float half_l, float inv_l,
+                       const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+                       const int32_t *__restrict__ ci, float *__restrict__ lmd,
+                       float *gmax) {
+  const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
+  const int i = blockIdx.x * 8 + grp;
   float lm = 0.0f;
-  const int cx = centre_axis(px, G, L, half_l, inv_l);
-  const int cy = centre_axis(py, G, L, half_l, inv_l);
-  const int cz = centre_axis(pz, G, L, half_l, inv_l);
-  for_stencil(cx, cy, cz, G, cs, cc, ci, [&](int j) {
-    if (!(i < j)) return;
-    const float dx = gwrap(pos[3 * j] - px, L, inv_l);
-    const float dy = gwrap(pos[3 * j + 1] - py, L, inv_l);
-    const float dz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
-    const float dist = sqrtf(dot3(dx, dy, dz) + 1e-8f);
-    const float td = ri + rad[j];
-    if (dist < td) lm = fmaxf(lm, td - dist);
-  });
-  lmd[i] = lm;
-  if (lm > 0.0f) atomicMax((int *)gmax, __float_as_int(lm));
+  if (i < n && l < 27) {
+    const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    const float ri = rad[i];
+    const int cx = centre_axis(px, G, L, half_l, inv_l);
+    const int cy = centre_axis(py, G, L, half_l, inv_l);
+    const int cz = centre_axis(pz, G, L, half_l, inv_l);
+    const int id = wrapi(cx + l / 9 - 1, G) * G * G + wrapi(cy + (l / 3) % 3 - 1, G) * G +
+                   wrapi(cz + l % 3 - 1, G);
+    for (int k = cs[id]; k < cs[id] + cc[id]; ++k) {
+      const int j = ci[k];
+      if (!(i < j)) continue;
+      const float dx = gwrap(pos[3 * j] - px, L, inv_l);
+      const float dy = gwrap(pos[3 * j + 1] - py, L, inv_l);
+      const float dz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
+      const float dist = sqrtf(dot3(dx, dy, dz) + 1e-8f);
+      const float td = ri + rad[j];
+      if (dist < td) lm = fmaxf(lm, td - dist);
+    }
+  }
+  for (int o = 16; o > 0; o >>= 1) lm = fmaxf(lm, __shfl_xor(lm, o, 32));
+  if (i < n && l == 0) {
+    lmd[i] = lm;
+    if (lm > 0.0f) atomicMax((int *)gmax, __float_as_int(lm));
+  }
 }
 
 // dynamics.py:1226-1240
@@ -205,7 +275,46 @@ __device__ __forceinline__ float pow_f32(float x, float y) {
   return (float)pow((double)x, (double)y);
 }
 
-// dynamics.py:1502-1543 (delta for particle i from the current radii)
+// r^p for the volume: with p == 3 the product (r*r)*r in double is the
+// correctly rounded cube (r*r is exact), i.e. the same value pow gives.
+__device__ __forceinline__ float vol_f32(float r, float p) {
+  if (p == 3.0f) {
+    const double d = (double)r;
+    return (float)(d * d * d);
+  }
+  return pow_f32(r, p);
+}
+
+// dynamics.py:1502-1543: radius change of particle i from the radii rin.
+__device__ __forceinline__ float pressure_delta(int i, int n, int frame,
+                                                const float *__restrict__ rin,
+                                                const int32_t *__restrict__ face_ids,
+                                                const int32_t *__restrict__ fsc, float k,
+                                                float p_exp, float inv_p, float cap,
+                                                int max_nei) {
+  const int fc = fsc[i];
+  float acc = 0.0f;
+  if (fc > 0) {
+    const float ri = rin[i];
+    const float vi = vol_f32(ri, p_exp);
+    const int an = fc < max_nei ? fc : max_nei;
+    int idx = lcg_hash(i, frame) % fc;
+    const int32_t *row = face_ids + (int64_t)i * FOS_MAX_NEIGHBORS;
+#pragma unroll 4
+    for (int q = 0; q < an; ++q) {
+      const int j = idx < FOS_MAX_NEIGHBORS ? row[idx] : -1;  // fsc <= 32 by construction
+      idx = (idx + 1 == fc) ? 0 : idx + 1;
+      if (j < 0 || j >= n || j == i) continue;
+      const float vj = vol_f32(rin[j], p_exp);
+      const float raw = k * (vi - vj);
+      const float vmin = fminf(vi, vj);
+      const float dv = fmaxf(-cap * vmin, fminf(cap * vmin, raw));
+      acc = acc + (pow_f32(vi - dv, inv_p) - ri);
+    }
+  }
+  return acc;
+}
+
 __global__ void k_pressure_delta(int n, int frame, const float *__restrict__ rad,
                                  float *__restrict__ delta_r,
                                  const int32_t *__restrict__ face_ids,
@@ -213,25 +322,29 @@ __global__ void k_pressure_delta(int n, int frame, const float *__restrict__ rad
                                  float p_exp, float inv_p, float cap, int max_nei) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int fc = fsc[i];
-  float acc = 0.0f;
-  if (fc > 0) {
-    const float ri = rad[i];
-    const float vi = pow_f32(ri, p_exp);
-    const int an = fc < max_nei ? fc : max_nei;
-    const int start = lcg_hash(i, frame) % fc;
-    for (int q = 0; q < an; ++q) {
-      const int idx = (start + q) % fc;
-      const int j = face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + idx];
-      if (j < 0 || j >= n || j == i) continue;
-      const float vj = pow_f32(rad[j], p_exp);
-      const float raw = k * (vi - vj);
-      const float vmin = fminf(vi, vj);
-      const float dv = fmaxf(-cap * vmin, fminf(cap * vmin, raw));
-      acc = acc + (pow_f32(vi - dv, inv_p) - ri);
+  delta_r[i] = pressure_delta(i, n, frame, rad, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
+}
+
+// One whole Jacobi iteration (dynamics.py:1497-1557) from rin into rout;
+// stats (when given) gets max|dr| and the changed count.
+__global__ void k_pressure_step(int n, int frame, const float *__restrict__ rin,
+                                float *__restrict__ rout, float *__restrict__ delta_r,
+                                const int32_t *__restrict__ face_ids,
+                                const int32_t *__restrict__ fsc, float k, float p_exp,
+                                float inv_p, float cap, int max_nei, float r_min,
+                                float r_max, float *stats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = pressure_delta(i, n, frame, rin, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
+  delta_r[i] = d;
+  if (stats) {
+    const float a = fabsf(d);
+    if (a > 1e-12f) {
+      atomicAdd((int *)&stats[1], 1);
+      atomicMax((int *)&stats[0], __float_as_int(a));
     }
   }
-  delta_r[i] = acc;
+  rout[i] = fmaxf(r_min, fminf(r_max, rin[i] + d));
 }
 
 // dynamics.py:1546-1557
@@ -435,8 +548,8 @@ int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad, int n,
                               cell_start && cell_count && cell_indices)),
               "fos_pbd_pass: bad arguments (n=%d G=%d)", n, G);
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_pbd, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
-                     pos_in, pos_out, rad, n, G, L, (float)(0.5 * (double)L),
+  hipLaunchKernelGGL(k_pbd_wave, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                     (hipStream_t)stream, pos_in, pos_out, rad, n, G, L, (float)(0.5 * (double)L),
                      (float)(1.0 / (double)L), gap_factor, max_disp_frac, disp_mult,
                      cell_start, cell_count, cell_indices);
   FOS_CHECK_LAUNCH("k_pbd");
@@ -453,7 +566,7 @@ int fos_max_overlap(const float *pos, const float *rad, int n, int G, float L,
               "fos_max_overlap: bad arguments (n=%d G=%d)", n, G);
   int rc = fos_fill_f32(max_depth, 1, 0.0f, stream);
   if (rc || n == 0) return rc;
-  hipLaunchKernelGGL(k_max_overlap, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+  hipLaunchKernelGGL(k_max_overlap_wave, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                      (hipStream_t)stream, pos, rad, n, G, L, (float)(0.5 * (double)L),
                      (float)(1.0 / (double)L), cell_start, cell_count, cell_indices,
                      local_max_depth, max_depth);
@@ -542,6 +655,37 @@ int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
   hipLaunchKernelGGL(k_pressure_apply, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
                      rad, delta_r, r_min, r_max, stats);
   FOS_CHECK_LAUNCH("k_pressure_apply");
+  return 0;
+}
+
+int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
+                     float *delta_r, const int32_t *face_ids, const int32_t *fsc, float k,
+                     float p_exp, float pair_cap, int max_nei, float r_min, float r_max,
+                     float *stats, void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && n <= ctx->max_n && iters >= 0 &&
+                  (n == 0 || (rad && delta_r && face_ids && fsc)),
+              "fos_pressure_run: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (stats) {
+    int rc = fos_fill_f32(stats, 2, 0.0f, stream);
+    if (rc) return rc;
+  }
+  if (n == 0 || iters == 0) return 0;
+  float *a = rad, *b = ctx->rad_tmp;
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_pressure_step, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n, frame,
+                       a, b, delta_r, face_ids, fsc, k, p_exp, 1.0f / p_exp, pair_cap,
+                       max_nei, r_min, r_max, it == iters - 1 ? stats : nullptr);
+    FOS_CHECK_LAUNCH("k_pressure_step");
+    float *t = a;
+    a = b;
+    b = t;
+  }
+  if (a != rad) {
+    hipLaunchKernelGGL(k_copy_f32, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, a, rad,
+                       (int64_t)n);
+    FOS_CHECK_LAUNCH("k_copy_f32");
+  }
   return 0;
 }
 

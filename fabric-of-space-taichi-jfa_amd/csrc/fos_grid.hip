@@ -66,6 +66,41 @@ __global__ void k_copy_cells(const int32_t *__restrict__ s, int32_t *__restrict_
   if (c < C) d[c] = s[c];
 }
 
+// grid.py:229-261 in one block: exclusive scan of up to 64K cell counts into
+// cell_start and the scatter pointers (each thread scans a contiguous run).
+__global__ void __launch_bounds__(1024)
+    k_scan_cells(const int32_t *__restrict__ count, int32_t *__restrict__ start,
+                 int32_t *__restrict__ wr, int C) {
+  __shared__ int s_wave[16];
+  const int t = (int)threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int per = (C + 1023) / 1024;
+  const int b = t * per, e = b + per < C ? b + per : C;
+  int s = 0;
+  for (int k = b; k < e; ++k) s += count[k];
+  int x = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    int w = lane < 16 ? s_wave[lane] : 0;
+    for (int o = 1; o < 16; o <<= 1) {
+      const int y = __shfl_up(w, o, 64);
+      if (lane >= o) w += y;
+    }
+    if (lane < 16) s_wave[lane] = w;
+  }
+  __syncthreads();
+  int ex = (wid ? s_wave[wid - 1] : 0) + x - s;
+  for (int k = b; k < e; ++k) {
+    start[k] = ex;
+    wr[k] = ex;
+    ex += count[k];
+  }
+}
+
 extern "C" {
 
 int fos_grid_count(const float *pos, int n, int G, float L, int32_t *cell_count,
@@ -119,11 +154,17 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
   if (rc) return rc;
   rc = fos_grid_count(pos, n, G, L, cell_count, stream);
   if (rc) return rc;
-  rc = exclusive_scan(ctx, cell_count, cell_start, C, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_copy_cells, dim3(blocks_for(C)), dim3(FOS_BLOCK), 0, st,
-                     cell_start, ctx->cell_write, C);
-  FOS_CHECK_LAUNCH("k_copy_cells");
+  if (C <= 65536) {
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, cell_count, cell_start,
+                       ctx->cell_write, C);
+    FOS_CHECK_LAUNCH("k_scan_cells");
+  } else {
+    rc = exclusive_scan(ctx, cell_count, cell_start, C, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_copy_cells, dim3(blocks_for(C)), dim3(FOS_BLOCK), 0, st,
+                       cell_start, ctx->cell_write, C);
+    FOS_CHECK_LAUNCH("k_copy_cells");
+  }
   return fos_grid_scatter(pos, n, G, L, ctx->cell_write, cell_indices, stream);
 }
 

@@ -1,6 +1,8 @@
 // fos_jfa.hip -- power-diagram Jump Flood and face-sharing-count extraction
 // (reference: jfa.py).  Labels are int32 owner ids in one dense res^3 grid,
 // index x*res*res + y*res + z (z fastest, as Taichi's ndrange).
+#include <string.h>
+
 #include "fos_common.h"
 
 namespace fos {
@@ -64,99 +66,253 @@ __global__ void k_seed(const float *__restrict__ pos,
 // Whether pass p runs under the device-side early exit (jfa.py:1104-1112):
 // it runs unless an earlier pass q >= min_passes-1 changed fewer than
 // th * V voxels.  Every thread derives the same answer from the history.
-__device__ __forceinline__ bool pass_active(const int32_t *info, int p,
-                                            int early, double th, int min_passes,
-                                            double V) {
+// Must be reached by whole waves (the striped totals use cross-lane sums).
+__device__ __forceinline__ bool pass_active(const int *stripes, int p, int early,
+                                            double th, int min_passes, double V) {
   if (!early) return true;
   for (int q = 0; q < p; ++q) {
-    if (q >= min_passes - 1 && (double)info[2 + q] / V < th) return false;
+    if (q >= min_passes - 1 && (double)stripes_total(stripes, q) / V < th) return false;
   }
   return true;
 }
 
-// Block (64, 4): 64 consecutive z voxels per wave; grid (ceil(res/64),
-// ceil(res/4), res).  Candidates equal to the incumbent or to an earlier
-// candidate are skipped: under the strict '<' they can never win, so the
-// result is identical to evaluating all 27.
-template <bool UNIT_L>
-__global__ void __launch_bounds__(256)
-    k_jfa_step(const float4 *__restrict__ seeds, int res, float L, float half_l,
-               float voxel, int step, const int32_t *__restrict__ in,
-               int32_t *__restrict__ out, int32_t *__restrict__ changes,
-               const int32_t *__restrict__ info, int pass, int early, double th,
-               int min_passes) {
-  if (info) {
-    double V = (double)res * res * res;
-    if (!pass_active(info, pass, early, th, min_passes, V)) return;
+// pass_active evaluated by wave 0 and shared with the block (all threads of
+// the block must call it).
+__device__ __forceinline__ bool block_pass_active(const int *stripes, int p, int early,
+                                                  double th, int min_passes, double V) {
+  if (!early) return true;
+  __shared__ int s_active;
+  if (threadIdx.x < 64) {
+    const bool a = pass_active(stripes, p, early, th, min_passes, V);
+    if (threadIdx.x == 0) s_active = a ? 1 : 0;
   }
-  const int z = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int x = blockIdx.z;
-  int changed = 0;
-  if (z < res && y < res) {
-    const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
-                cz = vcenter(z, voxel, half_l);
-    const int64_t v = ((int64_t)x * res + y) * res + z;
-    const int old = in[v];
-    int best = old;
-    float bd = 1e10f;
-    if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
-    int xs[3], ys[3], zs[3];
+  __syncthreads();
+  return s_active != 0;
+}
+
+__device__ __forceinline__ int *pass_stripe(int *stripes, int pass) {
+  return stripes + (pass * FOS_STRIPES + (int)(blockIdx.x % FOS_STRIPES)) * FOS_STRIPE_INTS;
+}
+
+// The last K owners this voxel evaluated.  A candidate equal to the current
+// best or to an already evaluated owner cannot win under the strict '<'
+// (its power distance is unchanged and best only decreases), so skipping it
+// leaves the result identical to evaluating all 27 in scan order.
+template <int K>
+struct Seen {
+  int e[K];
+  __device__ __forceinline__ void init(int first) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      xs[k] = wrap1(x + (k - 1) * step, res);
-      ys[k] = wrap1(y + (k - 1) * step, res);
-      zs[k] = wrap1(z + (k - 1) * step, res);
+    for (int k = 0; k < K; ++k) e[k] = -1;
+    e[0] = first;
+  }
+  __device__ __forceinline__ bool has(int c) const {
+    bool h = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) h |= (e[k] == c);
+    return h;
+  }
+  __device__ __forceinline__ void push(int c) {
+#pragma unroll
+    for (int k = K - 1; k > 0; --k) e[k] = e[k - 1];
+    e[0] = c;
+  }
+};
+
+template <bool UNIT_L, int K>
+__device__ __forceinline__ void consider(int cand, int &best, float &bd, Seen<K> &seen,
+                                         float cx, float cy, float cz,
+                                         const float4 *__restrict__ seeds, float L) {
+  if (cand >= 0 && cand != best && !seen.has(cand)) {
+    seen.push(cand);
+    const float pd = power_dist<UNIT_L>(cx, cy, cz, seeds[cand], L);
+    if (pd < bd) {
+      bd = pd;
+      best = cand;
     }
-    int last = best;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const int64_t row = ((int64_t)xs[a] * res + ys[b]) * res;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const int cand = in[row + zs[c]];
-          if (cand >= 0 && cand != best && cand != last) {
-            last = cand;
-            const float pd = power_dist<UNIT_L>(cx, cy, cz, seeds[cand], L);
-            if (pd < bd) {
-              bd = pd;
-              best = cand;
-            }
-          }
-        }
-      }
-    out[v] = best;
-    changed = (best != old);
   }
-  if (changes) block_add(changed, changes);
+}
+
+// Tile -> block mapping that keeps each XCD on a contiguous range of tiles
+// (workgroups are dispatched round-robin over the 8 XCDs, so consecutive
+// block ids land on different L2s).  Tiles are ordered x-major, so an XCD
+// sweeps a slab of x-planes whose neighbour rows stay in its own L2.
+#define FOS_XCDS 8
+__device__ __forceinline__ int xcd_tile(int bid, int per_xcd) {
+  return (bid % FOS_XCDS) * per_xcd + bid / FOS_XCDS;
+}
+
+// One pass, labels read straight from global memory (L2).  Block = 4 waves,
+// tile = 1 x-plane x 4 y-rows x 64 z (one wave per y-row).
+template <bool UNIT_L, int K>
+__global__ void __launch_bounds__(256)
+    k_jfa_pass(const float4 *__restrict__ seeds, int res, float L, float half_l,
+               float voxel, int step, const int32_t *__restrict__ in,
+               int32_t *__restrict__ out, int *__restrict__ stripes, int pass,
+               int early, double th, int min_passes, int nyb, int nzb, int ntiles,
+               int per_xcd) {
+  if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
+  const int tile = xcd_tile(blockIdx.x, per_xcd);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int changed = 0;
+  if (tile < ntiles) {
+    const int zb = tile % nzb, t2 = tile / nzb;
+    const int yb = t2 % nyb, x = t2 / nyb;
+    const int z = zb * 64 + lane, y = yb * 4 + w;
+    if (z < res && y < res) {
+      const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
+                  cz = vcenter(z, voxel, half_l);
+      const int v = (x * res + y) * res + z;
+      int lab[27];
+      int xs[3], ys[3], zs[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        xs[k] = wrap1(x + (k - 1) * step, res);
+        ys[k] = wrap1(y + (k - 1) * step, res);
+        zs[k] = wrap1(z + (k - 1) * step, res);
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const int row = (xs[a] * res + ys[b]) * res;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) lab[(a * 3 + b) * 3 + c] = in[row + zs[c]];
+        }
+      const int old = lab[13];
+      int best = old;
+      float bd = 1e10f;
+      if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
+      Seen<K> seen;
+      seen.init(best);
+#pragma unroll
+      for (int q = 0; q < 27; ++q) consider<UNIT_L, K>(lab[q], best, bd, seen, cx, cy, cz, seeds, L);
+      out[v] = best;
+      changed = (best != old);
+    }
+  }
+  block_add(changed, pass_stripe(stripes, pass));
+}
+
+// One pass for step <= halo with the labels of a TX x 4 x 64 tile plus a
+// halo of `step` staged in LDS (one global read per staged label instead of
+// 27 per voxel).  Each thread owns one (y, z) column and TX x-planes.
+template <bool UNIT_L, int K, int TX>
+__global__ void __launch_bounds__(256)
+    k_jfa_pass_lds(const float4 *__restrict__ seeds, int res, float L, float half_l,
+                   float voxel, int step, const int32_t *__restrict__ in,
+                   int32_t *__restrict__ out, int *__restrict__ stripes, int pass,
+                   int early, double th, int min_passes, int nyb, int nzb, int ntiles,
+                   int per_xcd) {
+  extern __shared__ int s_lab[];
+  if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
+  const int tile = xcd_tile(blockIdx.x, per_xcd);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int s = step;
+  const int AY = 4 + 2 * s, AZ = 64 + 2 * s, AX = TX + 2 * s;
+  int changed = 0;
+  const bool live = tile < ntiles;
+  int x0 = 0, y0 = 0, z0 = 0;
+  if (live) {
+    const int zb = tile % nzb, t2 = tile / nzb;
+    const int yb = t2 % nyb;
+    x0 = (t2 / nyb) * TX;
+    y0 = yb * 4;
+    z0 = zb * 64;
+    for (int r = w; r < AX * AY; r += 4) {
+      const int xi = r / AY, yi = r - xi * AY;
+      const int gx = wrapi(x0 - s + xi, res), gy = wrapi(y0 - s + yi, res);
+      const int base = (gx * res + gy) * res;
+      for (int zi = lane; zi < AZ; zi += 64)
+        s_lab[r * AZ + zi] = in[base + wrapi(z0 - s + zi, res)];
+    }
+  }
+  __syncthreads();
+  if (live) {
+    const int y = y0 + w, z = z0 + lane;
+    if (y < res && z < res) {
+      const float cy = vcenter(y, voxel, half_l), cz = vcenter(z, voxel, half_l);
+#pragma unroll 1
+      for (int xt = 0; xt < TX; ++xt) {
+        const int x = x0 + xt;
+        if (x >= res) break;
+        const float cx = vcenter(x, voxel, half_l);
+        const int cbase = ((xt + s) * AY + (w + s)) * AZ + (lane + s);
+        int lab[27];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              lab[(a * 3 + b) * 3 + c] =
+                  s_lab[cbase + ((a - 1) * AY + (b - 1)) * s * AZ + (c - 1) * s];
+        const int old = lab[13];
+        int best = old;
+        float bd = 1e10f;
+        if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
+        Seen<K> seen;
+        seen.init(best);
+#pragma unroll
+        for (int q = 0; q < 27; ++q)
+          consider<UNIT_L, K>(lab[q], best, bd, seen, cx, cy, cz, seeds, L);
+        out[(x * res + y) * res + z] = best;
+        changed += (best != old);
+      }
+    }
+  }
+  block_add(changed, pass_stripe(stripes, pass));
 }
 
 // Ends the pass loop: records passes executed and the exit flag, and copies
 // temp into grid when an odd number of passes ran (jfa.py:1115-1116).
+// The exit flag is set when the condition held after the last executed pass,
+// including the final scheduled one (jfa.py:1109-1112 breaks there too).
 __global__ void k_jfa_finish(int res, int num_passes, int early, double th,
-                             int min_passes, int32_t *__restrict__ info,
-                             const int32_t *__restrict__ temp,
+                             int min_passes, const int *__restrict__ stripes,
+                             int32_t *__restrict__ info, const int32_t *__restrict__ temp,
                              int32_t *__restrict__ grid) {
   const double V = (double)res * res * res;
-  int executed = 0;
-  for (int p = 0; p < num_passes; ++p) {
-    if (!pass_active(info, p, early, th, min_passes, V)) break;
-    executed = p + 1;
+  __shared__ int s_exec, s_fired;
+  if (threadIdx.x < 64) {
+    int executed = 0;
+    for (int p = 0; p < num_passes; ++p) {
+      if (!pass_active(stripes, p, early, th, min_passes, V)) break;
+      executed = p + 1;
+    }
+    const int fired = (early && executed > 0 &&
+                       !pass_active(stripes, executed, early, th, min_passes, V)) ? 1 : 0;
+    if (threadIdx.x == 0) {
+      s_exec = executed;
+      s_fired = fired;
+    }
   }
-  const int fired = (executed < num_passes) ? 1 : 0;
+  __syncthreads();
+  const int executed = s_exec, fired = s_fired;
   const int64_t total = (int64_t)res * res * res;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) {
-    info[0] = executed;
-    info[1] = fired;
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    for (int p = 0; p < executed; ++p) {
+      const int c = stripes_total(stripes, p);
+      if (threadIdx.x == 0) info[2 + p] = c;
+    }
+    if (threadIdx.x == 0) {
+      for (int p = executed; p < num_passes; ++p) info[2 + p] = 0;
+      info[0] = executed;
+      info[1] = fired;
+    }
   }
   // The last executed pass also decides the exit when it is the final one.
   if (executed & 1) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = t; i < total; i += stride) grid[i] = temp[i];
   }
+}
+
+// Adds the striped total of one pass slot into *dst (one wave).
+__global__ void k_stripes_add(const int *__restrict__ stripes, int slot, int32_t *dst) {
+  const int t = stripes_total(stripes, slot);
+  if (threadIdx.x == 0) *dst += t;
 }
 
 __global__ void k_copy_i32(const int32_t *__restrict__ s, int32_t *__restrict__ d,
@@ -170,26 +326,45 @@ __global__ void k_copy_i32(const int32_t *__restrict__ s, int32_t *__restrict__ 
 // Majority filter (jfa.py:640-699)
 // ---------------------------------------------------------------------------
 
+// This thread's voxel under the XCD-aware tile order (tile = 1 x-plane x 4
+// y-rows x 64 z, x-major); false for padding threads.
+__device__ __forceinline__ bool vox_of(int res, int nyb, int nzb, int ntiles, int per,
+                                       int &x, int &y, int &z) {
+  const int tile = xcd_tile(blockIdx.x, per);
+  if (tile >= ntiles) return false;
+  const int zb = tile % nzb, t2 = tile / nzb;
+  const int yb = t2 % nyb;
+  x = t2 / nyb;
+  z = zb * 64 + (int)(threadIdx.x & 63);
+  y = yb * 4 + (int)(threadIdx.x >> 6);
+  return z < res && y < res;
+}
+
+// The 6 face neighbours in reference order: axis 0..2, direction -1 then +1.
+__device__ __forceinline__ void face_neighbours(const int32_t *__restrict__ g, int res, int x,
+                                                int y, int z, int nb[6]) {
+  const int rr = res * res;
+  nb[0] = g[wrap1(x - 1, res) * rr + y * res + z];
+  nb[1] = g[wrap1(x + 1, res) * rr + y * res + z];
+  nb[2] = g[x * rr + wrap1(y - 1, res) * res + z];
+  nb[3] = g[x * rr + wrap1(y + 1, res) * res + z];
+  nb[4] = g[x * rr + y * res + wrap1(z - 1, res)];
+  nb[5] = g[x * rr + y * res + wrap1(z + 1, res)];
+}
+
 __global__ void __launch_bounds__(256)
-    k_majority(int res, const int32_t *__restrict__ g, int32_t *__restrict__ f) {
-  const int z = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int x = blockIdx.z;
-  if (z >= res || y >= res) return;
-  const int64_t v = ((int64_t)x * res + y) * res + z;
+    k_majority(int res, const int32_t *__restrict__ g, int32_t *__restrict__ f, int nyb,
+               int nzb, int ntiles, int per) {
+  int x, y, z;
+  if (!vox_of(res, nyb, nzb, ntiles, per, x, y, z)) return;
+  const int v = (x * res + y) * res + z;
   const int my = g[v];
   if (my < 0) {
     f[v] = -1;
     return;
   }
-  const int64_t rr = (int64_t)res * res;
   int nb[6];
-  nb[0] = g[(int64_t)wrap1(x - 1, res) * rr + (int64_t)y * res + z];
-  nb[1] = g[(int64_t)wrap1(x + 1, res) * rr + (int64_t)y * res + z];
-  nb[2] = g[(int64_t)x * rr + (int64_t)wrap1(y - 1, res) * res + z];
-  nb[3] = g[(int64_t)x * rr + (int64_t)wrap1(y + 1, res) * res + z];
-  nb[4] = g[(int64_t)x * rr + (int64_t)y * res + wrap1(z - 1, res)];
-  nb[5] = g[(int64_t)x * rr + (int64_t)y * res + wrap1(z + 1, res)];
+  face_neighbours(g, res, x, y, z, nb);
   int ids[7], cnt[7], nu = 1;
   ids[0] = my;
   cnt[0] = 1;
@@ -251,41 +426,66 @@ __device__ __forceinline__ void tab_insert(int *__restrict__ tb,
   atomicOr(overflow, 1);
 }
 
+__device__ __forceinline__ int wave_sum(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ unsigned wave_min_u(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned w = (unsigned)__shfl_xor((int)v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+// A voxel first merges its repeated partners (one entry per distinct b with
+// the smallest key and the count).  The wave then merges equal (a, b) pairs
+// across lanes -- neighbouring voxels on one boundary emit the same pair --
+// so each distinct pair costs one table insert per wave.  Counts add and keys
+// take the minimum, so the merge order does not matter.
 __global__ void __launch_bounds__(256)
-    k_face_emit(int res, int n, const int32_t *__restrict__ f, int *tb,
-                unsigned *tk, int *tc, int cap, int *overflow) {
-  const int z = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int x = blockIdx.z;
-  if (z >= res || y >= res) return;
-  const int64_t v = ((int64_t)x * res + y) * res + z;
-  const int a = f[v];
-  if (a < 0 || a >= n) return;
-  const int64_t rr = (int64_t)res * res;
-  int nb[6];
-  nb[0] = f[(int64_t)wrap1(x - 1, res) * rr + (int64_t)y * res + z];
-  nb[1] = f[(int64_t)wrap1(x + 1, res) * rr + (int64_t)y * res + z];
-  nb[2] = f[(int64_t)x * rr + (int64_t)wrap1(y - 1, res) * res + z];
-  nb[3] = f[(int64_t)x * rr + (int64_t)wrap1(y + 1, res) * res + z];
-  nb[4] = f[(int64_t)x * rr + (int64_t)y * res + wrap1(z - 1, res)];
-  nb[5] = f[(int64_t)x * rr + (int64_t)y * res + wrap1(z + 1, res)];
-  const unsigned kbase = 6u * (unsigned)v;
-  // Merge repeated partners of this voxel first: one insert per distinct b,
-  // carrying the smallest key (the first face in scan order) and the count.
+    k_face_emit(int res, int n, const int32_t *__restrict__ f, int *tb, unsigned *tk,
+                int *tc, int cap, int *overflow, int nyb, int nzb, int ntiles, int per) {
+  int x = 0, y = 0, z = 0;
+  const bool live = vox_of(res, nyb, nzb, ntiles, per, x, y, z);
+  const int lane = (int)(threadIdx.x & 63);
+  int a = -1;
+  int nb[6] = {-1, -1, -1, -1, -1, -1};
+  unsigned kbase = 0;
+  if (live) {
+    const int v = (x * res + y) * res + z;
+    a = f[v];
+    if (a >= 0 && a < n) {
+      face_neighbours(f, res, x, y, z, nb);
+      kbase = 6u * (unsigned)v;
+    } else {
+      a = -1;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const int b = nb[k];
-    if (b < 0 || b >= n || a >= b) continue;
-    bool dup = false;
+    bool has = a >= 0 && b >= 0 && b < n && a < b;
+    int cnt = 1;
 #pragma unroll
     for (int q = 0; q < k; ++q)
-      if (nb[q] == b) dup = true;
-    if (dup) continue;
-    int cnt = 1;
+      if (nb[q] == b) has = false;
 #pragma unroll
     for (int q = k + 1; q < 6; ++q)
       if (nb[q] == b) cnt += 1;
-    tab_insert(tb, tk, tc, cap, a, b, kbase + (unsigned)k, cnt, overflow);
+    const unsigned key = kbase + (unsigned)k;
+    unsigned long long pend = __ballot(has);
+    while (pend) {
+      const int leader = __ffsll((long long)pend) - 1;
+      const int la = __shfl(a, leader, 64), lb = __shfl(b, leader, 64);
+      const bool m = has && a == la && b == lb;
+      const int c = wave_sum(m ? cnt : 0);
+      const unsigned kk = wave_min_u(m ? key : 0xffffffffu);
+      if (lane == leader) tab_insert(tb, tk, tc, cap, la, lb, kk, c, overflow);
+      if (m) has = false;
+      pend = __ballot(has);
+    }
   }
 }
 
@@ -715,20 +915,94 @@ int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
   return 0;
 }
 
+}  // extern "C"
+
+// Pass-kernel tunables (fos_set_param): steps <= jfa_lds_max_step use the
+// LDS-tiled kernel with jfa_tile_x x-planes per block, larger steps the L2
+// kernel; jfa_dedup is the evaluated-owner list length.
+static int g_lds_max_step = 0;
+static int g_tile_x = 4;
+static int g_dedup = 4;
+
+template <bool U, int K>
+static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float voxel,
+                          int step, const int32_t *in, int32_t *out, int pass, int early,
+                          double th, int minp, hipStream_t st) {
+  int *stripes = ctx->stripes;
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
+  const size_t lds_bytes =
+      (size_t)(g_tile_x + 2 * step) * (4 + 2 * step) * (64 + 2 * step) * sizeof(int);
+  if (step <= g_lds_max_step && lds_bytes <= 65536) {
+    const int tx = g_tile_x;
+    const int nxb = (res + tx - 1) / tx;
+    const int ntiles = nxb * nyb * nzb;
+    const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
+    const size_t shm = (size_t)(tx + 2 * step) * (4 + 2 * step) * (64 + 2 * step) * sizeof(int);
+    auto kern = tx == 1 ? k_jfa_pass_lds<U, K, 1> : tx == 2 ? k_jfa_pass_lds<U, K, 2>
+                                                            : k_jfa_pass_lds<U, K, 4>;
+    hipLaunchKernelGGL(kern, dim3(per * FOS_XCDS), dim3(256), shm, st, ctx->seeds, res, L,
+                       half_l, voxel, step, in, out, stripes, pass, early, th, minp, nyb, nzb,
+                       ntiles, per);
+  } else {
+    const int ntiles = res * nyb * nzb;
+    const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
+    hipLaunchKernelGGL((k_jfa_pass<U, K>), dim3(per * FOS_XCDS), dim3(256), 0, st, ctx->seeds,
+                       res, L, half_l, voxel, step, in, out, stripes, pass, early, th, minp,
+                       nyb, nzb, ntiles, per);
+  }
+}
+
+// Launches pass `pass` (its changed-voxel count accumulates in stripe slot
+// `pass`; the caller zeroes the slot first).
 static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
-                       const int32_t *in, int32_t *out, int32_t *changes,
-                       const int32_t *info, int pass, int early, double th,
+                       const int32_t *in, int32_t *out, int pass, int early, double th,
                        int min_passes, hipStream_t st) {
   const float half_l = (float)(0.5 * (double)L);
-  if (unit_l(L))
-    hipLaunchKernelGGL(k_jfa_step<true>, vox_grid(res), dim3(64, 4), 0, st,
-                       ctx->seeds, res, L, half_l, voxel, step, in, out, changes,
-                       info, pass, early, th, min_passes);
+  const bool u = unit_l(L);
+#define FOS_PASS(U, K)                                                                  \
+  launch_pass_k<U, K>(ctx, res, L, half_l, voxel, step, in, out, pass, early, th, \
+                      min_passes, st)
+  if (g_dedup <= 1) {
+    if (u) FOS_PASS(true, 1); else FOS_PASS(false, 1);
+  } else if (g_dedup <= 4) {
+    if (u) FOS_PASS(true, 4); else FOS_PASS(false, 4);
+  } else {
+    if (u) FOS_PASS(true, 8); else FOS_PASS(false, 8);
+  }
+#undef FOS_PASS
+  FOS_CHECK_LAUNCH("k_jfa_pass");
+  return 0;
+}
+
+extern "C" {
+
+int fos_set_param(const char *name, int value) {
+  FOS_REQUIRE(name, "fos_set_param: name is NULL");
+  if (!strcmp(name, "jfa_lds_max_step")) {
+    FOS_REQUIRE(value >= 0 && value <= 8, "jfa_lds_max_step must be in [0, 8]");
+    g_lds_max_step = value;
+  } else if (!strcmp(name, "jfa_tile_x")) {
+    FOS_REQUIRE(value == 1 || value == 2 || value == 4, "jfa_tile_x must be 1, 2 or 4");
+    g_tile_x = value;
+  } else if (!strcmp(name, "jfa_dedup")) {
+    FOS_REQUIRE(value == 1 || value == 4 || value == 8, "jfa_dedup must be 1, 4 or 8");
+    g_dedup = value;
+  } else {
+    FOS_REQUIRE(false, "fos_set_param: unknown parameter '%s'", name);
+  }
+  return 0;
+}
+
+int fos_get_param(const char *name, int *value) {
+  FOS_REQUIRE(name && value, "fos_get_param: bad arguments");
+  if (!strcmp(name, "jfa_lds_max_step"))
+    *value = g_lds_max_step;
+  else if (!strcmp(name, "jfa_tile_x"))
+    *value = g_tile_x;
+  else if (!strcmp(name, "jfa_dedup"))
+    *value = g_dedup;
   else
-    hipLaunchKernelGGL(k_jfa_step<false>, vox_grid(res), dim3(64, 4), 0, st,
-                       ctx->seeds, res, L, half_l, voxel, step, in, out, changes,
-                       info, pass, early, th, min_passes);
-  FOS_CHECK_LAUNCH("k_jfa_step");
+    FOS_REQUIRE(false, "fos_get_param: unknown parameter '%s'", name);
   return 0;
 }
 
@@ -738,8 +1012,17 @@ int fos_jfa_step(fos_ctx *ctx, int res, float L, float voxel, int step_size,
   FOS_REQUIRE(ctx && in && out && in != out && res >= 1 && res <= ctx->max_res &&
                   step_size >= 1 && step_size <= res && L > 0.0f,
               "fos_jfa_step: bad arguments (res=%d step=%d)", res, step_size);
-  return launch_step(ctx, res, L, voxel, step_size, in, out, changes, nullptr, 0,
-                     0, 0.0, 0, (hipStream_t)stream);
+  hipStream_t st = (hipStream_t)stream;
+  const int slot = FOS_PASS_SLOTS - 1;
+  int *sl = ctx->stripes + slot * FOS_STRIPE_PASS_INTS;
+  int rc = fos_fill_i32(sl, FOS_STRIPE_PASS_INTS, 0, stream);
+  if (rc) return rc;
+  rc = launch_step(ctx, res, L, voxel, step_size, in, out, slot, 0, 0.0, 0, st);
+  if (rc || !changes) return rc;
+  hipLaunchKernelGGL(k_stripes_add, dim3(1), dim3(64), 0, st, (const int *)ctx->stripes, slot,
+                     changes);
+  FOS_CHECK_LAUNCH("k_stripes_add");
+  return 0;
 }
 
 int fos_jfa_copy(int res, const int32_t *src, int32_t *dst, void *stream) {
@@ -761,8 +1044,10 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
                   res <= ctx->max_res && num_passes >= 0 && num_passes <= 64 &&
                   L > 0.0f,
               "fos_jfa_passes: bad arguments (res=%d passes=%d)", res, num_passes);
+  FOS_REQUIRE(num_passes < FOS_PASS_SLOTS, "fos_jfa_passes: at most %d passes",
+              FOS_PASS_SLOTS - 1);
   hipStream_t st = (hipStream_t)stream;
-  int rc = fos_fill_i32(info, 2 + num_passes, 0, stream);
+  int rc = fos_fill_i32(ctx->stripes, (int64_t)num_passes * FOS_STRIPE_PASS_INTS, 0, stream);
   if (rc) return rc;
   int step = res / 2;
   for (int p = 0; p < num_passes; ++p) {
@@ -770,17 +1055,18 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
     const int32_t *in = (p & 1) ? temp : grid;
     int32_t *out = (p & 1) ? grid : temp;
     if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[p], st));
-    rc = launch_step(ctx, res, L, voxel, step, in, out, info + 2 + p, info, p,
-                     early_exit, th_pass_delta, min_passes, st);
+    rc = launch_step(ctx, res, L, voxel, step, in, out, p, early_exit, th_pass_delta,
+                     min_passes, st);
     if (rc) return rc;
     step /= 2;
   }
   if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[num_passes], st));
   const int64_t V = (int64_t)res * res * res;
   unsigned b = blocks_for(V);
-  if (b > 8192) b = 8192;
+  if (b > 2048) b = 2048;
   hipLaunchKernelGGL(k_jfa_finish, dim3(b), dim3(FOS_BLOCK), 0, st, res, num_passes,
-                     early_exit, th_pass_delta, min_passes, info, temp, grid);
+                     early_exit, th_pass_delta, min_passes, (const int *)ctx->stripes, info,
+                     temp, grid);
   FOS_CHECK_LAUNCH("k_jfa_finish");
   return 0;
 }
@@ -820,8 +1106,10 @@ int fos_events_elapsed(int count, void *const *events, float *ms) {
 int fos_majority(int res, const int32_t *grid, int32_t *filtered, void *stream) {
   FOS_REQUIRE(grid && filtered && grid != filtered && res >= 1,
               "fos_majority: bad arguments");
-  hipLaunchKernelGGL(k_majority, vox_grid(res), dim3(64, 4), 0, (hipStream_t)stream,
-                     res, grid, filtered);
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64, ntiles = res * nyb * nzb;
+  const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
+  hipLaunchKernelGGL(k_majority, dim3(per * FOS_XCDS), dim3(256), 0, (hipStream_t)stream,
+                     res, grid, filtered, nyb, nzb, ntiles, per);
   FOS_CHECK_LAUNCH("k_majority");
   return 0;
 }
@@ -835,10 +1123,12 @@ int fos_face_accumulate(fos_ctx *ctx, int res, int n, const int32_t *filtered,
               "fos_face_accumulate: res=%d exceeds the 32-bit face key", res);
   if (n == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64, ntiles = res * nyb * nzb;
+  const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
   while (true) {
-    hipLaunchKernelGGL(k_face_emit, vox_grid(res), dim3(64, 4), 0, st, res, n,
-                       filtered, ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap,
-                       ctx->flags);
+    hipLaunchKernelGGL(k_face_emit, dim3(per * FOS_XCDS), dim3(256), 0, st, res, n, filtered,
+                       ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap, ctx->flags, nyb, nzb,
+                       ntiles, per);
     FOS_CHECK_LAUNCH("k_face_emit");
     FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, st));
     FOS_HIP(hipStreamSynchronize(st));

@@ -102,11 +102,10 @@ class FoamSim:
 
     def pressure(self, frame):
         c = self.cfg
-        for _ in range(c.pressure_iters):
-            fl.call("fos_pressure_iter", self.n, frame, fl.ptr(self.rad), fl.ptr(self.delta_r),
-                    fl.ptr(self.face_ids), fl.ptr(self.fsc), c.pressure_k, c.pressure_exp,
-                    c.pressure_cap, c.pressure_max_nei, c.r_min, c.r_max,
-                    fl.ptr(self.pstats), fl.stream())
+        fl.call("fos_pressure_run", self.ctx.handle, self.n, frame, c.pressure_iters,
+                fl.ptr(self.rad), fl.ptr(self.delta_r), fl.ptr(self.face_ids), fl.ptr(self.fsc),
+                c.pressure_k, c.pressure_exp, c.pressure_cap, c.pressure_max_nei, c.r_min,
+                c.r_max, None, fl.stream())
 
     def run_jfa(self):
         """jfa.run_jfa without the host copies (jfa.py:1078-1132)."""

@@ -23,6 +23,8 @@ D = ctypes.c_double
 
 # name -> argtypes (every function returns int status)
 SIGNATURES = {
+    "fos_set_param": [ctypes.c_char_p, I],
+    "fos_get_param": [ctypes.c_char_p, ctypes.POINTER(I)],
     "fos_ctx_create": [ctypes.POINTER(P), I, I, I],
     "fos_ctx_destroy": [P],
     "fos_fill_i32": [P, I64, ctypes.c_int32, P],
@@ -55,6 +57,7 @@ SIGNATURES = {
     "fos_controller_stats": [P, P, P, P, I, I, I, P, P, P],
     "fos_nudge_radii": [P, P, P, I, F, F, F, F, F, I, F, P],
     "fos_pressure_iter": [I, I, P, P, P, P, F, F, F, I, F, F, P, P],
+    "fos_pressure_run": [P, I, I, I, P, P, P, P, F, F, F, I, F, F, P, P],
     "fos_pressure_stats": [P, P, I, P, P],
     "fos_brownian": [P, I, F, F, U64, I, I, P],
     "fos_ou_jitter": [P, P, P, I, F, F, F, F, F, F, U64, I, P],

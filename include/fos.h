@@ -35,6 +35,15 @@ typedef struct fos_ctx fos_ctx;
 int fos_abi_version(void);
 const char *fos_last_error(void);
 
+/* Launch tunables of the JFA pass kernels (process-wide), set by the tuner:
+ *   "jfa_lds_max_step" 0..8   steps up to this use the LDS-tiled kernel
+ *   "jfa_tile_x"       1,2,4  x-planes per LDS tile
+ *   "jfa_dedup"        1,4,8  evaluated-owner list length per voxel
+ * They change speed only, never results.  No reference counterpart (Taichi
+ * picks launch shapes itself). */
+int fos_set_param(const char *name, int value);
+int fos_get_param(const char *name, int *value);
+
 /* Context: scratch for up to max_n particles, res <= max_res voxels per axis,
  * max_cells hash cells.  Replaces the field allocation of jfa.init_jfa
  * (jfa.py:120-204), grid/dynamics fields (run.py:129-199). */
@@ -192,6 +201,16 @@ int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
                       const int32_t *face_ids, const int32_t *fsc, float k,
                       float p_exp, float pair_cap, int max_nei, float r_min,
                       float r_max, float *stats, void *stream);
+/* `iters` calls of equilibrate_pressure (dynamics.py:1454-1557) at one frame,
+ * each a Jacobi iteration, run as one kernel per iteration over a ping-pong
+ * radius buffer from the context; the result ends in rad, delta_r holds the
+ * last iteration's deltas and stats (device float[2], optional) its
+ * {max|dr|, changed count as int32 bits}.  The per-frame loop of run.py:755-771
+ * repeated `iters` times. */
+int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
+                     float *delta_r, const int32_t *face_ids, const int32_t *fsc,
+                     float k, float p_exp, float pair_cap, int max_nei, float r_min,
+                     float r_max, float *stats, void *stream);
 /* dynamics.py:1560-1589 compute_pressure_stats: host out[2] = {rmin, rmax}. */
 int fos_pressure_stats(fos_ctx *ctx, const float *rad, int n, float *out,
                        void *stream);

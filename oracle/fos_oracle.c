@@ -720,6 +720,16 @@ int32_t orc_hash(int32_t i, int32_t frame) {
  * fp32 result in all but ~2^-29 of cases); the product uses the same form. */
 static inline float pow_f32(float x, float y) { return (float)pow((double)x, (double)y); }
 
+/* r^p for volumes: for p == 3, (r*r)*r in double is the correctly rounded
+ * cube (r*r is exact), which is what a correctly rounded pow returns. */
+static inline float vol_f32(float r, float p) {
+  if (p == 3.0f) {
+    double d = (double)r;
+    return (float)(d * d * d);
+  }
+  return pow_f32(r, p);
+}
+
 float orc_pressure(int n, int frame, float *rad, float *delta_r,
                    const int32_t *face_ids, const int32_t *fsc, float k,
                    float p_exp, float pair_cap, int max_nei, float r_min,
@@ -728,7 +738,7 @@ float orc_pressure(int n, int frame, float *rad, float *delta_r,
   for (int i = 0; i < n; ++i) delta_r[i] = 0.0f;
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
-    float vi = pow_f32(rad[i], p_exp);
+    float vi = vol_f32(rad[i], p_exp);
     int fc = fsc[i];
     if (fc <= 0) continue;
     int an = fc < max_nei ? fc : max_nei;
@@ -738,7 +748,7 @@ float orc_pressure(int n, int frame, float *rad, float *delta_r,
       int idx = (start + kk) % fc;
       int j = face_ids[(int64_t)i * MAXN + idx];
       if (j < 0 || j >= n || j == i) continue;
-      float vj = pow_f32(rad[j], p_exp);
+      float vj = vol_f32(rad[j], p_exp);
       float dv_raw = k * (vi - vj);
       float vmin = fminf(vi, vj);
       float dv = fmaxf(-pair_cap * vmin, fminf(pair_cap * vmin, dv_raw));

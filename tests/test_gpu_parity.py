@@ -141,6 +141,20 @@ def test_jfa_early_exit_on_device(gpu, orc):
     assert (glab == lab).all()
 
 
+def test_jfa_early_exit_fires_on_last_pass(gpu, orc):
+    """Reference defaults (threshold 0.006, min 3 passes): the extra final
+    step-1 pass changes almost nothing, so the exit fires on the last pass;
+    all passes ran but early_exit is reported."""
+    n, res = 2000, 64
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+    voxel = fc.f32(1.0 / res)
+    lab, passes, early, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, 7, True, 0.006, 3)
+    assert early and passes == 7
+    glab, info = run_jfa_gpu(gpu, pos, rad, n, res, 7, True, 0.006, 3)
+    assert info[0] == 7 and info[1] == 1 and (info[2:9] == hist).all()
+    assert (glab == lab).all()
+
+
 def test_jfa_general_domain_size(gpu, orc):
     """L != 1 takes the division form of the minimum image."""
     n, res, L = 300, 32, 0.189
