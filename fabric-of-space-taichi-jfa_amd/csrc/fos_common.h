@@ -180,6 +180,7 @@ struct fos_ctx {
   int *in_start;          // [max_n+1]
   int *in_fill;           // [max_n]
   int *in_src;            // [max_n*32]
+  int *in_sorted;         // [max_n*32]
   int *acc_in;            // [max_n]
   // scan scratch
   int *scan_blocks;       // block partials

@@ -82,10 +82,38 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_add(int *__restrict__ out, int 
     if (base + k < n) out[base + k] += add;
 }
 
+// One block scans up to SCAN_BLOCK_MAX elements staged in LDS (coalesced
+// loads and stores; each thread scans a contiguous run).
+constexpr int SCAN_BLOCK_MAX = 12288;
+__global__ void __launch_bounds__(SCAN_T) k_scan_block(const int *__restrict__ in,
+                                                       int *__restrict__ out, int n) {
+  __shared__ int s_v[SCAN_BLOCK_MAX];
+  const int t = (int)threadIdx.x;
+  for (int k = t; k < n; k += SCAN_T) s_v[k] = in[k];
+  __syncthreads();
+  const int per = (n + SCAN_T - 1) / SCAN_T;
+  const int b = t * per, e = b + per < n ? b + per : n;
+  int s = 0;
+  for (int k = b; k < e; ++k) s += s_v[k];
+  int ex = block_exclusive_scan(s, nullptr);
+  for (int k = b; k < e; ++k) {
+    const int c = s_v[k];
+    s_v[k] = ex;
+    ex += c;
+  }
+  __syncthreads();
+  for (int k = t; k < n; k += SCAN_T) out[k] = s_v[k];
+}
+
 int exclusive_scan(fos_ctx *ctx, const int *in, int *out, int n,
                    hipStream_t st) {
   if (n <= 0) return 0;
   const int tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (n <= SCAN_BLOCK_MAX) {
+    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SCAN_T), 0, st, in, out, n);
+    FOS_CHECK_LAUNCH("scan block");
+    return 0;
+  }
   if (tiles == 1) {
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_T), 0, st, in, out, n,
                        (int *)nullptr);
@@ -171,6 +199,7 @@ int fos_ctx_destroy(fos_ctx *ctx) {
   void *ptrs[] = {ctx->seeds,    ctx->stripes,  ctx->tab_b,    ctx->tab_k,     ctx->tab_c,
                   ctx->flags,    ctx->score,    ctx->cls,       ctx->dec,
                   ctx->in_count, ctx->in_start, ctx->in_fill,   ctx->in_src,
+                  ctx->in_sorted,
                   ctx->acc_in,   ctx->scan_blocks, ctx->cell_write, ctx->cell_of,
                   ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch};
   for (void *p : ptrs)
@@ -244,6 +273,7 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
   ALLOC(c->in_start, ((size_t)max_n + 1) * sizeof(int));
   ALLOC(c->in_fill, (size_t)max_n * sizeof(int));
   ALLOC(c->in_src, nr * sizeof(int));
+  ALLOC(c->in_sorted, nr * sizeof(int));
   ALLOC(c->acc_in, (size_t)max_n * sizeof(int));
   int64_t biggest = max_cells > max_n + 1 ? max_cells : max_n + 1;
   c->scan_blocks_cap = (int)((biggest + SCAN_TILE - 1) / SCAN_TILE) + 1;

@@ -325,17 +325,51 @@ __global__ void k_pressure_delta(int n, int frame, const float *__restrict__ rad
   delta_r[i] = pressure_delta(i, n, frame, rad, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
 }
 
-// One whole Jacobi iteration (dynamics.py:1497-1557) from rin into rout;
-// stats (when given) gets max|dr| and the changed count.
-__global__ void k_pressure_step(int n, int frame, const float *__restrict__ rin,
-                                float *__restrict__ rout, float *__restrict__ delta_r,
-                                const int32_t *__restrict__ face_ids,
-                                const int32_t *__restrict__ fsc, float k, float p_exp,
-                                float inv_p, float cap, int max_nei, float r_min,
-                                float r_max, float *stats) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float d = pressure_delta(i, n, frame, rin, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
+// One whole Jacobi iteration (dynamics.py:1497-1557) from rin into rout with
+// 16 lanes per particle (16 particles per 256-thread block): lane q evaluates
+// the q-th budgeted neighbour of the rotation, lane 0 adds the terms in
+// rotation order (the serial order, bit for bit).  A budget above 16 falls
+// back to the serial loop.  stats (when given) gets max|dr| and the changed
+// count.
+__global__ void __launch_bounds__(256)
+    k_pressure_step(int n, int frame, const float *__restrict__ rin, float *__restrict__ rout,
+                    float *__restrict__ delta_r, const int32_t *__restrict__ face_ids,
+                    const int32_t *__restrict__ fsc, float k, float p_exp, float inv_p,
+                    float cap, int max_nei, float r_min, float r_max, float *stats) {
+  const int q = (int)(threadIdx.x & 15);
+  const int i = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+  const bool live = i < n;
+  const int fc = live ? fsc[i] : 0;
+  const int an = fc < max_nei ? fc : max_nei;
+  const float ri = live ? rin[i] : 0.0f;
+  float term = 0.0f;
+  bool has = false;
+  if (live && fc > 0 && an <= 16 && q < an) {
+    const float vi = vol_f32(ri, p_exp);
+    const int idx = (int)(((unsigned)lcg_hash(i, frame) % (unsigned)fc + (unsigned)q) %
+                          (unsigned)fc);
+    const int j = idx < FOS_MAX_NEIGHBORS ? face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + idx] : -1;
+    if (!(j < 0 || j >= n || j == i)) {
+      const float vj = vol_f32(rin[j], p_exp);
+      const float raw = k * (vi - vj);
+      const float vmin = fminf(vi, vj);
+      const float dv = fmaxf(-cap * vmin, fminf(cap * vmin, raw));
+      term = pow_f32(vi - dv, inv_p) - ri;
+      has = true;
+    }
+  }
+  // ordered sum over the 16-lane group
+  float acc = 0.0f;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const float t = __shfl(term, s, 16);
+    const int h = __shfl((int)has, s, 16);
+    if (h) acc = acc + t;
+  }
+  if (!live || q != 0) return;
+  float d = acc;
+  if (fc > 0 && an > 16)
+    d = pressure_delta(i, n, frame, rin, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
   delta_r[i] = d;
   if (stats) {
     const float a = fabsf(d);
@@ -673,7 +707,7 @@ int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
   if (n == 0 || iters == 0) return 0;
   float *a = rad, *b = ctx->rad_tmp;
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_pressure_step, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n, frame,
+    hipLaunchKernelGGL(k_pressure_step, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, st, n, frame,
                        a, b, delta_r, face_ids, fsc, k, p_exp, 1.0f / p_exp, pair_cap,
                        max_nei, r_min, r_max, it == iters - 1 ? stats : nullptr);
     FOS_CHECK_LAUNCH("k_pressure_step");

@@ -66,17 +66,22 @@ __global__ void k_copy_cells(const int32_t *__restrict__ s, int32_t *__restrict_
   if (c < C) d[c] = s[c];
 }
 
-// grid.py:229-261 in one block: exclusive scan of up to 64K cell counts into
-// cell_start and the scatter pointers (each thread scans a contiguous run).
+// grid.py:229-261 in one block: exclusive scan of up to SCAN_CELLS_MAX cell
+// counts into cell_start and the scatter pointers.  Counts are staged in LDS
+// with coalesced loads; each thread scans a contiguous run.
+#define SCAN_CELLS_MAX 12288
 __global__ void __launch_bounds__(1024)
     k_scan_cells(const int32_t *__restrict__ count, int32_t *__restrict__ start,
                  int32_t *__restrict__ wr, int C) {
   __shared__ int s_wave[16];
+  __shared__ int s_c[SCAN_CELLS_MAX];
   const int t = (int)threadIdx.x, lane = t & 63, wid = t >> 6;
+  for (int k = t; k < C; k += 1024) s_c[k] = count[k];
+  __syncthreads();
   const int per = (C + 1023) / 1024;
   const int b = t * per, e = b + per < C ? b + per : C;
   int s = 0;
-  for (int k = b; k < e; ++k) s += count[k];
+  for (int k = b; k < e; ++k) s += s_c[k];
   int x = s;
   for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(x, o, 64);
@@ -95,9 +100,15 @@ __global__ void __launch_bounds__(1024)
   __syncthreads();
   int ex = (wid ? s_wave[wid - 1] : 0) + x - s;
   for (int k = b; k < e; ++k) {
-    start[k] = ex;
-    wr[k] = ex;
-    ex += count[k];
+    const int c = s_c[k];
+    s_c[k] = ex;
+    ex += c;
+  }
+  __syncthreads();
+  for (int k = t; k < C; k += 1024) {
+    const int v = s_c[k];
+    start[k] = v;
+    wr[k] = v;
   }
 }
 
@@ -154,7 +165,7 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
   if (rc) return rc;
   rc = fos_grid_count(pos, n, G, L, cell_count, stream);
   if (rc) return rc;
-  if (C <= 65536) {
+  if (C <= SCAN_CELLS_MAX) {
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, cell_count, cell_start,
                        ctx->cell_write, C);
     FOS_CHECK_LAUNCH("k_scan_cells");

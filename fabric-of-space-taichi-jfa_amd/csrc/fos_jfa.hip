@@ -90,8 +90,15 @@ __device__ __forceinline__ bool block_pass_active(const int *stripes, int p, int
   return s_active != 0;
 }
 
-__device__ __forceinline__ int *pass_stripe(int *stripes, int pass) {
-  return stripes + (pass * FOS_STRIPES + (int)(blockIdx.x % FOS_STRIPES)) * FOS_STRIPE_INTS;
+// Adds a wave's changed-voxel count to pass `pass` (one atomic per wave, on
+// the stripe of its global wave id; no block barrier).
+__device__ __forceinline__ void wave_count(int changed, int *stripes, int pass) {
+  int c = changed;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) {
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    atomicAdd(stripes + (pass * FOS_STRIPES + wave % FOS_STRIPES) * FOS_STRIPE_INTS, c);
+  }
 }
 
 // The last K owners this voxel evaluated.  A candidate equal to the current
@@ -191,7 +198,7 @@ __global__ void __launch_bounds__(256)
       changed = (best != old);
     }
   }
-  block_add(changed, pass_stripe(stripes, pass));
+  wave_count(changed, stripes, pass);
 }
 
 // One pass for step <= halo with the labels of a TX x 4 x 64 tile plus a
@@ -261,7 +268,7 @@ __global__ void __launch_bounds__(256)
       }
     }
   }
-  block_add(changed, pass_stripe(stripes, pass));
+  wave_count(changed, stripes, pass);
 }
 
 // Ends the pass loop: records passes executed and the exit flag, and copies
@@ -475,6 +482,10 @@ __global__ void __launch_bounds__(256)
     for (int q = k + 1; q < 6; ++q)
       if (nb[q] == b) cnt += 1;
     const unsigned key = kbase + (unsigned)k;
+    // merge equal pairs; each distinct pair's leader lane keeps the totals
+    bool own = false;
+    int oc = 0;
+    unsigned okey = 0;
     unsigned long long pend = __ballot(has);
     while (pend) {
       const int leader = __ffsll((long long)pend) - 1;
@@ -482,10 +493,16 @@ __global__ void __launch_bounds__(256)
       const bool m = has && a == la && b == lb;
       const int c = wave_sum(m ? cnt : 0);
       const unsigned kk = wave_min_u(m ? key : 0xffffffffu);
-      if (lane == leader) tab_insert(tb, tk, tc, cap, la, lb, kk, c, overflow);
+      if (lane == leader) {
+        own = true;
+        oc = c;
+        okey = kk;
+      }
       if (m) has = false;
       pend = __ballot(has);
     }
+    // the leaders insert in parallel
+    if (own) tab_insert(tb, tk, tc, cap, a, b, okey, oc, overflow);
   }
 }
 
@@ -607,108 +624,104 @@ __global__ void k_fin_scatter(int n, const int32_t *__restrict__ face_ids,
   in_src[p] = (int)t;
 }
 
-// Sort each incoming list ascending (source row, then slot) so the
-// materialised neighbour rows list lower partners in row order.
-__global__ void k_fin_sort_in(int n, const int *__restrict__ in_start,
-                              int *__restrict__ in_src) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+// Places each incoming record at its rank within its row's list (ascending
+// source row, then slot) so the materialised neighbour rows list lower
+// partners in row order.  Lists are short; each record counts the smaller
+// ones in its segment.
+__global__ void k_fin_rank(int n, const int32_t *__restrict__ face_ids,
+                           const int *__restrict__ in_start, const int *__restrict__ in_src,
+                           int *__restrict__ in_sorted) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= in_start[n]) return;
+  const int src = in_src[p];
+  const int j = face_ids[src];
   const int b = in_start[j], e = in_start[j + 1];
-  for (int p = b + 1; p < e; ++p) {
-    const int key = in_src[p];
-    int q = p - 1;
-    while (q >= b && in_src[q] > key) {
-      in_src[q + 1] = in_src[q];
-      --q;
-    }
-    in_src[q + 1] = key;
-  }
+  int rank = 0;
+  for (int q = b; q < e; ++q) rank += in_src[q] < src ? 1 : 0;
+  in_sorted[b + rank] = src;
 }
 
-__global__ void k_fin_decide(int n, const unsigned char *__restrict__ cls,
-                             const int *__restrict__ acc_in,
-                             unsigned char *__restrict__ dec) {
+// One fixed-point round for row i: L_i = min(32, accepted appends from lower
+// rows under the previous round's decisions), then the row's decisions in
+// slot order (an ambiguous score is kept when its position was written this
+// frame -- s < used -- or last frame left it >= T_DROP).  Decisions are
+// packed as one bit per slot.
+__global__ void k_fin_round(int n, const unsigned char *__restrict__ cls,
+                            const int *__restrict__ in_start,
+                            const int *__restrict__ in_sorted,
+                            const unsigned *__restrict__ dec_prev,
+                            unsigned *__restrict__ dec_next, int *__restrict__ changed) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int used = acc_in[i] < FOS_MAX_NEIGHBORS ? acc_in[i] : FOS_MAX_NEIGHBORS;
-  const int64_t base = (int64_t)i * FOS_MAX_NEIGHBORS;
+  int acc = 0;
+  for (int p = in_start[i]; p < in_start[i + 1]; ++p) {
+    const int src = in_sorted[p];
+    acc += (dec_prev[src / FOS_MAX_NEIGHBORS] >> (src % FOS_MAX_NEIGHBORS)) & 1u;
+  }
+  int used = acc < FOS_MAX_NEIGHBORS ? acc : FOS_MAX_NEIGHBORS;
+  const uint4 *c4 = (const uint4 *)(cls + (int64_t)i * FOS_MAX_NEIGHBORS);
+  const uint4 lo = c4[0], hi = c4[1];
+  const unsigned words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  unsigned bits = 0;
+#pragma unroll
   for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
-    const unsigned char c = cls[base + s];
-    const int k = c & 3;
-    bool acc = false;
-    if (k == CLS_ACCEPT) {
-      acc = true;
-    } else if (k == CLS_AMBIG) {
-      acc = (s < used) || (c & CLS_OLD);
+    const unsigned c = (words[s >> 2] >> (8 * (s & 3))) & 0xffu;
+    const unsigned k = c & 3u;
+    const bool a = (k == CLS_ACCEPT) || (k == CLS_AMBIG && ((s < used) || (c & CLS_OLD)));
+    if (a) {
+      bits |= 1u << s;
+      used += used < FOS_MAX_NEIGHBORS ? 1 : 0;
     }
-    dec[base + s] = acc ? 1 : 0;
-    if (acc && used < FOS_MAX_NEIGHBORS) used += 1;
   }
+  dec_next[i] = bits;
+  if (bits != dec_prev[i]) atomicOr(changed, 1);
 }
 
-__global__ void k_fin_count(int n, const int *__restrict__ in_start,
-                            const int *__restrict__ in_src,
-                            const unsigned char *__restrict__ dec,
-                            int *__restrict__ acc_in, int *__restrict__ changed) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int a = 0;
-  for (int p = in_start[j]; p < in_start[j + 1]; ++p) a += dec[in_src[p]];
-  if (a != acc_in[j]) {
-    acc_in[j] = a;
-    atomicOr(changed, 1);
-  }
-}
-
+// Writes row i's final neighbour list, hysteresis scores and counts.  The
+// stores to prev follow the reference's chronology -- appends from lower
+// rows first, then row i's own slots in order (an accept appends at `used`,
+// a reject zeroes its own slot) -- and a later store to the same address
+// wins, so prev ends exactly as in the serial loop; untouched slots keep last
+// frame's value.
 __global__ void k_fin_materialize(int n, const int32_t *__restrict__ face_ids,
                                   const float *__restrict__ score,
                                   const unsigned char *__restrict__ cls,
-                                  const unsigned char *__restrict__ dec,
+                                  const unsigned *__restrict__ dec,
                                   const int *__restrict__ in_start,
-                                  const int *__restrict__ in_src,
-                                  float *__restrict__ prev,
-                                  int32_t *__restrict__ nl,
-                                  int32_t *__restrict__ nc,
-                                  int32_t *__restrict__ used_count,
+                                  const int *__restrict__ in_sorted,
+                                  float *__restrict__ prev, int32_t *__restrict__ nl,
+                                  int32_t *__restrict__ nc, int32_t *__restrict__ used_count,
                                   int32_t *__restrict__ fsc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t base = (int64_t)i * FOS_MAX_NEIGHBORS;
-  float pr[FOS_MAX_NEIGHBORS];
-  int row[FOS_MAX_NEIGHBORS];
-#pragma unroll
-  for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
-    pr[s] = prev[base + s];
-    row[s] = -1;
-  }
   int used = 0;
-  // appends from lower rows, in row order
   for (int p = in_start[i]; p < in_start[i + 1] && used < FOS_MAX_NEIGHBORS; ++p) {
-    const int src = in_src[p];
-    if (!dec[src]) continue;
-    row[used] = src / FOS_MAX_NEIGHBORS;
-    pr[used] = score[src];
+    const int src = in_sorted[p];
+    if (!((dec[src / FOS_MAX_NEIGHBORS] >> (src % FOS_MAX_NEIGHBORS)) & 1u)) continue;
+    nl[base + used] = src / FOS_MAX_NEIGHBORS;
+    prev[base + used] = score[src];
     used += 1;
   }
-  // own slots in order
+  const unsigned bits = dec[i];
+  const uint4 *c4 = (const uint4 *)(cls + base);
+  const uint4 lo = c4[0], hi = c4[1];
+  const unsigned words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
   for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
-    const unsigned char c = cls[base + s];
-    if ((c & 3) == CLS_INVALID) continue;
-    if (dec[base + s]) {
+    const unsigned c = (words[s >> 2] >> (8 * (s & 3))) & 0xffu;
+    if ((c & 3u) == CLS_INVALID) continue;
+    if ((bits >> s) & 1u) {
       if (used < FOS_MAX_NEIGHBORS) {
-        row[used] = face_ids[base + s];
-        pr[used] = score[base + s];
+        nl[base + used] = face_ids[base + s];
+        prev[base + used] = score[base + s];
         used += 1;
       }
     } else {
-      pr[s] = 0.0f;
+      prev[base + s] = 0.0f;
     }
   }
-#pragma unroll
-  for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
-    prev[base + s] = pr[s];
-    nl[base + s] = row[s];
-  }
+  for (int p = used; p < FOS_MAX_NEIGHBORS; ++p) nl[base + p] = -1;
   nc[i] = used;
   used_count[i] = used;
   fsc[i] = used;
@@ -1177,30 +1190,32 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
   hipLaunchKernelGGL(k_fin_scatter, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
                      face_ids, ctx->cls, ctx->in_start, ctx->in_fill, ctx->in_src);
   FOS_CHECK_LAUNCH("k_fin_scatter");
-  hipLaunchKernelGGL(k_fin_sort_in, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
-                     ctx->in_start, ctx->in_src);
-  FOS_CHECK_LAUNCH("k_fin_sort_in");
-  rc = fos_fill_i32(ctx->acc_in, n, 0, stream);
+  hipLaunchKernelGGL(k_fin_rank, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n, face_ids,
+                     ctx->in_start, ctx->in_src, ctx->in_sorted);
+  FOS_CHECK_LAUNCH("k_fin_rank");
+  // Fixed point from "nothing accepted": rounds alternate between two packed
+  // decision buffers; the host reads the change flag after every second
+  // round and stops once neither round of a pair changed anything.
+  unsigned *dec_a = (unsigned *)ctx->dec, *dec_b = dec_a + ctx->max_n;
+  rc = fos_fill_i32((int32_t *)dec_a, n, 0, stream);
   if (rc) return rc;
-  // Fixed point; every round is cheap, the host reads one flag per round.
-  for (int round = 0; round <= n + 1; ++round) {
+  for (int pair = 0; pair <= n + 1; ++pair) {
     rc = fos_fill_i32(ctx->flags + 1, 1, 0, stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fin_decide, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
-                       ctx->cls, ctx->acc_in, ctx->dec);
-    FOS_CHECK_LAUNCH("k_fin_decide");
-    hipLaunchKernelGGL(k_fin_count, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
-                       ctx->in_start, ctx->in_src, ctx->dec, ctx->acc_in, ctx->flags + 1);
-    FOS_CHECK_LAUNCH("k_fin_count");
+    hipLaunchKernelGGL(k_fin_round, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n, ctx->cls,
+                       ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1);
+    FOS_CHECK_LAUNCH("k_fin_round");
+    hipLaunchKernelGGL(k_fin_round, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n, ctx->cls,
+                       ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1);
+    FOS_CHECK_LAUNCH("k_fin_round");
     FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags + 1, sizeof(int),
                            hipMemcpyDeviceToHost, st));
     FOS_HIP(hipStreamSynchronize(st));
     if (!*(volatile int *)ctx->host_i) break;
   }
   hipLaunchKernelGGL(k_fin_materialize, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
-                     face_ids, ctx->score, ctx->cls, ctx->dec, ctx->in_start,
-                     ctx->in_src, prev_scores, neighbor_list, neighbor_count,
-                     used_count, fsc);
+                     face_ids, ctx->score, ctx->cls, dec_a, ctx->in_start, ctx->in_sorted,
+                     prev_scores, neighbor_list, neighbor_count, used_count, fsc);
   FOS_CHECK_LAUNCH("k_fin_materialize");
   return 0;
 }
