@@ -149,6 +149,85 @@ __device__ __forceinline__ int xcd_tile(int bid, int per_xcd) {
   return (bid % FOS_XCDS) * per_xcd + bid / FOS_XCDS;
 }
 
+// Candidate q = (dx+1)*9 + (dy+1)*3 + (dz+1) is skipped when its owner equals
+// the incumbent or an owner met earlier in the scan at the positions where
+// JFA label fields repeat most: the -1 neighbour along an axis where this
+// candidate sits at 0 or +1, and the mirror (-1 side) where it sits at +1.
+// An earlier equal owner has the same power distance, and under the strict
+// '<' a later duplicate never replaces it, so the scan result is unchanged.
+// The decision uses labels only, so the seed gathers of the candidates that
+// remain can be issued nine at a time (one dx-plane) before any is used.
+__device__ __forceinline__ unsigned need_mask(const int lab[27], int inc) {
+  unsigned m = 0;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int dx = q / 9, dy = (q / 3) % 3, dz = q % 3;
+    const int c = lab[q];
+    bool need = c >= 0 && c != inc;
+    if (dz >= 1) need &= c != lab[q - 1];
+    if (dz == 2) need &= c != lab[q - 2];
+    if (dy >= 1) need &= c != lab[q - 3];
+    if (dy == 2) need &= c != lab[q - 6];
+    if (dx >= 1) need &= c != lab[q - 9];
+    if (dx == 2) need &= c != lab[q - 18];
+    m |= need ? (1u << q) : 0u;
+  }
+  return m;
+}
+
+template <bool UNIT_L>
+__device__ __forceinline__ int scan_candidates(const int lab[27], int inc, float cx, float cy,
+                                               float cz, const float4 *__restrict__ seeds,
+                                               float L) {
+  int best = inc;
+  float bd = 1e10f;
+  if (inc >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[inc], L);
+  const unsigned need = need_mask(lab, inc);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float4 s[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const int q = a * 9 + r;
+      s[r] = (need >> q) & 1u ? seeds[lab[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const int q = a * 9 + r;
+      if ((need >> q) & 1u) {
+        const float pd = power_dist<UNIT_L>(cx, cy, cz, s[r], L);
+        if (pd < bd) {
+          bd = pd;
+          best = lab[q];
+        }
+      }
+    }
+  }
+  return best;
+}
+
+// Owner of a voxel after one pass over its 27 candidate labels (scan order
+// dx, dy, dz; incumbent lab[13] first; strict '<').  K == 0 skips by label
+// comparison and batches the gathers; K > 0 skips owners among the last K
+// evaluated and gathers one candidate at a time.
+template <bool UNIT_L, int K>
+__device__ __forceinline__ int scan_owner(const int lab[27], int inc, float cx, float cy,
+                                          float cz, const float4 *__restrict__ seeds,
+                                          float L) {
+  if constexpr (K == 0) {
+    return scan_candidates<UNIT_L>(lab, inc, cx, cy, cz, seeds, L);
+  } else {
+    int best = inc;
+    float bd = 1e10f;
+    if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
+    Seen<K> seen;
+    seen.init(best);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) consider<UNIT_L, K>(lab[q], best, bd, seen, cx, cy, cz, seeds, L);
+    return best;
+  }
+}
+
 // One pass, labels read straight from global memory (L2).  Block = 4 waves,
 // tile = 1 x-plane x 4 y-rows x 64 z (one wave per y-row).
 template <bool UNIT_L, int K>
@@ -187,13 +266,7 @@ __global__ void __launch_bounds__(256)
           for (int c = 0; c < 3; ++c) lab[(a * 3 + b) * 3 + c] = in[row + zs[c]];
         }
       const int old = lab[13];
-      int best = old;
-      float bd = 1e10f;
-      if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
-      Seen<K> seen;
-      seen.init(best);
-#pragma unroll
-      for (int q = 0; q < 27; ++q) consider<UNIT_L, K>(lab[q], best, bd, seen, cx, cy, cz, seeds, L);
+      const int best = scan_owner<UNIT_L, K>(lab, old, cx, cy, cz, seeds, L);
       out[v] = best;
       changed = (best != old);
     }
@@ -255,14 +328,7 @@ __global__ void __launch_bounds__(256)
               lab[(a * 3 + b) * 3 + c] =
                   s_lab[cbase + ((a - 1) * AY + (b - 1)) * s * AZ + (c - 1) * s];
         const int old = lab[13];
-        int best = old;
-        float bd = 1e10f;
-        if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
-        Seen<K> seen;
-        seen.init(best);
-#pragma unroll
-        for (int q = 0; q < 27; ++q)
-          consider<UNIT_L, K>(lab[q], best, bd, seen, cx, cy, cz, seeds, L);
+        const int best = scan_owner<UNIT_L, K>(lab, old, cx, cy, cz, seeds, L);
         out[(x * res + y) * res + z] = best;
         changed += (best != old);
       }
@@ -370,35 +436,28 @@ __global__ void __launch_bounds__(256)
     f[v] = -1;
     return;
   }
-  int nb[6];
-  face_neighbours(g, res, x, y, z, nb);
-  int ids[7], cnt[7], nu = 1;
-  ids[0] = my;
-  cnt[0] = 1;
+  int t[7];
+  t[0] = my;
+  face_neighbours(g, res, x, y, z, t + 1);
+  // Tally in first-seen order (centre, then the neighbours in order, -1
+  // skipped); a label's count is its number of occurrences among the seven
+  // (the centre counts once).  The first label with the strictly largest
+  // count wins.  Static indices only, so the tally stays in registers.
+  int maxc = 0, best = my;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int id = nb[k];
-    if (id >= 0) {
-      bool found = false;
-      for (int u = 0; u < nu; ++u)
-        if (ids[u] == id) {
-          cnt[u] += 1;
-          found = true;
-          break;
-        }
-      if (!found) {
-        ids[nu] = id;
-        cnt[nu] = 1;
-        nu += 1;
-      }
+  for (int k = 0; k < 7; ++k) {
+    const int id = t[k];
+    bool first = id >= 0;
+#pragma unroll
+    for (int j = 0; j < k; ++j) first &= (t[j] != id);
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) c += (t[j] == id) ? 1 : 0;
+    if (first && c > maxc) {
+      maxc = c;
+      best = id;
     }
   }
-  int maxc = 0, best = my;
-  for (int u = 0; u < nu; ++u)
-    if (cnt[u] > maxc) {
-      maxc = cnt[u];
-      best = ids[u];
-    }
   f[v] = best;
 }
 
@@ -470,6 +529,9 @@ __global__ void __launch_bounds__(256)
       a = -1;
     }
   }
+  bool own[6];
+  int oc[6];
+  unsigned okey[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const int b = nb[k];
@@ -483,9 +545,9 @@ __global__ void __launch_bounds__(256)
       if (nb[q] == b) cnt += 1;
     const unsigned key = kbase + (unsigned)k;
     // merge equal pairs; each distinct pair's leader lane keeps the totals
-    bool own = false;
-    int oc = 0;
-    unsigned okey = 0;
+    own[k] = false;
+    oc[k] = 0;
+    okey[k] = 0;
     unsigned long long pend = __ballot(has);
     while (pend) {
       const int leader = __ffsll((long long)pend) - 1;
@@ -494,15 +556,36 @@ __global__ void __launch_bounds__(256)
       const int c = wave_sum(m ? cnt : 0);
       const unsigned kk = wave_min_u(m ? key : 0xffffffffu);
       if (lane == leader) {
-        own = true;
-        oc = c;
-        okey = kk;
+        own[k] = true;
+        oc[k] = c;
+        okey[k] = kk;
       }
       if (m) has = false;
       pend = __ballot(has);
     }
-    // the leaders insert in parallel
-    if (own) tab_insert(tb, tk, tc, cap, a, b, okey, oc, overflow);
+  }
+  // Inserts: first probes of all owned pairs issued together, then resolved.
+  const int64_t base = (int64_t)(a < 0 ? 0 : a) * cap;
+  int probe[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    probe[k] = -2;
+    if (own[k]) {
+      const int s = (int)(((unsigned)nb[k] * 2654435761u) & (unsigned)(cap - 1));
+      probe[k] = tb[base + s];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (!own[k]) continue;
+    const int b = nb[k];
+    if (probe[k] == b) {
+      const int s = (int)(((unsigned)b * 2654435761u) & (unsigned)(cap - 1));
+      atomicMin(&tk[base + s], okey[k]);
+      atomicAdd(&tc[base + s], oc[k]);
+    } else {
+      tab_insert(tb, tk, tc, cap, a, b, okey[k], oc[k], overflow);
+    }
   }
 }
 
@@ -646,18 +729,26 @@ __global__ void k_fin_rank(int n, const int32_t *__restrict__ face_ids,
 // slot order (an ambiguous score is kept when its position was written this
 // frame -- s < used -- or last frame left it >= T_DROP).  Decisions are
 // packed as one bit per slot.
-__global__ void k_fin_round(int n, const unsigned char *__restrict__ cls,
-                            const int *__restrict__ in_start,
-                            const int *__restrict__ in_sorted,
-                            const unsigned *__restrict__ dec_prev,
-                            unsigned *__restrict__ dec_next, int *__restrict__ changed) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// 8 lanes per row gather the accepted-incoming count (an order-free sum);
+// lane 0 then decides the row.
+__global__ void __launch_bounds__(256)
+    k_fin_round(int n, const unsigned char *__restrict__ cls, const int *__restrict__ in_start,
+                const int *__restrict__ in_sorted, const unsigned *__restrict__ dec_prev,
+                unsigned *__restrict__ dec_next, int *__restrict__ changed) {
+  const int sub = (int)(threadIdx.x & 7);
+  const int i = (int)(blockIdx.x * 32 + (threadIdx.x >> 3));
   int acc = 0;
-  for (int p = in_start[i]; p < in_start[i + 1]; ++p) {
-    const int src = in_sorted[p];
-    acc += (dec_prev[src / FOS_MAX_NEIGHBORS] >> (src % FOS_MAX_NEIGHBORS)) & 1u;
+  if (i < n) {
+    const int e = in_start[i + 1];
+    for (int p = in_start[i] + sub; p < e; p += 8) {
+      const int src = in_sorted[p];
+      acc += (dec_prev[src / FOS_MAX_NEIGHBORS] >> (src % FOS_MAX_NEIGHBORS)) & 1u;
+    }
   }
+  acc += __shfl_xor(acc, 1, 8);
+  acc += __shfl_xor(acc, 2, 8);
+  acc += __shfl_xor(acc, 4, 8);
+  if (i >= n || sub != 0) return;
   int used = acc < FOS_MAX_NEIGHBORS ? acc : FOS_MAX_NEIGHBORS;
   const uint4 *c4 = (const uint4 *)(cls + (int64_t)i * FOS_MAX_NEIGHBORS);
   const uint4 lo = c4[0], hi = c4[1];
@@ -932,10 +1023,11 @@ int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
 
 // Pass-kernel tunables (fos_set_param): steps <= jfa_lds_max_step use the
 // LDS-tiled kernel with jfa_tile_x x-planes per block, larger steps the L2
-// kernel; jfa_dedup is the evaluated-owner list length.
+// kernel; jfa_dedup 0 skips candidates by label comparison and batches the
+// seed gathers, 1/4/8 keeps a list of that many evaluated owners.
 static int g_lds_max_step = 0;
 static int g_tile_x = 4;
-static int g_dedup = 4;
+static int g_dedup = 0;
 
 template <bool U, int K>
 static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float voxel,
@@ -975,7 +1067,9 @@ static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
 #define FOS_PASS(U, K)                                                                  \
   launch_pass_k<U, K>(ctx, res, L, half_l, voxel, step, in, out, pass, early, th, \
                       min_passes, st)
-  if (g_dedup <= 1) {
+  if (g_dedup == 0) {
+    if (u) FOS_PASS(true, 0); else FOS_PASS(false, 0);
+  } else if (g_dedup <= 1) {
     if (u) FOS_PASS(true, 1); else FOS_PASS(false, 1);
   } else if (g_dedup <= 4) {
     if (u) FOS_PASS(true, 4); else FOS_PASS(false, 4);
@@ -998,7 +1092,8 @@ int fos_set_param(const char *name, int value) {
     FOS_REQUIRE(value == 1 || value == 2 || value == 4, "jfa_tile_x must be 1, 2 or 4");
     g_tile_x = value;
   } else if (!strcmp(name, "jfa_dedup")) {
-    FOS_REQUIRE(value == 1 || value == 4 || value == 8, "jfa_dedup must be 1, 4 or 8");
+    FOS_REQUIRE(value == 0 || value == 1 || value == 4 || value == 8,
+                "jfa_dedup must be 0, 1, 4 or 8");
     g_dedup = value;
   } else {
     FOS_REQUIRE(false, "fos_set_param: unknown parameter '%s'", name);
@@ -1202,10 +1297,10 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
   for (int pair = 0; pair <= n + 1; ++pair) {
     rc = fos_fill_i32(ctx->flags + 1, 1, 0, stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fin_round, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n, ctx->cls,
+    hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n, ctx->cls,
                        ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1);
     FOS_CHECK_LAUNCH("k_fin_round");
-    hipLaunchKernelGGL(k_fin_round, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n, ctx->cls,
+    hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n, ctx->cls,
                        ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1);
     FOS_CHECK_LAUNCH("k_fin_round");
     FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags + 1, sizeof(int),

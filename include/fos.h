@@ -38,7 +38,9 @@ const char *fos_last_error(void);
 /* Launch tunables of the JFA pass kernels (process-wide), set by the tuner:
  *   "jfa_lds_max_step" 0..8   steps up to this use the LDS-tiled kernel
  *   "jfa_tile_x"       1,2,4  x-planes per LDS tile
- *   "jfa_dedup"        1,4,8  evaluated-owner list length per voxel
+ *   "jfa_dedup"        0,1,4,8  0: skip candidates by label comparison and
+ *                      batch the seed gathers (default); 1/4/8: skip owners
+ *                      among that many last evaluated
  * They change speed only, never results.  No reference counterpart (Taichi
  * picks launch shapes itself). */
 int fos_set_param(const char *name, int value);
