@@ -8,7 +8,11 @@ Inputs are resident in HBM before the timed region.
 The JSON line also carries:
   roofline      the JFA pass kernel: algorithmic 8 B per voxel-pass (read and
                 write one int32 label) times 128^3 voxels per launch, over the
-                average launch duration from HIP events on its stream
+                average launch duration from HIP events on its stream; traffic
+                is the HBM bytes per launch that rocprofv3 counted
+                (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter
+                reads for a plain copy of known size), read from the profile
+                summary under profiles/ when it is there
   cpu_baseline  the CPU restatement (C, OpenMP over the host cores) timed on
                 a bounded sample of the same frames
   cadence       the same workload with JFA on a 1/5 cadence after 20 frames
@@ -28,6 +32,37 @@ for _p in (ROOT, PKG):
 
 METRIC = "frames/s at N=10k,128³ JFA; Gvox-pass/s + %HBM; surrogate vs plain @FSC≤2%"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_jfa_pass_c2_hbm_traffic.md")
+COPY_CAL_RES = 400          # 400^3 int32 = 256,000,000 B read and written
+
+
+def measured_traffic():
+    """HBM bytes per pass launch from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """Three launches of a plain int32 copy of COPY_CAL_RES^3 elements, for
+    scaling the FETCH_SIZE / WRITE_SIZE counters (run under rocprofv3 --pmc)."""
+    import torch
+
+    import fos_lib as fl
+    n = COPY_CAL_RES ** 3
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        fl.call("fos_jfa_copy", COPY_CAL_RES, fl.ptr(src), fl.ptr(dst), fl.stream())
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+    del src, dst
+    torch.cuda.empty_cache()
 
 
 def parse():
@@ -35,9 +70,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-frames", type=int, default=3)
+    ap.add_argument("--cpu-frames", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cadence", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain copy of 400^3 int32 (counter scaling)")
     return ap.parse_args()
 
 
@@ -105,6 +142,8 @@ def main():
     import foam_config as fc
     import foam_inputs as fi
 
+    if args.copy_calibration:
+        copy_calibration()
     cfg = fc.preset("c2")
     pos, rad = fi.for_config(cfg)
     V = cfg.res ** 3
@@ -135,7 +174,7 @@ def main():
                    "frames": args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_jfa_step",
+                     "traffic": measured_traffic(), "kernel": "k_jfa_pass",
                      "avg_launch_us": round(avg_pass_s * 1e6, 3),
                      "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)]},
         "gvox_pass_per_s": round(gvox, 2),

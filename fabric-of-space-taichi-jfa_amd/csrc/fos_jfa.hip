@@ -337,6 +337,116 @@ __global__ void __launch_bounds__(256)
   wave_count(changed, stripes, pass);
 }
 
+// One pass with per-voxel candidate lists.  The pass is bound by vector ALU
+// issue: a wave pays for a candidate slot whenever any of its 64 voxels
+// evaluates it, and with 27 fixed slots nearly every slot has some voxel
+// that needs it.  Here each voxel writes the owners it must evaluate to its
+// own column of an LDS list (the incumbent first, then the skip-filtered
+// candidates in scan order), and the wave walks the lists in steps of
+// JFA_BATCH entries, so it runs max-over-lanes list length iterations instead
+// of 27.  Lists are padded to a multiple of JFA_BATCH with their last entry:
+// a repeated owner has the same distance and cannot win under '<'.
+//
+// Skip rule: an owner equal to the incumbent or to the mirror candidate on
+// the -1 side of an axis where this one sits at +1 (labels after a pass of
+// step s repeat with period 2s along each axis); NEIGH adds the -1 neighbour
+// along each axis, which pays once the field has converged (the repeated
+// step-1 pass).  Both compare only with earlier candidates.
+#define JFA_BATCH 2
+#define JFA_LIST (26 + JFA_BATCH)
+
+template <bool NEIGH>
+__device__ __forceinline__ bool jfa_skip(const int lab[27], int q, int inc) {
+  const int dx = q / 9, dy = (q / 3) % 3, dz = q % 3;
+  const int c = lab[q];
+  bool skip = c < 0 || c == inc;
+  if (dz == 2) skip |= c == lab[q - 2];
+  if (dy == 2) skip |= c == lab[q - 6];
+  if (dx == 2) skip |= c == lab[q - 18];
+  if (NEIGH) {
+    if (dz >= 1) skip |= c == lab[q - 1];
+    if (dy >= 1) skip |= c == lab[q - 3];
+    if (dx >= 1) skip |= c == lab[q - 9];
+  }
+  return skip;
+}
+
+template <bool UNIT_L, bool NEIGH>
+__global__ void __launch_bounds__(256)
+    k_jfa_pass_list(const float4 *__restrict__ seeds, int res, float L, float half_l,
+                    float voxel, int step, const int32_t *__restrict__ in,
+                    int32_t *__restrict__ out, int *__restrict__ stripes, int pass, int early,
+                    double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd) {
+  __shared__ int s_list[JFA_LIST][256];
+  if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
+  const int tile = xcd_tile(blockIdx.x, per_xcd);
+  const int t = (int)threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  int changed = 0;
+  if (tile < ntiles) {
+    const int zb = tile % nzb, t2 = tile / nzb;
+    const int yb = t2 % nyb, x = t2 / nyb;
+    const int z = zb * 64 + lane, y = yb * 4 + w;
+    if (z < res && y < res) {
+      const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
+                  cz = vcenter(z, voxel, half_l);
+      // x, y are wave-uniform: the nine row bases live in scalar registers
+      // and each label load adds one of three per-lane z offsets.
+      // Offsets are 32-bit byte counts so the loads take the scalar-base +
+      // 32-bit vector-offset form (res <= 1290 keeps a row under 2^32 B).
+      uint32_t zoff[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) zoff[c] = (uint32_t)wrap1(z + (c - 1) * step, res) << 2;
+      int lab[27];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const int xr = wrap1(x + (a - 1) * step, res), yr = wrap1(y + (b - 1) * step, res);
+          const char *row = (const char *)(in + ((size_t)xr * res + yr) * res);
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            lab[(a * 3 + b) * 3 + c] = *(const int32_t *)(row + zoff[c]);
+        }
+      const int inc = lab[13];
+      int cnt = 0;
+      if (inc >= 0) s_list[cnt++][t] = inc;
+#pragma unroll
+      for (int q = 0; q < 27; ++q) {
+        if (q == 13) continue;
+        if (!jfa_skip<NEIGH>(lab, q, inc)) s_list[cnt++][t] = lab[q];
+      }
+      int best = -1;
+      if (cnt > 0) {
+        const int last = s_list[cnt - 1][t];
+#pragma unroll
+        for (int i = 0; i < JFA_BATCH - 1; ++i) s_list[cnt + i][t] = last;
+        float bd = 1e10f;
+        for (int k = 0; k < cnt; k += JFA_BATCH) {
+          int c[JFA_BATCH];
+          float4 s[JFA_BATCH];
+#pragma unroll
+          for (int i = 0; i < JFA_BATCH; ++i) c[i] = s_list[k + i][t];
+#pragma unroll
+          for (int i = 0; i < JFA_BATCH; ++i)
+            s[i] = *(const float4 *)((const char *)seeds + ((uint32_t)c[i] << 4));
+#pragma unroll
+          for (int i = 0; i < JFA_BATCH; ++i) {
+            const float pd = power_dist<UNIT_L>(cx, cy, cz, s[i], L);
+            if (pd < bd) {
+              bd = pd;
+              best = c[i];
+            }
+          }
+        }
+      }
+      out[((size_t)x * res + y) * res + z] = best;
+      changed = (best != inc);
+    }
+  }
+  wave_count(changed, stripes, pass);
+}
+
 // Ends the pass loop: records passes executed and the exit flag, and copies
 // temp into grid when an odd number of passes ran (jfa.py:1115-1116).
 // The exit flag is set when the condition held after the last executed pass,
@@ -1024,10 +1134,13 @@ int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
 // Pass-kernel tunables (fos_set_param): steps <= jfa_lds_max_step use the
 // LDS-tiled kernel with jfa_tile_x x-planes per block, larger steps the L2
 // kernel; jfa_dedup 0 skips candidates by label comparison and batches the
-// seed gathers, 1/4/8 keeps a list of that many evaluated owners.
+// seed gathers, 1/4/8 keeps a list of that many evaluated owners.  These
+// apply with jfa_kernel 1 (fixed candidate slots); jfa_kernel 0 (default) is
+// the per-voxel candidate-list kernel.
 static int g_lds_max_step = 0;
 static int g_tile_x = 4;
 static int g_dedup = 0;
+static int g_kernel = 0;
 
 template <bool U, int K>
 static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float voxel,
@@ -1057,13 +1170,43 @@ static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float vo
   }
 }
 
+template <bool U, bool NEIGH>
+static void launch_pass_list(fos_ctx *ctx, int res, float L, float half_l, float voxel,
+                             int step, const int32_t *in, int32_t *out, int pass, int early,
+                             double th, int minp, hipStream_t st) {
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
+  const int ntiles = res * nyb * nzb;
+  const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
+  hipLaunchKernelGGL((k_jfa_pass_list<U, NEIGH>), dim3(per * FOS_XCDS), dim3(256), 0, st,
+                     ctx->seeds, res, L, half_l, voxel, step, in, out, ctx->stripes, pass,
+                     early, th, minp, nyb, nzb, ntiles, per);
+}
+
 // Launches pass `pass` (its changed-voxel count accumulates in stripe slot
-// `pass`; the caller zeroes the slot first).
+// `pass`; the caller zeroes the slot first).  `settled` marks a pass whose
+// step equals the previous pass's (the field is close to converged), where
+// the list kernel also skips owners repeated by -1 neighbours.
 static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
                        const int32_t *in, int32_t *out, int pass, int early, double th,
-                       int min_passes, hipStream_t st) {
+                       int min_passes, bool settled, hipStream_t st) {
   const float half_l = (float)(0.5 * (double)L);
   const bool u = unit_l(L);
+  if (g_kernel == 0) {
+    if (u && settled)
+      launch_pass_list<true, true>(ctx, res, L, half_l, voxel, step, in, out, pass, early,
+                                   th, min_passes, st);
+    else if (u)
+      launch_pass_list<true, false>(ctx, res, L, half_l, voxel, step, in, out, pass, early,
+                                    th, min_passes, st);
+    else if (settled)
+      launch_pass_list<false, true>(ctx, res, L, half_l, voxel, step, in, out, pass, early,
+                                    th, min_passes, st);
+    else
+      launch_pass_list<false, false>(ctx, res, L, half_l, voxel, step, in, out, pass, early,
+                                     th, min_passes, st);
+    FOS_CHECK_LAUNCH("k_jfa_pass_list");
+    return 0;
+  }
 #define FOS_PASS(U, K)                                                                  \
   launch_pass_k<U, K>(ctx, res, L, half_l, voxel, step, in, out, pass, early, th, \
                       min_passes, st)
@@ -1095,6 +1238,9 @@ int fos_set_param(const char *name, int value) {
     FOS_REQUIRE(value == 0 || value == 1 || value == 4 || value == 8,
                 "jfa_dedup must be 0, 1, 4 or 8");
     g_dedup = value;
+  } else if (!strcmp(name, "jfa_kernel")) {
+    FOS_REQUIRE(value == 0 || value == 1, "jfa_kernel must be 0 or 1");
+    g_kernel = value;
   } else {
     FOS_REQUIRE(false, "fos_set_param: unknown parameter '%s'", name);
   }
@@ -1109,6 +1255,8 @@ int fos_get_param(const char *name, int *value) {
     *value = g_tile_x;
   else if (!strcmp(name, "jfa_dedup"))
     *value = g_dedup;
+  else if (!strcmp(name, "jfa_kernel"))
+    *value = g_kernel;
   else
     FOS_REQUIRE(false, "fos_get_param: unknown parameter '%s'", name);
   return 0;
@@ -1125,7 +1273,8 @@ int fos_jfa_step(fos_ctx *ctx, int res, float L, float voxel, int step_size,
   int *sl = ctx->stripes + slot * FOS_STRIPE_PASS_INTS;
   int rc = fos_fill_i32(sl, FOS_STRIPE_PASS_INTS, 0, stream);
   if (rc) return rc;
-  rc = launch_step(ctx, res, L, voxel, step_size, in, out, slot, 0, 0.0, 0, st);
+  rc = launch_step(ctx, res, L, voxel, step_size, in, out, slot, 0, 0.0, 0, step_size == 1,
+                   st);
   if (rc || !changes) return rc;
   hipLaunchKernelGGL(k_stripes_add, dim3(1), dim3(64), 0, st, (const int *)ctx->stripes, slot,
                      changes);
@@ -1157,15 +1306,16 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
   hipStream_t st = (hipStream_t)stream;
   int rc = fos_fill_i32(ctx->stripes, (int64_t)num_passes * FOS_STRIPE_PASS_INTS, 0, stream);
   if (rc) return rc;
-  int step = res / 2;
+  int step = res / 2, prev = 0;
   for (int p = 0; p < num_passes; ++p) {
     if (step < 1) step = 1;
     const int32_t *in = (p & 1) ? temp : grid;
     int32_t *out = (p & 1) ? grid : temp;
     if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[p], st));
     rc = launch_step(ctx, res, L, voxel, step, in, out, p, early_exit, th_pass_delta,
-                     min_passes, st);
+                     min_passes, step == prev, st);
     if (rc) return rc;
+    prev = step;
     step /= 2;
   }
   if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[num_passes], st));

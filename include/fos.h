@@ -38,9 +38,12 @@ const char *fos_last_error(void);
 /* Launch tunables of the JFA pass kernels (process-wide), set by the tuner:
  *   "jfa_lds_max_step" 0..8   steps up to this use the LDS-tiled kernel
  *   "jfa_tile_x"       1,2,4  x-planes per LDS tile
- *   "jfa_dedup"        0,1,4,8  0: skip candidates by label comparison and
- *                      batch the seed gathers (default); 1/4/8: skip owners
- *                      among that many last evaluated
+ *   "jfa_kernel"       0,1    0: per-voxel candidate lists (default);
+ *                      1: fixed 27 candidate slots, tuned by jfa_dedup and
+ *                      the two LDS parameters above
+ *   "jfa_dedup"        0,1,4,8  (jfa_kernel 1) 0: skip candidates by label
+ *                      comparison and batch the seed gathers; 1/4/8: skip
+ *                      owners among that many last evaluated
  * They change speed only, never results.  No reference counterpart (Taichi
  * picks launch shapes itself). */
 int fos_set_param(const char *name, int value);

@@ -129,6 +129,45 @@ def test_jfa_single_steps_match(gpu, orc):
         cur = out
 
 
+PASS_VARIANTS = {
+    "list": {"jfa_kernel": 0},
+    "slots_batched": {"jfa_kernel": 1, "jfa_dedup": 0, "jfa_lds_max_step": 0},
+    "slots_seen1": {"jfa_kernel": 1, "jfa_dedup": 1, "jfa_lds_max_step": 0},
+    "slots_seen8": {"jfa_kernel": 1, "jfa_dedup": 8, "jfa_lds_max_step": 0},
+    "slots_lds_tx2": {"jfa_kernel": 1, "jfa_dedup": 4, "jfa_lds_max_step": 4, "jfa_tile_x": 2},
+}
+
+
+@pytest.mark.parametrize("variant", sorted(PASS_VARIANTS))
+def test_jfa_pass_kernel_variants(gpu, orc, variant):
+    """Every pass kernel and tunable gives the oracle's labels, including the
+    collision-heavy dense case and a res that is not a multiple of 64."""
+    import ctypes
+    saved = {}
+    for k in ("jfa_kernel", "jfa_dedup", "jfa_lds_max_step", "jfa_tile_x"):
+        v = ctypes.c_int()
+        gpu.call("fos_get_param", k.encode(), ctypes.byref(v))
+        saved[k] = v.value
+    try:
+        for k, v in PASS_VARIANTS[variant].items():
+            gpu.call("fos_set_param", k.encode(), v)
+        for (n, res, seed, dup) in ((1500, 72, 7, False), (3000, 16, 2, True)):
+            pos, rad = fi.uniform(n, 0.01, 0.03, seed=seed)
+            if dup:
+                pos[1::2] = pos[0::2]
+                rad[1::2] = rad[0::2]
+            voxel = fc.f32(1.0 / res)
+            P = int(math.ceil(math.log2(res))) + 1
+            print(f"[jfa variant] {variant} n={n} res={res} passes={P}", flush=True)
+            lab, passes, _, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, P)
+            glab, info = run_jfa_gpu(gpu, pos, rad, n, res, P)
+            assert (glab == lab).all()
+            assert (info[2:2 + passes] == hist).all()
+    finally:
+        for k, v in saved.items():
+            gpu.call("fos_set_param", k.encode(), v)
+
+
 def test_jfa_early_exit_on_device(gpu, orc):
     n, res = 2000, 64
     pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
