@@ -85,11 +85,16 @@ __device__ __forceinline__ float gwrap(float p, float L, float inv_l) {
   return p - L * floorf(p * inv_l + 0.5f);
 }
 
+// wrapi with the in-range case first (no integer division there).
+__device__ __forceinline__ int wrap_fast(int i, int r) {
+  return (i >= 0 && i < r) ? i : wrapi(i, r);
+}
+
 // grid.py:202-219 cell of a wrapped coordinate, with the 1e-7 hash epsilon.
 __device__ __forceinline__ int hash_axis(float pw, int G, float half_l,
                                          float inv_l) {
   float q = ((pw + half_l) + 1e-7f) * inv_l;
-  return wrapi((int)floorf(q * (float)G), G);
+  return wrap_fast((int)floorf(q * (float)G), G);
 }
 
 // dynamics.py:101-103 stencil centre: no epsilon, truncating cast, no wrap.

@@ -15,11 +15,11 @@ __device__ __forceinline__ void for_stencil(int cx, int cy, int cz, int G,
                                             const int32_t *__restrict__ cc,
                                             const int32_t *__restrict__ ci, F f) {
   for (int dx = -1; dx <= 1; ++dx) {
-    const int nx = wrapi(cx + dx, G);
+    const int nx = wrap_fast(cx + dx, G);
     for (int dy = -1; dy <= 1; ++dy) {
-      const int ny = wrapi(cy + dy, G);
+      const int ny = wrap_fast(cy + dy, G);
       for (int dz = -1; dz <= 1; ++dz) {
-        const int nz = wrapi(cz + dz, G);
+        const int nz = wrap_fast(cz + dz, G);
         const int id = nx * G * G + ny * G + nz;
         const int st = cs[id], ct = cc[id];
         for (int k = st; k < st + ct; ++k) f(ci[k]);
@@ -87,8 +87,8 @@ __global__ void __launch_bounds__(256)
     cz = centre_axis(pz, G, L, half_l, inv_l);
   }
   if (live && l < 27) {
-    const int id = wrapi(cx + l / 9 - 1, G) * G * G + wrapi(cy + (l / 3) % 3 - 1, G) * G +
-                   wrapi(cz + l % 3 - 1, G);
+    const int id = wrap_fast(cx + l / 9 - 1, G) * G * G + wrap_fast(cy + (l / 3) % 3 - 1, G) * G +
+                   wrap_fast(cz + l % 3 - 1, G);
     const int st = cs[id], ct = cc[id];
     int m = 0;
     for (int k = st; k < st + ct; ++k) {
@@ -160,8 +160,8 @@ __global__ void __launch_bounds__(256)
     const int cx = centre_axis(px, G, L, half_l, inv_l);
     const int cy = centre_axis(py, G, L, half_l, inv_l);
     const int cz = centre_axis(pz, G, L, half_l, inv_l);
-    const int id = wrapi(cx + l / 9 - 1, G) * G * G + wrapi(cy + (l / 3) % 3 - 1, G) * G +
-                   wrapi(cz + l % 3 - 1, G);
+    const int id = wrap_fast(cx + l / 9 - 1, G) * G * G + wrap_fast(cy + (l / 3) % 3 - 1, G) * G +
+                   wrap_fast(cz + l % 3 - 1, G);
     for (int k = cs[id]; k < cs[id] + cc[id]; ++k) {
       const int j = ci[k];
       if (!(i < j)) continue;
@@ -538,14 +538,14 @@ __global__ void k_topo(const float *__restrict__ pos, const float *__restrict__ 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
-  const int cx = wrapi(centre_axis(px, G, L, half_l, inv_l), G);
-  const int cy = wrapi(centre_axis(py, G, L, half_l, inv_l), G);
-  const int cz = wrapi(centre_axis(pz, G, L, half_l, inv_l), G);
+  const int cx = wrap_fast(centre_axis(px, G, L, half_l, inv_l), G);
+  const int cy = wrap_fast(centre_axis(py, G, L, half_l, inv_l), G);
+  const int cz = wrap_fast(centre_axis(pz, G, L, half_l, inv_l), G);
   int c = 0, tr = 0;
   for (int dx = -1; dx <= 1; ++dx)
     for (int dy = -1; dy <= 1; ++dy)
       for (int dz = -1; dz <= 1; ++dz) {
-        const int id = wrapi(cx + dx, G) * G * G + wrapi(cy + dy, G) * G + wrapi(cz + dz, G);
+        const int id = wrap_fast(cx + dx, G) * G * G + wrap_fast(cy + dy, G) * G + wrap_fast(cz + dz, G);
         for (int k = cs[id]; k < cs[id] + cc[id]; ++k) {
           const int j = ci[k];
           if (j <= i) continue;

@@ -40,7 +40,7 @@ __global__ void k_seed(const float *__restrict__ pos,
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     float nrm = (pos[3 * i + d] + half_l) / L;  // jfa.py:268-270
-    c[d] = wrapi((int)floorf(nrm * (float)res), res);
+    c[d] = wrap_fast((int)floorf(nrm * (float)res), res);
   }
   const int64_t v = ((int64_t)c[0] * res + c[1]) * res + c[2];
   const float cx = vcenter(c[0], voxel, half_l), cy = vcenter(c[1], voxel, half_l),
@@ -301,10 +301,10 @@ __global__ void __launch_bounds__(256)
     z0 = zb * 64;
     for (int r = w; r < AX * AY; r += 4) {
       const int xi = r / AY, yi = r - xi * AY;
-      const int gx = wrapi(x0 - s + xi, res), gy = wrapi(y0 - s + yi, res);
+      const int gx = wrap_fast(x0 - s + xi, res), gy = wrap_fast(y0 - s + yi, res);
       const int base = (gx * res + gy) * res;
       for (int zi = lane; zi < AZ; zi += 64)
-        s_lab[r * AZ + zi] = in[base + wrapi(z0 - s + zi, res)];
+        s_lab[r * AZ + zi] = in[base + wrap_fast(z0 - s + zi, res)];
     }
   }
   __syncthreads();
@@ -699,6 +699,100 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Workgroup-level merge: a tile of TX x-planes x 4 rows x 64 voxels merges
+// its pairs in an LDS hash keyed by (a, b) (count sum, key minimum) and then
+// inserts each distinct pair once into the row tables.  A pair that finds no
+// LDS slot within EMIT_PROBES probes goes straight to the row table, so the
+// LDS table can be smaller than the worst case.
+#define EMIT_HS 2048
+#define EMIT_PROBES 32
+#define EMIT_EMPTY 0xffffffffffffffffull
+
+__device__ __forceinline__ unsigned pair_hash(int a, int b) {
+  return ((unsigned)a * 0x9E3779B1u) ^ ((unsigned)b * 0x85EBCA77u);
+}
+
+template <int TX>
+__global__ void __launch_bounds__(256)
+    k_face_emit_lds(int res, int n, const int32_t *__restrict__ f, int *tb, unsigned *tk,
+                    int *tc, int cap, int *overflow, int nyb, int nzb, int ntiles, int per) {
+  __shared__ unsigned long long s_pair[EMIT_HS];
+  __shared__ unsigned s_key[EMIT_HS];
+  __shared__ int s_cnt[EMIT_HS];
+  const int t = (int)threadIdx.x;
+  for (int h = t; h < EMIT_HS; h += 256) {
+    s_pair[h] = EMIT_EMPTY;
+    s_key[h] = 0xffffffffu;
+    s_cnt[h] = 0;
+  }
+  __syncthreads();
+  const int tile = xcd_tile(blockIdx.x, per);
+  if (tile < ntiles) {
+    const int zb = tile % nzb, t2 = tile / nzb;
+    const int yb = t2 % nyb, x0 = (t2 / nyb) * TX;
+    const int z = zb * 64 + (t & 63), y = yb * 4 + (t >> 6);
+    for (int xt = 0; xt < TX; ++xt) {
+      const int x = x0 + xt;
+      if (x >= res || y >= res || z >= res) break;
+      const int v = (x * res + y) * res + z;
+      const int a = f[v];
+      if (a < 0 || a >= n) continue;
+      int nb[6];
+      face_neighbours(f, res, x, y, z, nb);
+      const unsigned kbase = 6u * (unsigned)v;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const int b = nb[k];
+        bool has = b >= 0 && b < n && a < b;
+        int cnt = 1;
+#pragma unroll
+        for (int q = 0; q < k; ++q)
+          if (nb[q] == b) has = false;
+#pragma unroll
+        for (int q = k + 1; q < 6; ++q)
+          if (nb[q] == b) cnt += 1;
+        if (!has) continue;
+        const unsigned key = kbase + (unsigned)k;
+        const unsigned long long pk = ((unsigned long long)(unsigned)a << 32) | (unsigned)b;
+        unsigned h = pair_hash(a, b) & (EMIT_HS - 1);
+        int slot = -1;
+        for (int p = 0; p < EMIT_PROBES; ++p) {
+          unsigned long long cur = s_pair[h];
+          if (cur == EMIT_EMPTY) {
+            cur = atomicCAS(&s_pair[h], EMIT_EMPTY, pk);
+            if (cur == EMIT_EMPTY) cur = pk;
+          }
+          if (cur == pk) {
+            slot = (int)h;
+            break;
+          }
+          h = (h + 1) & (EMIT_HS - 1);
+        }
+        if (slot >= 0) {
+          atomicMin(&s_key[slot], key);
+          atomicAdd(&s_cnt[slot], cnt);
+        } else {
+          tab_insert(tb, tk, tc, cap, a, b, key, cnt, overflow);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int h = t; h < EMIT_HS; h += 256) {
+    const unsigned long long pk = s_pair[h];
+    if (pk == EMIT_EMPTY) continue;
+    const int a = (int)(pk >> 32), b = (int)(pk & 0xffffffffu);
+    const int64_t base = (int64_t)a * cap;
+    const int s = (int)(((unsigned)b * 2654435761u) & (unsigned)(cap - 1));
+    if (tb[base + s] == b) {
+      atomicMin(&tk[base + s], s_key[h]);
+      atomicAdd(&tc[base + s], s_cnt[h]);
+    } else {
+      tab_insert(tb, tk, tc, cap, a, b, s_key[h], s_cnt[h], overflow);
+    }
+  }
+}
+
 // One wave per row: rank entries by key (keys are unique within a row), write
 // the first 32 into face_ids/face_counts, clear the rest, reset the table.
 __global__ void __launch_bounds__(256)
@@ -1043,8 +1137,8 @@ __global__ void k_dirty_mark(const float *__restrict__ pos,
   const float rd = fabsf(rad[i] - rad_prev[i]);
   int tc[3], tp[3];
   for (int d = 0; d < 3; ++d) {
-    tc[d] = wrapi((int)floorf(((pos[3 * i + d] + half_l) / L) * (float)res), res) / tile;
-    tp[d] = wrapi((int)floorf(((pos_prev[3 * i + d] + half_l) / L) * (float)res), res) / tile;
+    tc[d] = wrap_fast((int)floorf(((pos[3 * i + d] + half_l) / L) * (float)res), res) / tile;
+    tp[d] = wrap_fast((int)floorf(((pos_prev[3 * i + d] + half_l) / L) * (float)res), res) / tile;
   }
   const bool is_dirty = (pd > pos_thr * voxel) || (rd > rad_thr * voxel) ||
                         tc[0] != tp[0] || tc[1] != tp[1] || tc[2] != tp[2];
@@ -1053,8 +1147,8 @@ __global__ void k_dirty_mark(const float *__restrict__ pos,
   for (int a = -halo; a <= halo; ++a)
     for (int b = -halo; b <= halo; ++b)
       for (int c = -halo; c <= halo; ++c) {
-        const int t0 = wrapi(tc[0] + c, tpa), t1 = wrapi(tc[1] + b, tpa),
-                  t2 = wrapi(tc[2] + a, tpa);
+        const int t0 = wrap_fast(tc[0] + c, tpa), t1 = wrap_fast(tc[1] + b, tpa),
+                  t2 = wrap_fast(tc[2] + a, tpa);
         dirty[((int64_t)t0 * tpa + t1) * tpa + t2] = 1;
       }
 }
@@ -1141,6 +1235,8 @@ static int g_lds_max_step = 0;
 static int g_tile_x = 4;
 static int g_dedup = 0;
 static int g_kernel = 0;
+static int g_emit_tx = 2;
+extern int g_grid_block;  // fos_grid.hip
 
 template <bool U, int K>
 static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float voxel,
@@ -1241,6 +1337,13 @@ int fos_set_param(const char *name, int value) {
   } else if (!strcmp(name, "jfa_kernel")) {
     FOS_REQUIRE(value == 0 || value == 1, "jfa_kernel must be 0 or 1");
     g_kernel = value;
+  } else if (!strcmp(name, "face_emit_tx")) {
+    FOS_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4,
+                "face_emit_tx must be 0, 1, 2 or 4");
+    g_emit_tx = value;
+  } else if (!strcmp(name, "grid_block")) {
+    FOS_REQUIRE(value == 0 || value == 1, "grid_block must be 0 or 1");
+    g_grid_block = value;
   } else {
     FOS_REQUIRE(false, "fos_set_param: unknown parameter '%s'", name);
   }
@@ -1257,6 +1360,10 @@ int fos_get_param(const char *name, int *value) {
     *value = g_dedup;
   else if (!strcmp(name, "jfa_kernel"))
     *value = g_kernel;
+  else if (!strcmp(name, "grid_block"))
+    *value = g_grid_block;
+  else if (!strcmp(name, "face_emit_tx"))
+    *value = g_emit_tx;
   else
     FOS_REQUIRE(false, "fos_get_param: unknown parameter '%s'", name);
   return 0;
@@ -1383,10 +1490,21 @@ int fos_face_accumulate(fos_ctx *ctx, int res, int n, const int32_t *filtered,
   hipStream_t st = (hipStream_t)stream;
   const int nyb = (res + 3) / 4, nzb = (res + 63) / 64, ntiles = res * nyb * nzb;
   const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
+  const int tx = g_emit_tx;
+  const int ntiles_x = tx > 0 ? ((res + tx - 1) / tx) * nyb * nzb : ntiles;
+  const int per_x = (ntiles_x + FOS_XCDS - 1) / FOS_XCDS;
   while (true) {
-    hipLaunchKernelGGL(k_face_emit, dim3(per * FOS_XCDS), dim3(256), 0, st, res, n, filtered,
-                       ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap, ctx->flags, nyb, nzb,
-                       ntiles, per);
+    if (tx == 0) {
+      hipLaunchKernelGGL(k_face_emit, dim3(per * FOS_XCDS), dim3(256), 0, st, res, n,
+                         filtered, ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap,
+                         ctx->flags, nyb, nzb, ntiles, per);
+    } else {
+      auto kern = tx == 1 ? k_face_emit_lds<1> : tx == 2 ? k_face_emit_lds<2>
+                                                         : k_face_emit_lds<4>;
+      hipLaunchKernelGGL(kern, dim3(per_x * FOS_XCDS), dim3(256), 0, st, res, n, filtered,
+                         ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap, ctx->flags, nyb,
+                         nzb, ntiles_x, per_x);
+    }
     FOS_CHECK_LAUNCH("k_face_emit");
     FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, st));
     FOS_HIP(hipStreamSynchronize(st));

@@ -44,6 +44,11 @@ const char *fos_last_error(void);
  *   "jfa_dedup"        0,1,4,8  (jfa_kernel 1) 0: skip candidates by label
  *                      comparison and batch the seed gathers; 1/4/8: skip
  *                      owners among that many last evaluated
+ *   "face_emit_tx"     0,1,2,4  face-pair emission: 0 merges pairs within a
+ *                      wave; 1/2/4 merge within a workgroup tile of that
+ *                      many x-planes in LDS (default 2)
+ *   "grid_block"       0,1    1: hash-grid rebuilds with n <= 16256 and
+ *                      cells <= 16384 run as one workgroup (default)
  * They change speed only, never results.  No reference counterpart (Taichi
  * picks launch shapes itself). */
 int fos_set_param(const char *name, int value);

@@ -225,18 +225,23 @@ def gpu_faces(gpu, lab, res, n):
     return host(filt), host(ids), host(cts)
 
 
-def test_majority_and_faces_c1(gpu, orc):
+EMIT_TX = [0, 1, 2, 4]
+
+
+@pytest.mark.parametrize("tx", EMIT_TX)
+def test_majority_and_faces_c1(gpu, orc, tx):
     n, res = 2000, 64
     pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
     lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, fc.f32(1 / 64), 7)
     f = orc.majority(lab, res)
     ids, cts = orc.accumulate_faces(f, res, n)
-    gf, gids, gcts = gpu_faces(gpu, lab, res, n)
+    gf, gids, gcts = with_param(gpu, "face_emit_tx", tx, lambda: gpu_faces(gpu, lab, res, n))
     assert (gf == f).all()
     assert (gids == ids).all() and (gcts == cts).all()
 
 
-def test_faces_row_truncation_and_table_growth(gpu, orc):
+@pytest.mark.parametrize("tx", EMIT_TX)
+def test_faces_row_truncation_and_table_growth(gpu, orc, tx):
     """Row 0 meets 100 distinct higher partners (> 32 kept, > 64 table
     slots): the first 32 by scan order survive with full counts."""
     res, n = 112, 101
@@ -249,11 +254,37 @@ def test_faces_row_truncation_and_table_growth(gpu, orc):
     tl = dev(lab)
     gids = torch.empty(n, 32, dtype=torch.int32, device="cuda")
     gcts = torch.empty(n, 32, dtype=torch.int32, device="cuda")
-    g.call("fos_face_accumulate", g.h, res, n, g.ptr(tl), g.ptr(gids), g.ptr(gcts), gpu.stream())
-    assert (host(gids) == ids).all() and (host(gcts) == cts).all()
+
+    def run():
+        g.call("fos_face_accumulate", g.h, res, n, g.ptr(tl), g.ptr(gids), g.ptr(gcts),
+               gpu.stream())
+        return host(gids), host(gcts)
+    a, b = with_param(gpu, "face_emit_tx", tx, run)
+    assert (a == ids).all() and (b == cts).all()
     # the grown tables keep working on the next call
-    g.call("fos_face_accumulate", g.h, res, n, g.ptr(tl), g.ptr(gids), g.ptr(gcts), gpu.stream())
-    assert (host(gids) == ids).all() and (host(gcts) == cts).all()
+    a, b = with_param(gpu, "face_emit_tx", tx, run)
+    assert (a == ids).all() and (b == cts).all()
+
+
+@pytest.mark.parametrize("tx", EMIT_TX)
+def test_faces_random_labels(gpu, orc, tx):
+    """Every voxel labelled at random: up to six distinct pairs per voxel
+    overflow the workgroup merge table (pairs then go straight to the row
+    tables) and each row meets hundreds of partners (the row tables grow)."""
+    n, res = 1000, 32
+    lab = np.random.default_rng(11).integers(-1, n, res ** 3).astype(np.int32)
+    ids, cts = orc.accumulate_faces(lab, res, n)
+    g = G(gpu, n, res)
+    tl = dev(lab)
+    gids = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+    gcts = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+
+    def run():
+        g.call("fos_face_accumulate", g.h, res, n, g.ptr(tl), g.ptr(gids), g.ptr(gcts),
+               gpu.stream())
+        return host(gids), host(gcts)
+    a, b = with_param(gpu, "face_emit_tx", tx, run)
+    assert (a == ids).all() and (b == cts).all()
 
 
 def test_faces_exclude_ids_at_or_above_n(gpu, orc):
@@ -394,12 +425,29 @@ def gpu_grid(gpu, pos, n, G_):
     return tp, cc, cs, ci
 
 
-@pytest.mark.parametrize("n,G_,seed", [(2000, 13, 0), (10000, 21, 1), (5000, 3, 2)])
-def test_grid_build_bit_exact(gpu, orc, n, G_, seed):
+def with_param(gpu, name, value, fn):
+    import ctypes
+    old = ctypes.c_int()
+    gpu.call("fos_get_param", name.encode(), ctypes.byref(old))
+    gpu.call("fos_set_param", name.encode(), value)
+    try:
+        return fn()
+    finally:
+        gpu.call("fos_set_param", name.encode(), old.value)
+
+
+@pytest.mark.parametrize("block", [1, 0])
+@pytest.mark.parametrize("n,G_,seed", [(2000, 13, 0), (10000, 21, 1), (5000, 3, 2), (1, 3, 6),
+                                       (16256, 25, 7), (16257, 25, 8)])
+def test_grid_build_bit_exact(gpu, orc, n, G_, seed, block):
+    """One-workgroup rebuild (block=1, up to 16256 particles and 16384 cells)
+    and the multi-kernel path give the oracle's grid, wrapped positions and
+    in-cell ascending order, at and just past the one-workgroup limits."""
     pos, _ = fi.uniform(n, 0.01, 0.02, seed=seed)
     pos[::5] += F(0.75)                       # unwrapped inputs
+    print(f"[grid] n={n} G={G_} block={block}", flush=True)
     p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
-    tp, tcc, tcs, tci = gpu_grid(gpu, pos, n, G_)
+    tp, tcc, tcs, tci = with_param(gpu, "grid_block", block, lambda: gpu_grid(gpu, pos, n, G_))
     assert np.array_equal(host(tp), p)
     assert np.array_equal(host(tcc), cc) and np.array_equal(host(tcs), cs)
     assert np.array_equal(host(tci), ci)
