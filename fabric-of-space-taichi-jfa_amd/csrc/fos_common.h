@@ -169,6 +169,8 @@ struct fos_ctx {
   int max_cells;
   // seed table (x, y, z, (beta*r)^2)
   float4 *seeds;
+  // per-wave change masks of the last two JFA passes, [max_res^2][ceil(max_res/64)]
+  unsigned long long *chg[2];
   // striped per-pass changed-voxel counters [FOS_PASS_SLOTS][FOS_STRIPES][16]
   int *stripes;
   // face-pair row tables: capacity tab_cap entries per particle row

@@ -201,7 +201,8 @@ int fos_ctx_destroy(fos_ctx *ctx) {
                   ctx->in_count, ctx->in_start, ctx->in_fill,   ctx->in_src,
                   ctx->in_sorted,
                   ctx->acc_in,   ctx->scan_blocks, ctx->cell_write, ctx->cell_of,
-                  ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch};
+                  ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch,
+                  ctx->chg[0],   ctx->chg[1]};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->host_i) (void)hipHostFree(ctx->host_i);
@@ -264,6 +265,11 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
     }                                                      \
   } while (0)
   ALLOC(c->seeds, (size_t)max_n * sizeof(float4));
+  {
+    const size_t words = (size_t)max_res * max_res * ((max_res + 63) / 64);
+    ALLOC(c->chg[0], words * sizeof(unsigned long long));
+    ALLOC(c->chg[1], words * sizeof(unsigned long long));
+  }
   ALLOC(c->stripes, (size_t)FOS_PASS_SLOTS * FOS_STRIPE_PASS_INTS * sizeof(int));
   ALLOC(c->flags, 64 * sizeof(int));
   ALLOC(c->score, nr * sizeof(float));

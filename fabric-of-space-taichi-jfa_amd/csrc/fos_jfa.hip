@@ -371,29 +371,77 @@ __device__ __forceinline__ bool jfa_skip(const int lab[27], int q, int inc) {
   return skip;
 }
 
-template <bool UNIT_L, bool NEIGH>
+// Pass flavours of the list kernel.
+#define JFA_PLAIN 0    // mirror skip rule
+#define JFA_SETTLED 1  // also skip repeats of the -1 neighbours
+#define JFA_INCR 2     // settled, and recompute only next to last pass's changes
+#define JFA_HALF 3     // 2*step == res: the -step and +step candidates coincide
+
+// Per-wave change masks: bit l of word (x*res + y)*nzb + zb is set when voxel
+// (x, y, 64*zb + l) changed in the pass.  In a pass whose step equals the
+// previous pass's, a voxel whose 3x3x3 neighbourhood did not change in the
+// previous pass sees the same 27 labels that pass saw, so its result is its
+// current label; only voxels next to a change are recomputed.
+__device__ __forceinline__ unsigned long long incr_wave_mask(
+    const unsigned long long *__restrict__ chg, int res, int nzb, int x, int y, int zb) {
+  const int zl = zb > 0 ? zb - 1 : nzb - 1;
+  const int posl = zb > 0 ? 63 : (res - 1) & 63;
+  const int last = min(63, res - 1 - 64 * zb);
+  const int zr = (64 * zb + last + 1 < res) ? zb + 1 : 0;
+  unsigned long long w = 0;
+#pragma unroll
+  for (int a = -1; a <= 1; ++a)
+#pragma unroll
+    for (int b = -1; b <= 1; ++b) {
+      const size_t base = ((size_t)wrap1(x + a, res) * res + wrap1(y + b, res)) * nzb;
+      const unsigned long long m = chg[base + zb];
+      const unsigned long long l = (chg[base + zl] >> posl) & 1ull;
+      const unsigned long long r = chg[base + zr] & 1ull;
+      w |= m | (m << 1) | (m >> 1) | l | (r << last);
+    }
+  return w;
+}
+
+template <bool UNIT_L, int MODE>
 __global__ void __launch_bounds__(256)
     k_jfa_pass_list(const float4 *__restrict__ seeds, int res, float L, float half_l,
                     float voxel, int step, const int32_t *__restrict__ in,
                     int32_t *__restrict__ out, int *__restrict__ stripes, int pass, int early,
-                    double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd) {
+                    double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd,
+                    const unsigned long long *__restrict__ chg_prev,
+                    unsigned long long *__restrict__ chg_out) {
   __shared__ int s_list[JFA_LIST][256];
   if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
   const int tile = xcd_tile(blockIdx.x, per_xcd);
   const int t = (int)threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   int changed = 0;
+  bool row_live = false;
+  int x = 0, y = 0, zb = 0;
   if (tile < ntiles) {
-    const int zb = tile % nzb, t2 = tile / nzb;
-    const int yb = t2 % nyb, x = t2 / nyb;
-    const int z = zb * 64 + lane, y = yb * 4 + w;
-    if (z < res && y < res) {
+    zb = tile % nzb;
+    const int t2 = tile / nzb;
+    const int yb = t2 % nyb;
+    x = t2 / nyb;
+    y = yb * 4 + w;
+    row_live = y < res;
+    const int z = zb * 64 + lane;
+    bool work = row_live && z < res;
+    const size_t v = ((size_t)x * res + y) * res + z;
+    if (MODE == JFA_INCR && row_live) {
+      const unsigned long long m = incr_wave_mask(chg_prev, res, nzb, x, y, zb);
+      if (work && !((m >> lane) & 1ull)) {
+        out[v] = in[v];
+        work = false;
+      }
+    }
+    if (work) {
       const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
                   cz = vcenter(z, voxel, half_l);
       // x, y are wave-uniform: the nine row bases live in scalar registers
-      // and each label load adds one of three per-lane z offsets.
-      // Offsets are 32-bit byte counts so the loads take the scalar-base +
-      // 32-bit vector-offset form (res <= 1290 keeps a row under 2^32 B).
+      // and each label load adds one of three per-lane 32-bit z offsets
+      // (scalar base + vector offset addressing; res <= 1290 keeps a row
+      // under 2^32 B).
       uint32_t zoff[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) zoff[c] = (uint32_t)wrap1(z + (c - 1) * step, res) << 2;
@@ -405,34 +453,46 @@ __global__ void __launch_bounds__(256)
           const int xr = wrap1(x + (a - 1) * step, res), yr = wrap1(y + (b - 1) * step, res);
           const char *row = (const char *)(in + ((size_t)xr * res + yr) * res);
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            lab[(a * 3 + b) * 3 + c] = *(const int32_t *)(row + zoff[c]);
+          for (int c = 0; c < 3; ++c) {
+            const int q = (a * 3 + b) * 3 + c;
+            if (MODE == JFA_HALF && a == 2)
+              lab[q] = lab[q - 18];
+            else if (MODE == JFA_HALF && b == 2)
+              lab[q] = lab[q - 6];
+            else if (MODE == JFA_HALF && c == 2)
+              lab[q] = lab[q - 2];
+            else
+              lab[q] = *(const int32_t *)(row + zoff[c]);
+          }
         }
       const int inc = lab[13];
-      int cnt = 0;
-      if (inc >= 0) s_list[cnt++][t] = inc;
+      // Branch-free list build: every candidate is written at the current
+      // end and the end advances only for kept owners.
+      s_list[0][t] = inc;
+      int cnt = inc >= 0 ? 1 : 0;
 #pragma unroll
       for (int q = 0; q < 27; ++q) {
         if (q == 13) continue;
-        if (!jfa_skip<NEIGH>(lab, q, inc)) s_list[cnt++][t] = lab[q];
+        s_list[cnt][t] = lab[q];
+        cnt += jfa_skip<MODE != JFA_PLAIN && MODE != JFA_HALF>(lab, q, inc) ? 0 : 1;
       }
       int best = -1;
       if (cnt > 0) {
-        const int last = s_list[cnt - 1][t];
+        const int first = s_list[0][t];
 #pragma unroll
-        for (int i = 0; i < JFA_BATCH - 1; ++i) s_list[cnt + i][t] = last;
+        for (int i = 0; i < JFA_BATCH - 1; ++i) s_list[cnt + i][t] = first;
         float bd = 1e10f;
         for (int k = 0; k < cnt; k += JFA_BATCH) {
           int c[JFA_BATCH];
-          float4 s[JFA_BATCH];
+          float4 sd[JFA_BATCH];
 #pragma unroll
           for (int i = 0; i < JFA_BATCH; ++i) c[i] = s_list[k + i][t];
 #pragma unroll
           for (int i = 0; i < JFA_BATCH; ++i)
-            s[i] = *(const float4 *)((const char *)seeds + ((uint32_t)c[i] << 4));
+            sd[i] = *(const float4 *)((const char *)seeds + ((uint32_t)c[i] << 4));
 #pragma unroll
           for (int i = 0; i < JFA_BATCH; ++i) {
-            const float pd = power_dist<UNIT_L>(cx, cy, cz, s[i], L);
+            const float pd = power_dist<UNIT_L>(cx, cy, cz, sd[i], L);
             if (pd < bd) {
               bd = pd;
               best = c[i];
@@ -440,11 +500,17 @@ __global__ void __launch_bounds__(256)
           }
         }
       }
-      out[((size_t)x * res + y) * res + z] = best;
+      out[v] = best;
       changed = (best != inc);
     }
   }
-  wave_count(changed, stripes, pass);
+  const unsigned long long bal = __ballot(changed);
+  if (chg_out && row_live && lane == 0) chg_out[((size_t)x * res + y) * nzb + zb] = bal;
+  if (lane == 0 && bal) {
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    atomicAdd(stripes + (pass * FOS_STRIPES + wave % FOS_STRIPES) * FOS_STRIPE_INTS,
+              __popcll(bal));
+  }
 }
 
 // Ends the pass loop: records passes executed and the exit flag, and copies
@@ -1266,40 +1332,50 @@ static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float vo
   }
 }
 
-template <bool U, bool NEIGH>
+template <bool U, int MODE>
 static void launch_pass_list(fos_ctx *ctx, int res, float L, float half_l, float voxel,
                              int step, const int32_t *in, int32_t *out, int pass, int early,
-                             double th, int minp, hipStream_t st) {
+                             double th, int minp, const unsigned long long *chg_prev,
+                             unsigned long long *chg_out, hipStream_t st) {
   const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
   const int ntiles = res * nyb * nzb;
   const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
-  hipLaunchKernelGGL((k_jfa_pass_list<U, NEIGH>), dim3(per * FOS_XCDS), dim3(256), 0, st,
+  hipLaunchKernelGGL((k_jfa_pass_list<U, MODE>), dim3(per * FOS_XCDS), dim3(256), 0, st,
                      ctx->seeds, res, L, half_l, voxel, step, in, out, ctx->stripes, pass,
-                     early, th, minp, nyb, nzb, ntiles, per);
+                     early, th, minp, nyb, nzb, ntiles, per, chg_prev, chg_out);
 }
 
 // Launches pass `pass` (its changed-voxel count accumulates in stripe slot
 // `pass`; the caller zeroes the slot first).  `settled` marks a pass whose
-// step equals the previous pass's (the field is close to converged), where
-// the list kernel also skips owners repeated by -1 neighbours.
+// step equals the previous pass's (the field is close to converged): the
+// list kernel then also skips owners repeated by -1 neighbours, and with the
+// previous pass's change masks (chg_prev) recomputes only voxels next to a
+// change.  chg_out (may be NULL) receives this pass's change masks.
 static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
                        const int32_t *in, int32_t *out, int pass, int early, double th,
-                       int min_passes, bool settled, hipStream_t st) {
+                       int min_passes, bool settled, const unsigned long long *chg_prev,
+                       unsigned long long *chg_out, hipStream_t st) {
   const float half_l = (float)(0.5 * (double)L);
   const bool u = unit_l(L);
   if (g_kernel == 0) {
-    if (u && settled)
-      launch_pass_list<true, true>(ctx, res, L, half_l, voxel, step, in, out, pass, early,
-                                   th, min_passes, st);
-    else if (u)
-      launch_pass_list<true, false>(ctx, res, L, half_l, voxel, step, in, out, pass, early,
-                                    th, min_passes, st);
-    else if (settled)
-      launch_pass_list<false, true>(ctx, res, L, half_l, voxel, step, in, out, pass, early,
-                                    th, min_passes, st);
-    else
-      launch_pass_list<false, false>(ctx, res, L, half_l, voxel, step, in, out, pass, early,
-                                     th, min_passes, st);
+    const int mode = 2 * step == res ? JFA_HALF
+                     : settled       ? (chg_prev ? JFA_INCR : JFA_SETTLED)
+                                     : JFA_PLAIN;
+#define FOS_LIST(U, M)                                                                   \
+  launch_pass_list<U, M>(ctx, res, L, half_l, voxel, step, in, out, pass, early, th,    \
+                         min_passes, chg_prev, chg_out, st)
+    if (u) {
+      if (mode == JFA_HALF) FOS_LIST(true, JFA_HALF);
+      else if (mode == JFA_INCR) FOS_LIST(true, JFA_INCR);
+      else if (mode == JFA_SETTLED) FOS_LIST(true, JFA_SETTLED);
+      else FOS_LIST(true, JFA_PLAIN);
+    } else {
+      if (mode == JFA_HALF) FOS_LIST(false, JFA_HALF);
+      else if (mode == JFA_INCR) FOS_LIST(false, JFA_INCR);
+      else if (mode == JFA_SETTLED) FOS_LIST(false, JFA_SETTLED);
+      else FOS_LIST(false, JFA_PLAIN);
+    }
+#undef FOS_LIST
     FOS_CHECK_LAUNCH("k_jfa_pass_list");
     return 0;
   }
@@ -1381,7 +1457,7 @@ int fos_jfa_step(fos_ctx *ctx, int res, float L, float voxel, int step_size,
   int rc = fos_fill_i32(sl, FOS_STRIPE_PASS_INTS, 0, stream);
   if (rc) return rc;
   rc = launch_step(ctx, res, L, voxel, step_size, in, out, slot, 0, 0.0, 0, step_size == 1,
-                   st);
+                   nullptr, nullptr, st);
   if (rc || !changes) return rc;
   hipLaunchKernelGGL(k_stripes_add, dim3(1), dim3(64), 0, st, (const int *)ctx->stripes, slot,
                      changes);
@@ -1413,15 +1489,23 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
   hipStream_t st = (hipStream_t)stream;
   int rc = fos_fill_i32(ctx->stripes, (int64_t)num_passes * FOS_STRIPE_PASS_INTS, 0, stream);
   if (rc) return rc;
-  int step = res / 2, prev = 0;
+  // A pass is settled when its step equals the previous pass's; it recomputes
+  // only next to the previous pass's changes when that pass was settled too
+  // (after the first repeat the changes are few; the first step-1 pass after
+  // step 2 still changes most voxels).
+  int step = res / 2, prev = 0, prev2 = 0;
   for (int p = 0; p < num_passes; ++p) {
     if (step < 1) step = 1;
     const int32_t *in = (p & 1) ? temp : grid;
     int32_t *out = (p & 1) ? grid : temp;
     if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[p], st));
+    unsigned long long *chg_out = ctx->chg[p & 1];
+    const unsigned long long *chg_prev =
+        (step == prev && prev == prev2) ? ctx->chg[(p + 1) & 1] : nullptr;
     rc = launch_step(ctx, res, L, voxel, step, in, out, p, early_exit, th_pass_delta,
-                     min_passes, step == prev, st);
+                     min_passes, step == prev, chg_prev, chg_out, st);
     if (rc) return rc;
+    prev2 = prev;
     prev = step;
     step /= 2;
   }

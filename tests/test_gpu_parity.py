@@ -168,6 +168,22 @@ def test_jfa_pass_kernel_variants(gpu, orc, variant):
             gpu.call("fos_set_param", k.encode(), v)
 
 
+@pytest.mark.parametrize("res,extra", [(66, 3), (130, 2), (64, 4)])
+def test_jfa_repeated_final_passes(gpu, orc, res, extra):
+    """Several step-1 passes in a row: from the third on, a pass recomputes
+    only voxels next to the previous pass's changes.  res 66 and 130 leave a
+    partial last 64-voxel segment per row (z wrap across segments)."""
+    n = 900
+    pos, rad = fi.uniform(n, 0.01, 0.03, seed=res)
+    voxel = fc.f32(1.0 / res)
+    P = int(math.ceil(math.log2(res))) + 1 + extra
+    print(f"[jfa repeated] res={res} passes={P}", flush=True)
+    lab, passes, _, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, P)
+    glab, info = run_jfa_gpu(gpu, pos, rad, n, res, P)
+    assert (glab == lab).all()
+    assert info[0] == passes and (info[2:2 + passes] == hist).all()
+
+
 def test_jfa_early_exit_on_device(gpu, orc):
     n, res = 2000, 64
     pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)

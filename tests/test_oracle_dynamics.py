@@ -227,3 +227,91 @@ def test_topology_candidates_cap_and_order(orc):
         want = [j for j in ci if j > i and d[i, j] * d[i, j] <= th * th]
         assert list(pairs[i, :cnt[i]]) == want[:28]
         assert tr[i] == (len(want) > 28)
+
+
+# ---- pins for the controller stats, Gaussian noise, OU jitter and XSPH ------------
+
+def test_controller_stats_known_answers(orc):
+    """Band [12, 16]: 11 grows, 16 and 12 are inside, 17 shrinks; dr known."""
+    ema = np.array([11.0, 12.0, 16.0, 17.0, 3.0], F)
+    rad = np.array([0.010, 0.020, 0.030, 0.040, 0.050], F)
+    rt = np.array([0.011, 0.020, 0.027, 0.040, 0.058], F)
+    oi, of = orc.controller_stats(rad, rt, ema, 5, 12, 16)
+    assert list(oi) == [2, 1, 2]
+    dr = [F(0.011) - F(0.010), F(0.0), F(0.027) - F(0.030), F(0.0), F(0.058) - F(0.050)]
+    assert of[0] == pytest.approx(sum(float(x) for x in dr) / 5, rel=1e-6)
+    assert of[1] == pytest.approx(float(F(0.058) - F(0.050)), rel=1e-6)
+
+
+def test_gaussian_brownian_box_muller_from_philox(orc):
+    """Axis x of particle i uses Philox(i, frame, 0, 0) words 0,1, y words 2,3,
+    z Philox(i, frame, 1, 0) words 0,1, through Box-Muller in double; and the
+    sample moments are those of N(0, sigma^2)."""
+    seed, frame, s = 11, 5, F(1.0)
+    n, L = 4, 1000.0
+    pos = np.zeros((n, 3), F)
+    out = orc.brownian(pos, n, s, L, seed, frame, 1)
+    for i in range(n):
+        r0 = orc.philox((i, frame, 0, 0), (seed, 0))
+        r1 = orc.philox((i, frame, 1, 0), (seed, 0))
+
+        def bm(a, b):
+            u1 = ((int(a) >> 8) + 1) * 2.0 ** -24
+            u2 = (int(b) >> 8) * 2.0 ** -24
+            return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+        want = [bm(r0[0], r0[1]), bm(r0[2], r0[3]), bm(r1[0], r1[1])]
+        assert np.allclose(out[i], want, rtol=1e-6, atol=1e-7)
+    n = 40000
+    g = orc.brownian(np.zeros((n, 3), F), n, s, L, seed, frame, 1).astype(np.float64)
+    assert abs(g.mean()) < 4.0 / math.sqrt(3 * n)
+    assert g.std() == pytest.approx(1.0, rel=0.02)
+    kurt = ((g - g.mean()) ** 4).mean() / g.var() ** 2
+    assert kurt == pytest.approx(3.0, abs=0.15)
+    c = np.corrcoef(g.T)
+    assert np.abs(c - np.eye(3)).max() < 0.03
+
+
+def test_ou_jitter_closed_forms(orc):
+    n, dt, tau, rms = 20000, F(0.016), F(1.0), F(0.10)
+    rad = np.full(n, 0.02, F)
+    mean_r = F(0.02)
+    # decay only (sigma 0): v' = v + (-v/tau)*dt, pos' = wrap(pos + v'*dt)
+    v0 = np.full((n, 3), 0.01, F)
+    pos = np.zeros((n, 3), F)
+    p, v = orc.ou_jitter(pos, v0, rad, n, mean_r, dt, F(0.0), tau, F(1.0), 1.0, 3, 2)
+    want_v = v0 + ((-v0 / tau) * dt)
+    assert np.array_equal(v, want_v)
+    assert np.allclose(p, want_v * dt, rtol=1e-6)
+    # noise only: uniform increments of variance (sigma*sqrt(dt))^2 / 12
+    p, v = orc.ou_jitter(pos, np.zeros((n, 3), F), rad, n, mean_r, dt, rms, tau, F(1.0),
+                         1.0, 3, 2)
+    sdt = float(rms) * float(mean_r) * math.sqrt(2.0 / float(tau)) * math.sqrt(float(dt))
+    assert v.var() == pytest.approx(sdt * sdt / 12.0, rel=0.03)
+    assert np.abs(v).max() <= sdt / 2 * (1 + 1e-6)
+    # drift cap (dynamics.py:640-644): above cap = cap_frac * 2r the velocity
+    # is scaled by (cap / step) / dt, so the step becomes cap / dt (the
+    # reference divides by dt once more than a step-length cap would)
+    big = np.full((4, 3), 5.0, F)
+    cap_frac = F(0.2) * F(0.015)
+    p, v = orc.ou_jitter(np.zeros((4, 3), F), big, rad[:4], 4, mean_r, dt, F(0.0), tau,
+                         cap_frac, 1.0, 3, 2)
+    step = np.sqrt((v.astype(np.float64) ** 2).sum(1)) * float(dt)
+    cap = float(cap_frac) * 2 * 0.02
+    assert np.allclose(step, cap / float(dt), rtol=1e-5)
+    assert np.allclose(v, v[0], rtol=0) and (v > 0).all()      # direction kept
+
+
+def test_xsph_two_particles_and_isolated(orc):
+    """eps-blend with the mean velocity of neighbours inside lim: a moving
+    particle next to a resting one; a third particle out of range keeps its
+    velocity; zero velocities stay zero."""
+    n, G, L = 3, 5, 1.0
+    pos = np.array([[0.0, 0.0, 0.0], [0.01, 0.0, 0.0], [0.3, 0.3, 0.3]], F)
+    vel = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 2.0, 0.0]], F)
+    p, cc, cs, ci = orc.grid_build(pos, n, G, L)
+    eps, lim = F(0.03), F(0.05 * 0.05)
+    v = orc.xsph(p, vel, n, G, L, lim, eps, cs, cc, ci)
+    assert np.allclose(v[0], [1.0 - 0.03, 0, 0], rtol=1e-6)
+    assert np.allclose(v[1], [0.03, 0, 0], rtol=1e-6)
+    assert np.allclose(v[2], [0, 2.0, 0], rtol=1e-6)
+    assert (orc.xsph(p, np.zeros_like(vel), n, G, L, lim, eps, cs, cc, ci) == 0).all()
