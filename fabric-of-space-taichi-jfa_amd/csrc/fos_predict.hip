@@ -1,0 +1,496 @@
+// fos_predict.hip -- learned face predictor: scores hash-grid candidate pairs
+// for a shared power-diagram face on frames where the JFA is decimated.
+//
+// No reference counterpart: the reference always runs the JFA (jfa.run_jfa,
+// jfa.py:1056-1197).  Candidate generation follows the shape of
+// topology.build_topo_candidates (topology.py:68-133: 27 hash cells, a
+// capped list per particle), here for every j != i.
+//
+// Pipeline per frame: candidates -> 12 features per (i, j) -> MLP
+// 12 -> 64 -> 64 -> 1 (BF16 weights and activations on MFMA, FP32
+// accumulation, ReLU) -> logit > 0 is a face (sigmoid 0.5) -> FSC and face
+// rows.  Training kernels (FP32, Adam) fit the MLP to labels taken from the
+// plain JFA's neighbour lists.
+#include "fos_common.h"
+
+using namespace fos;
+
+#define PRED_FEAT 16  // feature slots per pair: 12 used, 4 zero (MFMA K = 16)
+#define MLP_H 64
+// flat FP32 parameter layout: W1[16][64] b1[64] W2[64][64] b2[64] w3[64] b3
+#define MLP_OFF_W1 0
+#define MLP_OFF_B1 (16 * MLP_H)
+#define MLP_OFF_W2 (MLP_OFF_B1 + MLP_H)
+#define MLP_OFF_B2 (MLP_OFF_W2 + MLP_H * MLP_H)
+#define MLP_OFF_W3 (MLP_OFF_B2 + MLP_H)
+#define MLP_OFF_B3 (MLP_OFF_W3 + MLP_H)
+#define MLP_NPARAM (MLP_OFF_B3 + 1)
+
+__device__ __forceinline__ float dot3f(float x, float y, float z) {
+  return (x * x + y * y) + z * z;
+}
+
+// ---------------------------------------------------------------------------
+// Candidates: every j != i with |d(i, j)| < mult * (ri + rj), scanning the 27
+// cells around i's cell (dx, dy, dz order, ascending index inside a cell),
+// first `cap` kept.  Needs a hash cell of at least mult * 2 * r_max.
+// ---------------------------------------------------------------------------
+__global__ void k_pred_cand(const float *__restrict__ pos, const float *__restrict__ rad, int n,
+                            int G, float L, float half_l, float inv_l, float mult, int cap,
+                            const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+                            const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
+                            int32_t *__restrict__ ncand, int32_t *__restrict__ trunc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2], ri = rad[i];
+  const int cx = wrap_fast(centre_axis(px, G, L, half_l, inv_l), G);
+  const int cy = wrap_fast(centre_axis(py, G, L, half_l, inv_l), G);
+  const int cz = wrap_fast(centre_axis(pz, G, L, half_l, inv_l), G);
+  int c = 0, tr = 0;
+  int32_t *row = cand + (int64_t)i * cap;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int id = (wrap1(cx + dx, G) * G + wrap1(cy + dy, G)) * G + wrap1(cz + dz, G);
+        const int b = cs[id], e = b + cc[id];
+        for (int k = b; k < e; ++k) {
+          const int j = ci[k];
+          if (j == i) continue;
+          const float ddx = gwrap(pos[3 * j] - px, L, inv_l);
+          const float ddy = gwrap(pos[3 * j + 1] - py, L, inv_l);
+          const float ddz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
+          const float th = mult * (ri + rad[j]);
+          if (dot3f(ddx, ddy, ddz) < th * th) {
+            if (c < cap)
+              row[c++] = j;
+            else
+              tr = 1;
+          }
+        }
+      }
+  for (int s = c; s < cap; ++s) row[s] = -1;
+  ncand[i] = c;
+  trunc[i] = tr;
+}
+
+// ---------------------------------------------------------------------------
+// Features of the directed pair (i, j) = (i, cand[i][s]).  With d = pj - pi
+// (minimum image), the power plane of i and j cuts the segment at
+// t = (|d|^2 + ri^2 - rj^2) / (2 |d|^2); m = pi + t d has equal power
+// p(m) = |m - pi|^2 - ri^2 for i and j.  A candidate k of i (or of j) with
+// |m - pk|^2 - rk^2 < p(m) owns m, so it may cut the face off.
+//   0 |d| / (ri + rj)         6 witnesses among i's candidates / 4 (<= 8)
+//   1 ri / r_mean             7 min witness margin on i's side / (ri + rj)^2
+//   2 rj / r_mean             8 witnesses among j's candidates / 4 (<= 8)
+//   3 t                       9 min witness margin on j's side / (ri + rj)^2
+//   4 |cand(i)| / 40         10 (ri - rj) / (ri + rj)
+//   5 |cand(j)| / 40         11 p(m) / (ri + rj)^2
+// Margins and p(m) are clamped to [-1, 1]; empty slots get all zeros.
+// ---------------------------------------------------------------------------
+__global__ void k_pred_feat(const float *__restrict__ pos, const float *__restrict__ rad, int n,
+                            float L, float inv_l, float inv_rbar,
+                            const int32_t *__restrict__ cand, const int32_t *__restrict__ ncand,
+                            int cap, float *__restrict__ feat) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= (int64_t)n * cap) return;
+  const int i = (int)(slot / cap), s = (int)(slot - (int64_t)i * cap);
+  float f[PRED_FEAT];
+#pragma unroll
+  for (int q = 0; q < PRED_FEAT; ++q) f[q] = 0.0f;
+  const int ni = ncand[i];
+  if (s < ni) {
+    const int j = cand[slot];
+    const float pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+    const float pjx = pos[3 * j], pjy = pos[3 * j + 1], pjz = pos[3 * j + 2];
+    const float ri = rad[i], rj = rad[j];
+    const float dx = gwrap(pjx - pix, L, inv_l), dy = gwrap(pjy - piy, L, inv_l),
+                dz = gwrap(pjz - piz, L, inv_l);
+    const float d2 = fmaxf(dot3f(dx, dy, dz), 1e-20f);
+    const float sr = ri + rj, sr2 = sr * sr;
+    const float t = (d2 + ri * ri - rj * rj) / (2.0f * d2);
+    const float mx = t * dx, my = t * dy, mz = t * dz;  // m relative to pi
+    const float pm = t * t * d2 - ri * ri;
+    int wi = 0, wj = 0;
+    float mi = 1e30f, mj = 1e30f;
+    for (int q = 0; q < ni; ++q) {
+      const int k = cand[(int64_t)i * cap + q];
+      if (k == j) continue;
+      const float kx = gwrap(pos[3 * k] - pix, L, inv_l) - mx;
+      const float ky = gwrap(pos[3 * k + 1] - piy, L, inv_l) - my;
+      const float kz = gwrap(pos[3 * k + 2] - piz, L, inv_l) - mz;
+      const float rk = rad[k];
+      const float w = dot3f(kx, ky, kz) - rk * rk - pm;
+      wi += w < 0.0f ? 1 : 0;
+      mi = fminf(mi, w);
+    }
+    const int nj = ncand[j];
+    const float qx = mx - dx, qy = my - dy, qz = mz - dz;  // m relative to pj
+    for (int q = 0; q < nj; ++q) {
+      const int k = cand[(int64_t)j * cap + q];
+      if (k == i) continue;
+      const float kx = gwrap(pos[3 * k] - pjx, L, inv_l) - qx;
+      const float ky = gwrap(pos[3 * k + 1] - pjy, L, inv_l) - qy;
+      const float kz = gwrap(pos[3 * k + 2] - pjz, L, inv_l) - qz;
+      const float rk = rad[k];
+      const float w = dot3f(kx, ky, kz) - rk * rk - pm;
+      wj += w < 0.0f ? 1 : 0;
+      mj = fminf(mj, w);
+    }
+    f[0] = sqrtf(d2) / sr;
+    f[1] = ri * inv_rbar;
+    f[2] = rj * inv_rbar;
+    f[3] = t;
+    f[4] = (float)ni * (1.0f / 40.0f);
+    f[5] = (float)nj * (1.0f / 40.0f);
+    f[6] = (float)min(wi, 8) * 0.25f;
+    f[7] = fminf(1.0f, fmaxf(-1.0f, mi / sr2));
+    f[8] = (float)min(wj, 8) * 0.25f;
+    f[9] = fminf(1.0f, fmaxf(-1.0f, mj / sr2));
+    f[10] = (ri - rj) / sr;
+    f[11] = fminf(1.0f, fmaxf(-1.0f, pm / sr2));
+  }
+  float4 *o = (float4 *)(feat + slot * PRED_FEAT);
+#pragma unroll
+  for (int q = 0; q < PRED_FEAT / 4; ++q)
+    o[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+}
+
+// ---------------------------------------------------------------------------
+// MLP inference on MFMA (v_mfma_f32_16x16x16_bf16).  Each layer is computed
+// transposed, Y^T = W^T X^T: the weights are the A operand (pre-packed on the
+// host in lane order), the activations the B operand.  The 16x16 f32 result
+// of a layer leaves lane l holding rows 4*(l/16)..+3 of column l%16, which
+// is the B-operand layout of the next layer's 16-row K block, so hidden
+// activations stay in registers.  A wave scores 16 pairs per step.
+// ---------------------------------------------------------------------------
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short f2bf(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (short)(u >> 16);
+}
+
+__global__ void __launch_bounds__(256)
+    k_mlp_infer(const float *__restrict__ feat, int64_t npairs,
+                const bf16x4 *__restrict__ w1p, const bf16x4 *__restrict__ w2p,
+                const float *__restrict__ b1, const float *__restrict__ b2,
+                const float *__restrict__ w3, float b3, float *__restrict__ logit) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int col = lane & 15, grp = lane >> 4;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  bf16x4 a1[4], a2[4][4];
+  float bb1[4][4], bb2[4][4], ww3[4][4];
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    a1[mb] = w1p[mb * 64 + lane];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) a2[mb][kb] = w2p[(mb * 4 + kb) * 64 + lane];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int h = 16 * mb + 4 * grp + v;
+      bb1[mb][v] = b1[h];
+      bb2[mb][v] = b2[h];
+      ww3[mb][v] = w3[h];
+    }
+  }
+  for (int64_t p0 = wave * 16; p0 < npairs; p0 += waves * 16) {
+    const float4 x = *(const float4 *)(feat + (p0 + col) * PRED_FEAT + 4 * grp);
+    bf16x4 xb;
+    xb[0] = f2bf(x.x);
+    xb[1] = f2bf(x.y);
+    xb[2] = f2bf(x.z);
+    xb[3] = f2bf(x.w);
+    bf16x4 h1[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[mb], xb, acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) h1[mb][v] = f2bf(fmaxf(acc[v] + bb1[mb][v], 0.0f));
+    }
+    float part = 0.0f;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[mb][kb], h1[kb], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) part += fmaxf(acc[v] + bb2[mb][v], 0.0f) * ww3[mb][v];
+    }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    if (grp == 0) logit[p0 + col] = part + b3;
+  }
+}
+
+// FSC from the scores: fsc[i] = kept pairs of row i (logit > 0); face rows
+// list the kept partners j > i in candidate order, at most 32, -1 padded
+// (the row shape accumulate_face_area produces for the pressure step).
+__global__ void k_pred_fsc(const float *__restrict__ logit, const int32_t *__restrict__ cand,
+                           const int32_t *__restrict__ ncand, int n, int cap,
+                           int32_t *__restrict__ fsc, int32_t *__restrict__ face_ids,
+                           int32_t *__restrict__ face_counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ni = ncand[i];
+  int c = 0, r = 0;
+  for (int s = 0; s < ni; ++s) {
+    if (logit[(int64_t)i * cap + s] > 0.0f) {
+      c += 1;
+      const int j = cand[(int64_t)i * cap + s];
+      if (face_ids && j > i && r < FOS_MAX_NEIGHBORS) {
+        face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + r] = j;
+        if (face_counts) face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + r] = 1;
+        r += 1;
+      }
+    }
+  }
+  fsc[i] = c;
+  if (face_ids)
+    for (; r < FOS_MAX_NEIGHBORS; ++r) {
+      face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + r] = -1;
+      if (face_counts) face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + r] = 0;
+    }
+}
+
+// Labels for training: 1 when j is in i's JFA neighbour list.
+__global__ void k_pred_labels(const int32_t *__restrict__ cand, const int32_t *__restrict__ ncand,
+                              int n, int cap, const int32_t *__restrict__ nl,
+                              const int32_t *__restrict__ nc, float *__restrict__ y) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= (int64_t)n * cap) return;
+  const int i = (int)(slot / cap), s = (int)(slot - (int64_t)i * cap);
+  float v = -1.0f;  // empty slot
+  if (s < ncand[i]) {
+    const int j = cand[slot];
+    const int m = min(nc[i], FOS_MAX_NEIGHBORS);
+    v = 0.0f;
+    for (int q = 0; q < m; ++q)
+      if (nl[(int64_t)i * FOS_MAX_NEIGHBORS + q] == j) v = 1.0f;
+  }
+  y[slot] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Training (FP32).  One step over a batch of B samples (indices into the
+// feature/label arrays): forward keeps H1, H2 and dL/dz3; backward forms
+// dZ2, dZ1; the gradient kernel sums per parameter over the batch (one
+// thread per parameter, fixed order, so steps are reproducible); Adam.
+// Loss: class-weighted binary cross entropy on the logit.
+// ---------------------------------------------------------------------------
+__global__ void k_mlp_fwd(const float *__restrict__ feat, const float *__restrict__ y,
+                          const int32_t *__restrict__ idx, int B, const float *__restrict__ prm,
+                          float pos_w, float *__restrict__ H1, float *__restrict__ H2,
+                          float *__restrict__ dz3, float *__restrict__ loss) {
+  const int b0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = b0 < B;
+  const int b = live ? b0 : B - 1;
+  const int64_t p = idx[b];
+  float x[12];
+#pragma unroll
+  for (int f = 0; f < 12; ++f) x[f] = feat[p * PRED_FEAT + f];
+  float h1[MLP_H];
+#pragma unroll
+  for (int a = 0; a < MLP_H; ++a) {
+    float z = prm[MLP_OFF_B1 + a];
+#pragma unroll
+    for (int f = 0; f < 12; ++f) z += x[f] * prm[MLP_OFF_W1 + f * MLP_H + a];
+    h1[a] = fmaxf(z, 0.0f);
+    if (live) H1[(int64_t)b * MLP_H + a] = h1[a];
+  }
+  float z3 = prm[MLP_OFF_B3];
+  for (int h = 0; h < MLP_H; ++h) {
+    float z = prm[MLP_OFF_B2 + h];
+#pragma unroll
+    for (int a = 0; a < MLP_H; ++a) z += h1[a] * prm[MLP_OFF_W2 + a * MLP_H + h];
+    z = fmaxf(z, 0.0f);
+    if (live) H2[(int64_t)b * MLP_H + h] = z;
+    z3 += z * prm[MLP_OFF_W3 + h];
+  }
+  const float t = y[p];
+  const float w = t > 0.5f ? pos_w : 1.0f;
+  const float sg = 1.0f / (1.0f + expf(-z3));
+  if (live) dz3[b] = w * (sg - t) / (float)B;
+  // log(1 + e^-|z|) + max(z, 0) - t z, summed per wave, one atomic per wave
+  float l = live ? w * (log1pf(expf(-fabsf(z3))) + fmaxf(z3, 0.0f) - t * z3) : 0.0f;
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, l / (float)B);
+}
+
+__global__ void k_mlp_bwd(const float *__restrict__ H1, const float *__restrict__ H2,
+                          const float *__restrict__ dz3, int B, const float *__restrict__ prm,
+                          float *__restrict__ dZ1, float *__restrict__ dZ2) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float d2[MLP_H];
+  const float g = dz3[b];
+#pragma unroll
+  for (int h = 0; h < MLP_H; ++h) {
+    d2[h] = H2[(int64_t)b * MLP_H + h] > 0.0f ? g * prm[MLP_OFF_W3 + h] : 0.0f;
+    dZ2[(int64_t)b * MLP_H + h] = d2[h];
+  }
+  for (int a = 0; a < MLP_H; ++a) {
+    float s = 0.0f;
+#pragma unroll
+    for (int h = 0; h < MLP_H; ++h) s += prm[MLP_OFF_W2 + a * MLP_H + h] * d2[h];
+    dZ1[(int64_t)b * MLP_H + a] = H1[(int64_t)b * MLP_H + a] > 0.0f ? s : 0.0f;
+  }
+}
+
+__global__ void k_mlp_grad(const float *__restrict__ feat, const int32_t *__restrict__ idx, int B,
+                           const float *__restrict__ H1, const float *__restrict__ H2,
+                           const float *__restrict__ dz3, const float *__restrict__ dZ1,
+                           const float *__restrict__ dZ2, float *__restrict__ grad) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= MLP_NPARAM) return;
+  float s = 0.0f;
+  if (q < MLP_OFF_B1) {  // W1[f][a]
+    const int f = q / MLP_H, a = q % MLP_H;
+    if (f < 12)
+      for (int b = 0; b < B; ++b)
+        s += feat[(int64_t)idx[b] * PRED_FEAT + f] * dZ1[(int64_t)b * MLP_H + a];
+  } else if (q < MLP_OFF_W2) {
+    const int a = q - MLP_OFF_B1;
+    for (int b = 0; b < B; ++b) s += dZ1[(int64_t)b * MLP_H + a];
+  } else if (q < MLP_OFF_B2) {
+    const int r = q - MLP_OFF_W2, a = r / MLP_H, h = r % MLP_H;
+    for (int b = 0; b < B; ++b) s += H1[(int64_t)b * MLP_H + a] * dZ2[(int64_t)b * MLP_H + h];
+  } else if (q < MLP_OFF_W3) {
+    const int h = q - MLP_OFF_B2;
+    for (int b = 0; b < B; ++b) s += dZ2[(int64_t)b * MLP_H + h];
+  } else if (q < MLP_OFF_B3) {
+    const int h = q - MLP_OFF_W3;
+    for (int b = 0; b < B; ++b) s += dz3[b] * H2[(int64_t)b * MLP_H + h];
+  } else {
+    for (int b = 0; b < B; ++b) s += dz3[b];
+  }
+  grad[q] = s;
+}
+
+__global__ void k_adam(float *__restrict__ prm, const float *__restrict__ grad,
+                       float *__restrict__ m, float *__restrict__ v, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= MLP_NPARAM) return;
+  const float g = grad[q];
+  const float mq = b1 * m[q] + (1.0f - b1) * g;
+  const float vq = b2 * v[q] + (1.0f - b2) * g * g;
+  m[q] = mq;
+  v[q] = vq;
+  prm[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+}
+
+extern "C" {
+
+int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float L, float mult,
+                        int cap, const int32_t *cell_start, const int32_t *cell_count,
+                        const int32_t *cell_indices, int32_t *cand, int32_t *ncand,
+                        int32_t *truncated, void *stream) {
+  FOS_REQUIRE(n >= 0 && G >= 3 && L > 0.0f && cap >= 1 && mult > 0.0f &&
+                  (n == 0 || (pos && rad && cell_start && cell_count && cell_indices && cand &&
+                              ncand && truncated)),
+              "fos_pred_candidates: bad arguments (n=%d G=%d cap=%d)", n, G, cap);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_pred_cand, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     pos, rad, n, G, L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), mult,
+                     cap, cell_start, cell_count, cell_indices, cand, ncand, truncated);
+  FOS_CHECK_LAUNCH("k_pred_cand");
+  return 0;
+}
+
+int fos_pred_features(const float *pos, const float *rad, int n, float L, float r_mean,
+                      const int32_t *cand, const int32_t *ncand, int cap, float *feat,
+                      void *stream) {
+  FOS_REQUIRE(n >= 0 && L > 0.0f && cap >= 1 && r_mean > 0.0f &&
+                  (n == 0 || (pos && rad && cand && ncand && feat)),
+              "fos_pred_features: bad arguments (n=%d cap=%d)", n, cap);
+  if (n == 0) return 0;
+  const int64_t slots = (int64_t)n * cap;
+  hipLaunchKernelGGL(k_pred_feat, dim3(blocks_for(slots)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, pos, rad, n, L, (float)(1.0 / (double)L),
+                     1.0f / r_mean, cand, ncand, cap, feat);
+  FOS_CHECK_LAUNCH("k_pred_feat");
+  return 0;
+}
+
+int fos_pred_mlp(const float *feat, int64_t npairs, const void *w1_packed, const void *w2_packed,
+                 const float *b1, const float *b2, const float *w3, float b3, float *logit,
+                 void *stream) {
+  FOS_REQUIRE(npairs >= 0 && npairs % 16 == 0 &&
+                  (npairs == 0 || (feat && w1_packed && w2_packed && b1 && b2 && w3 && logit)),
+              "fos_pred_mlp: bad arguments (npairs=%lld, must be a multiple of 16)",
+              (long long)npairs);
+  if (npairs == 0) return 0;
+  const int64_t groups = npairs / 16;                 // one wave step per 16 pairs
+  int64_t blocks = (groups + 3) / 4;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     feat, npairs, (const bf16x4 *)w1_packed, (const bf16x4 *)w2_packed, b1, b2,
+                     w3, b3, logit);
+  FOS_CHECK_LAUNCH("k_mlp_infer");
+  return 0;
+}
+
+int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *ncand, int n, int cap,
+                 int32_t *fsc, int32_t *face_ids, int32_t *face_counts, void *stream) {
+  FOS_REQUIRE(n >= 0 && cap >= 1 && (n == 0 || (logit && cand && ncand && fsc)),
+              "fos_pred_fsc: bad arguments (n=%d cap=%d)", n, cap);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_pred_fsc, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     logit, cand, ncand, n, cap, fsc, face_ids, face_counts);
+  FOS_CHECK_LAUNCH("k_pred_fsc");
+  return 0;
+}
+
+int fos_pred_labels(const int32_t *cand, const int32_t *ncand, int n, int cap,
+                    const int32_t *neighbor_list, const int32_t *neighbor_count, float *labels,
+                    void *stream) {
+  FOS_REQUIRE(n >= 0 && cap >= 1 &&
+                  (n == 0 || (cand && ncand && neighbor_list && neighbor_count && labels)),
+              "fos_pred_labels: bad arguments (n=%d cap=%d)", n, cap);
+  if (n == 0) return 0;
+  const int64_t slots = (int64_t)n * cap;
+  hipLaunchKernelGGL(k_pred_labels, dim3(blocks_for(slots)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, cand, ncand, n, cap, neighbor_list, neighbor_count,
+                     labels);
+  FOS_CHECK_LAUNCH("k_pred_labels");
+  return 0;
+}
+
+int fos_mlp_param_count(int *count) {
+  FOS_REQUIRE(count, "fos_mlp_param_count: count is NULL");
+  *count = MLP_NPARAM;
+  return 0;
+}
+
+int fos_mlp_train_step(float *params, float *grad, float *adam_m, float *adam_v, int step,
+                       float lr, const float *feat, const float *labels, const int32_t *batch_idx,
+                       int B, float pos_weight, float *scratch, float *loss, void *stream) {
+  FOS_REQUIRE(params && grad && adam_m && adam_v && feat && labels && batch_idx && scratch &&
+                  loss && B >= 1 && step >= 1 && lr > 0.0f,
+              "fos_mlp_train_step: bad arguments (B=%d step=%d)", B, step);
+  hipStream_t st = (hipStream_t)stream;
+  float *H1 = scratch, *H2 = H1 + (size_t)B * MLP_H, *dZ1 = H2 + (size_t)B * MLP_H,
+        *dZ2 = dZ1 + (size_t)B * MLP_H, *dz3 = dZ2 + (size_t)B * MLP_H;
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(blocks_for(B, 64)), dim3(64), 0, st, feat, labels,
+                     batch_idx, B, (const float *)params, pos_weight, H1, H2, dz3, loss);
+  FOS_CHECK_LAUNCH("k_mlp_fwd");
+  hipLaunchKernelGGL(k_mlp_bwd, dim3(blocks_for(B, 64)), dim3(64), 0, st, H1, H2, dz3, B,
+                     (const float *)params, dZ1, dZ2);
+  FOS_CHECK_LAUNCH("k_mlp_bwd");
+  hipLaunchKernelGGL(k_mlp_grad, dim3(blocks_for(MLP_NPARAM, 64)), dim3(64), 0, st, feat,
+                     batch_idx, B, H1, H2, dz3, dZ1, dZ2, grad);
+  FOS_CHECK_LAUNCH("k_mlp_grad");
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(MLP_NPARAM)), dim3(FOS_BLOCK), 0, st, params,
+                     (const float *)grad, adam_m, adam_v, lr, b1, b2, 1e-8f, c1, c2);
+  FOS_CHECK_LAUNCH("k_adam");
+  return 0;
+}
+
+}  // extern "C"

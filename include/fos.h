@@ -258,6 +258,52 @@ int fos_topo_candidates(const float *pos, const float *rad, int n, int G,
                         const int32_t *cell_indices, int32_t *pairs,
                         int32_t *count, int32_t *truncated, void *stream);
 
+/* ---- learned face predictor -------------------------------------------
+ * No reference counterpart: the reference always runs the JFA
+ * (jfa.run_jfa, jfa.py:1056-1197).  These score candidate pairs for a shared
+ * power-diagram face on frames where the JFA is decimated; the plain JFA
+ * stays the judge (see the frame driver). */
+
+/* Candidate pairs: every j != i within mult*(ri+rj) over the 27 hash cells
+ * around i (cell size >= mult*2*r_max), in cell/occupant order (the shape of
+ * topology.build_topo_candidates, topology.py:68-133), first `cap` kept in
+ * cand[n][cap] (-1 padded), counts in ncand[n], truncated[n] = 1 when more
+ * were found. */
+int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float L, float mult,
+                        int cap, const int32_t *cell_start, const int32_t *cell_count,
+                        const int32_t *cell_indices, int32_t *cand, int32_t *ncand,
+                        int32_t *truncated, void *stream);
+/* 12 features per candidate slot into feat[n*cap][16] (slots 12-15 and empty
+ * candidate slots zero); r_mean is the mean radius. */
+int fos_pred_features(const float *pos, const float *rad, int n, float L, float r_mean,
+                      const int32_t *cand, const int32_t *ncand, int cap, float *feat,
+                      void *stream);
+/* MLP 12->64->64->1 on MFMA with BF16 weights/activations and FP32
+ * accumulation: logit[p] for p < npairs (a multiple of 16).  w1_packed /
+ * w2_packed hold the BF16 weights in MFMA lane order (fos_lib.pack_mlp). */
+int fos_pred_mlp(const float *feat, int64_t npairs, const void *w1_packed, const void *w2_packed,
+                 const float *b1, const float *b2, const float *w3, float b3, float *logit,
+                 void *stream);
+/* fsc[i] = slots of row i with logit > 0; optional face rows (partners j > i
+ * in candidate order, at most 32, -1 padded; counts 1) for the pressure
+ * step (jfa.py:702-762 row shape). */
+int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *ncand, int n, int cap,
+                 int32_t *fsc, int32_t *face_ids, int32_t *face_counts, void *stream);
+/* Training labels: 1 when cand[i][s] is in i's JFA neighbour list, 0 when
+ * not, -1 for an empty slot. */
+int fos_pred_labels(const int32_t *cand, const int32_t *ncand, int n, int cap,
+                    const int32_t *neighbor_list, const int32_t *neighbor_count, float *labels,
+                    void *stream);
+/* Number of FP32 MLP parameters (flat layout W1[16][64] b1 W2[64][64] b2 w3
+ * b3) into host *count. */
+int fos_mlp_param_count(int *count);
+/* One Adam step (FP32) on a batch of B sample indices with class-weighted
+ * binary cross entropy; scratch holds 4*64*B + B floats; *loss (device) is
+ * increased by the batch mean loss. */
+int fos_mlp_train_step(float *params, float *grad, float *adam_m, float *adam_v, int step,
+                       float lr, const float *feat, const float *labels, const int32_t *batch_idx,
+                       int B, float pos_weight, float *scratch, float *loss, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

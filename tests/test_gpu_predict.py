@@ -1,0 +1,233 @@
+"""GPU checks of the learned face predictor (no reference counterpart):
+candidates against brute force, features and MLP against numpy, FSC/face
+rows and labels against their definitions, the training gradients against
+numpy backpropagation."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import foam_inputs as fi
+import face_predictor as fp
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+_alive = []
+
+
+@pytest.fixture(autouse=True)
+def _keep():
+    yield
+    torch.cuda.synchronize()
+    _alive.clear()
+
+
+def dev(a):
+    t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    _alive.append(t)
+    return t
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def gwrap(p, L=1.0):
+    return (p - F(L) * np.floor(p * F(1.0 / L) + F(0.5))).astype(F)
+
+
+def brute_candidates(pos, rad, G, L=1.0, mult=fp.MULT, cap=fp.CAP):
+    """Same fp32 expressions as the kernel; order: 27 cells around i's cell
+    (dx, dy, dz), ascending index within a cell."""
+    n = len(rad)
+    pw = gwrap(pos, L)                       # the grid rebuild stores this
+    half, inv = F(0.5 * L), F(1.0 / L)
+    cell = np.floor(((gwrap(pw, L) + half) * inv) * F(G)).astype(np.int64) % G
+    out = []
+    for i in range(n):
+        d = gwrap(pw - pw[i], L)
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        th = F(mult) * (rad[i] + rad)
+        ok = np.nonzero((d2 < th * th) & (np.arange(n) != i))[0]
+        rel = (cell[ok] - cell[i] + 1) % G                 # 0, 1, 2 for -1, 0, +1
+        q = (rel[:, 0] * 3 + rel[:, 1]) * 3 + rel[:, 2]
+        order = np.lexsort((ok, q))
+        out.append(ok[order][:cap])
+    return out
+
+
+def test_candidates_match_brute_force(gpu):
+    n = 1500
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=3)
+    G = fp.hash_res(1.0, 0.04)
+    pr = fp.FacePredictor(fp.init_params(), n, 1.0, 0.04)
+    tp, tr = dev(pos), dev(rad)
+    pr.candidates(tp, tr, n)
+    cand, nc = host(pr.cand), host(pr.ncand)
+    want = brute_candidates(pos, rad, G)
+    for i in range(n):
+        assert nc[i] == len(want[i]), i
+        assert (cand[i, :nc[i]] == want[i]).all(), i
+        assert (cand[i, nc[i]:] == -1).all()
+    assert nc.mean() > 3 and nc.max() <= fp.CAP
+
+
+def feat_reference(pos, rad, cand, nc, r_mean, L=1.0):
+    n, cap = cand.shape
+    out = np.zeros((n, cap, 16), np.float64)
+    p = gwrap(pos, L).astype(np.float64)
+    r = rad.astype(np.float64)
+
+    def md(a):
+        return a - L * np.floor(a / L + 0.5)
+    for i in range(n):
+        for s in range(nc[i]):
+            j = cand[i, s]
+            d = md(p[j] - p[i])
+            d2 = max(d @ d, 1e-20)
+            sr = r[i] + r[j]
+            t = (d2 + r[i] ** 2 - r[j] ** 2) / (2 * d2)
+            m = t * d
+            pm = t * t * d2 - r[i] ** 2
+            ki = [k for k in cand[i, :nc[i]] if k != j]
+            wi = np.array([((md(p[k] - p[i]) - m) ** 2).sum() - r[k] ** 2 - pm for k in ki])
+            kj = [k for k in cand[j, :nc[j]] if k != i]
+            wj = np.array([((md(p[k] - p[j]) - (m - d)) ** 2).sum() - r[k] ** 2 - pm for k in kj])
+            mi = wi.min() if len(wi) else 1e30
+            mj = wj.min() if len(wj) else 1e30
+            out[i, s, :12] = [math.sqrt(d2) / sr, r[i] / r_mean, r[j] / r_mean, t, nc[i] / 40,
+                              nc[j] / 40, min((wi < 0).sum(), 8) / 4,
+                              np.clip(mi / sr ** 2, -1, 1), min((wj < 0).sum(), 8) / 4,
+                              np.clip(mj / sr ** 2, -1, 1), (r[i] - r[j]) / sr,
+                              np.clip(pm / sr ** 2, -1, 1)]
+    return out
+
+
+def test_features_match_numpy(gpu):
+    n = 400
+    pos, rad = fi.uniform(n, 0.04, 0.08, seed=4)
+    pr = fp.FacePredictor(fp.init_params(), n, 1.0, 0.08)
+    tp, tr = dev(pos), dev(rad)
+    pr.candidates(tp, tr, n)
+    r_mean = float(rad.mean())
+    pr.features(tr, n, r_mean)
+    got = host(pr.feat).reshape(n, fp.CAP, 16)
+    cand, nc = host(pr.cand), host(pr.ncand)
+    want = feat_reference(pos, rad, cand, nc, r_mean)
+    cont = [0, 1, 2, 3, 4, 5, 7, 9, 10, 11]
+    assert np.allclose(got[..., cont], want[..., cont], rtol=1e-4, atol=2e-5)
+    # witness counts can differ only where a margin is within rounding of 0
+    for f in (6, 8):
+        assert (np.abs(got[..., f] - want[..., f]) > 1e-6).mean() < 1e-3
+    assert (got[..., 12:] == 0).all()
+
+
+def test_mlp_mfma_matches_numpy(gpu):
+    rng = np.random.default_rng(5)
+    P = 4096
+    feat = np.zeros((P, 16), F)
+    feat[:, :12] = rng.normal(0, 1, (P, 12))
+    params = fp.init_params(seed=7)
+    params[fp.OFF_B1:fp.OFF_W2] = rng.normal(0, 0.1, 64)
+    params[fp.OFF_B2:fp.OFF_W3] = rng.normal(0, 0.1, 64)
+    params[fp.OFF_B3] = 0.3
+    pk = fp.pack(params)
+    tf = dev(feat)
+    out = torch.empty(P, dtype=torch.float32, device="cuda")
+    gpu.call("fos_pred_mlp", gpu.ptr(tf), P, gpu.ptr(pk["w1p"]), gpu.ptr(pk["w2p"]),
+             gpu.ptr(pk["b1"]), gpu.ptr(pk["b2"]), gpu.ptr(pk["w3"]), pk["b3"], gpu.ptr(out),
+             gpu.stream())
+    got = host(out)
+    want = fp.mlp_reference(feat, params)
+    # fp32 accumulation order differs (MFMA vs numpy): a hidden value can land
+    # on the other side of a BF16 rounding step
+    err = np.abs(got - want)
+    assert err.mean() < 2e-3 * np.abs(want).mean()
+    assert err.max() < 0.05 * (1 + np.abs(want).max())
+    assert (np.sign(got) == np.sign(want)).mean() > 0.998
+
+
+def test_fsc_rows_and_labels(gpu):
+    n, cap = 300, fp.CAP
+    rng = np.random.default_rng(8)
+    nc = rng.integers(0, 50, n).astype(np.int32)
+    cand = np.full((n, cap), -1, np.int32)
+    for i in range(n):
+        cand[i, :nc[i]] = rng.choice(np.delete(np.arange(n), i), nc[i], replace=False)
+    logit = rng.normal(0, 1, (n, cap)).astype(F)
+    fsc = torch.empty(n, dtype=torch.int32, device="cuda")
+    ids = torch.empty((n, 32), dtype=torch.int32, device="cuda")
+    cts = torch.empty((n, 32), dtype=torch.int32, device="cuda")
+    gpu.call("fos_pred_fsc", gpu.ptr(dev(logit)), gpu.ptr(dev(cand)), gpu.ptr(dev(nc)), n, cap,
+             gpu.ptr(fsc), gpu.ptr(ids), gpu.ptr(cts), gpu.stream())
+    f, ii, cc = host(fsc), host(ids), host(cts)
+    for i in range(n):
+        keep = [int(cand[i, s]) for s in range(nc[i]) if logit[i, s] > 0]
+        assert f[i] == len(keep)
+        up = [j for j in keep if j > i][:32]
+        assert list(ii[i, :len(up)]) == up and (ii[i, len(up):] == -1).all()
+        assert (cc[i, :len(up)] == 1).all() and (cc[i, len(up):] == 0).all()
+    nl = np.full((n, 32), -1, np.int32)
+    nn = rng.integers(0, 33, n).astype(np.int32)
+    for i in range(n):
+        nl[i, :nn[i]] = rng.integers(0, n, nn[i])
+    y = torch.empty(n * cap, dtype=torch.float32, device="cuda")
+    gpu.call("fos_pred_labels", gpu.ptr(dev(cand)), gpu.ptr(dev(nc)), n, cap, gpu.ptr(dev(nl)),
+             gpu.ptr(dev(nn)), gpu.ptr(y), gpu.stream())
+    yy = host(y).reshape(n, cap)
+    for i in range(n):
+        for s in range(cap):
+            want = -1.0 if s >= nc[i] else float(cand[i, s] in set(nl[i, :nn[i]].tolist()))
+            assert yy[i, s] == want
+
+
+def numpy_grad(params, feat, y, idx, pos_w):
+    w1, b1, w2, b2, w3, b3 = [np.asarray(a, np.float64) for a in fp.unpack(params)]
+    x = feat[idx].astype(np.float64)
+    t = y[idx].astype(np.float64)
+    B = len(idx)
+    z1 = x @ w1 + b1
+    h1 = np.maximum(z1, 0)
+    z2 = h1 @ w2 + b2
+    h2 = np.maximum(z2, 0)
+    z3 = h2 @ w3 + b3
+    w = np.where(t > 0.5, pos_w, 1.0)
+    dz3 = w * (1 / (1 + np.exp(-z3)) - t) / B
+    dz2 = (dz3[:, None] * w3[None, :]) * (z2 > 0)
+    dz1 = (dz2 @ w2.T) * (z1 > 0)
+    g = np.concatenate([(x.T @ dz1).ravel(), dz1.sum(0), (h1.T @ dz2).ravel(), dz2.sum(0),
+                        h2.T @ dz3, [dz3.sum()]])
+    return g
+
+
+def test_training_gradients_and_loss(gpu):
+    rng = np.random.default_rng(9)
+    P = 20000
+    feat = np.zeros((P, 16), F)
+    feat[:, :12] = rng.normal(0, 1, (P, 12))
+    y = ((feat[:, 0] + 0.5 * feat[:, 3] - 0.3 * feat[:, 7]) > 0.2).astype(F)
+    params = fp.init_params(seed=1)
+    tr = fp.Trainer(params, batch=1024, lr=3e-3, pos_weight=1.5)
+    tf, ty = dev(feat), dev(y)
+    idx = np.arange(1024, dtype=np.int32)
+    gpu.call("fos_mlp_train_step", gpu.ptr(tr.p), gpu.ptr(tr.g), gpu.ptr(tr.m), gpu.ptr(tr.v), 1,
+             1e-9, gpu.ptr(tf), gpu.ptr(ty), gpu.ptr(dev(idx)), 1024, 1.5, gpu.ptr(tr.scratch),
+             gpu.ptr(tr.loss), gpu.stream())
+    g = host(tr.g).astype(np.float64)
+    want = numpy_grad(params, feat, y, idx, 1.5)
+    assert np.allclose(g, want, rtol=2e-3, atol=2e-6)
+    tr = fp.Trainer(params, batch=1024, lr=3e-3, pos_weight=1.0)
+    valid = torch.arange(P, device="cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0)
+    losses = [tr.epoch(tf, ty, valid, gen) for _ in range(8)]
+    assert losses[-1] < 0.5 * losses[0]
+    pr = fp.pack(tr.params())
+    out = torch.empty(P, dtype=torch.float32, device="cuda")
+    gpu.call("fos_pred_mlp", gpu.ptr(tf), P, gpu.ptr(pr["w1p"]), gpu.ptr(pr["w2p"]),
+             gpu.ptr(pr["b1"]), gpu.ptr(pr["b2"]), gpu.ptr(pr["w3"]), pr["b3"], gpu.ptr(out),
+             gpu.stream())
+    acc = ((host(out) > 0) == (y > 0.5)).mean()
+    assert acc > 0.95
