@@ -16,6 +16,10 @@ The JSON line also carries:
   cpu_baseline  the CPU restatement (C, OpenMP over the host cores) timed on
                 a bounded sample of the same frames
   cadence       the same workload with JFA on a 1/5 cadence after 20 frames
+  surrogate     the learned face predictor at N=50,000 / 256^3: frames/s with
+                the JFA every frame against the JFA every 8th frame (the
+                judge) and the predictor in between, with the judged FSC
+                errors (the frame's mean FSC, and per particle)
 """
 import argparse
 import ctypes
@@ -33,6 +37,7 @@ for _p in (ROOT, PKG):
 METRIC = "frames/s at N=10k,128³ JFA; Gvox-pass/s + %HBM; surrogate vs plain @FSC≤2%"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_jfa_pass_c2_hbm_traffic.md")
+WEIGHTS = os.path.join(PKG, "face_mlp_c5.npz")
 COPY_CAL_RES = 400          # 400^3 int32 = 256,000,000 B read and written
 
 
@@ -73,6 +78,8 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cadence", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=48)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain copy of 400^3 int32 (counter scaling)")
     return ap.parse_args()
@@ -83,12 +90,14 @@ def run_frames(sim, frames):
         sim.step()
 
 
-def timed(cfg, pos, rad, steps, warmup, with_events):
+def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
     import torch
 
     import fos_lib as fl
     from foam import FoamSim
     sim = FoamSim(cfg, pos, rad)
+    if setup is not None:
+        setup(sim)
     run_frames(sim, warmup)
     torch.cuda.synchronize()
     P = cfg.passes()
@@ -111,6 +120,39 @@ def timed(cfg, pos, rad, steps, warmup, with_events):
         pass_ms.append(list(ms))
         fl.call("fos_events_destroy", P + 1, arr)
     return t1 - t0, pass_ms, sim
+
+
+def surrogate_bench(steps, warmup):
+    """Learned face predictor on the N=50,000 / 256^3 workload: frames/s of the
+    plain path (JFA every frame) and of the predictor path (JFA every 8th
+    frame as the judge, predictor in between, fallback to the JFA while the
+    judged FSC error exceeds 2%), and the judged errors."""
+    import json as _json
+
+    import numpy as np
+
+    import foam_config as fc
+    import foam_inputs as fi
+    w = np.load(WEIGHTS, allow_pickle=False)
+    params, meta = w["params"], _json.loads(str(w["meta"]))
+    plain = fc.preset("c5")
+    pos, rad = fi.for_config(plain)
+    el_p, _, _ = timed(plain, pos, rad, steps, warmup, False)
+    surr = fc.preset("c5s")
+    el_s, _, sim = timed(surr, pos, rad, steps, warmup, False,
+                         setup=lambda s: s.set_predictor(params, meta["mult"]))
+    judged = [j for j in sim.judged if j[0] >= warmup]
+    fp_ = steps / el_p
+    fs_ = steps / el_s
+    return {"workload": "N=50000, radii U[0.005,0.011], JFA 256^3; predictor path: JFA every "
+                        "8th frame as judge, BF16 MLP on MFMA in between",
+            "plain_fps": round(fp_, 3), "surrogate_fps": round(fs_, 3),
+            "speedup": round(fs_ / fp_, 3), "frames": steps,
+            "judge_frames": len(judged),
+            "fsc_err_mean_fsc": round(float(np.mean([j[2] for j in judged])), 5) if judged else None,
+            "fsc_err_per_particle": round(float(np.mean([j[1] for j in judged])), 5) if judged else None,
+            "fallback_frames": int(sum(1 for j in judged if j[2] > surr.max_fsc_err)),
+            "candidate_mult": meta["mult"], "trained_on": f"{meta['frames']} plain-JFA frames"}
 
 
 def cpu_baseline(cfg, pos, rad, frames):
@@ -185,6 +227,8 @@ def main():
         el2, _, _ = timed(c2c, pos, rad, args.steps, args.warmup, False)
         out["cadence"] = {"workload": "same, JFA every 5th frame after a 20-frame warm start",
                           "value": round(args.steps / el2, 3), "unit": "frames/s"}
+    if not args.no_surrogate and os.path.exists(WEIGHTS):
+        out["surrogate"] = surrogate_bench(args.surrogate_steps, 8)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, pos, rad, args.cpu_frames)
         out["speedup_vs_cpu"] = round(fps / out["cpu_baseline"]["value"], 1)

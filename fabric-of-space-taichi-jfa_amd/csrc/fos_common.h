@@ -195,6 +195,7 @@ struct fos_ctx {
   // grid scratch
   int *cell_write;        // [max_cells]
   int *cell_of;           // [max_n]
+  int *grid_tmp;          // [max_n] cell-ordered list before the stable placement
   float *pos_tmp;         // [max_n*3]
   float *rad_tmp;         // [max_n] pressure ping-pong
   float *fscratch;        // [16] small device floats (reductions)

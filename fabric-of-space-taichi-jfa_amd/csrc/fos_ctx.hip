@@ -202,7 +202,7 @@ int fos_ctx_destroy(fos_ctx *ctx) {
                   ctx->in_sorted,
                   ctx->acc_in,   ctx->scan_blocks, ctx->cell_write, ctx->cell_of,
                   ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch,
-                  ctx->chg[0],   ctx->chg[1]};
+                  ctx->chg[0],   ctx->chg[1],   ctx->grid_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->host_i) (void)hipHostFree(ctx->host_i);
@@ -286,6 +286,7 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
   ALLOC(c->scan_blocks, 2 * (size_t)SCAN_TILE * sizeof(int));
   ALLOC(c->cell_write, (size_t)max_cells * sizeof(int));
   ALLOC(c->cell_of, (size_t)max_n * sizeof(int));
+  ALLOC(c->grid_tmp, (size_t)max_n * sizeof(int));
   ALLOC(c->pos_tmp, (size_t)max_n * 3 * sizeof(float));
   ALLOC(c->rad_tmp, (size_t)max_n * sizeof(float));
   ALLOC(c->fscratch, 64 * sizeof(float));

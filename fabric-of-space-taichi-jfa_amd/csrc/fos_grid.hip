@@ -60,6 +60,36 @@ __global__ void k_grid_sort_cells(int C, const int32_t *__restrict__ wr,
   }
 }
 
+// Large-system scatter: wraps pos, records each particle's cell and appends
+// it to a temporary cell-ordered list (arbitrary order inside a cell).
+__global__ void k_grid_scatter_tmp(float *__restrict__ pos, int n, int G, float L, float half_l,
+                                   float inv_l, int32_t *__restrict__ wr,
+                                   int32_t *__restrict__ cell_of, int32_t *__restrict__ tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float pw[3];
+  const int c = particle_cell(pos, i, G, L, half_l, inv_l, pw);
+  pos[3 * i] = pw[0];
+  pos[3 * i + 1] = pw[1];
+  pos[3 * i + 2] = pw[2];
+  cell_of[i] = c;
+  tmp[atomicAdd(&wr[c], 1)] = i;
+}
+
+// Places each particle at its cell's start plus the number of smaller
+// indices in its cell (the stable order), one thread per particle.
+__global__ void k_grid_rank(int n, const int32_t *__restrict__ cell_of,
+                            const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+                            const int32_t *__restrict__ tmp, int32_t *__restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_of[i];
+  const int b = start[c], e = b + count[c];
+  int r = b;
+  for (int q = b; q < e; ++q) r += tmp[q] < i ? 1 : 0;
+  idx[r] = i;
+}
+
 __global__ void k_copy_cells(const int32_t *__restrict__ s, int32_t *__restrict__ d,
                              int C) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -309,7 +339,17 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
                        cell_start, ctx->cell_write, C);
     FOS_CHECK_LAUNCH("k_copy_cells");
   }
-  return fos_grid_scatter(pos, n, G, L, ctx->cell_write, cell_indices, stream);
+  if (n == 0) return 0;
+  FOS_REQUIRE(pos && cell_indices, "fos_grid_build: NULL pos or cell_indices");
+  const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
+  hipLaunchKernelGGL(k_grid_scatter_tmp, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, n, G,
+                     L, half_l, inv_l, ctx->cell_write, ctx->cell_of, ctx->grid_tmp);
+  FOS_CHECK_LAUNCH("k_grid_scatter_tmp");
+  hipLaunchKernelGGL(k_grid_rank, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+                     (const int32_t *)ctx->cell_of, (const int32_t *)cell_start,
+                     (const int32_t *)cell_count, (const int32_t *)ctx->grid_tmp, cell_indices);
+  FOS_CHECK_LAUNCH("k_grid_rank");
+  return 0;
 }
 
 }  // extern "C"

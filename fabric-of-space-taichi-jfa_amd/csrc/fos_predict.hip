@@ -35,12 +35,16 @@ __device__ __forceinline__ float dot3f(float x, float y, float z) {
 // cells around i's cell (dx, dy, dz order, ascending index inside a cell),
 // first `cap` kept.  Needs a hash cell of at least mult * 2 * r_max.
 // ---------------------------------------------------------------------------
-__global__ void k_pred_cand(const float *__restrict__ pos, const float *__restrict__ rad, int n,
-                            int G, float L, float half_l, float inv_l, float mult, int cap,
-                            const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
-                            const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
-                            int32_t *__restrict__ ncand, int32_t *__restrict__ trunc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256)
+    k_pred_cand(const float *__restrict__ pos, const float *__restrict__ rad, int n, int G,
+                float L, float half_l, float inv_l, float mult, int cap,
+                const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+                const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
+                int32_t *__restrict__ ncand, int32_t *__restrict__ trunc) {
+  // one wave per particle: lane l tests the l-th occupant of a cell, and a
+  // ballot keeps the hits in occupant order
+  const int lane = (int)(threadIdx.x & 63);
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= n) return;
   const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2], ri = rad[i];
   const int cx = wrap_fast(centre_axis(px, G, L, half_l, inv_l), G);
@@ -48,108 +52,210 @@ __global__ void k_pred_cand(const float *__restrict__ pos, const float *__restri
   const int cz = wrap_fast(centre_axis(pz, G, L, half_l, inv_l), G);
   int c = 0, tr = 0;
   int32_t *row = cand + (int64_t)i * cap;
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int dx = -1; dx <= 1; ++dx)
     for (int dy = -1; dy <= 1; ++dy)
       for (int dz = -1; dz <= 1; ++dz) {
         const int id = (wrap1(cx + dx, G) * G + wrap1(cy + dy, G)) * G + wrap1(cz + dz, G);
         const int b = cs[id], e = b + cc[id];
-        for (int k = b; k < e; ++k) {
-          const int j = ci[k];
-          if (j == i) continue;
-          const float ddx = gwrap(pos[3 * j] - px, L, inv_l);
-          const float ddy = gwrap(pos[3 * j + 1] - py, L, inv_l);
-          const float ddz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
-          const float th = mult * (ri + rad[j]);
-          if (dot3f(ddx, ddy, ddz) < th * th) {
-            if (c < cap)
-              row[c++] = j;
-            else
-              tr = 1;
+        for (int k0 = b; k0 < e; k0 += 64) {
+          const int k = k0 + lane;
+          bool hit = false;
+          int j = -1;
+          if (k < e) {
+            j = ci[k];
+            if (j != i) {
+              const float ddx = gwrap(pos[3 * j] - px, L, inv_l);
+              const float ddy = gwrap(pos[3 * j + 1] - py, L, inv_l);
+              const float ddz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
+              const float th = mult * (ri + rad[j]);
+              hit = dot3f(ddx, ddy, ddz) < th * th;
+            }
           }
+          const unsigned long long m = __ballot(hit);
+          if (hit) {
+            const int slot = c + __popcll(m & below);
+            if (slot < cap) row[slot] = j;
+          }
+          c += __popcll(m);
         }
       }
-  for (int s = c; s < cap; ++s) row[s] = -1;
-  ncand[i] = c;
-  trunc[i] = tr;
+  if (c > cap) {
+    tr = 1;
+    c = cap;
+  }
+  for (int s2 = c + lane; s2 < cap; s2 += 64) row[s2] = -1;
+  if (lane == 0) {
+    ncand[i] = c;
+    trunc[i] = tr;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Features of the directed pair (i, j) = (i, cand[i][s]).  With d = pj - pi
 // (minimum image), the power plane of i and j cuts the segment at
 // t = (|d|^2 + ri^2 - rj^2) / (2 |d|^2); m = pi + t d has equal power
-// p(m) = |m - pi|^2 - ri^2 for i and j.  A candidate k of i (or of j) with
+// p(m) = |m - pi|^2 - ri^2 for i and j.  A candidate k of i with
 // |m - pk|^2 - rk^2 < p(m) owns m, so it may cut the face off.
+// The face itself is estimated as a square of half-size clip_r * (ri + rj)
+// around m in the power plane, clipped by the half-planes p_i(x) <= p_k(x)
+// of the PRED_NEAR candidates k of i whose lines pass closest to m; the area
+// comes from a half-plane intersection with fixed trip counts.  A voxelised
+// face of area A and unit normal n crosses about A (|nx| + |ny| + |nz|) /
+// voxel^2 voxel faces, so A (|nx| + |ny| + |nz|) / (1.5 (ri + rj)^2)
+// estimates the JFA's face score (jfa.py:379-397 expected_area).
 //   0 |d| / (ri + rj)         6 witnesses among i's candidates / 4 (<= 8)
-//   1 ri / r_mean             7 min witness margin on i's side / (ri + rj)^2
-//   2 rj / r_mean             8 witnesses among j's candidates / 4 (<= 8)
-//   3 t                       9 min witness margin on j's side / (ri + rj)^2
-//   4 |cand(i)| / 40         10 (ri - rj) / (ri + rj)
-//   5 |cand(j)| / 40         11 p(m) / (ri + rj)^2
-// Margins and p(m) are clamped to [-1, 1]; empty slots get all zeros.
+//   1 ri / r_mean             7 min witness margin / (ri + rj)^2
+//   2 rj / r_mean             8 |cand(i)| / 40
+//   3 t                       9 |cand(j)| / 40
+//   4 estimated score (<= 4) 10 (ri - rj) / (ri + rj)
+//   5 face area / (ri + rj)^2 (<= 4)   11 p(m) / (ri + rj)^2
+// Margins and p(m) are clamped to [-1, 1].  One workgroup per particle i:
+// i's candidates (relative positions, |pk - pi|^2 - rk^2) are staged in LDS
+// and each thread scores one slot against them.  The candidate relation is
+// symmetric, so only slots with j > i are scored (the pair's lower row);
+// other slots are not written.
 // ---------------------------------------------------------------------------
-__global__ void k_pred_feat(const float *__restrict__ pos, const float *__restrict__ rad, int n,
-                            float L, float inv_l, float inv_rbar,
-                            const int32_t *__restrict__ cand, const int32_t *__restrict__ ncand,
-                            int cap, float *__restrict__ feat) {
-  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= (int64_t)n * cap) return;
-  const int i = (int)(slot / cap), s = (int)(slot - (int64_t)i * cap);
-  float f[PRED_FEAT];
-#pragma unroll
-  for (int q = 0; q < PRED_FEAT; ++q) f[q] = 0.0f;
+#define PRED_NEAR 8   // planes kept for the face polygon (nearest to m)
+#define PRED_TPB 128  // threads per particle: covers cap <= 128
+
+__global__ void __launch_bounds__(PRED_TPB)
+    k_pred_feat(const float *__restrict__ pos, const float *__restrict__ rad, int n, float L,
+                float inv_l, float inv_rbar, float clip_r, const int32_t *__restrict__ cand,
+                const int32_t *__restrict__ ncand, int cap, float *__restrict__ feat) {
+  __shared__ float s_x[PRED_TPB], s_y[PRED_TPB], s_z[PRED_TPB], s_p[PRED_TPB];
+  const int i = blockIdx.x;
+  const int s = (int)threadIdx.x;
   const int ni = ncand[i];
+  const float pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
   if (s < ni) {
-    const int j = cand[slot];
-    const float pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
-    const float pjx = pos[3 * j], pjy = pos[3 * j + 1], pjz = pos[3 * j + 2];
-    const float ri = rad[i], rj = rad[j];
-    const float dx = gwrap(pjx - pix, L, inv_l), dy = gwrap(pjy - piy, L, inv_l),
-                dz = gwrap(pjz - piz, L, inv_l);
-    const float d2 = fmaxf(dot3f(dx, dy, dz), 1e-20f);
-    const float sr = ri + rj, sr2 = sr * sr;
-    const float t = (d2 + ri * ri - rj * rj) / (2.0f * d2);
-    const float mx = t * dx, my = t * dy, mz = t * dz;  // m relative to pi
-    const float pm = t * t * d2 - ri * ri;
-    int wi = 0, wj = 0;
-    float mi = 1e30f, mj = 1e30f;
-    for (int q = 0; q < ni; ++q) {
-      const int k = cand[(int64_t)i * cap + q];
-      if (k == j) continue;
-      const float kx = gwrap(pos[3 * k] - pix, L, inv_l) - mx;
-      const float ky = gwrap(pos[3 * k + 1] - piy, L, inv_l) - my;
-      const float kz = gwrap(pos[3 * k + 2] - piz, L, inv_l) - mz;
-      const float rk = rad[k];
-      const float w = dot3f(kx, ky, kz) - rk * rk - pm;
-      wi += w < 0.0f ? 1 : 0;
-      mi = fminf(mi, w);
-    }
-    const int nj = ncand[j];
-    const float qx = mx - dx, qy = my - dy, qz = mz - dz;  // m relative to pj
-    for (int q = 0; q < nj; ++q) {
-      const int k = cand[(int64_t)j * cap + q];
-      if (k == i) continue;
-      const float kx = gwrap(pos[3 * k] - pjx, L, inv_l) - qx;
-      const float ky = gwrap(pos[3 * k + 1] - pjy, L, inv_l) - qy;
-      const float kz = gwrap(pos[3 * k + 2] - pjz, L, inv_l) - qz;
-      const float rk = rad[k];
-      const float w = dot3f(kx, ky, kz) - rk * rk - pm;
-      wj += w < 0.0f ? 1 : 0;
-      mj = fminf(mj, w);
-    }
-    f[0] = sqrtf(d2) / sr;
-    f[1] = ri * inv_rbar;
-    f[2] = rj * inv_rbar;
-    f[3] = t;
-    f[4] = (float)ni * (1.0f / 40.0f);
-    f[5] = (float)nj * (1.0f / 40.0f);
-    f[6] = (float)min(wi, 8) * 0.25f;
-    f[7] = fminf(1.0f, fmaxf(-1.0f, mi / sr2));
-    f[8] = (float)min(wj, 8) * 0.25f;
-    f[9] = fminf(1.0f, fmaxf(-1.0f, mj / sr2));
-    f[10] = (ri - rj) / sr;
-    f[11] = fminf(1.0f, fmaxf(-1.0f, pm / sr2));
+    const int k = cand[(int64_t)i * cap + s];
+    const float kx = gwrap(pos[3 * k] - pix, L, inv_l);
+    const float ky = gwrap(pos[3 * k + 1] - piy, L, inv_l);
+    const float kz = gwrap(pos[3 * k + 2] - piz, L, inv_l);
+    const float rk = rad[k];
+    s_x[s] = kx;
+    s_y[s] = ky;
+    s_z[s] = kz;
+    s_p[s] = dot3f(kx, ky, kz) - rk * rk;
   }
-  float4 *o = (float4 *)(feat + slot * PRED_FEAT);
+  __syncthreads();
+  if (s >= ni) return;
+  const int j = cand[(int64_t)i * cap + s];
+  if (j < i) return;  // each unordered pair is scored once, in its lower row
+  const float ri = rad[i], rj = rad[j];
+  const float dx = s_x[s], dy = s_y[s], dz = s_z[s];
+  const float d2 = fmaxf(dot3f(dx, dy, dz), 1e-20f);
+  const float sr = ri + rj, sr2 = sr * sr;
+  const float t = (d2 + ri * ri - rj * rj) / (2.0f * d2);
+  const float pm = t * t * d2 - ri * ri;
+  const float ri2 = ri * ri;
+  // orthonormal basis (u, v) of the power plane
+  const float dl = sqrtf(d2);
+  const float nx = dx / dl, ny = dy / dl, nz = dz / dl;
+  float ux, uy, uz;
+  if (fabsf(nx) < 0.9f) {
+    ux = 1.0f - nx * nx; uy = -nx * ny; uz = -nx * nz;
+  } else {
+    ux = -ny * nx; uy = 1.0f - ny * ny; uz = -ny * nz;
+  }
+  const float ul = rsqrtf(dot3f(ux, uy, uz));
+  ux *= ul; uy *= ul; uz *= ul;
+  const float vx = ny * uz - nz * uy, vy = nz * ux - nx * uz, vz = nx * uy - ny * ux;
+  // nearest PRED_NEAR planes (unit normal in the plane, offset), ascending
+  float pa[PRED_NEAR], pb[PRED_NEAR], pc[PRED_NEAR];
+#pragma unroll
+  for (int h = 0; h < PRED_NEAR; ++h) {
+    pa[h] = 1.0f; pb[h] = 0.0f; pc[h] = 1e30f;
+  }
+  bool empty = false;
+  int wit = 0;
+  float mw = 1e30f;
+  const float mm = t * t * d2 - pm;  // |m|^2 - p(m) = ri^2
+  for (int q = 0; q < ni; ++q) {
+    if (q == s) continue;
+    const float kx = s_x[q], ky = s_y[q], kz = s_z[q], pk = s_p[q];
+    const float kd = kx * dx + ky * dy + kz * dz;
+    // |pk - m|^2 - rk^2 - p(m), with m = t d
+    const float wm = pk - 2.0f * t * kd + mm;
+    wit += wm < 0.0f ? 1 : 0;
+    mw = fminf(mw, wm);
+    // p_i(x) <= p_k(x) on the plane, x = m + a u + b v:  A a + B b <= C
+    const float A = 2.0f * (kx * ux + ky * uy + kz * uz);
+    const float B = 2.0f * (kx * vx + ky * vy + kz * vz);
+    const float C = pk + ri2 - 2.0f * t * kd;
+    const float q2 = A * A + B * B;
+    if (q2 < 1e-30f) {
+      empty |= C < 0.0f;
+      continue;
+    }
+    const float inv = rsqrtf(q2);
+    float ic = C * inv;
+    if (!(ic < pc[PRED_NEAR - 1])) continue;
+    float ia = A * inv, ib = B * inv;
+#pragma unroll
+    for (int h = 0; h < PRED_NEAR; ++h) {
+      const bool sw = ic < pc[h];
+      const float ta = pa[h], tb = pb[h], tc = pc[h];
+      pa[h] = sw ? ia : ta; pb[h] = sw ? ib : tb; pc[h] = sw ? ic : tc;
+      ia = sw ? ta : ia; ib = sw ? tb : ib; ic = sw ? tc : ic;
+    }
+  }
+  // Area of {x : a_h x <= c_h} over the kept planes and the four sides of a
+  // square of half-size clip_r*(ri+rj): each line contributes the edge it
+  // keeps after clipping by all the others, summed with the shoelace formula
+  // (lines traversed with the interior on the left).
+  const float R0 = clip_r * sr;
+  float la[PRED_NEAR + 4], lb[PRED_NEAR + 4], lc[PRED_NEAR + 4];
+#pragma unroll
+  for (int h = 0; h < PRED_NEAR; ++h) {
+    la[h] = pa[h]; lb[h] = pb[h]; lc[h] = pc[h];
+  }
+  la[PRED_NEAR] = 1.0f;  lb[PRED_NEAR] = 0.0f;  lc[PRED_NEAR] = R0;
+  la[PRED_NEAR + 1] = -1.0f; lb[PRED_NEAR + 1] = 0.0f; lc[PRED_NEAR + 1] = R0;
+  la[PRED_NEAR + 2] = 0.0f; lb[PRED_NEAR + 2] = 1.0f;  lc[PRED_NEAR + 2] = R0;
+  la[PRED_NEAR + 3] = 0.0f; lb[PRED_NEAR + 3] = -1.0f; lc[PRED_NEAR + 3] = R0;
+  float area = 0.0f;
+#pragma unroll
+  for (int h = 0; h < PRED_NEAR + 4; ++h) {
+    const float x0 = la[h] * lc[h], y0 = lb[h] * lc[h];  // closest point to m
+    const float tx = -lb[h], ty = la[h];
+    float lo = -1e30f, hi = 1e30f;
+#pragma unroll
+    for (int g = 0; g < PRED_NEAR + 4; ++g) {
+      if (g == h) continue;
+      const float kk = la[g] * tx + lb[g] * ty;
+      const float rr = lc[g] - (la[g] * x0 + lb[g] * y0);
+      if (kk > 1e-12f)
+        hi = fminf(hi, rr / kk);
+      else if (kk < -1e-12f)
+        lo = fmaxf(lo, rr / kk);
+      else if (rr < 0.0f || (rr == 0.0f && g < h))
+        hi = -1e30f;  // parallel and tighter (or an equal line met first)
+    }
+    if (lc[h] < 1e29f && lo < hi) {
+      const float x1 = x0 + lo * tx, y1 = y0 + lo * ty;
+      const float x2 = x0 + hi * tx, y2 = y0 + hi * ty;
+      area += x1 * y2 - x2 * y1;
+    }
+  }
+  area = empty ? 0.0f : fmaxf(0.0f, 0.5f * area);
+  const float na = area / sr2;
+  float f[PRED_FEAT];
+  f[0] = dl / sr;
+  f[1] = ri * inv_rbar;
+  f[2] = rj * inv_rbar;
+  f[3] = t;
+  f[4] = fminf(4.0f, na * (fabsf(nx) + fabsf(ny) + fabsf(nz)) / 1.5f);
+  f[5] = fminf(4.0f, na);
+  f[6] = (float)min(wit, 8) * 0.25f;
+  f[7] = fminf(1.0f, fmaxf(-1.0f, mw / sr2));
+  f[8] = (float)ni * (1.0f / 40.0f);
+  f[9] = (float)ncand[j] * (1.0f / 40.0f);
+  f[10] = (ri - rj) / sr;
+  f[11] = fminf(1.0f, fmaxf(-1.0f, pm / sr2));
+  f[12] = f[13] = f[14] = f[15] = 0.0f;
+  float4 *o = (float4 *)(feat + ((int64_t)i * cap + s) * PRED_FEAT);
 #pragma unroll
   for (int q = 0; q < PRED_FEAT / 4; ++q)
     o[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
@@ -227,9 +333,11 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// FSC from the scores: fsc[i] = kept pairs of row i (logit > 0); face rows
-// list the kept partners j > i in candidate order, at most 32, -1 padded
-// (the row shape accumulate_face_area produces for the pressure step).
+// FSC from the scores: a pair (i, j) is kept when the logit of its lower
+// row's slot is > 0 (sigmoid 0.5); fsc[i] counts kept pairs over i's row,
+// looking up i's slot in j's row for j < i.  Face rows list the kept
+// partners j > i in candidate order, at most 32, -1 padded (the row shape
+// accumulate_face_area produces for the pressure step).
 __global__ void k_pred_fsc(const float *__restrict__ logit, const int32_t *__restrict__ cand,
                            const int32_t *__restrict__ ncand, int n, int cap,
                            int32_t *__restrict__ fsc, int32_t *__restrict__ face_ids,
@@ -239,14 +347,24 @@ __global__ void k_pred_fsc(const float *__restrict__ logit, const int32_t *__res
   const int ni = ncand[i];
   int c = 0, r = 0;
   for (int s = 0; s < ni; ++s) {
-    if (logit[(int64_t)i * cap + s] > 0.0f) {
-      c += 1;
-      const int j = cand[(int64_t)i * cap + s];
-      if (face_ids && j > i && r < FOS_MAX_NEIGHBORS) {
-        face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + r] = j;
-        if (face_counts) face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + r] = 1;
-        r += 1;
-      }
+    const int j = cand[(int64_t)i * cap + s];
+    bool keep = false;
+    if (j > i) {
+      keep = logit[(int64_t)i * cap + s] > 0.0f;
+    } else {
+      const int nj = ncand[j];
+      for (int q = 0; q < nj; ++q)
+        if (cand[(int64_t)j * cap + q] == i) {
+          keep = logit[(int64_t)j * cap + q] > 0.0f;
+          break;
+        }
+    }
+    if (!keep) continue;
+    c += 1;
+    if (face_ids && j > i && r < FOS_MAX_NEIGHBORS) {
+      face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + r] = j;
+      if (face_counts) face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + r] = 1;
+      r += 1;
     }
   }
   fsc[i] = c;
@@ -257,15 +375,16 @@ __global__ void k_pred_fsc(const float *__restrict__ logit, const int32_t *__res
     }
 }
 
-// Labels for training: 1 when j is in i's JFA neighbour list.
+// Labels for training: 1 when j is in i's JFA neighbour list, for the
+// scored slots (j > i); -1 for empty slots and slots with j < i.
 __global__ void k_pred_labels(const int32_t *__restrict__ cand, const int32_t *__restrict__ ncand,
                               int n, int cap, const int32_t *__restrict__ nl,
                               const int32_t *__restrict__ nc, float *__restrict__ y) {
   const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= (int64_t)n * cap) return;
   const int i = (int)(slot / cap), s = (int)(slot - (int64_t)i * cap);
-  float v = -1.0f;  // empty slot
-  if (s < ncand[i]) {
+  float v = -1.0f;  // empty or unscored slot
+  if (s < ncand[i] && cand[slot] > i) {
     const int j = cand[slot];
     const int m = min(nc[i], FOS_MAX_NEIGHBORS);
     v = 0.0f;
@@ -278,8 +397,8 @@ __global__ void k_pred_labels(const int32_t *__restrict__ cand, const int32_t *_
 // ---------------------------------------------------------------------------
 // Training (FP32).  One step over a batch of B samples (indices into the
 // feature/label arrays): forward keeps H1, H2 and dL/dz3; backward forms
-// dZ2, dZ1; the gradient kernel sums per parameter over the batch (one
-// thread per parameter, fixed order, so steps are reproducible); Adam.
+// dZ2, dZ1; the gradient kernels sum per parameter over the batch in a
+// fixed order; Adam.
 // Loss: class-weighted binary cross entropy on the logit.
 // ---------------------------------------------------------------------------
 __global__ void k_mlp_fwd(const float *__restrict__ feat, const float *__restrict__ y,
@@ -341,33 +460,47 @@ __global__ void k_mlp_bwd(const float *__restrict__ H1, const float *__restrict_
   }
 }
 
+// Parameter gradients: block (x, y) sums parameters 64x..64x+63 over batch
+// slice y (fixed order); k_mlp_grad_sum adds the slices in order, so a step
+// is reproducible bit for bit.
+#define MLP_GRAD_SPLIT 32
 __global__ void k_mlp_grad(const float *__restrict__ feat, const int32_t *__restrict__ idx, int B,
                            const float *__restrict__ H1, const float *__restrict__ H2,
                            const float *__restrict__ dz3, const float *__restrict__ dZ1,
-                           const float *__restrict__ dZ2, float *__restrict__ grad) {
+                           const float *__restrict__ dZ2, float *__restrict__ part) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= MLP_NPARAM) return;
+  const int per = (B + MLP_GRAD_SPLIT - 1) / MLP_GRAD_SPLIT;
+  const int b0 = blockIdx.y * per, b1 = min(B, b0 + per);
   float s = 0.0f;
   if (q < MLP_OFF_B1) {  // W1[f][a]
     const int f = q / MLP_H, a = q % MLP_H;
     if (f < 12)
-      for (int b = 0; b < B; ++b)
+      for (int b = b0; b < b1; ++b)
         s += feat[(int64_t)idx[b] * PRED_FEAT + f] * dZ1[(int64_t)b * MLP_H + a];
   } else if (q < MLP_OFF_W2) {
     const int a = q - MLP_OFF_B1;
-    for (int b = 0; b < B; ++b) s += dZ1[(int64_t)b * MLP_H + a];
+    for (int b = b0; b < b1; ++b) s += dZ1[(int64_t)b * MLP_H + a];
   } else if (q < MLP_OFF_B2) {
     const int r = q - MLP_OFF_W2, a = r / MLP_H, h = r % MLP_H;
-    for (int b = 0; b < B; ++b) s += H1[(int64_t)b * MLP_H + a] * dZ2[(int64_t)b * MLP_H + h];
+    for (int b = b0; b < b1; ++b) s += H1[(int64_t)b * MLP_H + a] * dZ2[(int64_t)b * MLP_H + h];
   } else if (q < MLP_OFF_W3) {
     const int h = q - MLP_OFF_B2;
-    for (int b = 0; b < B; ++b) s += dZ2[(int64_t)b * MLP_H + h];
+    for (int b = b0; b < b1; ++b) s += dZ2[(int64_t)b * MLP_H + h];
   } else if (q < MLP_OFF_B3) {
     const int h = q - MLP_OFF_W3;
-    for (int b = 0; b < B; ++b) s += dz3[b] * H2[(int64_t)b * MLP_H + h];
+    for (int b = b0; b < b1; ++b) s += dz3[b] * H2[(int64_t)b * MLP_H + h];
   } else {
-    for (int b = 0; b < B; ++b) s += dz3[b];
+    for (int b = b0; b < b1; ++b) s += dz3[b];
   }
+  part[(int64_t)blockIdx.y * MLP_NPARAM + q] = s;
+}
+
+__global__ void k_mlp_grad_sum(const float *__restrict__ part, float *__restrict__ grad) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= MLP_NPARAM) return;
+  float s = 0.0f;
+  for (int k = 0; k < MLP_GRAD_SPLIT; ++k) s += part[(int64_t)k * MLP_NPARAM + q];
   grad[q] = s;
 }
 
@@ -395,7 +528,8 @@ int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float 
                               ncand && truncated)),
               "fos_pred_candidates: bad arguments (n=%d G=%d cap=%d)", n, G, cap);
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_pred_cand, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(k_pred_cand, dim3(blocks_for((int64_t)n * 64)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream,
                      pos, rad, n, G, L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), mult,
                      cap, cell_start, cell_count, cell_indices, cand, ncand, truncated);
   FOS_CHECK_LAUNCH("k_pred_cand");
@@ -403,16 +537,16 @@ int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float 
 }
 
 int fos_pred_features(const float *pos, const float *rad, int n, float L, float r_mean,
-                      const int32_t *cand, const int32_t *ncand, int cap, float *feat,
-                      void *stream) {
-  FOS_REQUIRE(n >= 0 && L > 0.0f && cap >= 1 && r_mean > 0.0f &&
+                      float clip_r, const int32_t *cand, const int32_t *ncand, int cap,
+                      float *feat, void *stream) {
+  FOS_REQUIRE(n >= 0 && L > 0.0f && cap >= 1 && r_mean > 0.0f && clip_r > 0.0f &&
                   (n == 0 || (pos && rad && cand && ncand && feat)),
               "fos_pred_features: bad arguments (n=%d cap=%d)", n, cap);
   if (n == 0) return 0;
-  const int64_t slots = (int64_t)n * cap;
-  hipLaunchKernelGGL(k_pred_feat, dim3(blocks_for(slots)), dim3(FOS_BLOCK), 0,
-                     (hipStream_t)stream, pos, rad, n, L, (float)(1.0 / (double)L),
-                     1.0f / r_mean, cand, ncand, cap, feat);
+  FOS_REQUIRE(cap <= PRED_TPB, "fos_pred_features: cap %d exceeds %d", cap, PRED_TPB);
+  hipLaunchKernelGGL(k_pred_feat, dim3((unsigned)n), dim3(PRED_TPB), 0, (hipStream_t)stream, pos,
+                     rad, n, L, (float)(1.0 / (double)L), 1.0f / r_mean, clip_r, cand, ncand, cap,
+                     feat);
   FOS_CHECK_LAUNCH("k_pred_feat");
   return 0;
 }
@@ -475,16 +609,19 @@ int fos_mlp_train_step(float *params, float *grad, float *adam_m, float *adam_v,
               "fos_mlp_train_step: bad arguments (B=%d step=%d)", B, step);
   hipStream_t st = (hipStream_t)stream;
   float *H1 = scratch, *H2 = H1 + (size_t)B * MLP_H, *dZ1 = H2 + (size_t)B * MLP_H,
-        *dZ2 = dZ1 + (size_t)B * MLP_H, *dz3 = dZ2 + (size_t)B * MLP_H;
+        *dZ2 = dZ1 + (size_t)B * MLP_H, *dz3 = dZ2 + (size_t)B * MLP_H, *part = dz3 + B;
   hipLaunchKernelGGL(k_mlp_fwd, dim3(blocks_for(B, 64)), dim3(64), 0, st, feat, labels,
                      batch_idx, B, (const float *)params, pos_weight, H1, H2, dz3, loss);
   FOS_CHECK_LAUNCH("k_mlp_fwd");
   hipLaunchKernelGGL(k_mlp_bwd, dim3(blocks_for(B, 64)), dim3(64), 0, st, H1, H2, dz3, B,
                      (const float *)params, dZ1, dZ2);
   FOS_CHECK_LAUNCH("k_mlp_bwd");
-  hipLaunchKernelGGL(k_mlp_grad, dim3(blocks_for(MLP_NPARAM, 64)), dim3(64), 0, st, feat,
-                     batch_idx, B, H1, H2, dz3, dZ1, dZ2, grad);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(blocks_for(MLP_NPARAM, 64), MLP_GRAD_SPLIT), dim3(64), 0,
+                     st, feat, batch_idx, B, H1, H2, dz3, dZ1, dZ2, part);
   FOS_CHECK_LAUNCH("k_mlp_grad");
+  hipLaunchKernelGGL(k_mlp_grad_sum, dim3(blocks_for(MLP_NPARAM)), dim3(FOS_BLOCK), 0, st,
+                     (const float *)part, grad);
+  FOS_CHECK_LAUNCH("k_mlp_grad_sum");
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
   hipLaunchKernelGGL(k_adam, dim3(blocks_for(MLP_NPARAM)), dim3(FOS_BLOCK), 0, st, params,

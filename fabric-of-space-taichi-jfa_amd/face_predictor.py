@@ -28,7 +28,8 @@ OFF_W3 = OFF_B2 + H
 OFF_B3 = OFF_W3 + H
 NPARAM = OFF_B3 + 1
 MULT = 1.6
-CAP = 64
+CAP = 96
+CLIP = 1.5      # the face polygon starts as a square of half-size CLIP*(ri+rj)
 
 
 def param_count():
@@ -100,7 +101,8 @@ def mlp_reference(feat, p):
 
 
 def hash_res(L, r_max, mult=MULT):
-    """Cells of at least mult * 2 * r_max (27-cell stencil reaches every candidate)."""
+    """Cells of at least mult * 2 * r_max (the 27-cell stencil reaches every
+    candidate)."""
     return max(3, int(math.floor(L / (mult * 2.0 * r_max))))
 
 
@@ -143,7 +145,8 @@ class FacePredictor:
         if r_mean is None:
             r_mean = float(rad[:n].mean().item())
         fl.call("fos_pred_features", fl.ptr(self.pos), fl.ptr(rad), n, self.L, float(r_mean),
-                fl.ptr(self.cand), fl.ptr(self.ncand), self.cap, fl.ptr(self.feat), fl.stream())
+                CLIP, fl.ptr(self.cand), fl.ptr(self.ncand), self.cap, fl.ptr(self.feat),
+                fl.stream())
 
     def logits(self, n):
         pk = self.packed
@@ -186,7 +189,8 @@ class Trainer:
         self.m = torch.zeros_like(self.p)
         self.v = torch.zeros_like(self.p)
         self.B, self.lr, self.pos_weight = int(batch), float(lr), float(pos_weight)
-        self.scratch = torch.empty(4 * H * self.B + self.B, dtype=torch.float32, device=d)
+        self.scratch = torch.empty(4 * H * self.B + self.B + 32 * NPARAM, dtype=torch.float32,
+                                   device=d)
         self.loss = torch.zeros(1, dtype=torch.float32, device=d)
         self.t = 0
 

@@ -78,6 +78,22 @@ class FoamSim:
         # optional list of ctypes arrays of passes+1 hipEvent_t handles; each
         # JFA frame consumes one to time its pass kernels on the stream
         self.pass_events = None
+        # optional callback(sim) after every JFA (training-data collection)
+        self.on_jfa = None
+        # learned face predictor (cfg.judge_every > 0): set_predictor()
+        self.predictor = None
+        self.fallback = False
+        self.judged = []          # (frame, fsc error, mean-FSC error) per judge frame
+        self.pred_fsc = torch.zeros(n, dtype=torch.int32, device=dev)
+
+    def set_predictor(self, params, mult=None):
+        """Enables the predictor path with fp32 MLP parameters (numpy) and the
+        candidate radius multiple they were trained with."""
+        from face_predictor import FacePredictor
+        c = self.cfg
+        self.predictor = FacePredictor(params, self.n, c.L, c.r_max, device=self.pos.device,
+                                       mult=c.pred_mult if mult is None else mult)
+        self.r_scale = 0.5 * (c.r_min + c.r_max)
 
     # -- building blocks ------------------------------------------------------
     def rebuild_grid(self):
@@ -163,14 +179,36 @@ class FoamSim:
                     fc.f32(fc.MAX_DRIFT_FRACTION * c.gap_fraction), c.L, c.seed, f, st)
         # 7
         self.rebuild_grid()
-        # 8 JFA on cadence frames
-        ran = c.jfa_runs(f)
-        if ran:
-            self.run_jfa()
+        # 8 JFA on cadence frames (or the predictor between judge frames)
+        if self.predictor is not None and c.judge_every > 0:
+            ran = self.fallback or f % c.judge_every == 0
+            if ran:
+                self.run_jfa()
+                self.predictor.predict(self.pos, self.rad, self.n, self.pred_fsc,
+                                       r_mean=self.r_scale)
+                from face_predictor import fsc_error
+                err, bias = fsc_error(self.pred_fsc, self.fsc)
+                # the judge compares the frame's mean FSC (bias); the
+                # per-particle error is recorded beside it
+                self.fallback = bias > c.max_fsc_err
+                self.judged.append((f, err, bias))
+            else:
+                self.predictor.predict(self.pos, self.rad, self.n, self.fsc, self.face_ids,
+                                       self.face_counts, r_mean=self.r_scale)
+                fl.call("fos_fsc_ema", self.n, c.ema_alpha, fl.ptr(self.fsc),
+                        fl.ptr(self.fsc_ema), st)
             f0 = float((self.fsc == 0).sum().item()) / float(max(1, self.n))
             self.growth = not ((f < c.warmstart_frames) or (f0 > c.fsc_zero_rate))
         else:
-            self.growth = True
+            ran = c.jfa_runs(f)
+            if ran:
+                self.run_jfa()
+                if self.on_jfa is not None:
+                    self.on_jfa(self)
+                f0 = float((self.fsc == 0).sum().item()) / float(max(1, self.n))
+                self.growth = not ((f < c.warmstart_frames) or (f0 > c.fsc_zero_rate))
+            else:
+                self.growth = True
         # 9 controller
         if self.growth:
             fl.call("fos_fsc_targets", fl.ptr(self.rad), fl.ptr(self.rad_target), fl.ptr(self.fsc),

@@ -11,6 +11,8 @@ the benchmark workloads:
   c2   N=10,000, 128^3, 200 frames, JFA every frame (headline)
   c2c  as c2 with JFA on a 1/5 cadence after a 20-frame warm start
   c3   N=200,000 clustered, 384^3, 50 frames
+  c5   N=50,000, 256^3, radii U[0.005, 0.011] (learned face predictor workload)
+  c5s  as c5 with the predictor between JFA judge frames (every 8th)
 """
 import dataclasses
 import math
@@ -65,8 +67,16 @@ JITTER_ENABLED = True          # config.py:493
 JITTER_RMS = 0.10              # config.py:494
 JITTER_TAU = 1.0               # config.py:496
 MAX_DRIFT_FRACTION = 0.20      # config.py:498
+TOPO_EMA_ALPHA = 0.1           # config.py:455
+TOPO_BATCHES = 12              # config.py:458
+TOPO_PAIR_SUBSAMPLE_Q = 3      # config.py:459
+TOPO_DEG_LOW = 10              # config.py:463
+TOPO_DEG_HIGH = 18             # config.py:464
 TOPO_MAX_RADIUS_MULTIPLE = 2.2 # config.py:467
 MAX_TOPO_NEIGHBORS = 28        # config.py:468
+TOPO_PAIR_SUBSAMPLING_STRIDE = 3     # config.py:471
+TOPO_EARLY_EXIT_HIGH_DEGREE = True   # config.py:472
+KNN_TOPO_K = 14                # config.py:487
 HALF_L = 0.5 * DOMAIN_SIZE     # config.py:213
 INV_L = 1.0 / DOMAIN_SIZE      # config.py:214
 RNG_SEED = 0                   # Philox key for the noise kernels
@@ -129,6 +139,13 @@ class FoamConfig:
     jfa_every: int = 1
     jfa_warm: int = 0
     frames: int = 20
+    # learned face predictor: JFA (the judge) every `judge_every` frames, the
+    # predictor in between (0 = plain JFA); fall back to the JFA on every
+    # frame while the last judged frame's FSC error exceeds max_fsc_err
+    judge_every: int = 0
+    max_fsc_err: float = 0.02
+    # candidate pairs of the predictor: |d| < pred_mult * (ri + rj)
+    pred_mult: float = 1.6
 
     def passes(self):
         return self.num_passes or int(math.ceil(math.log2(self.res))) + 1
@@ -172,6 +189,20 @@ def preset(name):
         return FoamConfig(name="c3", n=200000, res=384, r_min=0.0015, r_max=0.015,
                           pbd_base=PBD_BASE_PASSES, pbd_max=PBD_MAX_PASSES,
                           f_low=12, f_high=16, pressure_iters=1, frames=50)
+    if name in ("c5", "c5s"):
+        # the learned face predictor's workload: N=50,000, radii U[0.005, 0.011],
+        # JFA 256^3; c5s decimates the JFA to every 8th frame (the judge)
+        cfg = FoamConfig(name=name, n=50000, res=256, r_min=0.004, r_max=0.011,
+                         pbd_base=PBD_BASE_PASSES, pbd_max=12, f_low=12, f_high=16,
+                         pressure_iters=20, brownian=True, brownian_mode=1,
+                         brownian_strength=1e-4, frames=50)
+        # Radii U[0.005, 0.011] fill ~14% of the box at N=50,000, so faces
+        # reach well beyond 1.6 (ri + rj); 3.5 gives the ~40 candidates per
+        # particle the workload is sized for.
+        cfg.pred_mult = 3.5
+        if name == "c5s":
+            cfg.judge_every = 8
+        return cfg
     raise KeyError(name)
 
 

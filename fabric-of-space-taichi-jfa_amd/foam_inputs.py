@@ -46,6 +46,8 @@ def for_config(cfg):
         return uniform(cfg.n, 0.012, 0.024, seed=0)
     if cfg.name == "c3":
         return clustered(cfg.n, seed=0)
+    if cfg.name in ("c5", "c5s"):
+        return uniform(cfg.n, 0.005, 0.011, seed=0)
     raise KeyError(cfg.name)
 
 

@@ -258,6 +258,38 @@ int fos_topo_candidates(const float *pos, const float *rad, int n, int G,
                         const int32_t *cell_indices, int32_t *pairs,
                         int32_t *count, int32_t *truncated, void *stream);
 
+/* topology.py:140-262 gabriel_test_topological_degree: clears deg, then for
+ * every stored candidate pair (pairs[n][cap], pair_count) that passes the
+ * hash subsampling (stride > 1) and the high-degree early exit, adds 1 to
+ * both degrees when no other particle lies strictly inside the diameter
+ * sphere.  cell_size is the configured CELL_SIZE (stencil sizing). */
+int fos_topo_gabriel(const float *pos, int n, int G, float L, float cell_size,
+                     const int32_t *cell_start, const int32_t *cell_count,
+                     const int32_t *cell_indices, const int32_t *pairs, const int32_t *pair_count,
+                     int cap, int frame, int stride, int early_exit, float deg_high,
+                     const float *deg_ema, int32_t *deg, void *stream);
+/* topology.py:335-446 gabriel_update_batched: particles i % num_batches ==
+ * batch_idx rebuild their candidates (j > i, 1/q of pairs by hash) and add
+ * their Gabriel pairs to deg (not cleared).  pair_count counts past cap as
+ * the reference's counter does; only stored pairs are tested. */
+int fos_topo_gabriel_batched(const float *pos, const float *rad, int n, int G, float L,
+                             float cell_size, float mult, int cap, const int32_t *cell_start,
+                             const int32_t *cell_count, const int32_t *cell_indices,
+                             int batch_idx, int num_batches, int q, int frame, int32_t *pairs,
+                             int32_t *pair_count, int32_t *truncated, int32_t *deg,
+                             void *stream);
+/* topology.py:458-504 knn_topo_degree: deg[i] = min(k, neighbours within
+ * (mult*ri)^2 over the 27 cells). */
+int fos_topo_knn(const float *pos, const float *rad, int n, int G, float L, float mult, int k,
+                 const int32_t *cell_start, const int32_t *cell_count,
+                 const int32_t *cell_indices, int32_t *deg, void *stream);
+/* topology.py:266-283 update_topo_ema and :449-455 commit_topo_to_ema:
+ * ema = alpha*deg + (1-alpha)*ema. */
+int fos_topo_ema(const int32_t *deg, float *ema, int n, float alpha, void *stream);
+/* topology.py:287-322 update_radii_topological. */
+int fos_topo_radii(float *rad, const float *deg_eff, int n, float deg_low, float deg_high,
+                   float gain_grow, float gain_shrink, float r_min, float r_max, void *stream);
+
 /* ---- learned face predictor -------------------------------------------
  * No reference counterpart: the reference always runs the JFA
  * (jfa.run_jfa, jfa.py:1056-1197).  These score candidate pairs for a shared
@@ -273,24 +305,27 @@ int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float 
                         int cap, const int32_t *cell_start, const int32_t *cell_count,
                         const int32_t *cell_indices, int32_t *cand, int32_t *ncand,
                         int32_t *truncated, void *stream);
-/* 12 features per candidate slot into feat[n*cap][16] (slots 12-15 and empty
- * candidate slots zero); r_mean is the mean radius. */
+/* 12 features per scored candidate slot (j > i: each unordered pair once,
+ * in its lower row) into feat[n*cap][16] (entries 12-15 zero; other slots
+ * are not written); r_mean normalises the radii, clip_r*(ri+rj) is the
+ * half-size of the square clipped to the face polygon; cap <= 128. */
 int fos_pred_features(const float *pos, const float *rad, int n, float L, float r_mean,
-                      const int32_t *cand, const int32_t *ncand, int cap, float *feat,
-                      void *stream);
+                      float clip_r, const int32_t *cand, const int32_t *ncand, int cap,
+                      float *feat, void *stream);
 /* MLP 12->64->64->1 on MFMA with BF16 weights/activations and FP32
  * accumulation: logit[p] for p < npairs (a multiple of 16).  w1_packed /
  * w2_packed hold the BF16 weights in MFMA lane order (fos_lib.pack_mlp). */
 int fos_pred_mlp(const float *feat, int64_t npairs, const void *w1_packed, const void *w2_packed,
                  const float *b1, const float *b2, const float *w3, float b3, float *logit,
                  void *stream);
-/* fsc[i] = slots of row i with logit > 0; optional face rows (partners j > i
- * in candidate order, at most 32, -1 padded; counts 1) for the pressure
- * step (jfa.py:702-762 row shape). */
+/* fsc[i] = pairs of row i whose lower-row slot has logit > 0; optional face
+ * rows (kept partners j > i in candidate order, at most 32, -1 padded;
+ * counts 1) for the pressure step (jfa.py:702-762 row shape). */
 int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *ncand, int n, int cap,
                  int32_t *fsc, int32_t *face_ids, int32_t *face_counts, void *stream);
-/* Training labels: 1 when cand[i][s] is in i's JFA neighbour list, 0 when
- * not, -1 for an empty slot. */
+/* Training labels of the scored slots (cand[i][s] > i): 1 when cand[i][s]
+ * is in i's JFA neighbour list, 0 when not; -1 for empty and unscored
+ * slots. */
 int fos_pred_labels(const int32_t *cand, const int32_t *ncand, int n, int cap,
                     const int32_t *neighbor_list, const int32_t *neighbor_count, float *labels,
                     void *stream);
@@ -298,8 +333,8 @@ int fos_pred_labels(const int32_t *cand, const int32_t *ncand, int n, int cap,
  * b3) into host *count. */
 int fos_mlp_param_count(int *count);
 /* One Adam step (FP32) on a batch of B sample indices with class-weighted
- * binary cross entropy; scratch holds 4*64*B + B floats; *loss (device) is
- * increased by the batch mean loss. */
+ * binary cross entropy; scratch holds 4*64*B + B + 32*param_count floats;
+ * *loss (device) is increased by the batch mean loss. */
 int fos_mlp_train_step(float *params, float *grad, float *adam_m, float *adam_v, int step,
                        float lr, const float *feat, const float *labels, const int32_t *batch_idx,
                        int B, float pos_weight, float *scratch, float *loss, void *stream);

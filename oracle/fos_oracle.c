@@ -955,3 +955,175 @@ void orc_topo_candidates(const float *pos, const float *rad, int n, int G,
         }
   }
 }
+
+/* ---- topology.py:140-504: Gabriel / kNN degrees, EMA, radius rule ---- */
+
+/* topology.py:188 and :326-333 pair hash (int32 arithmetic wraps). */
+static inline int32_t topo_hash(int i, int j, int frame) {
+  uint32_t h = ((uint32_t)i * 73856093u) ^ ((uint32_t)j * 19349663u) ^
+               ((uint32_t)frame * 83492791u);
+  return (int32_t)h;
+}
+
+/* grid.py:46 wrapP applied to one component. */
+static inline void wrap3(float v[3], float L, float inv_l) {
+  for (int d = 0; d < 3; ++d) v[d] = gwrap(v[d], L, inv_l);
+}
+
+/* topology.py:198-252 witness test of pair (i, j): 1 when no other particle
+ * lies strictly inside the diameter sphere (scaled epsilon). */
+static int gabriel_pair(const float *pos, int i, int j, int G, float L, float cell_size,
+                        const int32_t *cs, const int32_t *cc, const int32_t *ci) {
+  const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
+  const float *pi_p = pos + 3 * i, *pj = pos + 3 * j;
+  float d[3] = {pj[0] - pi_p[0], pj[1] - pi_p[1], pj[2] - pi_p[2]};
+  wrap3(d, L, inv_l);
+  float d2 = d[0] * d[0] + d[1] * d[1];
+  d2 = d2 + d[2] * d[2];
+  if (d2 < 1e-16f) return -1; /* skipped pair */
+  float mid[3];
+  for (int a = 0; a < 3; ++a) mid[a] = pi_p[a] + 0.5f * d[a];
+  wrap3(mid, L, inv_l);
+  const float r_sq = 0.25f * d2;
+  int mc[3];
+  for (int a = 0; a < 3; ++a) {
+    float q = (mid[a] + half_l) * inv_l;
+    mc[a] = wrapi((int)(q * (float)G), G);
+  }
+  const float r_d = sqrtf(d2) * 0.5f;
+  int sc = (int)ceilf(r_d / cell_size);
+  sc = sc < 1 ? 1 : (sc > 3 ? 3 : sc);
+  if (sc > G / 2) sc = G / 2;
+  const float eps = fmaxf(1e-12f, 1e-6f * r_sq);
+  for (int dx = -sc; dx <= sc; ++dx)
+    for (int dy = -sc; dy <= sc; ++dy)
+      for (int dz = -sc; dz <= sc; ++dz) {
+        int id = wrapi(mc[0] + dx, G) * G * G + wrapi(mc[1] + dy, G) * G + wrapi(mc[2] + dz, G);
+        for (int l = cs[id]; l < cs[id] + cc[id]; ++l) {
+          int k = ci[l];
+          if (k == i || k == j) continue;
+          float e[3] = {mid[0] - pos[3 * k], mid[1] - pos[3 * k + 1], mid[2] - pos[3 * k + 2]};
+          wrap3(e, L, inv_l);
+          float dk = e[0] * e[0] + e[1] * e[1];
+          dk = dk + e[2] * e[2];
+          if (dk < r_sq - eps) return 0;
+        }
+      }
+  return 1;
+}
+
+void orc_topo_gabriel(const float *pos, int n, int G, float L, float cell_size,
+                      const int32_t *cs, const int32_t *cc, const int32_t *ci,
+                      const int32_t *pairs, const int32_t *pair_count, int cap, int frame,
+                      int stride, int early_exit, float deg_high, const float *deg_ema,
+                      int32_t *deg) {
+  for (int i = 0; i < n; ++i) deg[i] = 0;
+  for (int i = 0; i < n; ++i) {
+    int pc = pair_count[i] < cap ? pair_count[i] : cap;
+    for (int p = 0; p < pc; ++p) {
+      int j = pairs[(int64_t)i * cap + p];
+      if (stride > 1 && topo_hash(i, j, frame) % stride != 0) continue;
+      if (early_exit && deg_ema[i] >= deg_high && deg_ema[j] >= deg_high) continue;
+      if (gabriel_pair(pos, i, j, G, L, cell_size, cs, cc, ci) == 1 && j > i) {
+        deg[i] += 1;
+        deg[j] += 1;
+      }
+    }
+  }
+}
+
+void orc_topo_gabriel_batched(const float *pos, const float *rad, int n, int G, float L,
+                              float cell_size, float mult, int cap, const int32_t *cs,
+                              const int32_t *cc, const int32_t *ci, int batch_idx,
+                              int num_batches, int q, int frame, int32_t *pairs,
+                              int32_t *pair_count, int32_t *truncated, int32_t *deg) {
+  const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
+  for (int i = 0; i < n; ++i) {
+    if (i % num_batches != batch_idx) continue;
+    int count = 0;
+    truncated[i] = 0;
+    int mc[3];
+    centre_cell(pos + 3 * i, G, L, half_l, inv_l, mc);
+    for (int d = 0; d < 3; ++d) mc[d] = wrapi(mc[d], G);
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          int id = wrapi(mc[0] + dx, G) * G * G + wrapi(mc[1] + dy, G) * G + wrapi(mc[2] + dz, G);
+          for (int k = cs[id]; k < cs[id] + cc[id]; ++k) {
+            int j = ci[k];
+            if (j <= i) continue;
+            if (topo_hash(i, j, frame) % q != 0) continue;
+            float e[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1],
+                          pos[3 * j + 2] - pos[3 * i + 2]};
+            wrap3(e, L, inv_l);
+            float d2 = e[0] * e[0] + e[1] * e[1];
+            d2 = d2 + e[2] * e[2];
+            float th = mult * (rad[i] + rad[j]);
+            if (d2 <= th * th) {
+              int cur = count++;
+              if (cur < cap)
+                pairs[(int64_t)i * cap + cur] = j;
+              else
+                truncated[i] = 1;
+            }
+          }
+        }
+    /* topology.py:385 counts past the list; only stored pairs are tested */
+    pair_count[i] = count;
+    int pc = count < cap ? count : cap;
+    for (int p = 0; p < pc; ++p) {
+      int j = pairs[(int64_t)i * cap + p];
+      if (gabriel_pair(pos, i, j, G, L, cell_size, cs, cc, ci) == 1) {
+        deg[i] += 1;
+        deg[j] += 1;
+      }
+    }
+  }
+}
+
+void orc_topo_knn(const float *pos, const float *rad, int n, int G, float L, float mult, int k,
+                  const int32_t *cs, const int32_t *cc, const int32_t *ci, int32_t *deg) {
+  const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
+  for (int i = 0; i < n; ++i) {
+    int cnt = 0;
+    float r = mult * rad[i];
+    float th = r * r;
+    int mc[3];
+    centre_cell(pos + 3 * i, G, L, half_l, inv_l, mc);
+    for (int d = 0; d < 3; ++d) mc[d] = wrapi(mc[d], G);
+    for (int dx = -1; dx <= 1 && cnt < k; ++dx)
+      for (int dy = -1; dy <= 1 && cnt < k; ++dy)
+        for (int dz = -1; dz <= 1 && cnt < k; ++dz) {
+          int id = wrapi(mc[0] + dx, G) * G * G + wrapi(mc[1] + dy, G) * G + wrapi(mc[2] + dz, G);
+          for (int l = cs[id]; l < cs[id] + cc[id] && cnt < k; ++l) {
+            int j = ci[l];
+            if (j == i) continue;
+            float e[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1],
+                          pos[3 * j + 2] - pos[3 * i + 2]};
+            wrap3(e, L, inv_l);
+            float d2 = e[0] * e[0] + e[1] * e[1];
+            d2 = d2 + e[2] * e[2];
+            if (d2 <= th) cnt += 1;
+          }
+        }
+    deg[i] = cnt < k ? cnt : k;
+  }
+}
+
+/* topology.py:266-283 update_topo_ema and :449-455 commit_topo_to_ema. */
+void orc_topo_ema(const int32_t *deg, float *ema, int n, float alpha) {
+  for (int i = 0; i < n; ++i) ema[i] = alpha * (float)deg[i] + (1.0f - alpha) * ema[i];
+}
+
+/* topology.py:287-322 update_radii_topological. */
+void orc_topo_radii(float *rad, const float *deg_eff, int n, float deg_low, float deg_high,
+                    float gain_grow, float gain_shrink, float r_min, float r_max) {
+  for (int i = 0; i < n; ++i) {
+    float d = deg_eff[i];
+    if (d < deg_low)
+      rad[i] *= (1.0f + gain_grow);
+    else if (d > deg_high)
+      rad[i] *= (1.0f - gain_shrink);
+    rad[i] = fmaxf(r_min, fminf(r_max, rad[i]));
+  }
+}

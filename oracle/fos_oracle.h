@@ -138,6 +138,23 @@ void orc_topo_candidates(const float *pos, const float *rad, int n, int G,
                          const int32_t *cell_indices, int32_t *pairs,
                          int32_t *count, int32_t *truncated);
 
+/* topology.py:140-504 */
+void orc_topo_gabriel(const float *pos, int n, int G, float L, float cell_size,
+                      const int32_t *cs, const int32_t *cc, const int32_t *ci,
+                      const int32_t *pairs, const int32_t *pair_count, int cap, int frame,
+                      int stride, int early_exit, float deg_high, const float *deg_ema,
+                      int32_t *deg);
+void orc_topo_gabriel_batched(const float *pos, const float *rad, int n, int G, float L,
+                              float cell_size, float mult, int cap, const int32_t *cs,
+                              const int32_t *cc, const int32_t *ci, int batch_idx,
+                              int num_batches, int q, int frame, int32_t *pairs,
+                              int32_t *pair_count, int32_t *truncated, int32_t *deg);
+void orc_topo_knn(const float *pos, const float *rad, int n, int G, float L, float mult, int k,
+                  const int32_t *cs, const int32_t *cc, const int32_t *ci, int32_t *deg);
+void orc_topo_ema(const int32_t *deg, float *ema, int n, float alpha);
+void orc_topo_radii(float *rad, const float *deg_eff, int n, float deg_low, float deg_high,
+                    float gain_grow, float gain_shrink, float r_min, float r_max);
+
 #ifdef __cplusplus
 }
 #endif

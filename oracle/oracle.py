@@ -52,6 +52,13 @@ _SIG = {
     "orc_ou_jitter": (None, [_P, _P, _P, _I, _F, _F, _F, _F, _F, _F, _U64, _I]),
     "orc_xsph": (None, [_P, _P, _P, _I, _I, _F, _F, _F, _P, _P, _P]),
     "orc_topo_candidates": (None, [_P, _P, _I, _I, _F, _F, _F, _I, _P, _P, _P, _P, _P, _P]),
+    "orc_topo_gabriel": (None, [_P, _I, _I, _F, _F, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _P,
+                                _P]),
+    "orc_topo_gabriel_batched": (None, [_P, _P, _I, _I, _F, _F, _F, _I, _P, _P, _P, _I, _I, _I,
+                                        _I, _P, _P, _P, _P]),
+    "orc_topo_knn": (None, [_P, _P, _I, _I, _F, _F, _I, _P, _P, _P, _P]),
+    "orc_topo_ema": (None, [_P, _P, _I, _F]),
+    "orc_topo_radii": (None, [_P, _P, _I, _F, _F, _F, _F, _F, _F]),
 }
 
 _lib = None
@@ -300,3 +307,47 @@ def topo_candidates(pos, rad, n, G, L, mult, cap, cs, cc, ci, r_max=0.0):
     lib().orc_topo_candidates(_p(f32(pos)), _p(f32(rad)), n, G, L, mult, r_max, cap,
                               _p(i32(cs)), _p(i32(cc)), _p(i32(ci)), _p(pairs), _p(cnt), _p(tr))
     return pairs, cnt, tr
+
+
+# ---- topology.py:140-504 ------------------------------------------------------
+
+def topo_gabriel(pos, n, G, L, cell_size, cs, cc, ci, pairs, pair_count, cap, frame, stride=3,
+                 early_exit=True, deg_high=18.0, deg_ema=None):
+    deg = np.zeros(n, np.int32)
+    ema = np.zeros(n, np.float32) if deg_ema is None else f32(deg_ema)
+    lib().orc_topo_gabriel(_p(f32(pos)), n, G, L, cell_size, _p(i32(cs)), _p(i32(cc)),
+                           _p(i32(ci)), _p(i32(pairs)), _p(i32(pair_count)), cap, frame, stride,
+                           int(early_exit), deg_high, _p(ema), _p(deg))
+    return deg
+
+
+def topo_gabriel_batched(pos, rad, n, G, L, cell_size, mult, cap, cs, cc, ci, batch_idx,
+                         num_batches, q, frame, deg=None):
+    pairs = np.full((n, cap), -1, np.int32)
+    cnt = np.zeros(n, np.int32)
+    tr = np.zeros(n, np.int32)
+    d = np.zeros(n, np.int32) if deg is None else np.array(deg, np.int32, copy=True)
+    lib().orc_topo_gabriel_batched(_p(f32(pos)), _p(f32(rad)), n, G, L, cell_size, mult, cap,
+                                   _p(i32(cs)), _p(i32(cc)), _p(i32(ci)), batch_idx, num_batches,
+                                   q, frame, _p(pairs), _p(cnt), _p(tr), _p(d))
+    return pairs, cnt, tr, d
+
+
+def topo_knn(pos, rad, n, G, L, mult, k, cs, cc, ci):
+    deg = np.zeros(n, np.int32)
+    lib().orc_topo_knn(_p(f32(pos)), _p(f32(rad)), n, G, L, mult, k, _p(i32(cs)), _p(i32(cc)),
+                       _p(i32(ci)), _p(deg))
+    return deg
+
+
+def topo_ema(deg, ema, n, alpha):
+    e = np.array(ema, np.float32, copy=True)
+    lib().orc_topo_ema(_p(i32(deg)), _p(e), n, alpha)
+    return e
+
+
+def topo_radii(rad, deg_eff, n, deg_low, deg_high, gain_grow, gain_shrink, r_min, r_max):
+    r = np.array(rad, np.float32, copy=True)
+    lib().orc_topo_radii(_p(r), _p(f32(deg_eff)), n, deg_low, deg_high, gain_grow, gain_shrink,
+                         r_min, r_max)
+    return r

@@ -588,3 +588,56 @@ def test_topology_candidates(gpu, orc):
              gpu.stream())
     assert np.array_equal(host(gc), cnt) and np.array_equal(host(gt), tr)
     assert np.array_equal(host(gp), pairs)
+
+
+def test_topology_degree_kernels(gpu, orc):
+    """Gabriel (full and batched), k-NN, degree EMA and radius rule against
+    the oracle (topology.py:140-504)."""
+    n, G, cap = 3000, 12, 28
+    pos, rad = fi.uniform(n, 0.005, 0.02, seed=12)
+    p, cc, cs, ci = orc.grid_build(pos, n, G, 1.0)
+    pairs, cnt, tr = orc.topo_candidates(p, rad, n, G, 1.0, F(2.2), cap, cs, cc, ci)
+    ema = np.random.default_rng(3).uniform(10, 25, n).astype(F)
+    tp, tcs, tcc, tci = dev(p), dev(cs), dev(cc), dev(ci)
+    deg = torch.empty(n, dtype=torch.int32, device="cuda")
+    for stride, early in ((1, 0), (3, 1)):
+        want = orc.topo_gabriel(p, n, G, 1.0, 1.0 / G, cs, cc, ci, pairs, cnt, cap, 7, stride,
+                                bool(early), 18.0, ema)
+        gpu.call("fos_topo_gabriel", gpu.ptr(tp), n, G, 1.0, 1.0 / G, gpu.ptr(tcs), gpu.ptr(tcc),
+                 gpu.ptr(tci), gpu.ptr(dev(pairs)), gpu.ptr(dev(cnt)), cap, 7, stride, early, 18.0,
+                 gpu.ptr(dev(ema)), gpu.ptr(deg), gpu.stream())
+        assert np.array_equal(host(deg), want), (stride, early)
+    B, q = 4, 3
+    wdeg = np.zeros(n, np.int32)
+    gdeg = torch.zeros(n, dtype=torch.int32, device="cuda")
+    gpairs = torch.full((n, cap), -1, dtype=torch.int32, device="cuda")
+    gcnt = torch.zeros(n, dtype=torch.int32, device="cuda")
+    gtr = torch.zeros(n, dtype=torch.int32, device="cuda")
+    trd = dev(rad)
+    for b in range(B):
+        wp, wc, wt, wdeg = orc.topo_gabriel_batched(p, rad, n, G, 1.0, 1.0 / G, F(2.2), cap, cs,
+                                                    cc, ci, b, B, q, 11, wdeg)
+        gpu.call("fos_topo_gabriel_batched", gpu.ptr(tp), gpu.ptr(trd), n, G, 1.0, 1.0 / G, 2.2,
+                 cap, gpu.ptr(tcs), gpu.ptr(tcc), gpu.ptr(tci), b, B, q, 11, gpu.ptr(gpairs),
+                 gpu.ptr(gcnt), gpu.ptr(gtr), gpu.ptr(gdeg), gpu.stream())
+        mine = np.arange(n) % B == b
+        gc = host(gcnt)
+        assert np.array_equal(gc[mine], wc[mine]) and np.array_equal(host(gtr)[mine], wt[mine])
+        gpp = host(gpairs)
+        for i in np.nonzero(mine)[0]:
+            m = min(wc[i], cap)
+            assert (gpp[i, :m] == wp[i, :m]).all()
+    assert np.array_equal(host(gdeg), wdeg)
+    kd = orc.topo_knn(p, rad, n, G, 1.0, F(2.2), 14, cs, cc, ci)
+    gpu.call("fos_topo_knn", gpu.ptr(tp), gpu.ptr(trd), n, G, 1.0, 2.2, 14, gpu.ptr(tcs),
+             gpu.ptr(tcc), gpu.ptr(tci), gpu.ptr(deg), gpu.stream())
+    assert np.array_equal(host(deg), kd)
+    te = dev(ema)
+    gpu.call("fos_topo_ema", gpu.ptr(dev(kd)), gpu.ptr(te), n, 0.1, gpu.stream())
+    assert np.array_equal(host(te), orc.topo_ema(kd, ema, n, F(0.1)))
+    de = np.random.default_rng(4).uniform(5, 25, n).astype(F)
+    tr_ = dev(rad)
+    gpu.call("fos_topo_radii", gpu.ptr(tr_), gpu.ptr(dev(de)), n, 10.0, 18.0, 0.05, 0.05, 0.005,
+             0.02, gpu.stream())
+    assert np.array_equal(host(tr_), orc.topo_radii(rad, de, n, 10.0, 18.0, F(0.05), F(0.05),
+                                                     F(0.005), F(0.02)))
