@@ -33,31 +33,51 @@ __device__ __forceinline__ float dot3f(float x, float y, float z) {
 // ---------------------------------------------------------------------------
 // Candidates: every j != i with |d(i, j)| < mult * (ri + rj), scanning the 27
 // cells around i's cell (dx, dy, dz order, ascending index inside a cell),
-// first `cap` kept.  Needs a hash cell of at least mult * 2 * r_max.
+// at most `cap` kept in that order.  A row lists the partners j > i first
+// (nup[i] of them, in scan order), then the partners j < i.  Needs a hash
+// cell of at least mult * 2 * r_max.
 // ---------------------------------------------------------------------------
+#define PRED_CAP_MAX 128  // row capacity limit (LDS staging of the j < i block)
+
 __global__ void __launch_bounds__(256)
     k_pred_cand(const float *__restrict__ pos, const float *__restrict__ rad, int n, int G,
                 float L, float half_l, float inv_l, float mult, int cap,
                 const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
                 const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
-                int32_t *__restrict__ ncand, int32_t *__restrict__ trunc) {
-  // one wave per particle: lane l tests the l-th occupant of a cell, and a
-  // ballot keeps the hits in occupant order
+                int32_t *__restrict__ ncand, int32_t *__restrict__ nup,
+                int32_t *__restrict__ trunc) {
+  // One wave per particle: lane l tests the l-th particle of a run of cells
+  // and ballots keep the hits in scan order.  Partners j > i go straight to
+  // the front of the row; partners j < i are staged in LDS and copied behind
+  // them at the end.
+  __shared__ int32_t s_low[4][PRED_CAP_MAX];
   const int lane = (int)(threadIdx.x & 63);
+  const int w = (int)(threadIdx.x >> 6);
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= n) return;
   const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2], ri = rad[i];
   const int cx = wrap_fast(centre_axis(px, G, L, half_l, inv_l), G);
   const int cy = wrap_fast(centre_axis(py, G, L, half_l, inv_l), G);
   const int cz = wrap_fast(centre_axis(pz, G, L, half_l, inv_l), G);
-  int c = 0, tr = 0;
+  // the three cells cz-1..cz+1 of a column are contiguous in the CSR order
+  // unless z wraps
+  const bool run = cz >= 1 && cz + 1 < G;
+  int cu = 0, cl = 0, tr = 0;
   int32_t *row = cand + (int64_t)i * cap;
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int dx = -1; dx <= 1; ++dx)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dz = -1; dz <= 1; ++dz) {
-        const int id = (wrap1(cx + dx, G) * G + wrap1(cy + dy, G)) * G + wrap1(cz + dz, G);
-        const int b = cs[id], e = b + cc[id];
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int rbase = (wrap1(cx + dx, G) * G + wrap1(cy + dy, G)) * G;
+      for (int part = 0; part < (run ? 1 : 3); ++part) {
+        int b, e;
+        if (run) {
+          b = cs[rbase + cz - 1];
+          e = cs[rbase + cz + 1] + cc[rbase + cz + 1];
+        } else {
+          const int id = rbase + wrap1(cz + part - 1, G);
+          b = cs[id];
+          e = b + cc[id];
+        }
         for (int k0 = b; k0 < e; k0 += 64) {
           const int k = k0 + lane;
           bool hit = false;
@@ -72,21 +92,28 @@ __global__ void __launch_bounds__(256)
               hit = dot3f(ddx, ddy, ddz) < th * th;
             }
           }
-          const unsigned long long m = __ballot(hit);
-          if (hit) {
-            const int slot = c + __popcll(m & below);
-            if (slot < cap) row[slot] = j;
+          const unsigned long long mh = __ballot(hit);
+          const bool keep = hit && cu + cl + __popcll(mh & below) < cap;
+          const unsigned long long mk = __ballot(keep);
+          const unsigned long long mu = __ballot(keep && j > i);
+          if (keep) {
+            if (j > i)
+              row[cu + __popcll(mu & below)] = j;
+            else
+              s_low[w][cl + __popcll(mk & ~mu & below)] = j;
           }
-          c += __popcll(m);
+          tr |= (mh & ~mk) != 0ull;
+          cu += __popcll(mu);
+          cl += __popcll(mk & ~mu);
         }
       }
-  if (c > cap) {
-    tr = 1;
-    c = cap;
-  }
-  for (int s2 = c + lane; s2 < cap; s2 += 64) row[s2] = -1;
+    }
+  __builtin_amdgcn_wave_barrier();
+  for (int t = lane; t < cl; t += 64) row[cu + t] = s_low[w][t];
+  for (int t = cu + cl + lane; t < cap; t += 64) row[t] = -1;
   if (lane == 0) {
-    ncand[i] = c;
+    ncand[i] = cu + cl;
+    nup[i] = cu;
     trunc[i] = tr;
   }
 }
@@ -110,155 +137,163 @@ __global__ void __launch_bounds__(256)
 //   3 t                       9 |cand(j)| / 40
 //   4 estimated score (<= 4) 10 (ri - rj) / (ri + rj)
 //   5 face area / (ri + rj)^2 (<= 4)   11 p(m) / (ri + rj)^2
-// Margins and p(m) are clamped to [-1, 1].  One workgroup per particle i:
-// i's candidates (relative positions, |pk - pi|^2 - rk^2) are staged in LDS
-// and each thread scores one slot against them.  The candidate relation is
-// symmetric, so only slots with j > i are scored (the pair's lower row);
-// other slots are not written.
+// Margins and p(m) are clamped to [-1, 1].  The candidate relation is
+// symmetric, so each unordered pair is scored once, in the row of its lower
+// index: only the first nup[i] slots (the partners j > i) are written.
+// Half a wave per particle: i's candidates (relative positions and
+// |pk - pi|^2 - rk^2) are staged in LDS and lane l scores slots l, l + 32, ...
 // ---------------------------------------------------------------------------
-#define PRED_NEAR 8   // planes kept for the face polygon (nearest to m)
-#define PRED_TPB 128  // threads per particle: covers cap <= 128
+#define PRED_NEAR 8    // planes kept for the face polygon (nearest to m)
+#define PRED_GRP 32    // lanes per particle
+#define PRED_PPB 8     // particles per 256-thread workgroup
 
-__global__ void __launch_bounds__(PRED_TPB)
+__global__ void __launch_bounds__(256)
     k_pred_feat(const float *__restrict__ pos, const float *__restrict__ rad, int n, float L,
                 float inv_l, float inv_rbar, float clip_r, const int32_t *__restrict__ cand,
-                const int32_t *__restrict__ ncand, int cap, float *__restrict__ feat) {
-  __shared__ float s_x[PRED_TPB], s_y[PRED_TPB], s_z[PRED_TPB], s_p[PRED_TPB];
-  const int i = blockIdx.x;
-  const int s = (int)threadIdx.x;
-  const int ni = ncand[i];
-  const float pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
-  if (s < ni) {
-    const int k = cand[(int64_t)i * cap + s];
-    const float kx = gwrap(pos[3 * k] - pix, L, inv_l);
-    const float ky = gwrap(pos[3 * k + 1] - piy, L, inv_l);
-    const float kz = gwrap(pos[3 * k + 2] - piz, L, inv_l);
-    const float rk = rad[k];
-    s_x[s] = kx;
-    s_y[s] = ky;
-    s_z[s] = kz;
-    s_p[s] = dot3f(kx, ky, kz) - rk * rk;
+                const int32_t *__restrict__ ncand, const int32_t *__restrict__ nup, int cap,
+                float *__restrict__ feat) {
+  __shared__ float4 s_k[PRED_PPB][PRED_CAP_MAX];
+  const int g = (int)(threadIdx.x / PRED_GRP);
+  const int lane = (int)(threadIdx.x % PRED_GRP);
+  const int i = (int)blockIdx.x * PRED_PPB + g;
+  int ni = 0, nu = 0;
+  float pix = 0.0f, piy = 0.0f, piz = 0.0f;
+  if (i < n) {
+    ni = ncand[i];
+    nu = nup[i];
+    pix = pos[3 * i]; piy = pos[3 * i + 1]; piz = pos[3 * i + 2];
+    for (int q = lane; q < ni; q += PRED_GRP) {
+      const int k = cand[(int64_t)i * cap + q];
+      const float kx = gwrap(pos[3 * k] - pix, L, inv_l);
+      const float ky = gwrap(pos[3 * k + 1] - piy, L, inv_l);
+      const float kz = gwrap(pos[3 * k + 2] - piz, L, inv_l);
+      const float rk = rad[k];
+      s_k[g][q] = make_float4(kx, ky, kz, dot3f(kx, ky, kz) - rk * rk);
+    }
   }
   __syncthreads();
-  if (s >= ni) return;
-  const int j = cand[(int64_t)i * cap + s];
-  if (j < i) return;  // each unordered pair is scored once, in its lower row
-  const float ri = rad[i], rj = rad[j];
-  const float dx = s_x[s], dy = s_y[s], dz = s_z[s];
-  const float d2 = fmaxf(dot3f(dx, dy, dz), 1e-20f);
-  const float sr = ri + rj, sr2 = sr * sr;
-  const float t = (d2 + ri * ri - rj * rj) / (2.0f * d2);
-  const float pm = t * t * d2 - ri * ri;
-  const float ri2 = ri * ri;
-  // orthonormal basis (u, v) of the power plane
-  const float dl = sqrtf(d2);
-  const float nx = dx / dl, ny = dy / dl, nz = dz / dl;
-  float ux, uy, uz;
-  if (fabsf(nx) < 0.9f) {
-    ux = 1.0f - nx * nx; uy = -nx * ny; uz = -nx * nz;
-  } else {
-    ux = -ny * nx; uy = 1.0f - ny * ny; uz = -ny * nz;
-  }
-  const float ul = rsqrtf(dot3f(ux, uy, uz));
-  ux *= ul; uy *= ul; uz *= ul;
-  const float vx = ny * uz - nz * uy, vy = nz * ux - nx * uz, vz = nx * uy - ny * ux;
-  // nearest PRED_NEAR planes (unit normal in the plane, offset), ascending
-  float pa[PRED_NEAR], pb[PRED_NEAR], pc[PRED_NEAR];
-#pragma unroll
-  for (int h = 0; h < PRED_NEAR; ++h) {
-    pa[h] = 1.0f; pb[h] = 0.0f; pc[h] = 1e30f;
-  }
-  bool empty = false;
-  int wit = 0;
-  float mw = 1e30f;
-  const float mm = t * t * d2 - pm;  // |m|^2 - p(m) = ri^2
-  for (int q = 0; q < ni; ++q) {
-    if (q == s) continue;
-    const float kx = s_x[q], ky = s_y[q], kz = s_z[q], pk = s_p[q];
-    const float kd = kx * dx + ky * dy + kz * dz;
-    // |pk - m|^2 - rk^2 - p(m), with m = t d
-    const float wm = pk - 2.0f * t * kd + mm;
-    wit += wm < 0.0f ? 1 : 0;
-    mw = fminf(mw, wm);
-    // p_i(x) <= p_k(x) on the plane, x = m + a u + b v:  A a + B b <= C
-    const float A = 2.0f * (kx * ux + ky * uy + kz * uz);
-    const float B = 2.0f * (kx * vx + ky * vy + kz * vz);
-    const float C = pk + ri2 - 2.0f * t * kd;
-    const float q2 = A * A + B * B;
-    if (q2 < 1e-30f) {
-      empty |= C < 0.0f;
-      continue;
+  const float4 *sk = s_k[g];
+  for (int s = lane; s < nu; s += PRED_GRP) {
+    const int j = cand[(int64_t)i * cap + s];
+    const float ri = rad[i], rj = rad[j];
+    const float4 ks = sk[s];
+    const float dx = ks.x, dy = ks.y, dz = ks.z;
+    const float d2 = fmaxf(dot3f(dx, dy, dz), 1e-20f);
+    const float sr = ri + rj, sr2 = sr * sr;
+    const float t = (d2 + ri * ri - rj * rj) / (2.0f * d2);
+    const float pm = t * t * d2 - ri * ri;
+    const float ri2 = ri * ri;
+    // orthonormal basis (u, v) of the power plane
+    const float dl = sqrtf(d2);
+    const float nx = dx / dl, ny = dy / dl, nz = dz / dl;
+    float ux, uy, uz;
+    if (fabsf(nx) < 0.9f) {
+      ux = 1.0f - nx * nx; uy = -nx * ny; uz = -nx * nz;
+    } else {
+      ux = -ny * nx; uy = 1.0f - ny * ny; uz = -ny * nz;
     }
-    const float inv = rsqrtf(q2);
-    float ic = C * inv;
-    if (!(ic < pc[PRED_NEAR - 1])) continue;
-    float ia = A * inv, ib = B * inv;
+    const float ul = rsqrtf(dot3f(ux, uy, uz));
+    ux *= ul; uy *= ul; uz *= ul;
+    const float vx = ny * uz - nz * uy, vy = nz * ux - nx * uz, vz = nx * uy - ny * ux;
+    // nearest PRED_NEAR planes (unit normal in the plane, offset), ascending
+    float pa[PRED_NEAR], pb[PRED_NEAR], pc[PRED_NEAR];
 #pragma unroll
     for (int h = 0; h < PRED_NEAR; ++h) {
-      const bool sw = ic < pc[h];
-      const float ta = pa[h], tb = pb[h], tc = pc[h];
-      pa[h] = sw ? ia : ta; pb[h] = sw ? ib : tb; pc[h] = sw ? ic : tc;
-      ia = sw ? ta : ia; ib = sw ? tb : ib; ic = sw ? tc : ic;
+      pa[h] = 1.0f; pb[h] = 0.0f; pc[h] = 1e30f;
     }
-  }
-  // Area of {x : a_h x <= c_h} over the kept planes and the four sides of a
-  // square of half-size clip_r*(ri+rj): each line contributes the edge it
-  // keeps after clipping by all the others, summed with the shoelace formula
-  // (lines traversed with the interior on the left).
-  const float R0 = clip_r * sr;
-  float la[PRED_NEAR + 4], lb[PRED_NEAR + 4], lc[PRED_NEAR + 4];
+    bool empty = false;
+    int wit = 0;
+    float mw = 1e30f;
+    const float mm = t * t * d2 - pm;  // |m|^2 - p(m) = ri^2
+    for (int q = 0; q < ni; ++q) {
+      if (q == s) continue;
+      const float4 kq = sk[q];
+      const float kx = kq.x, ky = kq.y, kz = kq.z, pk = kq.w;
+      const float kd = kx * dx + ky * dy + kz * dz;
+      // |pk - m|^2 - rk^2 - p(m), with m = t d
+      const float wm = pk - 2.0f * t * kd + mm;
+      wit += wm < 0.0f ? 1 : 0;
+      mw = fminf(mw, wm);
+      // p_i(x) <= p_k(x) on the plane, x = m + a u + b v:  A a + B b <= C
+      const float A = 2.0f * (kx * ux + ky * uy + kz * uz);
+      const float B = 2.0f * (kx * vx + ky * vy + kz * vz);
+      const float C = pk + ri2 - 2.0f * t * kd;
+      const float q2 = A * A + B * B;
+      if (q2 < 1e-30f) {
+        empty |= C < 0.0f;
+        continue;
+      }
+      const float inv = rsqrtf(q2);
+      float ic = C * inv;
+      if (!(ic < pc[PRED_NEAR - 1])) continue;
+      float ia = A * inv, ib = B * inv;
 #pragma unroll
-  for (int h = 0; h < PRED_NEAR; ++h) {
-    la[h] = pa[h]; lb[h] = pb[h]; lc[h] = pc[h];
-  }
-  la[PRED_NEAR] = 1.0f;  lb[PRED_NEAR] = 0.0f;  lc[PRED_NEAR] = R0;
-  la[PRED_NEAR + 1] = -1.0f; lb[PRED_NEAR + 1] = 0.0f; lc[PRED_NEAR + 1] = R0;
-  la[PRED_NEAR + 2] = 0.0f; lb[PRED_NEAR + 2] = 1.0f;  lc[PRED_NEAR + 2] = R0;
-  la[PRED_NEAR + 3] = 0.0f; lb[PRED_NEAR + 3] = -1.0f; lc[PRED_NEAR + 3] = R0;
-  float area = 0.0f;
-#pragma unroll
-  for (int h = 0; h < PRED_NEAR + 4; ++h) {
-    const float x0 = la[h] * lc[h], y0 = lb[h] * lc[h];  // closest point to m
-    const float tx = -lb[h], ty = la[h];
-    float lo = -1e30f, hi = 1e30f;
-#pragma unroll
-    for (int g = 0; g < PRED_NEAR + 4; ++g) {
-      if (g == h) continue;
-      const float kk = la[g] * tx + lb[g] * ty;
-      const float rr = lc[g] - (la[g] * x0 + lb[g] * y0);
-      if (kk > 1e-12f)
-        hi = fminf(hi, rr / kk);
-      else if (kk < -1e-12f)
-        lo = fmaxf(lo, rr / kk);
-      else if (rr < 0.0f || (rr == 0.0f && g < h))
-        hi = -1e30f;  // parallel and tighter (or an equal line met first)
+      for (int h = 0; h < PRED_NEAR; ++h) {
+        const bool sw = ic < pc[h];
+        const float ta = pa[h], tb = pb[h], tc = pc[h];
+        pa[h] = sw ? ia : ta; pb[h] = sw ? ib : tb; pc[h] = sw ? ic : tc;
+        ia = sw ? ta : ia; ib = sw ? tb : ib; ic = sw ? tc : ic;
+      }
     }
-    if (lc[h] < 1e29f && lo < hi) {
-      const float x1 = x0 + lo * tx, y1 = y0 + lo * ty;
-      const float x2 = x0 + hi * tx, y2 = y0 + hi * ty;
-      area += x1 * y2 - x2 * y1;
-    }
-  }
-  area = empty ? 0.0f : fmaxf(0.0f, 0.5f * area);
-  const float na = area / sr2;
-  float f[PRED_FEAT];
-  f[0] = dl / sr;
-  f[1] = ri * inv_rbar;
-  f[2] = rj * inv_rbar;
-  f[3] = t;
-  f[4] = fminf(4.0f, na * (fabsf(nx) + fabsf(ny) + fabsf(nz)) / 1.5f);
-  f[5] = fminf(4.0f, na);
-  f[6] = (float)min(wit, 8) * 0.25f;
-  f[7] = fminf(1.0f, fmaxf(-1.0f, mw / sr2));
-  f[8] = (float)ni * (1.0f / 40.0f);
-  f[9] = (float)ncand[j] * (1.0f / 40.0f);
-  f[10] = (ri - rj) / sr;
-  f[11] = fminf(1.0f, fmaxf(-1.0f, pm / sr2));
-  f[12] = f[13] = f[14] = f[15] = 0.0f;
-  float4 *o = (float4 *)(feat + ((int64_t)i * cap + s) * PRED_FEAT);
+    // Area of {x : a_h x <= c_h} over the kept planes and the four sides of a
+    // square of half-size clip_r*(ri+rj): each line contributes the edge it
+    // keeps after clipping by all the others, summed with the shoelace formula
+    // (lines traversed with the interior on the left).
+    const float R0 = clip_r * sr;
+    float la[PRED_NEAR + 4], lb[PRED_NEAR + 4], lc[PRED_NEAR + 4];
 #pragma unroll
-  for (int q = 0; q < PRED_FEAT / 4; ++q)
-    o[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+    for (int h = 0; h < PRED_NEAR; ++h) {
+      la[h] = pa[h]; lb[h] = pb[h]; lc[h] = pc[h];
+    }
+    la[PRED_NEAR] = 1.0f;  lb[PRED_NEAR] = 0.0f;  lc[PRED_NEAR] = R0;
+    la[PRED_NEAR + 1] = -1.0f; lb[PRED_NEAR + 1] = 0.0f; lc[PRED_NEAR + 1] = R0;
+    la[PRED_NEAR + 2] = 0.0f; lb[PRED_NEAR + 2] = 1.0f;  lc[PRED_NEAR + 2] = R0;
+    la[PRED_NEAR + 3] = 0.0f; lb[PRED_NEAR + 3] = -1.0f; lc[PRED_NEAR + 3] = R0;
+    float area = 0.0f;
+#pragma unroll
+    for (int h = 0; h < PRED_NEAR + 4; ++h) {
+      const float x0 = la[h] * lc[h], y0 = lb[h] * lc[h];  // closest point to m
+      const float tx = -lb[h], ty = la[h];
+      float lo = -1e30f, hi = 1e30f;
+#pragma unroll
+      for (int g = 0; g < PRED_NEAR + 4; ++g) {
+        if (g == h) continue;
+        const float kk = la[g] * tx + lb[g] * ty;
+        const float rr = lc[g] - (la[g] * x0 + lb[g] * y0);
+        if (kk > 1e-12f)
+          hi = fminf(hi, rr / kk);
+        else if (kk < -1e-12f)
+          lo = fmaxf(lo, rr / kk);
+        else if (rr < 0.0f || (rr == 0.0f && g < h))
+          hi = -1e30f;  // parallel and tighter (or an equal line met first)
+      }
+      if (lc[h] < 1e29f && lo < hi) {
+        const float x1 = x0 + lo * tx, y1 = y0 + lo * ty;
+        const float x2 = x0 + hi * tx, y2 = y0 + hi * ty;
+        area += x1 * y2 - x2 * y1;
+      }
+    }
+    area = empty ? 0.0f : fmaxf(0.0f, 0.5f * area);
+    const float na = area / sr2;
+    float f[PRED_FEAT];
+    f[0] = dl / sr;
+    f[1] = ri * inv_rbar;
+    f[2] = rj * inv_rbar;
+    f[3] = t;
+    f[4] = fminf(4.0f, na * (fabsf(nx) + fabsf(ny) + fabsf(nz)) / 1.5f);
+    f[5] = fminf(4.0f, na);
+    f[6] = (float)min(wit, 8) * 0.25f;
+    f[7] = fminf(1.0f, fmaxf(-1.0f, mw / sr2));
+    f[8] = (float)ni * (1.0f / 40.0f);
+    f[9] = (float)ncand[j] * (1.0f / 40.0f);
+    f[10] = (ri - rj) / sr;
+    f[11] = fminf(1.0f, fmaxf(-1.0f, pm / sr2));
+    f[12] = f[13] = f[14] = f[15] = 0.0f;
+    float4 *o = (float4 *)(feat + ((int64_t)i * cap + s) * PRED_FEAT);
+#pragma unroll
+    for (int q = 0; q < PRED_FEAT / 4; ++q)
+      o[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -267,7 +302,9 @@ __global__ void __launch_bounds__(PRED_TPB)
 // host in lane order), the activations the B operand.  The 16x16 f32 result
 // of a layer leaves lane l holding rows 4*(l/16)..+3 of column l%16, which
 // is the B-operand layout of the next layer's 16-row K block, so hidden
-// activations stay in registers.  A wave scores 16 pairs per step.
+// activations stay in registers.  A wave scores 16 pairs per step.  Given
+// nup (rows of cap slots, cap a multiple of 16), groups of 16 slots at or
+// past nup[row] are skipped and their logits left unwritten.
 // ---------------------------------------------------------------------------
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -282,7 +319,8 @@ __global__ void __launch_bounds__(256)
     k_mlp_infer(const float *__restrict__ feat, int64_t npairs,
                 const bf16x4 *__restrict__ w1p, const bf16x4 *__restrict__ w2p,
                 const float *__restrict__ b1, const float *__restrict__ b2,
-                const float *__restrict__ w3, float b3, float *__restrict__ logit) {
+                const float *__restrict__ w3, float b3, float *__restrict__ logit,
+                const int32_t *__restrict__ nup, int cap) {
   const int lane = (int)(threadIdx.x & 63);
   const int col = lane & 15, grp = lane >> 4;
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
@@ -303,6 +341,10 @@ __global__ void __launch_bounds__(256)
     }
   }
   for (int64_t p0 = wave * 16; p0 < npairs; p0 += waves * 16) {
+    if (nup) {  // rows of cap slots: skip groups past the row's scored prefix
+      const int64_t r = p0 / cap;
+      if ((int)(p0 - r * cap) >= nup[r]) continue;
+    }
     const float4 x = *(const float4 *)(feat + (p0 + col) * PRED_FEAT + 4 * grp);
     bf16x4 xb;
     xb[0] = f2bf(x.x);
@@ -333,58 +375,63 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// FSC from the scores: a pair (i, j) is kept when the logit of its lower
-// row's slot is > 0 (sigmoid 0.5); fsc[i] counts kept pairs over i's row,
-// looking up i's slot in j's row for j < i.  Face rows list the kept
-// partners j > i in candidate order, at most 32, -1 padded (the row shape
-// accumulate_face_area produces for the pressure step).
-__global__ void k_pred_fsc(const float *__restrict__ logit, const int32_t *__restrict__ cand,
-                           const int32_t *__restrict__ ncand, int n, int cap,
-                           int32_t *__restrict__ fsc, int32_t *__restrict__ face_ids,
-                           int32_t *__restrict__ face_counts) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// FSC from the scores: a pair (i, j), j > i, is kept when the logit of its
+// slot in i's row is > 0 (sigmoid 0.5) and then counts for both i and j
+// (fsc must be zero on entry).  Face rows list the kept partners j > i in
+// candidate order, at most 32, -1 padded (the row shape
+// accumulate_face_area produces for the pressure step).  One wave per i.
+__global__ void __launch_bounds__(256)
+    k_pred_fsc(const float *__restrict__ logit, const int32_t *__restrict__ cand,
+               const int32_t *__restrict__ nup, int n, int cap, int32_t *__restrict__ fsc,
+               int32_t *__restrict__ face_ids, int32_t *__restrict__ face_counts) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= n) return;
-  const int ni = ncand[i];
-  int c = 0, r = 0;
-  for (int s = 0; s < ni; ++s) {
-    const int j = cand[(int64_t)i * cap + s];
+  const int nu = nup[i];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int c = 0;
+  for (int s0 = 0; s0 < nu; s0 += 64) {
+    const int s = s0 + lane;
     bool keep = false;
-    if (j > i) {
+    int j = -1;
+    if (s < nu) {
       keep = logit[(int64_t)i * cap + s] > 0.0f;
-    } else {
-      const int nj = ncand[j];
-      for (int q = 0; q < nj; ++q)
-        if (cand[(int64_t)j * cap + q] == i) {
-          keep = logit[(int64_t)j * cap + q] > 0.0f;
-          break;
-        }
+      j = cand[(int64_t)i * cap + s];
     }
-    if (!keep) continue;
-    c += 1;
-    if (face_ids && j > i && r < FOS_MAX_NEIGHBORS) {
-      face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + r] = j;
-      if (face_counts) face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + r] = 1;
-      r += 1;
+    const unsigned long long mk = __ballot(keep);
+    if (keep) {
+      atomicAdd(&fsc[j], 1);
+      const int r = c + __popcll(mk & below);
+      if (face_ids && r < FOS_MAX_NEIGHBORS) {
+        face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + r] = j;
+        if (face_counts) face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + r] = 1;
+      }
     }
+    c += __popcll(mk);
   }
-  fsc[i] = c;
+  if (lane == 0 && c) atomicAdd(&fsc[i], c);
   if (face_ids)
-    for (; r < FOS_MAX_NEIGHBORS; ++r) {
+    for (int r = min(c, FOS_MAX_NEIGHBORS) + lane; r < FOS_MAX_NEIGHBORS; r += 64) {
       face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + r] = -1;
       if (face_counts) face_counts[(int64_t)i * FOS_MAX_NEIGHBORS + r] = 0;
     }
 }
 
+__global__ void k_fill_i32(int32_t *__restrict__ a, int n, int32_t v) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) a[q] = v;
+}
+
 // Labels for training: 1 when j is in i's JFA neighbour list, for the
-// scored slots (j > i); -1 for empty slots and slots with j < i.
-__global__ void k_pred_labels(const int32_t *__restrict__ cand, const int32_t *__restrict__ ncand,
+// scored slots s < nup[i] (j > i); -1 for the other slots.
+__global__ void k_pred_labels(const int32_t *__restrict__ cand, const int32_t *__restrict__ nup,
                               int n, int cap, const int32_t *__restrict__ nl,
                               const int32_t *__restrict__ nc, float *__restrict__ y) {
   const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= (int64_t)n * cap) return;
   const int i = (int)(slot / cap), s = (int)(slot - (int64_t)i * cap);
-  float v = -1.0f;  // empty or unscored slot
-  if (s < ncand[i] && cand[slot] > i) {
+  float v = -1.0f;  // unscored slot
+  if (s < nup[i]) {
     const int j = cand[slot];
     const int m = min(nc[i], FOS_MAX_NEIGHBORS);
     v = 0.0f;
@@ -521,31 +568,33 @@ extern "C" {
 
 int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float L, float mult,
                         int cap, const int32_t *cell_start, const int32_t *cell_count,
-                        const int32_t *cell_indices, int32_t *cand, int32_t *ncand,
+                        const int32_t *cell_indices, int32_t *cand, int32_t *ncand, int32_t *nup,
                         int32_t *truncated, void *stream) {
   FOS_REQUIRE(n >= 0 && G >= 3 && L > 0.0f && cap >= 1 && mult > 0.0f &&
                   (n == 0 || (pos && rad && cell_start && cell_count && cell_indices && cand &&
-                              ncand && truncated)),
+                              ncand && nup && truncated)),
               "fos_pred_candidates: bad arguments (n=%d G=%d cap=%d)", n, G, cap);
+  FOS_REQUIRE(cap <= PRED_CAP_MAX, "fos_pred_candidates: cap %d exceeds %d", cap, PRED_CAP_MAX);
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_pred_cand, dim3(blocks_for((int64_t)n * 64)), dim3(FOS_BLOCK), 0,
                      (hipStream_t)stream,
                      pos, rad, n, G, L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), mult,
-                     cap, cell_start, cell_count, cell_indices, cand, ncand, truncated);
+                     cap, cell_start, cell_count, cell_indices, cand, ncand, nup, truncated);
   FOS_CHECK_LAUNCH("k_pred_cand");
   return 0;
 }
 
 int fos_pred_features(const float *pos, const float *rad, int n, float L, float r_mean,
-                      float clip_r, const int32_t *cand, const int32_t *ncand, int cap,
-                      float *feat, void *stream) {
+                      float clip_r, const int32_t *cand, const int32_t *ncand,
+                      const int32_t *nup, int cap, float *feat, void *stream) {
   FOS_REQUIRE(n >= 0 && L > 0.0f && cap >= 1 && r_mean > 0.0f && clip_r > 0.0f &&
-                  (n == 0 || (pos && rad && cand && ncand && feat)),
+                  (n == 0 || (pos && rad && cand && ncand && nup && feat)),
               "fos_pred_features: bad arguments (n=%d cap=%d)", n, cap);
+  FOS_REQUIRE(cap <= PRED_CAP_MAX, "fos_pred_features: cap %d exceeds %d", cap, PRED_CAP_MAX);
   if (n == 0) return 0;
-  FOS_REQUIRE(cap <= PRED_TPB, "fos_pred_features: cap %d exceeds %d", cap, PRED_TPB);
-  hipLaunchKernelGGL(k_pred_feat, dim3((unsigned)n), dim3(PRED_TPB), 0, (hipStream_t)stream, pos,
-                     rad, n, L, (float)(1.0 / (double)L), 1.0f / r_mean, clip_r, cand, ncand, cap,
+  hipLaunchKernelGGL(k_pred_feat, dim3((unsigned)((n + PRED_PPB - 1) / PRED_PPB)),
+                     dim3(PRED_PPB * PRED_GRP), 0, (hipStream_t)stream, pos, rad, n, L,
+                     (float)(1.0 / (double)L), 1.0f / r_mean, clip_r, cand, ncand, nup, cap,
                      feat);
   FOS_CHECK_LAUNCH("k_pred_feat");
   return 0;
@@ -553,43 +602,48 @@ int fos_pred_features(const float *pos, const float *rad, int n, float L, float 
 
 int fos_pred_mlp(const float *feat, int64_t npairs, const void *w1_packed, const void *w2_packed,
                  const float *b1, const float *b2, const float *w3, float b3, float *logit,
-                 void *stream) {
+                 const int32_t *nup, int cap, void *stream) {
   FOS_REQUIRE(npairs >= 0 && npairs % 16 == 0 &&
                   (npairs == 0 || (feat && w1_packed && w2_packed && b1 && b2 && w3 && logit)),
               "fos_pred_mlp: bad arguments (npairs=%lld, must be a multiple of 16)",
               (long long)npairs);
+  FOS_REQUIRE(!nup || (cap >= 16 && cap % 16 == 0 && npairs % cap == 0),
+              "fos_pred_mlp: with nup, cap (%d) must be a multiple of 16 dividing npairs", cap);
   if (npairs == 0) return 0;
   const int64_t groups = npairs / 16;                 // one wave step per 16 pairs
   int64_t blocks = (groups + 3) / 4;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                      feat, npairs, (const bf16x4 *)w1_packed, (const bf16x4 *)w2_packed, b1, b2,
-                     w3, b3, logit);
+                     w3, b3, logit, nup, cap);
   FOS_CHECK_LAUNCH("k_mlp_infer");
   return 0;
 }
 
-int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *ncand, int n, int cap,
+int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *nup, int n, int cap,
                  int32_t *fsc, int32_t *face_ids, int32_t *face_counts, void *stream) {
-  FOS_REQUIRE(n >= 0 && cap >= 1 && (n == 0 || (logit && cand && ncand && fsc)),
+  FOS_REQUIRE(n >= 0 && cap >= 1 && (n == 0 || (logit && cand && nup && fsc)),
               "fos_pred_fsc: bad arguments (n=%d cap=%d)", n, cap);
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_pred_fsc, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
-                     logit, cand, ncand, n, cap, fsc, face_ids, face_counts);
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     fsc, n, 0);
+  FOS_CHECK_LAUNCH("k_fill_i32");
+  hipLaunchKernelGGL(k_pred_fsc, dim3(blocks_for((int64_t)n * 64)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, logit, cand, nup, n, cap, fsc, face_ids, face_counts);
   FOS_CHECK_LAUNCH("k_pred_fsc");
   return 0;
 }
 
-int fos_pred_labels(const int32_t *cand, const int32_t *ncand, int n, int cap,
+int fos_pred_labels(const int32_t *cand, const int32_t *nup, int n, int cap,
                     const int32_t *neighbor_list, const int32_t *neighbor_count, float *labels,
                     void *stream) {
   FOS_REQUIRE(n >= 0 && cap >= 1 &&
-                  (n == 0 || (cand && ncand && neighbor_list && neighbor_count && labels)),
+                  (n == 0 || (cand && nup && neighbor_list && neighbor_count && labels)),
               "fos_pred_labels: bad arguments (n=%d cap=%d)", n, cap);
   if (n == 0) return 0;
   const int64_t slots = (int64_t)n * cap;
   hipLaunchKernelGGL(k_pred_labels, dim3(blocks_for(slots)), dim3(FOS_BLOCK), 0,
-                     (hipStream_t)stream, cand, ncand, n, cap, neighbor_list, neighbor_count,
+                     (hipStream_t)stream, cand, nup, n, cap, neighbor_list, neighbor_count,
                      labels);
   FOS_CHECK_LAUNCH("k_pred_labels");
   return 0;

@@ -123,6 +123,7 @@ class FacePredictor:
         self.ci = torch.empty(self.n_max, dtype=i32, device=d)
         self.cand = torch.empty((self.n_max, self.cap), dtype=i32, device=d)
         self.ncand = torch.empty(self.n_max, dtype=i32, device=d)
+        self.nup = torch.empty(self.n_max, dtype=i32, device=d)
         self.trunc = torch.empty(self.n_max, dtype=i32, device=d)
         self.feat = torch.empty((self.n_max * self.cap, SLOTS), dtype=f32, device=d)
         self.logit = torch.empty(self.n_max * self.cap, dtype=f32, device=d)
@@ -139,23 +140,23 @@ class FacePredictor:
                 fl.ptr(self.cc), fl.ptr(self.cs), fl.ptr(self.ci), st)
         fl.call("fos_pred_candidates", fl.ptr(self.pos), fl.ptr(rad), n, self.G, self.L,
                 self.mult, self.cap, fl.ptr(self.cs), fl.ptr(self.cc), fl.ptr(self.ci),
-                fl.ptr(self.cand), fl.ptr(self.ncand), fl.ptr(self.trunc), st)
+                fl.ptr(self.cand), fl.ptr(self.ncand), fl.ptr(self.nup), fl.ptr(self.trunc), st)
 
     def features(self, rad, n, r_mean=None):
         if r_mean is None:
             r_mean = float(rad[:n].mean().item())
         fl.call("fos_pred_features", fl.ptr(self.pos), fl.ptr(rad), n, self.L, float(r_mean),
-                CLIP, fl.ptr(self.cand), fl.ptr(self.ncand), self.cap, fl.ptr(self.feat),
-                fl.stream())
+                CLIP, fl.ptr(self.cand), fl.ptr(self.ncand), fl.ptr(self.nup), self.cap,
+                fl.ptr(self.feat), fl.stream())
 
     def logits(self, n):
         pk = self.packed
         fl.call("fos_pred_mlp", fl.ptr(self.feat), n * self.cap, fl.ptr(pk["w1p"]),
                 fl.ptr(pk["w2p"]), fl.ptr(pk["b1"]), fl.ptr(pk["b2"]), fl.ptr(pk["w3"]),
-                pk["b3"], fl.ptr(self.logit), fl.stream())
+                pk["b3"], fl.ptr(self.logit), fl.ptr(self.nup), self.cap, fl.stream())
 
     def fsc(self, n, fsc, face_ids=None, face_counts=None):
-        fl.call("fos_pred_fsc", fl.ptr(self.logit), fl.ptr(self.cand), fl.ptr(self.ncand), n,
+        fl.call("fos_pred_fsc", fl.ptr(self.logit), fl.ptr(self.cand), fl.ptr(self.nup), n,
                 self.cap, fl.ptr(fsc), fl.ptr(face_ids), fl.ptr(face_counts), fl.stream())
 
     def predict(self, pos, rad, n, fsc, face_ids=None, face_counts=None, r_mean=None):
@@ -166,7 +167,7 @@ class FacePredictor:
         self.fsc(n, fsc, face_ids, face_counts)
 
     def labels(self, n, neighbor_list, neighbor_count, out):
-        fl.call("fos_pred_labels", fl.ptr(self.cand), fl.ptr(self.ncand), n, self.cap,
+        fl.call("fos_pred_labels", fl.ptr(self.cand), fl.ptr(self.nup), n, self.cap,
                 fl.ptr(neighbor_list), fl.ptr(neighbor_count), fl.ptr(out), fl.stream())
 
 

@@ -70,9 +70,9 @@ SIGNATURES = {
     "fos_topo_knn": [P, P, I, I, F, F, I, P, P, P, P, P],
     "fos_topo_ema": [P, P, I, F, P],
     "fos_topo_radii": [P, P, I, F, F, F, F, F, F, P],
-    "fos_pred_candidates": [P, P, I, I, F, F, I, P, P, P, P, P, P, P],
-    "fos_pred_features": [P, P, I, F, F, F, P, P, I, P, P],
-    "fos_pred_mlp": [P, I64, P, P, P, P, P, F, P, P],
+    "fos_pred_candidates": [P, P, I, I, F, F, I, P, P, P, P, P, P, P, P],
+    "fos_pred_features": [P, P, I, F, F, F, P, P, P, I, P, P],
+    "fos_pred_mlp": [P, I64, P, P, P, P, P, F, P, P, I, P],
     "fos_pred_fsc": [P, P, P, I, I, P, P, P, P],
     "fos_pred_labels": [P, P, I, I, P, P, P, P],
     "fos_mlp_param_count": [ctypes.POINTER(I)],

@@ -51,7 +51,7 @@ def collect(cfg, warm, frames, seed):
         y = lab.view(n, pred.cap)
         found = (y > 0.5).sum().item()
         data.append({"feat": pred.feat.clone(), "y": lab.clone(), "cand": pred.cand.clone(),
-                     "ncand": pred.ncand.clone(),
+                     "nup": pred.nup.clone(),
                      "fsc": s.neighbor_count.clone(), "found": found,
                      "neighbours": int(s.neighbor_count.sum().item()),
                      "trunc": int(pred.trunc.sum().item()),
@@ -73,13 +73,13 @@ def evaluate(params, frames, n, cap):
     for d in frames:
         fl.call("fos_pred_mlp", fl.ptr(d["feat"]), n * cap, fl.ptr(pk["w1p"]),
                 fl.ptr(pk["w2p"]), fl.ptr(pk["b1"]), fl.ptr(pk["b2"]), fl.ptr(pk["w3"]),
-                pk["b3"], fl.ptr(logit), fl.stream())
+                pk["b3"], fl.ptr(logit), fl.ptr(d["nup"]), cap, fl.stream())
         y = d["y"].view(n, cap)
         valid = y >= 0
         pos = (logit.view(n, cap) > 0) & valid
         accs.append(float(((pos == (y > 0.5)) & valid).sum().item()) /
                     max(1.0, float(valid.sum().item())))
-        fl.call("fos_pred_fsc", fl.ptr(logit), fl.ptr(d["cand"]), fl.ptr(d["ncand"]), n, cap,
+        fl.call("fos_pred_fsc", fl.ptr(logit), fl.ptr(d["cand"]), fl.ptr(d["nup"]), n, cap,
                 fl.ptr(fsc), None, None, fl.stream())
         err, bias = fp.fsc_error(fsc, d["fsc"])
         errs.append(err)
@@ -104,7 +104,8 @@ def main():
     data, pred = collect(cfg, args.warm, args.frames, args.seed)
     cap = pred.cap
     train, hold = data[:-args.holdout], data[-args.holdout:]
-    cover = sum(d["found"] for d in data) / max(1, sum(d["neighbours"] for d in data))
+    # each accepted pair appears in two neighbour rows and in one scored slot
+    cover = 2 * sum(d["found"] for d in data) / max(1, sum(d["neighbours"] for d in data))
     X = torch.cat([d["feat"] for d in train])
     Y = torch.cat([d["y"] for d in train])
     valid = torch.nonzero(Y >= 0).squeeze(1)
@@ -136,7 +137,7 @@ def main():
     for d in train:
         fl.call("fos_pred_mlp", fl.ptr(d["feat"]), n * cap, fl.ptr(pk["w1p"]),
                 fl.ptr(pk["w2p"]), fl.ptr(pk["b1"]), fl.ptr(pk["b2"]), fl.ptr(pk["w3"]),
-                pk["b3"], fl.ptr(logit), fl.stream())
+                pk["b3"], fl.ptr(logit), fl.ptr(d["nup"]), cap, fl.stream())
         vals.append(logit[d["y"] >= 0].clone())
     allv = torch.cat(vals)
     sub = allv[torch.randperm(allv.numel(), device=allv.device, generator=gen)[:4_000_000]]

@@ -39,8 +39,10 @@ def gwrap(p, L=1.0):
 
 
 def brute_candidates(pos, rad, G, L=1.0, mult=fp.MULT, cap=fp.CAP):
-    """Same fp32 expressions as the kernel; order: 27 cells around i's cell
-    (dx, dy, dz), ascending index within a cell."""
+    """Same fp32 expressions as the kernel; scan order: 27 cells around i's
+    cell (dx, dy, dz), ascending index within a cell; the first cap kept.
+    Returns (row, nup): partners j > i first, then j < i, each in scan
+    order."""
     n = len(rad)
     pw = gwrap(pos, L)                       # the grid rebuild stores this
     half, inv = F(0.5 * L), F(1.0 / L)
@@ -54,7 +56,9 @@ def brute_candidates(pos, rad, G, L=1.0, mult=fp.MULT, cap=fp.CAP):
         rel = (cell[ok] - cell[i] + 1) % G                 # 0, 1, 2 for -1, 0, +1
         q = (rel[:, 0] * 3 + rel[:, 1]) * 3 + rel[:, 2]
         order = np.lexsort((ok, q))
-        out.append(ok[order][:cap])
+        kept = ok[order][:cap]
+        up, lo = kept[kept > i], kept[kept < i]
+        out.append((np.concatenate([up, lo]), len(up)))
     return out
 
 
@@ -65,13 +69,35 @@ def test_candidates_match_brute_force(gpu):
     pr = fp.FacePredictor(fp.init_params(), n, 1.0, 0.04)
     tp, tr = dev(pos), dev(rad)
     pr.candidates(tp, tr, n)
-    cand, nc = host(pr.cand), host(pr.ncand)
+    cand, nc, nu = host(pr.cand), host(pr.ncand), host(pr.nup)
     want = brute_candidates(pos, rad, G)
     for i in range(n):
-        assert nc[i] == len(want[i]), i
-        assert (cand[i, :nc[i]] == want[i]).all(), i
+        row, up = want[i]
+        assert nc[i] == len(row) and nu[i] == up, i
+        assert (cand[i, :nc[i]] == row).all(), i
         assert (cand[i, nc[i]:] == -1).all()
     assert nc.mean() > 3 and nc.max() <= fp.CAP
+    assert (host(pr.trunc) == 0).all()
+
+
+def test_candidates_truncated_rows(gpu):
+    """A cap smaller than the neighbourhood keeps the first cap hits in scan
+    order (both halves of the row) and flags the row."""
+    n, cap = 1200, 16
+    pos, rad = fi.uniform(n, 0.03, 0.05, seed=11)
+    G = fp.hash_res(1.0, 0.05)
+    pr = fp.FacePredictor(fp.init_params(), n, 1.0, 0.05, cap=cap)
+    tp, tr = dev(pos), dev(rad)
+    pr.candidates(tp, tr, n)
+    cand, nc, nu, trunc = host(pr.cand), host(pr.ncand), host(pr.nup), host(pr.trunc)
+    want = brute_candidates(pos, rad, G, cap=cap)
+    full = brute_candidates(pos, rad, G, cap=10 ** 6)
+    for i in range(n):
+        row, up = want[i]
+        assert nc[i] == len(row) and nu[i] == up, i
+        assert (cand[i, :nc[i]] == row).all(), i
+        assert trunc[i] == (len(full[i][0]) > cap), i
+    assert trunc.sum() > 0
 
 
 def face_area(d, ri, rj, ks, rks, kids, clip_r, near=8):
@@ -165,9 +191,10 @@ def test_features_match_numpy(gpu):
     r_mean = float(rad.mean())
     pr.features(tr, n, r_mean)
     got = host(pr.feat).reshape(n, fp.CAP, 16)
-    cand, nc = host(pr.cand), host(pr.ncand)
+    cand, nc, nu = host(pr.cand), host(pr.ncand), host(pr.nup)
     want = feat_reference(pos, rad, cand, nc, r_mean)
-    valid = (np.arange(fp.CAP)[None, :] < nc[:, None]) & (cand > np.arange(n)[:, None])
+    valid = np.arange(fp.CAP)[None, :] < nu[:, None]
+    assert (cand[valid] > np.nonzero(valid)[0]).all()
     got, want = got[valid], want[valid]                  # only slots with j > i are scored
     cont = [0, 1, 2, 3, 7, 8, 9, 10, 11]
     bad = {f: float(np.abs(got[:, f] - want[:, f]).max()) for f in range(12)}
@@ -196,7 +223,7 @@ def test_mlp_mfma_matches_numpy(gpu):
     out = torch.empty(P, dtype=torch.float32, device="cuda")
     gpu.call("fos_pred_mlp", gpu.ptr(tf), P, gpu.ptr(pk["w1p"]), gpu.ptr(pk["w2p"]),
              gpu.ptr(pk["b1"]), gpu.ptr(pk["b2"]), gpu.ptr(pk["w3"]), pk["b3"], gpu.ptr(out),
-             gpu.stream())
+             None, 0, gpu.stream())
     got = host(out)
     want = fp.mlp_reference(feat, params)
     # fp32 accumulation order differs (MFMA vs numpy): a hidden value can land
@@ -205,54 +232,73 @@ def test_mlp_mfma_matches_numpy(gpu):
     assert err.mean() < 2e-3 * np.abs(want).mean()
     assert err.max() < 0.05 * (1 + np.abs(want).max())
     assert (np.sign(got) == np.sign(want)).mean() > 0.998
+    # rows of 64 slots with a scored prefix nup: the groups of 16 that hold
+    # scored slots get the same logits, the others are not written
+    cap = 64
+    nup = rng.integers(0, cap + 1, P // cap).astype(np.int32)
+    part = torch.full((P,), 7.0, dtype=torch.float32, device="cuda")
+    gpu.call("fos_pred_mlp", gpu.ptr(tf), P, gpu.ptr(pk["w1p"]), gpu.ptr(pk["w2p"]),
+             gpu.ptr(pk["b1"]), gpu.ptr(pk["b2"]), gpu.ptr(pk["w3"]), pk["b3"], gpu.ptr(part),
+             gpu.ptr(dev(nup)), cap, gpu.stream())
+    pp = host(part).reshape(-1, cap)
+    grp = (np.arange(cap) // 16 * 16)[None, :] < nup[:, None]
+    assert (pp[grp] == got.reshape(-1, cap)[grp]).all()
+    assert (pp[~grp] == 7.0).all()
 
 
 def test_fsc_rows_and_labels(gpu):
-    """Symmetric candidate rows (as the candidate kernel makes them): a pair is
-    kept when its lower row's logit is > 0 and counts in both rows."""
+    """Rows as the candidate kernel makes them (partners j > i first, nup[i]
+    of them): a pair is kept when the logit of its slot in the lower index's
+    row is > 0 and counts for both particles."""
     n, cap = 300, fp.CAP
     rng = np.random.default_rng(8)
     adj = np.triu(rng.random((n, n)) < 0.08, 1)
     adj = adj | adj.T
+    adj[7, :] = False                         # an isolated particle
+    adj[:, 7] = False
     cand = np.full((n, cap), -1, np.int32)
-    nc = np.zeros(n, np.int32)
+    nu = np.zeros(n, np.int32)
     for i in range(n):
-        row = rng.permutation(np.nonzero(adj[i])[0])[:cap]
+        nb = rng.permutation(np.nonzero(adj[i])[0])
+        row = np.concatenate([nb[nb > i], nb[nb < i]])[:cap]
         cand[i, :len(row)] = row
-        nc[i] = len(row)
+        nu[i] = min(int((nb > i).sum()), cap)
     logit = rng.normal(0, 1, (n, cap)).astype(F)
-    fsc = torch.empty(n, dtype=torch.int32, device="cuda")
+    fsc = torch.full((n,), 99, dtype=torch.int32, device="cuda")
     ids = torch.empty((n, 32), dtype=torch.int32, device="cuda")
     cts = torch.empty((n, 32), dtype=torch.int32, device="cuda")
-    gpu.call("fos_pred_fsc", gpu.ptr(dev(logit)), gpu.ptr(dev(cand)), gpu.ptr(dev(nc)), n, cap,
+    gpu.call("fos_pred_fsc", gpu.ptr(dev(logit)), gpu.ptr(dev(cand)), gpu.ptr(dev(nu)), n, cap,
              gpu.ptr(fsc), gpu.ptr(ids), gpu.ptr(cts), gpu.stream())
     f, ii, cc = host(fsc), host(ids), host(cts)
-
-    def kept(a, b):
-        lo, hi = min(a, b), max(a, b)
-        s = list(cand[lo, :nc[lo]]).index(hi)
-        return logit[lo, s] > 0
+    want = np.zeros(n, np.int64)
     for i in range(n):
-        keep = [int(j) for j in cand[i, :nc[i]] if kept(i, int(j))]
-        assert f[i] == len(keep), i
-        up = [j for j in keep if j > i][:32]
-        assert list(ii[i, :len(up)]) == up and (ii[i, len(up):] == -1).all()
-        assert (cc[i, :len(up)] == 1).all() and (cc[i, len(up):] == 0).all()
+        up = [int(cand[i, s]) for s in range(nu[i]) if logit[i, s] > 0]
+        want[i] += len(up)
+        for j in up:
+            want[j] += 1
+        r = up[:32]
+        assert list(ii[i, :len(r)]) == r and (ii[i, len(r):] == -1).all(), i
+        assert (cc[i, :len(r)] == 1).all() and (cc[i, len(r):] == 0).all()
+    assert (f == want).all() and f[7] == 0
+    # without face rows only fsc is written
+    gpu.call("fos_pred_fsc", gpu.ptr(dev(logit)), gpu.ptr(dev(cand)), gpu.ptr(dev(nu)), n, cap,
+             gpu.ptr(fsc), None, None, gpu.stream())
+    assert (host(fsc) == want).all()
     nl = np.full((n, 32), -1, np.int32)
     nn = rng.integers(0, 33, n).astype(np.int32)
     for i in range(n):
         nl[i, :nn[i]] = rng.integers(0, n, nn[i])
     y = torch.empty(n * cap, dtype=torch.float32, device="cuda")
-    gpu.call("fos_pred_labels", gpu.ptr(dev(cand)), gpu.ptr(dev(nc)), n, cap, gpu.ptr(dev(nl)),
+    gpu.call("fos_pred_labels", gpu.ptr(dev(cand)), gpu.ptr(dev(nu)), n, cap, gpu.ptr(dev(nl)),
              gpu.ptr(dev(nn)), gpu.ptr(y), gpu.stream())
     yy = host(y).reshape(n, cap)
     for i in range(n):
         for s in range(cap):
-            if s >= nc[i] or cand[i, s] < i:
-                want = -1.0
+            if s >= nu[i]:
+                want_y = -1.0
             else:
-                want = float(cand[i, s] in set(nl[i, :nn[i]].tolist()))
-            assert yy[i, s] == want
+                want_y = float(cand[i, s] in set(nl[i, :nn[i]].tolist()))
+            assert yy[i, s] == want_y
 
 
 def numpy_grad(params, feat, y, idx, pos_w):
@@ -300,6 +346,6 @@ def test_training_gradients_and_loss(gpu):
     out = torch.empty(P, dtype=torch.float32, device="cuda")
     gpu.call("fos_pred_mlp", gpu.ptr(tf), P, gpu.ptr(pr["w1p"]), gpu.ptr(pr["w2p"]),
              gpu.ptr(pr["b1"]), gpu.ptr(pr["b2"]), gpu.ptr(pr["w3"]), pr["b3"], gpu.ptr(out),
-             gpu.stream())
+             None, 0, gpu.stream())
     acc = ((host(out) > 0) == (y > 0.5)).mean()
     assert acc > 0.95
