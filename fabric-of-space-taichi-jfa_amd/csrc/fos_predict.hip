@@ -32,90 +32,184 @@ __device__ __forceinline__ float dot3f(float x, float y, float z) {
 
 // ---------------------------------------------------------------------------
 // Candidates: every j != i with |d(i, j)| < mult * (ri + rj), scanning the 27
-// cells around i's cell (dx, dy, dz order, ascending index inside a cell),
-// at most `cap` kept in that order.  A row lists the partners j > i first
-// (nup[i] of them, in scan order), then the partners j < i.  Needs a hash
-// cell of at least mult * 2 * r_max.
+// cells around i's hash cell (dx, dy, dz order, ascending index inside a
+// cell), at most `cap` kept in that order.  A row lists the partners j > i
+// first (nup[i] of them, in scan order), then the partners j < i.  Needs a
+// hash cell of at least mult * 2 * r_max.
 // ---------------------------------------------------------------------------
-#define PRED_CAP_MAX 128  // row capacity limit (LDS staging of the j < i block)
+#define PRED_CAP_MAX 128   // row capacity limit (LDS staging of the j < i block)
+#define PRED_NB_MAX 2048   // particles of a 27-cell neighbourhood staged in LDS
 
+// Row state of one particle while a wave scans its neighbourhood 64 hits at
+// a time: partners j > i go straight to the front of the row, partners
+// j < i to the wave's LDS buffer (copied behind them by cand_finish).
+struct CandRow {
+  int cu, cl, tr;
+};
+
+__device__ __forceinline__ void cand_emit(CandRow &st, bool hit, int j, int i, int lane,
+                                          int cap, int32_t *__restrict__ row,
+                                          int32_t *__restrict__ low) {
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mh = __ballot(hit);
+  const bool keep = hit && st.cu + st.cl + __popcll(mh & below) < cap;
+  const unsigned long long mk = __ballot(keep);
+  const unsigned long long mu = __ballot(keep && j > i);
+  if (keep) {
+    if (j > i)
+      row[st.cu + __popcll(mu & below)] = j;
+    else
+      low[st.cl + __popcll(mk & ~mu & below)] = j;
+  }
+  st.tr |= (mh & ~mk) != 0ull;
+  st.cu += __popcll(mu);
+  st.cl += __popcll(mk & ~mu);
+}
+
+__device__ __forceinline__ void cand_finish(const CandRow &st, int i, int lane, int cap,
+                                            int32_t *__restrict__ row,
+                                            const int32_t *__restrict__ low,
+                                            int32_t *__restrict__ ncand,
+                                            int32_t *__restrict__ nup,
+                                            int32_t *__restrict__ trunc) {
+  __builtin_amdgcn_wave_barrier();
+  for (int t = lane; t < st.cl; t += 64) row[st.cu + t] = low[t];
+  for (int t = st.cu + st.cl + lane; t < cap; t += 64) row[t] = -1;
+  if (lane == 0) {
+    ncand[i] = st.cu + st.cl;
+    nup[i] = st.cu;
+    trunc[i] = st.tr;
+  }
+}
+
+// One workgroup per hash cell: the 27-cell neighbourhood (cell/occupant
+// order) is staged in LDS as (wrapped position, radius) and particle ids,
+// then each wave scans it for the cell's particles in turn.  A
+// neighbourhood of more than PRED_NB_MAX particles is left to
+// k_pred_cand_wave (its particles get ncand = -1 here).
 __global__ void __launch_bounds__(256)
-    k_pred_cand(const float *__restrict__ pos, const float *__restrict__ rad, int n, int G,
-                float L, float half_l, float inv_l, float mult, int cap,
-                const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
-                const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
-                int32_t *__restrict__ ncand, int32_t *__restrict__ nup,
-                int32_t *__restrict__ trunc) {
-  // One wave per particle: lane l tests the l-th particle of a run of cells
-  // and ballots keep the hits in scan order.  Partners j > i go straight to
-  // the front of the row; partners j < i are staged in LDS and copied behind
-  // them at the end.
+    k_pred_cand_cell(const float *__restrict__ pos, const float *__restrict__ rad, int G,
+                     float L, float inv_l, float mult, int cap,
+                     const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+                     const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
+                     int32_t *__restrict__ ncand, int32_t *__restrict__ nup,
+                     int32_t *__restrict__ trunc) {
+  __shared__ float4 s_p[PRED_NB_MAX];
+  __shared__ int32_t s_j[PRED_NB_MAX];
+  __shared__ int32_t s_low[4][PRED_CAP_MAX];
+  __shared__ int32_t s_beg[27], s_off[28];
+  const int c = (int)blockIdx.x;
+  const int nc = cc[c];
+  if (nc == 0) return;
+  const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c0 = cs[c];
+  if (w == 0) {
+    const int cx = c / (G * G), cy = (c / G) % G, cz = c % G;
+    int b = 0, m = 0;
+    if (lane < 27) {
+      const int id = (wrap1(cx + lane / 9 - 1, G) * G + wrap1(cy + (lane / 3) % 3 - 1, G)) * G +
+                     wrap1(cz + lane % 3 - 1, G);
+      b = cs[id];
+      m = cc[id];
+    }
+    int x = m;  // inclusive prefix over the 27 cells
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane < 27) {
+      s_beg[lane] = b;
+      s_off[lane + 1] = x;
+    }
+    if (lane == 0) s_off[0] = 0;
+  }
+  __syncthreads();
+  const int T = s_off[27];
+  if (T > PRED_NB_MAX) {
+    for (int p = tid; p < nc; p += 256) ncand[ci[c0 + p]] = -1;
+    return;
+  }
+  for (int t = tid; t < T; t += 256) {
+    int q = 0;  // cell of slot t: s_off[q] <= t < s_off[q + 1]
+#pragma unroll
+    for (int h = 16; h >= 1; h >>= 1)
+      if (q + h < 28 && s_off[q + h] <= t) q += h;
+    const int j = ci[s_beg[q] + (t - s_off[q])];
+    s_j[t] = j;
+    s_p[t] = make_float4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], rad[j]);
+  }
+  __syncthreads();
+  for (int p = w; p < nc; p += 4) {
+    const int i = ci[c0 + p];
+    const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2], ri = rad[i];
+    CandRow st = {0, 0, 0};
+    int32_t *row = cand + (int64_t)i * cap;
+    for (int t0 = 0; t0 < T; t0 += 64) {
+      const int t = t0 + lane;
+      bool hit = false;
+      int j = -1;
+      if (t < T) {
+        j = s_j[t];
+        if (j != i) {
+          const float4 q = s_p[t];
+          const float ddx = gwrap(q.x - px, L, inv_l);
+          const float ddy = gwrap(q.y - py, L, inv_l);
+          const float ddz = gwrap(q.z - pz, L, inv_l);
+          const float th = mult * (ri + q.w);
+          hit = dot3f(ddx, ddy, ddz) < th * th;
+        }
+      }
+      cand_emit(st, hit, j, i, lane, cap, row, s_low[w]);
+    }
+    cand_finish(st, i, lane, cap, row, s_low[w], ncand, nup, trunc);
+    __builtin_amdgcn_wave_barrier();  // s_low is reused by the wave's next particle
+  }
+}
+
+// One wave per particle over the 27 cells in global memory; only for
+// particles the cell kernel left (ncand[i] < 0).
+__global__ void __launch_bounds__(256)
+    k_pred_cand_wave(const float *__restrict__ pos, const float *__restrict__ rad, int n, int G,
+                     float L, float half_l, float inv_l, float mult, int cap,
+                     const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+                     const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
+                     int32_t *__restrict__ ncand, int32_t *__restrict__ nup,
+                     int32_t *__restrict__ trunc) {
   __shared__ int32_t s_low[4][PRED_CAP_MAX];
   const int lane = (int)(threadIdx.x & 63);
   const int w = (int)(threadIdx.x >> 6);
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (i >= n) return;
+  if (i >= n || ncand[i] >= 0) return;
   const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2], ri = rad[i];
-  const int cx = wrap_fast(centre_axis(px, G, L, half_l, inv_l), G);
-  const int cy = wrap_fast(centre_axis(py, G, L, half_l, inv_l), G);
-  const int cz = wrap_fast(centre_axis(pz, G, L, half_l, inv_l), G);
-  // the three cells cz-1..cz+1 of a column are contiguous in the CSR order
-  // unless z wraps
-  const bool run = cz >= 1 && cz + 1 < G;
-  int cu = 0, cl = 0, tr = 0;
+  // the particle's own hash cell, as the grid rebuild computed it
+  const int cx = hash_axis(gwrap(px, L, inv_l), G, half_l, inv_l);
+  const int cy = hash_axis(gwrap(py, L, inv_l), G, half_l, inv_l);
+  const int cz = hash_axis(gwrap(pz, L, inv_l), G, half_l, inv_l);
+  CandRow st = {0, 0, 0};
   int32_t *row = cand + (int64_t)i * cap;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  for (int dx = -1; dx <= 1; ++dx)
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int rbase = (wrap1(cx + dx, G) * G + wrap1(cy + dy, G)) * G;
-      for (int part = 0; part < (run ? 1 : 3); ++part) {
-        int b, e;
-        if (run) {
-          b = cs[rbase + cz - 1];
-          e = cs[rbase + cz + 1] + cc[rbase + cz + 1];
-        } else {
-          const int id = rbase + wrap1(cz + part - 1, G);
-          b = cs[id];
-          e = b + cc[id];
-        }
-        for (int k0 = b; k0 < e; k0 += 64) {
-          const int k = k0 + lane;
-          bool hit = false;
-          int j = -1;
-          if (k < e) {
-            j = ci[k];
-            if (j != i) {
-              const float ddx = gwrap(pos[3 * j] - px, L, inv_l);
-              const float ddy = gwrap(pos[3 * j + 1] - py, L, inv_l);
-              const float ddz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
-              const float th = mult * (ri + rad[j]);
-              hit = dot3f(ddx, ddy, ddz) < th * th;
-            }
-          }
-          const unsigned long long mh = __ballot(hit);
-          const bool keep = hit && cu + cl + __popcll(mh & below) < cap;
-          const unsigned long long mk = __ballot(keep);
-          const unsigned long long mu = __ballot(keep && j > i);
-          if (keep) {
-            if (j > i)
-              row[cu + __popcll(mu & below)] = j;
-            else
-              s_low[w][cl + __popcll(mk & ~mu & below)] = j;
-          }
-          tr |= (mh & ~mk) != 0ull;
-          cu += __popcll(mu);
-          cl += __popcll(mk & ~mu);
+  for (int q = 0; q < 27; ++q) {
+    const int id = (wrap1(cx + q / 9 - 1, G) * G + wrap1(cy + (q / 3) % 3 - 1, G)) * G +
+                   wrap1(cz + q % 3 - 1, G);
+    const int b = cs[id], e = b + cc[id];
+    for (int k0 = b; k0 < e; k0 += 64) {
+      const int k = k0 + lane;
+      bool hit = false;
+      int j = -1;
+      if (k < e) {
+        j = ci[k];
+        if (j != i) {
+          const float ddx = gwrap(pos[3 * j] - px, L, inv_l);
+          const float ddy = gwrap(pos[3 * j + 1] - py, L, inv_l);
+          const float ddz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
+          const float th = mult * (ri + rad[j]);
+          hit = dot3f(ddx, ddy, ddz) < th * th;
         }
       }
+      cand_emit(st, hit, j, i, lane, cap, row, s_low[w]);
     }
-  __builtin_amdgcn_wave_barrier();
-  for (int t = lane; t < cl; t += 64) row[cu + t] = s_low[w][t];
-  for (int t = cu + cl + lane; t < cap; t += 64) row[t] = -1;
-  if (lane == 0) {
-    ncand[i] = cu + cl;
-    nup[i] = cu;
-    trunc[i] = tr;
   }
+  cand_finish(st, i, lane, cap, row, s_low[w], ncand, nup, trunc);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,10 +354,11 @@ __global__ void __launch_bounds__(256)
         if (g == h) continue;
         const float kk = la[g] * tx + lb[g] * ty;
         const float rr = lc[g] - (la[g] * x0 + lb[g] * y0);
+        const float e = rr * __builtin_amdgcn_rcpf(kk);  // edge parameter, ~1 ulp
         if (kk > 1e-12f)
-          hi = fminf(hi, rr / kk);
+          hi = fminf(hi, e);
         else if (kk < -1e-12f)
-          lo = fmaxf(lo, rr / kk);
+          lo = fmaxf(lo, e);
         else if (rr < 0.0f || (rr == 0.0f && g < h))
           hi = -1e30f;  // parallel and tighter (or an equal line met first)
       }
@@ -309,11 +404,8 @@ __global__ void __launch_bounds__(256)
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ short f2bf(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-  return (short)(u >> 16);
-}
+// fp32 -> bf16 bits, round to nearest even (gfx950: v_cvt_pk_bf16_f32)
+__device__ __forceinline__ short f2bf(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 
 __global__ void __launch_bounds__(256)
     k_mlp_infer(const float *__restrict__ feat, int64_t npairs,
@@ -576,11 +668,17 @@ int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float 
               "fos_pred_candidates: bad arguments (n=%d G=%d cap=%d)", n, G, cap);
   FOS_REQUIRE(cap <= PRED_CAP_MAX, "fos_pred_candidates: cap %d exceeds %d", cap, PRED_CAP_MAX);
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_pred_cand, dim3(blocks_for((int64_t)n * 64)), dim3(FOS_BLOCK), 0,
+  FOS_REQUIRE((int64_t)G * G * G <= 0x7fffffff, "fos_pred_candidates: G=%d too large", G);
+  const float inv_l = (float)(1.0 / (double)L);
+  hipLaunchKernelGGL(k_pred_cand_cell, dim3((unsigned)(G * G * G)), dim3(256), 0,
+                     (hipStream_t)stream, pos, rad, G, L, inv_l, mult, cap, cell_start,
+                     cell_count, cell_indices, cand, ncand, nup, truncated);
+  FOS_CHECK_LAUNCH("k_pred_cand_cell");
+  hipLaunchKernelGGL(k_pred_cand_wave, dim3(blocks_for((int64_t)n * 64)), dim3(FOS_BLOCK), 0,
                      (hipStream_t)stream,
-                     pos, rad, n, G, L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), mult,
+                     pos, rad, n, G, L, (float)(0.5 * (double)L), inv_l, mult,
                      cap, cell_start, cell_count, cell_indices, cand, ncand, nup, truncated);
-  FOS_CHECK_LAUNCH("k_pred_cand");
+  FOS_CHECK_LAUNCH("k_pred_cand_wave");
   return 0;
 }
 
