@@ -42,24 +42,32 @@ __device__ __forceinline__ float dot3f(float x, float y, float z) {
 
 // Row state of one particle while a wave scans its neighbourhood 64 hits at
 // a time: partners j > i go straight to the front of the row, partners
-// j < i to the wave's LDS buffer (copied behind them by cand_finish).
+// j < i to the wave's LDS buffers (copied behind them by cand_finish).
+// Next to each kept partner k the row of `rel` (when given) holds
+// (pk - pi wrapped to the minimum image, |pk - pi|^2 - rk^2).
 struct CandRow {
   int cu, cl, tr;
 };
 
-__device__ __forceinline__ void cand_emit(CandRow &st, bool hit, int j, int i, int lane,
-                                          int cap, int32_t *__restrict__ row,
-                                          int32_t *__restrict__ low) {
+__device__ __forceinline__ void cand_emit(CandRow &st, bool hit, int j, float4 r, int i,
+                                          int lane, int cap, int32_t *__restrict__ row,
+                                          float4 *__restrict__ relrow,
+                                          int32_t *__restrict__ low, float4 *__restrict__ lowr) {
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned long long mh = __ballot(hit);
   const bool keep = hit && st.cu + st.cl + __popcll(mh & below) < cap;
   const unsigned long long mk = __ballot(keep);
   const unsigned long long mu = __ballot(keep && j > i);
   if (keep) {
-    if (j > i)
-      row[st.cu + __popcll(mu & below)] = j;
-    else
-      low[st.cl + __popcll(mk & ~mu & below)] = j;
+    if (j > i) {
+      const int o = st.cu + __popcll(mu & below);
+      row[o] = j;
+      if (relrow) relrow[o] = r;
+    } else {
+      const int o = st.cl + __popcll(mk & ~mu & below);
+      low[o] = j;
+      lowr[o] = r;
+    }
   }
   st.tr |= (mh & ~mk) != 0ull;
   st.cu += __popcll(mu);
@@ -68,18 +76,36 @@ __device__ __forceinline__ void cand_emit(CandRow &st, bool hit, int j, int i, i
 
 __device__ __forceinline__ void cand_finish(const CandRow &st, int i, int lane, int cap,
                                             int32_t *__restrict__ row,
+                                            float4 *__restrict__ relrow,
                                             const int32_t *__restrict__ low,
+                                            const float4 *__restrict__ lowr,
                                             int32_t *__restrict__ ncand,
                                             int32_t *__restrict__ nup,
                                             int32_t *__restrict__ trunc) {
   __builtin_amdgcn_wave_barrier();
-  for (int t = lane; t < st.cl; t += 64) row[st.cu + t] = low[t];
+  for (int t = lane; t < st.cl; t += 64) {
+    row[st.cu + t] = low[t];
+    if (relrow) relrow[st.cu + t] = lowr[t];
+  }
   for (int t = st.cu + st.cl + lane; t < cap; t += 64) row[t] = -1;
   if (lane == 0) {
     ncand[i] = st.cu + st.cl;
     nup[i] = st.cu;
     trunc[i] = st.tr;
   }
+}
+
+// The hit test of the candidate kernels and the rel entry of a hit.
+__device__ __forceinline__ bool cand_test(float qx, float qy, float qz, float rk, float px,
+                                          float py, float pz, float ri, float L, float inv_l,
+                                          float mult, float4 &r) {
+  const float ddx = gwrap(qx - px, L, inv_l);
+  const float ddy = gwrap(qy - py, L, inv_l);
+  const float ddz = gwrap(qz - pz, L, inv_l);
+  const float d2 = dot3f(ddx, ddy, ddz);
+  const float th = mult * (ri + rk);
+  r = make_float4(ddx, ddy, ddz, d2 - rk * rk);
+  return d2 < th * th;
 }
 
 // One workgroup per hash cell: the 27-cell neighbourhood (cell/occupant
@@ -93,10 +119,11 @@ __global__ void __launch_bounds__(256)
                      const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
                      const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
                      int32_t *__restrict__ ncand, int32_t *__restrict__ nup,
-                     int32_t *__restrict__ trunc) {
+                     float4 *__restrict__ rel, int32_t *__restrict__ trunc) {
   __shared__ float4 s_p[PRED_NB_MAX];
   __shared__ int32_t s_j[PRED_NB_MAX];
   __shared__ int32_t s_low[4][PRED_CAP_MAX];
+  __shared__ float4 s_lowr[4][PRED_CAP_MAX];
   __shared__ int32_t s_beg[27], s_off[28];
   const int c = (int)blockIdx.x;
   const int nc = cc[c];
@@ -145,25 +172,23 @@ __global__ void __launch_bounds__(256)
     const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2], ri = rad[i];
     CandRow st = {0, 0, 0};
     int32_t *row = cand + (int64_t)i * cap;
+    float4 *relrow = rel ? rel + (int64_t)i * cap : nullptr;
     for (int t0 = 0; t0 < T; t0 += 64) {
       const int t = t0 + lane;
       bool hit = false;
       int j = -1;
+      float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (t < T) {
         j = s_j[t];
         if (j != i) {
           const float4 q = s_p[t];
-          const float ddx = gwrap(q.x - px, L, inv_l);
-          const float ddy = gwrap(q.y - py, L, inv_l);
-          const float ddz = gwrap(q.z - pz, L, inv_l);
-          const float th = mult * (ri + q.w);
-          hit = dot3f(ddx, ddy, ddz) < th * th;
+          hit = cand_test(q.x, q.y, q.z, q.w, px, py, pz, ri, L, inv_l, mult, r);
         }
       }
-      cand_emit(st, hit, j, i, lane, cap, row, s_low[w]);
+      cand_emit(st, hit, j, r, i, lane, cap, row, relrow, s_low[w], s_lowr[w]);
     }
-    cand_finish(st, i, lane, cap, row, s_low[w], ncand, nup, trunc);
-    __builtin_amdgcn_wave_barrier();  // s_low is reused by the wave's next particle
+    cand_finish(st, i, lane, cap, row, relrow, s_low[w], s_lowr[w], ncand, nup, trunc);
+    __builtin_amdgcn_wave_barrier();  // the LDS buffers are reused by the next particle
   }
 }
 
@@ -175,8 +200,9 @@ __global__ void __launch_bounds__(256)
                      const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
                      const int32_t *__restrict__ ci, int32_t *__restrict__ cand,
                      int32_t *__restrict__ ncand, int32_t *__restrict__ nup,
-                     int32_t *__restrict__ trunc) {
+                     float4 *__restrict__ rel, int32_t *__restrict__ trunc) {
   __shared__ int32_t s_low[4][PRED_CAP_MAX];
+  __shared__ float4 s_lowr[4][PRED_CAP_MAX];
   const int lane = (int)(threadIdx.x & 63);
   const int w = (int)(threadIdx.x >> 6);
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -188,6 +214,7 @@ __global__ void __launch_bounds__(256)
   const int cz = hash_axis(gwrap(pz, L, inv_l), G, half_l, inv_l);
   CandRow st = {0, 0, 0};
   int32_t *row = cand + (int64_t)i * cap;
+  float4 *relrow = rel ? rel + (int64_t)i * cap : nullptr;
   for (int q = 0; q < 27; ++q) {
     const int id = (wrap1(cx + q / 9 - 1, G) * G + wrap1(cy + (q / 3) % 3 - 1, G)) * G +
                    wrap1(cz + q % 3 - 1, G);
@@ -196,20 +223,28 @@ __global__ void __launch_bounds__(256)
       const int k = k0 + lane;
       bool hit = false;
       int j = -1;
+      float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (k < e) {
         j = ci[k];
-        if (j != i) {
-          const float ddx = gwrap(pos[3 * j] - px, L, inv_l);
-          const float ddy = gwrap(pos[3 * j + 1] - py, L, inv_l);
-          const float ddz = gwrap(pos[3 * j + 2] - pz, L, inv_l);
-          const float th = mult * (ri + rad[j]);
-          hit = dot3f(ddx, ddy, ddz) < th * th;
-        }
+        if (j != i)
+          hit = cand_test(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], rad[j], px, py, pz, ri, L,
+                          inv_l, mult, r);
       }
-      cand_emit(st, hit, j, i, lane, cap, row, s_low[w]);
+      cand_emit(st, hit, j, r, i, lane, cap, row, relrow, s_low[w], s_lowr[w]);
     }
   }
-  cand_finish(st, i, lane, cap, row, s_low[w], ncand, nup, trunc);
+  cand_finish(st, i, lane, cap, row, relrow, s_low[w], s_lowr[w], ncand, nup, trunc);
+}
+
+// Flat list of the scored pairs (i, cand[i][s]), s < nup[i]: pair p belongs
+// to particle pair_i[p] and slot p - off[i]; off[n] is the number of pairs.
+__global__ void k_pred_pair_map(const int32_t *__restrict__ nup, int n, int32_t *__restrict__ off,
+                                int32_t *__restrict__ pair_i) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = off[i], m = nup[i];
+  for (int s2 = 0; s2 < m; ++s2) pair_i[o + s2] = i;
+  if (i == n - 1) off[n] = o + m;
 }
 
 // ---------------------------------------------------------------------------
@@ -233,44 +268,30 @@ __global__ void __launch_bounds__(256)
 //   5 face area / (ri + rj)^2 (<= 4)   11 p(m) / (ri + rj)^2
 // Margins and p(m) are clamped to [-1, 1].  The candidate relation is
 // symmetric, so each unordered pair is scored once, in the row of its lower
-// index: only the first nup[i] slots (the partners j > i) are written.
-// Half a wave per particle: i's candidates (relative positions and
-// |pk - pi|^2 - rk^2) are staged in LDS and lane l scores slots l, l + 32, ...
+// index: the pairs are the flat list of k_pred_pair_map, one thread each,
+// reading i's candidates (relative positions and |pk - pi|^2 - rk^2) from
+// the rel rows the candidate kernels wrote.
 // ---------------------------------------------------------------------------
 #define PRED_NEAR 8    // planes kept for the face polygon (nearest to m)
-#define PRED_GRP 32    // lanes per particle
-#define PRED_PPB 8     // particles per 256-thread workgroup
 
 __global__ void __launch_bounds__(256)
-    k_pred_feat(const float *__restrict__ pos, const float *__restrict__ rad, int n, float L,
-                float inv_l, float inv_rbar, float clip_r, const int32_t *__restrict__ cand,
-                const int32_t *__restrict__ ncand, const int32_t *__restrict__ nup, int cap,
+    k_pred_feat(const float *__restrict__ rad, int n, float inv_rbar, float clip_r,
+                const int32_t *__restrict__ cand, const int32_t *__restrict__ ncand,
+                const float4 *__restrict__ rel, const int32_t *__restrict__ off,
+                const int32_t *__restrict__ pair_i, int cap, int64_t max_pairs,
                 float *__restrict__ feat) {
-  __shared__ float4 s_k[PRED_PPB][PRED_CAP_MAX];
-  const int g = (int)(threadIdx.x / PRED_GRP);
-  const int lane = (int)(threadIdx.x % PRED_GRP);
-  const int i = (int)blockIdx.x * PRED_PPB + g;
-  int ni = 0, nu = 0;
-  float pix = 0.0f, piy = 0.0f, piz = 0.0f;
-  if (i < n) {
-    ni = ncand[i];
-    nu = nup[i];
-    pix = pos[3 * i]; piy = pos[3 * i + 1]; piz = pos[3 * i + 2];
-    for (int q = lane; q < ni; q += PRED_GRP) {
-      const int k = cand[(int64_t)i * cap + q];
-      const float kx = gwrap(pos[3 * k] - pix, L, inv_l);
-      const float ky = gwrap(pos[3 * k + 1] - piy, L, inv_l);
-      const float kz = gwrap(pos[3 * k + 2] - piz, L, inv_l);
-      const float rk = rad[k];
-      s_k[g][q] = make_float4(kx, ky, kz, dot3f(kx, ky, kz) - rk * rk);
-    }
-  }
-  __syncthreads();
-  const float4 *sk = s_k[g];
-  for (int s = lane; s < nu; s += PRED_GRP) {
-    const int j = cand[(int64_t)i * cap + s];
+  const int64_t P = min((int64_t)off[n], max_pairs);
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int i = pair_i[p];
+    const int s = (int)(p - off[i]);
+    const int64_t rb = (int64_t)i * cap;
+    const int j = cand[rb + s];
+    const int ni = ncand[i];
+    const float4 *sk = rel + rb;
     const float ri = rad[i], rj = rad[j];
     const float4 ks = sk[s];
+    float4 kn = sk[0];  // next candidate, loaded one iteration ahead
     const float dx = ks.x, dy = ks.y, dz = ks.z;
     const float d2 = fmaxf(dot3f(dx, dy, dz), 1e-20f);
     const float sr = ri + rj, sr2 = sr * sr;
@@ -300,8 +321,9 @@ __global__ void __launch_bounds__(256)
     float mw = 1e30f;
     const float mm = t * t * d2 - pm;  // |m|^2 - p(m) = ri^2
     for (int q = 0; q < ni; ++q) {
+      const float4 kq = kn;
+      if (q + 1 < ni) kn = sk[q + 1];
       if (q == s) continue;
-      const float4 kq = sk[q];
       const float kx = kq.x, ky = kq.y, kz = kq.z, pk = kq.w;
       const float kd = kx * dx + ky * dy + kz * dz;
       // |pk - m|^2 - rk^2 - p(m), with m = t d
@@ -384,7 +406,7 @@ __global__ void __launch_bounds__(256)
     f[10] = (ri - rj) / sr;
     f[11] = fminf(1.0f, fmaxf(-1.0f, pm / sr2));
     f[12] = f[13] = f[14] = f[15] = 0.0f;
-    float4 *o = (float4 *)(feat + ((int64_t)i * cap + s) * PRED_FEAT);
+    float4 *o = (float4 *)(feat + p * PRED_FEAT);
 #pragma unroll
     for (int q = 0; q < PRED_FEAT / 4; ++q)
       o[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
@@ -397,9 +419,8 @@ __global__ void __launch_bounds__(256)
 // host in lane order), the activations the B operand.  The 16x16 f32 result
 // of a layer leaves lane l holding rows 4*(l/16)..+3 of column l%16, which
 // is the B-operand layout of the next layer's 16-row K block, so hidden
-// activations stay in registers.  A wave scores 16 pairs per step.  Given
-// nup (rows of cap slots, cap a multiple of 16), groups of 16 slots at or
-// past nup[row] are skipped and their logits left unwritten.
+// activations stay in registers.  A wave scores 16 pairs per step.  The
+// pair count is npairs, or min(npairs, *npairs_dev) when that is given.
 // ---------------------------------------------------------------------------
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -412,7 +433,7 @@ __global__ void __launch_bounds__(256)
                 const bf16x4 *__restrict__ w1p, const bf16x4 *__restrict__ w2p,
                 const float *__restrict__ b1, const float *__restrict__ b2,
                 const float *__restrict__ w3, float b3, float *__restrict__ logit,
-                const int32_t *__restrict__ nup, int cap) {
+                const int32_t *__restrict__ npairs_dev) {
   const int lane = (int)(threadIdx.x & 63);
   const int col = lane & 15, grp = lane >> 4;
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
@@ -432,12 +453,11 @@ __global__ void __launch_bounds__(256)
       ww3[mb][v] = w3[h];
     }
   }
-  for (int64_t p0 = wave * 16; p0 < npairs; p0 += waves * 16) {
-    if (nup) {  // rows of cap slots: skip groups past the row's scored prefix
-      const int64_t r = p0 / cap;
-      if ((int)(p0 - r * cap) >= nup[r]) continue;
-    }
-    const float4 x = *(const float4 *)(feat + (p0 + col) * PRED_FEAT + 4 * grp);
+  const int64_t P = npairs_dev ? min(npairs, (int64_t)*npairs_dev) : npairs;
+  for (int64_t p0 = wave * 16; p0 < P; p0 += waves * 16) {
+    const int64_t pc = p0 + col;
+    const float4 x = pc < P ? *(const float4 *)(feat + pc * PRED_FEAT + 4 * grp)
+                            : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     bf16x4 xb;
     xb[0] = f2bf(x.x);
     xb[1] = f2bf(x.y);
@@ -463,23 +483,23 @@ __global__ void __launch_bounds__(256)
     }
     part += __shfl_xor(part, 16, 64);
     part += __shfl_xor(part, 32, 64);
-    if (grp == 0) logit[p0 + col] = part + b3;
+    if (grp == 0 && pc < P) logit[pc] = part + b3;
   }
 }
 
-// FSC from the scores: a pair (i, j), j > i, is kept when the logit of its
-// slot in i's row is > 0 (sigmoid 0.5) and then counts for both i and j
-// (fsc must be zero on entry).  Face rows list the kept partners j > i in
-// candidate order, at most 32, -1 padded (the row shape
-// accumulate_face_area produces for the pressure step).  One wave per i.
+// FSC from the scores: the pair p = (i, cand[i][s]), s = p - off[i], is kept
+// when logit[p] > 0 (sigmoid 0.5) and then counts for both particles (fsc
+// must be zero on entry).  Face rows list the kept partners j > i in
+// candidate order, at most 32, -1 padded (the row shape accumulate_face_area
+// produces for the pressure step).  One wave per i.
 __global__ void __launch_bounds__(256)
     k_pred_fsc(const float *__restrict__ logit, const int32_t *__restrict__ cand,
-               const int32_t *__restrict__ nup, int n, int cap, int32_t *__restrict__ fsc,
+               const int32_t *__restrict__ off, int n, int cap, int32_t *__restrict__ fsc,
                int32_t *__restrict__ face_ids, int32_t *__restrict__ face_counts) {
   const int lane = (int)(threadIdx.x & 63);
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= n) return;
-  const int nu = nup[i];
+  const int o = off[i], nu = off[i + 1] - o;
   const unsigned long long below = (1ull << lane) - 1ull;
   int c = 0;
   for (int s0 = 0; s0 < nu; s0 += 64) {
@@ -487,7 +507,7 @@ __global__ void __launch_bounds__(256)
     bool keep = false;
     int j = -1;
     if (s < nu) {
-      keep = logit[(int64_t)i * cap + s] > 0.0f;
+      keep = logit[o + s] > 0.0f;
       j = cand[(int64_t)i * cap + s];
     }
     const unsigned long long mk = __ballot(keep);
@@ -514,23 +534,23 @@ __global__ void k_fill_i32(int32_t *__restrict__ a, int n, int32_t v) {
   if (q < n) a[q] = v;
 }
 
-// Labels for training: 1 when j is in i's JFA neighbour list, for the
-// scored slots s < nup[i] (j > i); -1 for the other slots.
-__global__ void k_pred_labels(const int32_t *__restrict__ cand, const int32_t *__restrict__ nup,
-                              int n, int cap, const int32_t *__restrict__ nl,
+// Labels for training, one per pair p = (i, cand[i][s]): 1 when cand[i][s]
+// is in i's JFA neighbour list, else 0.
+__global__ void k_pred_labels(const int32_t *__restrict__ cand, const int32_t *__restrict__ off,
+                              const int32_t *__restrict__ pair_i, int n, int cap,
+                              int64_t max_pairs, const int32_t *__restrict__ nl,
                               const int32_t *__restrict__ nc, float *__restrict__ y) {
-  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= (int64_t)n * cap) return;
-  const int i = (int)(slot / cap), s = (int)(slot - (int64_t)i * cap);
-  float v = -1.0f;  // unscored slot
-  if (s < nup[i]) {
-    const int j = cand[slot];
+  const int64_t P = min((int64_t)off[n], max_pairs);
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int i = pair_i[p];
+    const int j = cand[(int64_t)i * cap + (p - off[i])];
     const int m = min(nc[i], FOS_MAX_NEIGHBORS);
-    v = 0.0f;
+    float v = 0.0f;
     for (int q = 0; q < m; ++q)
       if (nl[(int64_t)i * FOS_MAX_NEIGHBORS + q] == j) v = 1.0f;
+    y[p] = v;
   }
-  y[slot] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -661,88 +681,104 @@ extern "C" {
 int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float L, float mult,
                         int cap, const int32_t *cell_start, const int32_t *cell_count,
                         const int32_t *cell_indices, int32_t *cand, int32_t *ncand, int32_t *nup,
-                        int32_t *truncated, void *stream) {
+                        float *rel, int32_t *truncated, void *stream) {
   FOS_REQUIRE(n >= 0 && G >= 3 && L > 0.0f && cap >= 1 && mult > 0.0f &&
                   (n == 0 || (pos && rad && cell_start && cell_count && cell_indices && cand &&
                               ncand && nup && truncated)),
               "fos_pred_candidates: bad arguments (n=%d G=%d cap=%d)", n, G, cap);
   FOS_REQUIRE(cap <= PRED_CAP_MAX, "fos_pred_candidates: cap %d exceeds %d", cap, PRED_CAP_MAX);
-  if (n == 0) return 0;
   FOS_REQUIRE((int64_t)G * G * G <= 0x7fffffff, "fos_pred_candidates: G=%d too large", G);
+  if (n == 0) return 0;
   const float inv_l = (float)(1.0 / (double)L);
   hipLaunchKernelGGL(k_pred_cand_cell, dim3((unsigned)(G * G * G)), dim3(256), 0,
                      (hipStream_t)stream, pos, rad, G, L, inv_l, mult, cap, cell_start,
-                     cell_count, cell_indices, cand, ncand, nup, truncated);
+                     cell_count, cell_indices, cand, ncand, nup, (float4 *)rel, truncated);
   FOS_CHECK_LAUNCH("k_pred_cand_cell");
   hipLaunchKernelGGL(k_pred_cand_wave, dim3(blocks_for((int64_t)n * 64)), dim3(FOS_BLOCK), 0,
                      (hipStream_t)stream,
                      pos, rad, n, G, L, (float)(0.5 * (double)L), inv_l, mult,
-                     cap, cell_start, cell_count, cell_indices, cand, ncand, nup, truncated);
+                     cap, cell_start, cell_count, cell_indices, cand, ncand, nup, (float4 *)rel,
+                     truncated);
   FOS_CHECK_LAUNCH("k_pred_cand_wave");
   return 0;
 }
 
-int fos_pred_features(const float *pos, const float *rad, int n, float L, float r_mean,
-                      float clip_r, const int32_t *cand, const int32_t *ncand,
-                      const int32_t *nup, int cap, float *feat, void *stream) {
-  FOS_REQUIRE(n >= 0 && L > 0.0f && cap >= 1 && r_mean > 0.0f && clip_r > 0.0f &&
-                  (n == 0 || (pos && rad && cand && ncand && nup && feat)),
-              "fos_pred_features: bad arguments (n=%d cap=%d)", n, cap);
-  FOS_REQUIRE(cap <= PRED_CAP_MAX, "fos_pred_features: cap %d exceeds %d", cap, PRED_CAP_MAX);
+int fos_pred_pairs(fos_ctx *ctx, const int32_t *nup, int n, int32_t *pair_off, int32_t *pair_i,
+                   void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && (n == 0 || (nup && pair_off && pair_i)),
+              "fos_pred_pairs: bad arguments (n=%d)", n);
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_pred_feat, dim3((unsigned)((n + PRED_PPB - 1) / PRED_PPB)),
-                     dim3(PRED_PPB * PRED_GRP), 0, (hipStream_t)stream, pos, rad, n, L,
-                     (float)(1.0 / (double)L), 1.0f / r_mean, clip_r, cand, ncand, nup, cap,
-                     feat);
+  hipStream_t st = (hipStream_t)stream;
+  if (exclusive_scan(ctx, nup, pair_off, n, st)) return -1;
+  hipLaunchKernelGGL(k_pred_pair_map, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, nup, n,
+                     pair_off, pair_i);
+  FOS_CHECK_LAUNCH("k_pred_pair_map");
+  return 0;
+}
+
+static unsigned pair_blocks(int64_t max_pairs) {
+  const int64_t b = (max_pairs + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+
+int fos_pred_features(const float *rad, int n, float r_mean, float clip_r, const int32_t *cand,
+                      const int32_t *ncand, const float *rel, const int32_t *pair_off,
+                      const int32_t *pair_i, int cap, int64_t max_pairs, float *feat,
+                      void *stream) {
+  FOS_REQUIRE(n >= 0 && cap >= 1 && r_mean > 0.0f && clip_r > 0.0f && max_pairs >= 0 &&
+                  (n == 0 || (rad && cand && ncand && rel && pair_off && pair_i && feat)),
+              "fos_pred_features: bad arguments (n=%d cap=%d)", n, cap);
+  if (n == 0 || max_pairs == 0) return 0;
+  hipLaunchKernelGGL(k_pred_feat, dim3(pair_blocks(max_pairs)), dim3(256), 0,
+                     (hipStream_t)stream, rad, n, 1.0f / r_mean, clip_r, cand, ncand,
+                     (const float4 *)rel, pair_off, pair_i, cap, max_pairs, feat);
   FOS_CHECK_LAUNCH("k_pred_feat");
   return 0;
 }
 
 int fos_pred_mlp(const float *feat, int64_t npairs, const void *w1_packed, const void *w2_packed,
                  const float *b1, const float *b2, const float *w3, float b3, float *logit,
-                 const int32_t *nup, int cap, void *stream) {
-  FOS_REQUIRE(npairs >= 0 && npairs % 16 == 0 &&
+                 const int32_t *npairs_dev, void *stream) {
+  FOS_REQUIRE(npairs >= 0 &&
                   (npairs == 0 || (feat && w1_packed && w2_packed && b1 && b2 && w3 && logit)),
-              "fos_pred_mlp: bad arguments (npairs=%lld, must be a multiple of 16)",
-              (long long)npairs);
-  FOS_REQUIRE(!nup || (cap >= 16 && cap % 16 == 0 && npairs % cap == 0),
-              "fos_pred_mlp: with nup, cap (%d) must be a multiple of 16 dividing npairs", cap);
+              "fos_pred_mlp: bad arguments (npairs=%lld)", (long long)npairs);
   if (npairs == 0) return 0;
-  const int64_t groups = npairs / 16;                 // one wave step per 16 pairs
+  const int64_t groups = (npairs + 15) / 16;          // one wave step per 16 pairs
   int64_t blocks = (groups + 3) / 4;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                      feat, npairs, (const bf16x4 *)w1_packed, (const bf16x4 *)w2_packed, b1, b2,
-                     w3, b3, logit, nup, cap);
+                     w3, b3, logit, npairs_dev);
   FOS_CHECK_LAUNCH("k_mlp_infer");
   return 0;
 }
 
-int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *nup, int n, int cap,
+int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *pair_off, int n, int cap,
                  int32_t *fsc, int32_t *face_ids, int32_t *face_counts, void *stream) {
-  FOS_REQUIRE(n >= 0 && cap >= 1 && (n == 0 || (logit && cand && nup && fsc)),
+  FOS_REQUIRE(n >= 0 && cap >= 1 && (n == 0 || (logit && cand && pair_off && fsc)),
               "fos_pred_fsc: bad arguments (n=%d cap=%d)", n, cap);
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
                      fsc, n, 0);
   FOS_CHECK_LAUNCH("k_fill_i32");
   hipLaunchKernelGGL(k_pred_fsc, dim3(blocks_for((int64_t)n * 64)), dim3(FOS_BLOCK), 0,
-                     (hipStream_t)stream, logit, cand, nup, n, cap, fsc, face_ids, face_counts);
+                     (hipStream_t)stream, logit, cand, pair_off, n, cap, fsc, face_ids,
+                     face_counts);
   FOS_CHECK_LAUNCH("k_pred_fsc");
   return 0;
 }
 
-int fos_pred_labels(const int32_t *cand, const int32_t *nup, int n, int cap,
-                    const int32_t *neighbor_list, const int32_t *neighbor_count, float *labels,
-                    void *stream) {
-  FOS_REQUIRE(n >= 0 && cap >= 1 &&
-                  (n == 0 || (cand && nup && neighbor_list && neighbor_count && labels)),
+int fos_pred_labels(const int32_t *cand, const int32_t *pair_off, const int32_t *pair_i, int n,
+                    int cap, int64_t max_pairs, const int32_t *neighbor_list,
+                    const int32_t *neighbor_count, float *labels, void *stream) {
+  FOS_REQUIRE(n >= 0 && cap >= 1 && max_pairs >= 0 &&
+                  (n == 0 || (cand && pair_off && pair_i && neighbor_list && neighbor_count &&
+                              labels)),
               "fos_pred_labels: bad arguments (n=%d cap=%d)", n, cap);
-  if (n == 0) return 0;
-  const int64_t slots = (int64_t)n * cap;
-  hipLaunchKernelGGL(k_pred_labels, dim3(blocks_for(slots)), dim3(FOS_BLOCK), 0,
-                     (hipStream_t)stream, cand, nup, n, cap, neighbor_list, neighbor_count,
-                     labels);
+  if (n == 0 || max_pairs == 0) return 0;
+  hipLaunchKernelGGL(k_pred_labels, dim3(pair_blocks(max_pairs)), dim3(256), 0,
+                     (hipStream_t)stream, cand, pair_off, pair_i, n, cap, max_pairs,
+                     neighbor_list, neighbor_count, labels);
   FOS_CHECK_LAUNCH("k_pred_labels");
   return 0;
 }

@@ -1,8 +1,9 @@
 """Learned face predictor: host side over the fos_pred_* / fos_mlp_* kernels.
 
 No reference counterpart.  On frames where the JFA is decimated it predicts,
-for every hash-grid candidate pair (i, j) within MULT*(ri+rj), whether i and
-j share an accepted power-diagram face, and from that the FSC of every
+for every hash-grid candidate pair (i, j) within MULT*(ri+rj) (a flat list,
+each unordered pair once), whether i and j share an accepted power-diagram
+face, and from that the FSC of every
 particle and the face rows the pressure step reads.  The plain JFA produces
 the training labels (its neighbour lists) and stays the judge.
 
@@ -125,6 +126,9 @@ class FacePredictor:
         self.ncand = torch.empty(self.n_max, dtype=i32, device=d)
         self.nup = torch.empty(self.n_max, dtype=i32, device=d)
         self.trunc = torch.empty(self.n_max, dtype=i32, device=d)
+        self.rel = torch.empty((self.n_max, self.cap, 4), dtype=f32, device=d)
+        self.pair_off = torch.empty(self.n_max + 1, dtype=i32, device=d)
+        self.pair_i = torch.empty(self.n_max * self.cap, dtype=i32, device=d)
         self.feat = torch.empty((self.n_max * self.cap, SLOTS), dtype=f32, device=d)
         self.logit = torch.empty(self.n_max * self.cap, dtype=f32, device=d)
         self.set_params(params)
@@ -140,23 +144,30 @@ class FacePredictor:
                 fl.ptr(self.cc), fl.ptr(self.cs), fl.ptr(self.ci), st)
         fl.call("fos_pred_candidates", fl.ptr(self.pos), fl.ptr(rad), n, self.G, self.L,
                 self.mult, self.cap, fl.ptr(self.cs), fl.ptr(self.cc), fl.ptr(self.ci),
-                fl.ptr(self.cand), fl.ptr(self.ncand), fl.ptr(self.nup), fl.ptr(self.trunc), st)
+                fl.ptr(self.cand), fl.ptr(self.ncand), fl.ptr(self.nup), fl.ptr(self.rel),
+                fl.ptr(self.trunc), st)
+        fl.call("fos_pred_pairs", self.ctx.handle, fl.ptr(self.nup), n, fl.ptr(self.pair_off),
+                fl.ptr(self.pair_i), st)
+
+    def npairs(self, n):
+        """Number of scored pairs of the last candidates() call (syncs)."""
+        return int(self.pair_off[n].item())
 
     def features(self, rad, n, r_mean=None):
         if r_mean is None:
             r_mean = float(rad[:n].mean().item())
-        fl.call("fos_pred_features", fl.ptr(self.pos), fl.ptr(rad), n, self.L, float(r_mean),
-                CLIP, fl.ptr(self.cand), fl.ptr(self.ncand), fl.ptr(self.nup), self.cap,
-                fl.ptr(self.feat), fl.stream())
+        fl.call("fos_pred_features", fl.ptr(rad), n, float(r_mean), CLIP, fl.ptr(self.cand),
+                fl.ptr(self.ncand), fl.ptr(self.rel), fl.ptr(self.pair_off), fl.ptr(self.pair_i),
+                self.cap, n * self.cap, fl.ptr(self.feat), fl.stream())
 
     def logits(self, n):
         pk = self.packed
         fl.call("fos_pred_mlp", fl.ptr(self.feat), n * self.cap, fl.ptr(pk["w1p"]),
                 fl.ptr(pk["w2p"]), fl.ptr(pk["b1"]), fl.ptr(pk["b2"]), fl.ptr(pk["w3"]),
-                pk["b3"], fl.ptr(self.logit), fl.ptr(self.nup), self.cap, fl.stream())
+                pk["b3"], fl.ptr(self.logit), fl.ptr(self.pair_off[n:]), fl.stream())
 
     def fsc(self, n, fsc, face_ids=None, face_counts=None):
-        fl.call("fos_pred_fsc", fl.ptr(self.logit), fl.ptr(self.cand), fl.ptr(self.nup), n,
+        fl.call("fos_pred_fsc", fl.ptr(self.logit), fl.ptr(self.cand), fl.ptr(self.pair_off), n,
                 self.cap, fl.ptr(fsc), fl.ptr(face_ids), fl.ptr(face_counts), fl.stream())
 
     def predict(self, pos, rad, n, fsc, face_ids=None, face_counts=None, r_mean=None):
@@ -167,8 +178,9 @@ class FacePredictor:
         self.fsc(n, fsc, face_ids, face_counts)
 
     def labels(self, n, neighbor_list, neighbor_count, out):
-        fl.call("fos_pred_labels", fl.ptr(self.cand), fl.ptr(self.nup), n, self.cap,
-                fl.ptr(neighbor_list), fl.ptr(neighbor_count), fl.ptr(out), fl.stream())
+        fl.call("fos_pred_labels", fl.ptr(self.cand), fl.ptr(self.pair_off), fl.ptr(self.pair_i),
+                n, self.cap, n * self.cap, fl.ptr(neighbor_list), fl.ptr(neighbor_count),
+                fl.ptr(out), fl.stream())
 
 
 def fsc_error(pred, ref):

@@ -70,11 +70,12 @@ SIGNATURES = {
     "fos_topo_knn": [P, P, I, I, F, F, I, P, P, P, P, P],
     "fos_topo_ema": [P, P, I, F, P],
     "fos_topo_radii": [P, P, I, F, F, F, F, F, F, P],
-    "fos_pred_candidates": [P, P, I, I, F, F, I, P, P, P, P, P, P, P, P],
-    "fos_pred_features": [P, P, I, F, F, F, P, P, P, I, P, P],
-    "fos_pred_mlp": [P, I64, P, P, P, P, P, F, P, P, I, P],
+    "fos_pred_candidates": [P, P, I, I, F, F, I, P, P, P, P, P, P, P, P, P],
+    "fos_pred_pairs": [P, P, I, P, P, P],
+    "fos_pred_features": [P, I, F, F, P, P, P, P, P, I, I64, P, P],
+    "fos_pred_mlp": [P, I64, P, P, P, P, P, F, P, P, P],
     "fos_pred_fsc": [P, P, P, I, I, P, P, P, P],
-    "fos_pred_labels": [P, P, I, I, P, P, P, P],
+    "fos_pred_labels": [P, P, P, I, I, I64, P, P, P, P],
     "fos_mlp_param_count": [ctypes.POINTER(I)],
     "fos_mlp_train_step": [P, P, P, P, I, F, P, P, P, I, F, P, P, P],
 }

@@ -5,7 +5,8 @@
 Runs the plain foam (JFA every frame) of the given workload, and on every
 frame after a warm-up records, for each hash-grid candidate pair, the 12
 features (fos_pred_features) and the label "j is in i's JFA neighbour list"
-(fos_pred_labels).  The last --holdout frames are kept for validation.  The
+(fos_pred_labels), one per scored pair (each unordered pair once).  The
+last --holdout frames are kept for validation.  The
 MLP is trained with the hand-written FP32 kernels (fos_mlp_train_step, Adam)
 and evaluated with the BF16 MFMA inference kernel: pair accuracy and the
 per-frame FSC errors  sum|FSC_pred - FSC_jfa| / sum FSC_jfa  (per particle)
@@ -48,10 +49,10 @@ def collect(cfg, warm, frames, seed):
         pred.candidates(s.pos, s.rad, n)
         pred.features(s.rad, n, r_scale)
         pred.labels(n, s.neighbor_list, s.neighbor_count, lab)
-        y = lab.view(n, pred.cap)
-        found = (y > 0.5).sum().item()
-        data.append({"feat": pred.feat.clone(), "y": lab.clone(), "cand": pred.cand.clone(),
-                     "nup": pred.nup.clone(),
+        P = pred.npairs(n)
+        found = (lab[:P] > 0.5).sum().item()
+        data.append({"feat": pred.feat[:P].clone(), "y": lab[:P].clone(),
+                     "cand": pred.cand.clone(), "pair_off": pred.pair_off[:n + 1].clone(),
                      "fsc": s.neighbor_count.clone(), "found": found,
                      "neighbours": int(s.neighbor_count.sum().item()),
                      "trunc": int(pred.trunc.sum().item()),
@@ -71,15 +72,12 @@ def evaluate(params, frames, n, cap):
     fsc = torch.empty(n, dtype=torch.int32, device="cuda")
     accs, errs, biases = [], [], []
     for d in frames:
-        fl.call("fos_pred_mlp", fl.ptr(d["feat"]), n * cap, fl.ptr(pk["w1p"]),
+        P = d["y"].numel()
+        fl.call("fos_pred_mlp", fl.ptr(d["feat"]), P, fl.ptr(pk["w1p"]),
                 fl.ptr(pk["w2p"]), fl.ptr(pk["b1"]), fl.ptr(pk["b2"]), fl.ptr(pk["w3"]),
-                pk["b3"], fl.ptr(logit), fl.ptr(d["nup"]), cap, fl.stream())
-        y = d["y"].view(n, cap)
-        valid = y >= 0
-        pos = (logit.view(n, cap) > 0) & valid
-        accs.append(float(((pos == (y > 0.5)) & valid).sum().item()) /
-                    max(1.0, float(valid.sum().item())))
-        fl.call("fos_pred_fsc", fl.ptr(logit), fl.ptr(d["cand"]), fl.ptr(d["nup"]), n, cap,
+                pk["b3"], fl.ptr(logit), None, fl.stream())
+        accs.append(float(((logit[:P] > 0) == (d["y"] > 0.5)).sum().item()) / max(1, P))
+        fl.call("fos_pred_fsc", fl.ptr(logit), fl.ptr(d["cand"]), fl.ptr(d["pair_off"]), n, cap,
                 fl.ptr(fsc), None, None, fl.stream())
         err, bias = fp.fsc_error(fsc, d["fsc"])
         errs.append(err)
@@ -108,7 +106,7 @@ def main():
     cover = 2 * sum(d["found"] for d in data) / max(1, sum(d["neighbours"] for d in data))
     X = torch.cat([d["feat"] for d in train])
     Y = torch.cat([d["y"] for d in train])
-    valid = torch.nonzero(Y >= 0).squeeze(1)
+    valid = torch.arange(Y.numel(), device=Y.device)
     npos = float((Y > 0.5).sum().item())
     nneg = float(valid.numel()) - npos
     print(f"[train] {len(train)} frames, {valid.numel()} pairs, positives {npos / valid.numel():.3f},"
@@ -135,10 +133,11 @@ def main():
     logit = torch.empty(n * cap, dtype=torch.float32, device="cuda")
     vals = []
     for d in train:
-        fl.call("fos_pred_mlp", fl.ptr(d["feat"]), n * cap, fl.ptr(pk["w1p"]),
+        P = d["y"].numel()
+        fl.call("fos_pred_mlp", fl.ptr(d["feat"]), P, fl.ptr(pk["w1p"]),
                 fl.ptr(pk["w2p"]), fl.ptr(pk["b1"]), fl.ptr(pk["b2"]), fl.ptr(pk["w3"]),
-                pk["b3"], fl.ptr(logit), fl.ptr(d["nup"]), cap, fl.stream())
-        vals.append(logit[d["y"] >= 0].clone())
+                pk["b3"], fl.ptr(logit), None, fl.stream())
+        vals.append(logit[:P].clone())
     allv = torch.cat(vals)
     sub = allv[torch.randperm(allv.numel(), device=allv.device, generator=gen)[:4_000_000]]
     rate = npos / float(valid.numel())

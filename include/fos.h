@@ -297,43 +297,50 @@ int fos_topo_radii(float *rad, const float *deg_eff, int n, float deg_low, float
  * stays the judge (see the frame driver). */
 
 /* Candidate pairs: every j != i within mult*(ri+rj) over the 27 hash cells
- * around i (cell size >= mult*2*r_max), in cell/occupant order (the shape of
- * topology.build_topo_candidates, topology.py:68-133), the first `cap`
- * (<= 128) kept.  Row i of cand[n][cap] lists the kept partners j > i
- * first (nup[i] of them, scan order), then those j < i (scan order), -1
- * padded; ncand[n] counts both, truncated[n] = 1 when more were found. */
+ * around i's cell (cell size >= mult*2*r_max; the CSR of fos_grid_build),
+ * in cell/occupant order (the shape of topology.build_topo_candidates,
+ * topology.py:68-133), the first `cap` (<= 128) kept.  Row i of
+ * cand[n][cap] lists the kept partners j > i first (nup[i] of them, scan
+ * order), then those j < i (scan order), -1 padded; ncand[n] counts both,
+ * truncated[n] = 1 when more were found.  When rel is given, rel[n][cap][4]
+ * holds beside each kept partner k (pk - pi at the minimum image,
+ * |pk - pi|^2 - rk^2). */
 int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float L, float mult,
                         int cap, const int32_t *cell_start, const int32_t *cell_count,
                         const int32_t *cell_indices, int32_t *cand, int32_t *ncand, int32_t *nup,
-                        int32_t *truncated, void *stream);
-/* 12 features per scored slot s < nup[i] (j > i: each unordered pair once,
- * in the row of its lower index) into feat[n*cap][16] (entries 12-15 zero;
- * other slots are not written); r_mean normalises the radii,
- * clip_r*(ri+rj) is the half-size of the square clipped to the face
- * polygon; cap <= 128. */
-int fos_pred_features(const float *pos, const float *rad, int n, float L, float r_mean,
-                      float clip_r, const int32_t *cand, const int32_t *ncand,
-                      const int32_t *nup, int cap, float *feat, void *stream);
+                        float *rel, int32_t *truncated, void *stream);
+/* Flat list of the scored pairs (i, cand[i][s]), s < nup[i] (each
+ * unordered pair once, in the row of its lower index): pair_off[n+1] is the
+ * exclusive prefix of nup (pair_off[n] = number of pairs, on the device),
+ * pair_i[p] the particle of pair p (capacity sum of nup <= n*cap). */
+int fos_pred_pairs(fos_ctx *ctx, const int32_t *nup, int n, int32_t *pair_off, int32_t *pair_i,
+                   void *stream);
+/* 12 features per pair p < min(pair_off[n], max_pairs) into
+ * feat[max_pairs][16] (entries 12-15 zero) from the rel rows of
+ * fos_pred_candidates; r_mean normalises the radii, clip_r*(ri+rj) is the
+ * half-size of the square clipped to the face polygon. */
+int fos_pred_features(const float *rad, int n, float r_mean, float clip_r, const int32_t *cand,
+                      const int32_t *ncand, const float *rel, const int32_t *pair_off,
+                      const int32_t *pair_i, int cap, int64_t max_pairs, float *feat,
+                      void *stream);
 /* MLP 12->64->64->1 on MFMA with BF16 weights/activations and FP32
- * accumulation: logit[p] for p < npairs (a multiple of 16).  w1_packed /
- * w2_packed hold the BF16 weights in MFMA lane order (face_predictor.pack).
- * With nup non-NULL the pairs are rows of cap slots (cap a multiple of 16
- * dividing npairs) and groups of 16 slots at or past nup[row] are skipped
- * (their logits are not written); with NULL every pair is scored. */
+ * accumulation: logit[p] for p < npairs, or p < min(npairs, *npairs_dev)
+ * when npairs_dev (device) is given.  w1_packed / w2_packed hold the BF16
+ * weights in MFMA lane order (face_predictor.pack). */
 int fos_pred_mlp(const float *feat, int64_t npairs, const void *w1_packed, const void *w2_packed,
                  const float *b1, const float *b2, const float *w3, float b3, float *logit,
-                 const int32_t *nup, int cap, void *stream);
-/* A pair (i, cand[i][s]), s < nup[i], is kept when logit[i*cap+s] > 0 and
+                 const int32_t *npairs_dev, void *stream);
+/* Pair p = (i, cand[i][p - pair_off[i]]) is kept when logit[p] > 0 and
  * counts in fsc of both particles (fsc[n] is overwritten); optional face
  * rows (kept partners j > i in candidate order, at most 32, -1 padded;
  * counts 1) for the pressure step (jfa.py:702-762 row shape). */
-int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *nup, int n, int cap,
+int fos_pred_fsc(const float *logit, const int32_t *cand, const int32_t *pair_off, int n, int cap,
                  int32_t *fsc, int32_t *face_ids, int32_t *face_counts, void *stream);
-/* Training labels of the scored slots s < nup[i]: 1 when cand[i][s] is in
- * i's JFA neighbour list, 0 when not; -1 for the other slots. */
-int fos_pred_labels(const int32_t *cand, const int32_t *nup, int n, int cap,
-                    const int32_t *neighbor_list, const int32_t *neighbor_count, float *labels,
-                    void *stream);
+/* Training label of each pair p < min(pair_off[n], max_pairs): 1 when its
+ * partner is in i's JFA neighbour list, else 0. */
+int fos_pred_labels(const int32_t *cand, const int32_t *pair_off, const int32_t *pair_i, int n,
+                    int cap, int64_t max_pairs, const int32_t *neighbor_list,
+                    const int32_t *neighbor_count, float *labels, void *stream);
 /* Number of FP32 MLP parameters (flat layout W1[16][64] b1 W2[64][64] b2 w3
  * b3) into host *count. */
 int fos_mlp_param_count(int *count);

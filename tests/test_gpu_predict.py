@@ -70,14 +70,24 @@ def test_candidates_match_brute_force(gpu):
     tp, tr = dev(pos), dev(rad)
     pr.candidates(tp, tr, n)
     cand, nc, nu = host(pr.cand), host(pr.ncand), host(pr.nup)
+    rel = host(pr.rel)
     want = brute_candidates(pos, rad, G)
+    pw = gwrap(pos)
     for i in range(n):
         row, up = want[i]
         assert nc[i] == len(row) and nu[i] == up, i
         assert (cand[i, :nc[i]] == row).all(), i
         assert (cand[i, nc[i]:] == -1).all()
+        d = gwrap(pw[row] - pw[i])
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        assert np.array_equal(rel[i, :nc[i], :3], d), i
+        assert np.array_equal(rel[i, :nc[i], 3], d2 - rad[row] * rad[row]), i
     assert nc.mean() > 3 and nc.max() <= fp.CAP
     assert (host(pr.trunc) == 0).all()
+    # flat pair list: exclusive prefix of nup, owner of every pair
+    off, pi = host(pr.pair_off)[:n + 1], host(pr.pair_i)
+    assert off[0] == 0 and (np.diff(off) == nu).all() and pr.npairs(n) == nu.sum()
+    assert (pi[:off[n]] == np.repeat(np.arange(n), nu)).all()
 
 
 def test_candidates_truncated_rows(gpu):
@@ -190,12 +200,14 @@ def test_features_match_numpy(gpu):
     pr.candidates(tp, tr, n)
     r_mean = float(rad.mean())
     pr.features(tr, n, r_mean)
-    got = host(pr.feat).reshape(n, fp.CAP, 16)
+    got = host(pr.feat)
     cand, nc, nu = host(pr.cand), host(pr.ncand), host(pr.nup)
+    P = pr.npairs(n)
+    got = got.reshape(-1, 16)[:P]
     want = feat_reference(pos, rad, cand, nc, r_mean)
-    valid = np.arange(fp.CAP)[None, :] < nu[:, None]
-    assert (cand[valid] > np.nonzero(valid)[0]).all()
-    got, want = got[valid], want[valid]                  # only slots with j > i are scored
+    valid = np.arange(fp.CAP)[None, :] < nu[:, None]     # pair p = (i, s) in row-major order
+    assert (cand[valid] > np.nonzero(valid)[0]).all()    # only pairs with j > i are scored
+    want = want[valid]
     cont = [0, 1, 2, 3, 7, 8, 9, 10, 11]
     bad = {f: float(np.abs(got[:, f] - want[:, f]).max()) for f in range(12)}
     assert np.allclose(got[:, cont], want[:, cont], rtol=1e-4, atol=2e-5), " ".join(f"{k}:{v:.3g}" for k, v in bad.items())
@@ -223,7 +235,7 @@ def test_mlp_mfma_matches_numpy(gpu):
     out = torch.empty(P, dtype=torch.float32, device="cuda")
     gpu.call("fos_pred_mlp", gpu.ptr(tf), P, gpu.ptr(pk["w1p"]), gpu.ptr(pk["w2p"]),
              gpu.ptr(pk["b1"]), gpu.ptr(pk["b2"]), gpu.ptr(pk["w3"]), pk["b3"], gpu.ptr(out),
-             None, 0, gpu.stream())
+             None, gpu.stream())
     got = host(out)
     want = fp.mlp_reference(feat, params)
     # fp32 accumulation order differs (MFMA vs numpy): a hidden value can land
@@ -232,24 +244,21 @@ def test_mlp_mfma_matches_numpy(gpu):
     assert err.mean() < 2e-3 * np.abs(want).mean()
     assert err.max() < 0.05 * (1 + np.abs(want).max())
     assert (np.sign(got) == np.sign(want)).mean() > 0.998
-    # rows of 64 slots with a scored prefix nup: the groups of 16 that hold
-    # scored slots get the same logits, the others are not written
-    cap = 64
-    nup = rng.integers(0, cap + 1, P // cap).astype(np.int32)
+    # pair count on the device (not a multiple of 16): the first 1000 pairs get
+    # the same logits, the rest is not written
     part = torch.full((P,), 7.0, dtype=torch.float32, device="cuda")
+    cnt = dev(np.array([1000], np.int32))
     gpu.call("fos_pred_mlp", gpu.ptr(tf), P, gpu.ptr(pk["w1p"]), gpu.ptr(pk["w2p"]),
              gpu.ptr(pk["b1"]), gpu.ptr(pk["b2"]), gpu.ptr(pk["w3"]), pk["b3"], gpu.ptr(part),
-             gpu.ptr(dev(nup)), cap, gpu.stream())
-    pp = host(part).reshape(-1, cap)
-    grp = (np.arange(cap) // 16 * 16)[None, :] < nup[:, None]
-    assert (pp[grp] == got.reshape(-1, cap)[grp]).all()
-    assert (pp[~grp] == 7.0).all()
+             gpu.ptr(cnt), gpu.stream())
+    pp = host(part)
+    assert (pp[:1000] == got[:1000]).all() and (pp[1000:] == 7.0).all()
 
 
 def test_fsc_rows_and_labels(gpu):
     """Rows as the candidate kernel makes them (partners j > i first, nup[i]
-    of them): a pair is kept when the logit of its slot in the lower index's
-    row is > 0 and counts for both particles."""
+    of them) and the flat pair list over them: a pair is kept when its logit
+    is > 0 and counts for both particles."""
     n, cap = 300, fp.CAP
     rng = np.random.default_rng(8)
     adj = np.triu(rng.random((n, n)) < 0.08, 1)
@@ -263,16 +272,18 @@ def test_fsc_rows_and_labels(gpu):
         row = np.concatenate([nb[nb > i], nb[nb < i]])[:cap]
         cand[i, :len(row)] = row
         nu[i] = min(int((nb > i).sum()), cap)
-    logit = rng.normal(0, 1, (n, cap)).astype(F)
+    off = np.concatenate([[0], np.cumsum(nu)]).astype(np.int32)
+    P = int(off[-1])
+    logit = rng.normal(0, 1, P).astype(F)
     fsc = torch.full((n,), 99, dtype=torch.int32, device="cuda")
     ids = torch.empty((n, 32), dtype=torch.int32, device="cuda")
     cts = torch.empty((n, 32), dtype=torch.int32, device="cuda")
-    gpu.call("fos_pred_fsc", gpu.ptr(dev(logit)), gpu.ptr(dev(cand)), gpu.ptr(dev(nu)), n, cap,
+    gpu.call("fos_pred_fsc", gpu.ptr(dev(logit)), gpu.ptr(dev(cand)), gpu.ptr(dev(off)), n, cap,
              gpu.ptr(fsc), gpu.ptr(ids), gpu.ptr(cts), gpu.stream())
     f, ii, cc = host(fsc), host(ids), host(cts)
     want = np.zeros(n, np.int64)
     for i in range(n):
-        up = [int(cand[i, s]) for s in range(nu[i]) if logit[i, s] > 0]
+        up = [int(cand[i, s]) for s in range(nu[i]) if logit[off[i] + s] > 0]
         want[i] += len(up)
         for j in up:
             want[j] += 1
@@ -281,24 +292,31 @@ def test_fsc_rows_and_labels(gpu):
         assert (cc[i, :len(r)] == 1).all() and (cc[i, len(r):] == 0).all()
     assert (f == want).all() and f[7] == 0
     # without face rows only fsc is written
-    gpu.call("fos_pred_fsc", gpu.ptr(dev(logit)), gpu.ptr(dev(cand)), gpu.ptr(dev(nu)), n, cap,
+    gpu.call("fos_pred_fsc", gpu.ptr(dev(logit)), gpu.ptr(dev(cand)), gpu.ptr(dev(off)), n, cap,
              gpu.ptr(fsc), None, None, gpu.stream())
     assert (host(fsc) == want).all()
+    # the pair list of fos_pred_pairs over these nup
+    ctx = fp.fl.Context(n, 1, 27)
+    toff = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    tpi = torch.full((n * cap,), -7, dtype=torch.int32, device="cuda")
+    gpu.call("fos_pred_pairs", ctx.handle, gpu.ptr(dev(nu)), n, gpu.ptr(toff), gpu.ptr(tpi),
+             gpu.stream())
+    assert (host(toff) == off).all()
+    pi = host(tpi)
+    assert (pi[:P] == np.repeat(np.arange(n), nu)).all() and (pi[P:] == -7).all()
     nl = np.full((n, 32), -1, np.int32)
     nn = rng.integers(0, 33, n).astype(np.int32)
     for i in range(n):
         nl[i, :nn[i]] = rng.integers(0, n, nn[i])
-    y = torch.empty(n * cap, dtype=torch.float32, device="cuda")
-    gpu.call("fos_pred_labels", gpu.ptr(dev(cand)), gpu.ptr(dev(nu)), n, cap, gpu.ptr(dev(nl)),
-             gpu.ptr(dev(nn)), gpu.ptr(y), gpu.stream())
-    yy = host(y).reshape(n, cap)
+    y = torch.full((n * cap,), 5.0, dtype=torch.float32, device="cuda")
+    gpu.call("fos_pred_labels", gpu.ptr(dev(cand)), gpu.ptr(toff), gpu.ptr(tpi), n, cap, n * cap,
+             gpu.ptr(dev(nl)), gpu.ptr(dev(nn)), gpu.ptr(y), gpu.stream())
+    yy = host(y)
     for i in range(n):
-        for s in range(cap):
-            if s >= nu[i]:
-                want_y = -1.0
-            else:
-                want_y = float(cand[i, s] in set(nl[i, :nn[i]].tolist()))
-            assert yy[i, s] == want_y
+        for s in range(nu[i]):
+            assert yy[off[i] + s] == float(cand[i, s] in set(nl[i, :nn[i]].tolist()))
+    assert (yy[P:] == 5.0).all()
+    del ctx
 
 
 def numpy_grad(params, feat, y, idx, pos_w):
@@ -346,6 +364,6 @@ def test_training_gradients_and_loss(gpu):
     out = torch.empty(P, dtype=torch.float32, device="cuda")
     gpu.call("fos_pred_mlp", gpu.ptr(tf), P, gpu.ptr(pr["w1p"]), gpu.ptr(pr["w2p"]),
              gpu.ptr(pr["b1"]), gpu.ptr(pr["b2"]), gpu.ptr(pr["w3"]), pr["b3"], gpu.ptr(out),
-             None, 0, gpu.stream())
+             None, gpu.stream())
     acc = ((host(out) > 0) == (y > 0.5)).mean()
     assert acc > 0.95
