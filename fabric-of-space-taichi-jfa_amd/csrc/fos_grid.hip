@@ -262,7 +262,107 @@ __global__ void __launch_bounds__(1024)
 }
 
 // Single-workgroup rebuild on or off ("grid_block" in fos_set_param).
-int g_grid_block = 1;
+// Rebuild by cell ranges: workgroup w owns cells [w*cpw, (w+1)*cpw) (at
+// most GR_CELLS).  It reads every particle's cell, counts its own cells and
+// the particles in earlier cells (its global offset), scans its counts,
+// places its particles in arbitrary order within a cell and then sorts each
+// cell by particle index (the stable order of grid.py:285-309).  No
+// workgroup depends on another, so one launch does count, scan and scatter.
+// Up to GR_LIST particles of a range are staged in LDS; a range with more is
+// placed in cell_indices directly.
+#define GR_CELLS 1024
+#define GR_LIST 2048
+
+__device__ __forceinline__ void sort_run(int32_t *a, int m) {
+  for (int x = 1; x < m; ++x) {
+    const int v = a[x];
+    int y = x - 1;
+    while (y >= 0 && a[y] > v) {
+      a[y + 1] = a[y];
+      --y;
+    }
+    a[y + 1] = v;
+  }
+}
+
+__global__ void __launch_bounds__(1024)
+    k_grid_build_ranges(const int32_t *__restrict__ cell_of, int n, int C, int cpw,
+                        int32_t *__restrict__ count, int32_t *__restrict__ start,
+                        int32_t *__restrict__ idx) {
+  __shared__ int s_cnt[GR_CELLS], s_ex[GR_CELLS];
+  __shared__ int s_li[GR_LIST], s_lc[GR_LIST], s_out[GR_LIST];
+  __shared__ int s_w[16], s_m;
+  const int t = (int)threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int lo = (int)blockIdx.x * cpw;
+  const int hi = min(C, lo + cpw);
+  const int nc = hi - lo;
+  for (int k = t; k < nc; k += 1024) s_cnt[k] = 0;
+  if (t == 0) s_m = 0;
+  __syncthreads();
+  int before = 0;
+  for (int i = t; i < n; i += 1024) {
+    const int c = cell_of[i];
+    if (c < lo) {
+      ++before;
+    } else if (c < hi) {
+      atomicAdd(&s_cnt[c - lo], 1);
+      const int k = atomicAdd(&s_m, 1);
+      if (k < GR_LIST) {
+        s_li[k] = i;
+        s_lc[k] = c - lo;
+      }
+    }
+  }
+  for (int o = 32; o; o >>= 1) before += __shfl_xor(before, o, 64);
+  if (lane == 0) s_w[wid] = before;
+  __syncthreads();
+  int base = 0;  // particles in cells before lo
+#pragma unroll
+  for (int w = 0; w < 16; ++w) base += s_w[w];
+  const int m = s_m;
+  __syncthreads();
+  // exclusive scan of the range's counts, one cell per thread
+  const int v = t < nc ? s_cnt[t] : 0;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wid] = x;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < wid; ++w) wbase += s_w[w];
+  if (t < nc) {
+    const int ex = wbase + x - v;
+    s_ex[t] = ex;
+    s_cnt[t] = 0;  // becomes the placement cursor
+    count[lo + t] = v;
+    start[lo + t] = base + ex;
+  }
+  __syncthreads();
+  if (m <= GR_LIST) {
+    for (int k = t; k < m; k += 1024) {
+      const int c = s_lc[k];
+      s_out[s_ex[c] + atomicAdd(&s_cnt[c], 1)] = s_li[k];
+    }
+    __syncthreads();
+    for (int k = t; k < nc; k += 1024) sort_run(s_out + s_ex[k], s_cnt[k]);
+    __syncthreads();
+    for (int k = t; k < m; k += 1024) idx[base + k] = s_out[k];
+  } else {
+    for (int i = t; i < n; i += 1024) {
+      const int c = cell_of[i];
+      if (c >= lo && c < hi) idx[base + s_ex[c - lo] + atomicAdd(&s_cnt[c - lo], 1)] = i;
+    }
+    __threadfence();
+    __syncthreads();
+    for (int k = t; k < nc; k += 1024) sort_run(idx + base + s_ex[k], s_cnt[k]);
+  }
+}
+
+// 0: multi-kernel rebuild; 1: one workgroup where it fits (else 2);
+// 2: cell-range workgroups.
+int g_grid_block = 2;
 
 extern "C" {
 
@@ -313,7 +413,24 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
               "fos_grid_build: bad arguments (n=%d G=%d)", n, G);
   const int C = G * G * G;
   hipStream_t st = (hipStream_t)stream;
-  if (n >= 1 && n <= GRID_BLOCK_N && C <= GRID_BLOCK_C && g_grid_block) {
+  const bool one_block = n >= 1 && n <= GRID_BLOCK_N && C <= GRID_BLOCK_C && g_grid_block == 1;
+  if (g_grid_block && !one_block) {
+    FOS_REQUIRE(n == 0 || (cell_indices && pos), "fos_grid_build: NULL pos or cell_indices");
+    if (n > 0) {
+      hipLaunchKernelGGL(k_grid_cells, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, n, G,
+                         L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), ctx->cell_of);
+      FOS_CHECK_LAUNCH("k_grid_cells");
+    }
+    int cpw = (C + 191) / 192;  // about 192 workgroups
+    cpw = cpw < 64 ? 64 : (cpw > GR_CELLS ? GR_CELLS : cpw);
+    const int wgs = (C + cpw - 1) / cpw;
+    hipLaunchKernelGGL(k_grid_build_ranges, dim3((unsigned)wgs), dim3(1024), 0, st,
+                       (const int32_t *)ctx->cell_of, n, C, cpw, cell_count, cell_start,
+                       cell_indices);
+    FOS_CHECK_LAUNCH("k_grid_build_ranges");
+    return 0;
+  }
+  if (one_block) {
     FOS_REQUIRE(cell_indices && pos, "fos_grid_build: NULL pos or cell_indices");
     hipLaunchKernelGGL(k_grid_cells, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, n, G, L,
                        (float)(0.5 * (double)L), (float)(1.0 / (double)L), ctx->cell_of);

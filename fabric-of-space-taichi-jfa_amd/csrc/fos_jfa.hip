@@ -1418,7 +1418,7 @@ int fos_set_param(const char *name, int value) {
                 "face_emit_tx must be 0, 1, 2 or 4");
     g_emit_tx = value;
   } else if (!strcmp(name, "grid_block")) {
-    FOS_REQUIRE(value == 0 || value == 1, "grid_block must be 0 or 1");
+    FOS_REQUIRE(value == 0 || value == 1 || value == 2, "grid_block must be 0, 1 or 2");
     g_grid_block = value;
   } else {
     FOS_REQUIRE(false, "fos_set_param: unknown parameter '%s'", name);

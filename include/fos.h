@@ -47,8 +47,10 @@ const char *fos_last_error(void);
  *   "face_emit_tx"     0,1,2,4  face-pair emission: 0 merges pairs within a
  *                      wave; 1/2/4 merge within a workgroup tile of that
  *                      many x-planes in LDS (default 2)
- *   "grid_block"       0,1    1: hash-grid rebuilds with n <= 16256 and
- *                      cells <= 16384 run as one workgroup (default)
+ *   "grid_block"       0,1,2  hash-grid rebuild: 0 count/scan/scatter/rank
+ *                      kernels; 1 one workgroup when n <= 16256 and cells
+ *                      <= 16384, else as 2; 2 one workgroup per range of
+ *                      cells, independent of each other (default)
  * They change speed only, never results.  No reference counterpart (Taichi
  * picks launch shapes itself). */
 int fos_set_param(const char *name, int value);

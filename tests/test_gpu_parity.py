@@ -452,13 +452,14 @@ def with_param(gpu, name, value, fn):
         gpu.call("fos_set_param", name.encode(), old.value)
 
 
-@pytest.mark.parametrize("block", [1, 0])
+@pytest.mark.parametrize("block", [2, 1, 0])
 @pytest.mark.parametrize("n,G_,seed", [(2000, 13, 0), (10000, 21, 1), (5000, 3, 2), (1, 3, 6),
-                                       (16256, 25, 7), (16257, 25, 8)])
+                                       (16256, 25, 7), (16257, 25, 8), (50000, 45, 9)])
 def test_grid_build_bit_exact(gpu, orc, n, G_, seed, block):
-    """One-workgroup rebuild (block=1, up to 16256 particles and 16384 cells)
-    and the multi-kernel path give the oracle's grid, wrapped positions and
-    in-cell ascending order, at and just past the one-workgroup limits."""
+    """Cell-range workgroups (block=2), one workgroup (block=1, up to 16256
+    particles and 16384 cells, else as 2) and the multi-kernel path give the
+    oracle's grid, wrapped positions and in-cell ascending order, at and just
+    past the one-workgroup limits."""
     pos, _ = fi.uniform(n, 0.01, 0.02, seed=seed)
     pos[::5] += F(0.75)                       # unwrapped inputs
     print(f"[grid] n={n} G={G_} block={block}", flush=True)
@@ -475,6 +476,21 @@ def test_grid_clustered_cells(gpu, orc):
     p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
     tp, tcc, tcs, tci = gpu_grid(gpu, pos, n, G_)
     assert np.array_equal(host(tci), ci) and np.array_equal(host(tcs), cs)
+
+
+@pytest.mark.parametrize("block", [2, 0])
+def test_grid_crowded_range(gpu, orc, block):
+    """6000 particles in a few cells: a cell range holds more particles than
+    its workgroup stages in LDS, and cells hold hundreds."""
+    n, G_ = 6000, 34
+    rng = np.random.default_rng(12)
+    pos = (rng.random((n, 3)) * np.array([0.016, 0.06, 0.06]) + 0.1).astype(F)
+    p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
+    assert cc.max() > 300
+    tp, tcc, tcs, tci = with_param(gpu, "grid_block", block, lambda: gpu_grid(gpu, pos, n, G_))
+    assert np.array_equal(host(tp), p)
+    assert np.array_equal(host(tcc), cc) and np.array_equal(host(tcs), cs)
+    assert np.array_equal(host(tci), ci)
 
 
 def test_pbd_maxoverlap_bit_exact(gpu, orc):
