@@ -325,19 +325,20 @@ __global__ void k_pressure_delta(int n, int frame, const float *__restrict__ rad
   delta_r[i] = pressure_delta(i, n, frame, rad, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
 }
 
-// One whole Jacobi iteration (dynamics.py:1497-1557) from rin into rout with
-// 16 lanes per particle (16 particles per 256-thread block): lane q evaluates
-// the q-th budgeted neighbour of the rotation, lane 0 adds the terms in
-// rotation order (the serial order, bit for bit).  A budget above 16 falls
-// back to the serial loop.  stats (when given) gets max|dr| and the changed
-// count.
-__global__ void __launch_bounds__(256)
-    k_pressure_step(int n, int frame, const float *__restrict__ rin, float *__restrict__ rout,
-                    float *__restrict__ delta_r, const int32_t *__restrict__ face_ids,
-                    const int32_t *__restrict__ fsc, float k, float p_exp, float inv_p,
-                    float cap, int max_nei, float r_min, float r_max, float *stats) {
-  const int q = (int)(threadIdx.x & 15);
-  const int i = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+// One Jacobi iteration (dynamics.py:1497-1557) for particle i from rin into
+// rout, by a group of 16 lanes (q = lane in the group): lane q evaluates the
+// q-th budgeted neighbour of the rotation, lane 0 adds the terms in rotation
+// order (the serial order, bit for bit).  A budget above 16 falls back to
+// the serial loop.  stats (when given) gets max|dr| and the changed count.
+// Every lane of the group must call it (the shuffles span the group).
+__device__ __forceinline__ void pressure_group(int i, int q, int n, int frame,
+                                               const float *__restrict__ rin,
+                                               float *__restrict__ rout,
+                                               float *__restrict__ delta_r,
+                                               const int32_t *__restrict__ face_ids,
+                                               const int32_t *__restrict__ fsc, float k,
+                                               float p_exp, float inv_p, float cap, int max_nei,
+                                               float r_min, float r_max, float *stats) {
   const bool live = i < n;
   const int fc = live ? fsc[i] : 0;
   const int an = fc < max_nei ? fc : max_nei;
@@ -378,7 +379,18 @@ __global__ void __launch_bounds__(256)
       atomicMax((int *)&stats[0], __float_as_int(a));
     }
   }
-  rout[i] = fmaxf(r_min, fminf(r_max, rin[i] + d));
+  rout[i] = fmaxf(r_min, fminf(r_max, ri + d));
+}
+
+// One iteration per launch, 16 particles per 256-thread block.
+__global__ void __launch_bounds__(256)
+    k_pressure_step(int n, int frame, const float *__restrict__ rin, float *__restrict__ rout,
+                    float *__restrict__ delta_r, const int32_t *__restrict__ face_ids,
+                    const int32_t *__restrict__ fsc, float k, float p_exp, float inv_p,
+                    float cap, int max_nei, float r_min, float r_max, float *stats) {
+  pressure_group((int)(blockIdx.x * 16 + (threadIdx.x >> 4)), (int)(threadIdx.x & 15), n, frame,
+                 rin, rout, delta_r, face_ids, fsc, k, p_exp, inv_p, cap, max_nei, r_min, r_max,
+                 stats);
 }
 
 // dynamics.py:1546-1557

@@ -571,6 +571,37 @@ def test_pressure_iterations(gpu, orc):
     assert np.array_equal(host(tr), r)
 
 
+@pytest.mark.parametrize("n,iters,max_nei", [(2000, 20, 10), (2000, 7, 24), (50000, 3, 10)])
+def test_pressure_run(gpu, orc, n, iters, max_nei):
+    """fos_pressure_run (all iterations of a frame) equals the oracle's
+    iterations with the same frame; the stats are the last iteration's."""
+    res = 64 if n <= 2000 else 128
+    voxel = fc.f32(1 / res)
+    rmin, rmax = (0.02, 0.04) if n <= 2000 else (0.008, 0.016)
+    pos, rad = fi.uniform(n, rmin, rmax, seed=3)
+    lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, voxel, 8)
+    ids, cts = orc.accumulate_faces(orc.majority(lab, res), res, n)
+    _, _, _, _, fs = orc.finalize_faces(n, rad, voxel, ids, cts, np.zeros((n, 32), F))
+    frame = 5
+    r = rad.copy()
+    for _ in range(iters):
+        r, dr, mx, ch = orc.pressure(n, frame, r, ids, fs, fc.f32(0.1), fc.f32(3.0),
+                                     fc.f32(0.02), max_nei, fc.f32(rmin), fc.f32(rmax))
+    ctx = gpu.Context(n, 1, 27)
+    tr = dev(rad)
+    tdr = torch.empty(n, dtype=torch.float32, device="cuda")
+    st = torch.zeros(2, dtype=torch.float32, device="cuda")
+    print(f"[pressure_run] n={n} iters={iters}", flush=True)
+    gpu.call("fos_pressure_run", ctx.handle, n, frame, iters, gpu.ptr(tr), gpu.ptr(tdr),
+             gpu.ptr(dev(ids)), gpu.ptr(dev(fs)), 0.1, 3.0, 0.02, max_nei, rmin, rmax,
+             gpu.ptr(st), gpu.stream())
+    assert np.array_equal(host(tr), r)
+    assert np.array_equal(host(tdr), dr)
+    s = host(st)
+    assert s.view(np.int32)[1] == ch and s[0] == pytest.approx(mx, rel=1e-6)
+    del ctx
+
+
 def test_noise_kernels(gpu, orc):
     n = 10000
     pos, rad = fi.uniform(n, 0.012, 0.024, seed=0)
