@@ -300,8 +300,7 @@ __global__ void __launch_bounds__(1024)
   if (t == 0) s_m = 0;
   __syncthreads();
   int before = 0;
-  for (int i = t; i < n; i += 1024) {
-    const int c = cell_of[i];
+  auto take = [&](int i, int c) {
     if (c < lo) {
       ++before;
     } else if (c < hi) {
@@ -312,7 +311,28 @@ __global__ void __launch_bounds__(1024)
         s_lc[k] = c - lo;
       }
     }
+  };
+  // cells in 16-byte loads, four per thread in flight (cell_of is 16-byte
+  // aligned: a context allocation)
+  const int n4 = n >> 2;
+  const int4 *c4 = (const int4 *)cell_of;
+  for (int b = 0; b < n4; b += 4 * 1024) {
+    int4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = b + u * 1024 + t;
+      v[u] = k < n4 ? c4[k] : make_int4(C, C, C, C);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = 4 * (b + u * 1024 + t);
+      take(i, v[u].x);
+      take(i + 1, v[u].y);
+      take(i + 2, v[u].z);
+      take(i + 3, v[u].w);
+    }
   }
+  for (int i = 4 * n4 + t; i < n; i += 1024) take(i, cell_of[i]);
   for (int o = 32; o; o >>= 1) before += __shfl_xor(before, o, 64);
   if (lane == 0) s_w[wid] = before;
   __syncthreads();

@@ -234,11 +234,32 @@ class FoamSim:
         fl.call("fos_validate", self.ctx.handle, self.n, fl.ptr(self.neighbor_list),
                 fl.ptr(self.neighbor_count), vo, fl.stream())
         info = self.info.cpu().numpy()
-        return {"num_passes": int(info[0]), "early_exit": bool(info[1]),
-                "p_dir": int(oi[0]), "overflow_count": int(oi[1]), "min_fsc": int(oi[2]),
+        passes = int(info[0])
+        V = float(self.res) ** 3
+        return {"num_passes": passes, "early_exit": bool(info[1]),
+                "delta_history": [float(c) / V for c in info[2:2 + passes]],
+                "p_dir": int(oi[0]), "p_und_accepted": int(oi[0]) // 2,
+                "overflow_count": int(oi[1]),
+                "overflow_pct": 100.0 * int(oi[1]) / max(1, self.n), "min_fsc": int(oi[2]),
                 "max_fsc": int(oi[3]), "mean_fsc": of[0], "mean_fsc_ema": of[1],
                 "mean_score": of[2], "std_score": of[3], "asymmetric_pairs": int(vo[0]),
-                "self_loops": int(vo[2])}
+                "validation_overflow": int(vo[1]), "self_loops": int(vo[2])}
+
+    def log_lines(self):
+        """Reference-format telemetry of the frame just stepped: the JFA
+        summary (run.py:1095-1149) when the JFA ran with telemetry on, and
+        the frame summary (run.py:1303-1318) every 100th frame."""
+        import telemetry as tm
+        c, last, out = self.cfg, self.last, []
+        if self.telemetry and last.get("jfa") and "num_passes" in last:
+            val = {"passed": last["validation_overflow"] == 0 and last["self_loops"] == 0,
+                   "asymmetric_pairs": last["asymmetric_pairs"]}
+            out += tm.jfa_step4_lines(last, val, 0.0, self.res, self.n)
+        if self.frame % 100 == 0:
+            out += tm.frame_lines(self.frame, self.n, last.get("pbd_passes", 0),
+                                  last.get("max_depth", 0.0), self.fsc.cpu().numpy(),
+                                  self.rad.cpu().numpy(), c.f_low, c.f_high)
+        return out
 
     # -- state ------------------------------------------------------------------
     STATE = ("pos", "rad", "rad_target", "lmd", "v_jit", "face_ids", "face_counts", "fsc",

@@ -93,6 +93,27 @@ def test_snapshot_roundtrip(gpu, tmp_path):
     assert head == "ID,X,Y,Z,Radius,FSC"
 
 
+def test_telemetry_lines(gpu):
+    """Reference-format log lines of a frame: the JFA block parses back to
+    the frame's FSC statistics and the pass count of the JFA that ran."""
+    import telemetry as tm
+    from foam import FoamSim
+    cfg = small_cfg()
+    pos, rad = fi.uniform(cfg.n, 0.03, 0.06, seed=6)
+    sim = FoamSim(cfg, pos, rad, telemetry=True)
+    last = sim.step()
+    lines = sim.log_lines()
+    assert lines[0].startswith("[JFA Step 4] FSC")
+    got = tm.parse_step4(lines[:3])
+    fsc = sim.fsc.cpu().numpy()
+    assert got["min_fsc"] == fsc.min() and got["max_fsc"] == fsc.max()
+    assert abs(got["mean_fsc"] - fsc.mean()) <= 0.05
+    assert got["num_passes"] == last["num_passes"] and got["p_dir"] == last["p_dir"]
+    assert len(last["delta_history"]) == last["num_passes"]
+    sim.frame = 100                      # the every-100-frames block
+    assert sim.log_lines()[4].startswith("[Frame  100] N=")
+
+
 def test_dropin_run_jfa_matches_oracle(gpu, orc):
     import jfa
     fc.JFA_RES_MIN, fc.JFA_RES_MAX = 16, 320
