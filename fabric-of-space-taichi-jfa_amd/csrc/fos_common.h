@@ -205,6 +205,40 @@ struct fos_ctx {
   double *host_f;
 };
 
+// One foam of a batch (device and host descriptor).  Particles, labels and
+// hash cells of the batch's foams are concatenated; particle and label ids
+// are batch-global, so kernels that only follow ids (face rows, finalize,
+// pressure partners) need no instance data.
+struct fos_inst {
+  int n, p0;           // particles [p0, p0 + n)
+  int res, passes;     // JFA grid edge and scheduled passes
+  long long v0;        // labels [v0, v0 + res^3)
+  float voxel;         // L / res
+  int G, c0;           // hash grid edge, cells [c0, c0 + G^3)
+  float r_min, r_max;  // radius bounds
+  float f_low, f_high; // FSC band
+  unsigned seed;       // noise seed
+};
+
+#define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
+
+// A batch of foams: the descriptors and the per-foam work tables.
+struct fos_batch {
+  int B, n_total, c_total, max_res, max_passes, max_G;
+  long long v_total;
+  fos_inst *inst;          // device [B]
+  fos_inst *inst_h;        // host [B]
+  int *tstart;             // device [B+1] first pass/majority tile (1 x 4 x 64 voxels)
+  int *estart;             // device [B+1] first face-emission tile (2 x 4 x 64 voxels)
+  int *gstart;             // device [B+1] first grid-rebuild workgroup
+  int *tstart_h, *estart_h, *gstart_h;  // host copies
+  int *pinst;              // device [n_total] foam of each particle
+  float *pvox;             // device [n_total] voxel edge of each particle's foam
+  int *stripes;            // device [B][FOS_BATCH_PASSES][FOS_STRIPE_PASS_INTS]
+  float *fred;             // device [B][8] per-foam float reductions
+  int *ired;               // device [B][8] per-foam int reductions
+};
+
 namespace fos {
 int exclusive_scan(fos_ctx *ctx, const int *in, int *out, int n,
                    hipStream_t st);

@@ -402,17 +402,15 @@ __device__ __forceinline__ unsigned long long incr_wave_mask(
   return w;
 }
 
+// One tile (4 z-rows of 64 voxels at one x, this thread's wave one row) of a
+// list pass; returns this wave's change ballot (0 for an idle row).  The
+// caller has checked that the pass is active.
 template <bool UNIT_L, int MODE>
-__global__ void __launch_bounds__(256)
-    k_jfa_pass_list(const float4 *__restrict__ seeds, int res, float L, float half_l,
-                    float voxel, int step, const int32_t *__restrict__ in,
-                    int32_t *__restrict__ out, int *__restrict__ stripes, int pass, int early,
-                    double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd,
-                    const unsigned long long *__restrict__ chg_prev,
-                    unsigned long long *__restrict__ chg_out) {
-  __shared__ int s_list[JFA_LIST][256];
-  if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
-  const int tile = xcd_tile(blockIdx.x, per_xcd);
+__device__ __forceinline__ unsigned long long jfa_list_tile(
+    const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel, int step,
+    const int32_t *__restrict__ in, int32_t *__restrict__ out, int tile, int nyb, int nzb,
+    int ntiles, const unsigned long long *__restrict__ chg_prev,
+    unsigned long long *__restrict__ chg_out, int (*s_list)[256]) {
   const int t = (int)threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   int changed = 0;
@@ -506,7 +504,24 @@ __global__ void __launch_bounds__(256)
   }
   const unsigned long long bal = __ballot(changed);
   if (chg_out && row_live && lane == 0) chg_out[((size_t)x * res + y) * nzb + zb] = bal;
-  if (lane == 0 && bal) {
+  return bal;
+}
+
+template <bool UNIT_L, int MODE>
+__global__ void __launch_bounds__(256)
+    k_jfa_pass_list(const float4 *__restrict__ seeds, int res, float L, float half_l,
+                    float voxel, int step, const int32_t *__restrict__ in,
+                    int32_t *__restrict__ out, int *__restrict__ stripes, int pass, int early,
+                    double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd,
+                    const unsigned long long *__restrict__ chg_prev,
+                    unsigned long long *__restrict__ chg_out) {
+  __shared__ int s_list[JFA_LIST][256];
+  if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
+  const int tile = xcd_tile(blockIdx.x, per_xcd);
+  const unsigned long long bal = jfa_list_tile<UNIT_L, MODE>(
+      seeds, res, L, half_l, voxel, step, in, out, tile, nyb, nzb, ntiles, chg_prev, chg_out,
+      s_list);
+  if ((threadIdx.x & 63) == 0 && bal) {
     const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     atomicAdd(stripes + (pass * FOS_STRIPES + wave % FOS_STRIPES) * FOS_STRIPE_INTS,
               __popcll(bal));
@@ -601,11 +616,8 @@ __device__ __forceinline__ void face_neighbours(const int32_t *__restrict__ g, i
   nb[5] = g[x * rr + y * res + wrap1(z + 1, res)];
 }
 
-__global__ void __launch_bounds__(256)
-    k_majority(int res, const int32_t *__restrict__ g, int32_t *__restrict__ f, int nyb,
-               int nzb, int ntiles, int per) {
-  int x, y, z;
-  if (!vox_of(res, nyb, nzb, ntiles, per, x, y, z)) return;
+__device__ __forceinline__ void majority_vox(int res, const int32_t *__restrict__ g,
+                                             int32_t *__restrict__ f, int x, int y, int z) {
   const int v = (x * res + y) * res + z;
   const int my = g[v];
   if (my < 0) {
@@ -635,6 +647,14 @@ __global__ void __launch_bounds__(256)
     }
   }
   f[v] = best;
+}
+
+__global__ void __launch_bounds__(256)
+    k_majority(int res, const int32_t *__restrict__ g, int32_t *__restrict__ f, int nyb,
+               int nzb, int ntiles, int per) {
+  int x, y, z;
+  if (!vox_of(res, nyb, nzb, ntiles, per, x, y, z)) return;
+  majority_vox(res, g, f, x, y, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -778,13 +798,14 @@ __device__ __forceinline__ unsigned pair_hash(int a, int b) {
   return ((unsigned)a * 0x9E3779B1u) ^ ((unsigned)b * 0x85EBCA77u);
 }
 
+// One emission tile (all threads of the workgroup call it).
 template <int TX>
-__global__ void __launch_bounds__(256)
-    k_face_emit_lds(int res, int n, const int32_t *__restrict__ f, int *tb, unsigned *tk,
-                    int *tc, int cap, int *overflow, int nyb, int nzb, int ntiles, int per) {
-  __shared__ unsigned long long s_pair[EMIT_HS];
-  __shared__ unsigned s_key[EMIT_HS];
-  __shared__ int s_cnt[EMIT_HS];
+__device__ __forceinline__ void face_emit_tile(int tile, int res, int n,
+                                               const int32_t *__restrict__ f, int *tb,
+                                               unsigned *tk, int *tc, int cap, int *overflow,
+                                               int nyb, int nzb, int ntiles,
+                                               unsigned long long *s_pair, unsigned *s_key,
+                                               int *s_cnt) {
   const int t = (int)threadIdx.x;
   for (int h = t; h < EMIT_HS; h += 256) {
     s_pair[h] = EMIT_EMPTY;
@@ -792,7 +813,6 @@ __global__ void __launch_bounds__(256)
     s_cnt[h] = 0;
   }
   __syncthreads();
-  const int tile = xcd_tile(blockIdx.x, per);
   if (tile < ntiles) {
     const int zb = tile % nzb, t2 = tile / nzb;
     const int yb = t2 % nyb, x0 = (t2 / nyb) * TX;
@@ -857,6 +877,17 @@ __global__ void __launch_bounds__(256)
       tab_insert(tb, tk, tc, cap, a, b, s_key[h], s_cnt[h], overflow);
     }
   }
+}
+
+template <int TX>
+__global__ void __launch_bounds__(256)
+    k_face_emit_lds(int res, int n, const int32_t *__restrict__ f, int *tb, unsigned *tk,
+                    int *tc, int cap, int *overflow, int nyb, int nzb, int ntiles, int per) {
+  __shared__ unsigned long long s_pair[EMIT_HS];
+  __shared__ unsigned s_key[EMIT_HS];
+  __shared__ int s_cnt[EMIT_HS];
+  face_emit_tile<TX>(xcd_tile(blockIdx.x, per), res, n, f, tb, tk, tc, cap, overflow, nyb, nzb,
+                     ntiles, s_pair, s_key, s_cnt);
 }
 
 // One wave per row: rank entries by key (keys are unique within a row), write
@@ -931,7 +962,9 @@ __device__ __forceinline__ float expected_area(float ri, float rj, float voxel) 
   return 1.5f * (t * t);
 }
 
+// pvox (may be NULL): per-particle voxel edge (a batch of foams), else voxel.
 __global__ void k_fin_classify(int n, const float *__restrict__ rad, float voxel,
+                               const float *__restrict__ pvox,
                                const int32_t *__restrict__ face_ids,
                                const int32_t *__restrict__ face_counts,
                                const float *__restrict__ prev,
@@ -947,7 +980,7 @@ __global__ void k_fin_classify(int n, const float *__restrict__ rad, float voxel
   if (j >= 0 && j < n) {
     const int v = face_counts[t];
     if (v > 0) {
-      const float e = expected_area(rad[i], rad[j], voxel);
+      const float e = expected_area(rad[i], rad[j], pvox ? pvox[i] : voxel);
       if (e > 0.0f) {
         sc = (float)v / e;
         if (sc >= 0.15f)
@@ -1612,6 +1645,15 @@ int fos_face_accumulate(fos_ctx *ctx, int res, int n, const int32_t *filtered,
   return 0;
 }
 
+}  // extern "C"
+
+static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
+                         const int32_t *face_ids, const int32_t *face_counts,
+                         float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
+                         int32_t *used_count, int32_t *fsc, void *stream);
+
+extern "C" {
+
 int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
                        const int32_t *face_ids, const int32_t *face_counts,
                        float *prev_scores, int32_t *neighbor_list,
@@ -1621,13 +1663,23 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
                   (n == 0 || (rad && face_ids && face_counts && prev_scores &&
                               neighbor_list && neighbor_count && used_count && fsc)),
               "fos_finalize_faces: bad arguments (n=%d)", n);
+  return finalize_impl(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores,
+                       neighbor_list, neighbor_count, used_count, fsc, stream);
+}
+
+}  // extern "C"
+
+static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
+                         const int32_t *face_ids, const int32_t *face_counts,
+                         float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
+                         int32_t *used_count, int32_t *fsc, void *stream) {
   if (n == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   const int64_t m = (int64_t)n * FOS_MAX_NEIGHBORS;
   int rc = fos_fill_i32(ctx->in_count, n + 1, 0, stream);
   if (rc) return rc;
   hipLaunchKernelGGL(k_fin_classify, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
-                     rad, voxel, face_ids, face_counts, prev_scores, ctx->score,
+                     rad, voxel, pvox, face_ids, face_counts, prev_scores, ctx->score,
                      ctx->cls, ctx->in_count);
   FOS_CHECK_LAUNCH("k_fin_classify");
   rc = exclusive_scan(ctx, ctx->in_count, ctx->in_start, n + 1, st);
@@ -1666,6 +1718,8 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
   FOS_CHECK_LAUNCH("k_fin_materialize");
   return 0;
 }
+
+extern "C" {
 
 int fos_fsc_ema(int n, float alpha, const int32_t *fsc, float *fsc_ema,
                 void *stream) {
@@ -1790,6 +1844,302 @@ int fos_tile_cache_update(const float *pos, const float *rad, int n,
                      (hipStream_t)stream, pos, rad, n, pos_prev, rad_prev);
   FOS_CHECK_LAUNCH("k_tile_cache");
   return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of foams (no reference counterpart): every pass, the majority
+// filter and the face emission run as one launch over the tiles of all foams.
+// A workgroup finds its foam by binary search over the foams' first tiles
+// (wave-uniform scalar loads) and then runs the single-foam tile body on that
+// foam's label block.  Particle ids are batch-global, so the seed table, the
+// face row tables, finalize and the FSC EMA run over the batch as one foam.
+// ---------------------------------------------------------------------------
+
+// Largest b with start[b] <= t (start ascending, start[0] = 0).
+__device__ __forceinline__ int batch_find(const int *__restrict__ start, int B, int t) {
+  int lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Step of pass p for grid edge res: res/2, halved each pass, at least 1.
+__device__ __forceinline__ int batch_step(int res, int p) {
+  const int s = p + 1 < 31 ? (res >> (p + 1)) : 0;
+  return s < 1 ? 1 : s;
+}
+
+template <bool UNIT_L>
+__global__ void k_seed_b(const float *__restrict__ pos, const float4 *__restrict__ seeds,
+                         const int32_t *__restrict__ pinst, const fos_inst *__restrict__ inst,
+                         int n, float L, float half_l, int32_t *__restrict__ grid) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const fos_inst I = inst[pinst[i]];
+  const int res = I.res;
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float nrm = (pos[3 * i + d] + half_l) / L;  // jfa.py:268-270
+    c[d] = wrap_fast((int)floorf(nrm * (float)res), res);
+  }
+  int32_t *g = grid + I.v0;
+  const int64_t v = ((int64_t)c[0] * res + c[1]) * res + c[2];
+  const float cx = vcenter(c[0], I.voxel, half_l), cy = vcenter(c[1], I.voxel, half_l),
+              cz = vcenter(c[2], I.voxel, half_l);
+  const float pi = power_dist<UNIT_L>(cx, cy, cz, seeds[i], L);
+  int cur = __hip_atomic_load(&g[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    if (cur != -1) {
+      float pc = power_dist<UNIT_L>(cx, cy, cz, seeds[cur], L);
+      bool better = (pi < pc) || (pi == pc && i < cur);
+      if (!better) return;
+    }
+    int old = atomicCAS(&g[v], cur, i);
+    if (old == cur) return;
+    cur = old;
+  }
+}
+
+// Pass `pass` of every foam that still runs it (scheduled and not exited
+// early); each foam's counters are its own stripe block.
+template <bool UNIT_L>
+__global__ void __launch_bounds__(256)
+    k_jfa_pass_list_b(const float4 *__restrict__ seeds, const fos_inst *__restrict__ inst,
+                      const int *__restrict__ tstart, int B, int total, int per_xcd, float L,
+                      float half_l, int pass, const int32_t *__restrict__ in,
+                      int32_t *__restrict__ out, int *__restrict__ stripes, int early,
+                      double th, int min_passes) {
+  __shared__ int s_list[JFA_LIST][256];
+  const int T = xcd_tile(blockIdx.x, per_xcd);
+  if (T >= total) return;
+  const int b = batch_find(tstart, B, T);
+  const int res = inst[b].res;
+  if (pass >= inst[b].passes) return;
+  int *st = stripes + (size_t)b * FOS_BATCH_PASSES * FOS_STRIPE_PASS_INTS;
+  if (!block_pass_active(st, pass, early, th, min_passes, (double)res * res * res)) return;
+  const long long v0 = inst[b].v0;
+  const float voxel = inst[b].voxel;
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64, ntiles = res * nyb * nzb;
+  const int tile = T - tstart[b];
+  const int step = batch_step(res, pass);
+  const int prev = pass > 0 ? batch_step(res, pass - 1) : 0;
+  unsigned long long bal;
+  if (2 * step == res)
+    bal = jfa_list_tile<UNIT_L, JFA_HALF>(seeds, res, L, half_l, voxel, step, in + v0, out + v0,
+                                          tile, nyb, nzb, ntiles, nullptr, nullptr, s_list);
+  else if (step == prev)
+    bal = jfa_list_tile<UNIT_L, JFA_SETTLED>(seeds, res, L, half_l, voxel, step, in + v0,
+                                             out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
+                                             s_list);
+  else
+    bal = jfa_list_tile<UNIT_L, JFA_PLAIN>(seeds, res, L, half_l, voxel, step, in + v0,
+                                           out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
+                                           s_list);
+  if ((threadIdx.x & 63) == 0 && bal) {
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    atomicAdd(st + (pass * FOS_STRIPES + wave % FOS_STRIPES) * FOS_STRIPE_INTS, __popcll(bal));
+  }
+}
+
+// Per foam (one wave each): passes executed, exit flag and per-pass change
+// counts into info[b][2 + FOS_BATCH_PASSES] (the single-foam info layout).
+__global__ void k_jfa_finish_info_b(const fos_inst *__restrict__ inst, int early, double th,
+                                    int min_passes, const int *__restrict__ stripes,
+                                    int32_t *__restrict__ info) {
+  const int b = blockIdx.x;
+  const fos_inst I = inst[b];
+  const double V = (double)I.res * I.res * I.res;
+  const int *st = stripes + (size_t)b * FOS_BATCH_PASSES * FOS_STRIPE_PASS_INTS;
+  int executed = 0;
+  for (int p = 0; p < I.passes; ++p) {
+    if (!pass_active(st, p, early, th, min_passes, V)) break;
+    executed = p + 1;
+  }
+  const int fired = (early && executed > 0 && !pass_active(st, executed, early, th, min_passes, V))
+                        ? 1 : 0;
+  int32_t *o = info + (size_t)b * (2 + FOS_BATCH_PASSES);
+  for (int p = 0; p < FOS_BATCH_PASSES; ++p) {
+    const int c = p < executed ? stripes_total(st, p) : 0;
+    if (threadIdx.x == 0) o[2 + p] = c;
+  }
+  if (threadIdx.x == 0) {
+    o[0] = executed;
+    o[1] = fired;
+  }
+}
+
+// Foams that executed an odd number of passes hold their labels in temp:
+// copy them into grid (blockIdx.y = foam).
+__global__ void k_jfa_copy_odd_b(const fos_inst *__restrict__ inst,
+                                 const int32_t *__restrict__ info,
+                                 const int32_t *__restrict__ temp, int32_t *__restrict__ grid) {
+  const int b = blockIdx.y;
+  if (!(info[(size_t)b * (2 + FOS_BATCH_PASSES)] & 1)) return;
+  const fos_inst I = inst[b];
+  const int64_t V = (int64_t)I.res * I.res * I.res;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V;
+       i += (int64_t)gridDim.x * blockDim.x)
+    grid[I.v0 + i] = temp[I.v0 + i];
+}
+
+__global__ void __launch_bounds__(256)
+    k_majority_b(const fos_inst *__restrict__ inst, const int *__restrict__ tstart, int B,
+                 int total, int per_xcd, const int32_t *__restrict__ g, int32_t *__restrict__ f) {
+  const int T = xcd_tile(blockIdx.x, per_xcd);
+  if (T >= total) return;
+  const int b = batch_find(tstart, B, T);
+  const int res = inst[b].res;
+  const long long v0 = inst[b].v0;
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
+  const int tile = T - tstart[b];
+  const int zb = tile % nzb, t2 = tile / nzb;
+  const int yb = t2 % nyb, x = t2 / nyb;
+  const int z = zb * 64 + (int)(threadIdx.x & 63), y = yb * 4 + (int)(threadIdx.x >> 6);
+  if (z >= res || y >= res) return;
+  majority_vox(res, g + v0, f + v0, x, y, z);
+}
+
+__global__ void __launch_bounds__(256)
+    k_face_emit_b(const fos_inst *__restrict__ inst, const int *__restrict__ estart, int B,
+                  int total, int per_xcd, int n, const int32_t *__restrict__ f, int *tb,
+                  unsigned *tk, int *tc, int cap, int *overflow) {
+  __shared__ unsigned long long s_pair[EMIT_HS];
+  __shared__ unsigned s_key[EMIT_HS];
+  __shared__ int s_cnt[EMIT_HS];
+  const int T = xcd_tile(blockIdx.x, per_xcd);
+  if (T >= total) return;
+  const int b = batch_find(estart, B, T);
+  const int res = inst[b].res;
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
+  const int ntiles = ((res + 1) / 2) * nyb * nzb;
+  face_emit_tile<2>(T - estart[b], res, n, f + inst[b].v0, tb, tk, tc, cap, overflow, nyb, nzb,
+                    ntiles, s_pair, s_key, s_cnt);
+}
+
+extern "C" {
+
+int fos_batch_jfa_init(fos_batch *bt, fos_ctx *ctx, const float *pos, const float *rad,
+                       float L, float beta, int32_t *grid, int32_t *temp, void *stream) {
+  FOS_REQUIRE(bt && ctx && grid && pos && rad && L > 0.0f && bt->n_total <= ctx->max_n,
+              "fos_batch_jfa_init: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = fos_fill_i32(grid, bt->v_total, -1, stream);
+  if (rc) return rc;
+  if (temp) {
+    rc = fos_fill_i32(temp, bt->v_total, -1, stream);
+    if (rc) return rc;
+  }
+  const int n = bt->n_total;
+  if (n == 0) return 0;
+  rc = fos_jfa_seed_table(ctx, pos, rad, n, beta, stream);
+  if (rc) return rc;
+  const float half_l = (float)(0.5 * (double)L);
+  if (unit_l(L))
+    hipLaunchKernelGGL(k_seed_b<true>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos,
+                       ctx->seeds, bt->pinst, bt->inst, n, L, half_l, grid);
+  else
+    hipLaunchKernelGGL(k_seed_b<false>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos,
+                       ctx->seeds, bt->pinst, bt->inst, n, L, half_l, grid);
+  FOS_CHECK_LAUNCH("k_seed_b");
+  return 0;
+}
+
+int fos_batch_jfa_passes(fos_batch *bt, fos_ctx *ctx, float L, int early_exit,
+                         double th_pass_delta, int min_passes, int32_t *grid, int32_t *temp,
+                         int32_t *info, void *stream) {
+  FOS_REQUIRE(bt && ctx && grid && temp && info && grid != temp && L > 0.0f &&
+                  bt->max_passes <= FOS_BATCH_PASSES - 1,
+              "fos_batch_jfa_passes: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int B = bt->B;
+  int rc = fos_fill_i32(bt->stripes, (int64_t)B * FOS_BATCH_PASSES * FOS_STRIPE_PASS_INTS, 0,
+                        stream);
+  if (rc) return rc;
+  const int total = bt->tstart_h[B];
+  const int per = (total + FOS_XCDS - 1) / FOS_XCDS;
+  const float half_l = (float)(0.5 * (double)L);
+  for (int p = 0; p < bt->max_passes; ++p) {
+    const int32_t *in = (p & 1) ? temp : grid;
+    int32_t *out = (p & 1) ? grid : temp;
+    if (unit_l(L))
+      hipLaunchKernelGGL(k_jfa_pass_list_b<true>, dim3(per * FOS_XCDS), dim3(256), 0, st,
+                         ctx->seeds, bt->inst, bt->tstart, B, total, per, L, half_l, p, in, out,
+                         bt->stripes, early_exit, th_pass_delta, min_passes);
+    else
+      hipLaunchKernelGGL(k_jfa_pass_list_b<false>, dim3(per * FOS_XCDS), dim3(256), 0, st,
+                         ctx->seeds, bt->inst, bt->tstart, B, total, per, L, half_l, p, in, out,
+                         bt->stripes, early_exit, th_pass_delta, min_passes);
+    FOS_CHECK_LAUNCH("k_jfa_pass_list_b");
+  }
+  hipLaunchKernelGGL(k_jfa_finish_info_b, dim3(B), dim3(64), 0, st, bt->inst, early_exit,
+                     th_pass_delta, min_passes, (const int *)bt->stripes, info);
+  FOS_CHECK_LAUNCH("k_jfa_finish_info_b");
+  hipLaunchKernelGGL(k_jfa_copy_odd_b, dim3(256, B), dim3(FOS_BLOCK), 0, st, bt->inst,
+                     (const int32_t *)info, temp, grid);
+  FOS_CHECK_LAUNCH("k_jfa_copy_odd_b");
+  return 0;
+}
+
+int fos_batch_majority(fos_batch *bt, const int32_t *grid, int32_t *filtered, void *stream) {
+  FOS_REQUIRE(bt && grid && filtered && grid != filtered, "fos_batch_majority: bad arguments");
+  const int total = bt->tstart_h[bt->B];
+  const int per = (total + FOS_XCDS - 1) / FOS_XCDS;
+  hipLaunchKernelGGL(k_majority_b, dim3(per * FOS_XCDS), dim3(256), 0, (hipStream_t)stream,
+                     bt->inst, bt->tstart, bt->B, total, per, grid, filtered);
+  FOS_CHECK_LAUNCH("k_majority_b");
+  return 0;
+}
+
+int fos_batch_face_accumulate(fos_batch *bt, fos_ctx *ctx, const int32_t *filtered,
+                              int32_t *face_ids, int32_t *face_counts, void *stream) {
+  FOS_REQUIRE(bt && ctx && filtered && face_ids && face_counts && bt->n_total <= ctx->max_n,
+              "fos_batch_face_accumulate: bad arguments");
+  FOS_REQUIRE((double)bt->max_res * bt->max_res * bt->max_res * 6.0 < 4294967295.0,
+              "fos_batch_face_accumulate: res=%d exceeds the 32-bit face key", bt->max_res);
+  const int n = bt->n_total;
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int total = bt->estart_h[bt->B];
+  const int per = (total + FOS_XCDS - 1) / FOS_XCDS;
+  while (true) {
+    hipLaunchKernelGGL(k_face_emit_b, dim3(per * FOS_XCDS), dim3(256), 0, st, bt->inst,
+                       bt->estart, bt->B, total, per, n, filtered, ctx->tab_b, ctx->tab_k,
+                       ctx->tab_c, ctx->tab_cap, ctx->flags);
+    FOS_CHECK_LAUNCH("k_face_emit_b");
+    FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    FOS_HIP(hipStreamSynchronize(st));
+    if (!*(volatile int *)ctx->host_i) break;
+    const int cap = ctx->tab_cap * 4;
+    FOS_REQUIRE(cap <= 4096, "fos_batch_face_accumulate: row table overflow at %d",
+                ctx->tab_cap);
+    int rc = alloc_tables(ctx, cap);
+    if (rc) return rc;
+    rc = fos_fill_i32(ctx->flags, 1, 0, stream);
+    if (rc) return rc;
+  }
+  const int cap = ctx->tab_cap;
+  const size_t shmem = (size_t)4 * cap * sizeof(unsigned);
+  hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st, n,
+                     ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts);
+  FOS_CHECK_LAUNCH("k_face_compact");
+  return 0;
+}
+
+int fos_batch_finalize_faces(fos_batch *bt, fos_ctx *ctx, const float *rad,
+                             const int32_t *face_ids, const int32_t *face_counts,
+                             float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
+                             int32_t *used_count, int32_t *fsc, void *stream) {
+  FOS_REQUIRE(bt && ctx && bt->n_total <= ctx->max_n &&
+                  (bt->n_total == 0 || (rad && face_ids && face_counts && prev_scores &&
+                                        neighbor_list && neighbor_count && used_count && fsc)),
+              "fos_batch_finalize_faces: bad arguments");
+  return finalize_impl(ctx, bt->n_total, rad, 1.0f, bt->pvox, face_ids, face_counts,
+                       prev_scores, neighbor_list, neighbor_count, used_count, fsc, stream);
 }
 
 }  // extern "C"

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfos.so")
 SOURCES = ["fos_ctx.hip", "fos_jfa.hip", "fos_grid.hip", "fos_dynamics.hip", "fos_topology.hip",
-           "fos_predict.hip"]
+           "fos_predict.hip", "fos_batch.hip"]
 ARCH = os.environ.get("FOS_OFFLOAD_ARCH", "gfx950")
 
 

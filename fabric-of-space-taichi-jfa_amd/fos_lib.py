@@ -70,6 +70,14 @@ SIGNATURES = {
     "fos_topo_knn": [P, P, I, I, F, F, I, P, P, P, P, P],
     "fos_topo_ema": [P, P, I, F, P],
     "fos_topo_radii": [P, P, I, F, F, F, F, F, F, P],
+    "fos_batch_create": [P, I, P, P, P, P, P, P, P, P, P, F],
+    "fos_batch_destroy": [P],
+    "fos_batch_info": [P, P],
+    "fos_batch_jfa_init": [P, P, P, P, F, F, P, P, P],
+    "fos_batch_jfa_passes": [P, P, F, I, D, I, P, P, P, P],
+    "fos_batch_majority": [P, P, P, P],
+    "fos_batch_face_accumulate": [P, P, P, P, P, P],
+    "fos_batch_finalize_faces": [P, P, P, P, P, P, P, P, P, P, P],
     "fos_pred_candidates": [P, P, I, I, F, F, I, P, P, P, P, P, P, P, P, P],
     "fos_pred_pairs": [P, P, I, P, P, P],
     "fos_pred_features": [P, I, F, F, P, P, P, P, P, I, I64, P, P],

@@ -31,6 +31,7 @@ extern "C" {
 #endif
 
 typedef struct fos_ctx fos_ctx;
+typedef struct fos_batch fos_batch;
 
 int fos_abi_version(void);
 const char *fos_last_error(void);
@@ -352,6 +353,46 @@ int fos_mlp_param_count(int *count);
 int fos_mlp_train_step(float *params, float *grad, float *adam_m, float *adam_v, int step,
                        float lr, const float *feat, const float *labels, const int32_t *batch_idx,
                        int B, float pos_weight, float *scratch, float *loss, void *stream);
+
+/* ---- batches of foams ---------------------------------------------------
+ * No reference counterpart (the reference runs one foam per process).  A
+ * batch concatenates B independent foams of uneven size: foam b owns
+ * particles [p0_b, p0_b + n_b) of the batch arrays (p0 = prefix of n), labels
+ * [v0_b, v0_b + res_b^3) (v0 = prefix of res^3) and hash cells
+ * [c0_b, c0_b + G_b^3).  Particle and label ids are batch-global.  Each phase
+ * of the frame runs as one launch over all foams.  A context used with a
+ * batch must hold its total particle count. */
+
+/* Builds the descriptors and work tables of B foams: particle counts n,
+ * JFA grid edges res (voxel L/res), scheduled passes (<= 15), hash grid
+ * edges G, radius bounds, FSC bands and noise seeds. */
+int fos_batch_create(fos_batch **out, int B, const int *n, const int *res, const int *passes,
+                     const int *G, const float *r_min, const float *r_max, const float *f_low,
+                     const float *f_high, const unsigned *seed, float L);
+int fos_batch_destroy(fos_batch *batch);
+/* Host out[7] = {B, total particles, total voxels, total cells, max res,
+ * max passes, max G}. */
+int fos_batch_info(const fos_batch *batch, int64_t *out);
+/* jfa.py:485-520 for every foam: labels -1, seed table, seeding (lowest
+ * global index on ties = lowest index within the foam). */
+int fos_batch_jfa_init(fos_batch *batch, fos_ctx *ctx, const float *pos, const float *rad,
+                       float L, float beta, int32_t *grid, int32_t *temp, void *stream);
+/* jfa.py:1082-1116 for every foam, one launch per pass over all foams; a
+ * foam stops at its own schedule or early exit.  info[B][2 + 16] (device):
+ * passes executed, exit flag, changed voxels per pass; labels end in grid. */
+int fos_batch_jfa_passes(fos_batch *batch, fos_ctx *ctx, float L, int early_exit,
+                         double th_pass_delta, int min_passes, int32_t *grid, int32_t *temp,
+                         int32_t *info, void *stream);
+/* jfa.py:640-699 for every foam. */
+int fos_batch_majority(fos_batch *batch, const int32_t *grid, int32_t *filtered, void *stream);
+/* jfa.py:703-762 for every foam (rows are batch particles). */
+int fos_batch_face_accumulate(fos_batch *batch, fos_ctx *ctx, const int32_t *filtered,
+                              int32_t *face_ids, int32_t *face_counts, void *stream);
+/* jfa.py:789-857 for every foam (each foam's voxel edge in the scores). */
+int fos_batch_finalize_faces(fos_batch *batch, fos_ctx *ctx, const float *rad,
+                             const int32_t *face_ids, const int32_t *face_counts,
+                             float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
+                             int32_t *used_count, int32_t *fsc, void *stream);
 
 #ifdef __cplusplus
 }

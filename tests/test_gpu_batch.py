@@ -1,0 +1,67 @@
+"""Batched ensembles (no reference counterpart): every foam of a batch gives
+what the single-foam path (itself pinned to the oracle) gives on its own
+inputs, bit for bit, with batch-global particle ids."""
+import numpy as np
+import pytest
+import torch
+
+import foam_config as fc
+import foam_inputs as fi
+
+pytestmark = pytest.mark.gpu
+
+
+def foams(spec, early=False):
+    cfgs, inputs = [], []
+    for b, (n, res, rmin, rmax) in enumerate(spec):
+        cfg = fc.preset("c1")
+        cfg.n, cfg.res, cfg.r_min, cfg.r_max = n, res, rmin, rmax
+        cfg.early_exit = early
+        cfg.seed = 100 + b
+        cfgs.append(cfg)
+        inputs.append(fi.uniform(n, rmin, rmax, seed=20 + b))
+    return cfgs, inputs
+
+
+SPEC = [(300, 24, 0.03, 0.06), (700, 40, 0.02, 0.045), (500, 33, 0.025, 0.05),
+        (1200, 48, 0.015, 0.035)]
+
+
+def shift(a, p0):
+    """Local particle ids -> batch-global (-1 stays)."""
+    a = np.asarray(a).copy()
+    a[a >= 0] += p0
+    return a
+
+
+@pytest.mark.parametrize("early", [False, True])
+def test_batch_jfa_matches_single_foams(gpu, early):
+    from ensemble import Ensemble
+    from foam import FoamSim
+    cfgs, inputs = foams(SPEC, early)
+    print(f"[batch jfa] early={early}", flush=True)
+    ens = Ensemble(cfgs, inputs)
+    ens.run_jfa()
+    torch.cuda.synchronize()
+    info = ens.info.cpu().numpy()
+    for b, (cfg, (pos, rad)) in enumerate(zip(cfgs, inputs)):
+        sim = FoamSim(cfg, pos, rad)
+        sim.run_jfa()
+        torch.cuda.synchronize()
+        p0 = int(ens.p0[b])
+        si = sim.info.cpu().numpy()
+        ex = int(si[0])
+        assert info[b, 0] == ex and info[b, 1] == si[1], b
+        assert (info[b, 2:2 + ex] == si[2:2 + ex]).all(), b
+        assert np.array_equal(ens.labels(b).cpu().numpy(), shift(sim.grid.cpu().numpy(), p0)), b
+        assert np.array_equal(ens.labels(b, ens.filtered).cpu().numpy(),
+                              shift(sim.filtered.cpu().numpy(), p0)), b
+        for name in ("face_ids", "neighbor_list"):
+            got = ens.foam(b, getattr(ens, name)).cpu().numpy()
+            assert np.array_equal(got, shift(getattr(sim, name).cpu().numpy(), p0)), (b, name)
+        for name in ("face_counts", "fsc", "fsc_ema", "neighbor_count", "used_count",
+                     "prev_scores"):
+            got = ens.foam(b, getattr(ens, name)).cpu().numpy()
+            assert np.array_equal(got, getattr(sim, name).cpu().numpy()), (b, name)
+    if early:
+        assert len(set(info[:, 0])) > 1 or info[:, 1].any()
