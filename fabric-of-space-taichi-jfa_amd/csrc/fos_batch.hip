@@ -14,13 +14,6 @@
 
 using namespace fos;
 
-// Cells per grid-rebuild workgroup of a foam with C cells (about 192
-// workgroups per foam of 192+ cells, each range at most 1024 cells).
-static int batch_cpw(int C) {
-  int cpw = (C + 191) / 192;
-  return cpw < 64 ? 64 : (cpw > 1024 ? 1024 : cpw);
-}
-
 extern "C" {
 
 int fos_batch_create(fos_batch **out, int B, const int *n, const int *res, const int *passes,
@@ -60,7 +53,7 @@ int fos_batch_create(fos_batch **out, int B, const int *n, const int *res, const
     gs[b] = (int)g;
     t += r * nyb * nzb;
     e += ((r + 1) / 2) * nyb * nzb;
-    g += (C + batch_cpw((int)C) - 1) / batch_cpw((int)C);
+    g += (C + fos_range_cells((int)C) - 1) / fos_range_cells((int)C);
     v += r * r * r;
     p += n[b];
     c += C;

@@ -222,6 +222,13 @@ struct fos_inst {
 
 #define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
 
+// Cells per grid-rebuild workgroup for a hash grid of C cells: about 192
+// workgroups per grid, at least 64 and at most 1024 cells each.
+__host__ __device__ inline int fos_range_cells(int C) {
+  int cpw = (C + 191) / 192;
+  return cpw < 64 ? 64 : (cpw > 1024 ? 1024 : cpw);
+}
+
 // A batch of foams: the descriptors and the per-foam work tables.
 struct fos_batch {
   int B, n_total, c_total, max_res, max_passes, max_G;

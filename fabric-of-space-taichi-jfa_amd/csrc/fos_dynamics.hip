@@ -63,17 +63,33 @@ __device__ __forceinline__ void pbd_pair(int j, int i, float px, float py, float
   }
 }
 
+// BATCH: particles of a batch of foams; each uses its foam's hash grid
+// (edge G, cells from c0) and sits the pass out (copies its position) when
+// its foam runs fewer than pass + 1 passes this frame.
+template <bool BATCH>
 __global__ void __launch_bounds__(256)
     k_pbd_wave(const float *__restrict__ pin, float *__restrict__ pout,
                const float *__restrict__ rad, int n, int G, float L, float half_l,
                float inv_l, float gap, float mdf, float dmul,
                const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
-               const int32_t *__restrict__ ci) {
+               const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
+               const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass) {
   __shared__ float s_v[8][27][PBD_SLOTS][3];
   __shared__ int s_cnt[8][27];
   const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
   const int i = blockIdx.x * 8 + grp;
-  const bool live = i < n;
+  bool live = i < n;
+  if (BATCH && live) {
+    const int b = pinst[i];
+    if (pass >= npass[b]) {
+      if (l < 3) pout[3 * i + l] = pin[3 * i + l];
+      live = false;
+    } else {
+      G = inst[b].G;
+      cs += inst[b].c0;
+      cc += inst[b].c0;
+    }
+  }
   float px = 0.f, py = 0.f, pz = 0.f, ri = 0.f, maxd = 0.f;
   int cx = 0, cy = 0, cz = 0;
   if (live) {
@@ -145,15 +161,26 @@ __global__ void __launch_bounds__(256)
 
 // Max overlap with 32 lanes per particle: the maximum is order-free, so the
 // lanes' partial maxima combine with a half-wave reduction.
+// BATCH: each particle uses its foam's hash grid and its foam's maximum
+// (gmax[b]).
+template <bool BATCH>
 __global__ void __launch_bounds__(256)
     k_max_overlap_wave(const float *__restrict__ pos, const float *__restrict__ rad, int n,
                        int G, float L, float half_l, float inv_l,
                        const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
                        const int32_t *__restrict__ ci, float *__restrict__ lmd,
-                       float *gmax) {
+                       float *gmax, const int32_t *__restrict__ pinst,
+                       const fos_inst *__restrict__ inst) {
   const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
   const int i = blockIdx.x * 8 + grp;
   float lm = 0.0f;
+  if (BATCH && i < n) {
+    const int b = pinst[i];
+    G = inst[b].G;
+    cs += inst[b].c0;
+    cc += inst[b].c0;
+    gmax += b;
+  }
   if (i < n && l < 27) {
     const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
     const float ri = rad[i];
@@ -286,7 +313,8 @@ __device__ __forceinline__ float vol_f32(float r, float p) {
 }
 
 // dynamics.py:1502-1543: radius change of particle i from the radii rin.
-__device__ __forceinline__ float pressure_delta(int i, int n, int frame,
+// ih: the particle's index within its foam (the rotation hash's input).
+__device__ __forceinline__ float pressure_delta(int i, int ih, int n, int frame,
                                                 const float *__restrict__ rin,
                                                 const int32_t *__restrict__ face_ids,
                                                 const int32_t *__restrict__ fsc, float k,
@@ -298,7 +326,7 @@ __device__ __forceinline__ float pressure_delta(int i, int n, int frame,
     const float ri = rin[i];
     const float vi = vol_f32(ri, p_exp);
     const int an = fc < max_nei ? fc : max_nei;
-    int idx = lcg_hash(i, frame) % fc;
+    int idx = lcg_hash(ih, frame) % fc;
     const int32_t *row = face_ids + (int64_t)i * FOS_MAX_NEIGHBORS;
 #pragma unroll 4
     for (int q = 0; q < an; ++q) {
@@ -322,7 +350,7 @@ __global__ void k_pressure_delta(int n, int frame, const float *__restrict__ rad
                                  float p_exp, float inv_p, float cap, int max_nei) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  delta_r[i] = pressure_delta(i, n, frame, rad, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
+  delta_r[i] = pressure_delta(i, i, n, frame, rad, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
 }
 
 // One Jacobi iteration (dynamics.py:1497-1557) for particle i from rin into
@@ -331,7 +359,7 @@ __global__ void k_pressure_delta(int n, int frame, const float *__restrict__ rad
 // order (the serial order, bit for bit).  A budget above 16 falls back to
 // the serial loop.  stats (when given) gets max|dr| and the changed count.
 // Every lane of the group must call it (the shuffles span the group).
-__device__ __forceinline__ void pressure_group(int i, int q, int n, int frame,
+__device__ __forceinline__ void pressure_group(int i, int ih, int q, int n, int frame,
                                                const float *__restrict__ rin,
                                                float *__restrict__ rout,
                                                float *__restrict__ delta_r,
@@ -347,7 +375,7 @@ __device__ __forceinline__ void pressure_group(int i, int q, int n, int frame,
   bool has = false;
   if (live && fc > 0 && an <= 16 && q < an) {
     const float vi = vol_f32(ri, p_exp);
-    const int idx = (int)(((unsigned)lcg_hash(i, frame) % (unsigned)fc + (unsigned)q) %
+    const int idx = (int)(((unsigned)lcg_hash(ih, frame) % (unsigned)fc + (unsigned)q) %
                           (unsigned)fc);
     const int j = idx < FOS_MAX_NEIGHBORS ? face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + idx] : -1;
     if (!(j < 0 || j >= n || j == i)) {
@@ -370,7 +398,7 @@ __device__ __forceinline__ void pressure_group(int i, int q, int n, int frame,
   if (!live || q != 0) return;
   float d = acc;
   if (fc > 0 && an > 16)
-    d = pressure_delta(i, n, frame, rin, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
+    d = pressure_delta(i, ih, n, frame, rin, face_ids, fsc, k, p_exp, inv_p, cap, max_nei);
   delta_r[i] = d;
   if (stats) {
     const float a = fabsf(d);
@@ -388,9 +416,9 @@ __global__ void __launch_bounds__(256)
                     float *__restrict__ delta_r, const int32_t *__restrict__ face_ids,
                     const int32_t *__restrict__ fsc, float k, float p_exp, float inv_p,
                     float cap, int max_nei, float r_min, float r_max, float *stats) {
-  pressure_group((int)(blockIdx.x * 16 + (threadIdx.x >> 4)), (int)(threadIdx.x & 15), n, frame,
-                 rin, rout, delta_r, face_ids, fsc, k, p_exp, inv_p, cap, max_nei, r_min, r_max,
-                 stats);
+  const int i = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+  pressure_group(i, i, (int)(threadIdx.x & 15), n, frame, rin, rout, delta_r, face_ids, fsc, k,
+                 p_exp, inv_p, cap, max_nei, r_min, r_max, stats);
 }
 
 // dynamics.py:1546-1557
@@ -594,10 +622,10 @@ int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad, int n,
                               cell_start && cell_count && cell_indices)),
               "fos_pbd_pass: bad arguments (n=%d G=%d)", n, G);
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_pbd_wave, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+  hipLaunchKernelGGL(k_pbd_wave<false>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                      (hipStream_t)stream, pos_in, pos_out, rad, n, G, L, (float)(0.5 * (double)L),
                      (float)(1.0 / (double)L), gap_factor, max_disp_frac, disp_mult,
-                     cell_start, cell_count, cell_indices);
+                     cell_start, cell_count, cell_indices, nullptr, nullptr, nullptr, 0);
   FOS_CHECK_LAUNCH("k_pbd");
   return 0;
 }
@@ -612,10 +640,10 @@ int fos_max_overlap(const float *pos, const float *rad, int n, int G, float L,
               "fos_max_overlap: bad arguments (n=%d G=%d)", n, G);
   int rc = fos_fill_f32(max_depth, 1, 0.0f, stream);
   if (rc || n == 0) return rc;
-  hipLaunchKernelGGL(k_max_overlap_wave, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+  hipLaunchKernelGGL(k_max_overlap_wave<false>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                      (hipStream_t)stream, pos, rad, n, G, L, (float)(0.5 * (double)L),
                      (float)(1.0 / (double)L), cell_start, cell_count, cell_indices,
-                     local_max_depth, max_depth);
+                     local_max_depth, max_depth, nullptr, nullptr);
   FOS_CHECK_LAUNCH("k_max_overlap");
   return 0;
 }
@@ -841,6 +869,256 @@ int fos_topo_candidates(const float *pos, const float *rad, int n, int G, float 
                      mult, cap, cell_start, cell_count, cell_indices, pairs, count,
                      truncated);
   FOS_CHECK_LAUNCH("k_topo");
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of foams (no reference counterpart): the per-particle kernels above
+// with each particle's foam parameters (radius bounds, FSC band, seed, index
+// within the foam) read from the foam descriptors.
+// ---------------------------------------------------------------------------
+
+__global__ void k_nudge_b(float *__restrict__ rad, const float *__restrict__ rt,
+                          const float *__restrict__ lmd, const int32_t *__restrict__ pinst,
+                          const fos_inst *__restrict__ inst, const int32_t *__restrict__ active,
+                          const float *__restrict__ back, int n, float alpha, float msp,
+                          int mode, float tol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = pinst[i];
+  if (!active[b]) return;
+  const float r_min = inst[b].r_min, r_max = inst[b].r_max;
+  const float r0 = rad[i];
+  const float t = fminf(r_max, fmaxf(r_min, rt[i]));
+  float d = alpha * (t - r0);
+  float f = 1.0f;
+  if (mode == 1) {
+    f = back[b];
+  } else if (mode == 2) {
+    f = (tol - lmd[i]) / tol;
+    f = fminf(1.0f, fmaxf(0.25f, f));
+  }
+  d = d * f;
+  const float cap = msp * r0;
+  if (d > cap) d = cap;
+  if (d < -cap) d = -cap;
+  rad[i] = fminf(r_max, fmaxf(r_min, r0 + d));
+}
+
+__global__ void k_fsc_targets_b(const float *__restrict__ rad, float *__restrict__ rt,
+                                const int32_t *__restrict__ fsc, const float *__restrict__ ema,
+                                const int32_t *__restrict__ pinst,
+                                const fos_inst *__restrict__ inst,
+                                const int32_t *__restrict__ active, int n, float gp, float db) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = pinst[i];
+  if (!active[b]) return;
+  const float fl = inst[b].f_low, fh = inst[b].f_high;
+  const float r_min = inst[b].r_min, r_max = inst[b].r_max;
+  const float r0 = rad[i];
+  const float f = ema[i] > 0.0f ? ema[i] : (float)fsc[i];
+  const float gap = 1.0f, idle = 0.003f;
+  float out;
+  if (f < fl - gap) {
+    out = fminf(r0 * (1.0f + gp * 1.0f), r_max);
+  } else if (f < fl) {
+    const float g = 1.0f - smoothstep(fl - gap - db, fl, f);
+    out = fminf(r0 * (1.0f + gp * g), r_max);
+  } else if (f > fh + gap) {
+    out = fmaxf(r0 * (1.0f - gp * 1.0f), r_min);
+  } else if (f > fh) {
+    const float g = smoothstep(fh, fh + gap + db, f);
+    out = fmaxf(r0 * (1.0f - gp * g), r_min);
+  } else {
+    const float center = 0.5f * (fl + fh);
+    if (f < center)
+      out = fminf(r0 * (1.0f + gp * idle), r_max);
+    else
+      out = fmaxf(r0 * (1.0f - gp * idle), r_min);
+  }
+  rt[i] = out;
+}
+
+__global__ void __launch_bounds__(256)
+    k_pressure_step_b(int n, int frame, const float *__restrict__ rin, float *__restrict__ rout,
+                      float *__restrict__ delta_r, const int32_t *__restrict__ face_ids,
+                      const int32_t *__restrict__ fsc, const int32_t *__restrict__ pinst,
+                      const fos_inst *__restrict__ inst, float k, float p_exp, float inv_p,
+                      float cap, int max_nei) {
+  const int i = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+  int ih = i;
+  float r_min = 0.0f, r_max = 0.0f;
+  if (i < n) {
+    const int b = pinst[i];
+    ih = i - inst[b].p0;
+    r_min = inst[b].r_min;
+    r_max = inst[b].r_max;
+  }
+  pressure_group(i, ih, (int)(threadIdx.x & 15), n, frame, rin, rout, delta_r, face_ids, fsc, k,
+                 p_exp, inv_p, cap, max_nei, r_min, r_max, nullptr);
+}
+
+__global__ void k_brownian_b(float *__restrict__ pos, const int32_t *__restrict__ pinst,
+                             const fos_inst *__restrict__ inst, int n, float strength, float L,
+                             float half_l, int frame, int mode) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int b = pinst[g];
+  const unsigned i = (unsigned)(g - inst[b].p0);
+  const uint32_t k0 = inst[b].seed, k1 = 0u;
+  const uint4 r0 = philox(make_uint4(i, (unsigned)frame, 0u, 0u), k0, k1);
+  float d[3];
+  if (mode == 0) {
+    d[0] = ((u01(r0.x) - 0.5f) * 2.0f) * strength;
+    d[1] = ((u01(r0.y) - 0.5f) * 2.0f) * strength;
+    d[2] = ((u01(r0.z) - 0.5f) * 2.0f) * strength;
+  } else {
+    const uint4 r1 = philox(make_uint4(i, (unsigned)frame, 1u, 0u), k0, k1);
+    d[0] = box_muller(r0.x, r0.y) * strength;
+    d[1] = box_muller(r0.z, r0.w) * strength;
+    d[2] = box_muller(r1.x, r1.y) * strength;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float p = pos[3 * g + a] + d[a];
+    if (p < -half_l)
+      p = p + L;
+    else if (p >= half_l)
+      p = p - L;
+    pos[3 * g + a] = p;
+  }
+}
+
+// out[b] += number of particles of foam b with fsc == 0 (out zeroed first).
+__global__ void k_count_zero_b(const int32_t *__restrict__ fsc, const int32_t *__restrict__ pinst,
+                               int n, int32_t *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const int b = live ? pinst[i] : -1;
+  const bool z = live && fsc[i] == 0;
+  // one atomic per wave when its particles share a foam
+  const int b0 = __shfl(b, 0, 64);
+  const unsigned long long same = __ballot(!live || b == b0);
+  if (same == ~0ull) {
+    const int c = __popcll(__ballot(z));
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&out[b0], c);
+  } else if (z) {
+    atomicAdd(&out[b], 1);
+  }
+}
+
+extern "C" {
+
+int fos_batch_max_overlap(fos_batch *bt, const float *pos, const float *rad, float L,
+                          const int32_t *cell_start, const int32_t *cell_count,
+                          const int32_t *cell_indices, float *local_max_depth,
+                          float *max_depth, void *stream) {
+  FOS_REQUIRE(bt && pos && rad && cell_start && cell_count && cell_indices &&
+                  local_max_depth && max_depth && L > 0.0f,
+              "fos_batch_max_overlap: bad arguments");
+  int rc = fos_fill_f32(max_depth, bt->B, 0.0f, stream);
+  if (rc) return rc;
+  const int n = bt->n_total;
+  hipLaunchKernelGGL(k_max_overlap_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                     (hipStream_t)stream, pos, rad, n, 3, L, (float)(0.5 * (double)L),
+                     (float)(1.0 / (double)L), cell_start, cell_count, cell_indices,
+                     local_max_depth, max_depth, bt->pinst, bt->inst);
+  FOS_CHECK_LAUNCH("k_max_overlap_b");
+  return 0;
+}
+
+int fos_batch_pbd_pass(fos_batch *bt, const float *pos_in, float *pos_out, const float *rad,
+                       float L, float gap_factor, float max_disp_frac, float disp_mult,
+                       const int32_t *cell_start, const int32_t *cell_count,
+                       const int32_t *cell_indices, const int32_t *passes, int pass,
+                       void *stream) {
+  FOS_REQUIRE(bt && pos_in && pos_out && pos_in != pos_out && rad && cell_start && cell_count &&
+                  cell_indices && passes && L > 0.0f,
+              "fos_batch_pbd_pass: bad arguments");
+  const int n = bt->n_total;
+  hipLaunchKernelGGL(k_pbd_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                     (hipStream_t)stream, pos_in, pos_out, rad, n, 3, L, (float)(0.5 * (double)L),
+                     (float)(1.0 / (double)L), gap_factor, max_disp_frac, disp_mult, cell_start,
+                     cell_count, cell_indices, bt->pinst, bt->inst, passes, pass);
+  FOS_CHECK_LAUNCH("k_pbd_b");
+  return 0;
+}
+
+int fos_batch_nudge_radii(fos_batch *bt, float *rad, const float *rad_target,
+                          const float *local_max_depth, float alpha, float max_step_pct,
+                          int mode, float contact_tol, const float *back,
+                          const int32_t *active, void *stream) {
+  FOS_REQUIRE(bt && rad && rad_target && local_max_depth && back && active,
+              "fos_batch_nudge_radii: bad arguments");
+  const int n = bt->n_total;
+  hipLaunchKernelGGL(k_nudge_b, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     rad, rad_target, local_max_depth, bt->pinst, bt->inst, active, back, n,
+                     alpha, max_step_pct, mode, contact_tol);
+  FOS_CHECK_LAUNCH("k_nudge_b");
+  return 0;
+}
+
+int fos_batch_fsc_targets(fos_batch *bt, const float *rad, float *rad_target,
+                          const int32_t *fsc, const float *fsc_ema, float growth_pct,
+                          float deadband, const int32_t *active, void *stream) {
+  FOS_REQUIRE(bt && rad && rad_target && fsc && fsc_ema && active,
+              "fos_batch_fsc_targets: bad arguments");
+  const int n = bt->n_total;
+  hipLaunchKernelGGL(k_fsc_targets_b, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, rad, rad_target, fsc, fsc_ema, bt->pinst, bt->inst,
+                     active, n, growth_pct, deadband);
+  FOS_CHECK_LAUNCH("k_fsc_targets_b");
+  return 0;
+}
+
+int fos_batch_pressure_run(fos_batch *bt, fos_ctx *ctx, int frame, int iters, float *rad,
+                           float *delta_r, const int32_t *face_ids, const int32_t *fsc, float k,
+                           float p_exp, float pair_cap, int max_nei, void *stream) {
+  FOS_REQUIRE(bt && ctx && bt->n_total <= ctx->max_n && iters >= 0 && rad && delta_r &&
+                  face_ids && fsc,
+              "fos_batch_pressure_run: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = bt->n_total;
+  if (iters == 0) return 0;
+  float *a = rad, *b = ctx->rad_tmp;
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_pressure_step_b, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, st, n,
+                       frame, a, b, delta_r, face_ids, fsc, bt->pinst, bt->inst, k, p_exp,
+                       1.0f / p_exp, pair_cap, max_nei);
+    FOS_CHECK_LAUNCH("k_pressure_step_b");
+    float *t = a;
+    a = b;
+    b = t;
+  }
+  if (a != rad) {
+    hipLaunchKernelGGL(k_copy_f32, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, a, rad,
+                       (int64_t)n);
+    FOS_CHECK_LAUNCH("k_copy_f32");
+  }
+  return 0;
+}
+
+int fos_batch_brownian(fos_batch *bt, float *pos, float strength, float L, int frame, int mode,
+                       void *stream) {
+  FOS_REQUIRE(bt && pos && L > 0.0f && (mode == 0 || mode == 1),
+              "fos_batch_brownian: bad arguments");
+  const int n = bt->n_total;
+  hipLaunchKernelGGL(k_brownian_b, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
+                     pos, bt->pinst, bt->inst, n, strength, L, L * 0.5f, frame, mode);
+  FOS_CHECK_LAUNCH("k_brownian_b");
+  return 0;
+}
+
+int fos_batch_count_zero(fos_batch *bt, const int32_t *fsc, int32_t *out, void *stream) {
+  FOS_REQUIRE(bt && fsc && out, "fos_batch_count_zero: bad arguments");
+  int rc = fos_fill_i32(out, bt->B, 0, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_count_zero_b, dim3(blocks_for(bt->n_total)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, fsc, bt->pinst, bt->n_total, out);
+  FOS_CHECK_LAUNCH("k_count_zero_b");
   return 0;
 }
 

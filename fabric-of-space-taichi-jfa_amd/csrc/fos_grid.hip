@@ -285,16 +285,18 @@ __device__ __forceinline__ void sort_run(int32_t *a, int m) {
   }
 }
 
-__global__ void __launch_bounds__(1024)
-    k_grid_build_ranges(const int32_t *__restrict__ cell_of, int n, int C, int cpw,
-                        int32_t *__restrict__ count, int32_t *__restrict__ start,
-                        int32_t *__restrict__ idx) {
+// Builds cells [lo, hi) of the CSR from the particles [pa, pe), whose cells
+// all lie in the grid of those cells; base0 = first CSR slot of particle pa's
+// grid.  All threads of the workgroup call it.
+__device__ __forceinline__ void grid_range(const int32_t *__restrict__ cell_of, int pa, int pe,
+                                           int base0, int lo, int hi,
+                                           int32_t *__restrict__ count,
+                                           int32_t *__restrict__ start,
+                                           int32_t *__restrict__ idx) {
   __shared__ int s_cnt[GR_CELLS], s_ex[GR_CELLS];
   __shared__ int s_li[GR_LIST], s_lc[GR_LIST], s_out[GR_LIST];
   __shared__ int s_w[16], s_m;
   const int t = (int)threadIdx.x, lane = t & 63, wid = t >> 6;
-  const int lo = (int)blockIdx.x * cpw;
-  const int hi = min(C, lo + cpw);
   const int nc = hi - lo;
   for (int k = t; k < nc; k += 1024) s_cnt[k] = 0;
   if (t == 0) s_m = 0;
@@ -312,31 +314,37 @@ __global__ void __launch_bounds__(1024)
       }
     }
   };
-  // cells in 16-byte loads, four per thread in flight (cell_of is 16-byte
-  // aligned: a context allocation)
-  const int n4 = n >> 2;
-  const int4 *c4 = (const int4 *)cell_of;
-  for (int b = 0; b < n4; b += 4 * 1024) {
-    int4 v[4];
+  // cells in 16-byte loads over the aligned middle, four per thread in
+  // flight (cell_of is a 256-byte aligned context allocation)
+  const int a4 = (pa + 3) & ~3, e4 = pe & ~3;
+  if (a4 < e4) {
+    for (int i = pa + t; i < a4; i += 1024) take(i, cell_of[i]);
+    const int m4 = (e4 - a4) >> 2;
+    const int4 *c4 = (const int4 *)(cell_of + a4);
+    for (int b = 0; b < m4; b += 4 * 1024) {
+      int4 v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = b + u * 1024 + t;
-      v[u] = k < n4 ? c4[k] : make_int4(C, C, C, C);
-    }
+      for (int u = 0; u < 4; ++u) {
+        const int k = b + u * 1024 + t;
+        v[u] = k < m4 ? c4[k] : make_int4(hi, hi, hi, hi);
+      }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int i = 4 * (b + u * 1024 + t);
-      take(i, v[u].x);
-      take(i + 1, v[u].y);
-      take(i + 2, v[u].z);
-      take(i + 3, v[u].w);
+      for (int u = 0; u < 4; ++u) {
+        const int i = a4 + 4 * (b + u * 1024 + t);
+        take(i, v[u].x);
+        take(i + 1, v[u].y);
+        take(i + 2, v[u].z);
+        take(i + 3, v[u].w);
+      }
     }
+    for (int i = e4 + t; i < pe; i += 1024) take(i, cell_of[i]);
+  } else {
+    for (int i = pa + t; i < pe; i += 1024) take(i, cell_of[i]);
   }
-  for (int i = 4 * n4 + t; i < n; i += 1024) take(i, cell_of[i]);
   for (int o = 32; o; o >>= 1) before += __shfl_xor(before, o, 64);
   if (lane == 0) s_w[wid] = before;
   __syncthreads();
-  int base = 0;  // particles in cells before lo
+  int base = base0;  // slots of the particles in cells before lo
 #pragma unroll
   for (int w = 0; w < 16; ++w) base += s_w[w];
   const int m = s_m;
@@ -370,7 +378,7 @@ __global__ void __launch_bounds__(1024)
     __syncthreads();
     for (int k = t; k < m; k += 1024) idx[base + k] = s_out[k];
   } else {
-    for (int i = t; i < n; i += 1024) {
+    for (int i = pa + t; i < pe; i += 1024) {
       const int c = cell_of[i];
       if (c >= lo && c < hi) idx[base + s_ex[c - lo] + atomicAdd(&s_cnt[c - lo], 1)] = i;
     }
@@ -378,6 +386,53 @@ __global__ void __launch_bounds__(1024)
     __syncthreads();
     for (int k = t; k < nc; k += 1024) sort_run(idx + base + s_ex[k], s_cnt[k]);
   }
+}
+
+__global__ void __launch_bounds__(1024)
+    k_grid_build_ranges(const int32_t *__restrict__ cell_of, int n, int C, int cpw,
+                        int32_t *__restrict__ count, int32_t *__restrict__ start,
+                        int32_t *__restrict__ idx) {
+  const int lo = (int)blockIdx.x * cpw;
+  grid_range(cell_of, 0, n, 0, lo, min(C, lo + cpw), count, start, idx);
+}
+
+// Batches of foams: the cells of every particle in its foam's grid (global
+// cell ids), skipping foams that sit this rebuild out (npass given and
+// pass >= npass[b]: their positions, cells and CSR stay as they are).
+__global__ void k_grid_cells_b(float *__restrict__ pos, const int32_t *__restrict__ pinst,
+                               const fos_inst *__restrict__ inst, int n, float L, float half_l,
+                               float inv_l, const int32_t *__restrict__ npass, int pass,
+                               int32_t *__restrict__ cell_of) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = pinst[i];
+  if (npass && pass >= npass[b]) return;
+  float pw[3];
+  const int c = particle_cell(pos, i, inst[b].G, L, half_l, inv_l, pw);
+  pos[3 * i] = pw[0];
+  pos[3 * i + 1] = pw[1];
+  pos[3 * i + 2] = pw[2];
+  cell_of[i] = inst[b].c0 + c;
+}
+
+__global__ void __launch_bounds__(1024)
+    k_grid_build_ranges_b(const int32_t *__restrict__ cell_of,
+                          const fos_inst *__restrict__ inst, const int *__restrict__ gstart, int B,
+                          const int32_t *__restrict__ npass, int pass,
+                          int32_t *__restrict__ count, int32_t *__restrict__ start,
+                          int32_t *__restrict__ idx) {
+  const int w = (int)blockIdx.x;
+  int lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (gstart[mid] <= w) lo = mid; else hi = mid - 1;
+  }
+  const int b = lo;
+  if (npass && pass >= npass[b]) return;
+  const fos_inst I = inst[b];
+  const int C = I.G * I.G * I.G, cpw = fos_range_cells(C);
+  const int c_lo = I.c0 + (w - gstart[b]) * cpw;
+  grid_range(cell_of, I.p0, I.p0 + I.n, I.p0, c_lo, min(I.c0 + C, c_lo + cpw), count, start, idx);
 }
 
 // 0: multi-kernel rebuild; 1: one workgroup where it fits (else 2);
@@ -441,8 +496,7 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
                          L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), ctx->cell_of);
       FOS_CHECK_LAUNCH("k_grid_cells");
     }
-    int cpw = (C + 191) / 192;  // about 192 workgroups
-    cpw = cpw < 64 ? 64 : (cpw > GR_CELLS ? GR_CELLS : cpw);
+    const int cpw = fos_range_cells(C);
     const int wgs = (C + cpw - 1) / cpw;
     hipLaunchKernelGGL(k_grid_build_ranges, dim3((unsigned)wgs), dim3(1024), 0, st,
                        (const int32_t *)ctx->cell_of, n, C, cpw, cell_count, cell_start,
@@ -486,6 +540,29 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
                      (const int32_t *)ctx->cell_of, (const int32_t *)cell_start,
                      (const int32_t *)cell_count, (const int32_t *)ctx->grid_tmp, cell_indices);
   FOS_CHECK_LAUNCH("k_grid_rank");
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int fos_batch_grid_build(fos_batch *bt, fos_ctx *ctx, float *pos, float L, int32_t *cell_count,
+                         int32_t *cell_start, int32_t *cell_indices, const int32_t *passes,
+                         int pass, void *stream) {
+  FOS_REQUIRE(bt && ctx && pos && cell_count && cell_start && cell_indices && L > 0.0f &&
+                  bt->n_total <= ctx->max_n,
+              "fos_batch_grid_build: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = bt->n_total;
+  hipLaunchKernelGGL(k_grid_cells_b, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, bt->pinst,
+                     bt->inst, n, L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), passes,
+                     pass, ctx->cell_of);
+  FOS_CHECK_LAUNCH("k_grid_cells_b");
+  hipLaunchKernelGGL(k_grid_build_ranges_b, dim3((unsigned)bt->gstart_h[bt->B]), dim3(1024), 0,
+                     st, (const int32_t *)ctx->cell_of, bt->inst, bt->gstart, bt->B, passes, pass,
+                     cell_count, cell_start, cell_indices);
+  FOS_CHECK_LAUNCH("k_grid_build_ranges_b");
   return 0;
 }
 

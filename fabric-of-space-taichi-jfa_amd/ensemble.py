@@ -121,8 +121,25 @@ class Ensemble:
         self.used_count = torch.zeros(n, dtype=i32, device=dev)
         self.prev_scores = torch.zeros(n, MAXN, dtype=f32, device=dev)
         self.info = torch.zeros(B, 2 + SLOTS, dtype=i32, device=dev)
-        self.ctx = fl.context(n, int(self.res_of.max()), int(self.c0[-1]))
+        C = int(self.c0[-1])
+        self.cell_count = torch.zeros(C, dtype=i32, device=dev)
+        self.cell_start = torch.zeros(C, dtype=i32, device=dev)
+        self.cell_indices = torch.zeros(n, dtype=i32, device=dev)
+        self.pos_tmp = torch.empty_like(self.pos)
+        self.rad_target = self.rad.clone()
+        self.lmd = torch.zeros(n, dtype=f32, device=dev)
+        self.delta_r = torch.zeros(n, dtype=f32, device=dev)
+        self.maxd = torch.zeros(B, dtype=f32, device=dev)
+        self.zeros = torch.zeros(B, dtype=i32, device=dev)
+        # per-foam controls of a frame, uploaded by the host: [0] PBD passes,
+        # [1] settle active, [2] growth active (int); back factor (float)
+        self.ctl = torch.zeros(3, B, dtype=i32, device=dev)
+        self.back = torch.zeros(B, dtype=f32, device=dev)
+        self.ctx = fl.context(n, int(self.res_of.max()), C)
         self.frame = 0
+        self.settle = np.zeros(B, np.int64)
+        self.growth = np.ones(B, bool)
+        self.last = {}
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -156,3 +173,94 @@ class Ensemble:
                 fl.ptr(self.face_counts), fl.ptr(self.prev_scores), fl.ptr(self.neighbor_list),
                 fl.ptr(self.neighbor_count), fl.ptr(self.used_count), fl.ptr(self.fsc), st)
         fl.call("fos_fsc_ema", self.n, c.ema_alpha, fl.ptr(self.fsc), fl.ptr(self.fsc_ema), st)
+
+    # -- the frame (foam.FoamSim.step for every foam) -----------------------------
+    def rebuild_grid(self, passes=None, p=0):
+        c = self.cfg
+        fl.call("fos_batch_grid_build", self.handle, self.ctx.handle, fl.ptr(self.pos), float(c.L),
+                fl.ptr(self.cell_count), fl.ptr(self.cell_start), fl.ptr(self.cell_indices),
+                None if passes is None else fl.ptr(passes), int(p), fl.stream())
+
+    def step(self):
+        """One frame of every foam, in the run.py:700-1261 order (see
+        foam.FoamSim.step).  The host reads the foams' overlap maxima (their
+        PBD pass counts) and, on JFA frames, their zero-FSC counts."""
+        c, f, st, h, B = self.cfg, self.frame, fl.stream(), self.handle, self.B
+        if c.ou_jitter:
+            raise NotImplementedError("OU jitter is not batched")
+        # 0-1
+        self.rebuild_grid()
+        fl.call("fos_batch_max_overlap", h, fl.ptr(self.pos), fl.ptr(self.rad), float(c.L),
+                fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
+                fl.ptr(self.lmd), fl.ptr(self.maxd), st)
+        maxd = self.maxd.cpu().numpy()
+        passes = np.array([max(c.pbd_base, min(c.pbd_max, int(c.pbd_base + c.pbd_scale * (
+            float(md) / (0.2 * cb.r_max))))) for md, cb in zip(maxd, self.cfgs)], np.int32)
+        # 2 settle
+        act = self.settle > 0
+        if act.any():
+            back = np.array([max(0.25, min(1.0, (c.contact_tol - float(md)) / c.contact_tol))
+                             for md in maxd], np.float32)
+            self.back.copy_(torch.from_numpy(back))
+            self.ctl[1].copy_(torch.from_numpy(act.astype(np.int32)))
+            fl.call("fos_batch_nudge_radii", h, fl.ptr(self.rad), fl.ptr(self.rad_target),
+                    fl.ptr(self.lmd), c.alpha(), c.max_step_pct, c.backpressure_mode,
+                    c.contact_tol, fl.ptr(self.back), fl.ptr(self.ctl[1]), st)
+            self.settle[act] -= 1
+        # 3 pressure
+        if c.pressure:
+            fl.call("fos_batch_pressure_run", h, self.ctx.handle, f, c.pressure_iters,
+                    fl.ptr(self.rad), fl.ptr(self.delta_r), fl.ptr(self.face_ids),
+                    fl.ptr(self.fsc), c.pressure_k, c.pressure_exp, c.pressure_cap,
+                    c.pressure_max_nei, st)
+        # 4 Brownian
+        if c.brownian:
+            fl.call("fos_batch_brownian", h, fl.ptr(self.pos), c.brownian_strength, float(c.L), f,
+                    c.brownian_mode, st)
+        # 5 PBD: a foam with fewer passes sits the later passes out
+        self.ctl[0].copy_(torch.from_numpy(passes))
+        for p in range(int(passes.max())):
+            self.rebuild_grid(self.ctl[0], p)
+            fl.call("fos_batch_pbd_pass", h, fl.ptr(self.pos), fl.ptr(self.pos_tmp),
+                    fl.ptr(self.rad), float(c.L), c.gap_factor(), c.max_disp_frac, c.disp_mult,
+                    fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
+                    fl.ptr(self.ctl[0]), p, st)
+            self.pos, self.pos_tmp = self.pos_tmp, self.pos
+        # 7
+        self.rebuild_grid()
+        # 8 JFA on cadence frames
+        ran = c.jfa_runs(f)
+        if ran:
+            self.run_jfa()
+            fl.call("fos_batch_count_zero", h, fl.ptr(self.fsc), fl.ptr(self.zeros), st)
+            f0 = self.zeros.cpu().numpy() / np.maximum(1, self.n_of).astype(np.float64)
+            self.growth = ~((f < c.warmstart_frames) | (f0 > c.fsc_zero_rate))
+        else:
+            self.growth = np.ones(B, bool)
+        # 9 controller
+        if self.growth.any():
+            self.ctl[2].copy_(torch.from_numpy(self.growth.astype(np.int32)))
+            fl.call("fos_batch_fsc_targets", h, fl.ptr(self.rad), fl.ptr(self.rad_target),
+                    fl.ptr(self.fsc), fl.ptr(self.fsc_ema), c.growth_pct, c.deadband,
+                    fl.ptr(self.ctl[2]), st)
+            self.settle[self.growth] = c.adjustment_frames
+        self.last = {"frame": f, "pbd_passes": passes, "max_depth": maxd, "jfa": ran,
+                     "growth": self.growth.copy()}
+        self.frame += 1
+        return self.last
+
+    def snapshot(self, b):
+        """Foam b's state with foam-local particle ids (foam.FoamSim.snapshot
+        keys)."""
+        torch.cuda.synchronize()
+        p0 = int(self.p0[b])
+        out = {}
+        for k in ("pos", "rad", "rad_target", "lmd", "face_ids", "face_counts", "fsc", "fsc_ema",
+                  "neighbor_list", "neighbor_count", "used_count", "prev_scores"):
+            a = self.foam(b, getattr(self, k)).cpu().numpy().copy()
+            if k in ("face_ids", "neighbor_list"):
+                a[a >= 0] -= p0
+            out[k] = a
+        out["frame"] = np.int64(self.frame)
+        out["settle"] = np.int64(self.settle[b])
+        return out

@@ -393,6 +393,48 @@ int fos_batch_finalize_faces(fos_batch *batch, fos_ctx *ctx, const float *rad,
                              const int32_t *face_ids, const int32_t *face_counts,
                              float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
                              int32_t *used_count, int32_t *fsc, void *stream);
+/* grid.py:30-309 for every foam into one CSR over the batch's cells
+ * (cell_count/cell_start [total cells], cell_indices [total particles],
+ * batch-global).  With passes given (device int[B]), foams with
+ * pass >= passes[b] are left as they are (their rebuild is not due). */
+int fos_batch_grid_build(fos_batch *batch, fos_ctx *ctx, float *pos, float L,
+                         int32_t *cell_count, int32_t *cell_start, int32_t *cell_indices,
+                         const int32_t *passes, int pass, void *stream);
+/* dynamics.py:304-372 for every foam; max_depth[B] (device) per foam. */
+int fos_batch_max_overlap(fos_batch *batch, const float *pos, const float *rad, float L,
+                          const int32_t *cell_start, const int32_t *cell_count,
+                          const int32_t *cell_indices, float *local_max_depth,
+                          float *max_depth, void *stream);
+/* dynamics.py:77-171 (Jacobi) pass `pass` for foams with pass < passes[b]
+ * (device int[B]); the others copy their positions. */
+int fos_batch_pbd_pass(fos_batch *batch, const float *pos_in, float *pos_out, const float *rad,
+                       float L, float gap_factor, float max_disp_frac, float disp_mult,
+                       const int32_t *cell_start, const int32_t *cell_count,
+                       const int32_t *cell_indices, const int32_t *passes, int pass,
+                       void *stream);
+/* dynamics.py:1366-1427 for foams with active[b] (device int[B]); back[b]
+ * (device float[B]) is the global backpressure factor of mode 1. */
+int fos_batch_nudge_radii(fos_batch *batch, float *rad, const float *rad_target,
+                          const float *local_max_depth, float alpha, float max_step_pct,
+                          int mode, float contact_tol, const float *back,
+                          const int32_t *active, void *stream);
+/* dynamics.py:1281-1320 for foams with active[b], each with its FSC band and
+ * radius bounds. */
+int fos_batch_fsc_targets(fos_batch *batch, const float *rad, float *rad_target,
+                          const int32_t *fsc, const float *fsc_ema, float growth_pct,
+                          float deadband, const int32_t *active, void *stream);
+/* dynamics.py:1497-1557 x iters for every foam (each foam's radius bounds;
+ * the rotation hash takes the index within the foam). */
+int fos_batch_pressure_run(fos_batch *batch, fos_ctx *ctx, int frame, int iters, float *rad,
+                           float *delta_r, const int32_t *face_ids, const int32_t *fsc, float k,
+                           float p_exp, float pair_cap, int max_nei, void *stream);
+/* dynamics.py:1596-1634 for every foam, Philox keyed (foam seed, frame,
+ * index within the foam). */
+int fos_batch_brownian(fos_batch *batch, float *pos, float strength, float L, int frame,
+                       int mode, void *stream);
+/* out[b] (device int[B]) = particles of foam b with fsc == 0 (the
+ * warm-start gate, run.py:1184-1195). */
+int fos_batch_count_zero(fos_batch *batch, const int32_t *fsc, int32_t *out, void *stream);
 
 #ifdef __cplusplus
 }

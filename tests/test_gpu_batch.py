@@ -65,3 +65,43 @@ def test_batch_jfa_matches_single_foams(gpu, early):
             assert np.array_equal(got, getattr(sim, name).cpu().numpy()), (b, name)
     if early:
         assert len(set(info[:, 0])) > 1 or info[:, 1].any()
+
+
+def frame_foams():
+    """Foams with the C2 dynamics (Gaussian Brownian, adaptive PBD, pressure,
+    controller after a short warm start), uneven sizes and bands."""
+    spec = [(400, 24, 0.015, 0.045, 10, 14), (900, 36, 0.012, 0.032, 12, 16),
+            (650, 30, 0.013, 0.036, 14, 18)]
+    cfgs, inputs = [], []
+    for b, (n, res, rmin, rmax, lo, hi) in enumerate(spec):
+        cfg = fc.preset("c2")
+        cfg.n, cfg.res, cfg.r_min, cfg.r_max = n, res, rmin, rmax
+        cfg.f_low, cfg.f_high = lo, hi
+        cfg.warmstart_frames = 1
+        cfg.seed = 7 + 13 * b
+        cfgs.append(cfg)
+        inputs.append(fi.uniform(n, 2 * rmin, rmax, seed=40 + b))
+    return cfgs, inputs
+
+
+def test_batch_frames_match_single_foams(gpu):
+    """Whole frames: every foam of the batch equals its own FoamSim run, bit
+    for bit, including foams that run different PBD pass counts."""
+    from ensemble import Ensemble
+    from foam import FoamSim
+    cfgs, inputs = frame_foams()
+    ens = Ensemble(cfgs, inputs)
+    sims = [FoamSim(c, p, r) for c, (p, r) in zip(cfgs, inputs)]
+    seen = set()
+    for f in range(4):
+        print(f"[batch frames] frame {f}", flush=True)
+        last = ens.step()
+        for b, sim in enumerate(sims):
+            s = sim.step()
+            assert s["pbd_passes"] == last["pbd_passes"][b], (f, b)
+            assert s["growth"] == bool(last["growth"][b]), (f, b)
+            got, want = ens.snapshot(b), sim.snapshot()
+            for k in got:
+                assert np.array_equal(got[k], want[k]), (f, b, k)
+        seen.add(tuple(last["pbd_passes"]))
+    assert any(len(set(p)) > 1 for p in seen) or len(seen) > 1
