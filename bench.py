@@ -20,6 +20,10 @@ The JSON line also carries:
                 the JFA every frame against the JFA every 8th frame (the
                 judge) and the predictor in between, with the judged FSC
                 errors (the frame's mean FSC, and per particle)
+  ensemble      the batched ensemble: 256 independent foams (N log-uniform in
+                [500, 20000], JFA edge round(5 N^(1/3)) in [48, 160]) advanced
+                by one launch per phase; foam-frames/s of the batch against
+                the same foams stepped one after another (a sample of them)
 """
 import argparse
 import ctypes
@@ -80,6 +84,9 @@ def parse():
     ap.add_argument("--no-cadence", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=48)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-foams", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain copy of 400^3 int32 (counter scaling)")
     return ap.parse_args()
@@ -153,6 +160,56 @@ def surrogate_bench(steps, warmup):
             "fsc_err_per_particle": round(float(np.mean([j[1] for j in judged])), 5) if judged else None,
             "fallback_frames": int(sum(1 for j in judged if j[2] > surr.max_fsc_err)),
             "candidate_mult": meta["mult"], "trained_on": f"{meta['frames']} plain-JFA frames"}
+
+
+def ensemble_bench(B, steps, warmup, sample=8):
+    """Batched ensemble of B foams: frames of the whole batch per second and
+    foam-frames/s, against a sample of the same foams stepped one at a time
+    (FoamSim), whose mean frame time gives the one-after-another rate."""
+    import numpy as np
+    import torch
+
+    import ensemble as en
+    from foam import FoamSim
+    cfgs = en.c4_configs(B)
+    inputs = [en.c4_inputs(c, b) for b, c in enumerate(cfgs)]
+    ens = en.Ensemble(cfgs, inputs)
+    for _ in range(warmup):
+        ens.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    seq = []
+    for b in np.linspace(0, B - 1, sample).astype(int):
+        sim = FoamSim(cfgs[b], *inputs[b])
+        for _ in range(2):
+            sim.step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            sim.step()
+        torch.cuda.synchronize()
+        seq.append((time.perf_counter() - t1) / 3)
+    ns = np.array([c.n for c in cfgs])
+    rs = np.array([c.res for c in cfgs])
+    fps = steps / el
+    seq_rate = 1.0 / float(np.mean(seq))
+    return {"workload": f"{B} foams, N log-uniform [500, 20000], radii U[0.6,1.2]*(0.5/N)^(1/3), "
+                        "JFA edge round(5 N^(1/3)) in [48,160], bands {10-14,12-16,14-18}, "
+                        "C2 dynamics",
+            "foams": B, "particles": int(ns.sum()), "voxels": int((rs ** 3).sum()),
+            "frames": steps, "batch_frames_per_s": round(fps, 3),
+            "foam_frames_per_s": round(B * fps, 1),
+            "one_at_a_time_foam_frames_per_s": round(seq_rate, 1),
+            "speedup": round(B * fps / seq_rate, 2),
+            "spread": {"n": [int(ns.min()), int(np.median(ns)), int(ns.max())],
+                       "res": [int(rs.min()), int(np.median(rs)), int(rs.max())],
+                       "one_at_a_time_ms": [round(1e3 * min(seq), 3),
+                                            round(1e3 * float(np.median(seq)), 3),
+                                            round(1e3 * max(seq), 3)]}}
 
 
 def cpu_baseline(cfg, pos, rad, frames):
@@ -229,6 +286,8 @@ def main():
                           "value": round(args.steps / el2, 3), "unit": "frames/s"}
     if not args.no_surrogate and os.path.exists(WEIGHTS):
         out["surrogate"] = surrogate_bench(args.surrogate_steps, 8)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(args.ensemble_foams, args.ensemble_steps, 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, pos, rad, args.cpu_frames)
         out["speedup_vs_cpu"] = round(fps / out["cpu_baseline"]["value"], 1)

@@ -1010,22 +1010,219 @@ __global__ void k_count_zero_b(const int32_t *__restrict__ fsc, const int32_t *_
   }
 }
 
+// The batched PBD pass reads each stencil cell's occupants from the
+// cell-ordered (position, radius) records of the grid rebuild, so a lane's
+// loads are contiguous and independent of the id load.  Lanes first count
+// their cell's corrections, a prefix over the 27 lanes gives each its place
+// in the serial (cell, occupant) order, and the lanes then write the
+// corrections there for lane 0 to add in that order; more than PBD_HITS
+// corrections send the particle through the serial loop.
+#define PBD_HITS 48
+__device__ __forceinline__ bool pbd_hit(float qx, float qy, float qz, float rj, float px,
+                                        float py, float pz, float ri, float maxd, float gap,
+                                        float L, float inv_l, float &vx, float &vy, float &vz) {
+  const float dx = gwrap(qx - px, L, inv_l);
+  const float dy = gwrap(qy - py, L, inv_l);
+  const float dz = gwrap(qz - pz, L, inv_l);
+  const float dist = sqrtf(dot3(dx, dy, dz) + 1e-12f);
+  const float ts = (ri + rj) * gap;
+  if (!(dist < ts)) return false;
+  const float overlap = ts - dist;
+  const float ux = dx / dist, uy = dy / dist, uz = dz / dist;
+  const float mag = fminf(overlap * 0.5f, maxd);
+  vx = fmaxf(-maxd, fminf(maxd, -ux * mag));
+  vy = fmaxf(-maxd, fminf(maxd, -uy * mag));
+  vz = fmaxf(-maxd, fminf(maxd, -uz * mag));
+  return true;
+}
+
+__global__ void __launch_bounds__(256)
+    k_pbd_pr_b(const float *__restrict__ pin, float *__restrict__ pout,
+               const float4 *__restrict__ pr, const float *__restrict__ rad, int n, float L,
+               float half_l, float inv_l, float gap, float mdf, float dmul,
+               const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+               const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
+               const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass) {
+  __shared__ float s_v[8][PBD_HITS][3];
+  const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
+  const int i = blockIdx.x * 8 + grp;
+  bool live = i < n;
+  int G = 3;
+  if (live) {
+    const int b = pinst[i];
+    if (pass >= npass[b]) {
+      if (l < 3) pout[3 * i + l] = pin[3 * i + l];
+      live = false;
+    } else {
+      G = inst[b].G;
+      cs += inst[b].c0;
+      cc += inst[b].c0;
+    }
+  }
+  float px = 0.f, py = 0.f, pz = 0.f, ri = 0.f, maxd = 0.f;
+  int cx = 0, cy = 0, cz = 0, st = 0, ct = 0;
+  if (live) {
+    px = pin[3 * i];
+    py = pin[3 * i + 1];
+    pz = pin[3 * i + 2];
+    ri = rad[i];
+    maxd = mdf * ri;
+    cx = centre_axis(px, G, L, half_l, inv_l);
+    cy = centre_axis(py, G, L, half_l, inv_l);
+    cz = centre_axis(pz, G, L, half_l, inv_l);
+    if (l < 27) {
+      const int id = wrap_fast(cx + l / 9 - 1, G) * G * G +
+                     wrap_fast(cy + (l / 3) % 3 - 1, G) * G + wrap_fast(cz + l % 3 - 1, G);
+      st = cs[id];
+      ct = cc[id];
+    }
+  }
+  int h = 0;
+  float vx, vy, vz;
+  for (int k = st; k < st + ct; ++k) {
+    const float4 q = pr[k];
+    if (ci[k] != i && pbd_hit(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz))
+      ++h;
+  }
+  int x = h;  // inclusive prefix over the 32-lane group
+  for (int o = 1; o < 32; o <<= 1) {
+    const int y = __shfl_up(x, o, 32);
+    if (l >= o) x += y;
+  }
+  const int H = __shfl(x, 31, 32);
+  const bool fits = H <= PBD_HITS;
+  if (live && fits && h > 0) {
+    int m = x - h;
+    for (int k = st; k < st + ct; ++k) {
+      const float4 q = pr[k];
+      if (ci[k] != i &&
+          pbd_hit(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz)) {
+        s_v[grp][m][0] = vx;
+        s_v[grp][m][1] = vy;
+        s_v[grp][m][2] = vz;
+        ++m;
+      }
+    }
+  }
+  __syncthreads();
+  if (!live || l != 0) return;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  if (fits) {
+    for (int q = 0; q < H; ++q) {
+      ax = ax + s_v[grp][q][0];
+      ay = ay + s_v[grp][q][1];
+      az = az + s_v[grp][q][2];
+    }
+  } else {
+    for_stencil(cx, cy, cz, G, cs, cc, ci, [&](int j) {
+      bool hit;
+      float wx, wy, wz;
+      pbd_pair(j, i, px, py, pz, ri, maxd, gap, L, inv_l, pin, rad, hit, wx, wy, wz);
+      if (hit) {
+        ax = ax + wx;
+        ay = ay + wy;
+        az = az + wz;
+      }
+    });
+  }
+  const float maxt = maxd * dmul;
+  const float cm = sqrtf(dot3(ax, ay, az));
+  if (cm > maxt) {
+    const float sc = maxt / cm;
+    ax = ax * sc;
+    ay = ay * sc;
+    az = az * sc;
+  }
+  pout[3 * i] = gwrap(px + ax, L, inv_l);
+  pout[3 * i + 1] = gwrap(py + ay, L, inv_l);
+  pout[3 * i + 2] = gwrap(pz + az, L, inv_l);
+}
+
+__global__ void __launch_bounds__(256)
+    k_max_overlap_pr_b(const float *__restrict__ pos, const float4 *__restrict__ pr,
+                       const float *__restrict__ rad, int n, float L, float half_l, float inv_l,
+                       const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+                       const int32_t *__restrict__ ci, float *__restrict__ lmd, float *gmax,
+                       const int32_t *__restrict__ pinst, const fos_inst *__restrict__ inst) {
+  const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
+  const int i = blockIdx.x * 8 + grp;
+  float lm = 0.0f;
+  int b = 0;
+  if (i < n && l < 27) {
+    b = pinst[i];
+    const int G = inst[b].G;
+    const int32_t *csb = cs + inst[b].c0, *ccb = cc + inst[b].c0;
+    const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    const float ri = rad[i];
+    const int cx = centre_axis(px, G, L, half_l, inv_l);
+    const int cy = centre_axis(py, G, L, half_l, inv_l);
+    const int cz = centre_axis(pz, G, L, half_l, inv_l);
+    const int id = wrap_fast(cx + l / 9 - 1, G) * G * G + wrap_fast(cy + (l / 3) % 3 - 1, G) * G +
+                   wrap_fast(cz + l % 3 - 1, G);
+    const int st = csb[id], e = st + ccb[id];
+    for (int k = st; k < e; ++k) {
+      const int j = ci[k];
+      const float4 q = pr[k];
+      if (!(i < j)) continue;
+      const float dx = gwrap(q.x - px, L, inv_l);
+      const float dy = gwrap(q.y - py, L, inv_l);
+      const float dz = gwrap(q.z - pz, L, inv_l);
+      const float dist = sqrtf(dot3(dx, dy, dz) + 1e-8f);
+      const float td = ri + q.w;
+      if (dist < td) lm = fmaxf(lm, td - dist);
+    }
+  }
+  for (int o = 16; o > 0; o >>= 1) lm = fmaxf(lm, __shfl_xor(lm, o, 32));
+  // the workgroup's eight particles merge per foam (usually one) before the
+  // atomics: a foam's particles are contiguous, so one slot would otherwise
+  // take one atomic per particle
+  __shared__ float s_lm[8];
+  __shared__ int s_b[8];
+  if (l == 0) {
+    s_lm[grp] = i < n ? lm : 0.0f;
+    s_b[grp] = i < n ? pinst[i] : -1;
+    if (i < n) lmd[i] = lm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int cb = s_b[0];
+    float cm = s_lm[0];
+    for (int g = 1; g <= 8; ++g) {
+      if (g == 8 || s_b[g] != cb) {
+        if (cb >= 0 && cm > 0.0f) atomicMax((int *)&gmax[cb], __float_as_int(cm));
+        if (g == 8) break;
+        cb = s_b[g];
+        cm = s_lm[g];
+      } else {
+        cm = fmaxf(cm, s_lm[g]);
+      }
+    }
+  }
+}
+
 extern "C" {
 
 int fos_batch_max_overlap(fos_batch *bt, const float *pos, const float *rad, float L,
                           const int32_t *cell_start, const int32_t *cell_count,
-                          const int32_t *cell_indices, float *local_max_depth,
-                          float *max_depth, void *stream) {
+                          const int32_t *cell_indices, const float *cell_pr,
+                          float *local_max_depth, float *max_depth, void *stream) {
   FOS_REQUIRE(bt && pos && rad && cell_start && cell_count && cell_indices &&
                   local_max_depth && max_depth && L > 0.0f,
               "fos_batch_max_overlap: bad arguments");
   int rc = fos_fill_f32(max_depth, bt->B, 0.0f, stream);
   if (rc) return rc;
   const int n = bt->n_total;
-  hipLaunchKernelGGL(k_max_overlap_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
-                     (hipStream_t)stream, pos, rad, n, 3, L, (float)(0.5 * (double)L),
-                     (float)(1.0 / (double)L), cell_start, cell_count, cell_indices,
-                     local_max_depth, max_depth, bt->pinst, bt->inst);
+  const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
+  if (cell_pr)
+    hipLaunchKernelGGL(k_max_overlap_pr_b, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                       (hipStream_t)stream, pos, (const float4 *)cell_pr, rad, n, L, half_l,
+                       inv_l, cell_start, cell_count, cell_indices, local_max_depth, max_depth,
+                       bt->pinst, bt->inst);
+  else
+    hipLaunchKernelGGL(k_max_overlap_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                       (hipStream_t)stream, pos, rad, n, 3, L, half_l, inv_l, cell_start,
+                       cell_count, cell_indices, local_max_depth, max_depth, bt->pinst,
+                       bt->inst);
   FOS_CHECK_LAUNCH("k_max_overlap_b");
   return 0;
 }
@@ -1033,16 +1230,23 @@ int fos_batch_max_overlap(fos_batch *bt, const float *pos, const float *rad, flo
 int fos_batch_pbd_pass(fos_batch *bt, const float *pos_in, float *pos_out, const float *rad,
                        float L, float gap_factor, float max_disp_frac, float disp_mult,
                        const int32_t *cell_start, const int32_t *cell_count,
-                       const int32_t *cell_indices, const int32_t *passes, int pass,
-                       void *stream) {
+                       const int32_t *cell_indices, const float *cell_pr,
+                       const int32_t *passes, int pass, void *stream) {
   FOS_REQUIRE(bt && pos_in && pos_out && pos_in != pos_out && rad && cell_start && cell_count &&
                   cell_indices && passes && L > 0.0f,
               "fos_batch_pbd_pass: bad arguments");
   const int n = bt->n_total;
-  hipLaunchKernelGGL(k_pbd_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
-                     (hipStream_t)stream, pos_in, pos_out, rad, n, 3, L, (float)(0.5 * (double)L),
-                     (float)(1.0 / (double)L), gap_factor, max_disp_frac, disp_mult, cell_start,
-                     cell_count, cell_indices, bt->pinst, bt->inst, passes, pass);
+  const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
+  if (cell_pr)
+    hipLaunchKernelGGL(k_pbd_pr_b, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                       (hipStream_t)stream, pos_in, pos_out, (const float4 *)cell_pr, rad, n, L,
+                       half_l, inv_l, gap_factor, max_disp_frac, disp_mult, cell_start,
+                       cell_count, cell_indices, bt->pinst, bt->inst, passes, pass);
+  else
+    hipLaunchKernelGGL(k_pbd_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                       (hipStream_t)stream, pos_in, pos_out, rad, n, 3, L, half_l, inv_l,
+                       gap_factor, max_disp_frac, disp_mult, cell_start, cell_count,
+                       cell_indices, bt->pinst, bt->inst, passes, pass);
   FOS_CHECK_LAUNCH("k_pbd_b");
   return 0;
 }

@@ -288,11 +288,16 @@ __device__ __forceinline__ void sort_run(int32_t *a, int m) {
 // Builds cells [lo, hi) of the CSR from the particles [pa, pe), whose cells
 // all lie in the grid of those cells; base0 = first CSR slot of particle pa's
 // grid.  All threads of the workgroup call it.
+// With s4 given, s4[slot] = (position, radius) of the slot's particle (the
+// cell-ordered copy the batched neighbour kernels read).
 __device__ __forceinline__ void grid_range(const int32_t *__restrict__ cell_of, int pa, int pe,
                                            int base0, int lo, int hi,
                                            int32_t *__restrict__ count,
                                            int32_t *__restrict__ start,
-                                           int32_t *__restrict__ idx) {
+                                           int32_t *__restrict__ idx,
+                                           const float *__restrict__ pos = nullptr,
+                                           const float *__restrict__ rad = nullptr,
+                                           float4 *__restrict__ s4 = nullptr) {
   __shared__ int s_cnt[GR_CELLS], s_ex[GR_CELLS];
   __shared__ int s_li[GR_LIST], s_lc[GR_LIST], s_out[GR_LIST];
   __shared__ int s_w[16], s_m;
@@ -376,7 +381,11 @@ __device__ __forceinline__ void grid_range(const int32_t *__restrict__ cell_of, 
     __syncthreads();
     for (int k = t; k < nc; k += 1024) sort_run(s_out + s_ex[k], s_cnt[k]);
     __syncthreads();
-    for (int k = t; k < m; k += 1024) idx[base + k] = s_out[k];
+    for (int k = t; k < m; k += 1024) {
+      const int i = s_out[k];
+      idx[base + k] = i;
+      if (s4) s4[base + k] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], rad[i]);
+    }
   } else {
     for (int i = pa + t; i < pe; i += 1024) {
       const int c = cell_of[i];
@@ -385,6 +394,14 @@ __device__ __forceinline__ void grid_range(const int32_t *__restrict__ cell_of, 
     __threadfence();
     __syncthreads();
     for (int k = t; k < nc; k += 1024) sort_run(idx + base + s_ex[k], s_cnt[k]);
+    if (s4) {
+      __threadfence();
+      __syncthreads();
+      for (int k = t; k < m; k += 1024) {
+        const int i = idx[base + k];
+        s4[base + k] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], rad[i]);
+      }
+    }
   }
 }
 
@@ -420,7 +437,8 @@ __global__ void __launch_bounds__(1024)
                           const fos_inst *__restrict__ inst, const int *__restrict__ gstart, int B,
                           const int32_t *__restrict__ npass, int pass,
                           int32_t *__restrict__ count, int32_t *__restrict__ start,
-                          int32_t *__restrict__ idx) {
+                          int32_t *__restrict__ idx, const float *__restrict__ pos,
+                          const float *__restrict__ rad, float4 *__restrict__ s4) {
   const int w = (int)blockIdx.x;
   int lo = 0, hi = B - 1;
   while (lo < hi) {
@@ -432,7 +450,8 @@ __global__ void __launch_bounds__(1024)
   const fos_inst I = inst[b];
   const int C = I.G * I.G * I.G, cpw = fos_range_cells(C);
   const int c_lo = I.c0 + (w - gstart[b]) * cpw;
-  grid_range(cell_of, I.p0, I.p0 + I.n, I.p0, c_lo, min(I.c0 + C, c_lo + cpw), count, start, idx);
+  grid_range(cell_of, I.p0, I.p0 + I.n, I.p0, c_lo, min(I.c0 + C, c_lo + cpw), count, start, idx,
+             pos, rad, s4);
 }
 
 // 0: multi-kernel rebuild; 1: one workgroup where it fits (else 2);
@@ -549,9 +568,9 @@ extern "C" {
 
 int fos_batch_grid_build(fos_batch *bt, fos_ctx *ctx, float *pos, float L, int32_t *cell_count,
                          int32_t *cell_start, int32_t *cell_indices, const int32_t *passes,
-                         int pass, void *stream) {
+                         int pass, const float *rad, float *cell_pr, void *stream) {
   FOS_REQUIRE(bt && ctx && pos && cell_count && cell_start && cell_indices && L > 0.0f &&
-                  bt->n_total <= ctx->max_n,
+                  bt->n_total <= ctx->max_n && (!cell_pr || rad),
               "fos_batch_grid_build: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   const int n = bt->n_total;
@@ -561,7 +580,8 @@ int fos_batch_grid_build(fos_batch *bt, fos_ctx *ctx, float *pos, float L, int32
   FOS_CHECK_LAUNCH("k_grid_cells_b");
   hipLaunchKernelGGL(k_grid_build_ranges_b, dim3((unsigned)bt->gstart_h[bt->B]), dim3(1024), 0,
                      st, (const int32_t *)ctx->cell_of, bt->inst, bt->gstart, bt->B, passes, pass,
-                     cell_count, cell_start, cell_indices);
+                     cell_count, cell_start, cell_indices, (const float *)pos, rad,
+                     (float4 *)cell_pr);
   FOS_CHECK_LAUNCH("k_grid_build_ranges_b");
   return 0;
 }

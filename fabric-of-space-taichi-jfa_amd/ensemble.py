@@ -125,6 +125,7 @@ class Ensemble:
         self.cell_count = torch.zeros(C, dtype=i32, device=dev)
         self.cell_start = torch.zeros(C, dtype=i32, device=dev)
         self.cell_indices = torch.zeros(n, dtype=i32, device=dev)
+        self.cell_pr = torch.zeros(n, 4, dtype=f32, device=dev)   # slot (position, radius)
         self.pos_tmp = torch.empty_like(self.pos)
         self.rad_target = self.rad.clone()
         self.lmd = torch.zeros(n, dtype=f32, device=dev)
@@ -179,7 +180,8 @@ class Ensemble:
         c = self.cfg
         fl.call("fos_batch_grid_build", self.handle, self.ctx.handle, fl.ptr(self.pos), float(c.L),
                 fl.ptr(self.cell_count), fl.ptr(self.cell_start), fl.ptr(self.cell_indices),
-                None if passes is None else fl.ptr(passes), int(p), fl.stream())
+                None if passes is None else fl.ptr(passes), int(p), fl.ptr(self.rad),
+                fl.ptr(self.cell_pr), fl.stream())
 
     def step(self):
         """One frame of every foam, in the run.py:700-1261 order (see
@@ -192,7 +194,7 @@ class Ensemble:
         self.rebuild_grid()
         fl.call("fos_batch_max_overlap", h, fl.ptr(self.pos), fl.ptr(self.rad), float(c.L),
                 fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
-                fl.ptr(self.lmd), fl.ptr(self.maxd), st)
+                fl.ptr(self.cell_pr), fl.ptr(self.lmd), fl.ptr(self.maxd), st)
         maxd = self.maxd.cpu().numpy()
         passes = np.array([max(c.pbd_base, min(c.pbd_max, int(c.pbd_base + c.pbd_scale * (
             float(md) / (0.2 * cb.r_max))))) for md, cb in zip(maxd, self.cfgs)], np.int32)
@@ -224,7 +226,7 @@ class Ensemble:
             fl.call("fos_batch_pbd_pass", h, fl.ptr(self.pos), fl.ptr(self.pos_tmp),
                     fl.ptr(self.rad), float(c.L), c.gap_factor(), c.max_disp_frac, c.disp_mult,
                     fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
-                    fl.ptr(self.ctl[0]), p, st)
+                    fl.ptr(self.cell_pr), fl.ptr(self.ctl[0]), p, st)
             self.pos, self.pos_tmp = self.pos_tmp, self.pos
         # 7
         self.rebuild_grid()

@@ -78,9 +78,9 @@ SIGNATURES = {
     "fos_batch_majority": [P, P, P, P],
     "fos_batch_face_accumulate": [P, P, P, P, P, P],
     "fos_batch_finalize_faces": [P, P, P, P, P, P, P, P, P, P, P],
-    "fos_batch_grid_build": [P, P, P, F, P, P, P, P, I, P],
-    "fos_batch_max_overlap": [P, P, P, F, P, P, P, P, P, P],
-    "fos_batch_pbd_pass": [P, P, P, P, F, F, F, F, P, P, P, P, I, P],
+    "fos_batch_grid_build": [P, P, P, F, P, P, P, P, I, P, P, P],
+    "fos_batch_max_overlap": [P, P, P, F, P, P, P, P, P, P, P],
+    "fos_batch_pbd_pass": [P, P, P, P, F, F, F, F, P, P, P, P, P, I, P],
     "fos_batch_nudge_radii": [P, P, P, P, F, F, I, F, P, P, P],
     "fos_batch_fsc_targets": [P, P, P, P, P, F, F, P, P],
     "fos_batch_pressure_run": [P, P, I, I, P, P, P, P, F, F, F, I, P],

@@ -396,22 +396,28 @@ int fos_batch_finalize_faces(fos_batch *batch, fos_ctx *ctx, const float *rad,
 /* grid.py:30-309 for every foam into one CSR over the batch's cells
  * (cell_count/cell_start [total cells], cell_indices [total particles],
  * batch-global).  With passes given (device int[B]), foams with
- * pass >= passes[b] are left as they are (their rebuild is not due). */
+ * pass >= passes[b] are left as they are (their rebuild is not due).  With
+ * cell_pr given (and rad), cell_pr[total particles][4] receives each CSR
+ * slot's (wrapped position, radius), which the batched PBD and max-overlap
+ * kernels read. */
 int fos_batch_grid_build(fos_batch *batch, fos_ctx *ctx, float *pos, float L,
                          int32_t *cell_count, int32_t *cell_start, int32_t *cell_indices,
-                         const int32_t *passes, int pass, void *stream);
-/* dynamics.py:304-372 for every foam; max_depth[B] (device) per foam. */
+                         const int32_t *passes, int pass, const float *rad, float *cell_pr,
+                         void *stream);
+/* dynamics.py:304-372 for every foam; max_depth[B] (device) per foam.
+ * cell_pr (may be NULL): the slot records of the last grid rebuild. */
 int fos_batch_max_overlap(fos_batch *batch, const float *pos, const float *rad, float L,
                           const int32_t *cell_start, const int32_t *cell_count,
-                          const int32_t *cell_indices, float *local_max_depth,
-                          float *max_depth, void *stream);
+                          const int32_t *cell_indices, const float *cell_pr,
+                          float *local_max_depth, float *max_depth, void *stream);
 /* dynamics.py:77-171 (Jacobi) pass `pass` for foams with pass < passes[b]
- * (device int[B]); the others copy their positions. */
+ * (device int[B]); the others copy their positions.  cell_pr (may be NULL):
+ * the slot records of the grid rebuild of pos_in. */
 int fos_batch_pbd_pass(fos_batch *batch, const float *pos_in, float *pos_out, const float *rad,
                        float L, float gap_factor, float max_disp_frac, float disp_mult,
                        const int32_t *cell_start, const int32_t *cell_count,
-                       const int32_t *cell_indices, const int32_t *passes, int pass,
-                       void *stream);
+                       const int32_t *cell_indices, const float *cell_pr,
+                       const int32_t *passes, int pass, void *stream);
 /* dynamics.py:1366-1427 for foams with active[b] (device int[B]); back[b]
  * (device float[B]) is the global backpressure factor of mode 1. */
 int fos_batch_nudge_radii(fos_batch *batch, float *rad, const float *rad_target,
