@@ -942,12 +942,16 @@ __global__ void k_fsc_targets_b(const float *__restrict__ rad, float *__restrict
   rt[i] = out;
 }
 
+// frame_dev (may be NULL): the frame number in device memory (a captured
+// frame reads it at replay), else frame.
 __global__ void __launch_bounds__(256)
-    k_pressure_step_b(int n, int frame, const float *__restrict__ rin, float *__restrict__ rout,
+    k_pressure_step_b(int n, int frame, const int32_t *__restrict__ frame_dev,
+                      const float *__restrict__ rin, float *__restrict__ rout,
                       float *__restrict__ delta_r, const int32_t *__restrict__ face_ids,
                       const int32_t *__restrict__ fsc, const int32_t *__restrict__ pinst,
                       const fos_inst *__restrict__ inst, float k, float p_exp, float inv_p,
                       float cap, int max_nei) {
+  if (frame_dev) frame = *frame_dev;
   const int i = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
   int ih = i;
   float r_min = 0.0f, r_max = 0.0f;
@@ -963,9 +967,11 @@ __global__ void __launch_bounds__(256)
 
 __global__ void k_brownian_b(float *__restrict__ pos, const int32_t *__restrict__ pinst,
                              const fos_inst *__restrict__ inst, int n, float strength, float L,
-                             float half_l, int frame, int mode) {
+                             float half_l, int frame, const int32_t *__restrict__ frame_dev,
+                             int mode) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
+  if (frame_dev) frame = *frame_dev;
   const int b = pinst[g];
   const unsigned i = (unsigned)(g - inst[b].p0);
   const uint32_t k0 = inst[b].seed, k1 = 0u;
@@ -1200,6 +1206,44 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Per-foam frame control on the device (a captured frame has no host
+// round trip), with the host driver's double-precision arithmetic
+// (run.py:709-751 pass count and global backpressure, run.py:1184-1231
+// warm-start gate and settle counter).
+__global__ void k_batch_control_pre(int B, const float *__restrict__ maxd,
+                                    const double *__restrict__ r_max, int pbd_base, int pbd_max,
+                                    double pbd_scale, double contact_tol,
+                                    int32_t *__restrict__ settle, int32_t *__restrict__ passes,
+                                    int32_t *__restrict__ active, float *__restrict__ back) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const double md = (double)maxd[b];
+  const int p = (int)((double)pbd_base + pbd_scale * (md / (0.2 * r_max[b])));
+  passes[b] = max(pbd_base, min(pbd_max, p));
+  back[b] = (float)fmax(0.25, fmin(1.0, (contact_tol - md) / contact_tol));
+  const int s = settle[b];
+  active[b] = s > 0 ? 1 : 0;
+  if (s > 0) settle[b] = s - 1;
+}
+
+__global__ void k_batch_control_post(const fos_inst *__restrict__ inst, int B,
+                                     const int32_t *__restrict__ zeros,
+                                     const int32_t *__restrict__ frame_dev, int warmstart,
+                                     double zero_rate, int adjustment, int ran,
+                                     int32_t *__restrict__ settle, int32_t *__restrict__ growth) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  int g = 1;
+  if (ran) {
+    const double f0 = (double)zeros[b] / (double)max(1, inst[b].n);
+    g = (*frame_dev < warmstart || f0 > zero_rate) ? 0 : 1;
+  }
+  growth[b] = g;
+  if (g) settle[b] = adjustment;
+}
+
+__global__ void k_frame_advance(int32_t *__restrict__ frame_dev) { *frame_dev += 1; }
+
 extern "C" {
 
 int fos_batch_max_overlap(fos_batch *bt, const float *pos, const float *rad, float L,
@@ -1278,9 +1322,10 @@ int fos_batch_fsc_targets(fos_batch *bt, const float *rad, float *rad_target,
   return 0;
 }
 
-int fos_batch_pressure_run(fos_batch *bt, fos_ctx *ctx, int frame, int iters, float *rad,
-                           float *delta_r, const int32_t *face_ids, const int32_t *fsc, float k,
-                           float p_exp, float pair_cap, int max_nei, void *stream) {
+int fos_batch_pressure_run(fos_batch *bt, fos_ctx *ctx, int frame, const int32_t *frame_dev,
+                           int iters, float *rad, float *delta_r, const int32_t *face_ids,
+                           const int32_t *fsc, float k, float p_exp, float pair_cap, int max_nei,
+                           void *stream) {
   FOS_REQUIRE(bt && ctx && bt->n_total <= ctx->max_n && iters >= 0 && rad && delta_r &&
                   face_ids && fsc,
               "fos_batch_pressure_run: bad arguments");
@@ -1290,8 +1335,8 @@ int fos_batch_pressure_run(fos_batch *bt, fos_ctx *ctx, int frame, int iters, fl
   float *a = rad, *b = ctx->rad_tmp;
   for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_pressure_step_b, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, st, n,
-                       frame, a, b, delta_r, face_ids, fsc, bt->pinst, bt->inst, k, p_exp,
-                       1.0f / p_exp, pair_cap, max_nei);
+                       frame, frame_dev, a, b, delta_r, face_ids, fsc, bt->pinst, bt->inst, k,
+                       p_exp, 1.0f / p_exp, pair_cap, max_nei);
     FOS_CHECK_LAUNCH("k_pressure_step_b");
     float *t = a;
     a = b;
@@ -1305,13 +1350,13 @@ int fos_batch_pressure_run(fos_batch *bt, fos_ctx *ctx, int frame, int iters, fl
   return 0;
 }
 
-int fos_batch_brownian(fos_batch *bt, float *pos, float strength, float L, int frame, int mode,
-                       void *stream) {
+int fos_batch_brownian(fos_batch *bt, float *pos, float strength, float L, int frame,
+                       const int32_t *frame_dev, int mode, void *stream) {
   FOS_REQUIRE(bt && pos && L > 0.0f && (mode == 0 || mode == 1),
               "fos_batch_brownian: bad arguments");
   const int n = bt->n_total;
   hipLaunchKernelGGL(k_brownian_b, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
-                     pos, bt->pinst, bt->inst, n, strength, L, L * 0.5f, frame, mode);
+                     pos, bt->pinst, bt->inst, n, strength, L, L * 0.5f, frame, frame_dev, mode);
   FOS_CHECK_LAUNCH("k_brownian_b");
   return 0;
 }
@@ -1323,6 +1368,38 @@ int fos_batch_count_zero(fos_batch *bt, const int32_t *fsc, int32_t *out, void *
   hipLaunchKernelGGL(k_count_zero_b, dim3(blocks_for(bt->n_total)), dim3(FOS_BLOCK), 0,
                      (hipStream_t)stream, fsc, bt->pinst, bt->n_total, out);
   FOS_CHECK_LAUNCH("k_count_zero_b");
+  return 0;
+}
+
+int fos_batch_control_pre(fos_batch *bt, const float *max_depth, const double *r_max,
+                          int pbd_base, int pbd_max, double pbd_scale, double contact_tol,
+                          int32_t *settle, int32_t *passes, int32_t *active, float *back,
+                          void *stream) {
+  FOS_REQUIRE(bt && max_depth && r_max && settle && passes && active && back && contact_tol != 0.0,
+              "fos_batch_control_pre: bad arguments");
+  hipLaunchKernelGGL(k_batch_control_pre, dim3(blocks_for(bt->B)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, bt->B, max_depth, r_max, pbd_base, pbd_max, pbd_scale,
+                     contact_tol, settle, passes, active, back);
+  FOS_CHECK_LAUNCH("k_batch_control_pre");
+  return 0;
+}
+
+int fos_batch_control_post(fos_batch *bt, const int32_t *zeros, const int32_t *frame_dev,
+                           int warmstart_frames, double fsc_zero_rate, int adjustment_frames,
+                           int ran, int32_t *settle, int32_t *growth, void *stream) {
+  FOS_REQUIRE(bt && zeros && frame_dev && settle && growth,
+              "fos_batch_control_post: bad arguments");
+  hipLaunchKernelGGL(k_batch_control_post, dim3(blocks_for(bt->B)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, bt->inst, bt->B, zeros, frame_dev, warmstart_frames,
+                     fsc_zero_rate, adjustment_frames, ran, settle, growth);
+  FOS_CHECK_LAUNCH("k_batch_control_post");
+  return 0;
+}
+
+int fos_batch_frame_advance(int32_t *frame_dev, void *stream) {
+  FOS_REQUIRE(frame_dev, "fos_batch_frame_advance: bad arguments");
+  hipLaunchKernelGGL(k_frame_advance, dim3(1), dim3(1), 0, (hipStream_t)stream, frame_dev);
+  FOS_CHECK_LAUNCH("k_frame_advance");
   return 0;
 }
 

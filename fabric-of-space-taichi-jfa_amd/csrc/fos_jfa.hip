@@ -1034,10 +1034,14 @@ __global__ void k_fin_rank(int n, const int32_t *__restrict__ face_ids,
 // packed as one bit per slot.
 // 8 lanes per row gather the accepted-incoming count (an order-free sum);
 // lane 0 then decides the row.
+// done (may be NULL): when set, the fixed point was reached in an earlier
+// round pair and the round does nothing.
 __global__ void __launch_bounds__(256)
     k_fin_round(int n, const unsigned char *__restrict__ cls, const int *__restrict__ in_start,
                 const int *__restrict__ in_sorted, const unsigned *__restrict__ dec_prev,
-                unsigned *__restrict__ dec_next, int *__restrict__ changed) {
+                unsigned *__restrict__ dec_next, int *__restrict__ changed,
+                const int *__restrict__ done) {
+  if (done && *done) return;
   const int sub = (int)(threadIdx.x & 7);
   const int i = (int)(blockIdx.x * 32 + (threadIdx.x >> 3));
   int acc = 0;
@@ -1069,6 +1073,13 @@ __global__ void __launch_bounds__(256)
   }
   dec_next[i] = bits;
   if (bits != dec_prev[i]) atomicOr(changed, 1);
+}
+
+// After a round pair of the fixed-round form: the pair changed nothing ->
+// done; clears the change flag for the next pair.
+__global__ void k_fin_pair_end(int *__restrict__ changed, int *__restrict__ done) {
+  if (!*done && !*changed) *done = 1;
+  *changed = 0;
 }
 
 // Writes row i's final neighbour list, hysteresis scores and counts.  The
@@ -1650,7 +1661,7 @@ int fos_face_accumulate(fos_ctx *ctx, int res, int n, const int32_t *filtered,
 static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
                          const int32_t *face_ids, const int32_t *face_counts,
                          float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
-                         int32_t *used_count, int32_t *fsc, void *stream);
+                         int32_t *used_count, int32_t *fsc, int fixed_pairs, void *stream);
 
 extern "C" {
 
@@ -1664,15 +1675,18 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
                               neighbor_list && neighbor_count && used_count && fsc)),
               "fos_finalize_faces: bad arguments (n=%d)", n);
   return finalize_impl(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores,
-                       neighbor_list, neighbor_count, used_count, fsc, stream);
+                       neighbor_list, neighbor_count, used_count, fsc, 0, stream);
 }
 
 }  // extern "C"
 
+// fixed_pairs > 0: that many round pairs with no host round trip (a device
+// flag, ctx->flags[2], is set once a pair changes nothing and later rounds
+// skip); 0: pairs until the host sees no change.
 static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
                          const int32_t *face_ids, const int32_t *face_counts,
                          float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
-                         int32_t *used_count, int32_t *fsc, void *stream) {
+                         int32_t *used_count, int32_t *fsc, int fixed_pairs, void *stream) {
   if (n == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   const int64_t m = (int64_t)n * FOS_MAX_NEIGHBORS;
@@ -1698,19 +1712,36 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
   unsigned *dec_a = (unsigned *)ctx->dec, *dec_b = dec_a + ctx->max_n;
   rc = fos_fill_i32((int32_t *)dec_a, n, 0, stream);
   if (rc) return rc;
-  for (int pair = 0; pair <= n + 1; ++pair) {
-    rc = fos_fill_i32(ctx->flags + 1, 1, 0, stream);
+  if (fixed_pairs > 0) {
+    rc = fos_fill_i32(ctx->flags + 1, 2, 0, stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n, ctx->cls,
-                       ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1);
-    FOS_CHECK_LAUNCH("k_fin_round");
-    hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n, ctx->cls,
-                       ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1);
-    FOS_CHECK_LAUNCH("k_fin_round");
-    FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags + 1, sizeof(int),
-                           hipMemcpyDeviceToHost, st));
-    FOS_HIP(hipStreamSynchronize(st));
-    if (!*(volatile int *)ctx->host_i) break;
+    for (int pair = 0; pair < fixed_pairs; ++pair) {
+      hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
+                         ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1,
+                         (const int *)(ctx->flags + 2));
+      hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
+                         ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1,
+                         (const int *)(ctx->flags + 2));
+      hipLaunchKernelGGL(k_fin_pair_end, dim3(1), dim3(1), 0, st, ctx->flags + 1, ctx->flags + 2);
+      FOS_CHECK_LAUNCH("k_fin_round");
+    }
+  } else {
+    for (int pair = 0; pair <= n + 1; ++pair) {
+      rc = fos_fill_i32(ctx->flags + 1, 1, 0, stream);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
+                         ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1,
+                         (const int *)nullptr);
+      FOS_CHECK_LAUNCH("k_fin_round");
+      hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
+                         ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1,
+                         (const int *)nullptr);
+      FOS_CHECK_LAUNCH("k_fin_round");
+      FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags + 1, sizeof(int),
+                             hipMemcpyDeviceToHost, st));
+      FOS_HIP(hipStreamSynchronize(st));
+      if (!*(volatile int *)ctx->host_i) break;
+    }
   }
   hipLaunchKernelGGL(k_fin_materialize, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
                      face_ids, ctx->score, ctx->cls, dec_a, ctx->in_start, ctx->in_sorted,
@@ -2096,7 +2127,8 @@ int fos_batch_majority(fos_batch *bt, const int32_t *grid, int32_t *filtered, vo
 }
 
 int fos_batch_face_accumulate(fos_batch *bt, fos_ctx *ctx, const int32_t *filtered,
-                              int32_t *face_ids, int32_t *face_counts, void *stream) {
+                              int32_t *face_ids, int32_t *face_counts, int check,
+                              void *stream) {
   FOS_REQUIRE(bt && ctx && filtered && face_ids && face_counts && bt->n_total <= ctx->max_n,
               "fos_batch_face_accumulate: bad arguments");
   FOS_REQUIRE((double)bt->max_res * bt->max_res * bt->max_res * 6.0 < 4294967295.0,
@@ -2111,6 +2143,7 @@ int fos_batch_face_accumulate(fos_batch *bt, fos_ctx *ctx, const int32_t *filter
                        bt->estart, bt->B, total, per, n, filtered, ctx->tab_b, ctx->tab_k,
                        ctx->tab_c, ctx->tab_cap, ctx->flags);
     FOS_CHECK_LAUNCH("k_face_emit_b");
+    if (!check) break;  // the caller reads the overflow flag (fos_batch_flags)
     FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, st));
     FOS_HIP(hipStreamSynchronize(st));
     if (!*(volatile int *)ctx->host_i) break;
@@ -2133,13 +2166,26 @@ int fos_batch_face_accumulate(fos_batch *bt, fos_ctx *ctx, const int32_t *filter
 int fos_batch_finalize_faces(fos_batch *bt, fos_ctx *ctx, const float *rad,
                              const int32_t *face_ids, const int32_t *face_counts,
                              float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
-                             int32_t *used_count, int32_t *fsc, void *stream) {
+                             int32_t *used_count, int32_t *fsc, int fixed_pairs, void *stream) {
   FOS_REQUIRE(bt && ctx && bt->n_total <= ctx->max_n &&
                   (bt->n_total == 0 || (rad && face_ids && face_counts && prev_scores &&
                                         neighbor_list && neighbor_count && used_count && fsc)),
               "fos_batch_finalize_faces: bad arguments");
+  FOS_REQUIRE(fixed_pairs >= 0, "fos_batch_finalize_faces: fixed_pairs %d", fixed_pairs);
   return finalize_impl(ctx, bt->n_total, rad, 1.0f, bt->pvox, face_ids, face_counts,
-                       prev_scores, neighbor_list, neighbor_count, used_count, fsc, stream);
+                       prev_scores, neighbor_list, neighbor_count, used_count, fsc, fixed_pairs,
+                       stream);
+}
+
+int fos_batch_flags(fos_ctx *ctx, int32_t *out, void *stream) {
+  FOS_REQUIRE(ctx && out, "fos_batch_flags: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipStreamSynchronize(st));
+  const int *h = (const int *)ctx->host_i;
+  out[0] = h[0];
+  out[1] = h[2];
+  return 0;
 }
 
 }  // extern "C"

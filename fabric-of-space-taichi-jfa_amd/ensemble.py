@@ -21,6 +21,7 @@ import foam_config as fc
 
 MAXN = 32
 SLOTS = 16          # pass counter slots per foam (FOS_BATCH_PASSES)
+FIXED_PAIRS = 16    # finalize round pairs in a captured frame
 
 # settings every foam of a batch shares (the per-foam ones are n, res, the
 # radius bounds, the hash grid, the FSC band and the seed)
@@ -136,7 +137,8 @@ class Ensemble:
         # [1] settle active, [2] growth active (int); back factor (float)
         self.ctl = torch.zeros(3, B, dtype=i32, device=dev)
         self.back = torch.zeros(B, dtype=f32, device=dev)
-        self.ctx = fl.context(n, int(self.res_of.max()), C)
+        # its own context: a captured frame holds pointers into its scratch
+        self.ctx = fl.Context(n, int(self.res_of.max()), C)
         self.frame = 0
         self.settle = np.zeros(B, np.int64)
         self.growth = np.ones(B, bool)
@@ -160,8 +162,9 @@ class Ensemble:
         t = self.grid if t is None else t
         return t[int(self.v0[b]):int(self.v0[b + 1])]
 
-    def run_jfa(self):
-        """jfa.run_jfa for every foam (jfa.py:1078-1132), batched."""
+    def run_jfa(self, graph=False):
+        """jfa.run_jfa for every foam (jfa.py:1078-1132), batched.  graph:
+        no host round trips (the flags are read after the frame)."""
         c, ctx, st, h = self.cfg, self.ctx.handle, fl.stream(), self.handle
         fl.call("fos_batch_jfa_init", h, ctx, fl.ptr(self.pos), fl.ptr(self.rad), float(c.L),
                 float(c.beta), fl.ptr(self.grid), None, st)
@@ -169,10 +172,11 @@ class Ensemble:
                 c.min_passes, fl.ptr(self.grid), fl.ptr(self.temp), fl.ptr(self.info), st)
         fl.call("fos_batch_majority", h, fl.ptr(self.grid), fl.ptr(self.filtered), st)
         fl.call("fos_batch_face_accumulate", h, ctx, fl.ptr(self.filtered),
-                fl.ptr(self.face_ids), fl.ptr(self.face_counts), st)
+                fl.ptr(self.face_ids), fl.ptr(self.face_counts), int(not graph), st)
         fl.call("fos_batch_finalize_faces", h, ctx, fl.ptr(self.rad), fl.ptr(self.face_ids),
                 fl.ptr(self.face_counts), fl.ptr(self.prev_scores), fl.ptr(self.neighbor_list),
-                fl.ptr(self.neighbor_count), fl.ptr(self.used_count), fl.ptr(self.fsc), st)
+                fl.ptr(self.neighbor_count), fl.ptr(self.used_count), fl.ptr(self.fsc),
+                FIXED_PAIRS if graph else 0, st)
         fl.call("fos_fsc_ema", self.n, c.ema_alpha, fl.ptr(self.fsc), fl.ptr(self.fsc_ema), st)
 
     # -- the frame (foam.FoamSim.step for every foam) -----------------------------
@@ -188,6 +192,10 @@ class Ensemble:
         foam.FoamSim.step).  The host reads the foams' overlap maxima (their
         PBD pass counts) and, on JFA frames, their zero-FSC counts."""
         c, f, st, h, B = self.cfg, self.frame, fl.stream(), self.handle, self.B
+        if getattr(self, "graphs", None):
+            # an eager frame may regrow the face-row tables a graph points into
+            self.settle = self.settle_dev.cpu().numpy().astype(np.int64)
+            self.graphs = None
         if c.ou_jitter:
             raise NotImplementedError("OU jitter is not batched")
         # 0-1
@@ -211,14 +219,14 @@ class Ensemble:
             self.settle[act] -= 1
         # 3 pressure
         if c.pressure:
-            fl.call("fos_batch_pressure_run", h, self.ctx.handle, f, c.pressure_iters,
+            fl.call("fos_batch_pressure_run", h, self.ctx.handle, f, None, c.pressure_iters,
                     fl.ptr(self.rad), fl.ptr(self.delta_r), fl.ptr(self.face_ids),
                     fl.ptr(self.fsc), c.pressure_k, c.pressure_exp, c.pressure_cap,
                     c.pressure_max_nei, st)
         # 4 Brownian
         if c.brownian:
             fl.call("fos_batch_brownian", h, fl.ptr(self.pos), c.brownian_strength, float(c.L), f,
-                    c.brownian_mode, st)
+                    None, c.brownian_mode, st)
         # 5 PBD: a foam with fewer passes sits the later passes out
         self.ctl[0].copy_(torch.from_numpy(passes))
         for p in range(int(passes.max())):
@@ -251,6 +259,87 @@ class Ensemble:
         self.frame += 1
         return self.last
 
+    # -- captured frames (hipGraph) -------------------------------------------
+    def _frame_device(self, ran):
+        """One frame with every per-foam decision on the device and no host
+        round trip: what capture() records.  ran: whether the JFA runs."""
+        c, st, h = self.cfg, fl.stream(), self.handle
+        self.rebuild_grid()
+        fl.call("fos_batch_max_overlap", h, fl.ptr(self.pos), fl.ptr(self.rad), float(c.L),
+                fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
+                fl.ptr(self.cell_pr), fl.ptr(self.lmd), fl.ptr(self.maxd), st)
+        fl.call("fos_batch_control_pre", h, fl.ptr(self.maxd), fl.ptr(self.rmax_d), c.pbd_base,
+                c.pbd_max, float(c.pbd_scale), float(c.contact_tol), fl.ptr(self.settle_dev),
+                fl.ptr(self.ctl[0]), fl.ptr(self.ctl[1]), fl.ptr(self.back), st)
+        fl.call("fos_batch_nudge_radii", h, fl.ptr(self.rad), fl.ptr(self.rad_target),
+                fl.ptr(self.lmd), c.alpha(), c.max_step_pct, c.backpressure_mode, c.contact_tol,
+                fl.ptr(self.back), fl.ptr(self.ctl[1]), st)
+        if c.pressure:
+            fl.call("fos_batch_pressure_run", h, self.ctx.handle, 0, fl.ptr(self.frame_dev),
+                    c.pressure_iters, fl.ptr(self.rad), fl.ptr(self.delta_r),
+                    fl.ptr(self.face_ids), fl.ptr(self.fsc), c.pressure_k, c.pressure_exp,
+                    c.pressure_cap, c.pressure_max_nei, st)
+        if c.brownian:
+            fl.call("fos_batch_brownian", h, fl.ptr(self.pos), c.brownian_strength, float(c.L), 0,
+                    fl.ptr(self.frame_dev), c.brownian_mode, st)
+        for p in range(c.pbd_max):   # foams past their own pass count sit out
+            self.rebuild_grid(self.ctl[0], p)
+            fl.call("fos_batch_pbd_pass", h, fl.ptr(self.pos), fl.ptr(self.pos_tmp),
+                    fl.ptr(self.rad), float(c.L), c.gap_factor(), c.max_disp_frac, c.disp_mult,
+                    fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
+                    fl.ptr(self.cell_pr), fl.ptr(self.ctl[0]), p, st)
+            self.pos, self.pos_tmp = self.pos_tmp, self.pos
+        self.rebuild_grid()
+        if ran:
+            self.run_jfa(graph=True)
+            fl.call("fos_batch_count_zero", h, fl.ptr(self.fsc), fl.ptr(self.zeros), st)
+        fl.call("fos_batch_control_post", h, fl.ptr(self.zeros), fl.ptr(self.frame_dev),
+                c.warmstart_frames, float(c.fsc_zero_rate), c.adjustment_frames, int(ran),
+                fl.ptr(self.settle_dev), fl.ptr(self.ctl[2]), st)
+        fl.call("fos_batch_fsc_targets", h, fl.ptr(self.rad), fl.ptr(self.rad_target),
+                fl.ptr(self.fsc), fl.ptr(self.fsc_ema), c.growth_pct, c.deadband,
+                fl.ptr(self.ctl[2]), st)
+        fl.call("fos_batch_frame_advance", fl.ptr(self.frame_dev), st)
+
+    def capture(self):
+        """Records the frame as hip graphs (one with the JFA and, when the
+        cadence skips frames, one without); replay() then runs one frame per
+        graph launch.  Run eager step()s first: they size the face-row tables
+        the graphs write.  Eager steps after a capture need a new capture."""
+        c = self.cfg
+        if c.pbd_max % 2:
+            raise NotImplementedError("a captured frame needs an even PBD pass bound")
+        dev = self.pos.device
+        self.frame_dev = torch.tensor([self.frame], dtype=torch.int32, device=dev)
+        self.settle_dev = torch.tensor(self.settle.astype(np.int32), device=dev)
+        self.rmax_d = torch.tensor([float(cb.r_max) for cb in self.cfgs], dtype=torch.float64,
+                                   device=dev)
+        span = c.jfa_warm + max(1, c.jfa_every) + 1
+        kinds = sorted({c.jfa_runs(f) for f in range(self.frame, self.frame + span)})
+        torch.cuda.synchronize()
+        self.graphs = {}
+        for ran in kinds:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._frame_device(ran)
+            self.graphs[ran] = g
+        torch.cuda.synchronize()
+
+    def replay(self, check=True):
+        """One captured frame; check: read the face-table overflow and the
+        finalize fixed-point flags (one host round trip) and fail loudly if a
+        table overflowed or the fixed point needed more round pairs."""
+        ran = self.cfg.jfa_runs(self.frame)
+        self.graphs[ran].replay()
+        self.frame += 1
+        if check and ran:
+            out = (ctypes.c_int32 * 2)()
+            fl.call("fos_batch_flags", self.ctx.handle, out, fl.stream())
+            if out[0] or not out[1]:
+                raise RuntimeError(f"captured frame {self.frame - 1}: face-row table overflow "
+                                   f"({out[0]}) or finalize not converged in {FIXED_PAIRS} "
+                                   "round pairs; step() eagerly and capture again")
+
     def snapshot(self, b):
         """Foam b's state with foam-local particle ids (foam.FoamSim.snapshot
         keys)."""
@@ -264,5 +353,6 @@ class Ensemble:
                 a[a >= 0] -= p0
             out[k] = a
         out["frame"] = np.int64(self.frame)
-        out["settle"] = np.int64(self.settle[b])
+        settle = self.settle_dev.cpu().numpy() if getattr(self, "graphs", None) else self.settle
+        out["settle"] = np.int64(settle[b])
         return out

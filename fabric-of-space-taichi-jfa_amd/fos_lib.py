@@ -76,15 +76,19 @@ SIGNATURES = {
     "fos_batch_jfa_init": [P, P, P, P, F, F, P, P, P],
     "fos_batch_jfa_passes": [P, P, F, I, D, I, P, P, P, P],
     "fos_batch_majority": [P, P, P, P],
-    "fos_batch_face_accumulate": [P, P, P, P, P, P],
-    "fos_batch_finalize_faces": [P, P, P, P, P, P, P, P, P, P, P],
+    "fos_batch_face_accumulate": [P, P, P, P, P, I, P],
+    "fos_batch_finalize_faces": [P, P, P, P, P, P, P, P, P, P, I, P],
+    "fos_batch_flags": [P, P, P],
+    "fos_batch_control_pre": [P, P, P, I, I, D, D, P, P, P, P, P],
+    "fos_batch_control_post": [P, P, P, I, D, I, I, P, P, P],
+    "fos_batch_frame_advance": [P, P],
     "fos_batch_grid_build": [P, P, P, F, P, P, P, P, I, P, P, P],
     "fos_batch_max_overlap": [P, P, P, F, P, P, P, P, P, P, P],
     "fos_batch_pbd_pass": [P, P, P, P, F, F, F, F, P, P, P, P, P, I, P],
     "fos_batch_nudge_radii": [P, P, P, P, F, F, I, F, P, P, P],
     "fos_batch_fsc_targets": [P, P, P, P, P, F, F, P, P],
-    "fos_batch_pressure_run": [P, P, I, I, P, P, P, P, F, F, F, I, P],
-    "fos_batch_brownian": [P, P, F, F, I, I, P],
+    "fos_batch_pressure_run": [P, P, I, P, I, P, P, P, P, F, F, F, I, P],
+    "fos_batch_brownian": [P, P, F, F, I, P, I, P],
     "fos_batch_count_zero": [P, P, P, P],
     "fos_pred_candidates": [P, P, I, I, F, F, I, P, P, P, P, P, P, P, P, P],
     "fos_pred_pairs": [P, P, I, P, P, P],

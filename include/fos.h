@@ -385,14 +385,23 @@ int fos_batch_jfa_passes(fos_batch *batch, fos_ctx *ctx, float L, int early_exit
                          int32_t *info, void *stream);
 /* jfa.py:640-699 for every foam. */
 int fos_batch_majority(fos_batch *batch, const int32_t *grid, int32_t *filtered, void *stream);
-/* jfa.py:703-762 for every foam (rows are batch particles). */
+/* jfa.py:703-762 for every foam (rows are batch particles).  check: read
+ * the row-table overflow flag on the host and grow the tables and emit again
+ * (synchronises); 0: no host round trip, the caller reads the flag later
+ * (fos_batch_flags) -- for frames captured as a graph. */
 int fos_batch_face_accumulate(fos_batch *batch, fos_ctx *ctx, const int32_t *filtered,
-                              int32_t *face_ids, int32_t *face_counts, void *stream);
-/* jfa.py:789-857 for every foam (each foam's voxel edge in the scores). */
+                              int32_t *face_ids, int32_t *face_counts, int check, void *stream);
+/* jfa.py:789-857 for every foam (each foam's voxel edge in the scores).
+ * fixed_pairs > 0: that many fixed-point round pairs with no host round trip
+ * (a device flag marks convergence, fos_batch_flags); 0: until converged,
+ * the host checking after each pair. */
 int fos_batch_finalize_faces(fos_batch *batch, fos_ctx *ctx, const float *rad,
                              const int32_t *face_ids, const int32_t *face_counts,
                              float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
-                             int32_t *used_count, int32_t *fsc, void *stream);
+                             int32_t *used_count, int32_t *fsc, int fixed_pairs, void *stream);
+/* Host out[2] = {row-table overflow of the last emission, fixed point
+ * reached in the last fixed-round finalize} (synchronises the stream). */
+int fos_batch_flags(fos_ctx *ctx, int32_t *out, void *stream);
 /* grid.py:30-309 for every foam into one CSR over the batch's cells
  * (cell_count/cell_start [total cells], cell_indices [total particles],
  * batch-global).  With passes given (device int[B]), foams with
@@ -430,17 +439,36 @@ int fos_batch_fsc_targets(fos_batch *batch, const float *rad, float *rad_target,
                           const int32_t *fsc, const float *fsc_ema, float growth_pct,
                           float deadband, const int32_t *active, void *stream);
 /* dynamics.py:1497-1557 x iters for every foam (each foam's radius bounds;
- * the rotation hash takes the index within the foam). */
-int fos_batch_pressure_run(fos_batch *batch, fos_ctx *ctx, int frame, int iters, float *rad,
-                           float *delta_r, const int32_t *face_ids, const int32_t *fsc, float k,
-                           float p_exp, float pair_cap, int max_nei, void *stream);
+ * the rotation hash takes the index within the foam).  frame_dev (may be
+ * NULL): the frame number in device memory, read at run time. */
+int fos_batch_pressure_run(fos_batch *batch, fos_ctx *ctx, int frame, const int32_t *frame_dev,
+                           int iters, float *rad, float *delta_r, const int32_t *face_ids,
+                           const int32_t *fsc, float k, float p_exp, float pair_cap, int max_nei,
+                           void *stream);
 /* dynamics.py:1596-1634 for every foam, Philox keyed (foam seed, frame,
- * index within the foam). */
+ * index within the foam); frame_dev as for fos_batch_pressure_run. */
 int fos_batch_brownian(fos_batch *batch, float *pos, float strength, float L, int frame,
-                       int mode, void *stream);
+                       const int32_t *frame_dev, int mode, void *stream);
 /* out[b] (device int[B]) = particles of foam b with fsc == 0 (the
  * warm-start gate, run.py:1184-1195). */
 int fos_batch_count_zero(fos_batch *batch, const int32_t *fsc, int32_t *out, void *stream);
+/* The host driver's per-foam frame decisions on the device (all arrays
+ * device, [B]): from the overlap maxima, each foam's PBD pass count
+ * (run.py:709-714, r_max in double as the driver holds it), its global
+ * backpressure factor and whether it settles this frame (settle counters
+ * count down; run.py:733-751). */
+int fos_batch_control_pre(fos_batch *batch, const float *max_depth, const double *r_max,
+                          int pbd_base, int pbd_max, double pbd_scale, double contact_tol,
+                          int32_t *settle, int32_t *passes, int32_t *active, float *back,
+                          void *stream);
+/* After the JFA (ran = 1) or a frame without it (ran = 0): the warm-start
+ * gate per foam from its zero-FSC count and the device frame number
+ * (run.py:1184-1195); growing foams restart their settle counters. */
+int fos_batch_control_post(fos_batch *batch, const int32_t *zeros, const int32_t *frame_dev,
+                           int warmstart_frames, double fsc_zero_rate, int adjustment_frames,
+                           int ran, int32_t *settle, int32_t *growth, void *stream);
+/* *frame_dev += 1 on the device. */
+int fos_batch_frame_advance(int32_t *frame_dev, void *stream);
 
 #ifdef __cplusplus
 }

@@ -105,3 +105,24 @@ def test_batch_frames_match_single_foams(gpu):
                 assert np.array_equal(got[k], want[k]), (f, b, k)
         seen.add(tuple(last["pbd_passes"]))
     assert any(len(set(p)) > 1 for p in seen) or len(seen) > 1
+
+
+def test_batch_captured_frames_match_eager(gpu):
+    """The frame captured as a hip graph (per-foam decisions on the device,
+    fixed finalize rounds) replays to the eager frames bit for bit."""
+    from ensemble import Ensemble
+    cfgs, inputs = frame_foams()
+    a = Ensemble(cfgs, inputs)
+    b = Ensemble(cfgs, inputs)
+    for _ in range(2):
+        a.step()
+        b.step()
+    b.capture()
+    for f in range(4):
+        print(f"[batch graph] frame {f}", flush=True)
+        a.step()
+        b.replay()
+        for k in range(len(cfgs)):
+            ga, gb = a.snapshot(k), b.snapshot(k)
+            for key in ga:
+                assert np.array_equal(ga[key], gb[key]), (f, k, key)
