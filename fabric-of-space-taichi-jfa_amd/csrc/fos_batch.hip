@@ -46,6 +46,7 @@ int fos_batch_create(fos_batch **out, int B, const int *n, const int *res, const
     I.f_low = f_low[b];
     I.f_high = f_high[b];
     I.seed = seed[b];
+    I.tile = 0;
     const long long r = res[b], C = (long long)G[b] * G[b] * G[b];
     const long long nyb = (r + 3) / 4, nzb = (r + 63) / 64;
     ts[b] = (int)t;
@@ -84,14 +85,16 @@ int fos_batch_create(fos_batch **out, int B, const int *n, const int *res, const
   bt->max_G = max_G;
   bt->inst_h = (fos_inst *)malloc(sizeof(fos_inst) * B);
   bt->tstart_h = (int *)malloc(sizeof(int) * (B + 1));
+  bt->mstart_h = (int *)malloc(sizeof(int) * (B + 1));
   bt->estart_h = (int *)malloc(sizeof(int) * (B + 1));
   bt->gstart_h = (int *)malloc(sizeof(int) * (B + 1));
-  if (!bt->inst_h || !bt->tstart_h || !bt->estart_h || !bt->gstart_h) {
+  if (!bt->inst_h || !bt->tstart_h || !bt->mstart_h || !bt->estart_h || !bt->gstart_h) {
     fos_batch_destroy(bt);
     FOS_REQUIRE(false, "fos_batch_create: out of host memory");
   }
   memcpy(bt->inst_h, h.data(), sizeof(fos_inst) * B);
   memcpy(bt->tstart_h, ts.data(), sizeof(int) * (B + 1));
+  memcpy(bt->mstart_h, ts.data(), sizeof(int) * (B + 1));
   memcpy(bt->estart_h, es.data(), sizeof(int) * (B + 1));
   memcpy(bt->gstart_h, gs.data(), sizeof(int) * (B + 1));
   hipError_t err = hipSuccess;
@@ -99,6 +102,7 @@ int fos_batch_create(fos_batch **out, int B, const int *n, const int *res, const
   if (err == hipSuccess) err = hipMalloc((void **)&(ptr), (bytes))
   BALLOC(bt->inst, sizeof(fos_inst) * B);
   BALLOC(bt->tstart, sizeof(int) * (B + 1));
+  BALLOC(bt->mstart, sizeof(int) * (B + 1));
   BALLOC(bt->estart, sizeof(int) * (B + 1));
   BALLOC(bt->gstart, sizeof(int) * (B + 1));
   BALLOC(bt->pinst, sizeof(int) * p);
@@ -109,6 +113,7 @@ int fos_batch_create(fos_batch **out, int B, const int *n, const int *res, const
 #undef BALLOC
   if (err == hipSuccess) err = hipMemcpy(bt->inst, h.data(), sizeof(fos_inst) * B, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(bt->tstart, ts.data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(bt->mstart, ts.data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(bt->estart, es.data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(bt->gstart, gs.data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(bt->pinst, pinst.data(), sizeof(int) * p, hipMemcpyHostToDevice);
@@ -125,6 +130,7 @@ int fos_batch_destroy(fos_batch *bt) {
   if (!bt) return 0;
   hipFree(bt->inst);
   hipFree(bt->tstart);
+  hipFree(bt->mstart);
   hipFree(bt->estart);
   hipFree(bt->gstart);
   hipFree(bt->pinst);
@@ -134,9 +140,31 @@ int fos_batch_destroy(fos_batch *bt) {
   hipFree(bt->ired);
   free(bt->inst_h);
   free(bt->tstart_h);
+  free(bt->mstart_h);
   free(bt->estart_h);
   free(bt->gstart_h);
   free(bt);
+  return 0;
+}
+
+int fos_batch_set_tiles(fos_batch *bt, const int *kinds) {
+  FOS_REQUIRE(bt && kinds, "fos_batch_set_tiles: bad arguments");
+  const int B = bt->B;
+  std::vector<int> ts(B + 1);
+  long long t = 0;
+  for (int b = 0; b < B; ++b) {
+    FOS_REQUIRE(kinds[b] == 0 || kinds[b] == 1, "fos_batch_set_tiles: foam %d: tile %d", b,
+                kinds[b]);
+    const long long r = bt->inst_h[b].res;
+    ts[b] = (int)t;
+    t += kinds[b] ? r * ((r * r + 255) / 256) : r * ((r + 3) / 4) * ((r + 63) / 64);
+    FOS_REQUIRE(t < (1ll << 31), "fos_batch_set_tiles: tile count exceeds 32 bits");
+    bt->inst_h[b].tile = kinds[b];
+  }
+  ts[B] = (int)t;
+  memcpy(bt->tstart_h, ts.data(), sizeof(int) * (B + 1));
+  FOS_HIP(hipMemcpy(bt->tstart, ts.data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice));
+  FOS_HIP(hipMemcpy(bt->inst, bt->inst_h, sizeof(fos_inst) * B, hipMemcpyHostToDevice));
   return 0;
 }
 

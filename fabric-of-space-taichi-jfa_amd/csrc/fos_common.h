@@ -218,6 +218,7 @@ struct fos_inst {
   float r_min, r_max;  // radius bounds
   float f_low, f_high; // FSC band
   unsigned seed;       // noise seed
+  int tile;            // JFA pass tile: 0 rows (4 x 64 z), 1 flat (256 voxels of a plane)
 };
 
 #define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
@@ -235,10 +236,11 @@ struct fos_batch {
   long long v_total;
   fos_inst *inst;          // device [B]
   fos_inst *inst_h;        // host [B]
-  int *tstart;             // device [B+1] first pass/majority tile (1 x 4 x 64 voxels)
+  int *tstart;             // device [B+1] first JFA pass tile (row or flat, per foam)
+  int *mstart;             // device [B+1] first majority tile (1 x 4 x 64 voxels)
   int *estart;             // device [B+1] first face-emission tile (2 x 4 x 64 voxels)
   int *gstart;             // device [B+1] first grid-rebuild workgroup
-  int *tstart_h, *estart_h, *gstart_h;  // host copies
+  int *tstart_h, *mstart_h, *estart_h, *gstart_h;  // host copies
   int *pinst;              // device [n_total] foam of each particle
   float *pvox;             // device [n_total] voxel edge of each particle's foam
   int *stripes;            // device [B][FOS_BATCH_PASSES][FOS_STRIPE_PASS_INTS]

@@ -402,6 +402,52 @@ __device__ __forceinline__ unsigned long long incr_wave_mask(
   return w;
 }
 
+// The owner of one voxel from its 27 candidate labels (lab[13] = current
+// owner): the kept candidates go to this thread's LDS list column, then the
+// list is walked JFA_BATCH entries at a time in list order with strict '<'.
+template <bool UNIT_L, int MODE>
+__device__ __forceinline__ int jfa_list_eval(const int lab[27], float cx, float cy, float cz,
+                                             const float4 *__restrict__ seeds, float L,
+                                             int (*s_list)[256]) {
+  const int t = (int)threadIdx.x;
+  const int inc = lab[13];
+  // Branch-free list build: every candidate is written at the current
+  // end and the end advances only for kept owners.
+  s_list[0][t] = inc;
+  int cnt = inc >= 0 ? 1 : 0;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if (q == 13) continue;
+    s_list[cnt][t] = lab[q];
+    cnt += jfa_skip<MODE != JFA_PLAIN && MODE != JFA_HALF>(lab, q, inc) ? 0 : 1;
+  }
+  int best = -1;
+  if (cnt > 0) {
+    const int first = s_list[0][t];
+#pragma unroll
+    for (int i = 0; i < JFA_BATCH - 1; ++i) s_list[cnt + i][t] = first;
+    float bd = 1e10f;
+    for (int k = 0; k < cnt; k += JFA_BATCH) {
+      int c[JFA_BATCH];
+      float4 sd[JFA_BATCH];
+#pragma unroll
+      for (int i = 0; i < JFA_BATCH; ++i) c[i] = s_list[k + i][t];
+#pragma unroll
+      for (int i = 0; i < JFA_BATCH; ++i)
+        sd[i] = *(const float4 *)((const char *)seeds + ((uint32_t)c[i] << 4));
+#pragma unroll
+      for (int i = 0; i < JFA_BATCH; ++i) {
+        const float pd = power_dist<UNIT_L>(cx, cy, cz, sd[i], L);
+        if (pd < bd) {
+          bd = pd;
+          best = c[i];
+        }
+      }
+    }
+  }
+  return best;
+}
+
 // One tile (4 z-rows of 64 voxels at one x, this thread's wave one row) of a
 // list pass; returns this wave's change ballot (0 for an idle row).  The
 // caller has checked that the pass is active.
@@ -463,48 +509,65 @@ __device__ __forceinline__ unsigned long long jfa_list_tile(
               lab[q] = *(const int32_t *)(row + zoff[c]);
           }
         }
-      const int inc = lab[13];
-      // Branch-free list build: every candidate is written at the current
-      // end and the end advances only for kept owners.
-      s_list[0][t] = inc;
-      int cnt = inc >= 0 ? 1 : 0;
-#pragma unroll
-      for (int q = 0; q < 27; ++q) {
-        if (q == 13) continue;
-        s_list[cnt][t] = lab[q];
-        cnt += jfa_skip<MODE != JFA_PLAIN && MODE != JFA_HALF>(lab, q, inc) ? 0 : 1;
-      }
-      int best = -1;
-      if (cnt > 0) {
-        const int first = s_list[0][t];
-#pragma unroll
-        for (int i = 0; i < JFA_BATCH - 1; ++i) s_list[cnt + i][t] = first;
-        float bd = 1e10f;
-        for (int k = 0; k < cnt; k += JFA_BATCH) {
-          int c[JFA_BATCH];
-          float4 sd[JFA_BATCH];
-#pragma unroll
-          for (int i = 0; i < JFA_BATCH; ++i) c[i] = s_list[k + i][t];
-#pragma unroll
-          for (int i = 0; i < JFA_BATCH; ++i)
-            sd[i] = *(const float4 *)((const char *)seeds + ((uint32_t)c[i] << 4));
-#pragma unroll
-          for (int i = 0; i < JFA_BATCH; ++i) {
-            const float pd = power_dist<UNIT_L>(cx, cy, cz, sd[i], L);
-            if (pd < bd) {
-              bd = pd;
-              best = c[i];
-            }
-          }
-        }
-      }
+      const int best = jfa_list_eval<UNIT_L, MODE>(lab, cx, cy, cz, seeds, L, s_list);
       out[v] = best;
-      changed = (best != inc);
+      changed = (best != lab[13]);
     }
   }
   const unsigned long long bal = __ballot(changed);
   if (chg_out && row_live && lane == 0) chg_out[((size_t)x * res + y) * nzb + zb] = bal;
   return bal;
+}
+
+// One flat tile of a list pass: 256 consecutive voxels in (y, z) order of
+// one x-plane (tile = x * tpp + block of the plane, tpp = ceil(res^2/256)),
+// one per thread.  A wave may span two rows, so the row offsets are per
+// lane; no lanes idle at the end of a row whatever res is (only at the end
+// of the plane).  Returns this wave's change ballot.
+template <bool UNIT_L, int MODE>
+__device__ __forceinline__ unsigned long long jfa_flat_tile(
+    const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel, int step,
+    const int32_t *__restrict__ in, int32_t *__restrict__ out, int tile, int tpp,
+    int (*s_list)[256]) {
+  const int t = (int)threadIdx.x;
+  const int x = tile / tpp;
+  const int r = (tile - x * tpp) * 256 + t;
+  int changed = 0;
+  if (x < res && r < res * res) {
+    const int y = r / res, z = r - y * res;
+    const size_t v = ((size_t)x * res + y) * res + z;
+    const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
+                cz = vcenter(z, voxel, half_l);
+    uint32_t yo[3], zo[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      yo[c] = (uint32_t)wrap1(y + (c - 1) * step, res) * (uint32_t)res;
+      zo[c] = (uint32_t)wrap1(z + (c - 1) * step, res);
+    }
+    int lab[27];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int32_t *plane = in + (size_t)wrap1(x + (a - 1) * step, res) * res * res;
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int q = (a * 3 + b) * 3 + c;
+          if (MODE == JFA_HALF && a == 2)
+            lab[q] = lab[q - 18];
+          else if (MODE == JFA_HALF && b == 2)
+            lab[q] = lab[q - 6];
+          else if (MODE == JFA_HALF && c == 2)
+            lab[q] = lab[q - 2];
+          else
+            lab[q] = plane[yo[b] + zo[c]];
+        }
+    }
+    const int best = jfa_list_eval<UNIT_L, MODE>(lab, cx, cy, cz, seeds, L, s_list);
+    out[v] = best;
+    changed = (best != lab[13]);
+  }
+  return __ballot(changed);
 }
 
 template <bool UNIT_L, int MODE>
@@ -1960,7 +2023,18 @@ __global__ void __launch_bounds__(256)
   const int step = batch_step(res, pass);
   const int prev = pass > 0 ? batch_step(res, pass - 1) : 0;
   unsigned long long bal;
-  if (2 * step == res)
+  if (inst[b].tile) {
+    const int tpp = (res * res + 255) / 256;
+    if (2 * step == res)
+      bal = jfa_flat_tile<UNIT_L, JFA_HALF>(seeds, res, L, half_l, voxel, step, in + v0,
+                                            out + v0, tile, tpp, s_list);
+    else if (step == prev)
+      bal = jfa_flat_tile<UNIT_L, JFA_SETTLED>(seeds, res, L, half_l, voxel, step, in + v0,
+                                               out + v0, tile, tpp, s_list);
+    else
+      bal = jfa_flat_tile<UNIT_L, JFA_PLAIN>(seeds, res, L, half_l, voxel, step, in + v0,
+                                             out + v0, tile, tpp, s_list);
+  } else if (2 * step == res)
     bal = jfa_list_tile<UNIT_L, JFA_HALF>(seeds, res, L, half_l, voxel, step, in + v0, out + v0,
                                           tile, nyb, nzb, ntiles, nullptr, nullptr, s_list);
   else if (step == prev)
@@ -2118,10 +2192,10 @@ int fos_batch_jfa_passes(fos_batch *bt, fos_ctx *ctx, float L, int early_exit,
 
 int fos_batch_majority(fos_batch *bt, const int32_t *grid, int32_t *filtered, void *stream) {
   FOS_REQUIRE(bt && grid && filtered && grid != filtered, "fos_batch_majority: bad arguments");
-  const int total = bt->tstart_h[bt->B];
+  const int total = bt->mstart_h[bt->B];
   const int per = (total + FOS_XCDS - 1) / FOS_XCDS;
   hipLaunchKernelGGL(k_majority_b, dim3(per * FOS_XCDS), dim3(256), 0, (hipStream_t)stream,
-                     bt->inst, bt->tstart, bt->B, total, per, grid, filtered);
+                     bt->inst, bt->mstart, bt->B, total, per, grid, filtered);
   FOS_CHECK_LAUNCH("k_majority_b");
   return 0;
 }

@@ -140,6 +140,8 @@ class Ensemble:
         # its own context: a captured frame holds pointers into its scratch
         self.ctx = fl.Context(n, int(self.res_of.max()), C)
         self.frame = 0
+        self.tiles = [0] * B
+        self.tile_choice = "rows"
         self.settle = np.zeros(B, np.int64)
         self.growth = np.ones(B, bool)
         self.last = {}
@@ -152,6 +154,48 @@ class Ensemble:
             except Exception:
                 pass
             self.handle = None
+
+    # -- JFA tile shapes ------------------------------------------------------------
+    def set_tiles(self, kinds):
+        """JFA pass tile per foam: 0 rows (4 z-rows of 64 voxels per
+        workgroup), 1 flat (256 consecutive voxels of a y-z plane)."""
+        self.tiles = [int(k) for k in kinds]
+        fl.call("fos_batch_set_tiles", self.handle, (ctypes.c_int * self.B)(*self.tiles))
+
+    def autotune_tiles(self, reps=3):
+        """Times this batch's JFA passes under candidate tile assignments --
+        all row tiles, all flat tiles, and per foam the kind with fewer idle
+        lanes at its grid edge -- keeps the fastest and returns the times
+        (ms per pass sequence, best of reps).  Labels are re-seeded before
+        each timed run; results never depend on the tiles."""
+        def idle_free(r):
+            row = r / (64.0 * math.ceil(r / 64.0))
+            flat = r * r / (256.0 * math.ceil(r * r / 256.0))
+            return 0 if row >= flat else 1
+        cands = {"rows": [0] * self.B, "flat": [1] * self.B,
+                 "per_edge": [idle_free(int(r)) for r in self.res_of]}
+        c, st = self.cfg, fl.stream()
+        times = {}
+        for name, kinds in cands.items():
+            self.set_tiles(kinds)
+            best = None
+            for _ in range(reps):
+                fl.call("fos_batch_jfa_init", self.handle, self.ctx.handle, fl.ptr(self.pos),
+                        fl.ptr(self.rad), float(c.L), float(c.beta), fl.ptr(self.grid), None, st)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                fl.call("fos_batch_jfa_passes", self.handle, self.ctx.handle, float(c.L),
+                        int(c.early_exit), c.th_pass_delta, c.min_passes, fl.ptr(self.grid),
+                        fl.ptr(self.temp), fl.ptr(self.info), st)
+                e1.record()
+                e1.synchronize()
+                ms = e0.elapsed_time(e1)
+                best = ms if best is None else min(best, ms)
+            times[name] = round(best, 3)
+        pick = min(times, key=times.get)
+        self.set_tiles(cands[pick])
+        self.tile_choice = pick
+        return times
 
     def foam(self, b, t):
         """Foam b's rows of a batch tensor indexed by particle."""

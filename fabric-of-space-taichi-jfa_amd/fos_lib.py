@@ -73,6 +73,7 @@ SIGNATURES = {
     "fos_batch_create": [P, I, P, P, P, P, P, P, P, P, P, F],
     "fos_batch_destroy": [P],
     "fos_batch_info": [P, P],
+    "fos_batch_set_tiles": [P, P],
     "fos_batch_jfa_init": [P, P, P, P, F, F, P, P, P],
     "fos_batch_jfa_passes": [P, P, F, I, D, I, P, P, P, P],
     "fos_batch_majority": [P, P, P, P],

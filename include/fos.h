@@ -373,6 +373,12 @@ int fos_batch_destroy(fos_batch *batch);
 /* Host out[7] = {B, total particles, total voxels, total cells, max res,
  * max passes, max G}. */
 int fos_batch_info(const fos_batch *batch, int64_t *out);
+/* JFA pass tile of each foam (host int[B]): 0 rows (a workgroup = 4 z-rows
+ * of 64 voxels; lanes past res idle), 1 flat (256 consecutive voxels of a
+ * y-z plane; row offsets per lane).  Results do not depend on it; the
+ * ensemble autotuner times both per grid edge.  Synchronises (tables are
+ * copied to the device). */
+int fos_batch_set_tiles(fos_batch *batch, const int *kinds);
 /* jfa.py:485-520 for every foam: labels -1, seed table, seeding (lowest
  * global index on ties = lowest index within the foam). */
 int fos_batch_jfa_init(fos_batch *batch, fos_ctx *ctx, const float *pos, const float *rad,

@@ -34,13 +34,17 @@ def shift(a, p0):
     return a
 
 
-@pytest.mark.parametrize("early", [False, True])
-def test_batch_jfa_matches_single_foams(gpu, early):
+@pytest.mark.parametrize("early,tiles", [(False, "rows"), (True, "rows"), (False, "flat"),
+                                         (True, "mixed")])
+def test_batch_jfa_matches_single_foams(gpu, early, tiles):
+    """Row and flat pass tiles (per foam) give the same labels."""
     from ensemble import Ensemble
     from foam import FoamSim
     cfgs, inputs = foams(SPEC, early)
-    print(f"[batch jfa] early={early}", flush=True)
+    print(f"[batch jfa] early={early} tiles={tiles}", flush=True)
     ens = Ensemble(cfgs, inputs)
+    if tiles != "rows":
+        ens.set_tiles([1] * len(cfgs) if tiles == "flat" else [b % 2 for b in range(len(cfgs))])
     ens.run_jfa()
     torch.cuda.synchronize()
     info = ens.info.cpu().numpy()
@@ -126,3 +130,20 @@ def test_batch_captured_frames_match_eager(gpu):
             ga, gb = a.snapshot(k), b.snapshot(k)
             for key in ga:
                 assert np.array_equal(ga[key], gb[key]), (f, k, key)
+
+
+def test_batch_tile_autotuner(gpu):
+    """The autotuner times its candidates, keeps the fastest, and the frame
+    after it equals a batch that kept row tiles."""
+    from ensemble import Ensemble
+    cfgs, inputs = frame_foams()
+    a, b = Ensemble(cfgs, inputs), Ensemble(cfgs, inputs)
+    times = b.autotune_tiles(reps=2)
+    assert set(times) == {"rows", "flat", "per_edge"} and b.tile_choice in times
+    assert all(t > 0 for t in times.values())
+    a.step()
+    b.step()
+    for k in range(len(cfgs)):
+        ga, gb = a.snapshot(k), b.snapshot(k)
+        for key in ga:
+            assert np.array_equal(ga[key], gb[key]), (k, key)
