@@ -174,6 +174,7 @@ def ensemble_bench(B, steps, warmup, sample=8):
     cfgs = en.c4_configs(B)
     inputs = [en.c4_inputs(c, b) for b, c in enumerate(cfgs)]
     ens = en.Ensemble(cfgs, inputs)
+    tiles = ens.autotune_tiles()
     for _ in range(warmup):
         ens.step()
     torch.cuda.synchronize()
@@ -181,7 +182,16 @@ def ensemble_bench(B, steps, warmup, sample=8):
     for _ in range(steps):
         ens.step()
     torch.cuda.synchronize()
+    el_eager = time.perf_counter() - t0
+    ens.capture()                      # the frame as a hip graph
+    ens.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens.replay(check=False)
+    torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    ens.check()                        # sticky overflow / fixed-point flags
     seq = []
     for b in np.linspace(0, B - 1, sample).astype(int):
         sim = FoamSim(cfgs[b], *inputs[b])
@@ -203,6 +213,9 @@ def ensemble_bench(B, steps, warmup, sample=8):
             "foams": B, "particles": int(ns.sum()), "voxels": int((rs ** 3).sum()),
             "frames": steps, "batch_frames_per_s": round(fps, 3),
             "foam_frames_per_s": round(B * fps, 1),
+            "eager_foam_frames_per_s": round(B * steps / el_eager, 1),
+            "frame": "hip graph replay (eager frames beside it)",
+            "jfa_tiles": {"picked": ens.tile_choice, "ms_per_pass_sequence": tiles},
             "one_at_a_time_foam_frames_per_s": round(seq_rate, 1),
             "speedup": round(B * fps / seq_rate, 2),
             "spread": {"n": [int(ns.min()), int(np.median(ns)), int(ns.max())],

@@ -1139,10 +1139,14 @@ __global__ void __launch_bounds__(256)
 }
 
 // After a round pair of the fixed-round form: the pair changed nothing ->
-// done; clears the change flag for the next pair.
-__global__ void k_fin_pair_end(int *__restrict__ changed, int *__restrict__ done) {
+// done; clears the change flag for the next pair.  After the last pair,
+// `failed` (sticky until the caller clears it) records a fixed point not
+// reached.
+__global__ void k_fin_pair_end(int *__restrict__ changed, int *__restrict__ done,
+                               int *__restrict__ failed) {
   if (!*done && !*changed) *done = 1;
   *changed = 0;
+  if (failed && !*done) *failed = 1;
 }
 
 // Writes row i's final neighbour list, hysteresis scores and counts.  The
@@ -1785,7 +1789,8 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
       hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
                          ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1,
                          (const int *)(ctx->flags + 2));
-      hipLaunchKernelGGL(k_fin_pair_end, dim3(1), dim3(1), 0, st, ctx->flags + 1, ctx->flags + 2);
+      hipLaunchKernelGGL(k_fin_pair_end, dim3(1), dim3(1), 0, st, ctx->flags + 1, ctx->flags + 2,
+                         pair == fixed_pairs - 1 ? ctx->flags + 3 : nullptr);
       FOS_CHECK_LAUNCH("k_fin_round");
     }
   } else {
@@ -2254,11 +2259,11 @@ int fos_batch_finalize_faces(fos_batch *bt, fos_ctx *ctx, const float *rad,
 int fos_batch_flags(fos_ctx *ctx, int32_t *out, void *stream) {
   FOS_REQUIRE(ctx && out, "fos_batch_flags: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
   FOS_HIP(hipStreamSynchronize(st));
   const int *h = (const int *)ctx->host_i;
   out[0] = h[0];
-  out[1] = h[2];
+  out[1] = h[3];
   return 0;
 }
 

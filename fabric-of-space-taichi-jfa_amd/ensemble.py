@@ -370,19 +370,24 @@ class Ensemble:
         torch.cuda.synchronize()
 
     def replay(self, check=True):
-        """One captured frame; check: read the face-table overflow and the
-        finalize fixed-point flags (one host round trip) and fail loudly if a
-        table overflowed or the fixed point needed more round pairs."""
+        """One captured frame (enqueued; no host round trip unless check)."""
         ran = self.cfg.jfa_runs(self.frame)
         self.graphs[ran].replay()
         self.frame += 1
-        if check and ran:
-            out = (ctypes.c_int32 * 2)()
-            fl.call("fos_batch_flags", self.ctx.handle, out, fl.stream())
-            if out[0] or not out[1]:
-                raise RuntimeError(f"captured frame {self.frame - 1}: face-row table overflow "
-                                   f"({out[0]}) or finalize not converged in {FIXED_PAIRS} "
-                                   "round pairs; step() eagerly and capture again")
+        if check:
+            self.check()
+
+    def check(self):
+        """Fails loudly if any captured frame so far overflowed a face-row
+        table or ended finalize before its fixed point (sticky device flags;
+        one host round trip)."""
+        out = (ctypes.c_int32 * 2)()
+        fl.call("fos_batch_flags", self.ctx.handle, out, fl.stream())
+        if out[0] or out[1]:
+            raise RuntimeError(f"captured frames up to {self.frame - 1}: face-row table overflow "
+                               f"({out[0]}) or finalize short of its fixed point after "
+                               f"{FIXED_PAIRS} round pairs ({out[1]}); step() eagerly and "
+                               "capture again")
 
     def snapshot(self, b):
         """Foam b's state with foam-local particle ids (foam.FoamSim.snapshot

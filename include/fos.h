@@ -405,8 +405,10 @@ int fos_batch_finalize_faces(fos_batch *batch, fos_ctx *ctx, const float *rad,
                              const int32_t *face_ids, const int32_t *face_counts,
                              float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
                              int32_t *used_count, int32_t *fsc, int fixed_pairs, void *stream);
-/* Host out[2] = {row-table overflow of the last emission, fixed point
- * reached in the last fixed-round finalize} (synchronises the stream). */
+/* Host out[2] = {a row table overflowed in an emission, a fixed-round
+ * finalize ended before its fixed point}; both stay set once set (the
+ * context's flags), so one read covers every frame since the context was
+ * made.  Synchronises the stream. */
 int fos_batch_flags(fos_ctx *ctx, int32_t *out, void *stream);
 /* grid.py:30-309 for every foam into one CSR over the batch's cells
  * (cell_count/cell_start [total cells], cell_indices [total particles],
