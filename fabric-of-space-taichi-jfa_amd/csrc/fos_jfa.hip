@@ -405,25 +405,27 @@ __device__ __forceinline__ unsigned long long incr_wave_mask(
 // The owner of one voxel from its 27 candidate labels (lab[13] = current
 // owner): the kept candidates go to this thread's LDS list column, then the
 // list is walked JFA_BATCH entries at a time in list order with strict '<'.
-template <bool UNIT_L, int MODE>
+// LT: the LDS list entry type -- int, or unsigned short when every owner id
+// minus `base` is below 65536 (half the LDS, more resident workgroups).
+template <bool UNIT_L, int MODE, typename LT = int>
 __device__ __forceinline__ int jfa_list_eval(const int lab[27], float cx, float cy, float cz,
                                              const float4 *__restrict__ seeds, float L,
-                                             int (*s_list)[256]) {
+                                             LT (*s_list)[256], int base = 0) {
   const int t = (int)threadIdx.x;
   const int inc = lab[13];
   // Branch-free list build: every candidate is written at the current
   // end and the end advances only for kept owners.
-  s_list[0][t] = inc;
+  s_list[0][t] = (LT)(inc - base);
   int cnt = inc >= 0 ? 1 : 0;
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
     if (q == 13) continue;
-    s_list[cnt][t] = lab[q];
+    s_list[cnt][t] = (LT)(lab[q] - base);
     cnt += jfa_skip<MODE != JFA_PLAIN && MODE != JFA_HALF>(lab, q, inc) ? 0 : 1;
   }
   int best = -1;
   if (cnt > 0) {
-    const int first = s_list[0][t];
+    const LT first = s_list[0][t];
 #pragma unroll
     for (int i = 0; i < JFA_BATCH - 1; ++i) s_list[cnt + i][t] = first;
     float bd = 1e10f;
@@ -431,7 +433,7 @@ __device__ __forceinline__ int jfa_list_eval(const int lab[27], float cx, float 
       int c[JFA_BATCH];
       float4 sd[JFA_BATCH];
 #pragma unroll
-      for (int i = 0; i < JFA_BATCH; ++i) c[i] = s_list[k + i][t];
+      for (int i = 0; i < JFA_BATCH; ++i) c[i] = (int)s_list[k + i][t] + base;
 #pragma unroll
       for (int i = 0; i < JFA_BATCH; ++i)
         sd[i] = *(const float4 *)((const char *)seeds + ((uint32_t)c[i] << 4));
@@ -451,12 +453,12 @@ __device__ __forceinline__ int jfa_list_eval(const int lab[27], float cx, float 
 // One tile (4 z-rows of 64 voxels at one x, this thread's wave one row) of a
 // list pass; returns this wave's change ballot (0 for an idle row).  The
 // caller has checked that the pass is active.
-template <bool UNIT_L, int MODE>
+template <bool UNIT_L, int MODE, typename LT = int>
 __device__ __forceinline__ unsigned long long jfa_list_tile(
     const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel, int step,
     const int32_t *__restrict__ in, int32_t *__restrict__ out, int tile, int nyb, int nzb,
     int ntiles, const unsigned long long *__restrict__ chg_prev,
-    unsigned long long *__restrict__ chg_out, int (*s_list)[256]) {
+    unsigned long long *__restrict__ chg_out, LT (*s_list)[256], int base = 0) {
   const int t = (int)threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   int changed = 0;
@@ -509,7 +511,7 @@ __device__ __forceinline__ unsigned long long jfa_list_tile(
               lab[q] = *(const int32_t *)(row + zoff[c]);
           }
         }
-      const int best = jfa_list_eval<UNIT_L, MODE>(lab, cx, cy, cz, seeds, L, s_list);
+      const int best = jfa_list_eval<UNIT_L, MODE, LT>(lab, cx, cy, cz, seeds, L, s_list, base);
       out[v] = best;
       changed = (best != lab[13]);
     }
@@ -524,11 +526,11 @@ __device__ __forceinline__ unsigned long long jfa_list_tile(
 // one per thread.  A wave may span two rows, so the row offsets are per
 // lane; no lanes idle at the end of a row whatever res is (only at the end
 // of the plane).  Returns this wave's change ballot.
-template <bool UNIT_L, int MODE>
+template <bool UNIT_L, int MODE, typename LT = int>
 __device__ __forceinline__ unsigned long long jfa_flat_tile(
     const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel, int step,
     const int32_t *__restrict__ in, int32_t *__restrict__ out, int tile, int tpp,
-    int (*s_list)[256]) {
+    LT (*s_list)[256], int base = 0) {
   const int t = (int)threadIdx.x;
   const int x = tile / tpp;
   const int r = (tile - x * tpp) * 256 + t;
@@ -563,14 +565,14 @@ __device__ __forceinline__ unsigned long long jfa_flat_tile(
             lab[q] = plane[yo[b] + zo[c]];
         }
     }
-    const int best = jfa_list_eval<UNIT_L, MODE>(lab, cx, cy, cz, seeds, L, s_list);
+    const int best = jfa_list_eval<UNIT_L, MODE, LT>(lab, cx, cy, cz, seeds, L, s_list, base);
     out[v] = best;
     changed = (best != lab[13]);
   }
   return __ballot(changed);
 }
 
-template <bool UNIT_L, int MODE>
+template <bool UNIT_L, int MODE, typename LT>
 __global__ void __launch_bounds__(256)
     k_jfa_pass_list(const float4 *__restrict__ seeds, int res, float L, float half_l,
                     float voxel, int step, const int32_t *__restrict__ in,
@@ -578,10 +580,10 @@ __global__ void __launch_bounds__(256)
                     double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd,
                     const unsigned long long *__restrict__ chg_prev,
                     unsigned long long *__restrict__ chg_out) {
-  __shared__ int s_list[JFA_LIST][256];
+  __shared__ LT s_list[JFA_LIST][256];
   if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
   const int tile = xcd_tile(blockIdx.x, per_xcd);
-  const unsigned long long bal = jfa_list_tile<UNIT_L, MODE>(
+  const unsigned long long bal = jfa_list_tile<UNIT_L, MODE, LT>(
       seeds, res, L, half_l, voxel, step, in, out, tile, nyb, nzb, ntiles, chg_prev, chg_out,
       s_list);
   if ((threadIdx.x & 63) == 0 && bal) {
@@ -1451,9 +1453,17 @@ static void launch_pass_list(fos_ctx *ctx, int res, float L, float half_l, float
   const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
   const int ntiles = res * nyb * nzb;
   const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
-  hipLaunchKernelGGL((k_jfa_pass_list<U, MODE>), dim3(per * FOS_XCDS), dim3(256), 0, st,
-                     ctx->seeds, res, L, half_l, voxel, step, in, out, ctx->stripes, pass,
-                     early, th, minp, nyb, nzb, ntiles, per, chg_prev, chg_out);
+  // owner ids are below the context's particle capacity: 16-bit list
+  // entries when that is at most 65536
+  if (ctx->max_n <= 65536)
+    hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, unsigned short>), dim3(per * FOS_XCDS),
+                       dim3(256), 0, st, ctx->seeds, res, L, half_l, voxel, step, in, out,
+                       ctx->stripes, pass, early, th, minp, nyb, nzb, ntiles, per, chg_prev,
+                       chg_out);
+  else
+    hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, int>), dim3(per * FOS_XCDS), dim3(256), 0, st,
+                       ctx->seeds, res, L, half_l, voxel, step, in, out, ctx->stripes, pass,
+                       early, th, minp, nyb, nzb, ntiles, per, chg_prev, chg_out);
 }
 
 // Launches pass `pass` (its changed-voxel count accumulates in stripe slot
@@ -2006,14 +2016,16 @@ __global__ void k_seed_b(const float *__restrict__ pos, const float4 *__restrict
 
 // Pass `pass` of every foam that still runs it (scheduled and not exited
 // early); each foam's counters are its own stripe block.
-template <bool UNIT_L>
+// LT unsigned short when every foam has at most 65536 particles: list
+// entries hold ids minus the foam's first id.
+template <bool UNIT_L, typename LT>
 __global__ void __launch_bounds__(256)
     k_jfa_pass_list_b(const float4 *__restrict__ seeds, const fos_inst *__restrict__ inst,
                       const int *__restrict__ tstart, int B, int total, int per_xcd, float L,
                       float half_l, int pass, const int32_t *__restrict__ in,
                       int32_t *__restrict__ out, int *__restrict__ stripes, int early,
                       double th, int min_passes) {
-  __shared__ int s_list[JFA_LIST][256];
+  __shared__ LT s_list[JFA_LIST][256];
   const int T = xcd_tile(blockIdx.x, per_xcd);
   if (T >= total) return;
   const int b = batch_find(tstart, B, T);
@@ -2027,29 +2039,32 @@ __global__ void __launch_bounds__(256)
   const int tile = T - tstart[b];
   const int step = batch_step(res, pass);
   const int prev = pass > 0 ? batch_step(res, pass - 1) : 0;
+  const int base = inst[b].p0;
   unsigned long long bal;
   if (inst[b].tile) {
     const int tpp = (res * res + 255) / 256;
     if (2 * step == res)
-      bal = jfa_flat_tile<UNIT_L, JFA_HALF>(seeds, res, L, half_l, voxel, step, in + v0,
-                                            out + v0, tile, tpp, s_list);
+      bal = jfa_flat_tile<UNIT_L, JFA_HALF, LT>(seeds, res, L, half_l, voxel, step, in + v0,
+                                                out + v0, tile, tpp, s_list, base);
     else if (step == prev)
-      bal = jfa_flat_tile<UNIT_L, JFA_SETTLED>(seeds, res, L, half_l, voxel, step, in + v0,
-                                               out + v0, tile, tpp, s_list);
+      bal = jfa_flat_tile<UNIT_L, JFA_SETTLED, LT>(seeds, res, L, half_l, voxel, step, in + v0,
+                                                   out + v0, tile, tpp, s_list, base);
     else
-      bal = jfa_flat_tile<UNIT_L, JFA_PLAIN>(seeds, res, L, half_l, voxel, step, in + v0,
-                                             out + v0, tile, tpp, s_list);
-  } else if (2 * step == res)
-    bal = jfa_list_tile<UNIT_L, JFA_HALF>(seeds, res, L, half_l, voxel, step, in + v0, out + v0,
-                                          tile, nyb, nzb, ntiles, nullptr, nullptr, s_list);
-  else if (step == prev)
-    bal = jfa_list_tile<UNIT_L, JFA_SETTLED>(seeds, res, L, half_l, voxel, step, in + v0,
-                                             out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
-                                             s_list);
-  else
-    bal = jfa_list_tile<UNIT_L, JFA_PLAIN>(seeds, res, L, half_l, voxel, step, in + v0,
-                                           out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
-                                           s_list);
+      bal = jfa_flat_tile<UNIT_L, JFA_PLAIN, LT>(seeds, res, L, half_l, voxel, step, in + v0,
+                                                 out + v0, tile, tpp, s_list, base);
+  } else if (2 * step == res) {
+    bal = jfa_list_tile<UNIT_L, JFA_HALF, LT>(seeds, res, L, half_l, voxel, step, in + v0,
+                                              out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
+                                              s_list, base);
+  } else if (step == prev) {
+    bal = jfa_list_tile<UNIT_L, JFA_SETTLED, LT>(seeds, res, L, half_l, voxel, step, in + v0,
+                                                 out + v0, tile, nyb, nzb, ntiles, nullptr,
+                                                 nullptr, s_list, base);
+  } else {
+    bal = jfa_list_tile<UNIT_L, JFA_PLAIN, LT>(seeds, res, L, half_l, voxel, step, in + v0,
+                                               out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
+                                               s_list, base);
+  }
   if ((threadIdx.x & 63) == 0 && bal) {
     const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     atomicAdd(st + (pass * FOS_STRIPES + wave % FOS_STRIPES) * FOS_STRIPE_INTS, __popcll(bal));
@@ -2173,17 +2188,21 @@ int fos_batch_jfa_passes(fos_batch *bt, fos_ctx *ctx, float L, int early_exit,
   const int total = bt->tstart_h[B];
   const int per = (total + FOS_XCDS - 1) / FOS_XCDS;
   const float half_l = (float)(0.5 * (double)L);
+  bool small = true;  // every foam's ids fit 16 bits above its first id
+  for (int b = 0; b < B; ++b) small = small && bt->inst_h[b].n <= 65536;
   for (int p = 0; p < bt->max_passes; ++p) {
     const int32_t *in = (p & 1) ? temp : grid;
     int32_t *out = (p & 1) ? grid : temp;
-    if (unit_l(L))
-      hipLaunchKernelGGL(k_jfa_pass_list_b<true>, dim3(per * FOS_XCDS), dim3(256), 0, st,
-                         ctx->seeds, bt->inst, bt->tstart, B, total, per, L, half_l, p, in, out,
-                         bt->stripes, early_exit, th_pass_delta, min_passes);
-    else
-      hipLaunchKernelGGL(k_jfa_pass_list_b<false>, dim3(per * FOS_XCDS), dim3(256), 0, st,
-                         ctx->seeds, bt->inst, bt->tstart, B, total, per, L, half_l, p, in, out,
-                         bt->stripes, early_exit, th_pass_delta, min_passes);
+#define FOS_PASS_B(U, T)                                                                   \
+  hipLaunchKernelGGL((k_jfa_pass_list_b<U, T>), dim3(per * FOS_XCDS), dim3(256), 0, st,       \
+                     ctx->seeds, bt->inst, bt->tstart, B, total, per, L, half_l, p, in, out,   \
+                     bt->stripes, early_exit, th_pass_delta, min_passes)
+    if (unit_l(L)) {
+      if (small) FOS_PASS_B(true, unsigned short); else FOS_PASS_B(true, int);
+    } else {
+      if (small) FOS_PASS_B(false, unsigned short); else FOS_PASS_B(false, int);
+    }
+#undef FOS_PASS_B
     FOS_CHECK_LAUNCH("k_jfa_pass_list_b");
   }
   hipLaunchKernelGGL(k_jfa_finish_info_b, dim3(B), dim3(64), 0, st, bt->inst, early_exit,
