@@ -15,7 +15,8 @@
 
 using namespace fos;
 
-#define PRED_FEAT 16  // feature slots per pair: 12 used, 4 zero (MFMA K = 16)
+#define PRED_FEAT 16   // feature slots per pair: PRED_NFEAT used, the rest zero (MFMA K = 16)
+#define PRED_NFEAT 13
 #define MLP_H 64
 // flat FP32 parameter layout: W1[16][64] b1[64] W2[64][64] b2[64] w3[64] b3
 #define MLP_OFF_W1 0
@@ -266,6 +267,9 @@ __global__ void k_pred_pair_map(const int32_t *__restrict__ nup, int n, int32_t 
 //   3 t                       9 |cand(j)| / 40
 //   4 estimated score (<= 4) 10 (ri - rj) / (ri + rj)
 //   5 face area / (ri + rj)^2 (<= 4)   11 p(m) / (ri + rj)^2
+//  12 1 when j was among i's accepted partners j > i of the previous frame
+//     (prev_up rows; the JFA's hysteresis keeps faces accepted before,
+//     jfa.py:400-443), else 0
 // Margins and p(m) are clamped to [-1, 1].  The candidate relation is
 // symmetric, so each unordered pair is scored once, in the row of its lower
 // index: the pairs are the flat list of k_pred_pair_map, one thread each,
@@ -279,7 +283,7 @@ __global__ void __launch_bounds__(256)
                 const int32_t *__restrict__ cand, const int32_t *__restrict__ ncand,
                 const float4 *__restrict__ rel, const int32_t *__restrict__ off,
                 const int32_t *__restrict__ pair_i, int cap, int64_t max_pairs,
-                float *__restrict__ feat) {
+                const int32_t *__restrict__ prev_up, float *__restrict__ feat) {
   const int64_t P = min((int64_t)off[n], max_pairs);
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
        p += (int64_t)gridDim.x * blockDim.x) {
@@ -405,7 +409,14 @@ __global__ void __launch_bounds__(256)
     f[9] = (float)ncand[j] * (1.0f / 40.0f);
     f[10] = (ri - rj) / sr;
     f[11] = fminf(1.0f, fmaxf(-1.0f, pm / sr2));
-    f[12] = f[13] = f[14] = f[15] = 0.0f;
+    float was = 0.0f;
+    if (prev_up) {
+      const int32_t *pr = prev_up + (int64_t)i * FOS_MAX_NEIGHBORS;
+#pragma unroll 8
+      for (int q = 0; q < FOS_MAX_NEIGHBORS; ++q) was = pr[q] == j ? 1.0f : was;
+    }
+    f[12] = was;
+    f[13] = f[14] = f[15] = 0.0f;
     float4 *o = (float4 *)(feat + p * PRED_FEAT);
 #pragma unroll
     for (int q = 0; q < PRED_FEAT / 4; ++q)
@@ -529,6 +540,21 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Row i of prev_up: the partners j > i of i's neighbour list, in list order,
+// -1 padded (the accepted pairs of a JFA frame, each in its lower row).
+__global__ void k_pred_prev_rows(const int32_t *__restrict__ nl, const int32_t *__restrict__ nc,
+                                 int n, int32_t *__restrict__ prev_up) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int m = min(nc[i], FOS_MAX_NEIGHBORS);
+  int r = 0;
+  for (int q = 0; q < m; ++q) {
+    const int j = nl[(int64_t)i * FOS_MAX_NEIGHBORS + q];
+    if (j > i) prev_up[(int64_t)i * FOS_MAX_NEIGHBORS + r++] = j;
+  }
+  for (; r < FOS_MAX_NEIGHBORS; ++r) prev_up[(int64_t)i * FOS_MAX_NEIGHBORS + r] = -1;
+}
+
 __global__ void k_fill_i32(int32_t *__restrict__ a, int n, int32_t v) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) a[q] = v;
@@ -568,15 +594,15 @@ __global__ void k_mlp_fwd(const float *__restrict__ feat, const float *__restric
   const bool live = b0 < B;
   const int b = live ? b0 : B - 1;
   const int64_t p = idx[b];
-  float x[12];
+  float x[PRED_NFEAT];
 #pragma unroll
-  for (int f = 0; f < 12; ++f) x[f] = feat[p * PRED_FEAT + f];
+  for (int f = 0; f < PRED_NFEAT; ++f) x[f] = feat[p * PRED_FEAT + f];
   float h1[MLP_H];
 #pragma unroll
   for (int a = 0; a < MLP_H; ++a) {
     float z = prm[MLP_OFF_B1 + a];
 #pragma unroll
-    for (int f = 0; f < 12; ++f) z += x[f] * prm[MLP_OFF_W1 + f * MLP_H + a];
+    for (int f = 0; f < PRED_NFEAT; ++f) z += x[f] * prm[MLP_OFF_W1 + f * MLP_H + a];
     h1[a] = fmaxf(z, 0.0f);
     if (live) H1[(int64_t)b * MLP_H + a] = h1[a];
   }
@@ -634,7 +660,7 @@ __global__ void k_mlp_grad(const float *__restrict__ feat, const int32_t *__rest
   float s = 0.0f;
   if (q < MLP_OFF_B1) {  // W1[f][a]
     const int f = q / MLP_H, a = q % MLP_H;
-    if (f < 12)
+    if (f < PRED_NFEAT)
       for (int b = b0; b < b1; ++b)
         s += feat[(int64_t)idx[b] * PRED_FEAT + f] * dZ1[(int64_t)b * MLP_H + a];
   } else if (q < MLP_OFF_W2) {
@@ -723,16 +749,27 @@ static unsigned pair_blocks(int64_t max_pairs) {
 
 int fos_pred_features(const float *rad, int n, float r_mean, float clip_r, const int32_t *cand,
                       const int32_t *ncand, const float *rel, const int32_t *pair_off,
-                      const int32_t *pair_i, int cap, int64_t max_pairs, float *feat,
-                      void *stream) {
+                      const int32_t *pair_i, int cap, int64_t max_pairs, const int32_t *prev_up,
+                      float *feat, void *stream) {
   FOS_REQUIRE(n >= 0 && cap >= 1 && r_mean > 0.0f && clip_r > 0.0f && max_pairs >= 0 &&
                   (n == 0 || (rad && cand && ncand && rel && pair_off && pair_i && feat)),
               "fos_pred_features: bad arguments (n=%d cap=%d)", n, cap);
   if (n == 0 || max_pairs == 0) return 0;
   hipLaunchKernelGGL(k_pred_feat, dim3(pair_blocks(max_pairs)), dim3(256), 0,
                      (hipStream_t)stream, rad, n, 1.0f / r_mean, clip_r, cand, ncand,
-                     (const float4 *)rel, pair_off, pair_i, cap, max_pairs, feat);
+                     (const float4 *)rel, pair_off, pair_i, cap, max_pairs, prev_up, feat);
   FOS_CHECK_LAUNCH("k_pred_feat");
+  return 0;
+}
+
+int fos_pred_prev_rows(const int32_t *neighbor_list, const int32_t *neighbor_count, int n,
+                       int32_t *prev_up, void *stream) {
+  FOS_REQUIRE(n >= 0 && (n == 0 || (neighbor_list && neighbor_count && prev_up)),
+              "fos_pred_prev_rows: bad arguments (n=%d)", n);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_pred_prev_rows, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, neighbor_list, neighbor_count, n, prev_up);
+  FOS_CHECK_LAUNCH("k_pred_prev_rows");
   return 0;
 }
 

@@ -8,7 +8,10 @@ particle and the face rows the pressure step reads.  The plain JFA produces
 the training labels (its neighbour lists) and stays the judge.
 
 Parameters are one flat fp32 vector (W1[16][64] b1[64] W2[64][64] b2[64]
-w3[64] b3; rows 12-15 of W1 multiply the zero feature slots).  Inference
+w3[64] b3; rows 13-15 of W1 multiply the zero feature slots).  Feature 12 is
+the hysteresis the JFA's finalize applies: whether the pair was accepted in
+the previous frame (``prev_up``: the JFA's neighbour lists after a JFA frame,
+the predicted face rows after a predicted one).  Inference
 packs W1 and W2 to BF16 in MFMA lane order (``pack``).
 """
 import ctypes
@@ -20,7 +23,7 @@ import torch
 import fos_lib as fl
 
 H = 64
-NFEAT = 12
+NFEAT = 13
 SLOTS = 16
 OFF_W1, OFF_B1 = 0, 16 * H
 OFF_W2 = OFF_B1 + H
@@ -130,6 +133,8 @@ class FacePredictor:
         self.pair_off = torch.empty(self.n_max + 1, dtype=i32, device=d)
         self.pair_i = torch.empty(self.n_max * self.cap, dtype=i32, device=d)
         self.feat = torch.empty((self.n_max * self.cap, SLOTS), dtype=f32, device=d)
+        # previous frame's accepted partners j > i per row (-1: none yet)
+        self.prev_up = torch.full((self.n_max, 32), -1, dtype=i32, device=d)
         self.logit = torch.empty(self.n_max * self.cap, dtype=f32, device=d)
         self.set_params(params)
 
@@ -158,7 +163,7 @@ class FacePredictor:
             r_mean = float(rad[:n].mean().item())
         fl.call("fos_pred_features", fl.ptr(rad), n, float(r_mean), CLIP, fl.ptr(self.cand),
                 fl.ptr(self.ncand), fl.ptr(self.rel), fl.ptr(self.pair_off), fl.ptr(self.pair_i),
-                self.cap, n * self.cap, fl.ptr(self.feat), fl.stream())
+                self.cap, n * self.cap, fl.ptr(self.prev_up), fl.ptr(self.feat), fl.stream())
 
     def logits(self, n):
         pk = self.packed
@@ -171,11 +176,20 @@ class FacePredictor:
                 self.cap, fl.ptr(fsc), fl.ptr(face_ids), fl.ptr(face_counts), fl.stream())
 
     def predict(self, pos, rad, n, fsc, face_ids=None, face_counts=None, r_mean=None):
-        """Whole pipeline for one frame: fsc[n] (and face rows when given)."""
+        """Whole pipeline for one frame: fsc[n] (and face rows when given;
+        they become the next frame's previous accepted pairs)."""
         self.candidates(pos, rad, n)
         self.features(rad, n, r_mean)
         self.logits(n)
         self.fsc(n, fsc, face_ids, face_counts)
+        if face_ids is not None:
+            self.prev_up[:n].copy_(face_ids[:n])
+
+    def remember_jfa(self, neighbor_list, neighbor_count, n):
+        """After a JFA frame: its accepted pairs are the next frame's
+        previous ones."""
+        fl.call("fos_pred_prev_rows", fl.ptr(neighbor_list), fl.ptr(neighbor_count), n,
+                fl.ptr(self.prev_up), fl.stream())
 
     def labels(self, n, neighbor_list, neighbor_count, out):
         fl.call("fos_pred_labels", fl.ptr(self.cand), fl.ptr(self.pair_off), fl.ptr(self.pair_i),

@@ -186,6 +186,7 @@ class FoamSim:
                 self.run_jfa()
                 self.predictor.predict(self.pos, self.rad, self.n, self.pred_fsc,
                                        r_mean=self.r_scale)
+                self.predictor.remember_jfa(self.neighbor_list, self.neighbor_count, self.n)
                 from face_predictor import fsc_error
                 err, bias = fsc_error(self.pred_fsc, self.fsc)
                 # the judge compares the frame's mean FSC (bias); the

@@ -93,7 +93,8 @@ SIGNATURES = {
     "fos_batch_count_zero": [P, P, P, P],
     "fos_pred_candidates": [P, P, I, I, F, F, I, P, P, P, P, P, P, P, P, P],
     "fos_pred_pairs": [P, P, I, P, P, P],
-    "fos_pred_features": [P, I, F, F, P, P, P, P, P, I, I64, P, P],
+    "fos_pred_features": [P, I, F, F, P, P, P, P, P, I, I64, P, P, P],
+    "fos_pred_prev_rows": [P, P, I, P, P],
     "fos_pred_mlp": [P, I64, P, P, P, P, P, F, P, P, P],
     "fos_pred_fsc": [P, P, P, I, I, P, P, P, P],
     "fos_pred_labels": [P, P, P, I, I, I64, P, P, P, P],

@@ -3,8 +3,8 @@
     python train_face_mlp.py --config c5 --out face_mlp_c5.npz
 
 Runs the plain foam (JFA every frame) of the given workload, and on every
-frame after a warm-up records, for each hash-grid candidate pair, the 12
-features (fos_pred_features) and the label "j is in i's JFA neighbour list"
+frame after a warm-up records, for each hash-grid candidate pair, the 13
+features (fos_pred_features; feature 12 from the previous frame's JFA) and the label "j is in i's JFA neighbour list"
 (fos_pred_labels), one per scored pair (each unordered pair once).  The
 last --holdout frames are kept for validation.  The
 MLP is trained with the hand-written FP32 kernels (fos_mlp_train_step, Adam)
@@ -45,10 +45,12 @@ def collect(cfg, warm, frames, seed):
 
     def on_jfa(s):
         if s.frame < warm:
+            pred.remember_jfa(s.neighbor_list, s.neighbor_count, n)
             return
         pred.candidates(s.pos, s.rad, n)
-        pred.features(s.rad, n, r_scale)
+        pred.features(s.rad, n, r_scale)        # previous frame's accepted pairs
         pred.labels(n, s.neighbor_list, s.neighbor_count, lab)
+        pred.remember_jfa(s.neighbor_list, s.neighbor_count, n)
         P = pred.npairs(n)
         found = (lab[:P] > 0.5).sum().item()
         data.append({"feat": pred.feat[:P].clone(), "y": lab[:P].clone(),

@@ -318,14 +318,20 @@ int fos_pred_candidates(const float *pos, const float *rad, int n, int G, float 
  * pair_i[p] the particle of pair p (capacity sum of nup <= n*cap). */
 int fos_pred_pairs(fos_ctx *ctx, const int32_t *nup, int n, int32_t *pair_off, int32_t *pair_i,
                    void *stream);
-/* 12 features per pair p < min(pair_off[n], max_pairs) into
- * feat[max_pairs][16] (entries 12-15 zero) from the rel rows of
+/* 13 features per pair p < min(pair_off[n], max_pairs) into
+ * feat[max_pairs][16] (entries 13-15 zero) from the rel rows of
  * fos_pred_candidates; r_mean normalises the radii, clip_r*(ri+rj) is the
- * half-size of the square clipped to the face polygon. */
+ * half-size of the square clipped to the face polygon; feature 12 is 1 when
+ * the partner is in row i of prev_up[n][32] (the previous frame's accepted
+ * partners j > i; may be NULL: 0). */
 int fos_pred_features(const float *rad, int n, float r_mean, float clip_r, const int32_t *cand,
                       const int32_t *ncand, const float *rel, const int32_t *pair_off,
-                      const int32_t *pair_i, int cap, int64_t max_pairs, float *feat,
-                      void *stream);
+                      const int32_t *pair_i, int cap, int64_t max_pairs, const int32_t *prev_up,
+                      float *feat, void *stream);
+/* prev_up[n][32] from a JFA frame: row i = the partners j > i of i's
+ * neighbour list (jfa.py:789-857 output), in list order, -1 padded. */
+int fos_pred_prev_rows(const int32_t *neighbor_list, const int32_t *neighbor_count, int n,
+                       int32_t *prev_up, void *stream);
 /* MLP 12->64->64->1 on MFMA with BF16 weights/activations and FP32
  * accumulation: logit[p] for p < npairs, or p < min(npairs, *npairs_dev)
  * when npairs_dev (device) is given.  w1_packed / w2_packed hold the BF16

@@ -217,15 +217,38 @@ def test_features_match_numpy(gpu):
         assert close.mean() > 0.995, (f, close.mean())
     # witness counts can differ only where a margin is within rounding of 0
     assert (np.abs(got[:, 6] - want[:, 6]) > 1e-6).mean() < 1e-3
-    assert (got[:, 12:] == 0).all()
+    assert (got[:, 12:] == 0).all()                      # no previous frame yet
     assert (want[:, 5] > 0).mean() > 0.3                 # most candidate pairs keep a face
+    # feature 12: the pair was among the previous frame's accepted pairs.
+    # Previous rows from a neighbour list holding every other candidate j > i
+    # (and all j < i, which the rows must drop).
+    nl = np.full((n, 32), -1, np.int32)
+    ncount = np.zeros(n, np.int32)
+    was = np.zeros((n, fp.CAP), bool)
+    for i in range(n):
+        lst = [int(j) for s_, j in enumerate(cand[i, :nu[i]]) if s_ % 2 == 0][:20]
+        lst += [int(j) for j in cand[i, nu[i]:nc[i]]][:32 - len(lst)]
+        nl[i, :len(lst)] = lst
+        ncount[i] = len(lst)
+        up = [j for j in lst if j > i]
+        for s_ in range(nu[i]):
+            was[i, s_] = int(cand[i, s_]) in up
+    pr.remember_jfa(dev(nl), dev(ncount), n)
+    pv = host(pr.prev_up)
+    for i in range(n):
+        up = [j for j in nl[i, :ncount[i]] if j > i]
+        assert list(pv[i, :len(up)]) == up and (pv[i, len(up):] == -1).all(), i
+    pr.features(tr, n, r_mean)
+    got2 = host(pr.feat).reshape(-1, 16)[:P]
+    assert np.array_equal(got2[:, 12], was[valid].astype(np.float32))
+    assert np.array_equal(got2[:, :12], got[:, :12]) and got2[:, 12].sum() > 0
 
 
 def test_mlp_mfma_matches_numpy(gpu):
     rng = np.random.default_rng(5)
     P = 4096
     feat = np.zeros((P, 16), F)
-    feat[:, :12] = rng.normal(0, 1, (P, 12))
+    feat[:, :fp.NFEAT] = rng.normal(0, 1, (P, fp.NFEAT))
     params = fp.init_params(seed=7)
     params[fp.OFF_B1:fp.OFF_W2] = rng.normal(0, 0.1, 64)
     params[fp.OFF_B2:fp.OFF_W3] = rng.normal(0, 0.1, 64)
@@ -342,8 +365,8 @@ def test_training_gradients_and_loss(gpu):
     rng = np.random.default_rng(9)
     P = 20000
     feat = np.zeros((P, 16), F)
-    feat[:, :12] = rng.normal(0, 1, (P, 12))
-    y = ((feat[:, 0] + 0.5 * feat[:, 3] - 0.3 * feat[:, 7]) > 0.2).astype(F)
+    feat[:, :fp.NFEAT] = rng.normal(0, 1, (P, fp.NFEAT))
+    y = ((feat[:, 0] + 0.5 * feat[:, 3] - 0.3 * feat[:, 7] + 0.8 * feat[:, 12]) > 0.2).astype(F)
     params = fp.init_params(seed=1)
     tr = fp.Trainer(params, batch=1024, lr=3e-3, pos_weight=1.5)
     tf, ty = dev(feat), dev(y)
