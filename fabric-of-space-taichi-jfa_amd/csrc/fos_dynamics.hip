@@ -214,13 +214,15 @@ __device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
 }
 
 // dynamics.py:1281-1320
+// gate (may be NULL): run only when *gate > 0.5 (the warm-start gate decided
+// on the device, fos_fsc_gate).
 __global__ void k_fsc_targets(const float *__restrict__ rad, float *__restrict__ rt,
                               const int32_t *__restrict__ fsc,
                               const float *__restrict__ ema, int n, float fl,
                               float fh, float gp, float r_min, float r_max,
-                              float db) {
+                              float db, const float *__restrict__ gate) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (gate && !(*gate > 0.5f))) return;
   const float r0 = rad[i];
   const float f = ema[i] > 0.0f ? ema[i] : (float)fsc[i];
   const float gap = 1.0f, idle = 0.003f;
@@ -264,6 +266,20 @@ __global__ void k_controller_stats(const float *__restrict__ rad,
   atomicAdd(sum, (double)dr);
   const float a = fabsf(dr);
   if (a > 0.0f) atomicMax(maxbits, __float_as_int(a));
+}
+
+// run.py:1184-1195 warm-start gate on the device: *gate = 1 when the frame is
+// past the warm start and at most zero_rate of the particles have FSC 0.
+__global__ void k_count_zero(const int32_t *__restrict__ fsc, int n, int *__restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = __popcll(__ballot(i < n && fsc[i] == 0));
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+__global__ void k_gate(const int *__restrict__ count, int n, int frame, int warmstart,
+                       double zero_rate, float *__restrict__ gate) {
+  const double f0 = (double)*count / (double)(n > 1 ? n : 1);
+  *gate = (frame < warmstart || f0 > zero_rate) ? 0.0f : 1.0f;
 }
 
 // dynamics.py:1399-1427
@@ -657,8 +673,40 @@ int fos_fsc_targets(const float *rad, float *rad_target, const int32_t *fsc,
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_fsc_targets, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
                      (hipStream_t)stream, rad, rad_target, fsc, fsc_ema, n,
-                     (float)f_low, (float)f_high, growth_pct, r_min, r_max, deadband);
+                     (float)f_low, (float)f_high, growth_pct, r_min, r_max, deadband,
+                     (const float *)nullptr);
   FOS_CHECK_LAUNCH("k_fsc_targets");
+  return 0;
+}
+
+int fos_fsc_targets_gated(const float *rad, float *rad_target, const int32_t *fsc,
+                          const float *fsc_ema, int n, int f_low, int f_high, float growth_pct,
+                          float r_min, float r_max, float deadband, const float *gate,
+                          void *stream) {
+  FOS_REQUIRE(n >= 0 && gate && (n == 0 || (rad && rad_target && fsc && fsc_ema)),
+              "fos_fsc_targets_gated: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_fsc_targets, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, rad, rad_target, fsc, fsc_ema, n,
+                     (float)f_low, (float)f_high, growth_pct, r_min, r_max, deadband, gate);
+  FOS_CHECK_LAUNCH("k_fsc_targets");
+  return 0;
+}
+
+int fos_fsc_gate(fos_ctx *ctx, const int32_t *fsc, int n, int frame, int warmstart_frames,
+                 double fsc_zero_rate, float *gate, void *stream) {
+  FOS_REQUIRE(ctx && gate && n >= 0 && (n == 0 || fsc), "fos_fsc_gate: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = fos_fill_i32(ctx->iscratch, 1, 0, stream);
+  if (rc) return rc;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_count_zero, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, fsc, n,
+                       ctx->iscratch);
+    FOS_CHECK_LAUNCH("k_count_zero");
+  }
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(1), 0, st, (const int *)ctx->iscratch, n, frame,
+                     warmstart_frames, fsc_zero_rate, gate);
+  FOS_CHECK_LAUNCH("k_gate");
   return 0;
 }
 

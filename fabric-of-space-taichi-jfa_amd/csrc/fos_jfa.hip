@@ -1517,6 +1517,47 @@ static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
   return 0;
 }
 
+// Face-pair emission into the row tables (the kernel chosen by
+// face_emit_tx); the overflow flag (ctx->flags[0]) is left for the caller.
+static int face_emit(fos_ctx *ctx, int res, int n, const int32_t *filtered, hipStream_t st) {
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64, ntiles = res * nyb * nzb;
+  const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
+  const int tx = g_emit_tx;
+  const int ntiles_x = tx > 0 ? ((res + tx - 1) / tx) * nyb * nzb : ntiles;
+  const int per_x = (ntiles_x + FOS_XCDS - 1) / FOS_XCDS;
+  if (tx == 0) {
+    hipLaunchKernelGGL(k_face_emit, dim3(per * FOS_XCDS), dim3(256), 0, st, res, n, filtered,
+                       ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap, ctx->flags, nyb, nzb,
+                       ntiles, per);
+  } else {
+    auto kern = tx == 1 ? k_face_emit_lds<1> : tx == 2 ? k_face_emit_lds<2> : k_face_emit_lds<4>;
+    hipLaunchKernelGGL(kern, dim3(per_x * FOS_XCDS), dim3(256), 0, st, res, n, filtered,
+                       ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap, ctx->flags, nyb, nzb,
+                       ntiles_x, per_x);
+  }
+  FOS_CHECK_LAUNCH("k_face_emit");
+  return 0;
+}
+
+// After an overflow: tables four times larger (cleared), flag cleared.
+static int face_grow(fos_ctx *ctx, hipStream_t st) {
+  const int cap = ctx->tab_cap * 4;
+  FOS_REQUIRE(cap <= 4096, "face accumulation: row table overflow at %d", ctx->tab_cap);
+  int rc = alloc_tables(ctx, cap);
+  if (rc) return rc;
+  return fos_fill_i32(ctx->flags, 1, 0, st);
+}
+
+static int face_compact(fos_ctx *ctx, int n, int32_t *face_ids, int32_t *face_counts,
+                        hipStream_t st) {
+  const int cap = ctx->tab_cap;
+  const size_t shmem = (size_t)4 * cap * sizeof(unsigned);
+  hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st, n,
+                     ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts);
+  FOS_CHECK_LAUNCH("k_face_compact");
+  return 0;
+}
+
 extern "C" {
 
 int fos_set_param(const char *name, int value) {
@@ -1693,45 +1734,18 @@ int fos_face_accumulate(fos_ctx *ctx, int res, int n, const int32_t *filtered,
               "fos_face_accumulate: res=%d exceeds the 32-bit face key", res);
   if (n == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64, ntiles = res * nyb * nzb;
-  const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
-  const int tx = g_emit_tx;
-  const int ntiles_x = tx > 0 ? ((res + tx - 1) / tx) * nyb * nzb : ntiles;
-  const int per_x = (ntiles_x + FOS_XCDS - 1) / FOS_XCDS;
   while (true) {
-    if (tx == 0) {
-      hipLaunchKernelGGL(k_face_emit, dim3(per * FOS_XCDS), dim3(256), 0, st, res, n,
-                         filtered, ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap,
-                         ctx->flags, nyb, nzb, ntiles, per);
-    } else {
-      auto kern = tx == 1 ? k_face_emit_lds<1> : tx == 2 ? k_face_emit_lds<2>
-                                                         : k_face_emit_lds<4>;
-      hipLaunchKernelGGL(kern, dim3(per_x * FOS_XCDS), dim3(256), 0, st, res, n, filtered,
-                         ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap, ctx->flags, nyb,
-                         nzb, ntiles_x, per_x);
-    }
-    FOS_CHECK_LAUNCH("k_face_emit");
+    int rc = face_emit(ctx, res, n, filtered, st);
+    if (rc) return rc;
     FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, st));
     FOS_HIP(hipStreamSynchronize(st));
-    const int overflow = *(volatile int *)ctx->host_i;
-    if (!overflow) break;
-    // A row met more distinct partners than the table holds: grow the
-    // tables and emit again from scratch.
-    const int cap = ctx->tab_cap * 4;
-    FOS_REQUIRE(cap <= 4096, "fos_face_accumulate: row table overflow at %d",
-                ctx->tab_cap);
-    int rc = alloc_tables(ctx, cap);
-    if (rc) return rc;
-    rc = fos_fill_i32(ctx->flags, 1, 0, stream);
+    if (!*(volatile int *)ctx->host_i) break;
+    rc = face_grow(ctx, st);
     if (rc) return rc;
   }
-  const int cap = ctx->tab_cap;
-  const size_t shmem = (size_t)4 * cap * sizeof(unsigned);
-  hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st,
-                     n, ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts);
-  FOS_CHECK_LAUNCH("k_face_compact");
-  return 0;
+  return face_compact(ctx, n, face_ids, face_counts, st);
 }
+
 
 }  // extern "C"
 
@@ -1757,17 +1771,13 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
 
 }  // extern "C"
 
-// fixed_pairs > 0: that many round pairs with no host round trip (a device
-// flag, ctx->flags[2], is set once a pair changes nothing and later rounds
-// skip); 0: pairs until the host sees no change.
-static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
-                         const int32_t *face_ids, const int32_t *face_counts,
-                         float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
-                         int32_t *used_count, int32_t *fsc, int fixed_pairs, void *stream) {
-  if (n == 0) return 0;
-  hipStream_t st = (hipStream_t)stream;
+// Finalize in phases: prepare (classify, incoming lists, decisions cleared),
+// round pairs, materialize.
+static int fin_prepare(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
+                       const int32_t *face_ids, const int32_t *face_counts,
+                       const float *prev_scores, hipStream_t st) {
   const int64_t m = (int64_t)n * FOS_MAX_NEIGHBORS;
-  int rc = fos_fill_i32(ctx->in_count, n + 1, 0, stream);
+  int rc = fos_fill_i32(ctx->in_count, n + 1, 0, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_fin_classify, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
                      rad, voxel, pvox, face_ids, face_counts, prev_scores, ctx->score,
@@ -1775,7 +1785,7 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
   FOS_CHECK_LAUNCH("k_fin_classify");
   rc = exclusive_scan(ctx, ctx->in_count, ctx->in_start, n + 1, st);
   if (rc) return rc;
-  rc = fos_fill_i32(ctx->in_fill, n, 0, stream);
+  rc = fos_fill_i32(ctx->in_fill, n, 0, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_fin_scatter, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
                      face_ids, ctx->cls, ctx->in_start, ctx->in_fill, ctx->in_src);
@@ -1783,50 +1793,142 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
   hipLaunchKernelGGL(k_fin_rank, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n, face_ids,
                      ctx->in_start, ctx->in_src, ctx->in_sorted);
   FOS_CHECK_LAUNCH("k_fin_rank");
-  // Fixed point from "nothing accepted": rounds alternate between two packed
-  // decision buffers; the host reads the change flag after every second
-  // round and stops once neither round of a pair changed anything.
+  // the fixed point starts from "nothing accepted"
+  return fos_fill_i32((int32_t *)ctx->dec, n, 0, st);
+}
+
+// `pairs` round pairs with no host round trip: rounds alternate between two
+// packed decision buffers (the result is back in the first after a pair); a
+// pair that changes nothing sets the done flag (ctx->flags[2]) and later
+// rounds skip.  With `last_failed`, the last pair marks ctx->flags[3] when
+// the fixed point was not reached.  The caller clears flags[1..2] first.
+static int fin_pairs(fos_ctx *ctx, int n, int pairs, bool last_failed, hipStream_t st) {
   unsigned *dec_a = (unsigned *)ctx->dec, *dec_b = dec_a + ctx->max_n;
-  rc = fos_fill_i32((int32_t *)dec_a, n, 0, stream);
+  for (int pair = 0; pair < pairs; ++pair) {
+    hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
+                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1,
+                       (const int *)(ctx->flags + 2));
+    hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
+                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1,
+                       (const int *)(ctx->flags + 2));
+    hipLaunchKernelGGL(k_fin_pair_end, dim3(1), dim3(1), 0, st, ctx->flags + 1, ctx->flags + 2,
+                       last_failed && pair == pairs - 1 ? ctx->flags + 3 : nullptr);
+    FOS_CHECK_LAUNCH("k_fin_round");
+  }
+  return 0;
+}
+
+static int fin_materialize(fos_ctx *ctx, int n, const int32_t *face_ids, float *prev_scores,
+                           int32_t *neighbor_list, int32_t *neighbor_count, int32_t *used_count,
+                           int32_t *fsc, hipStream_t st) {
+  hipLaunchKernelGGL(k_fin_materialize, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+                     face_ids, ctx->score, ctx->cls, (const unsigned *)ctx->dec, ctx->in_start,
+                     ctx->in_sorted, prev_scores, neighbor_list, neighbor_count, used_count, fsc);
+  FOS_CHECK_LAUNCH("k_fin_materialize");
+  return 0;
+}
+
+// Reads ctx->flags[0..3] into host (one round trip).
+static int read_flags(fos_ctx *ctx, int out[4], hipStream_t st) {
+  FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipStreamSynchronize(st));
+  const volatile int *h = (const volatile int *)ctx->host_i;
+  for (int k = 0; k < 4; ++k) out[k] = h[k];
+  return 0;
+}
+
+// Round pairs until the fixed point: bursts of FIN_BURST pairs with
+// done-skips, the host reading the done flag after each burst (one round
+// trip for the usual two or three pairs).
+#define FIN_BURST 4
+static int fin_pairs_host(fos_ctx *ctx, int n, hipStream_t st) {
+  int rc = fos_fill_i32(ctx->flags + 1, 2, 0, st);
+  if (rc) return rc;
+  for (int done = 0, total = 0; !done; total += FIN_BURST) {
+    FOS_REQUIRE(total <= n + 1, "finalize: no fixed point after %d round pairs", total);
+    rc = fin_pairs(ctx, n, FIN_BURST, false, st);
+    if (rc) return rc;
+    int f[4];
+    rc = read_flags(ctx, f, st);
+    if (rc) return rc;
+    done = f[2];
+  }
+  return 0;
+}
+
+// fixed_pairs > 0: that many round pairs with no host round trip (flags[3]
+// stays set if the last pair had not reached the fixed point); 0: until the
+// fixed point (fin_pairs_host).
+static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
+                         const int32_t *face_ids, const int32_t *face_counts,
+                         float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
+                         int32_t *used_count, int32_t *fsc, int fixed_pairs, void *stream) {
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = fin_prepare(ctx, n, rad, voxel, pvox, face_ids, face_counts, prev_scores, st);
   if (rc) return rc;
   if (fixed_pairs > 0) {
     rc = fos_fill_i32(ctx->flags + 1, 2, 0, stream);
     if (rc) return rc;
-    for (int pair = 0; pair < fixed_pairs; ++pair) {
-      hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                         ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1,
-                         (const int *)(ctx->flags + 2));
-      hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                         ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1,
-                         (const int *)(ctx->flags + 2));
-      hipLaunchKernelGGL(k_fin_pair_end, dim3(1), dim3(1), 0, st, ctx->flags + 1, ctx->flags + 2,
-                         pair == fixed_pairs - 1 ? ctx->flags + 3 : nullptr);
-      FOS_CHECK_LAUNCH("k_fin_round");
-    }
+    rc = fin_pairs(ctx, n, fixed_pairs, true, st);
   } else {
-    for (int pair = 0; pair <= n + 1; ++pair) {
-      rc = fos_fill_i32(ctx->flags + 1, 1, 0, stream);
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                         ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1,
-                         (const int *)nullptr);
-      FOS_CHECK_LAUNCH("k_fin_round");
-      hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                         ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1,
-                         (const int *)nullptr);
-      FOS_CHECK_LAUNCH("k_fin_round");
-      FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags + 1, sizeof(int),
-                             hipMemcpyDeviceToHost, st));
-      FOS_HIP(hipStreamSynchronize(st));
-      if (!*(volatile int *)ctx->host_i) break;
-    }
+    rc = fin_pairs_host(ctx, n, st);
   }
-  hipLaunchKernelGGL(k_fin_materialize, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
-                     face_ids, ctx->score, ctx->cls, dec_a, ctx->in_start, ctx->in_sorted,
-                     prev_scores, neighbor_list, neighbor_count, used_count, fsc);
-  FOS_CHECK_LAUNCH("k_fin_materialize");
-  return 0;
+  if (rc) return rc;
+  return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
+                         used_count, fsc, st);
 }
+
+extern "C" {
+
+int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const float *rad,
+                  float voxel, int32_t *face_ids, int32_t *face_counts, float *prev_scores,
+                  int32_t *neighbor_list, int32_t *neighbor_count, int32_t *used_count,
+                  int32_t *fsc, void *stream) {
+  FOS_REQUIRE(ctx && filtered && face_ids && face_counts && res >= 1 && res <= ctx->max_res &&
+                  n >= 0 && n <= ctx->max_n && voxel > 0.0f &&
+                  (n == 0 || (rad && prev_scores && neighbor_list && neighbor_count &&
+                              used_count && fsc)),
+              "fos_jfa_faces: bad arguments (n=%d res=%d)", n, res);
+  FOS_REQUIRE((double)res * res * res * 6.0 < 4294967295.0,
+              "fos_jfa_faces: res=%d exceeds the 32-bit face key", res);
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  // Emission, compaction, finalize preparation and a burst of round pairs
+  // are queued together; one host read then sees both the row-table
+  // overflow flag and the fixed-point flag.  An overflow regrows the tables
+  // and starts over (nothing the caller keeps has been written yet: the
+  // neighbour lists and hysteresis scores are written by materialize).
+  while (true) {
+    int rc = face_emit(ctx, res, n, filtered, st);
+    if (!rc) rc = face_compact(ctx, n, face_ids, face_counts, st);
+    if (!rc) rc = fin_prepare(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores, st);
+    if (!rc) rc = fos_fill_i32(ctx->flags + 1, 2, 0, st);
+    if (!rc) rc = fin_pairs(ctx, n, FIN_BURST, false, st);
+    if (rc) return rc;
+    int f[4];
+    rc = read_flags(ctx, f, st);
+    if (rc) return rc;
+    if (f[0]) {
+      rc = face_grow(ctx, st);
+      if (rc) return rc;
+      continue;
+    }
+    if (!f[2]) {
+      for (int done = 0, total = FIN_BURST; !done; total += FIN_BURST) {
+        FOS_REQUIRE(total <= n + 1, "finalize: no fixed point after %d round pairs", total);
+        rc = fin_pairs(ctx, n, FIN_BURST, false, st);
+        if (!rc) rc = read_flags(ctx, f, st);
+        if (rc) return rc;
+        done = f[2];
+      }
+    }
+    return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
+                           used_count, fsc, st);
+  }
+}
+
+}  // extern "C"
 
 extern "C" {
 

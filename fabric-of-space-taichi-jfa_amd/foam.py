@@ -19,9 +19,13 @@ makes non-zero (rescue is off), so they are not run here (the drop-in
 ``dynamics`` module still exposes them).  PBD is the Jacobi form: every
 particle reads the pass-start positions.
 
-All work is enqueued on torch's current stream through libfos; the host
-synchronises once per frame to read the overlap maximum (it sets the PBD pass
-count, as in the reference) and once per JFA frame for the warm-start gate.
+All work is enqueued on torch's current stream through libfos.  The host
+reads the device once per frame for the overlap maximum (it sets the PBD pass
+count, as in the reference) and once per JFA frame where the face rows are
+finalized (row-table overflow and the fixed point).  The warm-start gate is
+decided on the device; the host learns it with the next frame's overlap
+maximum (both in one read) and only then sets the settle counter, which the
+next frame is the first to use.
 """
 import math
 
@@ -32,6 +36,24 @@ import fos_lib as fl
 import foam_config as fc
 
 MAXN = 32
+
+
+class FrameInfo(dict):
+    """What step() returns: frame, pbd_passes, max_depth, jfa, growth (and the
+    JFA telemetry when on).  "growth" is decided on the device; reading it
+    before the next frame has read it back costs one device read."""
+
+    def __init__(self, sim, **kw):
+        super().__init__(**kw)
+        self._sim = sim
+
+    def __getitem__(self, k):
+        if k == "growth" and dict.get(self, "growth") is None:
+            self._sim.resolve()
+        return dict.__getitem__(self, k)
+
+    def get(self, k, default=None):
+        return self[k] if k in self else default
 
 
 class FoamSim:
@@ -69,11 +91,14 @@ class FoamSim:
         self.prev_scores = torch.zeros(n, MAXN, dtype=torch.float32, device=dev)
         self.info = torch.zeros(2 + 64, dtype=torch.int32, device=dev)
         self.pstats = torch.zeros(2, dtype=torch.float32, device=dev)
-        self.maxd = torch.zeros(1, dtype=torch.float32, device=dev)
+        # [0] overlap maximum of the frame, [1] the warm-start gate (1.0 =
+        # growth) of the last frame that decided it on the device
+        self.maxd = torch.zeros(2, dtype=torch.float32, device=dev)
         self.ctx = fl.context(n, self.res, C)
         self.frame = 0
         self.settle = 0
         self.growth = True
+        self.pending = None        # FrameInfo whose gate is still on the device
         self.last = {}
         # optional list of ctypes arrays of passes+1 hipEvent_t handles; each
         # JFA frame consumes one to time its pass kernels on the stream
@@ -134,13 +159,36 @@ class FoamSim:
                 int(c.early_exit), c.th_pass_delta, c.min_passes, fl.ptr(self.grid),
                 fl.ptr(self.temp), fl.ptr(self.info), ev, st)
         fl.call("fos_majority", self.res, fl.ptr(self.grid), fl.ptr(self.filtered), st)
-        fl.call("fos_face_accumulate", ctx, self.res, self.n, fl.ptr(self.filtered),
-                fl.ptr(self.face_ids), fl.ptr(self.face_counts), st)
-        fl.call("fos_finalize_faces", ctx, self.n, fl.ptr(self.rad), voxel,
-                fl.ptr(self.face_ids), fl.ptr(self.face_counts), fl.ptr(self.prev_scores),
-                fl.ptr(self.neighbor_list), fl.ptr(self.neighbor_count),
-                fl.ptr(self.used_count), fl.ptr(self.fsc), st)
+        fl.call("fos_jfa_faces", ctx, self.res, self.n, fl.ptr(self.filtered),
+                fl.ptr(self.rad), voxel, fl.ptr(self.face_ids), fl.ptr(self.face_counts),
+                fl.ptr(self.prev_scores), fl.ptr(self.neighbor_list),
+                fl.ptr(self.neighbor_count), fl.ptr(self.used_count), fl.ptr(self.fsc), st)
         fl.call("fos_fsc_ema", self.n, c.ema_alpha, fl.ptr(self.fsc), fl.ptr(self.fsc_ema), st)
+
+    def resolve(self, gate=None):
+        """Applies the device-decided warm-start gate of the last frame (read
+        now unless the caller read it): settle reset and the frame's growth."""
+        if self.pending is None:
+            return
+        if gate is None:
+            gate = float(self.maxd[1].item())
+        g = gate > 0.5
+        self.growth = g
+        if g:
+            self.settle = self.cfg.adjustment_frames
+        dict.__setitem__(self.pending, "growth", g)
+        self.pending = None
+
+    def gate_growth(self, f):
+        """run.py:1184-1231 after a JFA (or predicted) frame: the gate and the
+        gated FSC targets on the device, no host read."""
+        c, st = self.cfg, fl.stream()
+        gate = self.maxd.data_ptr() + 4
+        fl.call("fos_fsc_gate", self.ctx.handle, fl.ptr(self.fsc), self.n, f,
+                c.warmstart_frames, c.fsc_zero_rate, gate, st)
+        fl.call("fos_fsc_targets_gated", fl.ptr(self.rad), fl.ptr(self.rad_target),
+                fl.ptr(self.fsc), fl.ptr(self.fsc_ema), self.n, c.f_low, c.f_high,
+                c.growth_pct, c.r_min, c.r_max, c.deadband, gate, st)
 
     # -- one frame ------------------------------------------------------------
     def step(self):
@@ -150,7 +198,9 @@ class FoamSim:
         # 0-1
         self.rebuild_grid()
         self.max_overlap()
-        max_depth = float(self.maxd.item())
+        md = self.maxd.cpu()
+        max_depth = float(md[0].item())
+        self.resolve(float(md[1].item()))
         passes = max(c.pbd_base, min(c.pbd_max,
                                      int(c.pbd_base + c.pbd_scale * (max_depth / (0.2 * c.r_max)))))
         # 2 settle
@@ -198,26 +248,26 @@ class FoamSim:
                                        self.face_counts, r_mean=self.r_scale)
                 fl.call("fos_fsc_ema", self.n, c.ema_alpha, fl.ptr(self.fsc),
                         fl.ptr(self.fsc_ema), st)
-            f0 = float((self.fsc == 0).sum().item()) / float(max(1, self.n))
-            self.growth = not ((f < c.warmstart_frames) or (f0 > c.fsc_zero_rate))
+            gated = True
         else:
             ran = c.jfa_runs(f)
             if ran:
                 self.run_jfa()
                 if self.on_jfa is not None:
                     self.on_jfa(self)
-                f0 = float((self.fsc == 0).sum().item()) / float(max(1, self.n))
-                self.growth = not ((f < c.warmstart_frames) or (f0 > c.fsc_zero_rate))
-            else:
-                self.growth = True
-        # 9 controller
-        if self.growth:
+            gated = ran
+        # 9 controller: after a JFA the gate is decided on the device
+        self.last = FrameInfo(self, frame=f, pbd_passes=passes, max_depth=max_depth, jfa=ran,
+                              growth=None if gated else True)
+        if gated:
+            self.gate_growth(f)
+            self.pending = self.last
+        else:
+            self.growth = True
             fl.call("fos_fsc_targets", fl.ptr(self.rad), fl.ptr(self.rad_target), fl.ptr(self.fsc),
                     fl.ptr(self.fsc_ema), self.n, c.f_low, c.f_high, c.growth_pct, c.r_min,
                     c.r_max, c.deadband, st)
             self.settle = c.adjustment_frames
-        self.last = {"frame": f, "pbd_passes": passes, "max_depth": max_depth,
-                     "jfa": ran, "growth": self.growth}
         if self.telemetry and ran:
             self.last.update(self.jfa_telemetry())
         self.frame += 1
@@ -267,6 +317,7 @@ class FoamSim:
              "fsc_ema", "neighbor_list", "neighbor_count", "used_count", "prev_scores")
 
     def snapshot(self):
+        self.resolve()
         torch.cuda.synchronize()
         out = {k: getattr(self, k).detach().cpu().numpy().copy() for k in self.STATE}
         out["frame"] = np.int64(self.frame)
@@ -278,6 +329,7 @@ class FoamSim:
             getattr(self, k).copy_(torch.from_numpy(np.ascontiguousarray(snap[k])))
         self.frame = int(snap["frame"])
         self.settle = int(snap["settle"])
+        self.pending = None
 
     def save(self, path):
         np.savez(path, **self.snapshot())

@@ -70,6 +70,9 @@ SIGNATURES = {
     "fos_topo_knn": [P, P, I, I, F, F, I, P, P, P, P, P],
     "fos_topo_ema": [P, P, I, F, P],
     "fos_topo_radii": [P, P, I, F, F, F, F, F, F, P],
+    "fos_jfa_faces": [P, I, I, P, P, F, P, P, P, P, P, P, P, P],
+    "fos_fsc_targets_gated": [P, P, P, P, I, I, I, F, F, F, F, P, P],
+    "fos_fsc_gate": [P, P, I, I, I, D, P, P],
     "fos_batch_create": [P, I, P, P, P, P, P, P, P, P, P, F],
     "fos_batch_destroy": [P],
     "fos_batch_info": [P, P],

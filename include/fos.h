@@ -69,6 +69,15 @@ int fos_fill_f32(float *dst, int64_t count, float value, void *stream);
 
 /* ---- jfa.py ------------------------------------------------------------ */
 
+/* fos_face_accumulate then fos_finalize_faces (jfa.py:703-857) with one host
+ * round trip in the usual case: emission, compaction and a burst of
+ * fixed-point rounds are queued together and one read sees both the
+ * row-table overflow and the fixed-point flags (an overflow regrows the
+ * tables and starts over before anything the caller keeps is written). */
+int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const float *rad,
+                  float voxel, int32_t *face_ids, int32_t *face_counts, float *prev_scores,
+                  int32_t *neighbor_list, int32_t *neighbor_count, int32_t *used_count,
+                  int32_t *fsc, void *stream);
 /* Per-frame seed table (x, y, z, (beta*r)^2) that the JFA kernels gather;
  * the reference recomputes the same terms per candidate (jfa.py:470-477). */
 int fos_jfa_seed_table(fos_ctx *ctx, const float *pos, const float *rad,
@@ -176,6 +185,16 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
 
 /* ---- dynamics.py ------------------------------------------------------- */
 
+/* fos_fsc_targets applied only when *gate (device) > 0.5 (the gate that
+ * fos_fsc_gate computed on the device; no host round trip). */
+int fos_fsc_targets_gated(const float *rad, float *rad_target, const int32_t *fsc,
+                          const float *fsc_ema, int n, int f_low, int f_high, float growth_pct,
+                          float r_min, float r_max, float deadband, const float *gate,
+                          void *stream);
+/* run.py:1184-1195 on the device: *gate = 0 while frame < warmstart_frames
+ * or the fraction of particles with FSC 0 exceeds fsc_zero_rate, else 1. */
+int fos_fsc_gate(fos_ctx *ctx, const int32_t *fsc, int n, int frame, int warmstart_frames,
+                 double fsc_zero_rate, float *gate, void *stream);
 /* dynamics.py:77-171 project_overlaps, Jacobi form: reads pos_in, writes
  * pos_out (distinct buffers).  gap_factor = (float)(1 + GAP_FRACTION). */
 int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad,
