@@ -178,7 +178,8 @@ struct fos_ctx {
   int *tab_b;
   unsigned *tab_k;
   int *tab_c;
-  int *flags;  // [0] row-table overflow, [1] fixed-point changed, [2..] misc
+  int *flags;  // [0] row-table overflow, [3] sticky finalize failure,
+               // [16..63] per-pair change flags of the finalize fixed point
   // finalize scratch
   float *score;           // [max_n*32]
   unsigned char *cls;     // [max_n*32]

@@ -850,9 +850,11 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Workgroup-level merge: a tile of TX x-planes x 4 rows x 64 voxels merges
-// its pairs in an LDS hash keyed by (a, b) (count sum, key minimum) and then
-// inserts each distinct pair once into the row tables.  A pair that finds no
+// Workgroup-level merge: a tile of TX x-planes x (4 * 64 / ZW) y-rows x ZW
+// z-voxels (each wave covers 64 / ZW rows of ZW voxels) merges its pairs in
+// an LDS hash keyed by (a, b) (count sum, key minimum) and then inserts each
+// distinct pair once into the row tables.  Squarer tiles (smaller ZW) cross
+// fewer cell boundaries per voxel, so fewer pairs reach the row tables.  A pair that finds no
 // LDS slot within EMIT_PROBES probes goes straight to the row table, so the
 // LDS table can be smaller than the worst case.
 #define EMIT_HS 2048
@@ -863,8 +865,9 @@ __device__ __forceinline__ unsigned pair_hash(int a, int b) {
   return ((unsigned)a * 0x9E3779B1u) ^ ((unsigned)b * 0x85EBCA77u);
 }
 
-// One emission tile (all threads of the workgroup call it).
-template <int TX>
+// One emission tile (all threads of the workgroup call it); nyb, nzb count
+// the tile's y and z blocks (y block 4 * 64 / ZW rows, z block ZW voxels).
+template <int TX, int ZW = 64>
 __device__ __forceinline__ void face_emit_tile(int tile, int res, int n,
                                                const int32_t *__restrict__ f, int *tb,
                                                unsigned *tk, int *tc, int cap, int *overflow,
@@ -881,7 +884,7 @@ __device__ __forceinline__ void face_emit_tile(int tile, int res, int n,
   if (tile < ntiles) {
     const int zb = tile % nzb, t2 = tile / nzb;
     const int yb = t2 % nyb, x0 = (t2 / nyb) * TX;
-    const int z = zb * 64 + (t & 63), y = yb * 4 + (t >> 6);
+    const int z = zb * ZW + (t % ZW), y = yb * (256 / ZW) + t / ZW;
     for (int xt = 0; xt < TX; ++xt) {
       const int x = x0 + xt;
       if (x >= res || y >= res || z >= res) break;
@@ -944,14 +947,14 @@ __device__ __forceinline__ void face_emit_tile(int tile, int res, int n,
   }
 }
 
-template <int TX>
+template <int TX, int ZW>
 __global__ void __launch_bounds__(256)
     k_face_emit_lds(int res, int n, const int32_t *__restrict__ f, int *tb, unsigned *tk,
                     int *tc, int cap, int *overflow, int nyb, int nzb, int ntiles, int per) {
   __shared__ unsigned long long s_pair[EMIT_HS];
   __shared__ unsigned s_key[EMIT_HS];
   __shared__ int s_cnt[EMIT_HS];
-  face_emit_tile<TX>(xcd_tile(blockIdx.x, per), res, n, f, tb, tk, tc, cap, overflow, nyb, nzb,
+  face_emit_tile<TX, ZW>(xcd_tile(blockIdx.x, per), res, n, f, tb, tk, tc, cap, overflow, nyb, nzb,
                      ntiles, s_pair, s_key, s_cnt);
 }
 
@@ -1099,14 +1102,20 @@ __global__ void k_fin_rank(int n, const int32_t *__restrict__ face_ids,
 // packed as one bit per slot.
 // 8 lanes per row gather the accepted-incoming count (an order-free sum);
 // lane 0 then decides the row.
-// done (may be NULL): when set, the fixed point was reached in an earlier
-// round pair and the round does nothing.
+// Round pairs keep one change flag each: both rounds of pair k set chg_cur
+// when a row's decisions change.  chg_prev (may be NULL), the previous pair's
+// flag, still clear means that pair changed nothing -- the fixed point -- and
+// the round does nothing (so its own flag stays clear and the skip cascades).
+// failed (may be NULL; given to the second round of a last pair) is set when
+// that pair changed anything.
 __global__ void __launch_bounds__(256)
     k_fin_round(int n, const unsigned char *__restrict__ cls, const int *__restrict__ in_start,
                 const int *__restrict__ in_sorted, const unsigned *__restrict__ dec_prev,
-                unsigned *__restrict__ dec_next, int *__restrict__ changed,
-                const int *__restrict__ done) {
-  if (done && *done) return;
+                unsigned *__restrict__ dec_next, const int *__restrict__ chg_prev,
+                int *__restrict__ chg_cur, int *__restrict__ failed) {
+  if (chg_prev && *chg_prev == 0) return;
+  // the pair's first round changed something
+  if (failed && blockIdx.x == 0 && threadIdx.x == 0 && *chg_cur) atomicOr(failed, 1);
   const int sub = (int)(threadIdx.x & 7);
   const int i = (int)(blockIdx.x * 32 + (threadIdx.x >> 3));
   int acc = 0;
@@ -1137,18 +1146,23 @@ __global__ void __launch_bounds__(256)
     }
   }
   dec_next[i] = bits;
-  if (bits != dec_prev[i]) atomicOr(changed, 1);
+  if (bits != dec_prev[i]) {
+    atomicOr(chg_cur, 1);
+    if (failed) atomicOr(failed, 1);
+  }
 }
 
-// After a round pair of the fixed-round form: the pair changed nothing ->
-// done; clears the change flag for the next pair.  After the last pair,
-// `failed` (sticky until the caller clears it) records a fixed point not
-// reached.
-__global__ void k_fin_pair_end(int *__restrict__ changed, int *__restrict__ done,
-                               int *__restrict__ failed) {
-  if (!*done && !*changed) *done = 1;
-  *changed = 0;
-  if (failed && !*done) *failed = 1;
+// Clears what finalize accumulates into: incoming counts (n + 1), fill
+// cursors and decisions (n) and the per-pair change flags (nchg).
+__global__ void k_fin_clear(int n, int *__restrict__ in_count, int *__restrict__ in_fill,
+                            unsigned *__restrict__ dec, int *__restrict__ chg, int nchg) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t <= n) in_count[t] = 0;
+  if (t < n) {
+    in_fill[t] = 0;
+    dec[t] = 0;
+  }
+  if (t < nchg) chg[t] = 0;
 }
 
 // Writes row i's final neighbour list, hysteresis scores and counts.  The
@@ -1157,48 +1171,79 @@ __global__ void k_fin_pair_end(int *__restrict__ changed, int *__restrict__ done
 // a reject zeroes its own slot) -- and a later store to the same address
 // wins, so prev ends exactly as in the serial loop; untouched slots keep last
 // frame's value.
-__global__ void k_fin_materialize(int n, const int32_t *__restrict__ face_ids,
-                                  const float *__restrict__ score,
-                                  const unsigned char *__restrict__ cls,
-                                  const unsigned *__restrict__ dec,
-                                  const int *__restrict__ in_start,
-                                  const int *__restrict__ in_sorted,
-                                  float *__restrict__ prev, int32_t *__restrict__ nl,
-                                  int32_t *__restrict__ nc, int32_t *__restrict__ used_count,
-                                  int32_t *__restrict__ fsc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t base = (int64_t)i * FOS_MAX_NEIGHBORS;
-  int used = 0;
-  for (int p = in_start[i]; p < in_start[i + 1] && used < FOS_MAX_NEIGHBORS; ++p) {
-    const int src = in_sorted[p];
-    if (!((dec[src / FOS_MAX_NEIGHBORS] >> (src % FOS_MAX_NEIGHBORS)) & 1u)) continue;
-    nl[base + used] = src / FOS_MAX_NEIGHBORS;
-    prev[base + used] = score[src];
-    used += 1;
-  }
-  const unsigned bits = dec[i];
-  const uint4 *c4 = (const uint4 *)(cls + base);
-  const uint4 lo = c4[0], hi = c4[1];
-  const unsigned words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-#pragma unroll
-  for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
-    const unsigned c = (words[s >> 2] >> (8 * (s & 3))) & 0xffu;
-    if ((c & 3u) == CLS_INVALID) continue;
-    if ((bits >> s) & 1u) {
-      if (used < FOS_MAX_NEIGHBORS) {
-        nl[base + used] = face_ids[base + s];
-        prev[base + used] = score[base + s];
-        used += 1;
+// 32 lanes per row (8 rows per 256-thread block), lane s owning position s.
+// The serial chronology is resolved without racing stores: the appends from
+// lower rows take positions 0 .. u_in - 1 (u_in = min(32, accepted
+// incoming)); an accepted own slot s appends at u_in + (accepted slots
+// before s), time s; a rejected own slot s zeroes position s, time s.  The
+// store with the latest time is the one that remains.
+__global__ void __launch_bounds__(256)
+    k_fin_materialize(int n, const int32_t *__restrict__ face_ids,
+                      const float *__restrict__ score, const unsigned char *__restrict__ cls,
+                      const unsigned *__restrict__ dec, const int *__restrict__ in_start,
+                      const int *__restrict__ in_sorted, float *__restrict__ prev,
+                      int32_t *__restrict__ nl, int32_t *__restrict__ nc,
+                      int32_t *__restrict__ used_count, int32_t *__restrict__ fsc) {
+  __shared__ int s_time[8][FOS_MAX_NEIGHBORS];
+  __shared__ float s_val[8][FOS_MAX_NEIGHBORS];
+  __shared__ int s_id[8][FOS_MAX_NEIGHBORS];
+  const int r = (int)(threadIdx.x >> 5), s = (int)(threadIdx.x & 31);
+  const int half = (int)(threadIdx.x & 32);
+  const unsigned below = (1u << s) - 1u;
+  const int i = (int)blockIdx.x * 8 + r;
+  const bool live = i < n;
+  const int64_t base = (int64_t)(live ? i : 0) * FOS_MAX_NEIGHBORS;
+  s_time[r][s] = -1;
+  __syncthreads();
+  // appends from lower rows, in incoming order
+  int u = 0;
+  if (live) {
+    const int e = in_start[i + 1];
+    for (int p0 = in_start[i]; p0 < e && u < FOS_MAX_NEIGHBORS; p0 += FOS_MAX_NEIGHBORS) {
+      const int p = p0 + s;
+      int src = 0;
+      bool acc = false;
+      if (p < e) {
+        src = in_sorted[p];
+        acc = (dec[src / FOS_MAX_NEIGHBORS] >> (src % FOS_MAX_NEIGHBORS)) & 1u;
       }
-    } else {
-      prev[base + s] = 0.0f;
+      const unsigned m = (unsigned)(__ballot(acc) >> half);
+      const int pos = u + __popc(m & below);
+      if (acc && pos < FOS_MAX_NEIGHBORS) {
+        s_val[r][pos] = score[src];
+        s_id[r][pos] = src / FOS_MAX_NEIGHBORS;
+      }
+      u += __popc(m);
     }
   }
-  for (int p = used; p < FOS_MAX_NEIGHBORS; ++p) nl[base + p] = -1;
-  nc[i] = used;
-  used_count[i] = used;
-  fsc[i] = used;
+  const int u_in = u < FOS_MAX_NEIGHBORS ? u : FOS_MAX_NEIGHBORS;
+  // the row's own slots
+  const unsigned c = live ? (unsigned)cls[base + s] : (unsigned)CLS_INVALID;
+  const bool valid = (c & 3u) != CLS_INVALID;
+  const bool accd = valid && ((dec[live ? i : 0] >> s) & 1u);
+  const bool rej = valid && !accd;
+  const unsigned am = (unsigned)(__ballot(accd) >> half);
+  const int rank = u_in + __popc(am & below);
+  if (accd && rank < FOS_MAX_NEIGHBORS) {
+    s_time[r][rank] = s;
+    s_val[r][rank] = score[base + s];
+    s_id[r][rank] = face_ids[base + s];
+  }
+  __syncthreads();
+  if (!live) return;
+  const int total = u_in + __popc(am);
+  const int used = total < FOS_MAX_NEIGHBORS ? total : FOS_MAX_NEIGHBORS;
+  const int ta = s_time[r][s], tr = rej ? s : -1;
+  if (ta >= 0 || tr >= 0)
+    prev[base + s] = ta > tr ? s_val[r][s] : 0.0f;
+  else if (s < u_in)
+    prev[base + s] = s_val[r][s];
+  nl[base + s] = s < used ? s_id[r][s] : -1;
+  if (s == 0) {
+    nc[i] = used;
+    used_count[i] = used;
+    fsc[i] = used;
+  }
 }
 
 // jfa.py:766-785
@@ -1415,6 +1460,7 @@ static int g_tile_x = 4;
 static int g_dedup = 0;
 static int g_kernel = 0;
 static int g_emit_tx = 2;
+static int g_emit_zw = 64;
 extern int g_grid_block;  // fos_grid.hip
 
 template <bool U, int K>
@@ -1520,17 +1566,27 @@ static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
 // Face-pair emission into the row tables (the kernel chosen by
 // face_emit_tx); the overflow flag (ctx->flags[0]) is left for the caller.
 static int face_emit(fos_ctx *ctx, int res, int n, const int32_t *filtered, hipStream_t st) {
-  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64, ntiles = res * nyb * nzb;
-  const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
   const int tx = g_emit_tx;
-  const int ntiles_x = tx > 0 ? ((res + tx - 1) / tx) * nyb * nzb : ntiles;
-  const int per_x = (ntiles_x + FOS_XCDS - 1) / FOS_XCDS;
   if (tx == 0) {
+    const int nyb = (res + 3) / 4, nzb = (res + 63) / 64, ntiles = res * nyb * nzb;
+    const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
     hipLaunchKernelGGL(k_face_emit, dim3(per * FOS_XCDS), dim3(256), 0, st, res, n, filtered,
                        ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap, ctx->flags, nyb, nzb,
                        ntiles, per);
   } else {
-    auto kern = tx == 1 ? k_face_emit_lds<1> : tx == 2 ? k_face_emit_lds<2> : k_face_emit_lds<4>;
+    const int zw = g_emit_zw, yt = 256 / zw;
+    const int nyb = (res + yt - 1) / yt, nzb = (res + zw - 1) / zw;
+    const int ntiles_x = ((res + tx - 1) / tx) * nyb * nzb;
+    const int per_x = (ntiles_x + FOS_XCDS - 1) / FOS_XCDS;
+#define EMIT_K(T)                                                             \
+  (zw == 64 ? k_face_emit_lds<T, 64> : zw == 32 ? k_face_emit_lds<T, 32> : \
+              k_face_emit_lds<T, 16>)
+    auto kern = tx == 1   ? EMIT_K(1)
+                : tx == 2 ? EMIT_K(2)
+                : tx == 4 ? EMIT_K(4)
+                : tx == 8 ? EMIT_K(8)
+                          : EMIT_K(16);
+#undef EMIT_K
     hipLaunchKernelGGL(kern, dim3(per_x * FOS_XCDS), dim3(256), 0, st, res, n, filtered,
                        ctx->tab_b, ctx->tab_k, ctx->tab_c, ctx->tab_cap, ctx->flags, nyb, nzb,
                        ntiles_x, per_x);
@@ -1576,9 +1632,13 @@ int fos_set_param(const char *name, int value) {
     FOS_REQUIRE(value == 0 || value == 1, "jfa_kernel must be 0 or 1");
     g_kernel = value;
   } else if (!strcmp(name, "face_emit_tx")) {
-    FOS_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4,
-                "face_emit_tx must be 0, 1, 2 or 4");
+    FOS_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4 || value == 8 ||
+                    value == 16,
+                "face_emit_tx must be 0, 1, 2, 4, 8 or 16");
     g_emit_tx = value;
+  } else if (!strcmp(name, "face_emit_zw")) {
+    FOS_REQUIRE(value == 16 || value == 32 || value == 64, "face_emit_zw must be 16, 32 or 64");
+    g_emit_zw = value;
   } else if (!strcmp(name, "grid_block")) {
     FOS_REQUIRE(value == 0 || value == 1 || value == 2, "grid_block must be 0, 1 or 2");
     g_grid_block = value;
@@ -1602,6 +1662,8 @@ int fos_get_param(const char *name, int *value) {
     *value = g_grid_block;
   else if (!strcmp(name, "face_emit_tx"))
     *value = g_emit_tx;
+  else if (!strcmp(name, "face_emit_zw"))
+    *value = g_emit_zw;
   else
     FOS_REQUIRE(false, "fos_get_param: unknown parameter '%s'", name);
   return 0;
@@ -1771,21 +1833,27 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
 
 }  // extern "C"
 
+// Per-pair change flags of the fixed point: a ring at ctx->flags + 16.
+#define FIN_RING 48
+#define FIN_BURST 4
+static inline int *fin_chg(fos_ctx *ctx) { return ctx->flags + 16; }
+
 // Finalize in phases: prepare (classify, incoming lists, decisions cleared),
 // round pairs, materialize.
 static int fin_prepare(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
                        const int32_t *face_ids, const int32_t *face_counts,
                        const float *prev_scores, hipStream_t st) {
   const int64_t m = (int64_t)n * FOS_MAX_NEIGHBORS;
-  int rc = fos_fill_i32(ctx->in_count, n + 1, 0, st);
-  if (rc) return rc;
+  // the fixed point starts from "nothing accepted"
+  hipLaunchKernelGGL(k_fin_clear, dim3(blocks_for((int64_t)(n + 1 > FIN_RING ? n + 1 : FIN_RING))),
+                     dim3(FOS_BLOCK), 0, st, n, ctx->in_count, ctx->in_fill,
+                     (unsigned *)ctx->dec, fin_chg(ctx), FIN_RING);
+  FOS_CHECK_LAUNCH("k_fin_clear");
   hipLaunchKernelGGL(k_fin_classify, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
                      rad, voxel, pvox, face_ids, face_counts, prev_scores, ctx->score,
                      ctx->cls, ctx->in_count);
   FOS_CHECK_LAUNCH("k_fin_classify");
-  rc = exclusive_scan(ctx, ctx->in_count, ctx->in_start, n + 1, st);
-  if (rc) return rc;
-  rc = fos_fill_i32(ctx->in_fill, n, 0, st);
+  int rc = exclusive_scan(ctx, ctx->in_count, ctx->in_start, n + 1, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_fin_scatter, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
                      face_ids, ctx->cls, ctx->in_start, ctx->in_fill, ctx->in_src);
@@ -1793,67 +1861,75 @@ static int fin_prepare(fos_ctx *ctx, int n, const float *rad, float voxel, const
   hipLaunchKernelGGL(k_fin_rank, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n, face_ids,
                      ctx->in_start, ctx->in_src, ctx->in_sorted);
   FOS_CHECK_LAUNCH("k_fin_rank");
-  // the fixed point starts from "nothing accepted"
-  return fos_fill_i32((int32_t *)ctx->dec, n, 0, st);
+  return 0;
 }
 
-// `pairs` round pairs with no host round trip: rounds alternate between two
-// packed decision buffers (the result is back in the first after a pair); a
-// pair that changes nothing sets the done flag (ctx->flags[2]) and later
-// rounds skip.  With `last_failed`, the last pair marks ctx->flags[3] when
-// the fixed point was not reached.  The caller clears flags[1..2] first.
-static int fin_pairs(fos_ctx *ctx, int n, int pairs, bool last_failed, hipStream_t st) {
+// Round pairs k0 .. k0 + pairs - 1 with no host round trip: rounds alternate
+// between two packed decision buffers (the result is back in the first after
+// a pair).  Pair k's change flag is fin_chg(ctx)[k % FIN_RING]; a pair after
+// one that changed nothing skips.  With `last_failed`, the last pair marks
+// ctx->flags[3] (sticky) when it changed anything.  Flags are cleared by
+// fin_prepare; a caller going past FIN_RING pairs clears the ring itself
+// (fin_ring_wrap).
+static int fin_pairs(fos_ctx *ctx, int n, int k0, int pairs, bool last_failed, hipStream_t st) {
   unsigned *dec_a = (unsigned *)ctx->dec, *dec_b = dec_a + ctx->max_n;
-  for (int pair = 0; pair < pairs; ++pair) {
+  int *chg = fin_chg(ctx);
+  for (int k = k0; k < k0 + pairs; ++k) {
+    const int *prev = k > 0 ? chg + (k - 1) % FIN_RING : nullptr;
+    int *cur = chg + k % FIN_RING;
+    int *failed = last_failed && k == k0 + pairs - 1 ? ctx->flags + 3 : nullptr;
     hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, ctx->flags + 1,
-                       (const int *)(ctx->flags + 2));
+                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, prev, cur,
+                       (int *)nullptr);
     hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, ctx->flags + 1,
-                       (const int *)(ctx->flags + 2));
-    hipLaunchKernelGGL(k_fin_pair_end, dim3(1), dim3(1), 0, st, ctx->flags + 1, ctx->flags + 2,
-                       last_failed && pair == pairs - 1 ? ctx->flags + 3 : nullptr);
+                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, prev, cur, failed);
     FOS_CHECK_LAUNCH("k_fin_round");
   }
   return 0;
 }
 
+// Before pairs k0 .. with k0 a multiple of FIN_RING (> 0): clears every
+// ring slot but the last one (pair k0 - 1's flag, which pair k0 reads).
+static int fin_ring_wrap(fos_ctx *ctx, int k0, hipStream_t st) {
+  if (k0 == 0 || k0 % FIN_RING) return 0;
+  return fos_fill_i32(fin_chg(ctx), FIN_RING - 1, 0, st);
+}
+
 static int fin_materialize(fos_ctx *ctx, int n, const int32_t *face_ids, float *prev_scores,
                            int32_t *neighbor_list, int32_t *neighbor_count, int32_t *used_count,
                            int32_t *fsc, hipStream_t st) {
-  hipLaunchKernelGGL(k_fin_materialize, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, n,
+  hipLaunchKernelGGL(k_fin_materialize, dim3((unsigned)((n + 7) / 8)), dim3(256), 0, st, n,
                      face_ids, ctx->score, ctx->cls, (const unsigned *)ctx->dec, ctx->in_start,
                      ctx->in_sorted, prev_scores, neighbor_list, neighbor_count, used_count, fsc);
   FOS_CHECK_LAUNCH("k_fin_materialize");
   return 0;
 }
 
-// Reads ctx->flags[0..3] into host (one round trip).
-static int read_flags(fos_ctx *ctx, int out[4], hipStream_t st) {
-  FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+// Reads ctx->flags (all 64 ints: overflow, sticky failed, the pair ring)
+// into host memory (one round trip).
+static int read_flags(fos_ctx *ctx, int out[64], hipStream_t st) {
+  FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, 64 * sizeof(int), hipMemcpyDeviceToHost, st));
   FOS_HIP(hipStreamSynchronize(st));
   const volatile int *h = (const volatile int *)ctx->host_i;
-  for (int k = 0; k < 4; ++k) out[k] = h[k];
+  for (int k = 0; k < 64; ++k) out[k] = h[k];
   return 0;
 }
 
-// Round pairs until the fixed point: bursts of FIN_BURST pairs with
-// done-skips, the host reading the done flag after each burst (one round
-// trip for the usual two or three pairs).
-#define FIN_BURST 4
-static int fin_pairs_host(fos_ctx *ctx, int n, hipStream_t st) {
-  int rc = fos_fill_i32(ctx->flags + 1, 2, 0, st);
-  if (rc) return rc;
-  for (int done = 0, total = 0; !done; total += FIN_BURST) {
-    FOS_REQUIRE(total <= n + 1, "finalize: no fixed point after %d round pairs", total);
-    rc = fin_pairs(ctx, n, FIN_BURST, false, st);
+// Bursts of FIN_BURST round pairs from pair k0 until the fixed point (a
+// burst whose last pair changed nothing), the host reading the flags after
+// each burst.  `known`: the flags of a burst already read (then k0 is the
+// pair after it); NULL starts with a burst.
+static int fin_pairs_host(fos_ctx *ctx, int n, int k0, const int *known, hipStream_t st) {
+  int f[64];
+  if (known && !known[16 + (k0 - 1) % FIN_RING]) return 0;
+  for (int k = k0;; k += FIN_BURST) {
+    FOS_REQUIRE(k <= n + 1, "finalize: no fixed point after %d round pairs", k);
+    int rc = fin_ring_wrap(ctx, k, st);
+    if (!rc) rc = fin_pairs(ctx, n, k, FIN_BURST, false, st);
+    if (!rc) rc = read_flags(ctx, f, st);
     if (rc) return rc;
-    int f[4];
-    rc = read_flags(ctx, f, st);
-    if (rc) return rc;
-    done = f[2];
+    if (!f[16 + (k + FIN_BURST - 1) % FIN_RING]) return 0;
   }
-  return 0;
 }
 
 // fixed_pairs > 0: that many round pairs with no host round trip (flags[3]
@@ -1868,11 +1944,11 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
   int rc = fin_prepare(ctx, n, rad, voxel, pvox, face_ids, face_counts, prev_scores, st);
   if (rc) return rc;
   if (fixed_pairs > 0) {
-    rc = fos_fill_i32(ctx->flags + 1, 2, 0, stream);
-    if (rc) return rc;
-    rc = fin_pairs(ctx, n, fixed_pairs, true, st);
+    FOS_REQUIRE(fixed_pairs <= FIN_RING, "finalize: fixed_pairs %d above %d", fixed_pairs,
+                FIN_RING);
+    rc = fin_pairs(ctx, n, 0, fixed_pairs, true, st);
   } else {
-    rc = fin_pairs_host(ctx, n, st);
+    rc = fin_pairs_host(ctx, n, 0, nullptr, st);
   }
   if (rc) return rc;
   return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
@@ -1903,10 +1979,9 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
     int rc = face_emit(ctx, res, n, filtered, st);
     if (!rc) rc = face_compact(ctx, n, face_ids, face_counts, st);
     if (!rc) rc = fin_prepare(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores, st);
-    if (!rc) rc = fos_fill_i32(ctx->flags + 1, 2, 0, st);
-    if (!rc) rc = fin_pairs(ctx, n, FIN_BURST, false, st);
+    if (!rc) rc = fin_pairs(ctx, n, 0, FIN_BURST, false, st);
     if (rc) return rc;
-    int f[4];
+    int f[64];
     rc = read_flags(ctx, f, st);
     if (rc) return rc;
     if (f[0]) {
@@ -1914,15 +1989,8 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
       if (rc) return rc;
       continue;
     }
-    if (!f[2]) {
-      for (int done = 0, total = FIN_BURST; !done; total += FIN_BURST) {
-        FOS_REQUIRE(total <= n + 1, "finalize: no fixed point after %d round pairs", total);
-        rc = fin_pairs(ctx, n, FIN_BURST, false, st);
-        if (!rc) rc = read_flags(ctx, f, st);
-        if (rc) return rc;
-        done = f[2];
-      }
-    }
+    rc = fin_pairs_host(ctx, n, FIN_BURST, f, st);
+    if (rc) return rc;
     return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
                            used_count, fsc, st);
   }

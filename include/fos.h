@@ -45,9 +45,11 @@ const char *fos_last_error(void);
  *   "jfa_dedup"        0,1,4,8  (jfa_kernel 1) 0: skip candidates by label
  *                      comparison and batch the seed gathers; 1/4/8: skip
  *                      owners among that many last evaluated
- *   "face_emit_tx"     0,1,2,4  face-pair emission: 0 merges pairs within a
- *                      wave; 1/2/4 merge within a workgroup tile of that
+ *   "face_emit_tx"     0,1,2,4,8,16  face-pair emission: 0 merges pairs
+ *                      within a wave; others merge within a workgroup tile of that
  *                      many x-planes in LDS (default 2)
+ *   "face_emit_zw"     16,32,64  z-extent of the emission tile (its y-extent
+ *                      is 256 / zw rows; default 64)
  *   "grid_block"       0,1,2  hash-grid rebuild: 0 count/scan/scatter/rank
  *                      kernels; 1 one workgroup when n <= 16256 and cells
  *                      <= 16384, else as 2; 2 one workgroup per range of

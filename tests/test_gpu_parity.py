@@ -241,7 +241,14 @@ def gpu_faces(gpu, lab, res, n):
     return host(filt), host(ids), host(cts)
 
 
-EMIT_TX = [0, 1, 2, 4]
+# (x-planes, z-extent) of the emission tile; x-planes 0 is the wave kernel
+EMIT_TX = [(0, 64), (1, 64), (2, 64), (4, 64), (2, 32), (4, 16), (8, 16), (16, 16), (16, 32)]
+
+
+def with_emit(gpu, shape, fn):
+    tx, zw = shape
+    return with_param(gpu, "face_emit_tx", tx,
+                      lambda: with_param(gpu, "face_emit_zw", zw, fn))
 
 
 @pytest.mark.parametrize("tx", EMIT_TX)
@@ -251,7 +258,7 @@ def test_majority_and_faces_c1(gpu, orc, tx):
     lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, fc.f32(1 / 64), 7)
     f = orc.majority(lab, res)
     ids, cts = orc.accumulate_faces(f, res, n)
-    gf, gids, gcts = with_param(gpu, "face_emit_tx", tx, lambda: gpu_faces(gpu, lab, res, n))
+    gf, gids, gcts = with_emit(gpu, tx, lambda: gpu_faces(gpu, lab, res, n))
     assert (gf == f).all()
     assert (gids == ids).all() and (gcts == cts).all()
 
@@ -275,10 +282,10 @@ def test_faces_row_truncation_and_table_growth(gpu, orc, tx):
         g.call("fos_face_accumulate", g.h, res, n, g.ptr(tl), g.ptr(gids), g.ptr(gcts),
                gpu.stream())
         return host(gids), host(gcts)
-    a, b = with_param(gpu, "face_emit_tx", tx, run)
+    a, b = with_emit(gpu, tx, run)
     assert (a == ids).all() and (b == cts).all()
     # the grown tables keep working on the next call
-    a, b = with_param(gpu, "face_emit_tx", tx, run)
+    a, b = with_emit(gpu, tx, run)
     assert (a == ids).all() and (b == cts).all()
 
 
@@ -299,7 +306,7 @@ def test_faces_random_labels(gpu, orc, tx):
         g.call("fos_face_accumulate", g.h, res, n, g.ptr(tl), g.ptr(gids), g.ptr(gcts),
                gpu.stream())
         return host(gids), host(gcts)
-    a, b = with_param(gpu, "face_emit_tx", tx, run)
+    a, b = with_emit(gpu, tx, run)
     assert (a == ids).all() and (b == cts).all()
 
 
