@@ -179,6 +179,7 @@ struct fos_ctx {
   unsigned *tab_k;
   int *tab_c;
   int *flags;  // [0] row-table overflow, [3] sticky finalize failure,
+               // [4..5] warm-start gate counters (zero between calls),
                // [16..63] per-pair change flags of the finalize fixed point
   // finalize scratch
   float *score;           // [max_n*32]

@@ -73,7 +73,8 @@ __global__ void __launch_bounds__(256)
                float inv_l, float gap, float mdf, float dmul,
                const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
                const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
-               const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass) {
+               const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass,
+               int32_t *__restrict__ cell_of) {
   __shared__ float s_v[8][27][PBD_SLOTS][3];
   __shared__ int s_cnt[8][27];
   const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
@@ -154,9 +155,20 @@ __global__ void __launch_bounds__(256)
     ay = ay * s;
     az = az * s;
   }
-  pout[3 * i] = gwrap(px + ax, L, inv_l);
-  pout[3 * i + 1] = gwrap(py + ay, L, inv_l);
-  pout[3 * i + 2] = gwrap(pz + az, L, inv_l);
+  float w0 = gwrap(px + ax, L, inv_l), w1 = gwrap(py + ay, L, inv_l),
+        w2 = gwrap(pz + az, L, inv_l);
+  if (!BATCH && cell_of) {
+    // the cell step of the rebuild that follows (grid.py:285-309): wrap
+    // again and hash
+    w0 = gwrap(w0, L, inv_l);
+    w1 = gwrap(w1, L, inv_l);
+    w2 = gwrap(w2, L, inv_l);
+    cell_of[i] = hash_axis(w0, G, half_l, inv_l) * G * G + hash_axis(w1, G, half_l, inv_l) * G +
+                 hash_axis(w2, G, half_l, inv_l);
+  }
+  pout[3 * i] = w0;
+  pout[3 * i + 1] = w1;
+  pout[3 * i + 2] = w2;
 }
 
 // Max overlap with 32 lanes per particle: the maximum is order-free, so the
@@ -270,16 +282,27 @@ __global__ void k_controller_stats(const float *__restrict__ rad,
 
 // run.py:1184-1195 warm-start gate on the device: *gate = 1 when the frame is
 // past the warm start and at most zero_rate of the particles have FSC 0.
-__global__ void k_count_zero(const int32_t *__restrict__ fsc, int n, int *__restrict__ count) {
+// Count and decision in one launch: every block adds its zeros to cnt[0];
+// the last block to finish (cnt[1] counts finished blocks) decides the gate
+// and resets both counters for the next call.
+__global__ void __launch_bounds__(FOS_BLOCK)
+    k_count_gate(const int32_t *__restrict__ fsc, int n, int frame, int warmstart,
+                 double zero_rate, int *__restrict__ cnt, float *__restrict__ gate) {
+  __shared__ int s_last;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = __popcll(__ballot(i < n && fsc[i] == 0));
-  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
-}
-
-__global__ void k_gate(const int *__restrict__ count, int n, int frame, int warmstart,
-                       double zero_rate, float *__restrict__ gate) {
-  const double f0 = (double)*count / (double)(n > 1 ? n : 1);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(&cnt[0], c);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(&cnt[1], 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  __threadfence();
+  const int zeros = atomicAdd(&cnt[0], 0);
+  const double f0 = (double)zeros / (double)(n > 1 ? n : 1);
   *gate = (frame < warmstart || f0 > zero_rate) ? 0.0f : 1.0f;
+  atomicExch(&cnt[0], 0);
+  atomicExch(&cnt[1], 0);
 }
 
 // dynamics.py:1399-1427
@@ -384,16 +407,23 @@ __device__ __forceinline__ void pressure_group(int i, int ih, int q, int n, int 
                                                float p_exp, float inv_p, float cap, int max_nei,
                                                float r_min, float r_max, float *stats) {
   const bool live = i < n;
+  // the row's 32 entries (two per lane) load beside fsc[i], so the
+  // neighbour's radius is the only load that waits on another
+  const int32_t *row = face_ids + (int64_t)(live ? i : 0) * FOS_MAX_NEIGHBORS;
+  const int e_lo = live ? row[q] : -1, e_hi = live ? row[q + 16] : -1;
   const int fc = live ? fsc[i] : 0;
   const int an = fc < max_nei ? fc : max_nei;
   const float ri = live ? rin[i] : 0.0f;
+  const int idx = fc > 0 ? (int)(((unsigned)lcg_hash(ih, frame) % (unsigned)fc + (unsigned)q) %
+                                 (unsigned)fc)
+                         : 0;
+  // every lane of the group takes part in both shuffles
+  const int j_lo = __shfl(e_lo, idx & 15, 16), j_hi = __shfl(e_hi, idx & 15, 16);
   float term = 0.0f;
   bool has = false;
   if (live && fc > 0 && an <= 16 && q < an) {
     const float vi = vol_f32(ri, p_exp);
-    const int idx = (int)(((unsigned)lcg_hash(ih, frame) % (unsigned)fc + (unsigned)q) %
-                          (unsigned)fc);
-    const int j = idx < FOS_MAX_NEIGHBORS ? face_ids[(int64_t)i * FOS_MAX_NEIGHBORS + idx] : -1;
+    const int j = idx < 16 ? j_lo : idx < FOS_MAX_NEIGHBORS ? j_hi : -1;
     if (!(j < 0 || j >= n || j == i)) {
       const float vj = vol_f32(rin[j], p_exp);
       const float raw = k * (vi - vj);
@@ -641,7 +671,27 @@ int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad, int n,
   hipLaunchKernelGGL(k_pbd_wave<false>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                      (hipStream_t)stream, pos_in, pos_out, rad, n, G, L, (float)(0.5 * (double)L),
                      (float)(1.0 / (double)L), gap_factor, max_disp_frac, disp_mult,
-                     cell_start, cell_count, cell_indices, nullptr, nullptr, nullptr, 0);
+                     cell_start, cell_count, cell_indices, nullptr, nullptr, nullptr, 0,
+                     (int32_t *)nullptr);
+  FOS_CHECK_LAUNCH("k_pbd");
+  return 0;
+}
+
+int fos_pbd_pass_cells(fos_ctx *ctx, const float *pos_in, float *pos_out, const float *rad,
+                       int n, int G, float L, float gap_factor, float max_disp_frac,
+                       float disp_mult, const int32_t *cell_start, const int32_t *cell_count,
+                       const int32_t *cell_indices, void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && n <= ctx->max_n && G >= 3 &&
+                  (int64_t)G * G * G <= ctx->max_cells && L > 0.0f &&
+                  (n == 0 || (pos_in && pos_out && pos_in != pos_out && rad && cell_start &&
+                              cell_count && cell_indices)),
+              "fos_pbd_pass_cells: bad arguments (n=%d G=%d)", n, G);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_pbd_wave<false>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                     (hipStream_t)stream, pos_in, pos_out, rad, n, G, L, (float)(0.5 * (double)L),
+                     (float)(1.0 / (double)L), gap_factor, max_disp_frac, disp_mult,
+                     cell_start, cell_count, cell_indices, nullptr, nullptr, nullptr, 0,
+                     ctx->cell_of);
   FOS_CHECK_LAUNCH("k_pbd");
   return 0;
 }
@@ -696,17 +746,11 @@ int fos_fsc_targets_gated(const float *rad, float *rad_target, const int32_t *fs
 int fos_fsc_gate(fos_ctx *ctx, const int32_t *fsc, int n, int frame, int warmstart_frames,
                  double fsc_zero_rate, float *gate, void *stream) {
   FOS_REQUIRE(ctx && gate && n >= 0 && (n == 0 || fsc), "fos_fsc_gate: bad arguments");
-  hipStream_t st = (hipStream_t)stream;
-  int rc = fos_fill_i32(ctx->iscratch, 1, 0, stream);
-  if (rc) return rc;
-  if (n > 0) {
-    hipLaunchKernelGGL(k_count_zero, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, fsc, n,
-                       ctx->iscratch);
-    FOS_CHECK_LAUNCH("k_count_zero");
-  }
-  hipLaunchKernelGGL(k_gate, dim3(1), dim3(1), 0, st, (const int *)ctx->iscratch, n, frame,
-                     warmstart_frames, fsc_zero_rate, gate);
-  FOS_CHECK_LAUNCH("k_gate");
+  // counters: ctx->flags[4..5], zero between calls
+  hipLaunchKernelGGL(k_count_gate, dim3(n > 0 ? blocks_for(n) : 1), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, fsc, n, frame, warmstart_frames, fsc_zero_rate,
+                     ctx->flags + 4, gate);
+  FOS_CHECK_LAUNCH("k_count_gate");
   return 0;
 }
 
@@ -1338,7 +1382,7 @@ int fos_batch_pbd_pass(fos_batch *bt, const float *pos_in, float *pos_out, const
     hipLaunchKernelGGL(k_pbd_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                        (hipStream_t)stream, pos_in, pos_out, rad, n, 3, L, half_l, inv_l,
                        gap_factor, max_disp_frac, disp_mult, cell_start, cell_count,
-                       cell_indices, bt->pinst, bt->inst, passes, pass);
+                       cell_indices, bt->pinst, bt->inst, passes, pass, (int32_t *)nullptr);
   FOS_CHECK_LAUNCH("k_pbd_b");
   return 0;
 }

@@ -499,6 +499,29 @@ int fos_grid_scatter(float *pos, int n, int G, float L, int32_t *cell_write,
   return 0;
 }
 
+int fos_grid_build_from_cells(fos_ctx *ctx, int n, int G, int32_t *cell_count,
+                              int32_t *cell_start, int32_t *cell_indices, void *stream) {
+  FOS_REQUIRE(ctx && G >= 3 && (int64_t)G * G * G <= ctx->max_cells && n >= 0 &&
+                  n <= ctx->max_n && cell_count && cell_start && (n == 0 || cell_indices),
+              "fos_grid_build_from_cells: bad arguments (n=%d G=%d)", n, G);
+  const int C = G * G * G;
+  hipStream_t st = (hipStream_t)stream;
+  if (n >= 1 && n <= GRID_BLOCK_N && C <= GRID_BLOCK_C && g_grid_block == 1) {
+    hipLaunchKernelGGL(k_grid_build_block, dim3(1), dim3(1024), 0, st,
+                       (const int32_t *)ctx->cell_of, n, C, cell_count, cell_start,
+                       cell_indices);
+    FOS_CHECK_LAUNCH("k_grid_build_block");
+    return 0;
+  }
+  const int cpw = fos_range_cells(C);
+  const int wgs = (C + cpw - 1) / cpw;
+  hipLaunchKernelGGL(k_grid_build_ranges, dim3((unsigned)wgs), dim3(1024), 0, st,
+                     (const int32_t *)ctx->cell_of, n, C, cpw, cell_count, cell_start,
+                     cell_indices);
+  FOS_CHECK_LAUNCH("k_grid_build_ranges");
+  return 0;
+}
+
 int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
                    int32_t *cell_count, int32_t *cell_start,
                    int32_t *cell_indices, void *stream) {

@@ -1459,8 +1459,8 @@ static int g_lds_max_step = 0;
 static int g_tile_x = 4;
 static int g_dedup = 0;
 static int g_kernel = 0;
-static int g_emit_tx = 2;
-static int g_emit_zw = 64;
+static int g_emit_tx = 8;
+static int g_emit_zw = 32;
 extern int g_grid_block;  // fos_grid.hip
 
 template <bool U, int K>

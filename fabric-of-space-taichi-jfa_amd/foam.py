@@ -133,13 +133,23 @@ class FoamSim:
                 fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
                 fl.ptr(self.lmd), fl.ptr(self.maxd), fl.stream())
 
-    def pbd_pass(self):
+    def pbd_pass(self, cells=False):
+        """One PBD pass; cells=True also runs the wrap-and-hash step of the
+        grid rebuild that follows it (finish with rebuild_from_cells)."""
         c = self.cfg
-        fl.call("fos_pbd_pass", fl.ptr(self.pos), fl.ptr(self.pos_tmp), fl.ptr(self.rad),
-                self.n, self.G, c.L, c.gap_factor(), c.max_disp_frac, c.disp_mult,
-                fl.ptr(self.cell_start), fl.ptr(self.cell_count), fl.ptr(self.cell_indices),
-                fl.stream())
+        args = (fl.ptr(self.pos), fl.ptr(self.pos_tmp), fl.ptr(self.rad), self.n, self.G, c.L,
+                c.gap_factor(), c.max_disp_frac, c.disp_mult, fl.ptr(self.cell_start),
+                fl.ptr(self.cell_count), fl.ptr(self.cell_indices), fl.stream())
+        if cells:
+            fl.call("fos_pbd_pass_cells", self.ctx.handle, *args)
+        else:
+            fl.call("fos_pbd_pass", *args)
         self.pos, self.pos_tmp = self.pos_tmp, self.pos
+
+    def rebuild_from_cells(self):
+        fl.call("fos_grid_build_from_cells", self.ctx.handle, self.n, self.G,
+                fl.ptr(self.cell_count), fl.ptr(self.cell_start), fl.ptr(self.cell_indices),
+                fl.stream())
 
     def pressure(self, frame):
         c = self.cfg
@@ -217,10 +227,16 @@ class FoamSim:
         if c.brownian:
             fl.call("fos_brownian", fl.ptr(self.pos), self.n, c.brownian_strength, c.L,
                     c.seed, f, c.brownian_mode, st)
-        # 5 PBD
-        for _ in range(passes):
-            self.rebuild_grid()
-            self.pbd_pass()
+        # 5 PBD; a pass followed directly by a rebuild runs that rebuild's
+        # wrap-and-hash step itself
+        ready = False
+        for p in range(passes):
+            if ready:
+                self.rebuild_from_cells()
+            else:
+                self.rebuild_grid()
+            ready = p < passes - 1 or not c.ou_jitter
+            self.pbd_pass(cells=ready)
         # 6 OU jitter
         if c.ou_jitter:
             mean_r = float(self.rad.mean().item())
@@ -228,7 +244,10 @@ class FoamSim:
                     self.n, mean_r, fc.DT, fc.JITTER_RMS, fc.JITTER_TAU,
                     fc.f32(fc.MAX_DRIFT_FRACTION * c.gap_fraction), c.L, c.seed, f, st)
         # 7
-        self.rebuild_grid()
+        if ready:
+            self.rebuild_from_cells()
+        else:
+            self.rebuild_grid()
         # 8 JFA on cadence frames (or the predictor between judge frames)
         if self.predictor is not None and c.judge_every > 0:
             ran = self.fallback or f % c.judge_every == 0

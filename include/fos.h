@@ -47,9 +47,9 @@ const char *fos_last_error(void);
  *                      owners among that many last evaluated
  *   "face_emit_tx"     0,1,2,4,8,16  face-pair emission: 0 merges pairs
  *                      within a wave; others merge within a workgroup tile of that
- *                      many x-planes in LDS (default 2)
+ *                      many x-planes in LDS (default 8)
  *   "face_emit_zw"     16,32,64  z-extent of the emission tile (its y-extent
- *                      is 256 / zw rows; default 64)
+ *                      is 256 / zw rows; default 32)
  *   "grid_block"       0,1,2  hash-grid rebuild: 0 count/scan/scatter/rank
  *                      kernels; 1 one workgroup when n <= 16256 and cells
  *                      <= 16384, else as 2; 2 one workgroup per range of
@@ -204,6 +204,19 @@ int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad,
                  float disp_mult, const int32_t *cell_start,
                  const int32_t *cell_count, const int32_t *cell_indices,
                  void *stream);
+/* fos_pbd_pass followed by the first step of the fos_grid_build that comes
+ * next in the frame (grid.py:285-309 position wrap and cell index, kept in
+ * the context), fused into the PBD kernel; fos_grid_build_from_cells then
+ * completes that rebuild.  Results equal fos_pbd_pass + fos_grid_build
+ * bit for bit.  No reference counterpart (a launch saved per PBD pass). */
+int fos_pbd_pass_cells(fos_ctx *ctx, const float *pos_in, float *pos_out, const float *rad,
+                       int n, int G, float L, float gap_factor, float max_disp_frac,
+                       float disp_mult, const int32_t *cell_start, const int32_t *cell_count,
+                       const int32_t *cell_indices, void *stream);
+/* The rest of fos_grid_build (grid.py:189-309 count, prefix sum, scatter)
+ * from the cells fos_pbd_pass_cells left in the context. */
+int fos_grid_build_from_cells(fos_ctx *ctx, int n, int G, int32_t *cell_count,
+                              int32_t *cell_start, int32_t *cell_indices, void *stream);
 /* dynamics.py:304-372 compute_local_max_overlaps + reduce_max_depth: fills
  * local_max_depth and the device float *max_depth (a true maximum). */
 int fos_max_overlap(const float *pos, const float *rad, int n, int G, float L,

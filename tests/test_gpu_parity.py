@@ -522,6 +522,35 @@ def test_pbd_maxoverlap_bit_exact(gpu, orc):
     assert np.array_equal(host(tl), lmd) and float(host(tm)[0]) == g
 
 
+@pytest.mark.parametrize("block", [2, 1])
+def test_pbd_pass_cells_equals_pass_then_rebuild(gpu, orc, block):
+    """The PBD pass that also runs the next rebuild's wrap-and-hash step,
+    finished by fos_grid_build_from_cells, equals fos_pbd_pass followed by
+    a full rebuild (the oracle's), including particles pushed across the
+    box faces (wrapped twice, as the reference does)."""
+    n, G_ = 3000, 14
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=5)
+    pos[:40, 0] = F(0.4999)                   # crowd the +x face
+    pos[40:80, 1] = F(-0.4999)
+    p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
+    gap = fc.f32(1.015)
+    moved = orc.pbd_jacobi(p, rad, n, G_, 1.0, gap, 0.2, 2.0, cs, cc, ci)
+    want = orc.grid_build(moved, n, G_, 1.0)
+
+    def run():
+        g = G(gpu, n, 1, G_ ** 3)
+        tp, tcc, tcs, tci = gpu_grid(gpu, pos, n, G_)
+        out = torch.empty_like(tp)
+        g.call("fos_pbd_pass_cells", g.h, g.ptr(tp), g.ptr(out), g.ptr(dev(rad)), n, G_, 1.0,
+               gap, 0.2, 2.0, g.ptr(tcs), g.ptr(tcc), g.ptr(tci), gpu.stream())
+        g.call("fos_grid_build_from_cells", g.h, n, G_, g.ptr(tcc), g.ptr(tcs), g.ptr(tci),
+               gpu.stream())
+        return host(out), host(tcc), host(tcs), host(tci)
+    got = with_param(gpu, "grid_block", block, run)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
 def test_controller_kernels(gpu, orc):
     n = 5000
     rng = np.random.default_rng(6)
