@@ -53,7 +53,8 @@ int fos_batch_create(fos_batch **out, int B, const int *n, const int *res, const
     es[b] = (int)e;
     gs[b] = (int)g;
     t += r * nyb * nzb;
-    e += ((r + 1) / 2) * nyb * nzb;
+    e += ((r + EMIT_B_TX - 1) / EMIT_B_TX) * ((r + 256 / EMIT_B_ZW - 1) / (256 / EMIT_B_ZW)) *
+         ((r + EMIT_B_ZW - 1) / EMIT_B_ZW);
     g += (C + fos_range_cells((int)C) - 1) / fos_range_cells((int)C);
     v += r * r * r;
     p += n[b];

@@ -167,6 +167,7 @@ struct fos_ctx {
   int max_n;
   int max_res;
   int max_cells;
+  int fin_burst;  // round pairs the last finalize needed (0: none yet); sizes the first burst
   // seed table (x, y, z, (beta*r)^2)
   float4 *seeds;
   // per-wave change masks of the last two JFA passes, [max_res^2][ceil(max_res/64)]
@@ -224,6 +225,10 @@ struct fos_inst {
 };
 
 #define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
+// Face-emission tile of a batched foam: EMIT_B_TX x-planes x (256 /
+// EMIT_B_ZW) y-rows x EMIT_B_ZW z-voxels per workgroup.
+#define EMIT_B_TX 8
+#define EMIT_B_ZW 32
 
 // Cells per grid-rebuild workgroup for a hash grid of C cells: about 192
 // workgroups per grid, at least 64 and at most 1024 cells each.
@@ -240,7 +245,7 @@ struct fos_batch {
   fos_inst *inst_h;        // host [B]
   int *tstart;             // device [B+1] first JFA pass tile (row or flat, per foam)
   int *mstart;             // device [B+1] first majority tile (1 x 4 x 64 voxels)
-  int *estart;             // device [B+1] first face-emission tile (2 x 4 x 64 voxels)
+  int *estart;             // device [B+1] first face-emission tile (EMIT_B_* shape)
   int *gstart;             // device [B+1] first grid-rebuild workgroup
   int *tstart_h, *mstart_h, *estart_h, *gstart_h;  // host copies
   int *pinst;              // device [n_total] foam of each particle

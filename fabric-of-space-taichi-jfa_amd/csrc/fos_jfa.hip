@@ -1870,7 +1870,7 @@ static int fin_prepare(fos_ctx *ctx, int n, const float *rad, float voxel, const
 // one that changed nothing skips.  With `last_failed`, the last pair marks
 // ctx->flags[3] (sticky) when it changed anything.  Flags are cleared by
 // fin_prepare; a caller going past FIN_RING pairs clears the ring itself
-// (fin_ring_wrap).
+// (fin_ring_clear).
 static int fin_pairs(fos_ctx *ctx, int n, int k0, int pairs, bool last_failed, hipStream_t st) {
   unsigned *dec_a = (unsigned *)ctx->dec, *dec_b = dec_a + ctx->max_n;
   int *chg = fin_chg(ctx);
@@ -1888,11 +1888,13 @@ static int fin_pairs(fos_ctx *ctx, int n, int k0, int pairs, bool last_failed, h
   return 0;
 }
 
-// Before pairs k0 .. with k0 a multiple of FIN_RING (> 0): clears every
-// ring slot but the last one (pair k0 - 1's flag, which pair k0 reads).
-static int fin_ring_wrap(fos_ctx *ctx, int k0, hipStream_t st) {
-  if (k0 == 0 || k0 % FIN_RING) return 0;
-  return fos_fill_i32(fin_chg(ctx), FIN_RING - 1, 0, st);
+// Clears every ring slot but `keep` (the flag of the pair before the next
+// burst, which that burst's first pair reads).
+static int fin_ring_clear(fos_ctx *ctx, int keep, hipStream_t st) {
+  int rc = keep > 0 ? fos_fill_i32(fin_chg(ctx), keep, 0, st) : 0;
+  if (!rc && keep + 1 < FIN_RING)
+    rc = fos_fill_i32(fin_chg(ctx) + keep + 1, FIN_RING - keep - 1, 0, st);
+  return rc;
 }
 
 static int fin_materialize(fos_ctx *ctx, int n, const int32_t *face_ids, float *prev_scores,
@@ -1915,20 +1917,50 @@ static int read_flags(fos_ctx *ctx, int out[64], hipStream_t st) {
   return 0;
 }
 
-// Bursts of FIN_BURST round pairs from pair k0 until the fixed point (a
-// burst whose last pair changed nothing), the host reading the flags after
-// each burst.  `known`: the flags of a burst already read (then k0 is the
-// pair after it); NULL starts with a burst.
-static int fin_pairs_host(fos_ctx *ctx, int n, int k0, const int *known, hipStream_t st) {
+// Pairs up to the first that changed nothing, from flags read after pair
+// `last` (the ring has not wrapped); FIN_BURST when it has.
+static int fin_needed(const int *f, int last) {
+  if (last >= FIN_RING) return FIN_BURST;
+  for (int k = 0; k <= last; ++k)
+    if (!f[16 + k]) return k + 1;
+  return FIN_BURST;
+}
+
+// The first burst's size: what the last finalize needed (consecutive frames
+// need about the same), 2 to 8.
+static int fin_first_burst(const fos_ctx *ctx) {
+  const int b = ctx->fin_burst > 0 ? ctx->fin_burst : FIN_BURST;
+  return b < 2 ? 2 : b > 8 ? 8 : b;
+}
+
+// Bursts of round pairs from pair k0 until the fixed point (a burst whose
+// last pair changed nothing), the host reading the flags after each burst:
+// the first of `first` pairs, then FIN_BURST.  `known`: the flags of a burst
+// already read (then k0 is the pair after it); NULL starts with a burst.
+// Records the pairs needed in ctx->fin_burst.
+static int fin_pairs_host(fos_ctx *ctx, int n, int k0, int first, const int *known,
+                          hipStream_t st) {
   int f[64];
-  if (known && !known[16 + (k0 - 1) % FIN_RING]) return 0;
-  for (int k = k0;; k += FIN_BURST) {
+  if (known && !known[16 + (k0 - 1) % FIN_RING]) {
+    ctx->fin_burst = fin_needed(known, k0 - 1);
+    return 0;
+  }
+  // pairs below lap_end have ring slots not used since fin_prepare cleared them
+  int lap_end = FIN_RING;
+  for (int k = k0, p = first;; k += p, p = FIN_BURST) {
     FOS_REQUIRE(k <= n + 1, "finalize: no fixed point after %d round pairs", k);
-    int rc = fin_ring_wrap(ctx, k, st);
-    if (!rc) rc = fin_pairs(ctx, n, k, FIN_BURST, false, st);
+    int rc = 0;
+    if (k + p > lap_end) {
+      rc = fin_ring_clear(ctx, (k - 1) % FIN_RING, st);
+      lap_end = k + FIN_RING - 1;
+    }
+    if (!rc) rc = fin_pairs(ctx, n, k, p, false, st);
     if (!rc) rc = read_flags(ctx, f, st);
     if (rc) return rc;
-    if (!f[16 + (k + FIN_BURST - 1) % FIN_RING]) return 0;
+    if (!f[16 + (k + p - 1) % FIN_RING]) {
+      ctx->fin_burst = fin_needed(f, k + p - 1);
+      return 0;
+    }
   }
 }
 
@@ -1948,7 +1980,7 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
                 FIN_RING);
     rc = fin_pairs(ctx, n, 0, fixed_pairs, true, st);
   } else {
-    rc = fin_pairs_host(ctx, n, 0, nullptr, st);
+    rc = fin_pairs_host(ctx, n, 0, fin_first_burst(ctx), nullptr, st);
   }
   if (rc) return rc;
   return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
@@ -1979,7 +2011,8 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
     int rc = face_emit(ctx, res, n, filtered, st);
     if (!rc) rc = face_compact(ctx, n, face_ids, face_counts, st);
     if (!rc) rc = fin_prepare(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores, st);
-    if (!rc) rc = fin_pairs(ctx, n, 0, FIN_BURST, false, st);
+    const int first = fin_first_burst(ctx);
+    if (!rc) rc = fin_pairs(ctx, n, 0, first, false, st);
     if (rc) return rc;
     int f[64];
     rc = read_flags(ctx, f, st);
@@ -1989,7 +2022,7 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
       if (rc) return rc;
       continue;
     }
-    rc = fin_pairs_host(ctx, n, FIN_BURST, f, st);
+    rc = fin_pairs_host(ctx, n, first, FIN_BURST, f, st);
     if (rc) return rc;
     return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
                            used_count, fsc, st);
@@ -2310,10 +2343,11 @@ __global__ void __launch_bounds__(256)
   if (T >= total) return;
   const int b = batch_find(estart, B, T);
   const int res = inst[b].res;
-  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
-  const int ntiles = ((res + 1) / 2) * nyb * nzb;
-  face_emit_tile<2>(T - estart[b], res, n, f + inst[b].v0, tb, tk, tc, cap, overflow, nyb, nzb,
-                    ntiles, s_pair, s_key, s_cnt);
+  const int nyb = (res + 256 / EMIT_B_ZW - 1) / (256 / EMIT_B_ZW),
+            nzb = (res + EMIT_B_ZW - 1) / EMIT_B_ZW;
+  const int ntiles = ((res + EMIT_B_TX - 1) / EMIT_B_TX) * nyb * nzb;
+  face_emit_tile<EMIT_B_TX, EMIT_B_ZW>(T - estart[b], res, n, f + inst[b].v0, tb, tk, tc, cap,
+                                       overflow, nyb, nzb, ntiles, s_pair, s_key, s_cnt);
 }
 
 extern "C" {

@@ -392,6 +392,36 @@ def test_finalize_chains_and_overflow(gpu, orc):
         assert np.array_equal(a, b), name
 
 
+def test_finalize_long_chain(gpu, orc):
+    """Row i's only slot (partner i + 1) has an ambiguous score, kept only
+    when row i - 1 appended to it: the acceptance travels one row per round,
+    so the fixed point takes ~n/2 round pairs -- several host bursts and more
+    pairs than the change-flag ring holds.  Two calls on one context (the
+    second sizes its first burst from the first)."""
+    n = 150
+    voxel = fc.f32(1 / 64)
+    rad = np.full(n, 0.02, F)              # expected area 1.5 * (0.04 * 64)^2 = 9.83
+    ids = np.full((n, 32), -1, np.int32)
+    cts = np.zeros((n, 32), np.int32)
+    ids[:n - 1, 0] = np.arange(1, n)
+    cts[:n - 1, 0] = 1                     # score 0.102: ambiguous
+    cts[0, 0] = 2                          # score 0.203: accepted
+    prev = np.zeros((n, 32), F)
+    o = orc.finalize_faces(n, rad, voxel, ids, cts, prev)
+    # the chain reached the last row
+    assert o[4][0] == 1 and (o[4][1:n - 1] == 2).all() and o[4][n - 1] == 1
+    g = G(gpu, n, 1)
+    for _ in range(2):
+        tp = dev(prev)
+        out = [torch.empty(n, 32, dtype=torch.int32, device="cuda")] + \
+            [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3)]
+        g.call("fos_finalize_faces", g.h, n, g.ptr(dev(rad)), voxel, g.ptr(dev(ids)),
+               g.ptr(dev(cts)), g.ptr(tp), *[g.ptr(t) for t in out], gpu.stream())
+        gout = [host(tp)] + [host(t) for t in out]
+        for a, b, name in zip(o, gout, ("prev", "nl", "nc", "used", "fsc")):
+            assert np.array_equal(a, b), name
+
+
 def test_ema_stats_validate(gpu, orc):
     n = 2000
     rng = np.random.default_rng(1)
