@@ -225,6 +225,14 @@ struct fos_inst {
 };
 
 #define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
+// Tile -> block mapping that keeps each XCD on a contiguous range of tiles
+// (workgroups are dispatched round-robin over the 8 XCDs, so consecutive
+// block ids land on different L2s).
+#define FOS_XCDS 8
+__device__ __forceinline__ int xcd_tile(int bid, int per_xcd) {
+  return (bid % FOS_XCDS) * per_xcd + bid / FOS_XCDS;
+}
+
 // Face-emission tile of a batched foam: EMIT_B_TX x-planes x (256 /
 // EMIT_B_ZW) y-rows x EMIT_B_ZW z-voxels per workgroup.
 #define EMIT_B_TX 8

@@ -1140,10 +1140,15 @@ __global__ void __launch_bounds__(256)
                float half_l, float inv_l, float gap, float mdf, float dmul,
                const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
                const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
-               const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass) {
+               const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass,
+               int per_xcd) {
   __shared__ float s_v[8][PBD_HITS][3];
   const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
-  const int i = blockIdx.x * 8 + grp;
+  // particles in cell order (slot k of the grid holds particle ci[k]): a
+  // workgroup's eight particles and an XCD's run of workgroups read the same
+  // neighbourhood of slot records from their L2
+  const int k8 = xcd_tile((int)blockIdx.x, per_xcd) * 8 + grp;
+  const int i = k8 < n ? ci[k8] : n;
   bool live = i < n;
   int G = 3;
   if (live) {
@@ -1241,9 +1246,12 @@ __global__ void __launch_bounds__(256)
                        const float *__restrict__ rad, int n, float L, float half_l, float inv_l,
                        const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
                        const int32_t *__restrict__ ci, float *__restrict__ lmd, float *gmax,
-                       const int32_t *__restrict__ pinst, const fos_inst *__restrict__ inst) {
+                       const int32_t *__restrict__ pinst, const fos_inst *__restrict__ inst,
+                       int per_xcd) {
   const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
-  const int i = blockIdx.x * 8 + grp;
+  // particles in cell order (see k_pbd_pr_b)
+  const int k8 = xcd_tile((int)blockIdx.x, per_xcd) * 8 + grp;
+  const int i = k8 < n ? ci[k8] : n;
   float lm = 0.0f;
   int b = 0;
   if (i < n && l < 27) {
@@ -1350,10 +1358,11 @@ int fos_batch_max_overlap(fos_batch *bt, const float *pos, const float *rad, flo
   const int n = bt->n_total;
   const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
   if (cell_pr)
-    hipLaunchKernelGGL(k_max_overlap_pr_b, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
-                       (hipStream_t)stream, pos, (const float4 *)cell_pr, rad, n, L, half_l,
-                       inv_l, cell_start, cell_count, cell_indices, local_max_depth, max_depth,
-                       bt->pinst, bt->inst);
+    hipLaunchKernelGGL(k_max_overlap_pr_b,
+                       dim3((unsigned)((((n + 7) / 8 + FOS_XCDS - 1) / FOS_XCDS) * FOS_XCDS)),
+                       dim3(256), 0, (hipStream_t)stream, pos, (const float4 *)cell_pr, rad, n,
+                       L, half_l, inv_l, cell_start, cell_count, cell_indices, local_max_depth,
+                       max_depth, bt->pinst, bt->inst, ((n + 7) / 8 + FOS_XCDS - 1) / FOS_XCDS);
   else
     hipLaunchKernelGGL(k_max_overlap_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                        (hipStream_t)stream, pos, rad, n, 3, L, half_l, inv_l, cell_start,
@@ -1373,11 +1382,12 @@ int fos_batch_pbd_pass(fos_batch *bt, const float *pos_in, float *pos_out, const
               "fos_batch_pbd_pass: bad arguments");
   const int n = bt->n_total;
   const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
+  const int per = ((n + 7) / 8 + FOS_XCDS - 1) / FOS_XCDS;
   if (cell_pr)
-    hipLaunchKernelGGL(k_pbd_pr_b, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+    hipLaunchKernelGGL(k_pbd_pr_b, dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0,
                        (hipStream_t)stream, pos_in, pos_out, (const float4 *)cell_pr, rad, n, L,
                        half_l, inv_l, gap_factor, max_disp_frac, disp_mult, cell_start,
-                       cell_count, cell_indices, bt->pinst, bt->inst, passes, pass);
+                       cell_count, cell_indices, bt->pinst, bt->inst, passes, pass, per);
   else
     hipLaunchKernelGGL(k_pbd_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                        (hipStream_t)stream, pos_in, pos_out, rad, n, 3, L, half_l, inv_l,

@@ -140,14 +140,8 @@ __device__ __forceinline__ void consider(int cand, int &best, float &bd, Seen<K>
   }
 }
 
-// Tile -> block mapping that keeps each XCD on a contiguous range of tiles
-// (workgroups are dispatched round-robin over the 8 XCDs, so consecutive
-// block ids land on different L2s).  Tiles are ordered x-major, so an XCD
-// sweeps a slab of x-planes whose neighbour rows stay in its own L2.
-#define FOS_XCDS 8
-__device__ __forceinline__ int xcd_tile(int bid, int per_xcd) {
-  return (bid % FOS_XCDS) * per_xcd + bid / FOS_XCDS;
-}
+// xcd_tile (fos_common.h): JFA tiles are ordered x-major, so an XCD sweeps a
+// slab of x-planes whose neighbour rows stay in its own L2.
 
 // Candidate q = (dx+1)*9 + (dy+1)*3 + (dz+1) is skipped when its owner equals
 // the incumbent or an owner met earlier in the scan at the positions where
