@@ -97,7 +97,13 @@ def run_frames(sim, frames):
         sim.step()
 
 
+EVENT_EVERY = 8   # timed frames between two with HIP events around every JFA pass
+
+
 def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
+    """Wall time of `steps` frames after `warmup`; with_events records HIP
+    events around each JFA pass on every EVENT_EVERY-th timed frame (event
+    records cost the stream time, so not on every frame)."""
     import torch
 
     import fos_lib as fl
@@ -110,11 +116,15 @@ def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
     P = cfg.passes()
     ev_arrays = []
     if with_events:
-        for _ in range(steps):
-            arr = (ctypes.c_void_p * (P + 1))()
-            fl.call("fos_events_create", P + 1, arr)
-            ev_arrays.append(arr)
-        sim.pass_events = list(ev_arrays)
+        slots = []
+        for k in range(steps):
+            arr = None
+            if k % EVENT_EVERY == 0:
+                arr = (ctypes.c_void_p * (P + 1))()
+                fl.call("fos_events_create", P + 1, arr)
+                ev_arrays.append(arr)
+            slots.append(arr)
+        sim.pass_events = slots
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     run_frames(sim, steps)
@@ -286,9 +296,12 @@ def main():
                    "frames": args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(), "kernel": "k_jfa_pass",
+                     "traffic": measured_traffic(), "kernel": "k_jfa_pass_list",
                      "avg_launch_us": round(avg_pass_s * 1e6, 3),
-                     "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)]},
+                     "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)],
+                     "timed_frames_with_events": len(pass_ms),
+                     "limiter": "instruction issue: VALU 60-80% and texture-address 63-68% busy "
+                                "(profiles/k_jfa_pass_list_c2_pmc.md)"},
         "gvox_pass_per_s": round(gvox, 2),
         "jfa_ms_per_frame": round(float(flat.sum(axis=1).mean()), 4),
     }

@@ -654,6 +654,17 @@ __global__ void k_topo(const float *__restrict__ pos, const float *__restrict__ 
   trunc[i] = tr;
 }
 
+// slot-record PBD pass (defined with the batched kernels below)
+template <bool BATCH>
+__global__ void __launch_bounds__(256)
+    k_pbd_pr(const float *__restrict__ pin, float *__restrict__ pout,
+             const float4 *__restrict__ pr, const float *__restrict__ rad, int n, int G0, float L,
+             float half_l, float inv_l, float gap, float mdf, float dmul,
+             const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+             const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
+             const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass,
+             int per_xcd, int32_t *__restrict__ cell_of);
+
 extern "C" {
 
 #define PART_ARGS_OK(n) ((n) >= 0)
@@ -680,13 +691,24 @@ int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad, int n,
 int fos_pbd_pass_cells(fos_ctx *ctx, const float *pos_in, float *pos_out, const float *rad,
                        int n, int G, float L, float gap_factor, float max_disp_frac,
                        float disp_mult, const int32_t *cell_start, const int32_t *cell_count,
-                       const int32_t *cell_indices, void *stream) {
+                       const int32_t *cell_indices, const float *cell_pr, void *stream) {
   FOS_REQUIRE(ctx && n >= 0 && n <= ctx->max_n && G >= 3 &&
                   (int64_t)G * G * G <= ctx->max_cells && L > 0.0f &&
                   (n == 0 || (pos_in && pos_out && pos_in != pos_out && rad && cell_start &&
                               cell_count && cell_indices)),
               "fos_pbd_pass_cells: bad arguments (n=%d G=%d)", n, G);
   if (n == 0) return 0;
+  if (cell_pr) {
+    const int per = ((n + 7) / 8 + FOS_XCDS - 1) / FOS_XCDS;
+    hipLaunchKernelGGL(k_pbd_pr<false>, dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0,
+                       (hipStream_t)stream, pos_in, pos_out, (const float4 *)cell_pr, rad, n, G,
+                       L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), gap_factor,
+                       max_disp_frac, disp_mult, cell_start, cell_count, cell_indices,
+                       (const int32_t *)nullptr, (const fos_inst *)nullptr,
+                       (const int32_t *)nullptr, 0, per, ctx->cell_of);
+    FOS_CHECK_LAUNCH("k_pbd_pr");
+    return 0;
+  }
   hipLaunchKernelGGL(k_pbd_wave<false>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                      (hipStream_t)stream, pos_in, pos_out, rad, n, G, L, (float)(0.5 * (double)L),
                      (float)(1.0 / (double)L), gap_factor, max_disp_frac, disp_mult,
@@ -1134,14 +1156,15 @@ __device__ __forceinline__ bool pbd_hit(float qx, float qy, float qz, float rj, 
   return true;
 }
 
+template <bool BATCH>
 __global__ void __launch_bounds__(256)
-    k_pbd_pr_b(const float *__restrict__ pin, float *__restrict__ pout,
-               const float4 *__restrict__ pr, const float *__restrict__ rad, int n, float L,
-               float half_l, float inv_l, float gap, float mdf, float dmul,
-               const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
-               const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
-               const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass,
-               int per_xcd) {
+    k_pbd_pr(const float *__restrict__ pin, float *__restrict__ pout,
+             const float4 *__restrict__ pr, const float *__restrict__ rad, int n, int G0, float L,
+             float half_l, float inv_l, float gap, float mdf, float dmul,
+             const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,
+             const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
+             const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass,
+             int per_xcd, int32_t *__restrict__ cell_of) {
   __shared__ float s_v[8][PBD_HITS][3];
   const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
   // particles in cell order (slot k of the grid holds particle ci[k]): a
@@ -1150,8 +1173,8 @@ __global__ void __launch_bounds__(256)
   const int k8 = xcd_tile((int)blockIdx.x, per_xcd) * 8 + grp;
   const int i = k8 < n ? ci[k8] : n;
   bool live = i < n;
-  int G = 3;
-  if (live) {
+  int G = G0;
+  if (BATCH && live) {
     const int b = pinst[i];
     if (pass >= npass[b]) {
       if (l < 3) pout[3 * i + l] = pin[3 * i + l];
@@ -1236,9 +1259,19 @@ __global__ void __launch_bounds__(256)
     ay = ay * sc;
     az = az * sc;
   }
-  pout[3 * i] = gwrap(px + ax, L, inv_l);
-  pout[3 * i + 1] = gwrap(py + ay, L, inv_l);
-  pout[3 * i + 2] = gwrap(pz + az, L, inv_l);
+  float w0 = gwrap(px + ax, L, inv_l), w1 = gwrap(py + ay, L, inv_l),
+        w2 = gwrap(pz + az, L, inv_l);
+  if (!BATCH && cell_of) {
+    // the cell step of the rebuild that follows (see k_pbd_wave)
+    w0 = gwrap(w0, L, inv_l);
+    w1 = gwrap(w1, L, inv_l);
+    w2 = gwrap(w2, L, inv_l);
+    cell_of[i] = hash_axis(w0, G, half_l, inv_l) * G * G + hash_axis(w1, G, half_l, inv_l) * G +
+                 hash_axis(w2, G, half_l, inv_l);
+  }
+  pout[3 * i] = w0;
+  pout[3 * i + 1] = w1;
+  pout[3 * i + 2] = w2;
 }
 
 __global__ void __launch_bounds__(256)
@@ -1249,7 +1282,7 @@ __global__ void __launch_bounds__(256)
                        const int32_t *__restrict__ pinst, const fos_inst *__restrict__ inst,
                        int per_xcd) {
   const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
-  // particles in cell order (see k_pbd_pr_b)
+  // particles in cell order (see k_pbd_pr)
   const int k8 = xcd_tile((int)blockIdx.x, per_xcd) * 8 + grp;
   const int i = k8 < n ? ci[k8] : n;
   float lm = 0.0f;
@@ -1384,10 +1417,11 @@ int fos_batch_pbd_pass(fos_batch *bt, const float *pos_in, float *pos_out, const
   const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
   const int per = ((n + 7) / 8 + FOS_XCDS - 1) / FOS_XCDS;
   if (cell_pr)
-    hipLaunchKernelGGL(k_pbd_pr_b, dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0,
-                       (hipStream_t)stream, pos_in, pos_out, (const float4 *)cell_pr, rad, n, L,
-                       half_l, inv_l, gap_factor, max_disp_frac, disp_mult, cell_start,
-                       cell_count, cell_indices, bt->pinst, bt->inst, passes, pass, per);
+    hipLaunchKernelGGL(k_pbd_pr<true>, dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0,
+                       (hipStream_t)stream, pos_in, pos_out, (const float4 *)cell_pr, rad, n, 3,
+                       L, half_l, inv_l, gap_factor, max_disp_frac, disp_mult, cell_start,
+                       cell_count, cell_indices, bt->pinst, bt->inst, passes, pass, per,
+                       (int32_t *)nullptr);
   else
     hipLaunchKernelGGL(k_pbd_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                        (hipStream_t)stream, pos_in, pos_out, rad, n, 3, L, half_l, inv_l,

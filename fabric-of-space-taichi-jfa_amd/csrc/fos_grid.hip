@@ -405,12 +405,23 @@ __device__ __forceinline__ void grid_range(const int32_t *__restrict__ cell_of, 
   }
 }
 
+// s4 (may be NULL): the slot records, written as the slots are placed
 __global__ void __launch_bounds__(1024)
     k_grid_build_ranges(const int32_t *__restrict__ cell_of, int n, int C, int cpw,
                         int32_t *__restrict__ count, int32_t *__restrict__ start,
-                        int32_t *__restrict__ idx) {
+                        int32_t *__restrict__ idx, const float *__restrict__ pos,
+                        const float *__restrict__ rad, float4 *__restrict__ s4) {
   const int lo = (int)blockIdx.x * cpw;
-  grid_range(cell_of, 0, n, 0, lo, min(C, lo + cpw), count, start, idx);
+  grid_range(cell_of, 0, n, 0, lo, min(C, lo + cpw), count, start, idx, pos, rad, s4);
+}
+
+// Slot records of a built CSR: s4[k] = (position, radius) of particle idx[k].
+__global__ void k_grid_records(const int32_t *__restrict__ idx, const float *__restrict__ pos,
+                               const float *__restrict__ rad, int n, float4 *__restrict__ s4) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = idx[k];
+  s4[k] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], rad[i]);
 }
 
 // Batches of foams: the cells of every particle in its foam's grid (global
@@ -499,10 +510,21 @@ int fos_grid_scatter(float *pos, int n, int G, float L, int32_t *cell_write,
   return 0;
 }
 
-int fos_grid_build_from_cells(fos_ctx *ctx, int n, int G, int32_t *cell_count,
-                              int32_t *cell_start, int32_t *cell_indices, void *stream) {
+static int grid_records(const float *pos, const float *rad, int n, const int32_t *cell_indices,
+                        float *cell_pr, hipStream_t st) {
+  if (!cell_pr || n == 0) return 0;
+  hipLaunchKernelGGL(k_grid_records, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, cell_indices,
+                     pos, rad, n, (float4 *)cell_pr);
+  FOS_CHECK_LAUNCH("k_grid_records");
+  return 0;
+}
+
+int fos_grid_build_from_cells(fos_ctx *ctx, const float *pos, const float *rad, int n, int G,
+                              int32_t *cell_count, int32_t *cell_start, int32_t *cell_indices,
+                              float *cell_pr, void *stream) {
   FOS_REQUIRE(ctx && G >= 3 && (int64_t)G * G * G <= ctx->max_cells && n >= 0 &&
-                  n <= ctx->max_n && cell_count && cell_start && (n == 0 || cell_indices),
+                  n <= ctx->max_n && cell_count && cell_start && (n == 0 || cell_indices) &&
+                  (!cell_pr || n == 0 || (pos && rad)),
               "fos_grid_build_from_cells: bad arguments (n=%d G=%d)", n, G);
   const int C = G * G * G;
   hipStream_t st = (hipStream_t)stream;
@@ -511,15 +533,37 @@ int fos_grid_build_from_cells(fos_ctx *ctx, int n, int G, int32_t *cell_count,
                        (const int32_t *)ctx->cell_of, n, C, cell_count, cell_start,
                        cell_indices);
     FOS_CHECK_LAUNCH("k_grid_build_block");
-    return 0;
+    return grid_records(pos, rad, n, cell_indices, cell_pr, st);
   }
   const int cpw = fos_range_cells(C);
   const int wgs = (C + cpw - 1) / cpw;
   hipLaunchKernelGGL(k_grid_build_ranges, dim3((unsigned)wgs), dim3(1024), 0, st,
                      (const int32_t *)ctx->cell_of, n, C, cpw, cell_count, cell_start,
-                     cell_indices);
+                     cell_indices, cell_pr ? pos : nullptr, cell_pr ? rad : nullptr,
+                     (float4 *)cell_pr);
   FOS_CHECK_LAUNCH("k_grid_build_ranges");
   return 0;
+}
+
+int fos_grid_build_pr(fos_ctx *ctx, float *pos, const float *rad, int n, int G, float L,
+                      int32_t *cell_count, int32_t *cell_start, int32_t *cell_indices,
+                      float *cell_pr, void *stream) {
+  FOS_REQUIRE(!cell_pr || n == 0 || rad, "fos_grid_build_pr: NULL rad");
+  const int C = G * G * G;
+  if (cell_pr && n > 0 && g_grid_block == 2 && ctx && G >= 3 &&
+      (int64_t)C <= ctx->max_cells && n <= ctx->max_n && pos && cell_indices && cell_count &&
+      cell_start) {
+    // the range form writes the records as it places the slots
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_grid_cells, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, n, G, L,
+                       (float)(0.5 * (double)L), (float)(1.0 / (double)L), ctx->cell_of);
+    FOS_CHECK_LAUNCH("k_grid_cells");
+    return fos_grid_build_from_cells(ctx, pos, rad, n, G, cell_count, cell_start, cell_indices,
+                                     cell_pr, stream);
+  }
+  int rc = fos_grid_build(ctx, pos, n, G, L, cell_count, cell_start, cell_indices, stream);
+  if (rc) return rc;
+  return grid_records(pos, rad, n, cell_indices, cell_pr, (hipStream_t)stream);
 }
 
 int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
@@ -542,7 +586,8 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
     const int wgs = (C + cpw - 1) / cpw;
     hipLaunchKernelGGL(k_grid_build_ranges, dim3((unsigned)wgs), dim3(1024), 0, st,
                        (const int32_t *)ctx->cell_of, n, C, cpw, cell_count, cell_start,
-                       cell_indices);
+                       cell_indices, (const float *)nullptr, (const float *)nullptr,
+                       (float4 *)nullptr);
     FOS_CHECK_LAUNCH("k_grid_build_ranges");
     return 0;
   }

@@ -76,6 +76,9 @@ class FoamSim:
         self.cell_count = torch.zeros(C, dtype=torch.int32, device=dev)
         self.cell_start = torch.zeros(C, dtype=torch.int32, device=dev)
         self.cell_indices = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        # slot records (position, radius per CSR slot) for the PBD passes
+        self.cell_pr = torch.zeros(max(n, 1), 4, dtype=torch.float32, device=dev)
+        self.slot_records = True
         self.res = int(cfg.res)
         V = self.res ** 3
         self.grid = torch.full((V,), -1, dtype=torch.int32, device=dev)
@@ -100,8 +103,9 @@ class FoamSim:
         self.growth = True
         self.pending = None        # FrameInfo whose gate is still on the device
         self.last = {}
-        # optional list of ctypes arrays of passes+1 hipEvent_t handles; each
-        # JFA frame consumes one to time its pass kernels on the stream
+        # optional list of ctypes arrays of passes+1 hipEvent_t handles (or
+        # None); each JFA frame consumes one entry and, when it is an array,
+        # times its pass kernels on the stream
         self.pass_events = None
         # optional callback(sim) after every JFA (training-data collection)
         self.on_jfa = None
@@ -121,8 +125,15 @@ class FoamSim:
         self.r_scale = 0.5 * (c.r_min + c.r_max)
 
     # -- building blocks ------------------------------------------------------
-    def rebuild_grid(self):
+    def rebuild_grid(self, pr=False):
+        """grid.rebuild_grid; pr=True also writes the slot records the PBD
+        pass reads (when slot_records is on)."""
         c = self.cfg
+        if pr and self.slot_records:
+            fl.call("fos_grid_build_pr", self.ctx.handle, fl.ptr(self.pos), fl.ptr(self.rad),
+                    self.n, self.G, c.L, fl.ptr(self.cell_count), fl.ptr(self.cell_start),
+                    fl.ptr(self.cell_indices), fl.ptr(self.cell_pr), fl.stream())
+            return
         fl.call("fos_grid_build", self.ctx.handle, fl.ptr(self.pos), self.n, self.G, c.L,
                 fl.ptr(self.cell_count), fl.ptr(self.cell_start), fl.ptr(self.cell_indices),
                 fl.stream())
@@ -141,15 +152,18 @@ class FoamSim:
                 c.gap_factor(), c.max_disp_frac, c.disp_mult, fl.ptr(self.cell_start),
                 fl.ptr(self.cell_count), fl.ptr(self.cell_indices), fl.stream())
         if cells:
-            fl.call("fos_pbd_pass_cells", self.ctx.handle, *args)
+            pr = fl.ptr(self.cell_pr) if self.slot_records else None
+            fl.call("fos_pbd_pass_cells", self.ctx.handle, *args[:-1], pr, args[-1])
         else:
             fl.call("fos_pbd_pass", *args)
         self.pos, self.pos_tmp = self.pos_tmp, self.pos
 
-    def rebuild_from_cells(self):
-        fl.call("fos_grid_build_from_cells", self.ctx.handle, self.n, self.G,
-                fl.ptr(self.cell_count), fl.ptr(self.cell_start), fl.ptr(self.cell_indices),
-                fl.stream())
+    def rebuild_from_cells(self, pr=False):
+        rec = pr and self.slot_records
+        fl.call("fos_grid_build_from_cells", self.ctx.handle, fl.ptr(self.pos),
+                fl.ptr(self.rad), self.n, self.G, fl.ptr(self.cell_count),
+                fl.ptr(self.cell_start), fl.ptr(self.cell_indices),
+                fl.ptr(self.cell_pr) if rec else None, fl.stream())
 
     def pressure(self, frame):
         c = self.cfg
@@ -232,9 +246,9 @@ class FoamSim:
         ready = False
         for p in range(passes):
             if ready:
-                self.rebuild_from_cells()
+                self.rebuild_from_cells(pr=True)
             else:
-                self.rebuild_grid()
+                self.rebuild_grid(pr=True)
             ready = p < passes - 1 or not c.ou_jitter
             self.pbd_pass(cells=ready)
         # 6 OU jitter

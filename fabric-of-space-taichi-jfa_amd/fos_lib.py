@@ -70,8 +70,9 @@ SIGNATURES = {
     "fos_topo_knn": [P, P, I, I, F, F, I, P, P, P, P, P],
     "fos_topo_ema": [P, P, I, F, P],
     "fos_topo_radii": [P, P, I, F, F, F, F, F, F, P],
-    "fos_pbd_pass_cells": [P, P, P, P, I, I, F, F, F, F, P, P, P, P],
-    "fos_grid_build_from_cells": [P, I, I, P, P, P, P],
+    "fos_pbd_pass_cells": [P, P, P, P, I, I, F, F, F, F, P, P, P, P, P],
+    "fos_grid_build_from_cells": [P, P, P, I, I, P, P, P, P, P],
+    "fos_grid_build_pr": [P, P, P, I, I, F, P, P, P, P, P],
     "fos_jfa_faces": [P, I, I, P, P, F, P, P, P, P, P, P, P, P],
     "fos_fsc_targets_gated": [P, P, P, P, I, I, I, F, F, F, F, P, P],
     "fos_fsc_gate": [P, P, I, I, I, D, P, P],

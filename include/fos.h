@@ -207,16 +207,28 @@ int fos_pbd_pass(const float *pos_in, float *pos_out, const float *rad,
 /* fos_pbd_pass followed by the first step of the fos_grid_build that comes
  * next in the frame (grid.py:285-309 position wrap and cell index, kept in
  * the context), fused into the PBD kernel; fos_grid_build_from_cells then
- * completes that rebuild.  Results equal fos_pbd_pass + fos_grid_build
+ * completes that rebuild.  cell_pr (may be NULL): the slot records of the
+ * current CSR (fos_grid_build_pr / fos_grid_build_from_cells), read instead
+ * of gathering each neighbour's position and radius by index; particles are
+ * then taken in cell order.  Results equal fos_pbd_pass + fos_grid_build
  * bit for bit.  No reference counterpart (a launch saved per PBD pass). */
 int fos_pbd_pass_cells(fos_ctx *ctx, const float *pos_in, float *pos_out, const float *rad,
                        int n, int G, float L, float gap_factor, float max_disp_frac,
                        float disp_mult, const int32_t *cell_start, const int32_t *cell_count,
-                       const int32_t *cell_indices, void *stream);
+                       const int32_t *cell_indices, const float *cell_pr, void *stream);
 /* The rest of fos_grid_build (grid.py:189-309 count, prefix sum, scatter)
- * from the cells fos_pbd_pass_cells left in the context. */
-int fos_grid_build_from_cells(fos_ctx *ctx, int n, int G, int32_t *cell_count,
-                              int32_t *cell_start, int32_t *cell_indices, void *stream);
+ * from the cells fos_pbd_pass_cells left in the context.  cell_pr (may be
+ * NULL, else pos and rad are read): cell_pr[k] = (position, radius) of the
+ * particle in slot k, float4 per slot. */
+int fos_grid_build_from_cells(fos_ctx *ctx, const float *pos, const float *rad, int n, int G,
+                              int32_t *cell_count, int32_t *cell_start, int32_t *cell_indices,
+                              float *cell_pr, void *stream);
+/* fos_grid_build (grid.py:30-309) that also writes the slot records
+ * cell_pr[k] = (wrapped position, radius) of the particle in slot k (cell_pr
+ * may be NULL).  No reference counterpart. */
+int fos_grid_build_pr(fos_ctx *ctx, float *pos, const float *rad, int n, int G, float L,
+                      int32_t *cell_count, int32_t *cell_start, int32_t *cell_indices,
+                      float *cell_pr, void *stream);
 /* dynamics.py:304-372 compute_local_max_overlaps + reduce_max_depth: fills
  * local_max_depth and the device float *max_depth (a true maximum). */
 int fos_max_overlap(const float *pos, const float *rad, int n, int G, float L,

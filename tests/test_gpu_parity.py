@@ -552,12 +552,15 @@ def test_pbd_maxoverlap_bit_exact(gpu, orc):
     assert np.array_equal(host(tl), lmd) and float(host(tm)[0]) == g
 
 
-@pytest.mark.parametrize("block", [2, 1])
-def test_pbd_pass_cells_equals_pass_then_rebuild(gpu, orc, block):
+@pytest.mark.parametrize("pr", [False, True])
+@pytest.mark.parametrize("block", [2, 1, 0])
+def test_pbd_pass_cells_equals_pass_then_rebuild(gpu, orc, block, pr):
     """The PBD pass that also runs the next rebuild's wrap-and-hash step,
     finished by fos_grid_build_from_cells, equals fos_pbd_pass followed by
     a full rebuild (the oracle's), including particles pushed across the
-    box faces (wrapped twice, as the reference does)."""
+    box faces (wrapped twice, as the reference does).  pr: the pass reads
+    slot records written by fos_grid_build_pr, and the finishing rebuild
+    writes the next ones (checked against the CSR)."""
     n, G_ = 3000, 14
     pos, rad = fi.uniform(n, 0.02, 0.04, seed=5)
     pos[:40, 0] = F(0.4999)                   # crowd the +x face
@@ -569,16 +572,27 @@ def test_pbd_pass_cells_equals_pass_then_rebuild(gpu, orc, block):
 
     def run():
         g = G(gpu, n, 1, G_ ** 3)
-        tp, tcc, tcs, tci = gpu_grid(gpu, pos, n, G_)
+        tp = dev(pos)
+        tr = dev(rad)
+        tcc = torch.empty(G_ ** 3, dtype=torch.int32, device="cuda")
+        tcs = torch.empty_like(tcc)
+        tci = torch.empty(n, dtype=torch.int32, device="cuda")
+        rec = torch.full((n, 4), -7.0, dtype=torch.float32, device="cuda") if pr else None
+        g.call("fos_grid_build_pr", g.h, g.ptr(tp), g.ptr(tr), n, G_, 1.0, g.ptr(tcc),
+               g.ptr(tcs), g.ptr(tci), g.ptr(rec) if pr else None, gpu.stream())
         out = torch.empty_like(tp)
-        g.call("fos_pbd_pass_cells", g.h, g.ptr(tp), g.ptr(out), g.ptr(dev(rad)), n, G_, 1.0,
-               gap, 0.2, 2.0, g.ptr(tcs), g.ptr(tcc), g.ptr(tci), gpu.stream())
-        g.call("fos_grid_build_from_cells", g.h, n, G_, g.ptr(tcc), g.ptr(tcs), g.ptr(tci),
+        g.call("fos_pbd_pass_cells", g.h, g.ptr(tp), g.ptr(out), g.ptr(tr), n, G_, 1.0,
+               gap, 0.2, 2.0, g.ptr(tcs), g.ptr(tcc), g.ptr(tci), g.ptr(rec) if pr else None,
                gpu.stream())
-        return host(out), host(tcc), host(tcs), host(tci)
-    got = with_param(gpu, "grid_block", block, run)
+        g.call("fos_grid_build_from_cells", g.h, g.ptr(out), g.ptr(tr), n, G_, g.ptr(tcc),
+               g.ptr(tcs), g.ptr(tci), g.ptr(rec) if pr else None, gpu.stream())
+        return host(out), host(tcc), host(tcs), host(tci), None if rec is None else host(rec)
+    *got, rec = with_param(gpu, "grid_block", block, run)
     for a, b in zip(got, want):
         assert np.array_equal(a, b)
+    if pr:
+        ci = want[3]
+        assert np.array_equal(rec[:, :3], want[0][ci]) and np.array_equal(rec[:, 3], rad[ci])
 
 
 def test_controller_kernels(gpu, orc):
