@@ -45,10 +45,10 @@ WEIGHTS = os.path.join(PKG, "face_mlp_c5.npz")
 COPY_CAL_RES = 400          # 400^3 int32 = 256,000,000 B read and written
 
 
-def measured_traffic():
-    """HBM bytes per pass launch from the committed counter summary, or None."""
+def measured_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per pass launch from a committed counter summary, or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return int(float(line.split(":", 1)[1]))
@@ -87,6 +87,8 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-foams", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-frames", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain copy of 400^3 int32 (counter scaling)")
     return ap.parse_args()
@@ -137,6 +139,55 @@ def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
         pass_ms.append(list(ms))
         fl.call("fos_events_destroy", P + 1, arr)
     return t1 - t0, pass_ms, sim
+
+
+C3_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_jfa_pass_c3_hbm_traffic.md")
+
+
+def c3_jfa_bench(frames):
+    """The JFA alone at C3 (N=200,000 clustered, 384^3, 10 passes): the label
+    ping-pong (2 x 226 MB) exceeds the 256 MB Infinity Cache, so this is the
+    size where the passes stream from HBM.  Each frame re-seeds and runs the
+    whole pass schedule with HIP events around every pass."""
+    import numpy as np
+    import torch
+
+    import foam_config as fc
+    import foam_inputs as fi
+    import fos_lib as fl
+    from foam import FoamSim
+    cfg = fc.preset("c3")
+    pos, rad = fi.for_config(cfg)
+    sim = FoamSim(cfg, pos, rad)
+    c, st, P, V = cfg, fl.stream(), cfg.passes(), cfg.res ** 3
+    voxel = c.voxel()
+    ms_all = []
+    for k in range(frames + 1):
+        arr = (ctypes.c_void_p * (P + 1))()
+        fl.call("fos_events_create", P + 1, arr)
+        fl.call("fos_jfa_init", sim.ctx.handle, fl.ptr(sim.pos), fl.ptr(sim.rad), sim.n, sim.res,
+                c.L, voxel, c.beta, fl.ptr(sim.grid), None, st)
+        fl.call("fos_jfa_passes_timed", sim.ctx.handle, sim.res, c.L, voxel, P,
+                int(c.early_exit), c.th_pass_delta, c.min_passes, fl.ptr(sim.grid),
+                fl.ptr(sim.temp), fl.ptr(sim.info), arr, st)
+        ms = (ctypes.c_float * P)()
+        fl.call("fos_events_elapsed", P + 1, arr, ms)
+        fl.call("fos_events_destroy", P + 1, arr)
+        if k > 0:                      # the first run warms up
+            ms_all.append(list(ms))
+    torch.cuda.synchronize()
+    flat = np.array(ms_all, dtype=np.float64)
+    avg_pass_s = float(flat.mean()) / 1e3
+    achieved = 8.0 * V / avg_pass_s / 1e9
+    return {"workload": "C3 JFA: N=200,000 clustered (64 centres, sigma 0.05, plus 100,000 uniform), "
+                        "radii lognormal in [0.003, 0.015], 384^3 int32 labels, 10 passes",
+            "frames": frames, "jfa_ms": round(float(flat.sum(axis=1).mean()), 3),
+            "gvox_pass_per_s": round(P * V / (float(flat.sum(axis=1).mean()) / 1e3) / 1e9, 2),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic(C3_TRAFFIC_FILE),
+                         "kernel": "k_jfa_pass_list", "avg_launch_us": round(avg_pass_s * 1e6, 2),
+                         "per_pass_us": [round(x * 1e3, 1) for x in flat.mean(axis=0)]}}
 
 
 def surrogate_bench(steps, warmup):
@@ -310,6 +361,8 @@ def main():
         el2, _, _ = timed(c2c, pos, rad, args.steps, args.warmup, False)
         out["cadence"] = {"workload": "same, JFA every 5th frame after a 20-frame warm start",
                           "value": round(args.steps / el2, 3), "unit": "frames/s"}
+    if not args.no_c3:
+        out["c3_jfa"] = c3_jfa_bench(args.c3_frames)
     if not args.no_surrogate and os.path.exists(WEIGHTS):
         out["surrogate"] = surrogate_bench(args.surrogate_steps, 8)
     if not args.no_ensemble:

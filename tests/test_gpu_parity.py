@@ -75,6 +75,21 @@ def run_jfa_gpu(fl, pos, rad, n, res, passes, early=False, th=0.006, minp=3):
 
 # ---- JFA ---------------------------------------------------------------------
 
+def test_jfa_c3_full_size(gpu, orc):
+    """The largest workload's JFA (N=200,000 clustered, 384^3, 10 passes;
+    ids above 65,535, so the passes keep 32-bit candidate lists): labels and
+    per-pass change counts bit-exact."""
+    cfg = fc.preset("c3")
+    pos, rad = fi.for_config(cfg)
+    n, res, P = cfg.n, cfg.res, cfg.passes()
+    print(f"[jfa parity] c3 n={n} res={res} passes={P}", flush=True)
+    lab, passes, early, hist = orc.jfa_run(pos, rad, n, res, 1.0, fc.f32(1.0 / res), P)
+    glab, info = run_jfa_gpu(gpu, pos, rad, n, res, P)
+    assert passes == P and not early
+    assert np.array_equal(glab, lab)
+    assert info[0] == passes and (info[2:2 + passes] == hist).all()
+
+
 @pytest.mark.parametrize("case", ["c1_64", "c1_48", "dense_seeds", "single"])
 def test_jfa_init_and_passes_bit_exact(gpu, orc, case):
     if case == "c1_64":
