@@ -610,6 +610,75 @@ def test_pbd_pass_cells_equals_pass_then_rebuild(gpu, orc, block, pr):
         assert np.array_equal(rec[:, :3], want[0][ci]) and np.array_equal(rec[:, 3], rad[ci])
 
 
+@pytest.mark.parametrize("case", ["c1", "overflow"])
+def test_jfa_faces_equals_accumulate_then_finalize(gpu, orc, case):
+    """fos_jfa_faces (emission, compaction and finalize behind one host read)
+    against the oracle's accumulate_face_area + finalize_faces, twice on one
+    context.  'overflow': row 0 meets 100 partners, so the first call's row
+    tables overflow, grow and the emission starts over."""
+    if case == "c1":
+        n, res = 2000, 64
+        pos, rad = fi.uniform(n, 0.02, 0.04, seed=0)
+        lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, fc.f32(1 / 64), 7)
+        lab = orc.majority(lab, res)
+    else:
+        res, n = 112, 101
+        lab = np.zeros((res, res, res), np.int32)
+        for k in range(100):
+            lab[4 + k, :, 7] = 1 + k
+        lab = lab.ravel()
+        rad = np.full(n, 0.02, F)
+    voxel = fc.f32(1.0 / res)
+    rng = np.random.default_rng(8)
+    prev = ((rng.random((n, 32)) < 0.3) * F(0.12)).astype(F)
+    ids, cts = orc.accumulate_faces(lab, res, n)
+    want = orc.finalize_faces(n, rad, voxel, ids, cts, prev)
+    g = G(gpu, n, res)
+    for call in range(2):
+        tp = dev(prev)
+        fid = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+        fct = torch.empty(n, 32, dtype=torch.int32, device="cuda")
+        out = [torch.empty(n, 32, dtype=torch.int32, device="cuda")] + \
+            [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3)]
+        g.call("fos_jfa_faces", g.h, res, n, g.ptr(dev(lab)), g.ptr(dev(rad)), voxel, g.ptr(fid),
+               g.ptr(fct), g.ptr(tp), *[g.ptr(t) for t in out], gpu.stream())
+        assert np.array_equal(host(fid), ids) and np.array_equal(host(fct), cts), call
+        got = [host(tp)] + [host(t) for t in out]
+        for a, b, name in zip(want, got, ("prev", "nl", "nc", "used", "fsc")):
+            assert np.array_equal(a, b), (call, name)
+
+
+def test_fsc_gate_and_gated_targets(gpu, orc):
+    """fos_fsc_gate: 0 while frame < warm start or more than zero_rate of
+    the particles have FSC 0, else 1 (run.py:1184-1195), on consecutive
+    calls (its counters reset themselves); fos_fsc_targets_gated leaves the
+    targets alone on 0 and equals fos_fsc_targets on 1."""
+    g = G(gpu, 5000, 1)
+    gate = torch.full((1,), -1.0, dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(4)
+    for n, zeros, frame, warm, rate in ((5000, 100, 30, 20, 0.05), (5000, 400, 30, 20, 0.05),
+                                        (3000, 0, 5, 20, 0.05), (5000, 250, 30, 20, 0.05),
+                                        (1, 1, 30, 20, 0.05), (0, 0, 30, 20, 0.05)):
+        fsc = rng.integers(1, 25, max(n, 1)).astype(np.int32)[:n]
+        fsc[rng.permutation(n)[:zeros]] = 0
+        f0 = zeros / max(1, n)
+        want = 0.0 if (frame < warm or f0 > rate) else 1.0
+        g.call("fos_fsc_gate", g.h, g.ptr(dev(fsc)) if n else None, n, frame, warm, rate,
+               g.ptr(gate), gpu.stream())
+        assert float(host(gate)[0]) == want, (n, zeros, frame)
+    n = 4000
+    rad = rng.uniform(0.01, 0.04, n).astype(F)
+    fsc = rng.integers(0, 25, n).astype(np.int32)
+    ema = (rng.random(n) * 25).astype(F)
+    rt = orc.fsc_targets(rad, fsc, ema, n, 12, 16, fc.f32(0.1), fc.f32(0.006), fc.f32(0.024))
+    for gv in (0.0, 1.0):
+        trt = dev(np.full(n, 7.0, F))
+        gt = dev(np.array([gv], F))
+        g.call("fos_fsc_targets_gated", g.ptr(dev(rad)), g.ptr(trt), g.ptr(dev(fsc)),
+               g.ptr(dev(ema)), n, 12, 16, 0.1, 0.006, 0.024, 1.0, g.ptr(gt), gpu.stream())
+        assert np.array_equal(host(trt), rt if gv else np.full(n, 7.0, F)), gv
+
+
 def test_controller_kernels(gpu, orc):
     n = 5000
     rng = np.random.default_rng(6)
