@@ -225,6 +225,12 @@ struct fos_inst {
 };
 
 #define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
+namespace fos {
+// lanes per particle of the slot-record PBD pass (fos_set_param "pbd_lanes")
+int pbd_lanes_set(int lanes);
+int pbd_lanes_get();
+}  // namespace fos
+
 // Tile -> block mapping that keeps each XCD on a contiguous range of tiles
 // (workgroups are dispatched round-robin over the 8 XCDs, so consecutive
 // block ids land on different L2s).

@@ -655,7 +655,7 @@ __global__ void k_topo(const float *__restrict__ pos, const float *__restrict__ 
 }
 
 // slot-record PBD pass (defined with the batched kernels below)
-template <bool BATCH>
+template <bool BATCH, int LPP>
 __global__ void __launch_bounds__(256)
     k_pbd_pr(const float *__restrict__ pin, float *__restrict__ pout,
              const float4 *__restrict__ pr, const float *__restrict__ rad, int n, int G0, float L,
@@ -664,6 +664,12 @@ __global__ void __launch_bounds__(256)
              const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
              const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass,
              int per_xcd, int32_t *__restrict__ cell_of);
+template <bool BATCH>
+static void launch_pbd_pr(hipStream_t st, int n, const float *pin, float *pout, const float *pr,
+                          const float *rad, int G, float L, float gap, float mdf, float dmul,
+                          const int32_t *cs, const int32_t *cc, const int32_t *ci,
+                          const int32_t *pinst, const fos_inst *inst, const int32_t *npass,
+                          int pass, int32_t *cell_of);
 
 extern "C" {
 
@@ -699,13 +705,9 @@ int fos_pbd_pass_cells(fos_ctx *ctx, const float *pos_in, float *pos_out, const 
               "fos_pbd_pass_cells: bad arguments (n=%d G=%d)", n, G);
   if (n == 0) return 0;
   if (cell_pr) {
-    const int per = ((n + 7) / 8 + FOS_XCDS - 1) / FOS_XCDS;
-    hipLaunchKernelGGL(k_pbd_pr<false>, dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0,
-                       (hipStream_t)stream, pos_in, pos_out, (const float4 *)cell_pr, rad, n, G,
-                       L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), gap_factor,
-                       max_disp_frac, disp_mult, cell_start, cell_count, cell_indices,
-                       (const int32_t *)nullptr, (const fos_inst *)nullptr,
-                       (const int32_t *)nullptr, 0, per, ctx->cell_of);
+    launch_pbd_pr<false>((hipStream_t)stream, n, pos_in, pos_out, cell_pr, rad, G, L, gap_factor,
+                         max_disp_frac, disp_mult, cell_start, cell_count, cell_indices, nullptr,
+                         nullptr, nullptr, 0, ctx->cell_of);
     FOS_CHECK_LAUNCH("k_pbd_pr");
     return 0;
   }
@@ -1156,7 +1158,10 @@ __device__ __forceinline__ bool pbd_hit(float qx, float qy, float qz, float rj, 
   return true;
 }
 
-template <bool BATCH>
+// LPP lanes per particle (8, 16 or 32; 256 / LPP particles per block): lane
+// l walks stencil cells [l*CPL, l*CPL + CPL) in order, so the hits, placed by
+// an exclusive prefix over the lanes, stay in stencil-then-slot order.
+template <bool BATCH, int LPP>
 __global__ void __launch_bounds__(256)
     k_pbd_pr(const float *__restrict__ pin, float *__restrict__ pout,
              const float4 *__restrict__ pr, const float *__restrict__ rad, int n, int G0, float L,
@@ -1165,13 +1170,14 @@ __global__ void __launch_bounds__(256)
              const int32_t *__restrict__ ci, const int32_t *__restrict__ pinst,
              const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass,
              int per_xcd, int32_t *__restrict__ cell_of) {
-  __shared__ float s_v[8][PBD_HITS][3];
-  const int grp = (int)(threadIdx.x >> 5), l = (int)(threadIdx.x & 31);
+  constexpr int NPB = 256 / LPP, CPL = (27 + LPP - 1) / LPP;
+  __shared__ float s_v[NPB][PBD_HITS][3];
+  const int grp = (int)(threadIdx.x / LPP), l = (int)(threadIdx.x % LPP);
   // particles in cell order (slot k of the grid holds particle ci[k]): a
-  // workgroup's eight particles and an XCD's run of workgroups read the same
+  // workgroup's particles and an XCD's run of workgroups read the same
   // neighbourhood of slot records from their L2
-  const int k8 = xcd_tile((int)blockIdx.x, per_xcd) * 8 + grp;
-  const int i = k8 < n ? ci[k8] : n;
+  const int kk = xcd_tile((int)blockIdx.x, per_xcd) * NPB + grp;
+  const int i = kk < n ? ci[kk] : n;
   bool live = i < n;
   int G = G0;
   if (BATCH && live) {
@@ -1186,7 +1192,10 @@ __global__ void __launch_bounds__(256)
     }
   }
   float px = 0.f, py = 0.f, pz = 0.f, ri = 0.f, maxd = 0.f;
-  int cx = 0, cy = 0, cz = 0, st = 0, ct = 0;
+  int cx = 0, cy = 0, cz = 0;
+  int st[CPL], ct[CPL];
+#pragma unroll
+  for (int u = 0; u < CPL; ++u) st[u] = ct[u] = 0;
   if (live) {
     px = pin[3 * i];
     py = pin[3 * i + 1];
@@ -1196,39 +1205,48 @@ __global__ void __launch_bounds__(256)
     cx = centre_axis(px, G, L, half_l, inv_l);
     cy = centre_axis(py, G, L, half_l, inv_l);
     cz = centre_axis(pz, G, L, half_l, inv_l);
-    if (l < 27) {
-      const int id = wrap_fast(cx + l / 9 - 1, G) * G * G +
-                     wrap_fast(cy + (l / 3) % 3 - 1, G) * G + wrap_fast(cz + l % 3 - 1, G);
-      st = cs[id];
-      ct = cc[id];
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) {
+      const int c = l * CPL + u;
+      if (c < 27) {
+        const int id = wrap_fast(cx + c / 9 - 1, G) * G * G +
+                       wrap_fast(cy + (c / 3) % 3 - 1, G) * G + wrap_fast(cz + c % 3 - 1, G);
+        st[u] = cs[id];
+        ct[u] = cc[id];
+      }
     }
   }
   int h = 0;
   float vx, vy, vz;
-  for (int k = st; k < st + ct; ++k) {
-    const float4 q = pr[k];
-    if (ci[k] != i && pbd_hit(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz))
-      ++h;
-  }
-  int x = h;  // inclusive prefix over the 32-lane group
-  for (int o = 1; o < 32; o <<= 1) {
-    const int y = __shfl_up(x, o, 32);
+#pragma unroll
+  for (int u = 0; u < CPL; ++u)
+    for (int k = st[u]; k < st[u] + ct[u]; ++k) {
+      const float4 q = pr[k];
+      if (ci[k] != i &&
+          pbd_hit(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz))
+        ++h;
+    }
+  int x = h;  // inclusive prefix over the particle's lanes
+  for (int o = 1; o < LPP; o <<= 1) {
+    const int y = __shfl_up(x, o, LPP);
     if (l >= o) x += y;
   }
-  const int H = __shfl(x, 31, 32);
+  const int H = __shfl(x, LPP - 1, LPP);
   const bool fits = H <= PBD_HITS;
   if (live && fits && h > 0) {
     int m = x - h;
-    for (int k = st; k < st + ct; ++k) {
-      const float4 q = pr[k];
-      if (ci[k] != i &&
-          pbd_hit(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz)) {
-        s_v[grp][m][0] = vx;
-        s_v[grp][m][1] = vy;
-        s_v[grp][m][2] = vz;
-        ++m;
+#pragma unroll
+    for (int u = 0; u < CPL; ++u)
+      for (int k = st[u]; k < st[u] + ct[u]; ++k) {
+        const float4 q = pr[k];
+        if (ci[k] != i &&
+            pbd_hit(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz)) {
+          s_v[grp][m][0] = vx;
+          s_v[grp][m][1] = vy;
+          s_v[grp][m][2] = vz;
+          ++m;
+        }
       }
-    }
   }
   __syncthreads();
   if (!live || l != 0) return;
@@ -1273,6 +1291,44 @@ __global__ void __launch_bounds__(256)
   pout[3 * i + 1] = w1;
   pout[3 * i + 2] = w2;
 }
+
+// Lanes per particle of the slot-record PBD pass (a speed parameter; 0:
+// by size -- small foams are latency-bound and gain from 32 lanes each,
+// batches of millions of particles from 8 lanes doing 4 cells each).
+static int g_pbd_lanes = 0;
+
+// Launches k_pbd_pr with g_pbd_lanes lanes per particle.
+template <bool BATCH>
+static void launch_pbd_pr(hipStream_t st, int n, const float *pin, float *pout, const float *pr,
+                          const float *rad, int G, float L, float gap, float mdf, float dmul,
+                          const int32_t *cs, const int32_t *cc, const int32_t *ci,
+                          const int32_t *pinst, const fos_inst *inst, const int32_t *npass,
+                          int pass, int32_t *cell_of) {
+  const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
+  const int lpp = g_pbd_lanes ? g_pbd_lanes : (n >= 131072 ? 8 : 32), npb = 256 / lpp;
+  const int per = ((n + npb - 1) / npb + FOS_XCDS - 1) / FOS_XCDS;
+#define PBD_PR(LP)                                                                               \
+  hipLaunchKernelGGL((k_pbd_pr<BATCH, LP>), dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0, st, \
+                     pin, pout, (const float4 *)pr, rad, n, G, L, half_l, inv_l, gap, mdf, dmul, \
+                     cs, cc, ci, pinst, inst, npass, pass, per, cell_of)
+  if (lpp == 8)
+    PBD_PR(8);
+  else if (lpp == 16)
+    PBD_PR(16);
+  else
+    PBD_PR(32);
+#undef PBD_PR
+}
+
+namespace fos {
+int pbd_lanes_set(int lanes) {
+  FOS_REQUIRE(lanes == 0 || lanes == 8 || lanes == 16 || lanes == 32,
+              "pbd_lanes must be 0, 8, 16 or 32");
+  g_pbd_lanes = lanes;
+  return 0;
+}
+int pbd_lanes_get() { return g_pbd_lanes; }
+}  // namespace fos
 
 __global__ void __launch_bounds__(256)
     k_max_overlap_pr_b(const float *__restrict__ pos, const float4 *__restrict__ pr,
@@ -1415,13 +1471,10 @@ int fos_batch_pbd_pass(fos_batch *bt, const float *pos_in, float *pos_out, const
               "fos_batch_pbd_pass: bad arguments");
   const int n = bt->n_total;
   const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
-  const int per = ((n + 7) / 8 + FOS_XCDS - 1) / FOS_XCDS;
   if (cell_pr)
-    hipLaunchKernelGGL(k_pbd_pr<true>, dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0,
-                       (hipStream_t)stream, pos_in, pos_out, (const float4 *)cell_pr, rad, n, 3,
-                       L, half_l, inv_l, gap_factor, max_disp_frac, disp_mult, cell_start,
-                       cell_count, cell_indices, bt->pinst, bt->inst, passes, pass, per,
-                       (int32_t *)nullptr);
+    launch_pbd_pr<true>((hipStream_t)stream, n, pos_in, pos_out, cell_pr, rad, 3, L, gap_factor,
+                        max_disp_frac, disp_mult, cell_start, cell_count, cell_indices, bt->pinst,
+                        bt->inst, passes, pass, nullptr);
   else
     hipLaunchKernelGGL(k_pbd_wave<true>, dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
                        (hipStream_t)stream, pos_in, pos_out, rad, n, 3, L, half_l, inv_l,

@@ -1636,6 +1636,8 @@ int fos_set_param(const char *name, int value) {
   } else if (!strcmp(name, "grid_block")) {
     FOS_REQUIRE(value == 0 || value == 1 || value == 2, "grid_block must be 0, 1 or 2");
     g_grid_block = value;
+  } else if (!strcmp(name, "pbd_lanes")) {
+    return pbd_lanes_set(value);
   } else {
     FOS_REQUIRE(false, "fos_set_param: unknown parameter '%s'", name);
   }
@@ -1658,6 +1660,8 @@ int fos_get_param(const char *name, int *value) {
     *value = g_emit_tx;
   else if (!strcmp(name, "face_emit_zw"))
     *value = g_emit_zw;
+  else if (!strcmp(name, "pbd_lanes"))
+    *value = pbd_lanes_get();
   else
     FOS_REQUIRE(false, "fos_get_param: unknown parameter '%s'", name);
   return 0;

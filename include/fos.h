@@ -54,6 +54,10 @@ const char *fos_last_error(void);
  *                      kernels; 1 one workgroup when n <= 16256 and cells
  *                      <= 16384, else as 2; 2 one workgroup per range of
  *                      cells, independent of each other (default)
+ *   "pbd_lanes"        0,8,16,32  lanes per particle of the slot-record PBD
+ *                      pass (fos_pbd_pass_cells with records,
+ *                      fos_batch_pbd_pass); 0 (default): 32 below 131,072
+ *                      particles, else 8
  * They change speed only, never results.  No reference counterpart (Taichi
  * picks launch shapes itself). */
 int fos_set_param(const char *name, int value);

@@ -567,15 +567,15 @@ def test_pbd_maxoverlap_bit_exact(gpu, orc):
     assert np.array_equal(host(tl), lmd) and float(host(tm)[0]) == g
 
 
-@pytest.mark.parametrize("pr", [False, True])
+@pytest.mark.parametrize("pr", [False, 8, 16, 32])
 @pytest.mark.parametrize("block", [2, 1, 0])
 def test_pbd_pass_cells_equals_pass_then_rebuild(gpu, orc, block, pr):
     """The PBD pass that also runs the next rebuild's wrap-and-hash step,
     finished by fos_grid_build_from_cells, equals fos_pbd_pass followed by
     a full rebuild (the oracle's), including particles pushed across the
-    box faces (wrapped twice, as the reference does).  pr: the pass reads
-    slot records written by fos_grid_build_pr, and the finishing rebuild
-    writes the next ones (checked against the CSR)."""
+    box faces (wrapped twice, as the reference does).  pr (lanes per
+    particle): the pass reads slot records written by fos_grid_build_pr, and
+    the finishing rebuild writes the next ones (checked against the CSR)."""
     n, G_ = 3000, 14
     pos, rad = fi.uniform(n, 0.02, 0.04, seed=5)
     pos[:40, 0] = F(0.4999)                   # crowd the +x face
@@ -602,7 +602,8 @@ def test_pbd_pass_cells_equals_pass_then_rebuild(gpu, orc, block, pr):
         g.call("fos_grid_build_from_cells", g.h, g.ptr(out), g.ptr(tr), n, G_, g.ptr(tcc),
                g.ptr(tcs), g.ptr(tci), g.ptr(rec) if pr else None, gpu.stream())
         return host(out), host(tcc), host(tcs), host(tci), None if rec is None else host(rec)
-    *got, rec = with_param(gpu, "grid_block", block, run)
+    *got, rec = with_param(gpu, "grid_block", block,
+                           lambda: with_param(gpu, "pbd_lanes", pr or 0, run))
     for a, b in zip(got, want):
         assert np.array_equal(a, b)
     if pr:
