@@ -88,9 +88,22 @@ def frame_foams():
     return cfgs, inputs
 
 
-def test_batch_frames_match_single_foams(gpu):
+@pytest.mark.parametrize("lanes", [0, 8])
+def test_batch_frames_match_single_foams(gpu, lanes):
     """Whole frames: every foam of the batch equals its own FoamSim run, bit
-    for bit, including foams that run different PBD pass counts."""
+    for bit, including foams that run different PBD pass counts.  lanes: the
+    PBD pass's lanes per particle (0: by size)."""
+    import ctypes
+    old = ctypes.c_int()
+    gpu.call("fos_get_param", b"pbd_lanes", ctypes.byref(old))
+    gpu.call("fos_set_param", b"pbd_lanes", lanes)
+    try:
+        _batch_frames_match_single_foams()
+    finally:
+        gpu.call("fos_set_param", b"pbd_lanes", old.value)
+
+
+def _batch_frames_match_single_foams():
     from ensemble import Ensemble
     from foam import FoamSim
     cfgs, inputs = frame_foams()
