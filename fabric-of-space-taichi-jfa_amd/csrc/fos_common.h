@@ -246,6 +246,8 @@ __device__ __forceinline__ int xcd_tile(int bid, int per_xcd) {
 
 // Cells per grid-rebuild workgroup for a hash grid of C cells: about 192
 // workgroups per grid, at least 64 and at most 1024 cells each.
+// most cells one range workgroup of the grid rebuild takes (one per thread)
+#define FOS_GR_CELLS 1024
 __host__ __device__ inline int fos_range_cells(int C) {
   int cpw = (C + 191) / 192;
   return cpw < 64 ? 64 : (cpw > 1024 ? 1024 : cpw);

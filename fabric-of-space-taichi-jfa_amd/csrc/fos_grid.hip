@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(1024)
 // workgroup depends on another, so one launch does count, scan and scatter.
 // Up to GR_LIST particles of a range are staged in LDS; a range with more is
 // placed in cell_indices directly.
-#define GR_CELLS 1024
+#define GR_CELLS FOS_GR_CELLS
 #define GR_LIST 2048
 
 __device__ __forceinline__ void sort_run(int32_t *a, int m) {
@@ -468,6 +468,12 @@ __global__ void __launch_bounds__(1024)
 // 0: multi-kernel rebuild; 1: one workgroup where it fits (else 2);
 // 2: cell-range workgroups.
 int g_grid_block = 2;
+// cells per range workgroup of a single-foam rebuild (0: fos_range_cells)
+int g_grid_cpw = 0;
+
+static int range_cells(int C) {
+  return g_grid_cpw > 0 ? (g_grid_cpw < C ? g_grid_cpw : C) : fos_range_cells(C);
+}
 
 extern "C" {
 
@@ -535,7 +541,7 @@ int fos_grid_build_from_cells(fos_ctx *ctx, const float *pos, const float *rad, 
     FOS_CHECK_LAUNCH("k_grid_build_block");
     return grid_records(pos, rad, n, cell_indices, cell_pr, st);
   }
-  const int cpw = fos_range_cells(C);
+  const int cpw = range_cells(C);
   const int wgs = (C + cpw - 1) / cpw;
   hipLaunchKernelGGL(k_grid_build_ranges, dim3((unsigned)wgs), dim3(1024), 0, st,
                      (const int32_t *)ctx->cell_of, n, C, cpw, cell_count, cell_start,
@@ -582,7 +588,7 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
                          L, (float)(0.5 * (double)L), (float)(1.0 / (double)L), ctx->cell_of);
       FOS_CHECK_LAUNCH("k_grid_cells");
     }
-    const int cpw = fos_range_cells(C);
+    const int cpw = range_cells(C);
     const int wgs = (C + cpw - 1) / cpw;
     hipLaunchKernelGGL(k_grid_build_ranges, dim3((unsigned)wgs), dim3(1024), 0, st,
                        (const int32_t *)ctx->cell_of, n, C, cpw, cell_count, cell_start,

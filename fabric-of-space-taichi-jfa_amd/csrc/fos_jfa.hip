@@ -1456,6 +1456,7 @@ static int g_kernel = 0;
 static int g_emit_tx = 8;
 static int g_emit_zw = 32;
 extern int g_grid_block;  // fos_grid.hip
+extern int g_grid_cpw;    // fos_grid.hip
 
 template <bool U, int K>
 static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float voxel,
@@ -1638,6 +1639,10 @@ int fos_set_param(const char *name, int value) {
     g_grid_block = value;
   } else if (!strcmp(name, "pbd_lanes")) {
     return pbd_lanes_set(value);
+  } else if (!strcmp(name, "grid_range_cells")) {
+    FOS_REQUIRE(value == 0 || (value >= 1 && value <= FOS_GR_CELLS),
+                "grid_range_cells must be 0 or in [1, %d]", FOS_GR_CELLS);
+    g_grid_cpw = value;
   } else {
     FOS_REQUIRE(false, "fos_set_param: unknown parameter '%s'", name);
   }
@@ -1662,6 +1667,8 @@ int fos_get_param(const char *name, int *value) {
     *value = g_emit_zw;
   else if (!strcmp(name, "pbd_lanes"))
     *value = pbd_lanes_get();
+  else if (!strcmp(name, "grid_range_cells"))
+    *value = g_grid_cpw;
   else
     FOS_REQUIRE(false, "fos_get_param: unknown parameter '%s'", name);
   return 0;

@@ -54,6 +54,8 @@ const char *fos_last_error(void);
  *                      kernels; 1 one workgroup when n <= 16256 and cells
  *                      <= 16384, else as 2; 2 one workgroup per range of
  *                      cells, independent of each other (default)
+ *   "grid_range_cells" 0..1024  cells per workgroup of the range rebuild of
+ *                      one foam (0, default: C/192 clamped to [64, 1024])
  *   "pbd_lanes"        0,8,16,32  lanes per particle of the slot-record PBD
  *                      pass (fos_pbd_pass_cells with records,
  *                      fos_batch_pbd_pass); 0 (default): 32 below 131,072
