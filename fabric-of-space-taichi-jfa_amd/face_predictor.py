@@ -133,8 +133,10 @@ class FacePredictor:
         self.pair_off = torch.empty(self.n_max + 1, dtype=i32, device=d)
         self.pair_i = torch.empty(self.n_max * self.cap, dtype=i32, device=d)
         self.feat = torch.empty((self.n_max * self.cap, SLOTS), dtype=f32, device=d)
-        # previous frame's accepted partners j > i per row (-1: none yet)
+        # previous frame's accepted partners j > i per row (-1: none yet);
+        # warm: prev_up holds a previous frame's rows
         self.prev_up = torch.full((self.n_max, 32), -1, dtype=i32, device=d)
+        self.warm = False
         self.logit = torch.empty(self.n_max * self.cap, dtype=f32, device=d)
         self.set_params(params)
 
@@ -184,12 +186,14 @@ class FacePredictor:
         self.fsc(n, fsc, face_ids, face_counts)
         if face_ids is not None:
             self.prev_up[:n].copy_(face_ids[:n])
+            self.warm = True
 
     def remember_jfa(self, neighbor_list, neighbor_count, n):
         """After a JFA frame: its accepted pairs are the next frame's
         previous ones."""
         fl.call("fos_pred_prev_rows", fl.ptr(neighbor_list), fl.ptr(neighbor_count), n,
                 fl.ptr(self.prev_up), fl.stream())
+        self.warm = True
 
     def labels(self, n, neighbor_list, neighbor_count, out):
         fl.call("fos_pred_labels", fl.ptr(self.cand), fl.ptr(self.pair_off), fl.ptr(self.pair_i),

@@ -267,14 +267,18 @@ class FoamSim:
             ran = self.fallback or f % c.judge_every == 0
             if ran:
                 self.run_jfa()
+                warm = self.predictor.warm
                 self.predictor.predict(self.pos, self.rad, self.n, self.pred_fsc,
                                        r_mean=self.r_scale)
                 self.predictor.remember_jfa(self.neighbor_list, self.neighbor_count, self.n)
                 from face_predictor import fsc_error
                 err, bias = fsc_error(self.pred_fsc, self.fsc)
                 # the judge compares the frame's mean FSC (bias); the
-                # per-particle error is recorded beside it
-                self.fallback = bias > c.max_fsc_err
+                # per-particle error is recorded beside it.  Without a
+                # previous frame's rows (a cold start) the hysteresis feature
+                # is empty, so that judgement says nothing about the frames
+                # that follow (they start from this JFA's rows): no fallback.
+                self.fallback = warm and bias > c.max_fsc_err
                 self.judged.append((f, err, bias))
             else:
                 self.predictor.predict(self.pos, self.rad, self.n, self.fsc, self.face_ids,

@@ -390,3 +390,42 @@ def test_training_gradients_and_loss(gpu):
              None, gpu.stream())
     acc = ((host(out) > 0) == (y > 0.5)).mean()
     assert acc > 0.95
+
+
+def test_frame_driver_judge_and_fallback(gpu):
+    """The frame driver's predictor path (FoamSim with judge_every): the JFA
+    runs on judge frames and after a fallback, the predictor in between
+    (its FSC and face rows feed the frame); a judged mean-FSC error above
+    the bound sends the next frame to the JFA -- except on a cold start,
+    where the hysteresis feature had no previous rows.  Untrained weights
+    keep the error large, so both branches run."""
+    import foam_config as fc
+    from foam import FoamSim
+    cfg = fc.preset("c5s")
+    cfg.n, cfg.res, cfg.judge_every, cfg.pressure_iters = 3000, 64, 3, 5
+    cfg.r_min, cfg.r_max = 0.012, 0.03
+    pos, rad = fi.uniform(cfg.n, 0.015, 0.025, seed=9)
+    sim = FoamSim(cfg, pos, rad)
+    sim.set_predictor(fp.init_params(seed=3), mult=2.0)
+    ran = []
+    for f in range(8):
+        fallback_before = sim.fallback
+        last = sim.step()
+        ran.append(last["jfa"])
+        assert last["jfa"] == (fallback_before or f % cfg.judge_every == 0), f
+        if last["jfa"]:
+            jf, err, bias = sim.judged[-1]
+            assert jf == f
+            if f == 0:
+                assert not sim.fallback          # cold start: no fallback
+            else:
+                assert sim.fallback == (bias > cfg.max_fsc_err), (f, bias)
+        fsc = host(sim.fsc)
+        nrow = (host(sim.face_ids) >= 0).sum(axis=1)
+        if not last["jfa"]:
+            # predicted frame: FSC counts kept pairs at both ends; rows list j > i
+            assert (fsc >= nrow).all() and fsc.sum() == 2 * nrow.sum()
+    assert ran[0] and not ran[1]                 # no fallback after the cold judge frame
+    # the untrained model misses on some warm judge frame: the fallback ran
+    assert any(b > cfg.max_fsc_err for jf, _, b in sim.judged if jf > 0)
+    assert len(sim.judged) == sum(ran)
