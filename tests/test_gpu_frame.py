@@ -75,6 +75,23 @@ def test_c1_first_frames_bit_exact(gpu):
         assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["fsc"], b["fsc"])
 
 
+def test_c2_frames_match_oracle(gpu):
+    """The benchmark workload itself (N=10,000, 128^3 JFA every frame, 20
+    pressure iterations, Brownian noise): three whole frames against the
+    oracle's frame, integer state bit-exact."""
+    from foam import FoamSim
+    from oracle.frame import OracleFoam
+    cfg = fc.preset("c2")
+    pos, rad = fi.for_config(cfg)
+    sim = FoamSim(cfg, pos, rad)
+    ref = OracleFoam(cfg, pos, rad)
+    for f in range(3):
+        print(f"[c2 frames] frame {f}", flush=True)
+        a, b = sim.step(), ref.step()
+        assert a["pbd_passes"] == b["pbd_passes"] and a["growth"] == b["growth"], f
+        compare(sim.snapshot(), ref.snapshot(), f)
+
+
 def test_snapshot_roundtrip(gpu, tmp_path):
     from foam import FoamSim
     cfg = small_cfg()
