@@ -74,6 +74,40 @@ def copy_calibration():
     torch.cuda.empty_cache()
 
 
+def copy_bandwidth(reps=10):
+    """Achieved HBM bandwidth of a plain int32 copy of COPY_CAL_RES^3 elements
+    (256,000,000 B read and 256,000,000 B written per launch), timed with HIP
+    events: the attainable ceiling beside the 8 TB/s peak."""
+    import torch
+
+    import fos_lib as fl
+    n = COPY_CAL_RES ** 3
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    fl.call("fos_jfa_copy", COPY_CAL_RES, fl.ptr(src), fl.ptr(dst), fl.stream())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fl.call("fos_jfa_copy", COPY_CAL_RES, fl.ptr(src), fl.ptr(dst), fl.stream())
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(2.0 * 4 * n / (ms / 1e3) / 1e9, 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,7 +320,7 @@ def ensemble_bench(B, steps, warmup, sample=8):
                                             round(1e3 * max(seq), 3)]}}
 
 
-def cpu_baseline(cfg, pos, rad, frames):
+def cpu_baseline(cfg, pos, rad, frames, frames_1t=2):
     from oracle import oracle as O
     from oracle.frame import OracleFoam
     threads = len(os.sched_getaffinity(0))
@@ -296,11 +330,22 @@ def cpu_baseline(cfg, pos, rad, frames):
     for _ in range(frames):
         ref.step()
     dt = time.perf_counter() - t0
-    return {"value": round(frames / dt, 4), "unit": "frames/s", "cores": threads,
-            "kind": "port",
-            "sample": f"first {frames} frames of the same N=10k/128^3 workload (JFA every "
-                      f"frame), C restatement of the reference loops, OpenMP on {threads} "
-                      f"host threads, {dt:.2f} s"}
+    out = {"value": round(frames / dt, 4), "unit": "frames/s", "cores": threads,
+           "kind": "port", "cpu_model": cpu_model(),
+           "sample": f"first {frames} frames of the same N=10k/128^3 workload (JFA every "
+                     f"frame), C restatement of the reference loops, OpenMP on {threads} "
+                     f"host threads, {dt:.2f} s"}
+    if frames_1t > 0:
+        O.set_threads(1)
+        ref = OracleFoam(cfg, pos, rad)
+        t0 = time.perf_counter()
+        for _ in range(frames_1t):
+            ref.step()
+        dt1 = time.perf_counter() - t0
+        O.set_threads(threads)
+        out["one_thread"] = {"value": round(frames_1t / dt1, 4), "unit": "frames/s",
+                             "sample": f"first {frames_1t} frames, 1 thread, {dt1:.2f} s"}
+    return out
 
 
 def main():
@@ -351,6 +396,7 @@ def main():
                      "avg_launch_us": round(avg_pass_s * 1e6, 3),
                      "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)],
                      "timed_frames_with_events": len(pass_ms),
+                     "copy_measured_gbs": copy_bandwidth(),
                      "limiter": "instruction issue: VALU 60-80% and texture-address 63-68% busy "
                                 "(profiles/k_jfa_pass_list_c2_pmc.md)"},
         "gvox_pass_per_s": round(gvox, 2),
