@@ -193,3 +193,51 @@ def test_dropin_grid_and_dynamics(gpu, orc):
     want = orc.pbd_jacobi(p, rad, n, G_, 1.0, fc.f32(1.015), 0.2, 2.0, cs, cc, ci)
     assert np.array_equal(tp.cpu().numpy(), want)
     assert dynamics.hash_func(7, 3) == orc.lcg_hash(7, 3)
+
+
+def test_runs_are_deterministic(gpu):
+    """Kernels are deterministic by construction (atomics only on integers
+    or order-free maxima, fp32 sums in fixed orders): two fresh runs of the
+    same frames agree bit for bit in every state array, floats included --
+    the plain C2 frame, the learned-predictor frame path and a batch."""
+    import face_predictor as fp
+    from ensemble import Ensemble
+    from foam import FoamSim
+
+    def plain():
+        cfg = fc.preset("c2")
+        pos, rad = fi.for_config(cfg)
+        sim = FoamSim(cfg, pos, rad)
+        for _ in range(4):
+            sim.step()
+        return [sim.snapshot()]
+
+    def predicted():
+        cfg = fc.preset("c5s")
+        cfg.n, cfg.res, cfg.judge_every, cfg.pressure_iters = 3000, 64, 3, 5
+        cfg.r_min, cfg.r_max = 0.012, 0.03
+        pos, rad = fi.uniform(cfg.n, 0.015, 0.025, seed=9)
+        sim = FoamSim(cfg, pos, rad)
+        sim.set_predictor(fp.init_params(seed=3), mult=2.0)
+        for _ in range(5):
+            sim.step()
+        return [sim.snapshot()]
+
+    def batch():
+        cfgs, inputs = [], []
+        for b, (n, res) in enumerate(((500, 40), (1500, 48), (900, 32))):
+            cfg = small_cfg()
+            cfg.n, cfg.res, cfg.seed = n, res, 5 + b
+            cfgs.append(cfg)
+            inputs.append(fi.uniform(n, 0.03, 0.06, seed=60 + b))
+        ens = Ensemble(cfgs, inputs)
+        for _ in range(3):
+            ens.step()
+        return [ens.snapshot(b) for b in range(len(cfgs))]
+
+    for run in (plain, predicted, batch):
+        print(f"[determinism] {run.__name__}", flush=True)
+        a, b = run(), run()
+        for sa, sb in zip(a, b):
+            for k in sa:
+                assert np.array_equal(sa[k], sb[k]), (run.__name__, k)
