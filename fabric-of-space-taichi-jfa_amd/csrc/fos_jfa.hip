@@ -1742,6 +1742,9 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
   const int64_t V = (int64_t)res * res * res;
   unsigned b = blocks_for(V);
   if (b > 2048) b = 2048;
+  // without early exit every pass runs: an even count leaves the labels in
+  // grid, and one block writes the totals
+  if (!early_exit && num_passes % 2 == 0) b = 1;
   hipLaunchKernelGGL(k_jfa_finish, dim3(b), dim3(FOS_BLOCK), 0, st, res, num_passes,
                      early_exit, th_pass_delta, min_passes, (const int *)ctx->stripes, info,
                      temp, grid);
