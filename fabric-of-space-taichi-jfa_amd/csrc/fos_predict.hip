@@ -314,11 +314,14 @@ __global__ void __launch_bounds__(256)
     const float ul = rsqrtf(dot3f(ux, uy, uz));
     ux *= ul; uy *= ul; uz *= ul;
     const float vx = ny * uz - nz * uy, vy = nz * ux - nx * uz, vz = nx * uy - ny * ux;
-    // nearest PRED_NEAR planes (unit normal in the plane, offset), ascending
+    // nearest PRED_NEAR planes, ascending offset: the insertion keeps only
+    // (offset, candidate) and the unit normals of the winners are recomputed
+    // afterwards with the same expressions (same bits, fewer swaps)
     float pa[PRED_NEAR], pb[PRED_NEAR], pc[PRED_NEAR];
+    int pq[PRED_NEAR];
 #pragma unroll
     for (int h = 0; h < PRED_NEAR; ++h) {
-      pa[h] = 1.0f; pb[h] = 0.0f; pc[h] = 1e30f;
+      pa[h] = 1.0f; pb[h] = 0.0f; pc[h] = 1e30f; pq[h] = -1;
     }
     bool empty = false;
     int wit = 0;
@@ -346,14 +349,26 @@ __global__ void __launch_bounds__(256)
       const float inv = rsqrtf(q2);
       float ic = C * inv;
       if (!(ic < pc[PRED_NEAR - 1])) continue;
-      float ia = A * inv, ib = B * inv;
+      int iq = q;
 #pragma unroll
       for (int h = 0; h < PRED_NEAR; ++h) {
         const bool sw = ic < pc[h];
-        const float ta = pa[h], tb = pb[h], tc = pc[h];
-        pa[h] = sw ? ia : ta; pb[h] = sw ? ib : tb; pc[h] = sw ? ic : tc;
-        ia = sw ? ta : ia; ib = sw ? tb : ib; ic = sw ? tc : ic;
+        const float tc = pc[h];
+        const int tq = pq[h];
+        pc[h] = sw ? ic : tc; pq[h] = sw ? iq : tq;
+        ic = sw ? tc : ic; iq = sw ? tq : iq;
       }
+    }
+#pragma unroll
+    for (int h = 0; h < PRED_NEAR; ++h) {
+      if (pq[h] < 0) continue;
+      const float4 kq = sk[pq[h]];
+      const float kx = kq.x, ky = kq.y, kz = kq.z;
+      const float A = 2.0f * (kx * ux + ky * uy + kz * uz);
+      const float B = 2.0f * (kx * vx + ky * vy + kz * vz);
+      const float inv = rsqrtf(A * A + B * B);
+      pa[h] = A * inv;
+      pb[h] = B * inv;
     }
     // Area of {x : a_h x <= c_h} over the kept planes and the four sides of a
     // square of half-size clip_r*(ri+rj): each line contributes the edge it
