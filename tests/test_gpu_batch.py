@@ -160,3 +160,25 @@ def test_batch_tile_autotuner(gpu):
         ga, gb = a.snapshot(k), b.snapshot(k)
         for key in ga:
             assert np.array_equal(ga[key], gb[key]), (k, key)
+
+
+def test_c4_configs_match_single_foams(gpu):
+    """The ensemble workload's own foam configurations (N log-uniform in
+    [500, 20000], JFA edge round(5 N^(1/3)), per-foam radius bounds, FSC
+    bands and seeds; 8 foams): every foam equals its own FoamSim run, bit for
+    bit, over three frames."""
+    import ensemble as en
+    from ensemble import Ensemble
+    from foam import FoamSim
+    cfgs = en.c4_configs(8, seed=5)
+    inputs = [en.c4_inputs(c, b) for b, c in enumerate(cfgs)]
+    ens = Ensemble(cfgs, inputs)
+    sims = [FoamSim(c, p, r) for c, (p, r) in zip(cfgs, inputs)]
+    print("[c4 configs] n =", [c.n for c in cfgs], "res =", [c.res for c in cfgs], flush=True)
+    for f in range(3):
+        ens.step()
+        for b, sim in enumerate(sims):
+            sim.step()
+            got, want = ens.snapshot(b), sim.snapshot()
+            for k in got:
+                assert np.array_equal(got[k], want[k]), (f, b, k)
