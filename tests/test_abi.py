@@ -55,3 +55,14 @@ def test_calls_fail_loudly_without_a_gpu():
         pytest.skip("GPU present")
     with pytest.raises(fos_lib.FosError):
         fos_lib.Context(10, 8, 27)
+
+
+def test_bench_rejects_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with status 2 and says so
+    (before touching a GPU)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "4"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
