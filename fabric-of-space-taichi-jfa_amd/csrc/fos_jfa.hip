@@ -29,13 +29,24 @@ __global__ void k_seed_table(const float *__restrict__ pos,
 // lexicographic minimum of (power distance to the voxel centre, index) over
 // the particles seeded there.  A CAS loop reaches that minimum under any
 // interleaving because the order is total.
+// The seed table and the seeding in one launch: thread i writes its own seed
+// entry, and a particle already holding the voxel has its entry rebuilt from
+// its position and radius with the same operations (the same bits), so no
+// thread reads an entry another is writing.
+__device__ __forceinline__ float4 seed_entry(const float *__restrict__ pos,
+                                             const float *__restrict__ rad, int i, float beta) {
+  const float b = beta * rad[i];
+  return make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], b * b);
+}
+
 template <bool UNIT_L>
-__global__ void k_seed(const float *__restrict__ pos,
-                       const float4 *__restrict__ seeds, int n, int res,
-                       float L, float half_l, float voxel,
-                       int32_t *__restrict__ grid) {
+__global__ void k_seed_fused(const float *__restrict__ pos, const float *__restrict__ rad,
+                             float beta, float4 *__restrict__ seeds, int n, int res, float L,
+                             float half_l, float voxel, int32_t *__restrict__ grid) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float4 si = seed_entry(pos, rad, i, beta);
+  seeds[i] = si;
   int c[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -45,11 +56,11 @@ __global__ void k_seed(const float *__restrict__ pos,
   const int64_t v = ((int64_t)c[0] * res + c[1]) * res + c[2];
   const float cx = vcenter(c[0], voxel, half_l), cy = vcenter(c[1], voxel, half_l),
               cz = vcenter(c[2], voxel, half_l);
-  const float pi = power_dist<UNIT_L>(cx, cy, cz, seeds[i], L);
+  const float pi = power_dist<UNIT_L>(cx, cy, cz, si, L);
   int cur = __hip_atomic_load(&grid[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   while (true) {
     if (cur != -1) {
-      float pc = power_dist<UNIT_L>(cx, cy, cz, seeds[cur], L);
+      float pc = power_dist<UNIT_L>(cx, cy, cz, seed_entry(pos, rad, cur, beta), L);
       bool better = (pi < pc) || (pi == pc && i < cur);
       if (!better) return;
     }
@@ -954,13 +965,28 @@ __global__ void __launch_bounds__(256)
 
 // One wave per row: rank entries by key (keys are unique within a row), write
 // the first 32 into face_ids/face_counts, clear the rest, reset the table.
+// in_count (may be NULL): also clears what finalize accumulates into (as
+// k_fin_clear does), for a finalize that follows at once.
 __global__ void __launch_bounds__(256)
     k_face_compact(int n, int *tb, unsigned *tk, int *tc, int cap,
                    int32_t *__restrict__ face_ids,
-                   int32_t *__restrict__ face_counts) {
+                   int32_t *__restrict__ face_counts, int *__restrict__ in_count,
+                   int *__restrict__ in_fill, unsigned *__restrict__ dec,
+                   int *__restrict__ chg, int nchg) {
   extern __shared__ unsigned s_keys[];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + wid;
+  if (in_count) {
+    if (row < n && lane == 0) {
+      in_count[row] = 0;
+      in_fill[row] = 0;
+      dec[row] = 0u;
+    }
+    if (blockIdx.x == 0) {
+      if (threadIdx.x == 0) in_count[n] = 0;
+      for (int t = (int)threadIdx.x; t < nchg; t += 256) chg[t] = 0;
+    }
+  }
   unsigned *keys = s_keys + wid * cap;
   const bool live = row < n;
   const int64_t base = (int64_t)row * cap;
@@ -1428,16 +1454,15 @@ int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
     if (rc) return rc;
   }
   if (n == 0) return 0;
-  rc = fos_jfa_seed_table(ctx, pos, rad, n, beta, stream);
-  if (rc) return rc;
+  FOS_REQUIRE(pos && rad, "fos_jfa_init: NULL pos or rad");
   const float half_l = (float)(0.5 * (double)L);
   if (unit_l(L))
-    hipLaunchKernelGGL(k_seed<true>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st,
-                       pos, ctx->seeds, n, res, L, half_l, voxel, grid);
+    hipLaunchKernelGGL(k_seed_fused<true>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos,
+                       rad, beta, ctx->seeds, n, res, L, half_l, voxel, grid);
   else
-    hipLaunchKernelGGL(k_seed<false>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st,
-                       pos, ctx->seeds, n, res, L, half_l, voxel, grid);
-  FOS_CHECK_LAUNCH("k_seed");
+    hipLaunchKernelGGL(k_seed_fused<false>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos,
+                       rad, beta, ctx->seeds, n, res, L, half_l, voxel, grid);
+  FOS_CHECK_LAUNCH("k_seed_fused");
   return 0;
 }
 
@@ -1599,12 +1624,18 @@ static int face_grow(fos_ctx *ctx, hipStream_t st) {
   return fos_fill_i32(ctx->flags, 1, 0, st);
 }
 
+static int *fin_chg(fos_ctx *ctx);
+#define FIN_RING 48
+
+// clear_fin: also clear the finalize accumulators (a finalize follows).
 static int face_compact(fos_ctx *ctx, int n, int32_t *face_ids, int32_t *face_counts,
-                        hipStream_t st) {
+                        hipStream_t st, bool clear_fin = false) {
   const int cap = ctx->tab_cap;
   const size_t shmem = (size_t)4 * cap * sizeof(unsigned);
   hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st, n,
-                     ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts);
+                     ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts,
+                     clear_fin ? ctx->in_count : nullptr, ctx->in_fill, (unsigned *)ctx->dec,
+                     fin_chg(ctx), FIN_RING);
   FOS_CHECK_LAUNCH("k_face_compact");
   return 0;
 }
@@ -1841,22 +1872,26 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
 
 }  // extern "C"
 
-// Per-pair change flags of the fixed point: a ring at ctx->flags + 16.
-#define FIN_RING 48
+// Per-pair change flags of the fixed point: a ring of FIN_RING at
+// ctx->flags + 16.
 #define FIN_BURST 4
-static inline int *fin_chg(fos_ctx *ctx) { return ctx->flags + 16; }
+static int *fin_chg(fos_ctx *ctx) { return ctx->flags + 16; }
 
 // Finalize in phases: prepare (classify, incoming lists, decisions cleared),
 // round pairs, materialize.
+// cleared: the accumulators were cleared already (face_compact with clear_fin).
 static int fin_prepare(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
                        const int32_t *face_ids, const int32_t *face_counts,
-                       const float *prev_scores, hipStream_t st) {
+                       const float *prev_scores, hipStream_t st, bool cleared = false) {
   const int64_t m = (int64_t)n * FOS_MAX_NEIGHBORS;
   // the fixed point starts from "nothing accepted"
-  hipLaunchKernelGGL(k_fin_clear, dim3(blocks_for((int64_t)(n + 1 > FIN_RING ? n + 1 : FIN_RING))),
-                     dim3(FOS_BLOCK), 0, st, n, ctx->in_count, ctx->in_fill,
-                     (unsigned *)ctx->dec, fin_chg(ctx), FIN_RING);
-  FOS_CHECK_LAUNCH("k_fin_clear");
+  if (!cleared) {
+    hipLaunchKernelGGL(k_fin_clear,
+                       dim3(blocks_for((int64_t)(n + 1 > FIN_RING ? n + 1 : FIN_RING))),
+                       dim3(FOS_BLOCK), 0, st, n, ctx->in_count, ctx->in_fill,
+                       (unsigned *)ctx->dec, fin_chg(ctx), FIN_RING);
+    FOS_CHECK_LAUNCH("k_fin_clear");
+  }
   hipLaunchKernelGGL(k_fin_classify, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
                      rad, voxel, pvox, face_ids, face_counts, prev_scores, ctx->score,
                      ctx->cls, ctx->in_count);
@@ -2017,8 +2052,9 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
   // neighbour lists and hysteresis scores are written by materialize).
   while (true) {
     int rc = face_emit(ctx, res, n, filtered, st);
-    if (!rc) rc = face_compact(ctx, n, face_ids, face_counts, st);
-    if (!rc) rc = fin_prepare(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores, st);
+    if (!rc) rc = face_compact(ctx, n, face_ids, face_counts, st, true);
+    if (!rc) rc = fin_prepare(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores, st,
+                              true);
     const int first = fin_first_burst(ctx);
     if (!rc) rc = fin_pairs(ctx, n, 0, first, false, st);
     if (rc) return rc;
@@ -2468,7 +2504,8 @@ int fos_batch_face_accumulate(fos_batch *bt, fos_ctx *ctx, const int32_t *filter
   const int cap = ctx->tab_cap;
   const size_t shmem = (size_t)4 * cap * sizeof(unsigned);
   hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st, n,
-                     ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts);
+                     ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts,
+                     (int *)nullptr, (int *)nullptr, (unsigned *)nullptr, (int *)nullptr, 0);
   FOS_CHECK_LAUNCH("k_face_compact");
   return 0;
 }
