@@ -92,6 +92,31 @@ def test_c2_frames_match_oracle(gpu):
         compare(sim.snapshot(), ref.snapshot(), f)
 
 
+def test_frame_start_reuses_the_end_of_frame_grid(gpu):
+    """A frame starts from the CSR the last frame's final rebuild left while
+    nothing has written the positions since: that CSR equals a fresh rebuild,
+    and a torch store into the positions makes the next frame rebuild."""
+    from foam import FoamSim
+    cfg = small_cfg()
+    pos, rad = fi.uniform(cfg.n, 0.03, 0.06, seed=11)
+    sim = FoamSim(cfg, pos, rad)
+    sim.step()
+    assert sim.grid_key == sim.grid_token()
+    kept = [t.clone() for t in (sim.cell_count, sim.cell_start, sim.cell_indices, sim.pos)]
+    sim.rebuild_grid()
+    fresh = (sim.cell_count, sim.cell_start, sim.cell_indices, sim.pos)
+    for a, b in zip(kept, fresh):
+        assert torch.equal(a, b)
+    sim.pos[0, 0] += 0.25
+    assert sim.grid_key != sim.grid_token()
+    # a copy with the same store and a forced rebuild steps identically
+    twin = FoamSim(cfg, pos, rad)
+    twin.restore(dict(sim.snapshot()))
+    sim.step()
+    twin.step()
+    compare(sim.snapshot(), twin.snapshot(), "after a store")
+
+
 def test_snapshot_roundtrip(gpu, tmp_path):
     from foam import FoamSim
     cfg = small_cfg()

@@ -32,6 +32,31 @@ def test_grid_is_stable_counting_sort(orc):
     assert np.allclose(np.mod(p - pos + 0.5, 1.0), 0.5, atol=1e-6)
 
 
+def test_rebuild_of_wrapped_positions_changes_nothing(orc):
+    """grid.py:30-48 wrapP with L = 1 is idempotent in fp32, so a rebuild of
+    positions a rebuild already wrapped reproduces positions and CSR bit for
+    bit (the frame driver relies on it to reuse the end-of-frame grid).
+    Exhaustive over every fp32 value with 0.25 <= |p| < 1 (where p + 0.5
+    rounds), plus boundary values through the oracle's rebuild."""
+    def wrap(p):
+        return p - F(1.0) * np.floor(p * F(1.0) + F(0.5))
+    lo, hi = F(0.25).view(np.uint32), F(1.0).view(np.uint32)
+    u = np.arange(lo, hi, dtype=np.uint32)
+    for sign in (0, 0x80000000):
+        q = wrap((u | np.uint32(sign)).view(np.float32))
+        assert (wrap(q).view(np.uint32) == q.view(np.uint32)).all()
+    edge = np.array([0.5 - 2.0 ** -25, 0.5, -0.5, 0.5 - 2.0 ** -24, -0.5 - 2.0 ** -24, 1.5,
+                     -1.5, 0.0, -0.0, 2.0 ** -30], np.float32)
+    rng = np.random.default_rng(4)
+    pos = np.concatenate([np.repeat(edge, 3),
+                          ((rng.random(3 * 500) - 0.5) * 6).astype(F)]).reshape(-1, 3)
+    n, G = pos.shape[0], 13
+    p1, cc1, cs1, ci1 = orc.grid_build(pos, n, G, 1.0)
+    p2, cc2, cs2, ci2 = orc.grid_build(p1, n, G, 1.0)
+    assert (p1.view(np.uint32) == p2.view(np.uint32)).all()
+    assert (cc1 == cc2).all() and (cs1 == cs2).all() and (ci1 == ci2).all()
+
+
 # ---- dynamics.py:77-171 PBD ----------------------------------------------------
 
 def pbd_args(cfg_gap=0.015):
