@@ -168,6 +168,7 @@ struct fos_ctx {
   int max_res;
   int max_cells;
   int fin_burst;  // round pairs the last finalize needed (0: none yet); sizes the first burst
+  int stripes_clear;  // 1: fos_jfa_init has queued the clear of pass slots 0..FOS_PASS_SLOTS-2
   // seed table (x, y, z, (beta*r)^2)
   float4 *seeds;
   // per-wave change masks of the last two JFA passes, [max_res^2][ceil(max_res/64)]

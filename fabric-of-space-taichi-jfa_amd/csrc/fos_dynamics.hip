@@ -498,10 +498,12 @@ __device__ __forceinline__ float box_muller(uint32_t a, uint32_t b) {
   return (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2));
 }
 
-// dynamics.py:1616-1634 with Philox noise keyed (seed, frame, i)
+// dynamics.py:1616-1634 with Philox noise keyed (seed, frame, i).
+// cell_of (may be NULL): also runs the first step of the grid rebuild that
+// follows (grid.py:285-309: wrap again, store, hash into a G^3 grid).
 __global__ void k_brownian(float *__restrict__ pos, int n, float strength, float L,
                            float half_l, uint32_t k0, uint32_t k1, int frame,
-                           int mode) {
+                           int mode, int32_t *__restrict__ cell_of, int G, float inv_l) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint4 r0 = philox(make_uint4((unsigned)i, (unsigned)frame, 0u, 0u), k0, k1);
@@ -516,6 +518,7 @@ __global__ void k_brownian(float *__restrict__ pos, int n, float strength, float
     d[1] = box_muller(r0.z, r0.w) * strength;
     d[2] = box_muller(r1.x, r1.y) * strength;
   }
+  float w[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     float p = pos[3 * i + a] + d[a];
@@ -523,8 +526,12 @@ __global__ void k_brownian(float *__restrict__ pos, int n, float strength, float
       p = p + L;
     else if (p >= half_l)
       p = p - L;
-    pos[3 * i + a] = p;
+    w[a] = cell_of ? gwrap(p, L, inv_l) : p;
+    pos[3 * i + a] = w[a];
   }
+  if (cell_of)
+    cell_of[i] = hash_axis(w[0], G, half_l, inv_l) * G * G + hash_axis(w[1], G, half_l, inv_l) * G +
+                 hash_axis(w[2], G, half_l, inv_l);
 }
 
 // dynamics.py:620-660
@@ -908,7 +915,23 @@ int fos_brownian(float *pos, int n, float strength, float L, uint64_t seed,
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_brownian, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
                      (hipStream_t)stream, pos, n, strength, L, L * 0.5f,
-                     (uint32_t)seed, (uint32_t)(seed >> 32), frame, mode);
+                     (uint32_t)seed, (uint32_t)(seed >> 32), frame, mode, (int32_t *)nullptr, 0,
+                     0.0f);
+  FOS_CHECK_LAUNCH("k_brownian");
+  return 0;
+}
+
+int fos_brownian_cells(fos_ctx *ctx, float *pos, int n, float strength, float L, uint64_t seed,
+                       int frame, int mode, int G, void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && n <= ctx->max_n && (mode == 0 || mode == 1) && L > 0.0f &&
+                  G >= 3 && G <= 1290 && (int64_t)G * G * G <= ctx->max_cells && (n == 0 || pos),
+              "fos_brownian_cells: bad arguments (n=%d G=%d)", n, G);
+  if (n == 0) return 0;
+  // half_l = L/2 is exact in fp32, so the noise wrap and the hash share it
+  hipLaunchKernelGGL(k_brownian, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, pos, n, strength, L, L * 0.5f,
+                     (uint32_t)seed, (uint32_t)(seed >> 32), frame, mode, ctx->cell_of, G,
+                     (float)(1.0 / (double)L));
   FOS_CHECK_LAUNCH("k_brownian");
   return 0;
 }

@@ -42,8 +42,11 @@ __device__ __forceinline__ float4 seed_entry(const float *__restrict__ pos,
 template <bool UNIT_L>
 __global__ void k_seed_fused(const float *__restrict__ pos, const float *__restrict__ rad,
                              float beta, float4 *__restrict__ seeds, int n, int res, float L,
-                             float half_l, float voxel, int32_t *__restrict__ grid) {
+                             float half_l, float voxel, int32_t *__restrict__ grid,
+                             int *__restrict__ counters, int ncounters) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // the pass counters of the fos_jfa_passes call that follows
+  for (int t = i; t < ncounters; t += (int)(gridDim.x * blockDim.x)) counters[t] = 0;
   if (i >= n) return;
   const float4 si = seed_entry(pos, rad, i, beta);
   seeds[i] = si;
@@ -1456,13 +1459,17 @@ int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
   if (n == 0) return 0;
   FOS_REQUIRE(pos && rad, "fos_jfa_init: NULL pos or rad");
   const float half_l = (float)(0.5 * (double)L);
+  // Pass slots 0 .. FOS_PASS_SLOTS - 2 (what fos_jfa_passes uses) are
+  // cleared here too; fos_jfa_passes skips its own clear while that holds.
+  const int ncl = (FOS_PASS_SLOTS - 1) * FOS_STRIPE_PASS_INTS;
   if (unit_l(L))
     hipLaunchKernelGGL(k_seed_fused<true>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos,
-                       rad, beta, ctx->seeds, n, res, L, half_l, voxel, grid);
+                       rad, beta, ctx->seeds, n, res, L, half_l, voxel, grid, ctx->stripes, ncl);
   else
     hipLaunchKernelGGL(k_seed_fused<false>, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos,
-                       rad, beta, ctx->seeds, n, res, L, half_l, voxel, grid);
+                       rad, beta, ctx->seeds, n, res, L, half_l, voxel, grid, ctx->stripes, ncl);
   FOS_CHECK_LAUNCH("k_seed_fused");
+  ctx->stripes_clear = 1;
   return 0;
 }
 
@@ -1747,8 +1754,12 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
   FOS_REQUIRE(num_passes < FOS_PASS_SLOTS, "fos_jfa_passes: at most %d passes",
               FOS_PASS_SLOTS - 1);
   hipStream_t st = (hipStream_t)stream;
-  int rc = fos_fill_i32(ctx->stripes, (int64_t)num_passes * FOS_STRIPE_PASS_INTS, 0, stream);
-  if (rc) return rc;
+  int rc = 0;
+  if (!ctx->stripes_clear) {
+    rc = fos_fill_i32(ctx->stripes, (int64_t)num_passes * FOS_STRIPE_PASS_INTS, 0, stream);
+    if (rc) return rc;
+  }
+  ctx->stripes_clear = 0;
   // A pass is settled when its step equals the previous pass's; it recomputes
   // only next to the previous pass's changes when that pass was settled too
   // (after the first repeat the changes are few; the first step-1 pass after

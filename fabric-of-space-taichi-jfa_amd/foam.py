@@ -252,13 +252,19 @@ class FoamSim:
         # 3 pressure
         if c.pressure:
             self.pressure(f)
-        # 4 Brownian
+        # 4 Brownian; with PBD passes to follow, the noise kernel also runs
+        # the wrap-and-hash step of the first pass's rebuild
+        ready = False
         if c.brownian:
-            fl.call("fos_brownian", fl.ptr(self.pos), self.n, c.brownian_strength, c.L,
-                    c.seed, f, c.brownian_mode, st)
+            if passes > 0:
+                fl.call("fos_brownian_cells", self.ctx.handle, fl.ptr(self.pos), self.n,
+                        c.brownian_strength, c.L, c.seed, f, c.brownian_mode, self.G, st)
+                ready = True
+            else:
+                fl.call("fos_brownian", fl.ptr(self.pos), self.n, c.brownian_strength, c.L,
+                        c.seed, f, c.brownian_mode, st)
         # 5 PBD; a pass followed directly by a rebuild runs that rebuild's
         # wrap-and-hash step itself
-        ready = False
         for p in range(passes):
             if ready:
                 self.rebuild_from_cells(pr=True)

@@ -92,7 +92,8 @@ int fos_jfa_seed_table(fos_ctx *ctx, const float *pos, const float *rad,
                        int n, float beta, void *stream);
 /* jfa.py:485-520 jfa_init: clears grid and temp (res^3) to -1, builds the
  * seed table, seeds every particle's voxel (argmin power distance, lowest
- * index on ties). */
+ * index on ties).  With n > 0 it also clears the context's pass counters, so
+ * the next fos_jfa_passes on the same stream skips that clear. */
 int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
                  int res, float L, float voxel, float beta, int32_t *grid,
                  int32_t *temp, void *stream);
@@ -284,6 +285,13 @@ int fos_pressure_stats(fos_ctx *ctx, const float *rad, int n, float *out,
  * form); mode 1: Gaussian with sigma = strength. */
 int fos_brownian(float *pos, int n, float strength, float L, uint64_t seed,
                  int frame, int mode, void *stream);
+/* fos_brownian followed by the first step of the fos_grid_build that comes
+ * next in the frame (grid.py:285-309 position wrap and cell index into a
+ * G^3 grid, kept in the context); fos_grid_build_from_cells completes that
+ * rebuild.  Results equal fos_brownian + fos_grid_build bit for bit.  No
+ * reference counterpart (a launch saved per frame). */
+int fos_brownian_cells(fos_ctx *ctx, float *pos, int n, float strength, float L, uint64_t seed,
+                       int frame, int mode, int G, void *stream);
 /* dynamics.py:607-660 apply_brownian (Ornstein-Uhlenbeck velocity update with
  * Philox uniforms keyed (seed, frame, particle)) fused with integrate_jitter;
  * pos == NULL updates v_jit only.  mean_r is the host-computed mean radius

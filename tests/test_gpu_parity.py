@@ -786,6 +786,66 @@ def test_noise_kernels(gpu, orc):
     assert np.array_equal(host(tv), wv) and np.array_equal(host(tp), wp)
 
 
+def test_brownian_cells_equals_brownian_then_rebuild(gpu):
+    """fos_brownian_cells + fos_grid_build_from_cells (with slot records)
+    equals fos_brownian + fos_grid_build_pr bit for bit, both noise modes,
+    with particles pushed across the box faces."""
+    n, G_ = 5000, 17
+    pos, rad = fi.uniform(n, 0.01, 0.03, seed=21)
+    pos[:50, 0] = F(0.49995)
+    pos[50:100, 2] = F(-0.5)
+    for mode, s in ((0, 2e-4), (1, 1e-4)):
+        outs = []
+        for fused in (False, True):
+            g = G(gpu, n, 1, G_ ** 3)
+            tp, tr = dev(pos), dev(rad)
+            cc = torch.empty(G_ ** 3, dtype=torch.int32, device="cuda")
+            cs, ci = torch.empty_like(cc), torch.empty(n, dtype=torch.int32, device="cuda")
+            rec = torch.empty(n, 4, dtype=torch.float32, device="cuda")
+            st = gpu.stream()
+            if fused:
+                g.call("fos_brownian_cells", g.h, g.ptr(tp), n, s, 1.0, 777, 5, mode, G_, st)
+                g.call("fos_grid_build_from_cells", g.h, g.ptr(tp), g.ptr(tr), n, G_, g.ptr(cc),
+                       g.ptr(cs), g.ptr(ci), g.ptr(rec), st)
+            else:
+                g.call("fos_brownian", g.ptr(tp), n, s, 1.0, 777, 5, mode, st)
+                g.call("fos_grid_build_pr", g.h, g.ptr(tp), g.ptr(tr), n, G_, 1.0, g.ptr(cc),
+                       g.ptr(cs), g.ptr(ci), g.ptr(rec), st)
+            outs.append([host(t) for t in (tp, cc, cs, ci, rec)])
+        for a, b in zip(*outs):
+            assert np.array_equal(a.view(np.uint32) if a.dtype == F else a,
+                                  b.view(np.uint32) if b.dtype == F else b), mode
+
+
+def test_jfa_passes_counters_after_init_and_alone(gpu, orc):
+    """fos_jfa_init clears the pass counters for the fos_jfa_passes call that
+    follows; a second fos_jfa_passes call without an init clears them itself.
+    Both give the oracle's labels and per-pass change counts (early exit on,
+    so stale counts would also change the schedule)."""
+    n, res = 2000, 48
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=1)
+    voxel = fc.f32(1.0 / res)
+    P = int(math.ceil(math.log2(res))) + 1
+    lab, passes, early, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, P, True, 0.006, 3)
+    g = G(gpu, n, res)
+    st = gpu.stream()
+    grid = torch.empty(res ** 3, dtype=torch.int32, device="cuda")
+    temp = torch.empty_like(grid)
+    g.call("fos_jfa_init", g.h, g.ptr(dev(pos)), g.ptr(dev(rad)), n, res, 1.0, voxel, 1.0,
+           g.ptr(grid), g.ptr(temp), st)
+    seeded = grid.clone()
+    for call in range(3):
+        grid.copy_(seeded)
+        temp.fill_(-1)
+        info = torch.zeros(66, dtype=torch.int32, device="cuda")
+        g.call("fos_jfa_passes", g.h, res, 1.0, voxel, P, 1, 0.006, 3, g.ptr(grid), g.ptr(temp),
+               g.ptr(info), st)
+        inf = host(info)
+        assert np.array_equal(host(grid), lab), call
+        assert inf[0] == passes and inf[1] == int(early), call
+        assert (inf[2:2 + passes] == hist).all(), call
+
+
 def test_topology_candidates(gpu, orc):
     n, G_ = 3000, 10
     pos, rad = fi.uniform(n, 0.01, 0.03, seed=8)
