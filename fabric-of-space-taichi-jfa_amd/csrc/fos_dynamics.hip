@@ -306,12 +306,11 @@ __global__ void __launch_bounds__(FOS_BLOCK)
 }
 
 // dynamics.py:1399-1427
-__global__ void k_nudge(float *__restrict__ rad, const float *__restrict__ rt,
-                        const float *__restrict__ lmd, int n, float alpha,
-                        float msp, float r_min, float r_max, float back, int mode,
-                        float tol) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void nudge_one(int i, float *__restrict__ rad,
+                                          const float *__restrict__ rt,
+                                          const float *__restrict__ lmd, float alpha, float msp,
+                                          float r_min, float r_max, float back, int mode,
+                                          float tol) {
   const float r0 = rad[i];
   const float t = fminf(r_max, fmaxf(r_min, rt[i]));
   float d = alpha * (t - r0);
@@ -327,6 +326,39 @@ __global__ void k_nudge(float *__restrict__ rad, const float *__restrict__ rt,
   if (d > cap) d = cap;
   if (d < -cap) d = -cap;
   rad[i] = fminf(r_max, fmaxf(r_min, r0 + d));
+}
+
+__global__ void k_nudge(float *__restrict__ rad, const float *__restrict__ rt,
+                        const float *__restrict__ lmd, int n, float alpha,
+                        float msp, float r_min, float r_max, float back, int mode,
+                        float tol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  nudge_one(i, rad, rt, lmd, alpha, msp, r_min, r_max, back, mode, tol);
+}
+
+// run.py:733-751 and 1184-1231 with the settle counter on the device: the
+// counter entering the frame is adj when the last frame skipped the gate
+// (force), else the stored one, reset to adj when the last frame's device
+// gate (maxd[1]) said growth (apply_gate); a positive counter nudges and
+// counts down.  The counter leaving the frame goes to settle_out (a
+// different buffer, so no block reads what another writes).  The global
+// backpressure (mode 1) is run.py:744-749 from the overlap maximum maxd[0],
+// in double as the host computes it, rounded once.
+__global__ void k_settle_nudge(float *__restrict__ rad, const float *__restrict__ rt,
+                               const float *__restrict__ lmd, int n, float alpha, float msp,
+                               float r_min, float r_max, int mode, float tol, double tol_d,
+                               const float *__restrict__ maxd, const int *__restrict__ settle_in,
+                               int *__restrict__ settle_out, int adj, int force,
+                               int apply_gate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = force ? adj : *settle_in;
+  if (apply_gate && maxd[1] > 0.5f) s = adj;
+  const bool active = s > 0;
+  if (i == 0) *settle_out = active ? s - 1 : s;
+  if (!active || i >= n) return;
+  const float back = (float)fmax(0.25, fmin(1.0, (tol_d - (double)maxd[0]) / tol_d));
+  nudge_one(i, rad, rt, lmd, alpha, msp, r_min, r_max, back, mode, tol);
 }
 
 // dynamics.py:1451, int32 wraparound arithmetic
@@ -833,6 +865,24 @@ int fos_nudge_radii(float *rad, const float *rad_target,
                      (hipStream_t)stream, rad, rad_target, local_max_depth, n, alpha,
                      max_step_pct, r_min, r_max, back_global, mode, contact_tol);
   FOS_CHECK_LAUNCH("k_nudge");
+  return 0;
+}
+
+int fos_settle_nudge(float *rad, const float *rad_target, const float *local_max_depth, int n,
+                     float alpha, float max_step_pct, float r_min, float r_max, int mode,
+                     float contact_tol, double contact_tol_d, const float *maxd,
+                     const int32_t *settle_in, int32_t *settle_out, int adjustment_frames,
+                     int force, int apply_gate, void *stream) {
+  FOS_REQUIRE(n >= 0 && maxd && settle_in && settle_out && settle_in != settle_out &&
+                  contact_tol_d != 0.0 && (n == 0 || (rad && rad_target && local_max_depth)),
+              "fos_settle_nudge: bad arguments");
+  // one thread at least, to carry the counter
+  hipLaunchKernelGGL(k_settle_nudge, dim3(blocks_for(n > 0 ? n : 1)), dim3(FOS_BLOCK), 0,
+                     (hipStream_t)stream, rad, rad_target, local_max_depth, n, alpha,
+                     max_step_pct, r_min, r_max, mode, contact_tol, contact_tol_d, maxd,
+                     (const int *)settle_in, (int *)settle_out, adjustment_frames, force,
+                     apply_gate);
+  FOS_CHECK_LAUNCH("k_settle_nudge");
   return 0;
 }
 

@@ -22,10 +22,11 @@ particle reads the pass-start positions.
 All work is enqueued on torch's current stream through libfos.  The host
 reads the device once per frame for the overlap maximum (it sets the PBD pass
 count, as in the reference) and once per JFA frame where the face rows are
-finalized (row-table overflow and the fixed point).  The warm-start gate is
-decided on the device; the host learns it with the next frame's overlap
-maximum (both in one read) and only then sets the settle counter, which the
-next frame is the first to use.
+finalized (row-table overflow and the fixed point).  The warm-start gate and
+the settle counter it resets live on the device, so steps 2-4 (which need
+neither the overlap maximum nor the gate on the host) are queued before the
+host waits for the overlap maximum, behind an event rather than a drain of
+the stream: the GPU works through them while the host reads.
 """
 import math
 
@@ -99,7 +100,16 @@ class FoamSim:
         self.maxd = torch.zeros(2, dtype=torch.float32, device=dev)
         self.ctx = fl.context(n, self.res, C)
         self.frame = 0
-        self.settle = 0
+        # settle counter (run.py:744-751, 1231) on the device, ping-pong:
+        # slot sidx enters the next frame.  settle_force: the last frame set
+        # it to adjustment_frames on the host (no gate); settle_gate: the
+        # last frame's device gate still has to be applied to it.
+        self.settle_dev = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.sidx = 0
+        self.settle_force = False
+        self.settle_gate = False
+        # the overlap maximum and gate, copied to the host without a drain
+        self.maxd_host = torch.zeros(2, dtype=torch.float32).pin_memory()
         self.growth = True
         self.pending = None        # FrameInfo whose gate is still on the device
         # (positions tensor, its version, hash grid edge) when the CSR was
@@ -199,16 +209,15 @@ class FoamSim:
         fl.call("fos_fsc_ema", self.n, c.ema_alpha, fl.ptr(self.fsc), fl.ptr(self.fsc_ema), st)
 
     def resolve(self, gate=None):
-        """Applies the device-decided warm-start gate of the last frame (read
-        now unless the caller read it): settle reset and the frame's growth."""
+        """Records the device-decided warm-start gate of the last frame (read
+        now unless the caller read it) as that frame's growth; the settle
+        reset it implies is applied on the device (fos_settle_nudge)."""
         if self.pending is None:
             return
         if gate is None:
             gate = float(self.maxd[1].item())
         g = gate > 0.5
         self.growth = g
-        if g:
-            self.settle = self.cfg.adjustment_frames
         dict.__setitem__(self.pending, "growth", g)
         self.pending = None
 
@@ -237,32 +246,39 @@ class FoamSim:
         if self.cfg.L != 1.0 or self.grid_key != self.grid_token():
             self.rebuild_grid()
         self.max_overlap()
-        md = self.maxd.cpu()
-        max_depth = float(md[0].item())
-        self.resolve(float(md[1].item()))
-        passes = max(c.pbd_base, min(c.pbd_max,
-                                     int(c.pbd_base + c.pbd_scale * (max_depth / (0.2 * c.r_max)))))
-        # 2 settle
-        if self.settle > 0:
-            back = max(0.25, min(1.0, (c.contact_tol - max_depth) / c.contact_tol))
-            fl.call("fos_nudge_radii", fl.ptr(self.rad), fl.ptr(self.rad_target), fl.ptr(self.lmd),
-                    self.n, c.alpha(), c.max_step_pct, c.r_min, c.r_max, back,
-                    c.backpressure_mode, c.contact_tol, st)
-            self.settle -= 1
+        # the overlap maximum (and the last frame's gate) go to the host
+        # behind an event; settle, pressure and noise need neither on the
+        # host and are queued before the host waits, so the GPU keeps busy
+        self.maxd_host.copy_(self.maxd, non_blocking=True)
+        ready_ev = torch.cuda.Event()
+        ready_ev.record()
+        # 2 settle (counter and gate on the device)
+        sin, sout = self.settle_dev[self.sidx:], self.settle_dev[1 - self.sidx:]
+        fl.call("fos_settle_nudge", fl.ptr(self.rad), fl.ptr(self.rad_target), fl.ptr(self.lmd),
+                self.n, c.alpha(), c.max_step_pct, c.r_min, c.r_max, c.backpressure_mode,
+                c.contact_tol, float(c.contact_tol), fl.ptr(self.maxd), fl.ptr(sin), fl.ptr(sout),
+                c.adjustment_frames, int(self.settle_force), int(self.settle_gate), st)
+        self.sidx = 1 - self.sidx
+        self.settle_force = self.settle_gate = False
         # 3 pressure
         if c.pressure:
             self.pressure(f)
-        # 4 Brownian; with PBD passes to follow, the noise kernel also runs
-        # the wrap-and-hash step of the first pass's rebuild
+        # 4 Brownian; with PBD passes to follow (pbd_base > 0), the noise
+        # kernel also runs the wrap-and-hash step of the first pass's rebuild
         ready = False
         if c.brownian:
-            if passes > 0:
+            if c.pbd_base > 0:
                 fl.call("fos_brownian_cells", self.ctx.handle, fl.ptr(self.pos), self.n,
                         c.brownian_strength, c.L, c.seed, f, c.brownian_mode, self.G, st)
                 ready = True
             else:
                 fl.call("fos_brownian", fl.ptr(self.pos), self.n, c.brownian_strength, c.L,
                         c.seed, f, c.brownian_mode, st)
+        ready_ev.synchronize()
+        max_depth = float(self.maxd_host[0].item())
+        self.resolve(float(self.maxd_host[1].item()))
+        passes = max(c.pbd_base, min(c.pbd_max,
+                                     int(c.pbd_base + c.pbd_scale * (max_depth / (0.2 * c.r_max)))))
         # 5 PBD; a pass followed directly by a rebuild runs that rebuild's
         # wrap-and-hash step itself
         for p in range(passes):
@@ -321,12 +337,13 @@ class FoamSim:
         if gated:
             self.gate_growth(f)
             self.pending = self.last
+            self.settle_gate = True
         else:
             self.growth = True
             fl.call("fos_fsc_targets", fl.ptr(self.rad), fl.ptr(self.rad_target), fl.ptr(self.fsc),
                     fl.ptr(self.fsc_ema), self.n, c.f_low, c.f_high, c.growth_pct, c.r_min,
                     c.r_max, c.deadband, st)
-            self.settle = c.adjustment_frames
+            self.settle_force = True
         if self.telemetry and ran:
             self.last.update(self.jfa_telemetry())
         self.frame += 1
@@ -383,11 +400,23 @@ class FoamSim:
         out["settle"] = np.int64(self.settle)
         return out
 
+    @property
+    def settle(self):
+        """The settle counter the next frame starts from (run.py:744-751,
+        1231), read from the device."""
+        if self.settle_force:
+            return int(self.cfg.adjustment_frames)
+        s = int(self.settle_dev[self.sidx].item())
+        if self.settle_gate and float(self.maxd[1].item()) > 0.5:
+            s = int(self.cfg.adjustment_frames)
+        return s
+
     def restore(self, snap):
         for k in self.STATE:
             getattr(self, k).copy_(torch.from_numpy(np.ascontiguousarray(snap[k])))
         self.frame = int(snap["frame"])
-        self.settle = int(snap["settle"])
+        self.settle_dev[self.sidx] = int(snap["settle"])
+        self.settle_force = self.settle_gate = False
         self.pending = None
         self.grid_key = None
 

@@ -61,6 +61,7 @@ SIGNATURES = {
     "fos_pressure_stats": [P, P, I, P, P],
     "fos_brownian": [P, I, F, F, U64, I, I, P],
     "fos_brownian_cells": [P, P, I, F, F, U64, I, I, I, P],
+    "fos_settle_nudge": [P, P, P, I, F, F, F, F, I, F, D, P, P, P, I, I, I, P],
     "fos_ou_jitter": [P, P, P, I, F, F, F, F, F, F, U64, I, P],
     "fos_integrate_jitter": [P, P, I, F, F, P],
     "fos_xsph": [P, P, P, I, I, F, F, F, P, P, P, P],

@@ -259,6 +259,20 @@ int fos_nudge_radii(float *rad, const float *rad_target,
                     float max_step_pct, float r_min, float r_max,
                     float back_global, int mode, float contact_tol,
                     void *stream);
+/* run.py:733-751 settle step with its counter on the device, so the frame
+ * needs no host read before it: the counter entering the frame is
+ * adjustment_frames when force (the last frame set it without a gate), else
+ * *settle_in, reset to adjustment_frames when apply_gate and the last
+ * frame's device gate maxd[1] (run.py:1184-1231, fos_fsc_gate) is > 0.5.  A
+ * positive counter runs nudge_radii_adaptive_ema (dynamics.py:1366-1427;
+ * mode 1 takes the global backpressure run.py:744-749 from the overlap
+ * maximum maxd[0], computed in double with contact_tol_d) and counts down.
+ * The counter leaving the frame is written to *settle_out (!= settle_in). */
+int fos_settle_nudge(float *rad, const float *rad_target, const float *local_max_depth, int n,
+                     float alpha, float max_step_pct, float r_min, float r_max, int mode,
+                     float contact_tol, double contact_tol_d, const float *maxd,
+                     const int32_t *settle_in, int32_t *settle_out, int adjustment_frames,
+                     int force, int apply_gate, void *stream);
 /* dynamics.py:1454-1557 equilibrate_pressure, one Jacobi iteration: delta_r
  * from the current radii, then the clamped apply.  stats (device float[2])
  * receives {max|dr|, changed count as float bits of an int32} and is

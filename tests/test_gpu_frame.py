@@ -37,7 +37,7 @@ def small_cfg(**kw):
     return cfg
 
 
-@pytest.mark.parametrize("variant", ["plain", "noise", "cadence"])
+@pytest.mark.parametrize("variant", ["plain", "noise", "cadence", "global_back"])
 def test_frames_match_oracle(gpu, variant):
     from foam import FoamSim
     from oracle.frame import OracleFoam
@@ -46,6 +46,8 @@ def test_frames_match_oracle(gpu, variant):
         kw = dict(brownian=True, brownian_mode=1, brownian_strength=1e-3, ou_jitter=True)
     if variant == "cadence":
         kw = dict(jfa_every=2, jfa_warm=1, pressure_iters=3)
+    if variant == "global_back":          # settle with run.py's global backpressure
+        kw = dict(backpressure_mode=1)
     cfg = small_cfg(**kw)
     pos, rad = fi.uniform(cfg.n, 0.03, 0.06, seed=21)
     cfg.r_min, cfg.r_max = 0.02, 0.07
@@ -57,7 +59,10 @@ def test_frames_match_oracle(gpu, variant):
         b = ref.step()
         assert a["pbd_passes"] == b["pbd_passes"] and a["jfa"] == b["jfa"]
         assert a["growth"] == b["growth"]
-        compare(sim.snapshot(), ref.snapshot(), f)
+        sa, sb = sim.snapshot(), ref.snapshot()
+        compare(sa, sb, f)
+        assert sa["settle"] == sb["settle"], f
+        print(f"[frame parity] variant={variant} frame={f} settle={int(sa['settle'])}", flush=True)
 
 
 def test_c1_first_frames_bit_exact(gpu):
