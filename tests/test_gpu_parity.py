@@ -846,6 +846,40 @@ def test_jfa_passes_counters_after_init_and_alone(gpu, orc):
         assert (inf[2:2 + passes] == hist).all(), call
 
 
+def test_settle_nudge_counter_gate_and_backpressure(gpu, orc):
+    """fos_settle_nudge against the host rule of run.py:733-751 and 1231
+    with the oracle's nudge: the counter entering the frame (forced, stored,
+    or reset by the last frame's gate), the countdown it writes, and the
+    global backpressure (mode 1) computed from the device overlap maximum in
+    double as the host computes it."""
+    n, adj = 3000, 15
+    rng = np.random.default_rng(12)
+    rad = rng.uniform(0.006, 0.024, n).astype(F)
+    rt = rng.uniform(0.006, 0.024, n).astype(F)
+    lmd = (rng.random(n) * 0.05).astype(F)
+    a = fc.f32(1 - 0.05 ** (1 / 15))
+    tol = fc.f32(0.035)
+    for mode in (1, 2):
+        for s_in, force, apply_gate, gate, md in ((3, 0, 0, 0.0, 0.01), (0, 0, 1, 1.0, 0.03),
+                                                  (0, 0, 1, 0.0, 0.0), (0, 1, 0, 0.0, 0.05),
+                                                  (1, 0, 1, 0.0, 0.002), (5, 0, 1, 1.0, 0.02)):
+            s = adj if force else s_in
+            if apply_gate and gate > 0.5:
+                s = adj
+            back = max(0.25, min(1.0, (float(tol) - float(F(md))) / float(tol)))
+            want = (orc.nudge_radii(rad, rt, lmd, n, a, fc.f32(0.12), fc.f32(0.006),
+                                    fc.f32(0.024), fc.f32(back), mode, tol) if s > 0 else rad)
+            tr = dev(rad)
+            settle = torch.tensor([s_in, -99], dtype=torch.int32, device="cuda")
+            maxd = torch.tensor([md, gate], dtype=torch.float32, device="cuda")
+            gpu.call("fos_settle_nudge", gpu.ptr(tr), gpu.ptr(dev(rt)), gpu.ptr(dev(lmd)), n, a,
+                     0.12, 0.006, 0.024, mode, tol, float(tol), gpu.ptr(maxd), gpu.ptr(settle),
+                     gpu.ptr(settle[1:]), adj, force, apply_gate, gpu.stream())
+            case = (mode, s_in, force, apply_gate, gate)
+            assert np.array_equal(host(tr), want), case
+            assert int(host(settle)[1]) == (s - 1 if s > 0 else s), case
+
+
 def test_topology_candidates(gpu, orc):
     n, G_ = 3000, 10
     pos, rad = fi.uniform(n, 0.01, 0.03, seed=8)
