@@ -975,7 +975,8 @@ __global__ void __launch_bounds__(256)
                    int32_t *__restrict__ face_ids,
                    int32_t *__restrict__ face_counts, int *__restrict__ in_count,
                    int *__restrict__ in_fill, unsigned *__restrict__ dec,
-                   int *__restrict__ chg, int nchg) {
+                   int *__restrict__ chg, int nchg, int *__restrict__ dirty0,
+                   int *__restrict__ dirty1) {
   extern __shared__ unsigned s_keys[];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + wid;
@@ -984,6 +985,8 @@ __global__ void __launch_bounds__(256)
       in_count[row] = 0;
       in_fill[row] = 0;
       dec[row] = 0u;
+      dirty0[row] = 0;
+      dirty1[row] = 0;
     }
     if (blockIdx.x == 0) {
       if (threadIdx.x == 0) in_count[n] = 0;
@@ -1131,20 +1134,59 @@ __global__ void k_fin_rank(int n, const int32_t *__restrict__ face_ids,
 // the round does nothing (so its own flag stays clear and the skip cascades).
 // failed (may be NULL; given to the second round of a last pair) is set when
 // that pair changed anything.
+// Dirty rows: a row's decisions depend only on the bits of its incoming
+// slots, so a row none of whose incoming bits changed in the previous round
+// keeps its decisions.  The previous round marked the targets of the bits it
+// changed in dirty_in (face_ids[row][slot] of each changed slot); a row not
+// marked copies its decisions forward, a marked row clears its mark and
+// recomputes.  Rows whose bits change mark their targets in dirty_out for
+// the next round.  dirty_in NULL: every row recomputes (the first round).
 __global__ void __launch_bounds__(256)
     k_fin_round(int n, const unsigned char *__restrict__ cls, const int *__restrict__ in_start,
                 const int *__restrict__ in_sorted, const unsigned *__restrict__ dec_prev,
                 unsigned *__restrict__ dec_next, const int *__restrict__ chg_prev,
-                int *__restrict__ chg_cur, int *__restrict__ failed) {
-  if (chg_prev && *chg_prev == 0) return;
-  // the pair's first round changed something
-  if (failed && blockIdx.x == 0 && threadIdx.x == 0 && *chg_cur) atomicOr(failed, 1);
+                int *__restrict__ chg_cur, int *__restrict__ failed,
+                const int32_t *__restrict__ face_ids, int *__restrict__ dirty_in,
+                int *__restrict__ dirty_out) {
   const int sub = (int)(threadIdx.x & 7);
   const int i = (int)(blockIdx.x * 32 + (threadIdx.x >> 3));
+  const bool live = i < n;
+  // Every load that does not depend on another is issued before the first
+  // branch, so the round costs one memory latency before its gathers (and
+  // a skipped row none after it).
+  const int go = chg_prev ? *chg_prev : 1;
+  const int mark = (dirty_in && live) ? dirty_in[i] : 1;
+  const unsigned dprev = live ? dec_prev[i] : 0u;
+  int b = 0, e = 0;
+  uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
+  // this lane's four slots of the row's partners (slots 4*sub .. 4*sub+3),
+  // for marking the targets of changed bits
+  int4 tgt = make_int4(-1, -1, -1, -1);
+  if (live && dirty_out)
+    tgt = ((const int4 *)(face_ids + (int64_t)i * FOS_MAX_NEIGHBORS))[sub];
+  if (live) {
+    b = in_start[i];
+    e = in_start[i + 1];
+    if (sub == 0) {
+      const uint4 *c4 = (const uint4 *)(cls + (int64_t)i * FOS_MAX_NEIGHBORS);
+      lo = c4[0];
+      hi = c4[1];
+    }
+  }
+  if (go == 0) return;
+  // the pair's first round changed something
+  if (failed && blockIdx.x == 0 && threadIdx.x == 0 && *chg_cur) atomicOr(failed, 1);
+  if (dirty_in && live) {
+    // the 8 lanes of the row read the same mark, so they leave together
+    if (mark == 0) {
+      if (sub == 0) dec_next[i] = dprev;
+      return;
+    }
+    if (sub == 0) dirty_in[i] = 0;
+  }
   int acc = 0;
-  if (i < n) {
-    const int e = in_start[i + 1];
-    for (int p = in_start[i] + sub; p < e; p += 8) {
+  if (live) {
+    for (int p = b + sub; p < e; p += 8) {
       const int src = in_sorted[p];
       acc += (dec_prev[src / FOS_MAX_NEIGHBORS] >> (src % FOS_MAX_NEIGHBORS)) & 1u;
     }
@@ -1152,38 +1194,57 @@ __global__ void __launch_bounds__(256)
   acc += __shfl_xor(acc, 1, 8);
   acc += __shfl_xor(acc, 2, 8);
   acc += __shfl_xor(acc, 4, 8);
-  if (i >= n || sub != 0) return;
-  int used = acc < FOS_MAX_NEIGHBORS ? acc : FOS_MAX_NEIGHBORS;
-  const uint4 *c4 = (const uint4 *)(cls + (int64_t)i * FOS_MAX_NEIGHBORS);
-  const uint4 lo = c4[0], hi = c4[1];
-  const unsigned words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
   unsigned bits = 0;
+  if (live && sub == 0) {
+    int used = acc < FOS_MAX_NEIGHBORS ? acc : FOS_MAX_NEIGHBORS;
+    const unsigned words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
-  for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
-    const unsigned c = (words[s >> 2] >> (8 * (s & 3))) & 0xffu;
-    const unsigned k = c & 3u;
-    const bool a = (k == CLS_ACCEPT) || (k == CLS_AMBIG && ((s < used) || (c & CLS_OLD)));
-    if (a) {
-      bits |= 1u << s;
-      used += used < FOS_MAX_NEIGHBORS ? 1 : 0;
+    for (int s = 0; s < FOS_MAX_NEIGHBORS; ++s) {
+      const unsigned c = (words[s >> 2] >> (8 * (s & 3))) & 0xffu;
+      const unsigned k = c & 3u;
+      const bool a = (k == CLS_ACCEPT) || (k == CLS_AMBIG && ((s < used) || (c & CLS_OLD)));
+      if (a) {
+        bits |= 1u << s;
+        used += used < FOS_MAX_NEIGHBORS ? 1 : 0;
+      }
     }
+    dec_next[i] = bits;
   }
-  dec_next[i] = bits;
-  if (bits != dec_prev[i]) {
+  // the row's decisions to all 8 lanes, which mark the targets of their
+  // own changed slots
+  bits = (unsigned)__shfl((int)bits, 0, 8);
+  if (!live) return;
+  const unsigned diff = bits ^ dprev;
+  // one flag update per wave, not one per changed row: atomics on one
+  // address serialise (round 0 changes nearly every row)
+  const unsigned long long any = __ballot(diff != 0u);
+  if (any && (int)(threadIdx.x & 63) == __ffsll((long long)any) - 1) {
     atomicOr(chg_cur, 1);
     if (failed) atomicOr(failed, 1);
+  }
+  if (diff == 0) return;
+  if (dirty_out) {
+    const unsigned mine = (diff >> (4 * sub)) & 0xfu;
+    const int t4[4] = {tgt.x, tgt.y, tgt.z, tgt.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (((mine >> q) & 1u) && t4[q] >= 0 && t4[q] < n) dirty_out[t4[q]] = 1;
   }
 }
 
 // Clears what finalize accumulates into: incoming counts (n + 1), fill
-// cursors and decisions (n) and the per-pair change flags (nchg).
+// cursors, decisions and both dirty-row mark arrays (n) and the per-pair
+// change flags (nchg).
 __global__ void k_fin_clear(int n, int *__restrict__ in_count, int *__restrict__ in_fill,
-                            unsigned *__restrict__ dec, int *__restrict__ chg, int nchg) {
+                            unsigned *__restrict__ dec, int *__restrict__ chg, int nchg,
+                            int *__restrict__ dirty0, int *__restrict__ dirty1) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t <= n) in_count[t] = 0;
   if (t < n) {
     in_fill[t] = 0;
     dec[t] = 0;
+    dirty0[t] = 0;
+    dirty1[t] = 0;
   }
   if (t < nchg) chg[t] = 0;
 }
@@ -1632,6 +1693,7 @@ static int face_grow(fos_ctx *ctx, hipStream_t st) {
 }
 
 static int *fin_chg(fos_ctx *ctx);
+static int *fin_dirty(fos_ctx *ctx, int which);
 #define FIN_RING 48
 
 // clear_fin: also clear the finalize accumulators (a finalize follows).
@@ -1642,7 +1704,7 @@ static int face_compact(fos_ctx *ctx, int n, int32_t *face_ids, int32_t *face_co
   hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st, n,
                      ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts,
                      clear_fin ? ctx->in_count : nullptr, ctx->in_fill, (unsigned *)ctx->dec,
-                     fin_chg(ctx), FIN_RING);
+                     fin_chg(ctx), FIN_RING, fin_dirty(ctx, 0), fin_dirty(ctx, 1));
   FOS_CHECK_LAUNCH("k_face_compact");
   return 0;
 }
@@ -1900,7 +1962,8 @@ static int fin_prepare(fos_ctx *ctx, int n, const float *rad, float voxel, const
     hipLaunchKernelGGL(k_fin_clear,
                        dim3(blocks_for((int64_t)(n + 1 > FIN_RING ? n + 1 : FIN_RING))),
                        dim3(FOS_BLOCK), 0, st, n, ctx->in_count, ctx->in_fill,
-                       (unsigned *)ctx->dec, fin_chg(ctx), FIN_RING);
+                       (unsigned *)ctx->dec, fin_chg(ctx), FIN_RING, fin_dirty(ctx, 0),
+                       fin_dirty(ctx, 1));
     FOS_CHECK_LAUNCH("k_fin_clear");
   }
   hipLaunchKernelGGL(k_fin_classify, dim3(blocks_for(m)), dim3(FOS_BLOCK), 0, st, n,
@@ -1925,18 +1988,31 @@ static int fin_prepare(fos_ctx *ctx, int n, const float *rad, float voxel, const
 // ctx->flags[3] (sticky) when it changed anything.  Flags are cleared by
 // fin_prepare; a caller going past FIN_RING pairs clears the ring itself
 // (fin_ring_clear).
-static int fin_pairs(fos_ctx *ctx, int n, int k0, int pairs, bool last_failed, hipStream_t st) {
+// Dirty-row marks of the rounds: two int arrays of max_n in the decision
+// scratch after the two decision buffers; round r reads (and clears) array
+// r & 1 and marks array (r + 1) & 1.
+static int *fin_dirty(fos_ctx *ctx, int which) {
+  return (int *)ctx->dec + (size_t)(2 + which) * ctx->max_n;
+}
+
+static int fin_pairs(fos_ctx *ctx, int n, int k0, int pairs, bool last_failed,
+                     const int32_t *face_ids, hipStream_t st) {
   unsigned *dec_a = (unsigned *)ctx->dec, *dec_b = dec_a + ctx->max_n;
   int *chg = fin_chg(ctx);
+  int *d0 = fin_dirty(ctx, 0), *d1 = fin_dirty(ctx, 1);
   for (int k = k0; k < k0 + pairs; ++k) {
     const int *prev = k > 0 ? chg + (k - 1) % FIN_RING : nullptr;
     int *cur = chg + k % FIN_RING;
     int *failed = last_failed && k == k0 + pairs - 1 ? ctx->flags + 3 : nullptr;
+    // round 2k reads array 0 and marks array 1; round 2k + 1 reads array 1
+    // and marks array 0.  Round 0 changes nearly every row (from nothing
+    // accepted), so it marks nothing and round 1 recomputes every row.
     hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
                        ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, prev, cur,
-                       (int *)nullptr);
+                       (int *)nullptr, face_ids, k > 0 ? d0 : nullptr, k > 0 ? d1 : nullptr);
     hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, prev, cur, failed);
+                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, prev, cur, failed,
+                       face_ids, k > 0 ? d1 : nullptr, d0);
     FOS_CHECK_LAUNCH("k_fin_round");
   }
   return 0;
@@ -1993,7 +2069,7 @@ static int fin_first_burst(const fos_ctx *ctx) {
 // already read (then k0 is the pair after it); NULL starts with a burst.
 // Records the pairs needed in ctx->fin_burst.
 static int fin_pairs_host(fos_ctx *ctx, int n, int k0, int first, const int *known,
-                          hipStream_t st) {
+                          const int32_t *face_ids, hipStream_t st) {
   int f[64];
   if (known && !known[16 + (k0 - 1) % FIN_RING]) {
     ctx->fin_burst = fin_needed(known, k0 - 1);
@@ -2008,7 +2084,7 @@ static int fin_pairs_host(fos_ctx *ctx, int n, int k0, int first, const int *kno
       rc = fin_ring_clear(ctx, (k - 1) % FIN_RING, st);
       lap_end = k + FIN_RING - 1;
     }
-    if (!rc) rc = fin_pairs(ctx, n, k, p, false, st);
+    if (!rc) rc = fin_pairs(ctx, n, k, p, false, face_ids, st);
     if (!rc) rc = read_flags(ctx, f, st);
     if (rc) return rc;
     if (!f[16 + (k + p - 1) % FIN_RING]) {
@@ -2032,9 +2108,9 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
   if (fixed_pairs > 0) {
     FOS_REQUIRE(fixed_pairs <= FIN_RING, "finalize: fixed_pairs %d above %d", fixed_pairs,
                 FIN_RING);
-    rc = fin_pairs(ctx, n, 0, fixed_pairs, true, st);
+    rc = fin_pairs(ctx, n, 0, fixed_pairs, true, face_ids, st);
   } else {
-    rc = fin_pairs_host(ctx, n, 0, fin_first_burst(ctx), nullptr, st);
+    rc = fin_pairs_host(ctx, n, 0, fin_first_burst(ctx), nullptr, face_ids, st);
   }
   if (rc) return rc;
   return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
@@ -2067,7 +2143,7 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
     if (!rc) rc = fin_prepare(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores, st,
                               true);
     const int first = fin_first_burst(ctx);
-    if (!rc) rc = fin_pairs(ctx, n, 0, first, false, st);
+    if (!rc) rc = fin_pairs(ctx, n, 0, first, false, face_ids, st);
     if (rc) return rc;
     int f[64];
     rc = read_flags(ctx, f, st);
@@ -2077,7 +2153,7 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
       if (rc) return rc;
       continue;
     }
-    rc = fin_pairs_host(ctx, n, first, FIN_BURST, f, st);
+    rc = fin_pairs_host(ctx, n, first, FIN_BURST, f, face_ids, st);
     if (rc) return rc;
     return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
                            used_count, fsc, st);
@@ -2516,7 +2592,8 @@ int fos_batch_face_accumulate(fos_batch *bt, fos_ctx *ctx, const int32_t *filter
   const size_t shmem = (size_t)4 * cap * sizeof(unsigned);
   hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st, n,
                      ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts,
-                     (int *)nullptr, (int *)nullptr, (unsigned *)nullptr, (int *)nullptr, 0);
+                     (int *)nullptr, (int *)nullptr, (unsigned *)nullptr, (int *)nullptr, 0,
+                     (int *)nullptr, (int *)nullptr);
   FOS_CHECK_LAUNCH("k_face_compact");
   return 0;
 }
