@@ -215,7 +215,21 @@ __global__ void __launch_bounds__(256)
   for (int o = 16; o > 0; o >>= 1) lm = fmaxf(lm, __shfl_xor(lm, o, 32));
   if (i < n && l == 0) {
     lmd[i] = lm;
-    if (lm > 0.0f) atomicMax((int *)gmax, __float_as_int(lm));
+    if (BATCH && lm > 0.0f) atomicMax((int *)gmax, __float_as_int(lm));
+  }
+  if (!BATCH) {
+    // one atomic per block on the single maximum, not one per particle
+    // (atomics on one address serialise); lm >= 0, so the int image of the
+    // float orders as the float does
+    __shared__ float s_m[8];
+    if (l == 0) s_m[grp] = (i < n) ? lm : 0.0f;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = s_m[0];
+#pragma unroll
+      for (int g = 1; g < 8; ++g) m = fmaxf(m, s_m[g]);
+      if (m > 0.0f) atomicMax((int *)gmax, __float_as_int(m));
+    }
   }
 }
 
