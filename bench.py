@@ -45,16 +45,54 @@ WEIGHTS = os.path.join(PKG, "face_mlp_c5.npz")
 COPY_CAL_RES = 400          # 400^3 int32 = 256,000,000 B read and written
 
 
+JFA_SRC = os.path.join(PKG, "csrc", "fos_jfa.hip")
+COMMON_SRC = os.path.join(PKG, "csrc", "fos_common.h")
+
+
+def kernel_signature():
+    """SHA-256 (first 16 hex digits) of the JFA pass kernel's source -- the
+    regions of fos_jfa.hip between its pass-kernel and pass-launch markers,
+    the shared device helpers and the build flags -- so a counter profile can
+    be matched to the kernel the bench times."""
+    import hashlib
+    import fos_build
+    src = open(JFA_SRC).read()
+    h = hashlib.sha256()
+    for tag in ("kernel", "launch"):
+        a = src.index(f"// @jfa-pass-{tag}-begin")
+        b = src.index(f"// @jfa-pass-{tag}-end")
+        h.update(src[a:b].encode())
+    h.update(open(COMMON_SRC).read().encode())
+    h.update(" ".join(fos_build.FLAGS).encode())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(path=TRAFFIC_FILE):
-    """HBM bytes per pass launch from a committed counter summary, or None."""
+    """(HBM bytes per pass launch, kernel signature) from a committed counter
+    summary, or (None, None)."""
+    traffic = sig = None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
-                    return int(float(line.split(":", 1)[1]))
+                    traffic = int(float(line.split(":", 1)[1]))
+                elif line.startswith("kernel_signature:"):
+                    sig = line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return None
+    return traffic, sig
+
+
+def traffic_fields(path):
+    """roofline.traffic for the timed kernel: the profile's figure when the
+    profile was taken on this kernel source, else null with the reason."""
+    traffic, sig = measured_traffic(path)
+    now = kernel_signature()
+    if traffic is not None and sig == now:
+        return {"traffic": traffic, "kernel_signature": now}
+    return {"traffic": None, "kernel_signature": now,
+            "traffic_note": (f"{os.path.relpath(path, ROOT)} was taken on kernel signature {sig}"
+                             if traffic is not None else "no counter profile")}
 
 
 def copy_calibration():
@@ -219,7 +257,7 @@ def c3_jfa_bench(frames):
             "gvox_pass_per_s": round(P * V / (float(flat.sum(axis=1).mean()) / 1e3) / 1e9, 2),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                         "traffic": measured_traffic(C3_TRAFFIC_FILE),
+                         **traffic_fields(C3_TRAFFIC_FILE),
                          "kernel": "k_jfa_pass_list", "avg_launch_us": round(avg_pass_s * 1e6, 2),
                          "per_pass_us": [round(x * 1e3, 1) for x in flat.mean(axis=0)]}}
 
@@ -240,21 +278,36 @@ def surrogate_bench(steps, warmup):
     plain = fc.preset("c5")
     pos, rad = fi.for_config(plain)
     el_p, _, _ = timed(plain, pos, rad, steps, warmup, False)
-    surr = fc.preset("c5s")
-    el_s, _, sim = timed(surr, pos, rad, steps, warmup, False,
-                         setup=lambda s: s.set_predictor(params, meta["mult"]))
-    judged = [j for j in sim.judged if j[0] >= warmup]
     fp_ = steps / el_p
-    fs_ = steps / el_s
-    return {"workload": "N=50000, radii U[0.005,0.011], JFA 256^3; predictor path: JFA every "
-                        "8th frame as judge, BF16 MLP on MFMA in between",
-            "plain_fps": round(fp_, 3), "surrogate_fps": round(fs_, 3),
-            "speedup": round(fs_ / fp_, 3), "frames": steps,
+    out = {"workload": "N=50000, radii U[0.005,0.011], JFA 256^3; predictor path: JFA every "
+                       "8th frame as judge, BF16 MLP on MFMA in between, JFA on every frame "
+                       "while the judged error exceeds 2%",
+           "plain_fps": round(fp_, 3), "frames": steps,
+           "candidate_mult": meta["mult"], "trained_on": f"{meta['frames']} plain-JFA frames"}
+    # the judge bounds either every particle's FSC (per_particle) or the
+    # frame's mean FSC (mean, where over- and under-counts cancel)
+    for judge_on in ("per_particle", "mean"):
+        surr = fc.preset("c5s")
+        surr.judge_on = judge_on
+        el_s, _, sim = timed(surr, pos, rad, steps, warmup, False,
+                             setup=lambda s: s.set_predictor(params, meta["mult"]))
+        judged = [j for j in sim.judged if j[0] >= warmup]
+        k = 1 if judge_on == "per_particle" else 2
+        fs_ = steps / el_s
+        out[f"judge_{judge_on}"] = {
+            "surrogate_fps": round(fs_, 3), "speedup": round(fs_ / fp_, 3),
             "judge_frames": len(judged),
-            "fsc_err_mean_fsc": round(float(np.mean([j[2] for j in judged])), 5) if judged else None,
-            "fsc_err_per_particle": round(float(np.mean([j[1] for j in judged])), 5) if judged else None,
-            "fallback_frames": int(sum(1 for j in judged if j[2] > surr.max_fsc_err)),
-            "candidate_mult": meta["mult"], "trained_on": f"{meta['frames']} plain-JFA frames"}
+            "jfa_frames": int(sum(1 for f in range(warmup, warmup + steps)
+                                  if f in {j[0] for j in sim.judged})),
+            "fsc_err_per_particle": (round(float(np.mean([j[1] for j in judged])), 5)
+                                     if judged else None),
+            "fsc_err_mean_fsc": (round(float(np.mean([j[2] for j in judged])), 5)
+                                 if judged else None),
+            "fallback_judgements": int(sum(1 for j in judged if j[k] > surr.max_fsc_err))}
+    out["note"] = ("the <=2% bound holds for the frame's mean FSC (judge_mean); the per-particle "
+                   "error is higher, and judged per particle at 2% the path falls back to the "
+                   "JFA (judge_per_particle)")
+    return out
 
 
 def ensemble_bench(B, steps, warmup, sample=8):
@@ -302,6 +355,10 @@ def ensemble_bench(B, steps, warmup, sample=8):
     rs = np.array([c.res for c in cfgs])
     fps = steps / el
     seq_rate = 1.0 / float(np.mean(seq))
+    # where the foams' FSC sits against their bands at the end of the run
+    mean_fsc = np.array([float(ens.snapshot(b)["fsc"].mean()) for b in range(B)])
+    rad_rel = np.array([float(np.mean(ens.snapshot(b)["rad"] / np.float32(cfgs[b].r_max)))
+                        for b in range(B)])
     return {"workload": f"{B} foams, N log-uniform [500, 20000], radii U[0.6,1.2]*(0.5/N)^(1/3), "
                         "JFA edge round(5 N^(1/3)) in [48,160], bands {10-14,12-16,14-18}, "
                         "C2 dynamics",
@@ -312,6 +369,17 @@ def ensemble_bench(B, steps, warmup, sample=8):
             "frame": "hip graph replay (eager frames beside it)",
             "jfa_tiles": {"picked": ens.tile_choice, "ms_per_pass_sequence": tiles},
             "one_at_a_time_foam_frames_per_s": round(seq_rate, 1),
+            "fsc": {"mean_fsc": [round(float(mean_fsc.min()), 2),
+                                 round(float(np.median(mean_fsc)), 2),
+                                 round(float(mean_fsc.max()), 2)],
+                    "band_low": [int(min(c.f_low for c in cfgs)), int(max(c.f_low for c in cfgs))],
+                    "mean_rad_over_r_max": round(float(rad_rel.mean()), 3),
+                    "frames_run": int(ens.frame),
+                    "note": "the foams' mean FSC stays below their bands (10-18): at a JFA edge "
+                            "of 5*N^(1/3) voxels a face spans few voxels against the score "
+                            "rule's expected area 1.5*((ri+rj)/voxel)^2 (jfa.py:379-443), so the "
+                            "controller keeps asking for growth once the warm-start gate opens "
+                            "(frame 30) while r_max = 1.2*(0.5/N)^(1/3) bounds the radii"},
             "speedup": round(B * fps / seq_rate, 2),
             "spread": {"n": [int(ns.min()), int(np.median(ns)), int(ns.max())],
                        "res": [int(rs.min()), int(np.median(rs)), int(rs.max())],
@@ -392,7 +460,7 @@ def main():
                    "frames": args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(), "kernel": "k_jfa_pass_list",
+                     **traffic_fields(TRAFFIC_FILE), "kernel": "k_jfa_pass_list",
                      "avg_launch_us": round(avg_pass_s * 1e6, 3),
                      "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)],
                      "timed_frames_with_events": len(pass_ms),
@@ -402,11 +470,21 @@ def main():
         "gvox_pass_per_s": round(gvox, 2),
         "jfa_ms_per_frame": round(float(flat.sum(axis=1).mean()), 4),
     }
+    gate = cfg.warmstart_frames
+    out["config"]["window"] = {
+        "frames": [args.warmup, args.warmup + args.steps - 1], "growth_gate_frame": gate,
+        "frames_past_gate": max(0, args.warmup + args.steps - gate),
+        "note": ("the controller (warm-start gate, run.py:1184-1197) is closed before frame "
+                 f"{gate}; frames before it run the same kernels except the settle nudge")}
     if not args.no_cadence:
         c2c = fc.preset("c2c")
-        el2, _, _ = timed(c2c, pos, rad, args.steps, args.warmup, False)
+        # the window starts past the JFA warm start whatever --warmup is
+        w2 = max(args.warmup, c2c.jfa_warm)
+        el2, _, _ = timed(c2c, pos, rad, args.steps, w2, False)
+        jfa_frames = sum(1 for f in range(w2, w2 + args.steps) if c2c.jfa_runs(f))
         out["cadence"] = {"workload": "same, JFA every 5th frame after a 20-frame warm start",
-                          "value": round(args.steps / el2, 3), "unit": "frames/s"}
+                          "value": round(args.steps / el2, 3), "unit": "frames/s",
+                          "frames": [w2, w2 + args.steps - 1], "jfa_frames": jfa_frames}
     if not args.no_c3:
         out["c3_jfa"] = c3_jfa_bench(args.c3_frames)
     if not args.no_surrogate and os.path.exists(WEIGHTS):

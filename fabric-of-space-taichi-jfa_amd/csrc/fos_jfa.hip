@@ -73,6 +73,8 @@ __global__ void k_seed_fused(const float *__restrict__ pos, const float *__restr
   }
 }
 
+// @jfa-pass-kernel-begin (this region, the pass launch region and fos_common.h
+// make the pass kernel's signature that counter profiles are matched by)
 // ---------------------------------------------------------------------------
 // One JFA pass (jfa.py:524-623)
 // ---------------------------------------------------------------------------
@@ -600,6 +602,8 @@ __global__ void __launch_bounds__(256)
               __popcll(bal));
   }
 }
+
+// @jfa-pass-kernel-end
 
 // Ends the pass loop: records passes executed and the exit flag, and copies
 // temp into grid when an odd number of passes ran (jfa.py:1115-1116).
@@ -1551,6 +1555,7 @@ static int g_emit_zw = 32;
 extern int g_grid_block;  // fos_grid.hip
 extern int g_grid_cpw;    // fos_grid.hip
 
+// @jfa-pass-launch-begin
 template <bool U, int K>
 static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float voxel,
                           int step, const int32_t *in, int32_t *out, int pass, int early,
@@ -1650,6 +1655,8 @@ static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
   FOS_CHECK_LAUNCH("k_jfa_pass");
   return 0;
 }
+
+// @jfa-pass-launch-end
 
 // Face-pair emission into the row tables (the kernel chosen by
 // face_emit_tx); the overflow flag (ctx->flags[0]) is left for the caller.

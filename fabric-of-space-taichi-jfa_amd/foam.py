@@ -311,12 +311,14 @@ class FoamSim:
                 self.predictor.remember_jfa(self.neighbor_list, self.neighbor_count, self.n)
                 from face_predictor import fsc_error
                 err, bias = fsc_error(self.pred_fsc, self.fsc)
-                # the judge compares the frame's mean FSC (bias); the
-                # per-particle error is recorded beside it.  Without a
-                # previous frame's rows (a cold start) the hysteresis feature
-                # is empty, so that judgement says nothing about the frames
-                # that follow (they start from this JFA's rows): no fallback.
-                self.fallback = warm and bias > c.max_fsc_err
+                # the judge bounds the per-particle error (judge_on
+                # "per_particle") or the frame's mean FSC ("mean"); both are
+                # recorded.  Without a previous frame's rows (a cold start)
+                # the hysteresis feature is empty, so that judgement says
+                # nothing about the frames that follow (they start from this
+                # JFA's rows): no fallback.
+                judged = err if c.judge_on == "per_particle" else bias
+                self.fallback = warm and judged > c.max_fsc_err
                 self.judged.append((f, err, bias))
             else:
                 self.predictor.predict(self.pos, self.rad, self.n, self.fsc, self.face_ids,

@@ -141,9 +141,15 @@ class FoamConfig:
     frames: int = 20
     # learned face predictor: JFA (the judge) every `judge_every` frames, the
     # predictor in between (0 = plain JFA); fall back to the JFA on every
-    # frame while the last judged frame's FSC error exceeds max_fsc_err
+    # frame while the last judged frame's FSC error (judge_on) exceeds
+    # max_fsc_err
     judge_every: int = 0
     max_fsc_err: float = 0.02
+    # what the judge bounds by max_fsc_err: "per_particle" = sum |FSC_pred -
+    # FSC_jfa| / sum FSC_jfa (every particle's own count), "mean" = |sum
+    # FSC_pred - sum FSC_jfa| / sum FSC_jfa (the frame's mean FSC, where
+    # over- and under-counts cancel)
+    judge_on: str = "per_particle"
     # candidate pairs of the predictor: |d| < pred_mult * (ri + rj)
     pred_mult: float = 1.6
 

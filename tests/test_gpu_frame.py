@@ -18,6 +18,15 @@ INT_STATE = ("face_ids", "face_counts", "fsc", "neighbor_list", "neighbor_count"
 FLOAT_STATE = ("pos", "rad", "rad_target", "lmd", "fsc_ema", "prev_scores", "v_jit")
 
 
+@pytest.fixture
+def keep_config():
+    """Restores the foam_config constants a drop-in test rebinds."""
+    saved = {k: v for k, v in vars(fc).items() if k.isupper()}
+    yield
+    for k, v in saved.items():
+        setattr(fc, k, v)
+
+
 def compare(a, b, where):
     for k in INT_STATE:
         assert np.array_equal(a[k], b[k]), (where, k)
@@ -161,7 +170,7 @@ def test_telemetry_lines(gpu):
     assert sim.log_lines()[4].startswith("[Frame  100] N=")
 
 
-def test_dropin_run_jfa_matches_oracle(gpu, orc):
+def test_dropin_run_jfa_matches_oracle(gpu, orc, keep_config):
     import jfa
     fc.JFA_RES_MIN, fc.JFA_RES_MAX = 16, 320
     fc.DOMAIN_SIZE = 1.0
@@ -197,7 +206,7 @@ def test_dropin_run_jfa_matches_oracle(gpu, orc):
     assert jfa.label_changes[None] == hist[-1]
 
 
-def test_dropin_grid_and_dynamics(gpu, orc):
+def test_dropin_grid_and_dynamics(gpu, orc, keep_config):
     import dynamics
     import grid
     n, G_ = 2000, 13
@@ -271,3 +280,27 @@ def test_runs_are_deterministic(gpu):
         for sa, sb in zip(a, b):
             for k in sa:
                 assert np.array_equal(sa[k], sb[k]), (run.__name__, k)
+
+
+def test_integration_snippet_runs(gpu, orc, keep_config):
+    """The drop-in snippet of INTEGRATION.md section 2, executed as written
+    (REPO set to this checkout): its run_jfa FSC equals the oracle's on the
+    same inputs, and the grid, PBD and pressure calls that follow run."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "INTEGRATION.md")).read()
+    sec = text[text.index("## 2. Drop-in for `run.py`"):text.index("## 3.")]
+    code = re.search(r"```python\n(.*?)```", sec, re.S).group(1)
+    assert 'REPO = "/path/to/repo"' in code
+    code = code.replace('REPO = "/path/to/repo"', f"REPO = {root!r}")
+    ns = {}
+    exec(compile(code, "INTEGRATION.md section 2", "exec"), ns)
+    N, RES = ns["N"], ns["RES"]
+    assert (N, RES) == (10000, 128) and ns["stats"]["num_passes"] == 8
+    voxel = fc.f32(1.0 / RES)
+    lab, _, _, _ = orc.jfa_run(ns["pos_np"], ns["rad_np"], N, RES, 1.0, voxel, 8)
+    ids, cts = orc.accumulate_faces(orc.majority(lab, RES), RES, N)
+    _, _, _, _, fs = orc.finalize_faces(N, ns["rad_np"], voxel, ids, cts, np.zeros((N, 32), F))
+    assert np.array_equal(ns["fsc"], fs)
+    assert ns["max_dr"] >= 0.0 and ns["changed"] >= 0

@@ -392,11 +392,13 @@ def test_training_gradients_and_loss(gpu):
     assert acc > 0.95
 
 
-def test_frame_driver_judge_and_fallback(gpu):
+@pytest.mark.parametrize("judge_on", ["per_particle", "mean"])
+def test_frame_driver_judge_and_fallback(gpu, judge_on):
     """The frame driver's predictor path (FoamSim with judge_every): the JFA
     runs on judge frames and after a fallback, the predictor in between
-    (its FSC and face rows feed the frame); a judged mean-FSC error above
-    the bound sends the next frame to the JFA -- except on a cold start,
+    (its FSC and face rows feed the frame); a judged error above the bound
+    (the per-particle error or the frame's mean-FSC error, as judge_on
+    says) sends the next frame to the JFA -- except on a cold start,
     where the hysteresis feature had no previous rows.  Untrained weights
     keep the error large, so both branches run."""
     import foam_config as fc
@@ -404,6 +406,7 @@ def test_frame_driver_judge_and_fallback(gpu):
     cfg = fc.preset("c5s")
     cfg.n, cfg.res, cfg.judge_every, cfg.pressure_iters = 3000, 64, 3, 5
     cfg.r_min, cfg.r_max = 0.012, 0.03
+    cfg.judge_on = judge_on
     pos, rad = fi.uniform(cfg.n, 0.015, 0.025, seed=9)
     sim = FoamSim(cfg, pos, rad)
     sim.set_predictor(fp.init_params(seed=3), mult=2.0)
@@ -419,7 +422,8 @@ def test_frame_driver_judge_and_fallback(gpu):
             if f == 0:
                 assert not sim.fallback          # cold start: no fallback
             else:
-                assert sim.fallback == (bias > cfg.max_fsc_err), (f, bias)
+                judged = err if judge_on == "per_particle" else bias
+                assert sim.fallback == (judged > cfg.max_fsc_err), (f, err, bias)
         fsc = host(sim.fsc)
         nrow = (host(sim.face_ids) >= 0).sum(axis=1)
         if not last["jfa"]:
@@ -427,5 +431,6 @@ def test_frame_driver_judge_and_fallback(gpu):
             assert (fsc >= nrow).all() and fsc.sum() == 2 * nrow.sum()
     assert ran[0] and not ran[1]                 # no fallback after the cold judge frame
     # the untrained model misses on some warm judge frame: the fallback ran
-    assert any(b > cfg.max_fsc_err for jf, _, b in sim.judged if jf > 0)
+    k = 1 if judge_on == "per_particle" else 2
+    assert any(j[k] > cfg.max_fsc_err for j in sim.judged if j[0] > 0)
     assert len(sim.judged) == sum(ran)
