@@ -35,6 +35,8 @@ SIGNATURES = {
     "fos_jfa_copy": [I, P, P, P],
     "fos_jfa_passes": [P, I, F, F, I, I, D, I, P, P, P, P],
     "fos_jfa_passes_timed": [P, I, F, F, I, I, D, I, P, P, P, P, P],
+    "fos_jfa_labels_noalloc": [P, P, I, I, F, F, F, I, I, D, I, P, P, P, P],
+    "fos_static_limits": [P, P, P],
     "fos_events_create": [I, P],
     "fos_events_destroy": [I, P],
     "fos_events_elapsed": [I, P, P],

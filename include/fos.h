@@ -122,6 +122,26 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
                          int num_passes, int early_exit, double th_pass_delta,
                          int min_passes, int32_t *grid, int32_t *temp,
                          int32_t *info, void *const *events, void *stream);
+/* Limits of fos_jfa_labels_noalloc's static scratch. */
+#define FOS_STATIC_MAX_N 262144
+#define FOS_STATIC_MAX_RES 384
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Computes what fos_jfa_init (grid only, temp untouched before
+ * the passes) followed by fos_jfa_passes computes -- the seeding and the
+ * whole pass schedule, the final labels in grid and info as fos_jfa_passes
+ * writes it -- with no context: the seed table, pass counters and change
+ * masks live in one static device array (about 17.8 MB per device, held
+ * until the process ends).  Calls no hipMalloc/hipFree.  Rejects
+ * n > FOS_STATIC_MAX_N or res > FOS_STATIC_MAX_RES with FOS_EINVAL.  Every
+ * call on one device shares that device's array: calls that overlap on one
+ * device race. */
+int fos_jfa_labels_noalloc(const float *pos, const float *rad, int n, int res, float L,
+                           float voxel, float beta, int num_passes, int early_exit,
+                           double th_pass_delta, int min_passes, int32_t *grid, int32_t *temp,
+                           int32_t *info, void *stream);
+/* The static scratch limits (host): particles, resolution, bytes. */
+int fos_static_limits(int *max_n, int *max_res, int64_t *bytes);
 /* hipEvent_t helpers for fos_jfa_passes_timed: create count events, destroy
  * them, and read the count-1 elapsed intervals (ms) between consecutive
  * events after waiting for the last one. */
