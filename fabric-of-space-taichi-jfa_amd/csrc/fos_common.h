@@ -173,6 +173,9 @@ struct fos_ctx {
   float4 *seeds;
   // per-wave change masks of the last two JFA passes, [max_res^2][ceil(max_res/64)]
   unsigned long long *chg[2];
+  // each voxel's winning power distance in the last two JFA passes [max_res^3]
+  // (NULL: the passes evaluate every incumbent)
+  float *dist[2];
   // striped per-pass changed-voxel counters [FOS_PASS_SLOTS][FOS_STRIPES][16]
   int *stripes;
   // face-pair row tables: capacity tab_cap entries per particle row

@@ -202,7 +202,7 @@ int fos_ctx_destroy(fos_ctx *ctx) {
                   ctx->in_sorted,
                   ctx->acc_in,   ctx->scan_blocks, ctx->cell_write, ctx->cell_of,
                   ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch,
-                  ctx->chg[0],   ctx->chg[1],   ctx->grid_tmp};
+                  ctx->chg[0],   ctx->chg[1],   ctx->grid_tmp, ctx->dist[0], ctx->dist[1]};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->host_i) (void)hipHostFree(ctx->host_i);
@@ -269,6 +269,14 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
     const size_t words = (size_t)max_res * max_res * ((max_res + 63) / 64);
     ALLOC(c->chg[0], words * sizeof(unsigned long long));
     ALLOC(c->chg[1], words * sizeof(unsigned long long));
+    // per-voxel power distances carried between JFA passes, for grids up to
+    // 256^3 (there the carry was measured faster; beyond, the extra label-
+    // side traffic costs more than the evaluation it saves)
+    if (max_res <= 256) {
+      const size_t vox = (size_t)max_res * max_res * max_res;
+      ALLOC(c->dist[0], vox * sizeof(float));
+      ALLOC(c->dist[1], vox * sizeof(float));
+    }
   }
   ALLOC(c->stripes, (size_t)FOS_PASS_SLOTS * FOS_STRIPE_PASS_INTS * sizeof(int));
   ALLOC(c->flags, 64 * sizeof(int));

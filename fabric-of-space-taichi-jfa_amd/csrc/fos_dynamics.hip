@@ -580,13 +580,42 @@ __global__ void k_brownian(float *__restrict__ pos, int n, float strength, float
                  hash_axis(w[2], G, half_l, inv_l);
 }
 
-// dynamics.py:620-660
+// run.py:860-861 mean radius for the OU jitter, and the noise scale
+// sigma * sqrt(dt) it sets (dynamics.py:626-630), into out[0] (mean) and
+// out[1] (scale).  The radii are summed in double: the partial sums are
+// multiples of the smallest radius's ulp below n * 2 * r_max, so they are
+// exact (any order) while n * r_max / r_min < 2^28, and the mean is the
+// correctly rounded double quotient rounded to fp32 -- the value
+// f32(mean(rad, float64)) the host would compute.
+__global__ void __launch_bounds__(1024) k_rad_mean(const float *__restrict__ rad, int n,
+                                                   float dt, float jitter_rms, float tau,
+                                                   float *__restrict__ out) {
+  __shared__ double s[1024];
+  double acc = 0.0;
+  for (int i = (int)threadIdx.x; i < n; i += 1024) acc += (double)rad[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float mean_r = (float)(s[0] / (double)n);
+    const float sigma = jitter_rms * mean_r * sqrtf(2.0f / tau);
+    out[0] = mean_r;
+    out[1] = sigma * sqrtf(dt);
+  }
+}
+
+// dynamics.py:620-660; sdt_dev (when not NULL) holds the noise scale
+// computed on the device and replaces sdt.
 __global__ void k_ou(float *__restrict__ pos, float *__restrict__ vj,
                      const float *__restrict__ rad, int n, float tau, float dt,
                      float sdt, float capf, float L, float inv_l, uint32_t k0,
-                     uint32_t k1, int frame) {
+                     uint32_t k1, int frame, const float *__restrict__ sdt_dev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (sdt_dev) sdt = *sdt_dev;
   const uint4 r = philox(make_uint4((unsigned)i, (unsigned)frame, 2u, 0u), k0, k1);
   const unsigned rr[3] = {r.x, r.y, r.z};
   float v[3];
@@ -1013,8 +1042,28 @@ int fos_ou_jitter(float *pos, float *v_jit, const float *rad, int n, float mean_
   hipLaunchKernelGGL(k_ou, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, (hipStream_t)stream,
                      pos, v_jit, rad, n, tau, dt, sdt, drift_cap_frac, L,
                      (float)(1.0 / (double)L), (uint32_t)seed ^ 0x5bd1e995u,
-                     (uint32_t)(seed >> 32), frame);
+                     (uint32_t)(seed >> 32), frame, (const float *)nullptr);
   FOS_CHECK_LAUNCH("k_ou");
+  return 0;
+}
+
+int fos_ou_jitter_mean(fos_ctx *ctx, float *pos, float *v_jit, const float *rad, int n,
+                       float dt, float jitter_rms, float jitter_tau, float drift_cap_frac,
+                       float L, uint64_t seed, int frame, float *mean_out, void *stream) {
+  FOS_REQUIRE(ctx && n >= 0 && L > 0.0f && dt > 0.0f && (n == 0 || (v_jit && rad)),
+              "fos_ou_jitter_mean: bad arguments");
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const float tau = jitter_tau > 1e-6f ? jitter_tau : 1e-6f;
+  float *m = ctx->fscratch + 8;  // [0] mean radius, [1] sigma * sqrt(dt)
+  hipLaunchKernelGGL(k_rad_mean, dim3(1), dim3(1024), 0, st, rad, n, dt, jitter_rms, tau, m);
+  FOS_CHECK_LAUNCH("k_rad_mean");
+  hipLaunchKernelGGL(k_ou, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, v_jit, rad, n, tau,
+                     dt, 0.0f, drift_cap_frac, L, (float)(1.0 / (double)L),
+                     (uint32_t)seed ^ 0x5bd1e995u, (uint32_t)(seed >> 32), frame,
+                     (const float *)(m + 1));
+  FOS_CHECK_LAUNCH("k_ou");
+  if (mean_out) FOS_HIP(hipMemcpyAsync(mean_out, m, sizeof(float), hipMemcpyDeviceToDevice, st));
   return 0;
 }
 

@@ -106,251 +106,25 @@ __device__ __forceinline__ bool block_pass_active(const int *stripes, int p, int
   return s_active != 0;
 }
 
-// Adds a wave's changed-voxel count to pass `pass` (one atomic per wave, on
-// the stripe of its global wave id; no block barrier).
-__device__ __forceinline__ void wave_count(int changed, int *stripes, int pass) {
-  int c = changed;
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if ((threadIdx.x & 63) == 0 && c) {
-    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    atomicAdd(stripes + (pass * FOS_STRIPES + wave % FOS_STRIPES) * FOS_STRIPE_INTS, c);
-  }
-}
-
-// The last K owners this voxel evaluated.  A candidate equal to the current
-// best or to an already evaluated owner cannot win under the strict '<'
-// (its power distance is unchanged and best only decreases), so skipping it
-// leaves the result identical to evaluating all 27 in scan order.
-template <int K>
-struct Seen {
-  int e[K];
-  __device__ __forceinline__ void init(int first) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) e[k] = -1;
-    e[0] = first;
-  }
-  __device__ __forceinline__ bool has(int c) const {
-    bool h = false;
-#pragma unroll
-    for (int k = 0; k < K; ++k) h |= (e[k] == c);
-    return h;
-  }
-  __device__ __forceinline__ void push(int c) {
-#pragma unroll
-    for (int k = K - 1; k > 0; --k) e[k] = e[k - 1];
-    e[0] = c;
-  }
-};
-
-template <bool UNIT_L, int K>
-__device__ __forceinline__ void consider(int cand, int &best, float &bd, Seen<K> &seen,
-                                         float cx, float cy, float cz,
-                                         const float4 *__restrict__ seeds, float L) {
-  if (cand >= 0 && cand != best && !seen.has(cand)) {
-    seen.push(cand);
-    const float pd = power_dist<UNIT_L>(cx, cy, cz, seeds[cand], L);
-    if (pd < bd) {
-      bd = pd;
-      best = cand;
-    }
-  }
-}
-
-// xcd_tile (fos_common.h): JFA tiles are ordered x-major, so an XCD sweeps a
-// slab of x-planes whose neighbour rows stay in its own L2.
-
-// Candidate q = (dx+1)*9 + (dy+1)*3 + (dz+1) is skipped when its owner equals
-// the incumbent or an owner met earlier in the scan at the positions where
-// JFA label fields repeat most: the -1 neighbour along an axis where this
-// candidate sits at 0 or +1, and the mirror (-1 side) where it sits at +1.
-// An earlier equal owner has the same power distance, and under the strict
-// '<' a later duplicate never replaces it, so the scan result is unchanged.
-// The decision uses labels only, so the seed gathers of the candidates that
-// remain can be issued nine at a time (one dx-plane) before any is used.
-__device__ __forceinline__ unsigned need_mask(const int lab[27], int inc) {
-  unsigned m = 0;
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    const int dx = q / 9, dy = (q / 3) % 3, dz = q % 3;
-    const int c = lab[q];
-    bool need = c >= 0 && c != inc;
-    if (dz >= 1) need &= c != lab[q - 1];
-    if (dz == 2) need &= c != lab[q - 2];
-    if (dy >= 1) need &= c != lab[q - 3];
-    if (dy == 2) need &= c != lab[q - 6];
-    if (dx >= 1) need &= c != lab[q - 9];
-    if (dx == 2) need &= c != lab[q - 18];
-    m |= need ? (1u << q) : 0u;
-  }
-  return m;
-}
-
-template <bool UNIT_L>
-__device__ __forceinline__ int scan_candidates(const int lab[27], int inc, float cx, float cy,
-                                               float cz, const float4 *__restrict__ seeds,
-                                               float L) {
-  int best = inc;
-  float bd = 1e10f;
-  if (inc >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[inc], L);
-  const unsigned need = need_mask(lab, inc);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    float4 s[9];
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int q = a * 9 + r;
-      s[r] = (need >> q) & 1u ? seeds[lab[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int q = a * 9 + r;
-      if ((need >> q) & 1u) {
-        const float pd = power_dist<UNIT_L>(cx, cy, cz, s[r], L);
-        if (pd < bd) {
-          bd = pd;
-          best = lab[q];
-        }
-      }
-    }
-  }
-  return best;
-}
-
-// Owner of a voxel after one pass over its 27 candidate labels (scan order
-// dx, dy, dz; incumbent lab[13] first; strict '<').  K == 0 skips by label
-// comparison and batches the gathers; K > 0 skips owners among the last K
-// evaluated and gathers one candidate at a time.
-template <bool UNIT_L, int K>
-__device__ __forceinline__ int scan_owner(const int lab[27], int inc, float cx, float cy,
-                                          float cz, const float4 *__restrict__ seeds,
-                                          float L) {
-  if constexpr (K == 0) {
-    return scan_candidates<UNIT_L>(lab, inc, cx, cy, cz, seeds, L);
-  } else {
-    int best = inc;
-    float bd = 1e10f;
-    if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
-    Seen<K> seen;
-    seen.init(best);
-#pragma unroll
-    for (int q = 0; q < 27; ++q) consider<UNIT_L, K>(lab[q], best, bd, seen, cx, cy, cz, seeds, L);
-    return best;
-  }
-}
-
-// One pass, labels read straight from global memory (L2).  Block = 4 waves,
-// tile = 1 x-plane x 4 y-rows x 64 z (one wave per y-row).
-template <bool UNIT_L, int K>
-__global__ void __launch_bounds__(256)
-    k_jfa_pass(const float4 *__restrict__ seeds, int res, float L, float half_l,
-               float voxel, int step, const int32_t *__restrict__ in,
-               int32_t *__restrict__ out, int *__restrict__ stripes, int pass,
-               int early, double th, int min_passes, int nyb, int nzb, int ntiles,
-               int per_xcd) {
-  if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
-  const int tile = xcd_tile(blockIdx.x, per_xcd);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int changed = 0;
-  if (tile < ntiles) {
-    const int zb = tile % nzb, t2 = tile / nzb;
-    const int yb = t2 % nyb, x = t2 / nyb;
-    const int z = zb * 64 + lane, y = yb * 4 + w;
-    if (z < res && y < res) {
-      const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
-                  cz = vcenter(z, voxel, half_l);
-      const int v = (x * res + y) * res + z;
-      int lab[27];
-      int xs[3], ys[3], zs[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        xs[k] = wrap1(x + (k - 1) * step, res);
-        ys[k] = wrap1(y + (k - 1) * step, res);
-        zs[k] = wrap1(z + (k - 1) * step, res);
-      }
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          const int row = (xs[a] * res + ys[b]) * res;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) lab[(a * 3 + b) * 3 + c] = in[row + zs[c]];
-        }
-      const int old = lab[13];
-      const int best = scan_owner<UNIT_L, K>(lab, old, cx, cy, cz, seeds, L);
-      out[v] = best;
-      changed = (best != old);
-    }
-  }
-  wave_count(changed, stripes, pass);
-}
-
-// One pass for step <= halo with the labels of a TX x 4 x 64 tile plus a
-// halo of `step` staged in LDS (one global read per staged label instead of
-// 27 per voxel).  Each thread owns one (y, z) column and TX x-planes.
-template <bool UNIT_L, int K, int TX>
-__global__ void __launch_bounds__(256)
-    k_jfa_pass_lds(const float4 *__restrict__ seeds, int res, float L, float half_l,
-                   float voxel, int step, const int32_t *__restrict__ in,
-                   int32_t *__restrict__ out, int *__restrict__ stripes, int pass,
-                   int early, double th, int min_passes, int nyb, int nzb, int ntiles,
-                   int per_xcd) {
-  extern __shared__ int s_lab[];
-  if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
-  const int tile = xcd_tile(blockIdx.x, per_xcd);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int s = step;
-  const int AY = 4 + 2 * s, AZ = 64 + 2 * s, AX = TX + 2 * s;
-  int changed = 0;
-  const bool live = tile < ntiles;
-  int x0 = 0, y0 = 0, z0 = 0;
-  if (live) {
-    const int zb = tile % nzb, t2 = tile / nzb;
-    const int yb = t2 % nyb;
-    x0 = (t2 / nyb) * TX;
-    y0 = yb * 4;
-    z0 = zb * 64;
-    for (int r = w; r < AX * AY; r += 4) {
-      const int xi = r / AY, yi = r - xi * AY;
-      const int gx = wrap_fast(x0 - s + xi, res), gy = wrap_fast(y0 - s + yi, res);
-      const int base = (gx * res + gy) * res;
-      for (int zi = lane; zi < AZ; zi += 64)
-        s_lab[r * AZ + zi] = in[base + wrap_fast(z0 - s + zi, res)];
-    }
-  }
+// Adds a workgroup's changed-voxel count (the popcounts of its waves'
+// change ballots) to pass `pass`: an LDS sum and one atomic per workgroup, on
+// the stripe of its block id.  A wave that ends on its own atomic keeps its
+// slot until the atomic returns, which costs more than the two barriers.
+// Every thread of the block must call it.
+__device__ __forceinline__ void block_count(unsigned long long bal, int *stripes, int pass) {
+  __shared__ int s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
-  if (live) {
-    const int y = y0 + w, z = z0 + lane;
-    if (y < res && z < res) {
-      const float cy = vcenter(y, voxel, half_l), cz = vcenter(z, voxel, half_l);
-#pragma unroll 1
-      for (int xt = 0; xt < TX; ++xt) {
-        const int x = x0 + xt;
-        if (x >= res) break;
-        const float cx = vcenter(x, voxel, half_l);
-        const int cbase = ((xt + s) * AY + (w + s)) * AZ + (lane + s);
-        int lab[27];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-          for (int b = 0; b < 3; ++b)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              lab[(a * 3 + b) * 3 + c] =
-                  s_lab[cbase + ((a - 1) * AY + (b - 1)) * s * AZ + (c - 1) * s];
-        const int old = lab[13];
-        const int best = scan_owner<UNIT_L, K>(lab, old, cx, cy, cz, seeds, L);
-        out[(x * res + y) * res + z] = best;
-        changed += (best != old);
-      }
-    }
-  }
-  wave_count(changed, stripes, pass);
+  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&s_cnt, __popcll(bal));
+  __syncthreads();
+  if (threadIdx.x == 0 && s_cnt)
+    atomicAdd(stripes + (pass * FOS_STRIPES + blockIdx.x % FOS_STRIPES) * FOS_STRIPE_INTS, s_cnt);
 }
 
 // One pass with per-voxel candidate lists.  The pass is bound by vector ALU
 // issue: a wave pays for a candidate slot whenever any of its 64 voxels
 // evaluates it, and with 27 fixed slots nearly every slot has some voxel
-// that needs it.  Here each voxel writes the owners it must evaluate to its
+// that needs it.  So each voxel writes the owners it must evaluate to its
 // own column of an LDS list (the incumbent first, then the skip-filtered
 // candidates in scan order), and the wave walks the lists in steps of
 // JFA_BATCH entries, so it runs max-over-lanes list length iterations instead
@@ -363,7 +137,9 @@ __global__ void __launch_bounds__(256)
 // along each axis, which pays once the field has converged (the repeated
 // step-1 pass).  Both compare only with earlier candidates.
 #define JFA_BATCH 2
-#define JFA_LIST (26 + JFA_BATCH)
+// list rows: 27 owners plus the B - 1 entries a batch of B reads past the end
+#define JFA_LIST_ROWS(B) (27 + (B) - 1)
+#define JFA_LIST JFA_LIST_ROWS(JFA_BATCH)
 
 template <bool NEIGH>
 __device__ __forceinline__ bool jfa_skip(const int lab[27], int q, int inc) {
@@ -414,41 +190,48 @@ __device__ __forceinline__ unsigned long long incr_wave_mask(
 
 // The owner of one voxel from its 27 candidate labels (lab[13] = current
 // owner): the kept candidates go to this thread's LDS list column, then the
-// list is walked JFA_BATCH entries at a time in list order with strict '<'.
+// list is walked BATCH entries at a time in list order with strict '<'.
 // LT: the LDS list entry type -- int, or unsigned short when every owner id
 // minus `base` is below 65536 (half the LDS, more resident workgroups).
-template <bool UNIT_L, int MODE, typename LT = int>
+// DIN: bd holds the incumbent's power distance on entry (the previous
+// pass's result for this voxel: the same voxel centre and seed, so the same
+// fp32 value), and the incumbent is not evaluated again.  On return bd is
+// the winner's power distance (1e10 when no owner).
+template <bool UNIT_L, int MODE, typename LT = int, int BATCH = JFA_BATCH, bool DIN = false>
 __device__ __forceinline__ int jfa_list_eval(const int lab[27], float cx, float cy, float cz,
                                              const float4 *__restrict__ seeds, float L,
-                                             LT (*s_list)[256], int base = 0) {
+                                             LT (*s_list)[256], int base, float &bd) {
   const int t = (int)threadIdx.x;
   const int inc = lab[13];
+  const bool have = DIN && inc >= 0;
   // Branch-free list build: every candidate is written at the current
   // end and the end advances only for kept owners.
   s_list[0][t] = (LT)(inc - base);
-  int cnt = inc >= 0 ? 1 : 0;
+  int cnt = (inc >= 0 && !have) ? 1 : 0;
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
     if (q == 13) continue;
     s_list[cnt][t] = (LT)(lab[q] - base);
     cnt += jfa_skip<MODE != JFA_PLAIN && MODE != JFA_HALF>(lab, q, inc) ? 0 : 1;
   }
-  int best = -1;
+  int best = have ? inc : -1;
+  if (!have) bd = 1e10f;
   if (cnt > 0) {
+    // entries past the end repeat the first kept owner (a valid id; a
+    // repeat never wins under the strict '<')
     const LT first = s_list[0][t];
 #pragma unroll
-    for (int i = 0; i < JFA_BATCH - 1; ++i) s_list[cnt + i][t] = first;
-    float bd = 1e10f;
-    for (int k = 0; k < cnt; k += JFA_BATCH) {
-      int c[JFA_BATCH];
-      float4 sd[JFA_BATCH];
+    for (int i = 0; i < BATCH - 1; ++i) s_list[cnt + i][t] = first;
+    for (int k = 0; k < cnt; k += BATCH) {
+      int c[BATCH];
+      float4 sd[BATCH];
 #pragma unroll
-      for (int i = 0; i < JFA_BATCH; ++i) c[i] = (int)s_list[k + i][t] + base;
+      for (int i = 0; i < BATCH; ++i) c[i] = (int)s_list[k + i][t] + base;
 #pragma unroll
-      for (int i = 0; i < JFA_BATCH; ++i)
+      for (int i = 0; i < BATCH; ++i)
         sd[i] = *(const float4 *)((const char *)seeds + ((uint32_t)c[i] << 4));
 #pragma unroll
-      for (int i = 0; i < JFA_BATCH; ++i) {
+      for (int i = 0; i < BATCH; ++i) {
         const float pd = power_dist<UNIT_L>(cx, cy, cz, sd[i], L);
         if (pd < bd) {
           bd = pd;
@@ -463,12 +246,15 @@ __device__ __forceinline__ int jfa_list_eval(const int lab[27], float cx, float 
 // One tile (4 z-rows of 64 voxels at one x, this thread's wave one row) of a
 // list pass; returns this wave's change ballot (0 for an idle row).  The
 // caller has checked that the pass is active.
-template <bool UNIT_L, int MODE, typename LT = int>
+// DM: 0 no distance buffers; 1 writes each voxel's winning power distance to
+// dist_out; 2 also starts from dist_in (the previous pass's dist_out).
+template <bool UNIT_L, int MODE, typename LT = int, int BATCH = JFA_BATCH, int DM = 0>
 __device__ __forceinline__ unsigned long long jfa_list_tile(
     const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel, int step,
     const int32_t *__restrict__ in, int32_t *__restrict__ out, int tile, int nyb, int nzb,
     int ntiles, const unsigned long long *__restrict__ chg_prev,
-    unsigned long long *__restrict__ chg_out, LT (*s_list)[256], int base = 0) {
+    unsigned long long *__restrict__ chg_out, LT (*s_list)[256], int base = 0,
+    const float *__restrict__ dist_in = nullptr, float *__restrict__ dist_out = nullptr) {
   const int t = (int)threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   int changed = 0;
@@ -488,12 +274,14 @@ __device__ __forceinline__ unsigned long long jfa_list_tile(
       const unsigned long long m = incr_wave_mask(chg_prev, res, nzb, x, y, zb);
       if (work && !((m >> lane) & 1ull)) {
         out[v] = in[v];
+        if (DM == 2) dist_out[v] = dist_in[v];
         work = false;
       }
     }
     if (work) {
       const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
                   cz = vcenter(z, voxel, half_l);
+      float bd = DM == 2 ? dist_in[v] : 0.0f;
       // x, y are wave-uniform: the nine row bases live in scalar registers
       // and each label load adds one of three per-lane 32-bit z offsets
       // (scalar base + vector offset addressing; res <= 1290 keeps a row
@@ -521,8 +309,10 @@ __device__ __forceinline__ unsigned long long jfa_list_tile(
               lab[q] = *(const int32_t *)(row + zoff[c]);
           }
         }
-      const int best = jfa_list_eval<UNIT_L, MODE, LT>(lab, cx, cy, cz, seeds, L, s_list, base);
+      const int best = jfa_list_eval<UNIT_L, MODE, LT, BATCH, DM == 2>(lab, cx, cy, cz, seeds, L,
+                                                                         s_list, base, bd);
       out[v] = best;
+      if (DM) dist_out[v] = bd;
       changed = (best != lab[13]);
     }
   }
@@ -575,32 +365,30 @@ __device__ __forceinline__ unsigned long long jfa_flat_tile(
             lab[q] = plane[yo[b] + zo[c]];
         }
     }
-    const int best = jfa_list_eval<UNIT_L, MODE, LT>(lab, cx, cy, cz, seeds, L, s_list, base);
+    float bd;
+    const int best = jfa_list_eval<UNIT_L, MODE, LT>(lab, cx, cy, cz, seeds, L, s_list, base, bd);
     out[v] = best;
     changed = (best != lab[13]);
   }
   return __ballot(changed);
 }
 
-template <bool UNIT_L, int MODE, typename LT>
+template <bool UNIT_L, int MODE, typename LT, int BATCH = JFA_BATCH, int DM = 0>
 __global__ void __launch_bounds__(256)
     k_jfa_pass_list(const float4 *__restrict__ seeds, int res, float L, float half_l,
                     float voxel, int step, const int32_t *__restrict__ in,
                     int32_t *__restrict__ out, int *__restrict__ stripes, int pass, int early,
                     double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd,
                     const unsigned long long *__restrict__ chg_prev,
-                    unsigned long long *__restrict__ chg_out) {
-  __shared__ LT s_list[JFA_LIST][256];
+                    unsigned long long *__restrict__ chg_out, const float *__restrict__ dist_in,
+                    float *__restrict__ dist_out) {
+  __shared__ LT s_list[JFA_LIST_ROWS(BATCH)][256];
   if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
   const int tile = xcd_tile(blockIdx.x, per_xcd);
-  const unsigned long long bal = jfa_list_tile<UNIT_L, MODE, LT>(
+  const unsigned long long bal = jfa_list_tile<UNIT_L, MODE, LT, BATCH, DM>(
       seeds, res, L, half_l, voxel, step, in, out, tile, nyb, nzb, ntiles, chg_prev, chg_out,
-      s_list);
-  if ((threadIdx.x & 63) == 0 && bal) {
-    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    atomicAdd(stripes + (pass * FOS_STRIPES + wave % FOS_STRIPES) * FOS_STRIPE_INTS,
-              __popcll(bal));
-  }
+      s_list, 0, dist_in, dist_out);
+  block_count(bal, stripes, pass);
 }
 
 // @jfa-pass-kernel-end
@@ -1540,69 +1328,66 @@ int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
 
 }  // extern "C"
 
-// Pass-kernel tunables (fos_set_param): steps <= jfa_lds_max_step use the
-// LDS-tiled kernel with jfa_tile_x x-planes per block, larger steps the L2
-// kernel; jfa_dedup 0 skips candidates by label comparison and batches the
-// seed gathers, 1/4/8 keeps a list of that many evaluated owners.  These
-// apply with jfa_kernel 1 (fixed candidate slots); jfa_kernel 0 (default) is
-// the per-voxel candidate-list kernel.
-static int g_lds_max_step = 0;
-static int g_tile_x = 4;
-static int g_dedup = 0;
-static int g_kernel = 0;
 static int g_emit_tx = 8;
 static int g_emit_zw = 32;
 extern int g_grid_block;  // fos_grid.hip
 extern int g_grid_cpw;    // fos_grid.hip
 
 // @jfa-pass-launch-begin
-template <bool U, int K>
-static void launch_pass_k(fos_ctx *ctx, int res, float L, float half_l, float voxel,
-                          int step, const int32_t *in, int32_t *out, int pass, int early,
-                          double th, int minp, hipStream_t st) {
-  int *stripes = ctx->stripes;
-  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
-  const size_t lds_bytes =
-      (size_t)(g_tile_x + 2 * step) * (4 + 2 * step) * (64 + 2 * step) * sizeof(int);
-  if (step <= g_lds_max_step && lds_bytes <= 65536) {
-    const int tx = g_tile_x;
-    const int nxb = (res + tx - 1) / tx;
-    const int ntiles = nxb * nyb * nzb;
-    const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
-    const size_t shm = (size_t)(tx + 2 * step) * (4 + 2 * step) * (64 + 2 * step) * sizeof(int);
-    auto kern = tx == 1 ? k_jfa_pass_lds<U, K, 1> : tx == 2 ? k_jfa_pass_lds<U, K, 2>
-                                                            : k_jfa_pass_lds<U, K, 4>;
-    hipLaunchKernelGGL(kern, dim3(per * FOS_XCDS), dim3(256), shm, st, ctx->seeds, res, L,
-                       half_l, voxel, step, in, out, stripes, pass, early, th, minp, nyb, nzb,
-                       ntiles, per);
-  } else {
-    const int ntiles = res * nyb * nzb;
-    const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
-    hipLaunchKernelGGL((k_jfa_pass<U, K>), dim3(per * FOS_XCDS), dim3(256), 0, st, ctx->seeds,
-                       res, L, half_l, voxel, step, in, out, stripes, pass, early, th, minp,
-                       nyb, nzb, ntiles, per);
-  }
+// fos_set_param "jfa_variant": bit 0 evaluates four list entries per
+// iteration (two otherwise), bit 1 carries each voxel's winning power
+// distance into the next pass (the incumbent is not evaluated again); -1
+// (default) picks four entries for 32-bit lists (ids above 65,535: the
+// large grids, where it was measured faster) and the distance carry
+// whenever the context holds the distance buffers (grids up to 256^3).
+static int g_jfa_variant = -1;
+static int jfa_variant(const fos_ctx *ctx) {
+  if (g_jfa_variant >= 0) return g_jfa_variant;
+  return (ctx->max_n > 65536 ? 1 : 0) | (ctx->dist[0] ? 2 : 0);
 }
 
+template <bool U, int MODE, typename LT, int BATCH, int DM>
+static void launch_pass_list_v(fos_ctx *ctx, int res, float L, float half_l, float voxel,
+                               int step, const int32_t *in, int32_t *out, int pass, int early,
+                               double th, int minp, const unsigned long long *chg_prev,
+                               unsigned long long *chg_out, const float *dist_in,
+                               float *dist_out, hipStream_t st) {
+  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
+  const int ntiles = res * nyb * nzb;
+  const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
+  hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, LT, BATCH, DM>), dim3(per * FOS_XCDS), dim3(256),
+                     0, st, ctx->seeds, res, L, half_l, voxel, step, in, out, ctx->stripes, pass,
+                     early, th, minp, nyb, nzb, ntiles, per, chg_prev, chg_out, dist_in,
+                     dist_out);
+}
+
+// dm: 0 no distance buffers, 1 write dist_out, 2 read dist_in and write dist_out
 template <bool U, int MODE>
 static void launch_pass_list(fos_ctx *ctx, int res, float L, float half_l, float voxel,
                              int step, const int32_t *in, int32_t *out, int pass, int early,
                              double th, int minp, const unsigned long long *chg_prev,
-                             unsigned long long *chg_out, hipStream_t st) {
-  const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
-  const int ntiles = res * nyb * nzb;
-  const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
+                             unsigned long long *chg_out, int dm, const float *dist_in,
+                             float *dist_out, hipStream_t st) {
+  const bool b4 = (jfa_variant(ctx) & 1) != 0;
+#define FOS_LV(LT, B, D)                                                                     \
+  launch_pass_list_v<U, MODE, LT, B, D>(ctx, res, L, half_l, voxel, step, in, out, pass,     \
+                                        early, th, minp, chg_prev, chg_out, dist_in, dist_out, \
+                                        st)
+#define FOS_LD(LT, B)                                                                        \
+  do {                                                                                       \
+    if (dm == 2) FOS_LV(LT, B, 2);                                                           \
+    else if (dm == 1) FOS_LV(LT, B, 1);                                                      \
+    else FOS_LV(LT, B, 0);                                                                   \
+  } while (0)
   // owner ids are below the context's particle capacity: 16-bit list
   // entries when that is at most 65536
-  if (ctx->max_n <= 65536)
-    hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, unsigned short>), dim3(per * FOS_XCDS),
-                       dim3(256), 0, st, ctx->seeds, res, L, half_l, voxel, step, in, out,
-                       ctx->stripes, pass, early, th, minp, nyb, nzb, ntiles, per, chg_prev,
-                       chg_out);
-  else
-    hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, int>), dim3(per * FOS_XCDS), dim3(256), 0, st,
-                       ctx->seeds, res, L, half_l, voxel, step, in, out, ctx->stripes, pass,
-                       early, th, minp, nyb, nzb, ntiles, per, chg_prev, chg_out);
+  if (ctx->max_n <= 65536) {
+    if (b4) FOS_LD(unsigned short, 4); else FOS_LD(unsigned short, 2);
+  } else {
+    if (b4) FOS_LD(int, 4); else FOS_LD(int, 2);
+  }
+#undef FOS_LD
+#undef FOS_LV
 }
 
 // Launches pass `pass` (its changed-voxel count accumulates in stripe slot
@@ -1610,49 +1395,34 @@ static void launch_pass_list(fos_ctx *ctx, int res, float L, float half_l, float
 // step equals the previous pass's (the field is close to converged): the
 // list kernel then also skips owners repeated by -1 neighbours, and with the
 // previous pass's change masks (chg_prev) recomputes only voxels next to a
-// change.  chg_out (may be NULL) receives this pass's change masks.
+// change.  chg_out (may be NULL) receives this pass's change masks.  dm and
+// the distance buffers as for launch_pass_list.
 static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
                        const int32_t *in, int32_t *out, int pass, int early, double th,
                        int min_passes, bool settled, const unsigned long long *chg_prev,
-                       unsigned long long *chg_out, hipStream_t st) {
+                       unsigned long long *chg_out, hipStream_t st, int dm = 0,
+                       const float *dist_in = nullptr, float *dist_out = nullptr) {
   const float half_l = (float)(0.5 * (double)L);
   const bool u = unit_l(L);
-  if (g_kernel == 0) {
-    const int mode = 2 * step == res ? JFA_HALF
-                     : settled       ? (chg_prev ? JFA_INCR : JFA_SETTLED)
-                                     : JFA_PLAIN;
+  const int mode = 2 * step == res ? JFA_HALF
+                   : settled       ? (chg_prev ? JFA_INCR : JFA_SETTLED)
+                                   : JFA_PLAIN;
 #define FOS_LIST(U, M)                                                                   \
   launch_pass_list<U, M>(ctx, res, L, half_l, voxel, step, in, out, pass, early, th,    \
-                         min_passes, chg_prev, chg_out, st)
-    if (u) {
-      if (mode == JFA_HALF) FOS_LIST(true, JFA_HALF);
-      else if (mode == JFA_INCR) FOS_LIST(true, JFA_INCR);
-      else if (mode == JFA_SETTLED) FOS_LIST(true, JFA_SETTLED);
-      else FOS_LIST(true, JFA_PLAIN);
-    } else {
-      if (mode == JFA_HALF) FOS_LIST(false, JFA_HALF);
-      else if (mode == JFA_INCR) FOS_LIST(false, JFA_INCR);
-      else if (mode == JFA_SETTLED) FOS_LIST(false, JFA_SETTLED);
-      else FOS_LIST(false, JFA_PLAIN);
-    }
-#undef FOS_LIST
-    FOS_CHECK_LAUNCH("k_jfa_pass_list");
-    return 0;
-  }
-#define FOS_PASS(U, K)                                                                  \
-  launch_pass_k<U, K>(ctx, res, L, half_l, voxel, step, in, out, pass, early, th, \
-                      min_passes, st)
-  if (g_dedup == 0) {
-    if (u) FOS_PASS(true, 0); else FOS_PASS(false, 0);
-  } else if (g_dedup <= 1) {
-    if (u) FOS_PASS(true, 1); else FOS_PASS(false, 1);
-  } else if (g_dedup <= 4) {
-    if (u) FOS_PASS(true, 4); else FOS_PASS(false, 4);
+                         min_passes, chg_prev, chg_out, dm, dist_in, dist_out, st)
+  if (u) {
+    if (mode == JFA_HALF) FOS_LIST(true, JFA_HALF);
+    else if (mode == JFA_INCR) FOS_LIST(true, JFA_INCR);
+    else if (mode == JFA_SETTLED) FOS_LIST(true, JFA_SETTLED);
+    else FOS_LIST(true, JFA_PLAIN);
   } else {
-    if (u) FOS_PASS(true, 8); else FOS_PASS(false, 8);
+    if (mode == JFA_HALF) FOS_LIST(false, JFA_HALF);
+    else if (mode == JFA_INCR) FOS_LIST(false, JFA_INCR);
+    else if (mode == JFA_SETTLED) FOS_LIST(false, JFA_SETTLED);
+    else FOS_LIST(false, JFA_PLAIN);
   }
-#undef FOS_PASS
-  FOS_CHECK_LAUNCH("k_jfa_pass");
+#undef FOS_LIST
+  FOS_CHECK_LAUNCH("k_jfa_pass_list");
   return 0;
 }
 
@@ -1720,19 +1490,9 @@ extern "C" {
 
 int fos_set_param(const char *name, int value) {
   FOS_REQUIRE(name, "fos_set_param: name is NULL");
-  if (!strcmp(name, "jfa_lds_max_step")) {
-    FOS_REQUIRE(value >= 0 && value <= 8, "jfa_lds_max_step must be in [0, 8]");
-    g_lds_max_step = value;
-  } else if (!strcmp(name, "jfa_tile_x")) {
-    FOS_REQUIRE(value == 1 || value == 2 || value == 4, "jfa_tile_x must be 1, 2 or 4");
-    g_tile_x = value;
-  } else if (!strcmp(name, "jfa_dedup")) {
-    FOS_REQUIRE(value == 0 || value == 1 || value == 4 || value == 8,
-                "jfa_dedup must be 0, 1, 4 or 8");
-    g_dedup = value;
-  } else if (!strcmp(name, "jfa_kernel")) {
-    FOS_REQUIRE(value == 0 || value == 1, "jfa_kernel must be 0 or 1");
-    g_kernel = value;
+  if (!strcmp(name, "jfa_variant")) {
+    FOS_REQUIRE(value >= -1 && value <= 3, "jfa_variant must be in [-1, 3]");
+    g_jfa_variant = value;
   } else if (!strcmp(name, "face_emit_tx")) {
     FOS_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4 || value == 8 ||
                     value == 16,
@@ -1758,14 +1518,8 @@ int fos_set_param(const char *name, int value) {
 
 int fos_get_param(const char *name, int *value) {
   FOS_REQUIRE(name && value, "fos_get_param: bad arguments");
-  if (!strcmp(name, "jfa_lds_max_step"))
-    *value = g_lds_max_step;
-  else if (!strcmp(name, "jfa_tile_x"))
-    *value = g_tile_x;
-  else if (!strcmp(name, "jfa_dedup"))
-    *value = g_dedup;
-  else if (!strcmp(name, "jfa_kernel"))
-    *value = g_kernel;
+  if (!strcmp(name, "jfa_variant"))
+    *value = g_jfa_variant;
   else if (!strcmp(name, "grid_block"))
     *value = g_grid_block;
   else if (!strcmp(name, "face_emit_tx"))
@@ -1839,11 +1593,20 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
     const int32_t *in = (p & 1) ? temp : grid;
     int32_t *out = (p & 1) ? grid : temp;
     if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[p], st));
-    unsigned long long *chg_out = ctx->chg[p & 1];
+    // masks are read by the next pass only when it repeats this step a
+    // third time in a row (step == prev there)
+    const int next = step / 2 < 1 ? 1 : step / 2;
+    unsigned long long *chg_out =
+        (p + 1 < num_passes && next == step && step == prev) ? ctx->chg[p & 1] : nullptr;
     const unsigned long long *chg_prev =
         (step == prev && prev == prev2) ? ctx->chg[(p + 1) & 1] : nullptr;
+    // winning power distances carried from pass to pass (the first pass
+    // evaluates the seeded incumbents and writes them)
+    const bool carry = ctx->dist[0] && (jfa_variant(ctx) & 2);
     rc = launch_step(ctx, res, L, voxel, step, in, out, p, early_exit, th_pass_delta,
-                     min_passes, step == prev, chg_prev, chg_out, st);
+                     min_passes, step == prev, chg_prev, chg_out, st, carry ? (p ? 2 : 1) : 0,
+                     carry ? ctx->dist[(p + 1) & 1] : nullptr,
+                     carry ? ctx->dist[p & 1] : nullptr);
     if (rc) return rc;
     prev2 = prev;
     prev = step;
@@ -2406,10 +2169,7 @@ __global__ void __launch_bounds__(256)
                                                out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
                                                s_list, base);
   }
-  if ((threadIdx.x & 63) == 0 && bal) {
-    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    atomicAdd(st + (pass * FOS_STRIPES + wave % FOS_STRIPES) * FOS_STRIPE_INTS, __popcll(bal));
-  }
+  block_count(bal, st, pass);
 }
 
 // Per foam (one wave each): passes executed, exit flag and per-pass change

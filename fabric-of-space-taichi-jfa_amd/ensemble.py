@@ -47,6 +47,7 @@ def c4_configs(B=256, seed=0, n_min=500, n_max=20000):
         res = int(min(160, max(48, round(5.0 * n ** (1.0 / 3.0)))))
         lo, hi = bands[int(rng.integers(0, 3))]
         cfg = fc.preset("c2")
+        cfg.ou_jitter = False          # the batch has no OU jitter kernel
         cfg.name = f"c4[{b}]"
         cfg.n, cfg.res = n, res
         cfg.r_min, cfg.r_max = fc.f32(0.6 * s * 0.5), fc.f32(1.2 * s * 1.0)

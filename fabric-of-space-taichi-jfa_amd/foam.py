@@ -288,12 +288,11 @@ class FoamSim:
                 self.rebuild_grid(pr=True)
             ready = p < passes - 1 or not c.ou_jitter
             self.pbd_pass(cells=ready)
-        # 6 OU jitter
+        # 6 OU jitter (the mean radius of run.py:860-861 computed on the device)
         if c.ou_jitter:
-            mean_r = float(self.rad.mean().item())
-            fl.call("fos_ou_jitter", fl.ptr(self.pos), fl.ptr(self.v_jit), fl.ptr(self.rad),
-                    self.n, mean_r, fc.DT, fc.JITTER_RMS, fc.JITTER_TAU,
-                    fc.f32(fc.MAX_DRIFT_FRACTION * c.gap_fraction), c.L, c.seed, f, st)
+            fl.call("fos_ou_jitter_mean", self.ctx.handle, fl.ptr(self.pos), fl.ptr(self.v_jit),
+                    fl.ptr(self.rad), self.n, fc.DT, fc.JITTER_RMS, fc.JITTER_TAU,
+                    fc.f32(fc.MAX_DRIFT_FRACTION * c.gap_fraction), c.L, c.seed, f, None, st)
         # 7
         if ready:
             self.rebuild_from_cells()

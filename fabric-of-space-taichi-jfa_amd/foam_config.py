@@ -8,7 +8,8 @@ JIT time, they are read at call time: ``apply(cfg)`` rebinds them for a run.
 ``FoamConfig`` is the headless driver's configuration; ``preset(name)`` gives
 the benchmark workloads:
   c1   N=2,000, 64^3, uniform unit box, 20 frames, JFA every frame
-  c2   N=10,000, 128^3, 200 frames, JFA every frame (headline)
+  c2   N=10,000, 128^3, 200 frames, JFA every frame, Gaussian Brownian noise
+       and the OU jitter (headline)
   c2c  as c2 with JFA on a 1/5 cadence after a 20-frame warm start
   c3   N=200,000 clustered, 384^3, 50 frames
   c5   N=50,000, 256^3, radii U[0.005, 0.011] (learned face predictor workload)
@@ -184,10 +185,11 @@ def preset(name):
                           pbd_base=8, pbd_max=8, f_low=10, f_high=16,
                           pressure_iters=10, frames=20, warmstart_frames=5)
     if name in ("c2", "c2c"):
+        # the reference frame also runs the OU jitter every frame (run.py:857-867)
         cfg = FoamConfig(name=name, n=10000, res=128, r_min=0.006, r_max=0.024,
                          pbd_base=PBD_BASE_PASSES, pbd_max=12, f_low=12, f_high=16,
                          pressure_iters=20, brownian=True, brownian_mode=1,
-                         brownian_strength=1e-4, frames=200)
+                         brownian_strength=1e-4, ou_jitter=True, frames=200)
         if name == "c2c":
             cfg.jfa_every, cfg.jfa_warm = 5, 20
         return cfg

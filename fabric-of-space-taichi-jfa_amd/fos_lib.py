@@ -65,6 +65,7 @@ SIGNATURES = {
     "fos_brownian_cells": [P, P, I, F, F, U64, I, I, I, P],
     "fos_settle_nudge": [P, P, P, I, F, F, F, F, I, F, D, P, P, P, I, I, I, P],
     "fos_ou_jitter": [P, P, P, I, F, F, F, F, F, F, U64, I, P],
+    "fos_ou_jitter_mean": [P, P, P, P, I, F, F, F, F, F, U64, I, P, P],
     "fos_integrate_jitter": [P, P, I, F, F, P],
     "fos_xsph": [P, P, P, I, I, F, F, F, P, P, P, P],
     "fos_damp": [P, I, F, P],

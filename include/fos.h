@@ -36,15 +36,13 @@ typedef struct fos_batch fos_batch;
 int fos_abi_version(void);
 const char *fos_last_error(void);
 
-/* Launch tunables of the JFA pass kernels (process-wide), set by the tuner:
- *   "jfa_lds_max_step" 0..8   steps up to this use the LDS-tiled kernel
- *   "jfa_tile_x"       1,2,4  x-planes per LDS tile
- *   "jfa_kernel"       0,1    0: per-voxel candidate lists (default);
- *                      1: fixed 27 candidate slots, tuned by jfa_dedup and
- *                      the two LDS parameters above
- *   "jfa_dedup"        0,1,4,8  (jfa_kernel 1) 0: skip candidates by label
- *                      comparison and batch the seed gathers; 1/4/8: skip
- *                      owners among that many last evaluated
+/* Launch tunables (process-wide):
+ *   "jfa_variant"      -1..3  JFA pass kernel: bit 0 evaluates four candidate
+ *                      list entries per loop iteration (two otherwise), bit 1
+ *                      carries each voxel's winning power distance from pass
+ *                      to pass (needs a context with max_res <= 256); -1
+ *                      (default) picks per context: four entries when ids
+ *                      exceed 65,535, the carry whenever available
  *   "face_emit_tx"     0,1,2,4,8,16  face-pair emission: 0 merges pairs
  *                      within a wave; others merge within a workgroup tile of that
  *                      many x-planes in LDS (default 8)
@@ -334,6 +332,13 @@ int fos_ou_jitter(float *pos, float *v_jit, const float *rad, int n,
                   float mean_r, float dt, float jitter_rms, float jitter_tau,
                   float drift_cap_frac, float L, uint64_t seed, int frame,
                   void *stream);
+/* fos_ou_jitter with the mean radius computed on the device (run.py:860-861
+ * computes it on the host): the exact double sum of the radii over n, rounded
+ * to fp32, sets the noise scale; no host read.  mean_out (device float, may be
+ * NULL) receives that mean. */
+int fos_ou_jitter_mean(fos_ctx *ctx, float *pos, float *v_jit, const float *rad, int n,
+                       float dt, float jitter_rms, float jitter_tau, float drift_cap_frac,
+                       float L, uint64_t seed, int frame, float *mean_out, void *stream);
 /* dynamics.py:649-660 integrate_jitter: pos = wrapP(pos + v_jit*dt). */
 int fos_integrate_jitter(float *pos, const float *v_jit, int n, float dt,
                          float L, void *stream);

@@ -79,6 +79,7 @@ def frame_foams():
     cfgs, inputs = [], []
     for b, (n, res, rmin, rmax, lo, hi) in enumerate(spec):
         cfg = fc.preset("c2")
+        cfg.ou_jitter = False              # the batch has no OU jitter kernel
         cfg.n, cfg.res, cfg.r_min, cfg.r_max = n, res, rmin, rmax
         cfg.f_low, cfg.f_high = lo, hi
         cfg.warmstart_frames = 1

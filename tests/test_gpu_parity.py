@@ -144,43 +144,41 @@ def test_jfa_single_steps_match(gpu, orc):
         cur = out
 
 
-PASS_VARIANTS = {
-    "list": {"jfa_kernel": 0},
-    "slots_batched": {"jfa_kernel": 1, "jfa_dedup": 0, "jfa_lds_max_step": 0},
-    "slots_seen1": {"jfa_kernel": 1, "jfa_dedup": 1, "jfa_lds_max_step": 0},
-    "slots_seen8": {"jfa_kernel": 1, "jfa_dedup": 8, "jfa_lds_max_step": 0},
-    "slots_lds_tx2": {"jfa_kernel": 1, "jfa_dedup": 4, "jfa_lds_max_step": 4, "jfa_tile_x": 2},
-}
+# fos_set_param "jfa_variant": bit 0 four list entries per iteration (two
+# otherwise), bit 1 the winning power distance carried from pass to pass
+PASS_VARIANTS = {"batch2": 0, "batch4": 1, "batch2_carry": 2, "batch4_carry": 3}
 
 
 @pytest.mark.parametrize("variant", sorted(PASS_VARIANTS))
 def test_jfa_pass_kernel_variants(gpu, orc, variant):
-    """Every pass kernel and tunable gives the oracle's labels, including the
-    collision-heavy dense case and a res that is not a multiple of 64."""
+    """Every pass-kernel variant gives the oracle's labels and change counts,
+    including the collision-heavy dense case, a res that is not a multiple of
+    64, more step-1 passes than the schedule (the incremental passes), early
+    exit, and ids above 65,535 (32-bit candidate lists)."""
     import ctypes
-    saved = {}
-    for k in ("jfa_kernel", "jfa_dedup", "jfa_lds_max_step", "jfa_tile_x"):
-        v = ctypes.c_int()
-        gpu.call("fos_get_param", k.encode(), ctypes.byref(v))
-        saved[k] = v.value
+    v0 = ctypes.c_int()
+    gpu.call("fos_get_param", b"jfa_variant", ctypes.byref(v0))
     try:
-        for k, v in PASS_VARIANTS[variant].items():
-            gpu.call("fos_set_param", k.encode(), v)
-        for (n, res, seed, dup) in ((1500, 72, 7, False), (3000, 16, 2, True)):
-            pos, rad = fi.uniform(n, 0.01, 0.03, seed=seed)
+        gpu.call("fos_set_param", b"jfa_variant", PASS_VARIANTS[variant])
+        cases = ((1500, 72, 7, False, 0, False), (3000, 16, 2, True, 0, False),
+                 (900, 66, 5, False, 3, False), (2500, 48, 9, False, 0, True),
+                 (70000, 96, 4, False, 0, False))
+        for (n, res, seed, dup, extra, early) in cases:
+            pos, rad = fi.uniform(n, 0.005, 0.03, seed=seed)
             if dup:
                 pos[1::2] = pos[0::2]
                 rad[1::2] = rad[0::2]
             voxel = fc.f32(1.0 / res)
-            P = int(math.ceil(math.log2(res))) + 1
-            print(f"[jfa variant] {variant} n={n} res={res} passes={P}", flush=True)
-            lab, passes, _, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, P)
-            glab, info = run_jfa_gpu(gpu, pos, rad, n, res, P)
+            P = int(math.ceil(math.log2(res))) + 1 + extra
+            print(f"[jfa variant] {variant} n={n} res={res} passes={P} early={early}",
+                  flush=True)
+            lab, passes, ex, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, P, early)
+            glab, info = run_jfa_gpu(gpu, pos, rad, n, res, P, early)
             assert (glab == lab).all()
+            assert info[0] == passes and info[1] == int(ex)
             assert (info[2:2 + passes] == hist).all()
     finally:
-        for k, v in saved.items():
-            gpu.call("fos_set_param", k.encode(), v)
+        gpu.call("fos_set_param", b"jfa_variant", v0.value)
 
 
 @pytest.mark.parametrize("res,extra", [(66, 3), (130, 2), (64, 4)])
@@ -784,6 +782,21 @@ def test_noise_kernels(gpu, orc):
     gpu.call("fos_ou_jitter", gpu.ptr(tp), gpu.ptr(tv), gpu.ptr(dev(rad)), n, 0.018, 0.016, 0.1,
              1.0, 0.003, 1.0, 99, 4, gpu.stream())
     assert np.array_equal(host(tv), wv) and np.array_equal(host(tp), wp)
+    # the mean radius computed on the device (run.py:860-861): the exact
+    # double sum rounded to fp32, so the OU step matches the oracle's frame
+    # (which takes f32(mean(rad, float64))) bit for bit, for ragged n too
+    g = G(gpu, n, 1)
+    for m in (n, 1, 1023, 4097):
+        r = rad[:m].copy()
+        mean = F(np.mean(r, dtype=np.float64))
+        wp, wv = orc.ou_jitter(pos[:m], v[:m], r, m, mean, fc.f32(0.016), fc.f32(0.1),
+                               fc.f32(1.0), fc.f32(0.003), 1.0, 99, 4)
+        tp, tv = dev(pos[:m]), dev(v[:m])
+        tm = torch.zeros(1, dtype=torch.float32, device="cuda")
+        gpu.call("fos_ou_jitter_mean", g.h, gpu.ptr(tp), gpu.ptr(tv), gpu.ptr(dev(r)), m, 0.016,
+                 0.1, 1.0, 0.003, 1.0, 99, 4, gpu.ptr(tm), gpu.stream())
+        assert host(tm)[0] == mean, m
+        assert np.array_equal(host(tv), wv) and np.array_equal(host(tp), wp), m
 
 
 def test_brownian_cells_equals_brownian_then_rebuild(gpu):
