@@ -455,7 +455,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "uniform box N=10000, radii U[0.012,0.024], JFA 128^3 int32, "
                                "8 passes, JFA every frame, PBD<=12, pressure x20, "
-                               "Gaussian Brownian 1e-4",
+                               "Gaussian Brownian 1e-4, OU jitter (run.py:857-867); the frame "
+                               "issued by fos_frame_step",
                    "n": cfg.n, "jfa_res": cfg.res, "jfa_passes": P, "hash_res": cfg.hash_res(),
                    "frames": args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

@@ -319,6 +319,58 @@ __global__ void __launch_bounds__(FOS_BLOCK)
   atomicExch(&cnt[1], 0);
 }
 
+// run.py:1184-1195 gate from a complete zero-FSC count, then (gate open)
+// dynamics.py:1281-1320 set_fsc_targets: one launch where fos_fsc_gate and
+// fos_fsc_targets_gated take two.  Every thread derives the same gate in
+// double; block 0 thread 0 publishes it to *gate (1.0 = growth).
+__global__ void k_gate_targets(const float *__restrict__ rad, float *__restrict__ rt,
+                               const int32_t *__restrict__ fsc, const float *__restrict__ ema,
+                               int n, int frame, int warmstart, double zero_rate,
+                               const int *__restrict__ zero_cnt, float fl, float fh, float gp,
+                               float r_min, float r_max, float db, float *__restrict__ gate) {
+  const double f0 = (double)*zero_cnt / (double)(n > 1 ? n : 1);
+  const bool open = !(frame < warmstart || f0 > zero_rate);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *gate = open ? 1.0f : 0.0f;
+  if (!open) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float r0 = rad[i];
+  const float f = ema[i] > 0.0f ? ema[i] : (float)fsc[i];
+  const float gap = 1.0f, idle = 0.003f;
+  float out;
+  if (f < fl - gap) {
+    out = fminf(r0 * (1.0f + gp * 1.0f), r_max);
+  } else if (f < fl) {
+    const float g = 1.0f - smoothstep(fl - gap - db, fl, f);
+    out = fminf(r0 * (1.0f + gp * g), r_max);
+  } else if (f > fh + gap) {
+    out = fmaxf(r0 * (1.0f - gp * 1.0f), r_min);
+  } else if (f > fh) {
+    const float g = smoothstep(fh, fh + gap + db, f);
+    out = fmaxf(r0 * (1.0f - gp * g), r_min);
+  } else {
+    const float center = 0.5f * (fl + fh);
+    if (f < center)
+      out = fminf(r0 * (1.0f + gp * idle), r_max);
+    else
+      out = fmaxf(r0 * (1.0f - gp * idle), r_min);
+  }
+  rt[i] = out;
+}
+
+namespace fos {
+int gate_targets(const float *rad, float *rad_target, const int32_t *fsc, const float *fsc_ema,
+                 int n, int frame, int warmstart, double zero_rate, const int *zero_cnt,
+                 int f_low, int f_high, float growth_pct, float r_min, float r_max,
+                 float deadband, float *gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_gate_targets, dim3(n > 0 ? blocks_for(n) : 1), dim3(FOS_BLOCK), 0, st,
+                     rad, rad_target, fsc, fsc_ema, n, frame, warmstart, zero_rate, zero_cnt,
+                     (float)f_low, (float)f_high, growth_pct, r_min, r_max, deadband, gate);
+  FOS_CHECK_LAUNCH("k_gate_targets");
+  return 0;
+}
+}  // namespace fos
+
 // dynamics.py:1399-1427
 __device__ __forceinline__ void nudge_one(int i, float *__restrict__ rad,
                                           const float *__restrict__ rt,

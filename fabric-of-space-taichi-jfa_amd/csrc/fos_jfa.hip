@@ -8,6 +8,10 @@
 namespace fos {
 int fill_u32(unsigned *p, int64_t n, unsigned v, hipStream_t st);
 int alloc_tables(fos_ctx *ctx, int cap);
+int gate_targets(const float *rad, float *rad_target, const int32_t *fsc, const float *fsc_ema,
+                 int n, int frame, int warmstart, double zero_rate, const int *zero_cnt,
+                 int f_low, int f_high, float growth_pct, float r_min, float r_max,
+                 float deadband, float *gate, hipStream_t st);
 }  // namespace fos
 
 using namespace fos;
@@ -768,10 +772,11 @@ __global__ void __launch_bounds__(256)
                    int32_t *__restrict__ face_counts, int *__restrict__ in_count,
                    int *__restrict__ in_fill, unsigned *__restrict__ dec,
                    int *__restrict__ chg, int nchg, int *__restrict__ dirty0,
-                   int *__restrict__ dirty1) {
+                   int *__restrict__ dirty1, int *__restrict__ zero_cnt) {
   extern __shared__ unsigned s_keys[];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + wid;
+  if (zero_cnt && blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
   if (in_count) {
     if (row < n && lane == 0) {
       in_count[row] = 0;
@@ -920,12 +925,12 @@ __global__ void k_fin_rank(int n, const int32_t *__restrict__ face_ids,
 // packed as one bit per slot.
 // 8 lanes per row gather the accepted-incoming count (an order-free sum);
 // lane 0 then decides the row.
-// Round pairs keep one change flag each: both rounds of pair k set chg_cur
-// when a row's decisions change.  chg_prev (may be NULL), the previous pair's
-// flag, still clear means that pair changed nothing -- the fixed point -- and
-// the round does nothing (so its own flag stays clear and the skip cascades).
-// failed (may be NULL; given to the second round of a last pair) is set when
-// that pair changed anything.
+// Each round keeps one change flag: the round sets chg_cur when a row's
+// decisions change.  chg_prev (may be NULL), the previous round's flag,
+// still clear means that round changed nothing -- the fixed point -- and the
+// round does nothing (so its own flag stays clear and the skip cascades).
+// failed (may be NULL; given to the last round of a fixed count) is set when
+// that round changed anything.
 // Dirty rows: a row's decisions depend only on the bits of its incoming
 // slots, so a row none of whose incoming bits changed in the previous round
 // keeps its decisions.  The previous round marked the targets of the bits it
@@ -1059,7 +1064,9 @@ __global__ void __launch_bounds__(256)
                       const unsigned *__restrict__ dec, const int *__restrict__ in_start,
                       const int *__restrict__ in_sorted, float *__restrict__ prev,
                       int32_t *__restrict__ nl, int32_t *__restrict__ nc,
-                      int32_t *__restrict__ used_count, int32_t *__restrict__ fsc) {
+                      int32_t *__restrict__ used_count, int32_t *__restrict__ fsc,
+                      float *__restrict__ ema, float alpha, int *__restrict__ zero_cnt) {
+  __shared__ int s_zero;
   __shared__ int s_time[8][FOS_MAX_NEIGHBORS];
   __shared__ float s_val[8][FOS_MAX_NEIGHBORS];
   __shared__ int s_id[8][FOS_MAX_NEIGHBORS];
@@ -1070,6 +1077,7 @@ __global__ void __launch_bounds__(256)
   const bool live = i < n;
   const int64_t base = (int64_t)(live ? i : 0) * FOS_MAX_NEIGHBORS;
   s_time[r][s] = -1;
+  if (threadIdx.x == 0) s_zero = 0;
   __syncthreads();
   // appends from lower rows, in incoming order
   int u = 0;
@@ -1106,19 +1114,30 @@ __global__ void __launch_bounds__(256)
     s_id[r][rank] = face_ids[base + s];
   }
   __syncthreads();
-  if (!live) return;
-  const int total = u_in + __popc(am);
-  const int used = total < FOS_MAX_NEIGHBORS ? total : FOS_MAX_NEIGHBORS;
-  const int ta = s_time[r][s], tr = rej ? s : -1;
-  if (ta >= 0 || tr >= 0)
-    prev[base + s] = ta > tr ? s_val[r][s] : 0.0f;
-  else if (s < u_in)
-    prev[base + s] = s_val[r][s];
-  nl[base + s] = s < used ? s_id[r][s] : -1;
-  if (s == 0) {
-    nc[i] = used;
-    used_count[i] = used;
-    fsc[i] = used;
+  if (live) {
+    const int total = u_in + __popc(am);
+    const int used = total < FOS_MAX_NEIGHBORS ? total : FOS_MAX_NEIGHBORS;
+    const int ta = s_time[r][s], tr = rej ? s : -1;
+    if (ta >= 0 || tr >= 0)
+      prev[base + s] = ta > tr ? s_val[r][s] : 0.0f;
+    else if (s < u_in)
+      prev[base + s] = s_val[r][s];
+    nl[base + s] = s < used ? s_id[r][s] : -1;
+    if (s == 0) {
+      nc[i] = used;
+      used_count[i] = used;
+      fsc[i] = used;
+      // jfa.py:766-785 update_fsc_ema on the row's new FSC
+      if (ema) {
+        const float e = ema[i];
+        ema[i] = (e == 0.0f && used > 0) ? (float)used : alpha * (float)used + (1.0f - alpha) * e;
+      }
+      if (zero_cnt && used == 0) atomicAdd(&s_zero, 1);
+    }
+  }
+  if (zero_cnt) {
+    __syncthreads();
+    if (threadIdx.x == 0 && s_zero) atomicAdd(zero_cnt, s_zero);
   }
 }
 
@@ -1475,13 +1494,13 @@ static int *fin_dirty(fos_ctx *ctx, int which);
 
 // clear_fin: also clear the finalize accumulators (a finalize follows).
 static int face_compact(fos_ctx *ctx, int n, int32_t *face_ids, int32_t *face_counts,
-                        hipStream_t st, bool clear_fin = false) {
+                        hipStream_t st, bool clear_fin = false, int *zero_cnt = nullptr) {
   const int cap = ctx->tab_cap;
   const size_t shmem = (size_t)4 * cap * sizeof(unsigned);
   hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st, n,
                      ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts,
                      clear_fin ? ctx->in_count : nullptr, ctx->in_fill, (unsigned *)ctx->dec,
-                     fin_chg(ctx), FIN_RING, fin_dirty(ctx, 0), fin_dirty(ctx, 1));
+                     fin_chg(ctx), FIN_RING, fin_dirty(ctx, 0), fin_dirty(ctx, 1), zero_cnt);
   FOS_CHECK_LAUNCH("k_face_compact");
   return 0;
 }
@@ -1715,8 +1734,9 @@ int fos_finalize_faces(fos_ctx *ctx, int n, const float *rad, float voxel,
 
 }  // extern "C"
 
-// Per-pair change flags of the fixed point: a ring of FIN_RING at
-// ctx->flags + 16.
+// Per-round change flags of the fixed point: a ring of FIN_RING at
+// ctx->flags + 16.  FIN_BURST: rounds queued per host read after the first
+// burst.
 #define FIN_BURST 4
 static int *fin_chg(fos_ctx *ctx) { return ctx->flags + 16; }
 
@@ -1751,38 +1771,38 @@ static int fin_prepare(fos_ctx *ctx, int n, const float *rad, float voxel, const
   return 0;
 }
 
-// Round pairs k0 .. k0 + pairs - 1 with no host round trip: rounds alternate
-// between two packed decision buffers (the result is back in the first after
-// a pair).  Pair k's change flag is fin_chg(ctx)[k % FIN_RING]; a pair after
-// one that changed nothing skips.  With `last_failed`, the last pair marks
-// ctx->flags[3] (sticky) when it changed anything.  Flags are cleared by
-// fin_prepare; a caller going past FIN_RING pairs clears the ring itself
-// (fin_ring_clear).
+// Rounds r0 .. r0 + rounds - 1 with no host round trip.  Round r reads the
+// decision buffer r & 1 (a, b, a, ...) and writes the other; its change
+// flag is fin_chg(ctx)[r % FIN_RING], and a round after one that changed
+// nothing returns at once.  Any fixed point of the round map is the serial
+// answer (a row's decisions depend only on lower rows', so by induction over
+// rows a fixed point agrees with the serial loop), so the first round that
+// changes nothing ends the iteration, and it leaves both buffers equal.
+// With `last_failed`, the last round marks ctx->flags[3] (sticky) when it
+// changed anything.  Flags are cleared by fin_prepare; a caller going past
+// FIN_RING rounds clears the ring itself (fin_ring_clear).
 // Dirty-row marks of the rounds: two int arrays of max_n in the decision
-// scratch after the two decision buffers; round r reads (and clears) array
-// r & 1 and marks array (r + 1) & 1.
+// scratch after the two decision buffers; round r >= 2 reads (and clears)
+// array r & 1 and marks array (r + 1) & 1.
 static int *fin_dirty(fos_ctx *ctx, int which) {
   return (int *)ctx->dec + (size_t)(2 + which) * ctx->max_n;
 }
 
-static int fin_pairs(fos_ctx *ctx, int n, int k0, int pairs, bool last_failed,
-                     const int32_t *face_ids, hipStream_t st) {
-  unsigned *dec_a = (unsigned *)ctx->dec, *dec_b = dec_a + ctx->max_n;
+static int fin_rounds(fos_ctx *ctx, int n, int r0, int rounds, bool last_failed,
+                      const int32_t *face_ids, hipStream_t st) {
+  unsigned *dec[2] = {(unsigned *)ctx->dec, (unsigned *)ctx->dec + ctx->max_n};
   int *chg = fin_chg(ctx);
-  int *d0 = fin_dirty(ctx, 0), *d1 = fin_dirty(ctx, 1);
-  for (int k = k0; k < k0 + pairs; ++k) {
-    const int *prev = k > 0 ? chg + (k - 1) % FIN_RING : nullptr;
-    int *cur = chg + k % FIN_RING;
-    int *failed = last_failed && k == k0 + pairs - 1 ? ctx->flags + 3 : nullptr;
-    // round 2k reads array 0 and marks array 1; round 2k + 1 reads array 1
-    // and marks array 0.  Round 0 changes nearly every row (from nothing
-    // accepted), so it marks nothing and round 1 recomputes every row.
+  int *d[2] = {fin_dirty(ctx, 0), fin_dirty(ctx, 1)};
+  for (int r = r0; r < r0 + rounds; ++r) {
+    const int *prev = r > 0 ? chg + (r - 1) % FIN_RING : nullptr;
+    int *cur = chg + r % FIN_RING;
+    int *failed = last_failed && r == r0 + rounds - 1 ? ctx->flags + 3 : nullptr;
+    // round 0 changes nearly every row (from nothing accepted), so it marks
+    // nothing and round 1 recomputes every row
     hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_a, dec_b, prev, cur,
-                       (int *)nullptr, face_ids, k > 0 ? d0 : nullptr, k > 0 ? d1 : nullptr);
-    hipLaunchKernelGGL(k_fin_round, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, st, n,
-                       ctx->cls, ctx->in_start, ctx->in_sorted, dec_b, dec_a, prev, cur, failed,
-                       face_ids, k > 0 ? d1 : nullptr, d0);
+                       ctx->cls, ctx->in_start, ctx->in_sorted, dec[r & 1], dec[(r + 1) & 1],
+                       prev, cur, failed, face_ids, r >= 2 ? d[r & 1] : nullptr,
+                       r >= 1 ? d[(r + 1) & 1] : nullptr);
     FOS_CHECK_LAUNCH("k_fin_round");
   }
   return 0;
@@ -1797,12 +1817,16 @@ static int fin_ring_clear(fos_ctx *ctx, int keep, hipStream_t st) {
   return rc;
 }
 
+// ema (may be NULL): also the FSC EMA update with alpha; zero_cnt (may be
+// NULL, zero before): also counts the rows whose FSC is 0.
 static int fin_materialize(fos_ctx *ctx, int n, const int32_t *face_ids, float *prev_scores,
                            int32_t *neighbor_list, int32_t *neighbor_count, int32_t *used_count,
-                           int32_t *fsc, hipStream_t st) {
+                           int32_t *fsc, hipStream_t st, float *ema = nullptr, float alpha = 0.0f,
+                           int *zero_cnt = nullptr) {
   hipLaunchKernelGGL(k_fin_materialize, dim3((unsigned)((n + 7) / 8)), dim3(256), 0, st, n,
                      face_ids, ctx->score, ctx->cls, (const unsigned *)ctx->dec, ctx->in_start,
-                     ctx->in_sorted, prev_scores, neighbor_list, neighbor_count, used_count, fsc);
+                     ctx->in_sorted, prev_scores, neighbor_list, neighbor_count, used_count, fsc,
+                     ema, alpha, zero_cnt);
   FOS_CHECK_LAUNCH("k_fin_materialize");
   return 0;
 }
@@ -1817,56 +1841,56 @@ static int read_flags(fos_ctx *ctx, int out[64], hipStream_t st) {
   return 0;
 }
 
-// Pairs up to the first that changed nothing, from flags read after pair
+// Rounds up to the first that changed nothing, from flags read after round
 // `last` (the ring has not wrapped); FIN_BURST when it has.
 static int fin_needed(const int *f, int last) {
   if (last >= FIN_RING) return FIN_BURST;
-  for (int k = 0; k <= last; ++k)
-    if (!f[16 + k]) return k + 1;
+  for (int r = 0; r <= last; ++r)
+    if (!f[16 + r]) return r + 1;
   return FIN_BURST;
 }
 
 // The first burst's size: what the last finalize needed (consecutive frames
-// need about the same), 2 to 8.
+// need about the same), 2 to 16 rounds.
 static int fin_first_burst(const fos_ctx *ctx) {
   const int b = ctx->fin_burst > 0 ? ctx->fin_burst : FIN_BURST;
-  return b < 2 ? 2 : b > 8 ? 8 : b;
+  return b < 2 ? 2 : b > 16 ? 16 : b;
 }
 
-// Bursts of round pairs from pair k0 until the fixed point (a burst whose
-// last pair changed nothing), the host reading the flags after each burst:
-// the first of `first` pairs, then FIN_BURST.  `known`: the flags of a burst
-// already read (then k0 is the pair after it); NULL starts with a burst.
-// Records the pairs needed in ctx->fin_burst.
-static int fin_pairs_host(fos_ctx *ctx, int n, int k0, int first, const int *known,
-                          const int32_t *face_ids, hipStream_t st) {
+// Bursts of rounds from round r0 until the fixed point (a burst whose last
+// round changed nothing), the host reading the flags after each burst: the
+// first of `first` rounds, then FIN_BURST.  `known`: the flags of a burst
+// already read (then r0 is the round after it); NULL starts with a burst.
+// Records the rounds needed in ctx->fin_burst.
+static int fin_rounds_host(fos_ctx *ctx, int n, int r0, int first, const int *known,
+                           const int32_t *face_ids, hipStream_t st) {
   int f[64];
-  if (known && !known[16 + (k0 - 1) % FIN_RING]) {
-    ctx->fin_burst = fin_needed(known, k0 - 1);
+  if (known && !known[16 + (r0 - 1) % FIN_RING]) {
+    ctx->fin_burst = fin_needed(known, r0 - 1);
     return 0;
   }
-  // pairs below lap_end have ring slots not used since fin_prepare cleared them
+  // rounds below lap_end have ring slots not used since fin_prepare cleared them
   int lap_end = FIN_RING;
-  for (int k = k0, p = first;; k += p, p = FIN_BURST) {
-    FOS_REQUIRE(k <= n + 1, "finalize: no fixed point after %d round pairs", k);
+  for (int r = r0, p = first;; r += p, p = FIN_BURST) {
+    FOS_REQUIRE(r <= 2 * n + 2, "finalize: no fixed point after %d rounds", r);
     int rc = 0;
-    if (k + p > lap_end) {
-      rc = fin_ring_clear(ctx, (k - 1) % FIN_RING, st);
-      lap_end = k + FIN_RING - 1;
+    if (r + p > lap_end) {
+      rc = fin_ring_clear(ctx, (r - 1) % FIN_RING, st);
+      lap_end = r + FIN_RING - 1;
     }
-    if (!rc) rc = fin_pairs(ctx, n, k, p, false, face_ids, st);
+    if (!rc) rc = fin_rounds(ctx, n, r, p, false, face_ids, st);
     if (!rc) rc = read_flags(ctx, f, st);
     if (rc) return rc;
-    if (!f[16 + (k + p - 1) % FIN_RING]) {
-      ctx->fin_burst = fin_needed(f, k + p - 1);
+    if (!f[16 + (r + p - 1) % FIN_RING]) {
+      ctx->fin_burst = fin_needed(f, r + p - 1);
       return 0;
     }
   }
 }
 
-// fixed_pairs > 0: that many round pairs with no host round trip (flags[3]
-// stays set if the last pair had not reached the fixed point); 0: until the
-// fixed point (fin_pairs_host).
+// fixed_pairs > 0: that many round pairs (2 rounds each) with no host round
+// trip (flags[3] stays set if the last round had not reached the fixed
+// point); 0: until the fixed point (fin_rounds_host).
 static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, const float *pvox,
                          const int32_t *face_ids, const int32_t *face_counts,
                          float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
@@ -1876,11 +1900,11 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
   int rc = fin_prepare(ctx, n, rad, voxel, pvox, face_ids, face_counts, prev_scores, st);
   if (rc) return rc;
   if (fixed_pairs > 0) {
-    FOS_REQUIRE(fixed_pairs <= FIN_RING, "finalize: fixed_pairs %d above %d", fixed_pairs,
-                FIN_RING);
-    rc = fin_pairs(ctx, n, 0, fixed_pairs, true, face_ids, st);
+    FOS_REQUIRE(2 * fixed_pairs <= FIN_RING, "finalize: fixed_pairs %d above %d", fixed_pairs,
+                FIN_RING / 2);
+    rc = fin_rounds(ctx, n, 0, 2 * fixed_pairs, true, face_ids, st);
   } else {
-    rc = fin_pairs_host(ctx, n, 0, fin_first_burst(ctx), nullptr, face_ids, st);
+    rc = fin_rounds_host(ctx, n, 0, fin_first_burst(ctx), nullptr, face_ids, st);
   }
   if (rc) return rc;
   return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
@@ -1888,6 +1912,42 @@ static int finalize_impl(fos_ctx *ctx, int n, const float *rad, float voxel, con
 }
 
 extern "C" {
+
+// fos_jfa_faces, and with ema: the FSC EMA in the materialize launch and the
+// zero-FSC count the warm-start gate needs (ctx->flags[6]).
+static int jfa_faces_impl(fos_ctx *ctx, int res, int n, const int32_t *filtered, const float *rad,
+                          float voxel, int32_t *face_ids, int32_t *face_counts,
+                          float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,
+                          int32_t *used_count, int32_t *fsc, float *ema, float alpha,
+                          hipStream_t st) {
+  int *zero_cnt = ema ? ctx->flags + 6 : nullptr;
+  // Emission, compaction, finalize preparation and a burst of round pairs
+  // are queued together; one host read then sees both the row-table
+  // overflow flag and the fixed-point flag.  An overflow regrows the tables
+  // and starts over (nothing the caller keeps has been written yet: the
+  // neighbour lists and hysteresis scores are written by materialize).
+  while (true) {
+    int rc = face_emit(ctx, res, n, filtered, st);
+    if (!rc) rc = face_compact(ctx, n, face_ids, face_counts, st, true, zero_cnt);
+    if (!rc) rc = fin_prepare(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores, st,
+                              true);
+    const int first = fin_first_burst(ctx);
+    if (!rc) rc = fin_rounds(ctx, n, 0, first, false, face_ids, st);
+    if (rc) return rc;
+    int f[64];
+    rc = read_flags(ctx, f, st);
+    if (rc) return rc;
+    if (f[0]) {
+      rc = face_grow(ctx, st);
+      if (rc) return rc;
+      continue;
+    }
+    rc = fin_rounds_host(ctx, n, first, FIN_BURST, f, face_ids, st);
+    if (rc) return rc;
+    return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
+                           used_count, fsc, st, ema, alpha, zero_cnt);
+  }
+}
 
 int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const float *rad,
                   float voxel, int32_t *face_ids, int32_t *face_counts, float *prev_scores,
@@ -1901,33 +1961,33 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
   FOS_REQUIRE((double)res * res * res * 6.0 < 4294967295.0,
               "fos_jfa_faces: res=%d exceeds the 32-bit face key", res);
   if (n == 0) return 0;
+  return jfa_faces_impl(ctx, res, n, filtered, rad, voxel, face_ids, face_counts, prev_scores,
+                        neighbor_list, neighbor_count, used_count, fsc, nullptr, 0.0f,
+                        (hipStream_t)stream);
+}
+
+int fos_jfa_faces_controller(fos_ctx *ctx, int res, int n, const int32_t *filtered,
+                             float *rad_target, const float *rad, float voxel, int32_t *face_ids,
+                             int32_t *face_counts, float *prev_scores, int32_t *neighbor_list,
+                             int32_t *neighbor_count, int32_t *used_count, int32_t *fsc,
+                             float *fsc_ema, float alpha, int frame, int warmstart_frames,
+                             double fsc_zero_rate, int f_low, int f_high, float growth_pct,
+                             float r_min, float r_max, float deadband, float *gate,
+                             void *stream) {
+  FOS_REQUIRE(ctx && filtered && face_ids && face_counts && res >= 1 && res <= ctx->max_res &&
+                  n >= 1 && n <= ctx->max_n && voxel > 0.0f && rad && rad_target &&
+                  prev_scores && neighbor_list && neighbor_count && used_count && fsc &&
+                  fsc_ema && gate,
+              "fos_jfa_faces_controller: bad arguments (n=%d res=%d)", n, res);
+  FOS_REQUIRE((double)res * res * res * 6.0 < 4294967295.0,
+              "fos_jfa_faces_controller: res=%d exceeds the 32-bit face key", res);
   hipStream_t st = (hipStream_t)stream;
-  // Emission, compaction, finalize preparation and a burst of round pairs
-  // are queued together; one host read then sees both the row-table
-  // overflow flag and the fixed-point flag.  An overflow regrows the tables
-  // and starts over (nothing the caller keeps has been written yet: the
-  // neighbour lists and hysteresis scores are written by materialize).
-  while (true) {
-    int rc = face_emit(ctx, res, n, filtered, st);
-    if (!rc) rc = face_compact(ctx, n, face_ids, face_counts, st, true);
-    if (!rc) rc = fin_prepare(ctx, n, rad, voxel, nullptr, face_ids, face_counts, prev_scores, st,
-                              true);
-    const int first = fin_first_burst(ctx);
-    if (!rc) rc = fin_pairs(ctx, n, 0, first, false, face_ids, st);
-    if (rc) return rc;
-    int f[64];
-    rc = read_flags(ctx, f, st);
-    if (rc) return rc;
-    if (f[0]) {
-      rc = face_grow(ctx, st);
-      if (rc) return rc;
-      continue;
-    }
-    rc = fin_pairs_host(ctx, n, first, FIN_BURST, f, face_ids, st);
-    if (rc) return rc;
-    return fin_materialize(ctx, n, face_ids, prev_scores, neighbor_list, neighbor_count,
-                           used_count, fsc, st);
-  }
+  int rc = jfa_faces_impl(ctx, res, n, filtered, rad, voxel, face_ids, face_counts, prev_scores,
+                          neighbor_list, neighbor_count, used_count, fsc, fsc_ema, alpha, st);
+  if (rc) return rc;
+  return fos::gate_targets(rad, rad_target, fsc, fsc_ema, n, frame, warmstart_frames,
+                           fsc_zero_rate, ctx->flags + 6, f_low, f_high, growth_pct, r_min, r_max,
+                           deadband, gate, st);
 }
 
 }  // extern "C"
@@ -2360,7 +2420,7 @@ int fos_batch_face_accumulate(fos_batch *bt, fos_ctx *ctx, const int32_t *filter
   hipLaunchKernelGGL(k_face_compact, dim3((unsigned)((n + 3) / 4)), dim3(256), shmem, st, n,
                      ctx->tab_b, ctx->tab_k, ctx->tab_c, cap, face_ids, face_counts,
                      (int *)nullptr, (int *)nullptr, (unsigned *)nullptr, (int *)nullptr, 0,
-                     (int *)nullptr, (int *)nullptr);
+                     (int *)nullptr, (int *)nullptr, (int *)nullptr);
   FOS_CHECK_LAUNCH("k_face_compact");
   return 0;
 }

@@ -28,6 +28,7 @@ neither the overlap maximum nor the gate on the host) are queued before the
 host waits for the overlap maximum, behind an event rather than a drain of
 the stream: the GPU works through them while the host reads.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -65,9 +66,12 @@ class FoamSim:
         n = int(cfg.n)
         dev = torch.device(device)
         self.n = n
-        self.pos = torch.as_tensor(np.asarray(pos, np.float32).reshape(n, 3), device=dev).contiguous()
+        p0 = torch.as_tensor(np.asarray(pos, np.float32).reshape(n, 3), device=dev).contiguous()
+        # positions ping-pong between two buffers (PBD reads one, writes the
+        # other); _cur says which holds the positions
+        self._pair = [p0, torch.empty_like(p0)]
+        self._cur = 0
         self.rad = torch.as_tensor(np.asarray(rad, np.float32).reshape(n), device=dev).contiguous()
-        self.pos_tmp = torch.empty_like(self.pos)
         self.rad_target = self.rad.clone()
         self.lmd = torch.zeros(n, dtype=torch.float32, device=dev)
         self.delta_r = torch.zeros(n, dtype=torch.float32, device=dev)
@@ -128,6 +132,27 @@ class FoamSim:
         self.fallback = False
         self.judged = []          # (frame, fsc error, mean-FSC error) per judge frame
         self.pred_fsc = torch.zeros(n, dtype=torch.int32, device=dev)
+        # the plain-JFA frame runs as one library call (fos_frame_step) unless
+        # native is False or a predictor or an on_jfa hook is set
+        self.native = True
+        self._fdesc = self._fbufs = None
+        self._fstate = fl.FrameState()
+
+    @property
+    def pos(self):
+        return self._pair[self._cur]
+
+    @property
+    def pos_tmp(self):
+        return self._pair[1 - self._cur]
+
+    def __del__(self):
+        st = getattr(self, "_fstate", None)
+        if st is not None and st.event and fl._lib is not None:
+            try:
+                fl._lib.fos_frame_release(ctypes.byref(st))
+            except Exception:
+                pass
 
     def set_predictor(self, params, mult=None):
         """Enables the predictor path with fp32 MLP parameters (numpy) and the
@@ -175,7 +200,7 @@ class FoamSim:
             fl.call("fos_pbd_pass_cells", self.ctx.handle, *args[:-1], pr, args[-1])
         else:
             fl.call("fos_pbd_pass", *args)
-        self.pos, self.pos_tmp = self.pos_tmp, self.pos
+        self._cur = 1 - self._cur
 
     def rebuild_from_cells(self, pr=False):
         rec = pr and self.slot_records
@@ -233,7 +258,76 @@ class FoamSim:
                 c.growth_pct, c.r_min, c.r_max, c.deadband, gate, st)
 
     # -- one frame ------------------------------------------------------------
+    def _frame_desc(self):
+        """fos_frame_desc from the configuration, with the values the Python
+        frame passes (fp32 where it passes fp32, double where Python floats
+        enter double arithmetic)."""
+        c = self.cfg
+        return fl.FrameDesc(
+            n=self.n, res=self.res, G=self.G, jfa_passes=c.passes(), L=c.L, voxel=c.voxel(),
+            beta=c.beta, ema_alpha=c.ema_alpha, early_exit=int(c.early_exit),
+            min_passes=c.min_passes, th_pass_delta=c.th_pass_delta, pbd_base=c.pbd_base,
+            pbd_max=c.pbd_max, pbd_scale=float(c.pbd_scale), r_max_d=float(c.r_max),
+            gap_factor=c.gap_factor(), max_disp_frac=c.max_disp_frac, disp_mult=c.disp_mult,
+            alpha=c.alpha(), max_step_pct=c.max_step_pct, r_min=c.r_min, r_max=c.r_max,
+            contact_tol=c.contact_tol, contact_tol_d=float(c.contact_tol),
+            backpressure_mode=c.backpressure_mode, adjustment_frames=c.adjustment_frames,
+            warmstart_frames=c.warmstart_frames, fsc_zero_rate=c.fsc_zero_rate, f_low=c.f_low,
+            f_high=c.f_high, growth_pct=c.growth_pct, deadband=c.deadband,
+            pressure=int(c.pressure), pressure_iters=c.pressure_iters,
+            pressure_max_nei=c.pressure_max_nei, pressure_k=c.pressure_k,
+            pressure_exp=c.pressure_exp, pressure_cap=c.pressure_cap,
+            brownian=int(c.brownian), brownian_mode=c.brownian_mode,
+            brownian_strength=c.brownian_strength, seed=c.seed, ou_jitter=int(c.ou_jitter),
+            dt=fc.DT, jitter_rms=fc.JITTER_RMS, jitter_tau=fc.JITTER_TAU,
+            drift_cap_frac=fc.f32(fc.MAX_DRIFT_FRACTION * c.gap_fraction),
+            jfa_every=c.jfa_every, jfa_warm=c.jfa_warm, slot_records=int(self.slot_records))
+
+    def _frame_bufs(self):
+        b = fl.FrameBufs()
+        b.pos[0], b.pos[1] = self._pair[0].data_ptr(), self._pair[1].data_ptr()
+        for k in ("rad", "rad_target", "lmd", "delta_r", "v_jit", "cell_count", "cell_start",
+                  "cell_indices", "cell_pr", "grid", "temp", "filtered", "face_ids",
+                  "face_counts", "fsc", "fsc_ema", "neighbor_list", "neighbor_count",
+                  "used_count", "prev_scores", "info", "maxd", "maxd_host", "settle_dev"):
+            setattr(b, k, getattr(self, k).data_ptr())
+        return b
+
+    def _step_native(self):
+        """step() as one fos_frame_step call (the same calls in the same
+        order, issued from C++)."""
+        c, f = self.cfg, self.frame
+        s = self._fstate
+        if self._fdesc is None:
+            self._fdesc, self._fbufs = self._frame_desc(), self._frame_bufs()
+        s.frame, s.pos_cur, s.sidx = f, self._cur, self.sidx
+        s.settle_force, s.settle_gate = int(self.settle_force), int(self.settle_gate)
+        s.need_rebuild = int(self.cfg.L != 1.0 or self.grid_key != self.grid_token())
+        s.pending = int(self.pending is not None)
+        ev = self.pass_events.pop(0) if self.pass_events else None
+        fl.call("fos_frame_step", self.ctx.handle, ctypes.byref(self._fdesc),
+                ctypes.byref(self._fbufs), ctypes.byref(s), ev, fl.stream())
+        if s.gate_read:
+            self.resolve(float(s.gate))
+        self._cur, self.sidx = s.pos_cur, s.sidx
+        self.settle_force, self.settle_gate = bool(s.settle_force), bool(s.settle_gate)
+        self.grid_key = self.grid_token()
+        ran = bool(s.jfa_ran)
+        self.last = FrameInfo(self, frame=f, pbd_passes=int(s.pbd_passes),
+                              max_depth=float(s.max_depth), jfa=ran,
+                              growth=None if ran else True)
+        if ran:
+            self.pending = self.last
+        else:
+            self.growth = True
+        if self.telemetry and ran:
+            self.last.update(self.jfa_telemetry())
+        self.frame += 1
+        return self.last
+
     def step(self):
+        if self.native and self.predictor is None and self.on_jfa is None:
+            return self._step_native()
         c = self.cfg
         f = self.frame
         st = fl.stream()

@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfos.so")
 OBJ = os.path.join(HERE, "_build")
 SOURCES = ["fos_ctx.hip", "fos_jfa.hip", "fos_grid.hip", "fos_dynamics.hip", "fos_topology.hip",
-           "fos_predict.hip", "fos_batch.hip", "fos_static.hip"]
+           "fos_predict.hip", "fos_batch.hip", "fos_static.hip",
+           "fos_frame.hip"]
 ARCH = os.environ.get("FOS_OFFLOAD_ARCH", "gfx950")
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

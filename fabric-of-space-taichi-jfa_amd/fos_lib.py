@@ -79,6 +79,8 @@ SIGNATURES = {
     "fos_grid_build_from_cells": [P, P, P, I, I, P, P, P, P, P],
     "fos_grid_build_pr": [P, P, P, I, I, F, P, P, P, P, P],
     "fos_jfa_faces": [P, I, I, P, P, F, P, P, P, P, P, P, P, P],
+    "fos_jfa_faces_controller": [P, I, I, P, P, P, F, P, P, P, P, P, P, P, P, F, I, I, D, I, I,
+                                 F, F, F, F, P, P],
     "fos_fsc_targets_gated": [P, P, P, P, I, I, I, F, F, F, F, P, P],
     "fos_fsc_gate": [P, P, I, I, I, D, P, P],
     "fos_batch_create": [P, I, P, P, P, P, P, P, P, P, P, F],
@@ -111,7 +113,46 @@ SIGNATURES = {
     "fos_pred_labels": [P, P, P, I, I, I64, P, P, P, P],
     "fos_mlp_param_count": [ctypes.POINTER(I)],
     "fos_mlp_train_step": [P, P, P, P, I, F, P, P, P, I, F, P, P, P],
+    "fos_frame_step": [P, P, P, P, P, P],
+    "fos_frame_release": [P],
 }
+
+
+class FrameDesc(ctypes.Structure):
+    """fos_frame_desc (include/fos.h): the frame's configuration."""
+    _fields_ = [("n", I), ("res", I), ("G", I), ("jfa_passes", I),
+                ("L", F), ("voxel", F), ("beta", F), ("ema_alpha", F),
+                ("early_exit", I), ("min_passes", I), ("th_pass_delta", D),
+                ("pbd_base", I), ("pbd_max", I), ("pbd_scale", D), ("r_max_d", D),
+                ("gap_factor", F), ("max_disp_frac", F), ("disp_mult", F),
+                ("alpha", F), ("max_step_pct", F), ("r_min", F), ("r_max", F),
+                ("contact_tol", F), ("contact_tol_d", D),
+                ("backpressure_mode", I), ("adjustment_frames", I), ("warmstart_frames", I),
+                ("fsc_zero_rate", D), ("f_low", I), ("f_high", I),
+                ("growth_pct", F), ("deadband", F),
+                ("pressure", I), ("pressure_iters", I), ("pressure_max_nei", I),
+                ("pressure_k", F), ("pressure_exp", F), ("pressure_cap", F),
+                ("brownian", I), ("brownian_mode", I), ("brownian_strength", F),
+                ("seed", U64), ("ou_jitter", I),
+                ("dt", F), ("jitter_rms", F), ("jitter_tau", F), ("drift_cap_frac", F),
+                ("jfa_every", I), ("jfa_warm", I), ("slot_records", I)]
+
+
+class FrameBufs(ctypes.Structure):
+    """fos_frame_bufs (include/fos.h): device pointers of the frame's arrays."""
+    _fields_ = [("pos", P * 2)] + [(k, P) for k in (
+        "rad", "rad_target", "lmd", "delta_r", "v_jit", "cell_count", "cell_start",
+        "cell_indices", "cell_pr", "grid", "temp", "filtered", "face_ids", "face_counts", "fsc",
+        "fsc_ema", "neighbor_list", "neighbor_count", "used_count", "prev_scores", "info", "maxd",
+        "maxd_host", "settle_dev")]
+
+
+class FrameState(ctypes.Structure):
+    """fos_frame_state (include/fos.h): what carries from frame to frame."""
+    _fields_ = [("frame", I), ("pos_cur", I), ("sidx", I), ("settle_force", I),
+                ("settle_gate", I), ("need_rebuild", I), ("pending", I),
+                ("pbd_passes", I), ("jfa_ran", I), ("gate_read", I),
+                ("max_depth", F), ("gate", F), ("event", P)]
 
 _lib = None
 

@@ -84,6 +84,20 @@ int fos_jfa_faces(fos_ctx *ctx, int res, int n, const int32_t *filtered, const f
                   float voxel, int32_t *face_ids, int32_t *face_counts, float *prev_scores,
                   int32_t *neighbor_list, int32_t *neighbor_count, int32_t *used_count,
                   int32_t *fsc, void *stream);
+/* The tail of a JFA frame in fewer launches: fos_jfa_faces, then
+ * fos_fsc_ema (jfa.py:766-785, inside the finalize's last launch), then
+ * fos_fsc_gate + fos_fsc_targets_gated (run.py:1184-1231) as one launch;
+ * *gate (device float) receives the warm-start gate (1.0 = growth).  Results
+ * equal the four calls bit for bit.  No reference counterpart (launches
+ * saved per frame). */
+int fos_jfa_faces_controller(fos_ctx *ctx, int res, int n, const int32_t *filtered,
+                             float *rad_target, const float *rad, float voxel, int32_t *face_ids,
+                             int32_t *face_counts, float *prev_scores, int32_t *neighbor_list,
+                             int32_t *neighbor_count, int32_t *used_count, int32_t *fsc,
+                             float *fsc_ema, float alpha, int frame, int warmstart_frames,
+                             double fsc_zero_rate, int f_low, int f_high, float growth_pct,
+                             float r_min, float r_max, float deadband, float *gate,
+                             void *stream);
 /* Per-frame seed table (x, y, z, (beta*r)^2) that the JFA kernels gather;
  * the reference recomputes the same terms per candidate (jfa.py:470-477). */
 int fos_jfa_seed_table(fos_ctx *ctx, const float *pos, const float *rad,
@@ -350,6 +364,68 @@ int fos_xsph(const float *pos, float *vel, float *vel_tmp, int n, int G,
 /* dynamics.py:473-480 apply_global_damping. */
 int fos_damp(float *vel, int n, float factor, void *stream);
 
+/* ---- one whole frame ----------------------------------------------------- */
+
+/* The frame of run.py:700-1261 (plain JFA path) as one call: what
+ * foam.FoamSim.step() issues, in the same order, with the same host reads.
+ * fos_frame_desc holds the configuration, fos_frame_bufs the caller-owned
+ * device arrays (pos[2] is a ping-pong pair: state.pos_cur says which holds
+ * the positions; maxd_host is pinned host memory of 2 floats), and
+ * fos_frame_state what carries from frame to frame (the caller zeroes it
+ * once, sets need_rebuild when it wrote the positions since the last step,
+ * and calls fos_frame_release when done).  After a step: pbd_passes,
+ * max_depth and jfa_ran describe it; gate_read says whether the previous
+ * gated frame's warm-start gate was read (its value in gate); pending says
+ * whether this frame's gate is still on the device (maxd[1]).  pass_events
+ * (may be NULL) as for fos_jfa_passes_timed.  No reference counterpart (the
+ * reference's loop is Python). */
+typedef struct fos_frame_desc {
+  int n, res, G, jfa_passes;
+  float L, voxel, beta, ema_alpha;
+  int early_exit, min_passes;
+  double th_pass_delta;
+  int pbd_base, pbd_max;
+  double pbd_scale, r_max_d;
+  float gap_factor, max_disp_frac, disp_mult;
+  float alpha, max_step_pct, r_min, r_max, contact_tol;
+  double contact_tol_d;
+  int backpressure_mode, adjustment_frames, warmstart_frames;
+  double fsc_zero_rate;
+  int f_low, f_high;
+  float growth_pct, deadband;
+  int pressure, pressure_iters, pressure_max_nei;
+  float pressure_k, pressure_exp, pressure_cap;
+  int brownian, brownian_mode;
+  float brownian_strength;
+  uint64_t seed;
+  int ou_jitter;
+  float dt, jitter_rms, jitter_tau, drift_cap_frac;
+  int jfa_every, jfa_warm, slot_records;
+} fos_frame_desc;
+typedef struct fos_frame_bufs {
+  float *pos[2];
+  float *rad, *rad_target, *lmd, *delta_r, *v_jit;
+  int32_t *cell_count, *cell_start, *cell_indices;
+  float *cell_pr;
+  int32_t *grid, *temp, *filtered, *face_ids, *face_counts, *fsc;
+  float *fsc_ema;
+  int32_t *neighbor_list, *neighbor_count, *used_count;
+  float *prev_scores;
+  int32_t *info;
+  float *maxd, *maxd_host;
+  int32_t *settle_dev;
+} fos_frame_bufs;
+typedef struct fos_frame_state {
+  int frame, pos_cur, sidx, settle_force, settle_gate, need_rebuild, pending;
+  int pbd_passes, jfa_ran, gate_read;
+  float max_depth, gate;
+  void *event;
+} fos_frame_state;
+int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
+                   fos_frame_state *s, void *const *pass_events, void *stream);
+/* Frees what fos_frame_step created in the state (its host-read event). */
+int fos_frame_release(fos_frame_state *s);
+
 /* ---- topology.py ------------------------------------------------------- */
 
 /* topology.py:68-133 build_topo_candidates: pairs[n][cap], count[n],
@@ -503,9 +579,9 @@ int fos_batch_majority(fos_batch *batch, const int32_t *grid, int32_t *filtered,
 int fos_batch_face_accumulate(fos_batch *batch, fos_ctx *ctx, const int32_t *filtered,
                               int32_t *face_ids, int32_t *face_counts, int check, void *stream);
 /* jfa.py:789-857 for every foam (each foam's voxel edge in the scores).
- * fixed_pairs > 0: that many fixed-point round pairs with no host round trip
- * (a device flag marks convergence, fos_batch_flags); 0: until converged,
- * the host checking after each pair. */
+ * fixed_pairs > 0: that many fixed-point round pairs (2 rounds each) with no
+ * host round trip (a device flag marks convergence, fos_batch_flags); 0:
+ * until converged, the host checking after each burst of rounds. */
 int fos_batch_finalize_faces(fos_batch *batch, fos_ctx *ctx, const float *rad,
                              const int32_t *face_ids, const int32_t *face_counts,
                              float *prev_scores, int32_t *neighbor_list, int32_t *neighbor_count,

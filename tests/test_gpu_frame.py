@@ -304,3 +304,37 @@ def test_integration_snippet_runs(gpu, orc, keep_config):
     _, _, _, _, fs = orc.finalize_faces(N, ns["rad_np"], voxel, ids, cts, np.zeros((N, 32), F))
     assert np.array_equal(ns["fsc"], fs)
     assert ns["max_dr"] >= 0.0 and ns["changed"] >= 0
+
+
+@pytest.mark.parametrize("variant", ["c2", "noise_cadence", "global_back", "no_pressure"])
+def test_native_frame_equals_python_frame(gpu, variant):
+    """fos_frame_step (the frame issued from C++) against the same frame
+    issued call by call from Python: every state array and every frame's
+    record (PBD passes, overlap maximum, JFA, gate) bit for bit, across a
+    snapshot/restore, with the warm-start gate opening inside the run."""
+    from foam import FoamSim
+    if variant == "c2":
+        cfg = fc.preset("c2")
+        cfg.warmstart_frames = 3
+        pos, rad = fi.for_config(cfg)
+    else:
+        kw = {"noise_cadence": dict(brownian=True, brownian_mode=1, brownian_strength=1e-3,
+                                    ou_jitter=True, jfa_every=2, jfa_warm=2),
+              "global_back": dict(backpressure_mode=1),
+              "no_pressure": dict(pressure=False, pbd_base=0, pbd_max=0)}[variant]
+        cfg = small_cfg(**kw)
+        cfg.r_min, cfg.r_max = 0.02, 0.07
+        pos, rad = fi.uniform(cfg.n, 0.03, 0.06, seed=33)
+    a, b = FoamSim(cfg, pos, rad), FoamSim(cfg, pos, rad)
+    b.native = False
+    for f in range(7):
+        print(f"[native frame] {variant} frame {f}", flush=True)
+        ra, rb = a.step(), b.step()
+        for k in ("pbd_passes", "max_depth", "jfa", "growth"):
+            assert ra[k] == rb[k], (f, k, ra[k], rb[k])
+        sa, sb = a.snapshot(), b.snapshot()
+        for k in sa:
+            assert np.array_equal(sa[k], sb[k]), (f, k)
+        if f == 3:                        # a restore forces the frame-start rebuild
+            a.restore(sa)
+            b.restore(sb)
