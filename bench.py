@@ -16,6 +16,8 @@ The JSON line also carries:
   cpu_baseline  the CPU restatement (C, OpenMP over the host cores) timed on
                 a bounded sample of the same frames
   cadence       the same workload with JFA on a 1/5 cadence after 20 frames
+  c3_jfa        the JFA alone on the large clustered foam (N=200,000, 384^3)
+  c3_frames     whole frames of that foam
   surrogate     the learned face predictor at N=50,000 / 256^3: frames/s with
                 the JFA every frame against the JFA every 8th frame (the
                 judge) and the predictor in between, with the judged FSC
@@ -262,6 +264,23 @@ def c3_jfa_bench(frames):
                          "per_pass_us": [round(x * 1e3, 1) for x in flat.mean(axis=0)]}}
 
 
+def c3_frames_bench(frames, warmup=2):
+    """Whole C3 frames (N=200,000 clustered, 384^3, the run.py:700-1261
+    order): frames/s of FoamSim on the large foam, inputs resident."""
+    import foam_config as fc
+    import foam_inputs as fi
+    cfg = fc.preset("c3")
+    pos, rad = fi.for_config(cfg)
+    el, _, sim = timed(cfg, pos, rad, frames, warmup, False)
+    return {"workload": (f"C3 frames: N={cfg.n:,} clustered, radii lognormal clipped to "
+                         f"[0.003, 0.015], JFA {cfg.res}^3 every frame, PBD {cfg.pbd_base}.."
+                         f"{cfg.pbd_max}, hash {cfg.hash_res()}^3, pressure {cfg.pressure_iters} "
+                         f"iteration"),
+            "frames": frames, "frames_per_s": round(frames / el, 3),
+            "ms_per_frame": round(1e3 * el / frames, 3),
+            "pbd_passes_last": int(sim.last["pbd_passes"])}
+
+
 def surrogate_bench(steps, warmup):
     """Learned face predictor on the N=50,000 / 256^3 workload: frames/s of the
     plain path (JFA every frame) and of the predictor path (JFA every 8th
@@ -488,6 +507,7 @@ def main():
                           "frames": [w2, w2 + args.steps - 1], "jfa_frames": jfa_frames}
     if not args.no_c3:
         out["c3_jfa"] = c3_jfa_bench(args.c3_frames)
+        out["c3_frames"] = c3_frames_bench(args.c3_frames)
     if not args.no_surrogate and os.path.exists(WEIGHTS):
         out["surrogate"] = surrogate_bench(args.surrogate_steps, 8)
     if not args.no_ensemble:
