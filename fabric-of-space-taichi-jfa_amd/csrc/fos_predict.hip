@@ -6,11 +6,12 @@
 // topology.build_topo_candidates (topology.py:68-133: 27 hash cells, a
 // capped list per particle), here for every j != i.
 //
-// Pipeline per frame: candidates -> 12 features per (i, j) -> MLP
-// 12 -> 64 -> 64 -> 1 (BF16 weights and activations on MFMA, FP32
-// accumulation, ReLU) -> logit > 0 is a face (sigmoid 0.5) -> FSC and face
-// rows.  Training kernels (FP32, Adam) fit the MLP to labels taken from the
-// plain JFA's neighbour lists.
+// Pipeline per frame: candidates -> 13 features per (i, j) (12 geometric,
+// one hysteresis) -> MLP 13 -> 64 -> 64 -> 1 (BF16 weights and activations
+// on MFMA, FP32 accumulation, ReLU) -> logit > 0 is a face (sigmoid 0.5) ->
+// FSC and face rows.  The training step (the same GEMMs on MFMA, forward and
+// backward, with Adam in FP32) fits the MLP to labels taken from the plain
+// JFA's neighbour lists.
 #include "fos_common.h"
 
 using namespace fos;
@@ -595,121 +596,242 @@ __global__ void k_pred_labels(const int32_t *__restrict__ cand, const int32_t *_
 }
 
 // ---------------------------------------------------------------------------
-// Training (FP32).  One step over a batch of B samples (indices into the
-// feature/label arrays): forward keeps H1, H2 and dL/dz3; backward forms
-// dZ2, dZ1; the gradient kernels sum per parameter over the batch in a
-// fixed order; Adam.
+// Training on MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation).  One step
+// over a batch of B samples (indices into the feature/label arrays) is two
+// launches:
+//   k_mlp_train  one wave per workgroup; wave w takes the 16-sample tiles
+//                w, w + W, w + 2W, ...  Per tile, in registers:
+//                  forward   Z1^T = W1^T X^T, H1 = relu(Z1 + b1),
+//                            Z2^T = W2^T H1^T, H2 = relu(Z2 + b2),
+//                            z3 = H2 w3 + b3 (the inference kernel's layout);
+//                  backward  dz3 = w (sigmoid(z3) - t) / B,
+//                            dZ2 = [Z2 + b2 > 0] dz3 w3^T,
+//                            dZ1^T = [Z1 + b1 > 0] (W2 dZ2^T);
+//                  gradients gW2^T += dZ2^T H1, gW1^T += dZ1^T X,
+//                            with the batch as the MFMA K dimension: H1^T,
+//                            dZ2^T, dZ1^T and X^T go through LDS to put the
+//                            sample index in the K slots;
+//                            b1, b2, w3, b3 and the loss in FP32 VALU.
+//                The wave writes its partial sums (and loss) to part[w].
+//   k_mlp_adam   sums the W partials per parameter in wave order, then Adam.
+// X, W1, W2, H1, dZ2 and dZ1 enter the MFMAs as BF16 (round to nearest
+// even, as the inference kernel rounds X, W and H1); Z1, Z2, H2, dz3, dZ2,
+// dZ1 and every sum are FP32.  Feature slots PRED_NFEAT..15 are read as 0,
+// so rows PRED_NFEAT..15 of W1 get a zero gradient.  The result does not
+// depend on the launch beyond W, and W depends only on B, so a step is
+// reproducible bit for bit.
 // Loss: class-weighted binary cross entropy on the logit.
 // ---------------------------------------------------------------------------
-__global__ void k_mlp_fwd(const float *__restrict__ feat, const float *__restrict__ y,
-                          const int32_t *__restrict__ idx, int B, const float *__restrict__ prm,
-                          float pos_w, float *__restrict__ H1, float *__restrict__ H2,
-                          float *__restrict__ dz3, float *__restrict__ loss) {
-  const int b0 = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = b0 < B;
-  const int b = live ? b0 : B - 1;
-  const int64_t p = idx[b];
-  float x[PRED_NFEAT];
+#define MLP_TRAIN_WAVES 64
+#define MLP_PART (MLP_NPARAM + 1)   // per-wave partial: the gradient, then the loss sum
+static_assert(MLP_TRAIN_WAVES * MLP_PART == FOS_MLP_TRAIN_SCRATCH, "fos.h scratch size");
+
+__global__ void __launch_bounds__(64)
+    k_mlp_train(const float *__restrict__ feat, const float *__restrict__ y,
+                const int32_t *__restrict__ idx, int B, const float *__restrict__ prm,
+                float pos_w, float *__restrict__ part) {
+  // transposes: rows are a hidden unit (or feature), columns the 16 samples
+  __shared__ __align__(16) short sH1[MLP_H][16], sD2[MLP_H][16], sD1[MLP_H][16], sX[16][16];
+  const int lane = (int)threadIdx.x;
+  const int col = lane & 15, grp = lane >> 4;
+  const int w = (int)blockIdx.x, W = (int)gridDim.x;
+  // weights as A operands: W1^T and W2^T blocks for the forward (lane holds
+  // W[16kb + 4grp + v][16mb + col]), W2 blocks for the backward (lane holds
+  // W2[16ma + col][16kb + 4grp + v])
+  bf16x4 a1[4], a2[4][4], a2b[4][4];
+  float bb1[4][4], bb2[4][4], ww3[4][4];
 #pragma unroll
-  for (int f = 0; f < PRED_NFEAT; ++f) x[f] = feat[p * PRED_FEAT + f];
-  float h1[MLP_H];
+  for (int mb = 0; mb < 4; ++mb) {
 #pragma unroll
-  for (int a = 0; a < MLP_H; ++a) {
-    float z = prm[MLP_OFF_B1 + a];
+    for (int v = 0; v < 4; ++v) {
+      const int k = 4 * grp + v, h = 16 * mb + k;
+      a1[mb][v] = f2bf(prm[MLP_OFF_W1 + k * MLP_H + 16 * mb + col]);
 #pragma unroll
-    for (int f = 0; f < PRED_NFEAT; ++f) z += x[f] * prm[MLP_OFF_W1 + f * MLP_H + a];
-    h1[a] = fmaxf(z, 0.0f);
-    if (live) H1[(int64_t)b * MLP_H + a] = h1[a];
+      for (int kb = 0; kb < 4; ++kb) {
+        a2[mb][kb][v] = f2bf(prm[MLP_OFF_W2 + (16 * kb + k) * MLP_H + 16 * mb + col]);
+        a2b[mb][kb][v] = f2bf(prm[MLP_OFF_W2 + (16 * mb + col) * MLP_H + 16 * kb + k]);
+      }
+      bb1[mb][v] = prm[MLP_OFF_B1 + h];
+      bb2[mb][v] = prm[MLP_OFF_B2 + h];
+      ww3[mb][v] = prm[MLP_OFF_W3 + h];
+    }
   }
-  float z3 = prm[MLP_OFF_B3];
-  for (int h = 0; h < MLP_H; ++h) {
-    float z = prm[MLP_OFF_B2 + h];
+  const float b3 = prm[MLP_OFF_B3];
+  const float inv_b = 1.0f / (float)B;
+  f32x4 gw2[4][4], gw1[4];
+  float gb1[4][4], gb2[4][4], gw3[4][4];
 #pragma unroll
-    for (int a = 0; a < MLP_H; ++a) z += h1[a] * prm[MLP_OFF_W2 + a * MLP_H + h];
-    z = fmaxf(z, 0.0f);
-    if (live) H2[(int64_t)b * MLP_H + h] = z;
-    z3 += z * prm[MLP_OFF_W3 + h];
+  for (int i = 0; i < 4; ++i) {
+    gw1[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      gw2[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      gb1[i][j] = gb2[i][j] = gw3[i][j] = 0.0f;
+    }
   }
-  const float t = y[p];
-  const float w = t > 0.5f ? pos_w : 1.0f;
-  const float sg = 1.0f / (1.0f + expf(-z3));
-  if (live) dz3[b] = w * (sg - t) / (float)B;
-  // log(1 + e^-|z|) + max(z, 0) - t z, summed per wave, one atomic per wave
-  float l = live ? w * (log1pf(expf(-fabsf(z3))) + fmaxf(z3, 0.0f) - t * z3) : 0.0f;
-  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(loss, l / (float)B);
+  float gb3 = 0.0f, loss = 0.0f;
+  const int tiles = (B + 15) >> 4;
+  for (int t = w; t < tiles; t += W) {
+    const int s = 16 * t + col;
+    const bool live = s < B;
+    const int64_t p = live ? (int64_t)idx[s] : 0;
+    float4 x = live ? *(const float4 *)(feat + p * PRED_FEAT + 4 * grp)
+                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    static_assert(PRED_NFEAT == 13, "feature slots 13..15 are the zero ones");
+    if (grp == 3) x.y = x.z = x.w = 0.0f;
+    bf16x4 xb;
+    xb[0] = f2bf(x.x);
+    xb[1] = f2bf(x.y);
+    xb[2] = f2bf(x.z);
+    xb[3] = f2bf(x.w);
+    // forward
+    bf16x4 h1b[4];
+    unsigned m1 = 0u;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[mb], xb, acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float z = acc[v] + bb1[mb][v];
+        if (z > 0.0f) m1 |= 1u << (4 * mb + v);
+        h1b[mb][v] = f2bf(fmaxf(z, 0.0f));
+      }
+    }
+    float z2[4][4];
+    float part3 = 0.0f;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[mb][kb], h1b[kb], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        z2[mb][v] = acc[v] + bb2[mb][v];
+        part3 += fmaxf(z2[mb][v], 0.0f) * ww3[mb][v];
+      }
+    }
+    // the four lanes of a sample add the same four partials (same bits)
+    part3 += __shfl_xor(part3, 16, 64);
+    part3 += __shfl_xor(part3, 32, 64);
+    const float z3 = part3 + b3;
+    const float tg = live ? y[p] : 0.0f;
+    const float wt = tg > 0.5f ? pos_w : 1.0f;
+    const float dz3 = live ? wt * (1.0f / (1.0f + expf(-z3)) - tg) * inv_b : 0.0f;
+    if (grp == 0 && live) {
+      // log(1 + e^-|z|) + max(z, 0) - t z
+      loss += wt * (log1pf(expf(-fabsf(z3))) + fmaxf(z3, 0.0f) - tg * z3);
+      gb3 += dz3;
+    }
+    // backward
+    bf16x4 d2b[4], d1b[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float d = z2[mb][v] > 0.0f ? dz3 * ww3[mb][v] : 0.0f;
+        gb2[mb][v] += d;
+        gw3[mb][v] += dz3 * fmaxf(z2[mb][v], 0.0f);
+        d2b[mb][v] = f2bf(d);
+      }
+    }
+#pragma unroll
+    for (int ma = 0; ma < 4; ++ma) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2b[ma][kb], d2b[kb], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float d = (m1 >> (4 * ma + v)) & 1u ? acc[v] : 0.0f;
+        gb1[ma][v] += d;
+        d1b[ma][v] = f2bf(d);
+      }
+    }
+    // transposes through LDS: lane (col, grp) holds rows 16mb + 4grp + v of
+    // sample col and reads back row 16mb + col, samples 4grp..4grp+3
+    __syncthreads();
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * mb + 4 * grp + v;
+        sH1[r][col] = h1b[mb][v];
+        sD2[r][col] = d2b[mb][v];
+        sD1[r][col] = d1b[mb][v];
+      }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sX[4 * grp + v][col] = xb[v];
+    __syncthreads();
+    bf16x4 tH1[4], tD2[4], tD1[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      tH1[mb] = *(const bf16x4 *)&sH1[16 * mb + col][4 * grp];
+      tD2[mb] = *(const bf16x4 *)&sD2[16 * mb + col][4 * grp];
+      tD1[mb] = *(const bf16x4 *)&sD1[16 * mb + col][4 * grp];
+    }
+    const bf16x4 tX = *(const bf16x4 *)&sX[col][4 * grp];
+    // gW2^T[16mh + 4grp + v][16na + col] and gW1^T[16ma + 4grp + v][col]
+#pragma unroll
+    for (int mh = 0; mh < 4; ++mh)
+#pragma unroll
+      for (int na = 0; na < 4; ++na)
+        gw2[mh][na] =
+            __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tD2[mh], tH1[na], gw2[mh][na], 0, 0, 0);
+#pragma unroll
+    for (int ma = 0; ma < 4; ++ma)
+      gw1[ma] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tD1[ma], tX, gw1[ma], 0, 0, 0);
+  }
+  float *out = part + (int64_t)w * MLP_PART;
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int h = 16 * mb + 4 * grp + v;
+      out[MLP_OFF_W1 + col * MLP_H + h] = gw1[mb][v];
+#pragma unroll
+      for (int na = 0; na < 4; ++na) out[MLP_OFF_W2 + (16 * na + col) * MLP_H + h] = gw2[mb][na][v];
+      // the 16 sample lanes of a row group, in a fixed butterfly
+      float s1 = gb1[mb][v], s2 = gb2[mb][v], s3 = gw3[mb][v];
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+        s3 += __shfl_xor(s3, o, 64);
+      }
+      if (col == 0) {
+        out[MLP_OFF_B1 + h] = s1;
+        out[MLP_OFF_B2 + h] = s2;
+        out[MLP_OFF_W3 + h] = s3;
+      }
+    }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    gb3 += __shfl_xor(gb3, o, 64);
+    loss += __shfl_xor(loss, o, 64);
+  }
+  if (lane == 0) {
+    out[MLP_OFF_B3] = gb3;
+    out[MLP_NPARAM] = loss;
+  }
 }
 
-__global__ void k_mlp_bwd(const float *__restrict__ H1, const float *__restrict__ H2,
-                          const float *__restrict__ dz3, int B, const float *__restrict__ prm,
-                          float *__restrict__ dZ1, float *__restrict__ dZ2) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  float d2[MLP_H];
-  const float g = dz3[b];
-#pragma unroll
-  for (int h = 0; h < MLP_H; ++h) {
-    d2[h] = H2[(int64_t)b * MLP_H + h] > 0.0f ? g * prm[MLP_OFF_W3 + h] : 0.0f;
-    dZ2[(int64_t)b * MLP_H + h] = d2[h];
-  }
-  for (int a = 0; a < MLP_H; ++a) {
-    float s = 0.0f;
-#pragma unroll
-    for (int h = 0; h < MLP_H; ++h) s += prm[MLP_OFF_W2 + a * MLP_H + h] * d2[h];
-    dZ1[(int64_t)b * MLP_H + a] = H1[(int64_t)b * MLP_H + a] > 0.0f ? s : 0.0f;
-  }
-}
-
-// Parameter gradients: block (x, y) sums parameters 64x..64x+63 over batch
-// slice y (fixed order); k_mlp_grad_sum adds the slices in order, so a step
-// is reproducible bit for bit.
-#define MLP_GRAD_SPLIT 32
-__global__ void k_mlp_grad(const float *__restrict__ feat, const int32_t *__restrict__ idx, int B,
-                           const float *__restrict__ H1, const float *__restrict__ H2,
-                           const float *__restrict__ dz3, const float *__restrict__ dZ1,
-                           const float *__restrict__ dZ2, float *__restrict__ part) {
+// Sum of the W partials per parameter (wave order) into grad, then Adam;
+// slot MLP_NPARAM adds the batch mean loss to *loss.
+__global__ void k_mlp_adam(const float *__restrict__ part, int W, float *__restrict__ grad,
+                           float *__restrict__ prm, float *__restrict__ m, float *__restrict__ v,
+                           float lr, float b1, float b2, float eps, float c1, float c2,
+                           float inv_b, float *__restrict__ loss) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= MLP_NPARAM) return;
-  const int per = (B + MLP_GRAD_SPLIT - 1) / MLP_GRAD_SPLIT;
-  const int b0 = blockIdx.y * per, b1 = min(B, b0 + per);
-  float s = 0.0f;
-  if (q < MLP_OFF_B1) {  // W1[f][a]
-    const int f = q / MLP_H, a = q % MLP_H;
-    if (f < PRED_NFEAT)
-      for (int b = b0; b < b1; ++b)
-        s += feat[(int64_t)idx[b] * PRED_FEAT + f] * dZ1[(int64_t)b * MLP_H + a];
-  } else if (q < MLP_OFF_W2) {
-    const int a = q - MLP_OFF_B1;
-    for (int b = b0; b < b1; ++b) s += dZ1[(int64_t)b * MLP_H + a];
-  } else if (q < MLP_OFF_B2) {
-    const int r = q - MLP_OFF_W2, a = r / MLP_H, h = r % MLP_H;
-    for (int b = b0; b < b1; ++b) s += H1[(int64_t)b * MLP_H + a] * dZ2[(int64_t)b * MLP_H + h];
-  } else if (q < MLP_OFF_W3) {
-    const int h = q - MLP_OFF_B2;
-    for (int b = b0; b < b1; ++b) s += dZ2[(int64_t)b * MLP_H + h];
-  } else if (q < MLP_OFF_B3) {
-    const int h = q - MLP_OFF_W3;
-    for (int b = b0; b < b1; ++b) s += dz3[b] * H2[(int64_t)b * MLP_H + h];
-  } else {
-    for (int b = b0; b < b1; ++b) s += dz3[b];
+  if (q > MLP_NPARAM) return;
+  float g = 0.0f;
+  for (int k = 0; k < W; ++k) g += part[(int64_t)k * MLP_PART + q];
+  if (q == MLP_NPARAM) {
+    loss[0] += g * inv_b;
+    return;
   }
-  part[(int64_t)blockIdx.y * MLP_NPARAM + q] = s;
-}
-
-__global__ void k_mlp_grad_sum(const float *__restrict__ part, float *__restrict__ grad) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= MLP_NPARAM) return;
-  float s = 0.0f;
-  for (int k = 0; k < MLP_GRAD_SPLIT; ++k) s += part[(int64_t)k * MLP_NPARAM + q];
-  grad[q] = s;
-}
-
-__global__ void k_adam(float *__restrict__ prm, const float *__restrict__ grad,
-                       float *__restrict__ m, float *__restrict__ v, float lr, float b1,
-                       float b2, float eps, float c1, float c2) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= MLP_NPARAM) return;
-  const float g = grad[q];
+  grad[q] = g;
   const float mq = b1 * m[q] + (1.0f - b1) * g;
   const float vq = b2 * v[q] + (1.0f - b2) * g * g;
   m[q] = mq;
@@ -848,25 +970,17 @@ int fos_mlp_train_step(float *params, float *grad, float *adam_m, float *adam_v,
                   loss && B >= 1 && step >= 1 && lr > 0.0f,
               "fos_mlp_train_step: bad arguments (B=%d step=%d)", B, step);
   hipStream_t st = (hipStream_t)stream;
-  float *H1 = scratch, *H2 = H1 + (size_t)B * MLP_H, *dZ1 = H2 + (size_t)B * MLP_H,
-        *dZ2 = dZ1 + (size_t)B * MLP_H, *dz3 = dZ2 + (size_t)B * MLP_H, *part = dz3 + B;
-  hipLaunchKernelGGL(k_mlp_fwd, dim3(blocks_for(B, 64)), dim3(64), 0, st, feat, labels,
-                     batch_idx, B, (const float *)params, pos_weight, H1, H2, dz3, loss);
-  FOS_CHECK_LAUNCH("k_mlp_fwd");
-  hipLaunchKernelGGL(k_mlp_bwd, dim3(blocks_for(B, 64)), dim3(64), 0, st, H1, H2, dz3, B,
-                     (const float *)params, dZ1, dZ2);
-  FOS_CHECK_LAUNCH("k_mlp_bwd");
-  hipLaunchKernelGGL(k_mlp_grad, dim3(blocks_for(MLP_NPARAM, 64), MLP_GRAD_SPLIT), dim3(64), 0,
-                     st, feat, batch_idx, B, H1, H2, dz3, dZ1, dZ2, part);
-  FOS_CHECK_LAUNCH("k_mlp_grad");
-  hipLaunchKernelGGL(k_mlp_grad_sum, dim3(blocks_for(MLP_NPARAM)), dim3(FOS_BLOCK), 0, st,
-                     (const float *)part, grad);
-  FOS_CHECK_LAUNCH("k_mlp_grad_sum");
+  const int tiles = (B + 15) / 16;
+  const int W = tiles < MLP_TRAIN_WAVES ? tiles : MLP_TRAIN_WAVES;
+  hipLaunchKernelGGL(k_mlp_train, dim3(W), dim3(64), 0, st, feat, labels, batch_idx, B,
+                     (const float *)params, pos_weight, scratch);
+  FOS_CHECK_LAUNCH("k_mlp_train");
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
-  hipLaunchKernelGGL(k_adam, dim3(blocks_for(MLP_NPARAM)), dim3(FOS_BLOCK), 0, st, params,
-                     (const float *)grad, adam_m, adam_v, lr, b1, b2, 1e-8f, c1, c2);
-  FOS_CHECK_LAUNCH("k_adam");
+  hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(MLP_PART)), dim3(FOS_BLOCK), 0, st,
+                     (const float *)scratch, W, grad, params, adam_m, adam_v, lr, b1, b2, 1e-8f,
+                     c1, c2, 1.0f / (float)B, loss);
+  FOS_CHECK_LAUNCH("k_mlp_adam");
   return 0;
 }
 

@@ -210,8 +210,12 @@ def fsc_error(pred, ref):
     return float((p - r).abs().sum().item()) / tot, abs(float(p.sum().item()) - tot) / tot
 
 
+TRAIN_SCRATCH = 64 * (NPARAM + 1)   # fos.h FOS_MLP_TRAIN_SCRATCH
+
+
 class Trainer:
-    """Adam on the fp32 parameters with the hand-written training kernels."""
+    """Adam on the fp32 parameters with the hand-written MFMA training step
+    (BF16 GEMM operands, FP32 accumulation and optimizer state)."""
 
     def __init__(self, params, device="cuda", batch=4096, lr=2e-3, pos_weight=1.0):
         d = torch.device(device)
@@ -220,8 +224,7 @@ class Trainer:
         self.m = torch.zeros_like(self.p)
         self.v = torch.zeros_like(self.p)
         self.B, self.lr, self.pos_weight = int(batch), float(lr), float(pos_weight)
-        self.scratch = torch.empty(4 * H * self.B + self.B + 32 * NPARAM, dtype=torch.float32,
-                                   device=d)
+        self.scratch = torch.empty(TRAIN_SCRATCH, dtype=torch.float32, device=d)
         self.loss = torch.zeros(1, dtype=torch.float32, device=d)
         self.t = 0
 
@@ -229,16 +232,16 @@ class Trainer:
         """One pass over valid_idx (int64 device tensor of sample rows) in
         shuffled batches; returns the mean batch loss."""
         perm = valid_idx[torch.randperm(valid_idx.numel(), device=valid_idx.device,
-                                        generator=gen)]
+                                        generator=gen)].to(torch.int32).contiguous()
         nb = perm.numel() // self.B
         self.loss.zero_()
+        base, st = perm.data_ptr(), fl.stream()
         for b in range(nb):
-            idx = perm[b * self.B:(b + 1) * self.B].to(torch.int32).contiguous()
             self.t += 1
             fl.call("fos_mlp_train_step", fl.ptr(self.p), fl.ptr(self.g), fl.ptr(self.m),
-                    fl.ptr(self.v), self.t, self.lr, fl.ptr(feat), fl.ptr(labels), fl.ptr(idx),
-                    self.B, self.pos_weight, fl.ptr(self.scratch), fl.ptr(self.loss),
-                    fl.stream())
+                    fl.ptr(self.v), self.t, self.lr, fl.ptr(feat), fl.ptr(labels),
+                    ctypes.c_void_p(base + 4 * b * self.B), self.B, self.pos_weight,
+                    fl.ptr(self.scratch), fl.ptr(self.loss), st)
         return float(self.loss.item()) / max(1, nb)
 
     def params(self):

@@ -7,8 +7,8 @@ frame after a warm-up records, for each hash-grid candidate pair, the 13
 features (fos_pred_features; feature 12 from the previous frame's JFA) and the label "j is in i's JFA neighbour list"
 (fos_pred_labels), one per scored pair (each unordered pair once).  The
 last --holdout frames are kept for validation.  The
-MLP is trained with the hand-written FP32 kernels (fos_mlp_train_step, Adam)
-and evaluated with the BF16 MFMA inference kernel: pair accuracy and the
+MLP is trained with the hand-written MFMA training step (fos_mlp_train_step:
+BF16 GEMM operands, FP32 accumulation, Adam in FP32) and evaluated with the BF16 MFMA inference kernel: pair accuracy and the
 per-frame FSC errors  sum|FSC_pred - FSC_jfa| / sum FSC_jfa  (per particle)
 and  |sum FSC_pred - sum FSC_jfa| / sum FSC_jfa  (the frame's mean FSC).
 After training the output bias is shifted so the training frames' predicted

@@ -528,9 +528,14 @@ int fos_pred_labels(const int32_t *cand, const int32_t *pair_off, const int32_t 
 /* Number of FP32 MLP parameters (flat layout W1[16][64] b1 W2[64][64] b2 w3
  * b3) into host *count. */
 int fos_mlp_param_count(int *count);
-/* One Adam step (FP32) on a batch of B sample indices with class-weighted
- * binary cross entropy; scratch holds 4*64*B + B + 32*param_count floats;
- * *loss (device) is increased by the batch mean loss. */
+/* One Adam step on a batch of B sample indices with class-weighted binary
+ * cross entropy.  Forward, backward and weight-gradient GEMMs run on MFMA
+ * with BF16 operands (X, W1, W2, H1, dZ2, dZ1) and FP32 accumulation; bias
+ * and last-layer gradients, the loss and Adam are FP32.  scratch holds at
+ * least FOS_MLP_TRAIN_SCRATCH floats (per-wave partial gradients); *loss
+ * (device) is increased by the batch mean loss.  Bit-reproducible for a
+ * given B. */
+#define FOS_MLP_TRAIN_SCRATCH (64 * (5313 + 1))
 int fos_mlp_train_step(float *params, float *grad, float *adam_m, float *adam_v, int step,
                        float lr, const float *feat, const float *labels, const int32_t *batch_idx,
                        int B, float pos_weight, float *scratch, float *loss, void *stream);
