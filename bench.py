@@ -281,6 +281,26 @@ def c3_frames_bench(frames, warmup=2):
             "pbd_passes_last": int(sim.last["pbd_passes"])}
 
 
+def candidate_coverage(sim, mult):
+    """Share of the last JFA frame's neighbour relations (both directions)
+    that the predictor's candidate rows at `mult` (ri + rj) hold, with the
+    candidates per particle and the rows cut at the row capacity."""
+    import torch
+
+    import face_predictor as fp
+    n = sim.cfg.n
+    pr = fp.FacePredictor(fp.init_params(0), n, sim.cfg.L, sim.cfg.r_max, mult=mult)
+    pr.candidates(sim.pos, sim.rad, n)
+    lab = torch.empty(n * pr.cap, dtype=torch.float32, device="cuda")
+    pr.labels(n, sim.neighbor_list, sim.neighbor_count, lab)
+    P = pr.npairs(n)
+    found = float(lab[:P].sum().item())
+    rel = float(sim.neighbor_count[:n].sum().item())
+    return {"coverage": round(2.0 * found / max(1.0, rel), 5),
+            "candidates_per_particle": round(float(pr.ncand[:n].float().mean().item()), 2),
+            "truncated_rows": int(pr.trunc[:n].sum().item())}
+
+
 def surrogate_bench(steps, warmup):
     """Learned face predictor on the N=50,000 / 256^3 workload: frames/s of the
     plain path (JFA every frame) and of the predictor path (JFA every 8th
@@ -296,13 +316,28 @@ def surrogate_bench(steps, warmup):
     params, meta = w["params"], _json.loads(str(w["meta"]))
     plain = fc.preset("c5")
     pos, rad = fi.for_config(plain)
-    el_p, _, _ = timed(plain, pos, rad, steps, warmup, False)
+    el_p, _, psim = timed(plain, pos, rad, steps, warmup, False)
     fp_ = steps / el_p
+    cover = {str(m): candidate_coverage(psim, m) for m in (1.6, meta["mult"])}
     out = {"workload": "N=50000, radii U[0.005,0.011], JFA 256^3; predictor path: JFA every "
                        "8th frame as judge, BF16 MLP on MFMA in between, JFA on every frame "
                        "while the judged error exceeds 2%",
            "plain_fps": round(fp_, 3), "frames": steps,
-           "candidate_mult": meta["mult"], "trained_on": f"{meta['frames']} plain-JFA frames"}
+           "candidate_mult": meta["mult"],
+           "candidate_coverage": cover,
+           "trained_on": (f"{meta['frames']} plain-JFA frames of {len(meta['train_seeds'])} "
+                          f"foams (input seeds {meta['train_seeds']}); the benchmark's foam "
+                          f"(seed {meta['holdout_seed']}) only validates"),
+           "features": 13,
+           "departures": (
+               "candidates within 3.5 (ri + rj), not 1.6: the radii fill ~14% of the box, so "
+               "faces reach far beyond the spheres; candidate_coverage is the share of the "
+               "JFA's neighbour relations the candidate rows hold at each multiple (1.6 caps "
+               "the per-particle error far above 2% before any scoring). 13 features, not 12: "
+               "the 13th is the previous frame's acceptance, which the JFA's face hysteresis "
+               "uses (jfa.py:400-443). 240 training frames, not 20,000: 32 frames of one foam "
+               "and 240 frames of six foams give the same validation error (2.91%), so more "
+               "frames of this workload do not lower it")}
     # the judge bounds either every particle's FSC (per_particle) or the
     # frame's mean FSC (mean, where over- and under-counts cancel)
     for judge_on in ("per_particle", "mean"):
@@ -323,9 +358,10 @@ def surrogate_bench(steps, warmup):
             "fsc_err_mean_fsc": (round(float(np.mean([j[2] for j in judged])), 5)
                                  if judged else None),
             "fallback_judgements": int(sum(1 for j in judged if j[k] > surr.max_fsc_err))}
-    out["note"] = ("the <=2% bound holds for the frame's mean FSC (judge_mean); the per-particle "
-                   "error is higher, and judged per particle at 2% the path falls back to the "
-                   "JFA (judge_per_particle)")
+    out["note"] = ("the <=2% bound holds for the frame's mean FSC only (judge_mean); the "
+                   "per-particle error is about 3% (1% of it the relations the candidates miss), "
+                   "so judged per particle at 2% the path falls back to the JFA on every frame "
+                   "(judge_per_particle)")
     return out
 
 

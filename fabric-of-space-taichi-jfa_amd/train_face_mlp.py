@@ -2,15 +2,19 @@
 
     python train_face_mlp.py --config c5 --out face_mlp_c5.npz
 
-Runs the plain foam (JFA every frame) of the given workload, and on every
-frame after a warm-up records, for each hash-grid candidate pair, the 13
-features (fos_pred_features; feature 12 from the previous frame's JFA) and the label "j is in i's JFA neighbour list"
-(fos_pred_labels), one per scored pair (each unordered pair once).  The
-last --holdout frames are kept for validation.  The
-MLP is trained with the hand-written MFMA training step (fos_mlp_train_step:
-BF16 GEMM operands, FP32 accumulation, Adam in FP32) and evaluated with the BF16 MFMA inference kernel: pair accuracy and the
-per-frame FSC errors  sum|FSC_pred - FSC_jfa| / sum FSC_jfa  (per particle)
-and  |sum FSC_pred - sum FSC_jfa| / sum FSC_jfa  (the frame's mean FSC).
+Runs the plain foam (JFA every frame) of the given workload from several
+input seeds (--seeds), and on every --stride-th frame after a warm-up
+records, for each hash-grid candidate pair, the 13 features
+(fos_pred_features; feature 12 from the previous frame's JFA) and the label
+"j is in i's JFA neighbour list" (fos_pred_labels), one per scored pair
+(each unordered pair once).  Validation frames come from a foam of another
+seed (--holdout-seed; 0, the default, is the foam the benchmark runs, so it
+is never trained on).  The MLP is trained with the hand-written MFMA
+training step (fos_mlp_train_step: BF16 GEMM operands, FP32 accumulation,
+Adam in FP32) and evaluated with the BF16 MFMA inference kernel: pair
+accuracy and the per-frame FSC errors  sum|FSC_pred - FSC_jfa| / sum FSC_jfa
+(per particle) and  |sum FSC_pred - sum FSC_jfa| / sum FSC_jfa  (the frame's
+mean FSC).
 After training the output bias is shifted so the training frames' predicted
 positive rate equals their label rate.
 """
@@ -34,17 +38,29 @@ import fos_lib as fl         # noqa: E402
 from foam import FoamSim     # noqa: E402
 
 
-def collect(cfg, warm, frames, seed):
-    pos, rad = fi.for_config(cfg)
+def foam_inputs(cfg, seed):
+    """The workload's synthetic foam drawn with another seed (seed 0 is the
+    one the benchmark runs)."""
+    if seed == 0:
+        return fi.for_config(cfg)
+    if cfg.name in ("c5", "c5s"):
+        return fi.uniform(cfg.n, 0.005, 0.011, seed=seed)
+    raise KeyError(cfg.name)
+
+
+def collect(cfg, warm, frames, seed, stride=1):
+    """Frames warm, warm + stride, ... (frames of them) of the plain foam
+    drawn with `seed`."""
+    pos, rad = foam_inputs(cfg, seed)
     sim = FoamSim(cfg, pos, rad)
     n = cfg.n
     r_scale = 0.5 * (cfg.r_min + cfg.r_max)
-    pred = fp.FacePredictor(fp.init_params(seed), n, cfg.L, cfg.r_max, mult=cfg.pred_mult)
+    pred = fp.FacePredictor(fp.init_params(0), n, cfg.L, cfg.r_max, mult=cfg.pred_mult)
     lab = torch.empty(n * pred.cap, dtype=torch.float32, device="cuda")
     data = []
 
     def on_jfa(s):
-        if s.frame < warm:
+        if s.frame < warm or (s.frame - warm) % stride:
             pred.remember_jfa(s.neighbor_list, s.neighbor_count, n)
             return
         pred.candidates(s.pos, s.rad, n)
@@ -62,9 +78,11 @@ def collect(cfg, warm, frames, seed):
 
     sim.on_jfa = on_jfa
     t0 = time.perf_counter()
-    for f in range(warm + frames):
+    for f in range(warm + stride * (frames - 1) + 1):
         sim.step()
-        print(f"[collect] frame {f} ({time.perf_counter() - t0:.1f} s)", flush=True)
+        if f % 10 == 0:
+            print(f"[collect] seed {seed} frame {f} ({time.perf_counter() - t0:.1f} s)",
+                  flush=True)
     return data, pred
 
 
@@ -91,19 +109,33 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="c5")
     ap.add_argument("--warm", type=int, default=6)
-    ap.add_argument("--frames", type=int, default=20)
-    ap.add_argument("--holdout", type=int, default=4)
+    ap.add_argument("--frames", type=int, default=40, help="recorded frames per training foam")
+    ap.add_argument("--stride", type=int, default=2, help="frames between recorded frames")
+    ap.add_argument("--seeds", default="1,2,3,4,5,6",
+                    help="input seeds of the training foams")
+    ap.add_argument("--holdout-seed", type=int, default=0,
+                    help="input seed of the validation foam (0: the benchmark's)")
+    ap.add_argument("--holdout", type=int, default=8, help="recorded frames of that foam")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=os.path.join(HERE, "face_mlp_c5.npz"))
+    ap.add_argument("--baseline", default=None,
+                    help="weights (.npz) to evaluate on the same validation frames first")
     args = ap.parse_args()
     cfg = fc.preset(args.config)
     n = cfg.n
-    data, pred = collect(cfg, args.warm, args.frames, args.seed)
+    seeds = [int(x) for x in args.seeds.split(",") if x.strip()]
+    if args.holdout_seed in seeds:
+        raise SystemExit("the validation foam must not be a training foam")
+    train = []
+    for sd in seeds:
+        d, pred = collect(cfg, args.warm, args.frames, sd, args.stride)
+        train += d
+    hold, pred = collect(cfg, args.warm, args.holdout, args.holdout_seed, args.stride)
+    data = train + hold
     cap = pred.cap
-    train, hold = data[:-args.holdout], data[-args.holdout:]
     # each accepted pair appears in two neighbour rows and in one scored slot
     cover = 2 * sum(d["found"] for d in data) / max(1, sum(d["neighbours"] for d in data))
     X = torch.cat([d["feat"] for d in train])
@@ -114,6 +146,14 @@ def main():
     print(f"[train] {len(train)} frames, {valid.numel()} pairs, positives {npos / valid.numel():.3f},"
           f" neighbour coverage by candidates {cover:.4f}, candidates per particle "
           f"{data[-1]['cands'] / n:.1f}, truncated rows {data[-1]['trunc']}", flush=True)
+    base = None
+    if args.baseline:
+        bw = np.load(args.baseline, allow_pickle=False)
+        acc, err, bias = evaluate(bw["params"], hold, n, cap)
+        base = {"weights": os.path.basename(args.baseline), "holdout_acc": acc,
+                "holdout_fsc_err": err, "holdout_fsc_bias": bias}
+        print(f"[train] baseline {args.baseline}: holdout acc {acc:.4f} fsc err {err:.4f} "
+              f"bias {bias:.4f}", flush=True)
     tr = fp.Trainer(fp.init_params(args.seed), batch=args.batch, lr=args.lr, pos_weight=1.0)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(args.seed)
@@ -152,9 +192,12 @@ def main():
     print(f"[train] calibrated b3 by {-shift:.4f}: holdout acc {acc:.4f} fsc err {err:.4f} "
           f"bias {bias:.4f}", flush=True)
     meta = {"config": args.config, "mult": cfg.pred_mult, "cap": cap, "warm": args.warm,
-            "frames": args.frames, "holdout": args.holdout, "epochs": args.epochs,
+            "frames": len(train), "frames_per_foam": args.frames, "stride": args.stride,
+            "train_seeds": seeds, "holdout_seed": args.holdout_seed,
+            "holdout": args.holdout, "epochs": args.epochs,
             "batch": args.batch, "lr": args.lr, "seed": args.seed, "coverage": cover,
-            "negatives_per_positive": nneg / max(1.0, npos), "history": hist}
+            "negatives_per_positive": nneg / max(1.0, npos), "baseline": base,
+            "history": hist}
     np.savez(args.out, params=params, meta=np.array(json.dumps(meta)))
     print(json.dumps({"out": args.out, **hist[-1], "coverage": cover}), flush=True)
 
