@@ -205,6 +205,12 @@ struct fos_ctx {
   int *grid_tmp;          // [max_n] cell-ordered list before the stable placement
   float *pos_tmp;         // [max_n*3]
   float *rad_tmp;         // [max_n] pressure ping-pong
+  // pressure iterations of one launch: slot[k][i] = (tag << 32) | radius
+  // bits of particle i after iteration k, [FOS_FLOW_MAX_ITERS][flow_n]
+  unsigned long long *flow;
+  int flow_n;             // particles the slots hold: min(max_n, FOS_FLOW_MAX_N)
+  unsigned flow_tag;      // tag of the last such launch (0: none; slots start at 0)
+  int flow_blocks;        // co-resident blocks of that launch (0: not asked yet, -1: none)
   float *fscratch;        // [16] small device floats (reductions)
   int *iscratch;          // [16] small device ints
   // host pinned mirror for small reads
@@ -229,7 +235,13 @@ struct fos_inst {
 };
 
 #define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
+#define FOS_FLOW_MAX_ITERS 32  // pressure iterations one cooperative launch can run
+#define FOS_FLOW_MAX_N 32768   // particles it can hold (all groups co-resident)
+#define FOS_FLAG_FLOW 8        // ctx->flags slot: a pressure dependency wait timed out
 namespace fos {
+// pressure iterations in one cooperative launch (fos_set_param "pressure_flow")
+int pressure_flow_set(int on);
+int pressure_flow_get();
 // lanes per particle of the slot-record PBD pass (fos_set_param "pbd_lanes")
 int pbd_lanes_set(int lanes);
 int pbd_lanes_get();

@@ -202,7 +202,8 @@ int fos_ctx_destroy(fos_ctx *ctx) {
                   ctx->in_sorted,
                   ctx->acc_in,   ctx->scan_blocks, ctx->cell_write, ctx->cell_of,
                   ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch,
-                  ctx->chg[0],   ctx->chg[1],   ctx->grid_tmp, ctx->dist[0], ctx->dist[1]};
+                  ctx->chg[0],   ctx->chg[1],   ctx->grid_tmp, ctx->dist[0], ctx->dist[1],
+                  ctx->flow};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->host_i) (void)hipHostFree(ctx->host_i);
@@ -297,6 +298,8 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
   ALLOC(c->grid_tmp, (size_t)max_n * sizeof(int));
   ALLOC(c->pos_tmp, (size_t)max_n * 3 * sizeof(float));
   ALLOC(c->rad_tmp, (size_t)max_n * sizeof(float));
+  c->flow_n = max_n < FOS_FLOW_MAX_N ? max_n : FOS_FLOW_MAX_N;
+  ALLOC(c->flow, (size_t)FOS_FLOW_MAX_ITERS * c->flow_n * sizeof(unsigned long long));
   ALLOC(c->fscratch, 64 * sizeof(float));
   ALLOC(c->iscratch, 64 * sizeof(int));
 #undef ALLOC
@@ -313,6 +316,8 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
     return rc;
   }
   rc = fos_fill_i32(c->flags, 64, 0, nullptr);
+  if (!rc)
+    rc = fos_fill_i32((int32_t *)c->flow, (int64_t)2 * FOS_FLOW_MAX_ITERS * c->flow_n, 0, nullptr);
   if (rc) {
     fos_ctx_destroy(c);
     return rc;

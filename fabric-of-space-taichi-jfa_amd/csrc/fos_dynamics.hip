@@ -565,6 +565,116 @@ __global__ void __launch_bounds__(256)
                  p_exp, inv_p, cap, max_nei, r_min, r_max, stats);
 }
 
+// ---------------------------------------------------------------------------
+// All iterations of one fos_pressure_run call in one cooperative launch, with
+// no grid-wide barrier.  Particle i's radius after iteration k is published
+// in slot[k][i] = (tag << 32) | radius bits, written once per call (tag: the
+// call's number, never 0); iteration k + 1 of i waits only on the slots k of
+// its budgeted partners.  Value and tag travel in one 64-bit relaxed atomic,
+// so no ordering between locations is needed.  Every group is resident
+// (cooperative launch), so every wait is for a group that runs; each wait
+// also ends after FLOW_SPINS polls (and records flags[FOS_FLAG_FLOW]), so
+// the kernel always exits.  Slot 0 holds the input radii, so rad is read
+// and written only by its own particle's group.  The arithmetic is
+// pressure_group's, term for term and in the same order, so the result is
+// the per-iteration launches' bit for bit.
+// ---------------------------------------------------------------------------
+#define FLOW_SPINS (1u << 16)
+
+__device__ __forceinline__ void flow_put(unsigned long long *slot, unsigned tag, float r) {
+  const unsigned long long v = ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(r);
+  __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float flow_get(const unsigned long long *slot, unsigned tag,
+                                          int *err) {
+  unsigned long long v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned spins = 0;
+  while ((unsigned)(v >> 32) != tag) {
+    if (++spins > FLOW_SPINS) {
+      atomicOr(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return __uint_as_float((unsigned)v);
+}
+
+// 16 particles per 256-thread block, 16 lanes each (needs max_nei <= 16).
+__global__ void __launch_bounds__(256)
+    k_pressure_flow(int n, int frame, int iters, unsigned tag, float *__restrict__ rad,
+                    unsigned long long *__restrict__ slot, float *__restrict__ delta_r,
+                    const int32_t *__restrict__ face_ids, const int32_t *__restrict__ fsc,
+                    float k, float p_exp, float inv_p, float cap, int max_nei, float r_min,
+                    float r_max, float *stats, int *err) {
+  const int i = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+  const int q = (int)(threadIdx.x & 15);
+  const bool live = i < n;
+  // the partner of lane q, fixed for the call (pressure_group's selection)
+  const int32_t *row = face_ids + (int64_t)(live ? i : 0) * FOS_MAX_NEIGHBORS;
+  const int e_lo = live ? row[q] : -1, e_hi = live ? row[q + 16] : -1;
+  const int fc = live ? fsc[i] : 0;
+  const int an = fc < max_nei ? fc : max_nei;
+  const int idx = fc > 0 ? (int)(((unsigned)lcg_hash(i, frame) % (unsigned)fc + (unsigned)q) %
+                                 (unsigned)fc)
+                         : 0;
+  const int j_lo = __shfl(e_lo, idx & 15, 16), j_hi = __shfl(e_hi, idx & 15, 16);
+  int j = -1;
+  if (live && fc > 0 && q < an) {
+    j = idx < 16 ? j_lo : idx < FOS_MAX_NEIGHBORS ? j_hi : -1;
+    if (j >= n || j == i) j = -1;
+  }
+  float ri = live ? rad[i] : 0.0f;
+  if (live && q == 0) flow_put(slot + i, tag, ri);
+  float d = 0.0f;
+  for (int it = 0; it < iters; ++it) {
+    float term = 0.0f;
+    bool has = false;
+    if (j >= 0) {
+      const float rj = flow_get(slot + (int64_t)it * n + j, tag, err);
+      const float vi = vol_f32(ri, p_exp);
+      const float vj = vol_f32(rj, p_exp);
+      const float raw = k * (vi - vj);
+      const float vmin = fminf(vi, vj);
+      const float dv = fmaxf(-cap * vmin, fminf(cap * vmin, raw));
+      term = pow_f32(vi - dv, inv_p) - ri;
+      has = true;
+    }
+    // ordered sum over the 16-lane group; every lane gets the same sum
+    float acc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const float t = __shfl(term, s, 16);
+      const int h = __shfl((int)has, s, 16);
+      if (h) acc = acc + t;
+    }
+    d = acc;
+    ri = fmaxf(r_min, fminf(r_max, ri + d));
+    if (live && q == 0 && it + 1 < iters) flow_put(slot + (int64_t)(it + 1) * n + i, tag, ri);
+  }
+  if (!live || q != 0) return;
+  delta_r[i] = d;
+  if (stats) {
+    const float a = fabsf(d);
+    if (a > 1e-12f) {
+      atomicAdd((int *)&stats[1], 1);
+      atomicMax((int *)&stats[0], __float_as_int(a));
+    }
+  }
+  rad[i] = ri;
+}
+
+namespace fos {
+static int g_pressure_flow = 1;
+int pressure_flow_set(int on) {
+  FOS_REQUIRE(on == 0 || on == 1, "pressure_flow must be 0 or 1");
+  g_pressure_flow = on;
+  return 0;
+}
+int pressure_flow_get() { return g_pressure_flow; }
+}  // namespace fos
+
 // dynamics.py:1546-1557
 __global__ void k_pressure_apply(int n, float *__restrict__ rad,
                                  const float *__restrict__ delta_r, float r_min,
@@ -1000,6 +1110,25 @@ int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
   return 0;
 }
 
+// Blocks of k_pressure_flow the device holds at once (asked once per
+// context; -1 when the device cannot launch cooperatively).
+static int flow_capacity(fos_ctx *ctx) {
+  if (ctx->flow_blocks == 0) {
+    int coop = 0, cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pressure_flow, 256, 0) !=
+            hipSuccess ||
+        !coop || per_cu < 1)
+      ctx->flow_blocks = -1;
+    else
+      ctx->flow_blocks = per_cu * cus;
+  }
+  return ctx->flow_blocks;
+}
+
 int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
                      float *delta_r, const int32_t *face_ids, const int32_t *fsc, float k,
                      float p_exp, float pair_cap, int max_nei, float r_min, float r_max,
@@ -1013,6 +1142,27 @@ int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
     if (rc) return rc;
   }
   if (n == 0 || iters == 0) return 0;
+  // all iterations in one cooperative launch when every group fits on the
+  // chip at once (not under graph capture: the tag is a host counter)
+  hipStreamCaptureStatus cap_st = hipStreamCaptureStatusNone;
+  FOS_HIP(hipStreamIsCapturing(st, &cap_st));
+  const unsigned flow_grid = (unsigned)((n + 15) / 16);
+  if (g_pressure_flow && iters >= 2 && iters <= FOS_FLOW_MAX_ITERS && max_nei <= 16 &&
+      n <= ctx->flow_n && cap_st == hipStreamCaptureStatusNone &&
+      (int)flow_grid <= flow_capacity(ctx)) {
+    unsigned tag = ++ctx->flow_tag;
+    if (tag == 0) tag = ++ctx->flow_tag;
+    float inv_p = 1.0f / p_exp;
+    float *rad_a = rad;
+    const int32_t *fids = face_ids, *fs = fsc;
+    int *err = ctx->flags + FOS_FLAG_FLOW;
+    unsigned long long *slots = ctx->flow;
+    void *args[] = {&n, &frame, &iters, &tag, &rad_a, &slots, &delta_r, &fids, &fs, &k, &p_exp,
+                    &inv_p, &pair_cap, &max_nei, &r_min, &r_max, &stats, &err};
+    FOS_HIP(hipLaunchCooperativeKernel((const void *)k_pressure_flow, dim3(flow_grid), dim3(256),
+                                       args, 0, st));
+    return 0;
+  }
   float *a = rad, *b = ctx->rad_tmp;
   for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_pressure_step, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, st, n, frame,

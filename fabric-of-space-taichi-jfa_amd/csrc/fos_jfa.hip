@@ -1525,6 +1525,8 @@ int fos_set_param(const char *name, int value) {
     g_grid_block = value;
   } else if (!strcmp(name, "pbd_lanes")) {
     return pbd_lanes_set(value);
+  } else if (!strcmp(name, "pressure_flow")) {
+    return pressure_flow_set(value);
   } else if (!strcmp(name, "grid_range_cells")) {
     FOS_REQUIRE(value == 0 || (value >= 1 && value <= FOS_GR_CELLS),
                 "grid_range_cells must be 0 or in [1, %d]", FOS_GR_CELLS);
@@ -1547,6 +1549,8 @@ int fos_get_param(const char *name, int *value) {
     *value = g_emit_zw;
   else if (!strcmp(name, "pbd_lanes"))
     *value = pbd_lanes_get();
+  else if (!strcmp(name, "pressure_flow"))
+    *value = pressure_flow_get();
   else if (!strcmp(name, "grid_range_cells"))
     *value = g_grid_cpw;
   else
@@ -1937,6 +1941,9 @@ static int jfa_faces_impl(fos_ctx *ctx, int res, int n, const int32_t *filtered,
     int f[64];
     rc = read_flags(ctx, f, st);
     if (rc) return rc;
+    FOS_REQUIRE(!f[FOS_FLAG_FLOW],
+                "pressure: a wait of the one-launch iterations timed out (results since are "
+                "wrong); set pressure_flow to 0");
     if (f[0]) {
       rc = face_grow(ctx, st);
       if (rc) return rc;

@@ -58,6 +58,9 @@ const char *fos_last_error(void);
  *                      pass (fos_pbd_pass_cells with records,
  *                      fos_batch_pbd_pass); 0 (default): 32 below 131,072
  *                      particles, else 8
+ *   "pressure_flow"    0,1  1 (default): fos_pressure_run runs its iterations
+ *                      in one cooperative launch where it can; 0: one launch
+ *                      per iteration
  * They change speed only, never results.  No reference counterpart (Taichi
  * picks launch shapes itself). */
 int fos_set_param(const char *name, int value);
@@ -314,11 +317,16 @@ int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
                       float p_exp, float pair_cap, int max_nei, float r_min,
                       float r_max, float *stats, void *stream);
 /* `iters` calls of equilibrate_pressure (dynamics.py:1454-1557) at one frame,
- * each a Jacobi iteration, run as one kernel per iteration over a ping-pong
- * radius buffer from the context; the result ends in rad, delta_r holds the
- * last iteration's deltas and stats (device float[2], optional) its
+ * each a Jacobi iteration; the result ends in rad, delta_r holds the last
+ * iteration's deltas and stats (device float[2], optional) its
  * {max|dr|, changed count as int32 bits}.  The per-frame loop of run.py:755-771
- * repeated `iters` times. */
+ * repeated `iters` times.  With 2 <= iters <= 32, max_nei <= 16,
+ * n <= min(max_n, 32768), every group co-resident and the stream not
+ * capturing, all iterations run in one cooperative launch in which each
+ * particle waits only on its own partners' previous iteration (tunable
+ * "pressure_flow"); otherwise one kernel per iteration over a ping-pong
+ * buffer.  Both give the same bits.  Calls on one context must not
+ * overlap. */
 int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
                      float *delta_r, const int32_t *face_ids, const int32_t *fsc,
                      float k, float p_exp, float pair_cap, int max_nei, float r_min,

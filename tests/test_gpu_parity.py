@@ -6,6 +6,7 @@ neighbour lists, FSC, hash CSR) bit-exact; fp32 outputs bit-exact where the
 two sides evaluate the same correctly rounded operations in the same order
 (positions, radii, scores), otherwise within 1e-5 relative as stated per test.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -762,6 +763,57 @@ def test_pressure_run(gpu, orc, n, iters, max_nei):
     assert np.array_equal(host(tdr), dr)
     s = host(st)
     assert s.view(np.int32)[1] == ch and s[0] == pytest.approx(mx, rel=1e-6)
+    del ctx
+
+
+def test_pressure_one_launch_equals_oracle(gpu, orc):
+    """The one-launch pressure iterations (cooperative, per-partner waits)
+    give the oracle's radii bit for bit, call after call on one context:
+    a repeated call, a call with other radii and frame, a smaller n with a
+    ragged last block, two iterations, and the per-launch path beside it
+    (pressure_flow 0)."""
+    n, res = 3000, 64
+    voxel = fc.f32(1 / res)
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=11)
+    lab, _, _, _ = orc.jfa_run(pos, rad, n, res, 1.0, voxel, 8)
+    ids, cts = orc.accumulate_faces(orc.majority(lab, res), res, n)
+    _, _, _, _, fs = orc.finalize_faces(n, rad, voxel, ids, cts, np.zeros((n, 32), F))
+    ctx = gpu.Context(n, 1, 27)
+    v = ctypes.c_int()
+    gpu.call("fos_get_param", b"pressure_flow", ctypes.byref(v))
+    assert v.value == 1
+    rng = np.random.default_rng(4)
+    other = (rad * rng.uniform(0.9, 1.1, n)).astype(F)
+    cases = [(n, 5, 20, rad), (n, 5, 20, rad), (n, 9, 20, other), (1237, 2, 13, rad),
+             (n, 3, 2, rad), (n, 4, 32, rad)]
+    try:
+        for flow in (1, 0):
+            gpu.call("fos_set_param", b"pressure_flow", flow)
+            for m, frame, iters, r0 in cases:
+                # rows of the first m particles that point past m are skipped
+                # the same way by the oracle and the kernels (j >= n)
+                r = np.clip(r0[:m], fc.f32(0.02), fc.f32(0.04)).astype(F)
+                want = r.copy()
+                for _ in range(iters):
+                    want, dr, mx, ch = orc.pressure(m, frame, want, ids[:m], fs[:m],
+                                                    fc.f32(0.1), fc.f32(3.0), fc.f32(0.02), 10,
+                                                    fc.f32(0.02), fc.f32(0.04))
+                tr = dev(r)
+                tdr = torch.empty(m, dtype=torch.float32, device="cuda")
+                st = torch.zeros(2, dtype=torch.float32, device="cuda")
+                print(f"[pressure_flow] flow={flow} n={m} frame={frame} iters={iters}",
+                      flush=True)
+                gpu.call("fos_pressure_run", ctx.handle, m, frame, iters, gpu.ptr(tr),
+                         gpu.ptr(tdr), gpu.ptr(dev(ids[:m])), gpu.ptr(dev(fs[:m])), 0.1, 3.0,
+                         0.02, 10, 0.02, 0.04, gpu.ptr(st), gpu.stream())
+                assert np.array_equal(host(tr), want), (flow, m, frame, iters)
+                assert np.array_equal(host(tdr), dr), (flow, m, frame, iters)
+                s = host(st)
+                assert s.view(np.int32)[1] == ch and s[0] == pytest.approx(mx, rel=1e-6)
+    finally:
+        gpu.call("fos_set_param", b"pressure_flow", 1)
+    with pytest.raises(gpu.FosError):
+        gpu.call("fos_set_param", b"pressure_flow", 2)
     del ctx
 
 
