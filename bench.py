@@ -163,6 +163,9 @@ def parse():
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-frames", type=int, default=4)
+    ap.add_argument("--pressure-per-launch", action="store_true",
+                    help="one pressure kernel per iteration (fos_set_param pressure_flow 0), for "
+                         "counter runs under profilers that do not follow cooperative launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain copy of 400^3 int32 (counter scaling)")
     return ap.parse_args()
@@ -483,6 +486,9 @@ def main():
     import foam_config as fc
     import foam_inputs as fi
 
+    if args.pressure_per_launch:
+        import fos_lib as fl
+        fl.call("fos_set_param", b"pressure_flow", 0)
     if args.copy_calibration:
         copy_calibration()
     cfg = fc.preset("c2")
