@@ -164,16 +164,14 @@ def parse():
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-frames", type=int, default=4)
     ap.add_argument("--pressure-per-launch", action="store_true",
-                    help="one pressure kernel per iteration (fos_set_param pressure_flow 0), for "
-                         "counter runs under profilers that do not follow cooperative launches")
+                    help="one pressure kernel per iteration (fos_set_param pressure_flow 0)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain copy of 400^3 int32 (counter scaling)")
     return ap.parse_args()
 
 
 def run_frames(sim, frames):
-    for _ in range(frames):
-        sim.step()
+    sim.run(frames)
 
 
 EVENT_EVERY = 8   # timed frames between two with HIP events around every JFA pass

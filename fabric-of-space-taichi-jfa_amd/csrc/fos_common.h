@@ -211,6 +211,11 @@ struct fos_ctx {
   int flow_n;             // particles the slots hold: min(max_n, FOS_FLOW_MAX_N)
   unsigned flow_tag;      // tag of the last such launch (0: none; slots start at 0)
   int flow_blocks;        // co-resident blocks of that launch (0: not asked yet, -1: none)
+  // a face step queued without its host read (fos::face_step_defer): the
+  // flags' copy event, and the first burst's round count
+  void *spec_event;
+  int spec_pending;
+  int spec_first;
   float *fscratch;        // [16] small device floats (reductions)
   int *iscratch;          // [16] small device ints
   // host pinned mirror for small reads
@@ -235,11 +240,36 @@ struct fos_inst {
 };
 
 #define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
-#define FOS_FLOW_MAX_ITERS 32  // pressure iterations one cooperative launch can run
+#define FOS_FLOW_MAX_ITERS 32  // pressure iterations one launch can run
 #define FOS_FLOW_MAX_N 32768   // particles it can hold (all groups co-resident)
 #define FOS_FLAG_FLOW 8        // ctx->flags slot: a pressure dependency wait timed out
 namespace fos {
-// pressure iterations in one cooperative launch (fos_set_param "pressure_flow")
+// The arguments of fos_jfa_faces_controller.
+struct FaceStep {
+  int res, n;
+  const int32_t *filtered;
+  float *rad_target;
+  const float *rad;
+  float voxel;
+  int32_t *face_ids, *face_counts;
+  float *prev_scores;
+  int32_t *neighbor_list, *neighbor_count, *used_count, *fsc;
+  float *fsc_ema, alpha;
+  int frame, warmstart_frames;
+  double fsc_zero_rate;
+  int f_low, f_high;
+  float growth_pct, r_min, r_max, deadband;
+  float *gate;
+};
+// fos_jfa_faces_controller without its host read: the finalize's first
+// burst, then materialize and the gate held on the device unless the flags
+// show no row-table overflow and a fixed point.  face_step_finish (before
+// anything reads the step's results) waits for the flags and, when they
+// held the step back, finishes it on the host the way the synchronous call
+// would; the same bits either way.
+int face_step_defer(fos_ctx *ctx, const FaceStep &a, hipStream_t st);
+int face_step_finish(fos_ctx *ctx, const FaceStep &a, hipStream_t st);
+// pressure iterations in one launch (fos_set_param "pressure_flow")
 int pressure_flow_set(int on);
 int pressure_flow_get();
 // lanes per particle of the slot-record PBD pass (fos_set_param "pbd_lanes")

@@ -206,6 +206,7 @@ int fos_ctx_destroy(fos_ctx *ctx) {
                   ctx->flow};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (ctx->spec_event) (void)hipEventDestroy((hipEvent_t)ctx->spec_event);
   if (ctx->host_i) (void)hipHostFree(ctx->host_i);
   if (ctx->host_f) (void)hipHostFree(ctx->host_f);
   delete ctx;

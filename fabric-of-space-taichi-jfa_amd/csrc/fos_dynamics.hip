@@ -327,7 +327,10 @@ __global__ void k_gate_targets(const float *__restrict__ rad, float *__restrict_
                                const int32_t *__restrict__ fsc, const float *__restrict__ ema,
                                int n, int frame, int warmstart, double zero_rate,
                                const int *__restrict__ zero_cnt, float fl, float fh, float gp,
-                               float r_min, float r_max, float db, float *__restrict__ gate) {
+                               float r_min, float r_max, float db, float *__restrict__ gate,
+                               const int *__restrict__ hold_a, const int *__restrict__ hold_b) {
+  // a face step whose finalize must still be finished on the host
+  if ((hold_a && *hold_a) || (hold_b && *hold_b)) return;
   const double f0 = (double)*zero_cnt / (double)(n > 1 ? n : 1);
   const bool open = !(frame < warmstart || f0 > zero_rate);
   if (blockIdx.x == 0 && threadIdx.x == 0) *gate = open ? 1.0f : 0.0f;
@@ -362,10 +365,12 @@ namespace fos {
 int gate_targets(const float *rad, float *rad_target, const int32_t *fsc, const float *fsc_ema,
                  int n, int frame, int warmstart, double zero_rate, const int *zero_cnt,
                  int f_low, int f_high, float growth_pct, float r_min, float r_max,
-                 float deadband, float *gate, hipStream_t st) {
+                 float deadband, float *gate, hipStream_t st, const int *hold_a,
+                 const int *hold_b) {
   hipLaunchKernelGGL(k_gate_targets, dim3(n > 0 ? blocks_for(n) : 1), dim3(FOS_BLOCK), 0, st,
                      rad, rad_target, fsc, fsc_ema, n, frame, warmstart, zero_rate, zero_cnt,
-                     (float)f_low, (float)f_high, growth_pct, r_min, r_max, deadband, gate);
+                     (float)f_low, (float)f_high, growth_pct, r_min, r_max, deadband, gate,
+                     hold_a, hold_b);
   FOS_CHECK_LAUNCH("k_gate_targets");
   return 0;
 }
@@ -566,15 +571,19 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// All iterations of one fos_pressure_run call in one cooperative launch, with
-// no grid-wide barrier.  Particle i's radius after iteration k is published
+// All iterations of one fos_pressure_run call in one launch, with no
+// grid-wide barrier.  Particle i's radius after iteration k is published
 // in slot[k][i] = (tag << 32) | radius bits, written once per call (tag: the
 // call's number, never 0); iteration k + 1 of i waits only on the slots k of
 // its budgeted partners.  Value and tag travel in one 64-bit relaxed atomic,
-// so no ordering between locations is needed.  Every group is resident
-// (cooperative launch), so every wait is for a group that runs; each wait
-// also ends after FLOW_SPINS polls (and records flags[FOS_FLAG_FLOW]), so
-// the kernel always exits.  Slot 0 holds the input radii, so rad is read
+// so no ordering between locations is needed.  The grid fits the device at
+// once (checked against the occupancy limit before the launch) and the
+// stream runs nothing beside it, so every group becomes resident and every
+// wait is for a group that runs.  Each wait also ends after FLOW_SPINS polls
+// (recording flags[FOS_FLAG_FLOW], which the next face step's host read
+// turns into an error), so the kernel always exits.  (A cooperative launch
+// would guarantee the residency too, but its dispatch costs more than the
+// launches it saves.)  Slot 0 holds the input radii, so rad is read
 // and written only by its own particle's group.  The arithmetic is
 // pressure_group's, term for term and in the same order, so the result is
 // the per-iteration launches' bit for bit.
@@ -586,9 +595,14 @@ __device__ __forceinline__ void flow_put(unsigned long long *slot, unsigned tag,
   __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ float flow_get(const unsigned long long *slot, unsigned tag,
-                                          int *err) {
-  unsigned long long v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned long long flow_peek(const unsigned long long *slot) {
+  return __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The radius in `slot` once it carries `tag`; v is a value read from the
+// slot earlier (polled again while its tag is an older one).
+__device__ __forceinline__ float flow_get(unsigned long long v, const unsigned long long *slot,
+                                          unsigned tag, int *err) {
   unsigned spins = 0;
   while ((unsigned)(v >> 32) != tag) {
     if (++spins > FLOW_SPINS) {
@@ -596,7 +610,7 @@ __device__ __forceinline__ float flow_get(const unsigned long long *slot, unsign
       break;
     }
     __builtin_amdgcn_s_sleep(1);
-    v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = flow_peek(slot);
   }
   return __uint_as_float((unsigned)v);
 }
@@ -628,27 +642,26 @@ __global__ void __launch_bounds__(256)
   float ri = live ? rad[i] : 0.0f;
   if (live && q == 0) flow_put(slot + i, tag, ri);
   float d = 0.0f;
+  unsigned long long v = j >= 0 ? flow_peek(slot + j) : 0ull;
   for (int it = 0; it < iters; ++it) {
+    // a lane without a partner adds +0, which leaves the sum's bits alone
+    // (the sum starts at +0 and is never -0)
     float term = 0.0f;
-    bool has = false;
     if (j >= 0) {
-      const float rj = flow_get(slot + (int64_t)it * n + j, tag, err);
+      const float rj = flow_get(v, slot + (int64_t)it * n + j, tag, err);
+      // the next iteration's read is issued before this one's arithmetic
+      if (it + 1 < iters) v = flow_peek(slot + (int64_t)(it + 1) * n + j);
       const float vi = vol_f32(ri, p_exp);
       const float vj = vol_f32(rj, p_exp);
       const float raw = k * (vi - vj);
       const float vmin = fminf(vi, vj);
       const float dv = fmaxf(-cap * vmin, fminf(cap * vmin, raw));
       term = pow_f32(vi - dv, inv_p) - ri;
-      has = true;
     }
-    // ordered sum over the 16-lane group; every lane gets the same sum
+    // ordered sum over the group's first max_nei lanes (no other lane has
+    // a partner); every lane gets the same sum
     float acc = 0.0f;
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const float t = __shfl(term, s, 16);
-      const int h = __shfl((int)has, s, 16);
-      if (h) acc = acc + t;
-    }
+    for (int s = 0; s < max_nei; ++s) acc = acc + __shfl(term, s, 16);
     d = acc;
     ri = fmaxf(r_min, fminf(r_max, ri + d));
     if (live && q == 0 && it + 1 < iters) flow_put(slot + (int64_t)(it + 1) * n + i, tag, ri);
@@ -1111,17 +1124,15 @@ int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
 }
 
 // Blocks of k_pressure_flow the device holds at once (asked once per
-// context; -1 when the device cannot launch cooperatively).
+// context; -1 when the query fails).
 static int flow_capacity(fos_ctx *ctx) {
   if (ctx->flow_blocks == 0) {
-    int coop = 0, cus = 0, per_cu = 0;
-    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device) !=
-            hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
             hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pressure_flow, 256, 0) !=
             hipSuccess ||
-        !coop || per_cu < 1)
+        per_cu < 1)
       ctx->flow_blocks = -1;
     else
       ctx->flow_blocks = per_cu * cus;
@@ -1142,8 +1153,8 @@ int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
     if (rc) return rc;
   }
   if (n == 0 || iters == 0) return 0;
-  // all iterations in one cooperative launch when every group fits on the
-  // chip at once (not under graph capture: the tag is a host counter)
+  // all iterations in one launch when every group fits on the chip at once
+  // (not under graph capture: the tag is a host counter)
   hipStreamCaptureStatus cap_st = hipStreamCaptureStatusNone;
   FOS_HIP(hipStreamIsCapturing(st, &cap_st));
   const unsigned flow_grid = (unsigned)((n + 15) / 16);
@@ -1157,10 +1168,10 @@ int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
     const int32_t *fids = face_ids, *fs = fsc;
     int *err = ctx->flags + FOS_FLAG_FLOW;
     unsigned long long *slots = ctx->flow;
-    void *args[] = {&n, &frame, &iters, &tag, &rad_a, &slots, &delta_r, &fids, &fs, &k, &p_exp,
-                    &inv_p, &pair_cap, &max_nei, &r_min, &r_max, &stats, &err};
-    FOS_HIP(hipLaunchCooperativeKernel((const void *)k_pressure_flow, dim3(flow_grid), dim3(256),
-                                       args, 0, st));
+    hipLaunchKernelGGL(k_pressure_flow, dim3(flow_grid), dim3(256), 0, st, n, frame, iters, tag,
+                       rad_a, slots, delta_r, fids, fs, k, p_exp, inv_p, pair_cap, max_nei, r_min,
+                       r_max, stats, err);
+    FOS_CHECK_LAUNCH("k_pressure_flow");
     return 0;
   }
   float *a = rad, *b = ctx->rad_tmp;

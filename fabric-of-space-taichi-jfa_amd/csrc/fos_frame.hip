@@ -17,6 +17,37 @@ bool jfa_runs(const fos_frame_desc *d, int frame) {
   return (frame - d->jfa_warm) % every == 0;
 }
 
+// The face step's arguments for frame `frame` (fos_jfa_faces_controller).
+fos::FaceStep face_step(const fos_frame_desc *d, const fos_frame_bufs *b, int frame) {
+  fos::FaceStep a;
+  a.res = d->res;
+  a.n = d->n;
+  a.filtered = b->filtered;
+  a.rad_target = b->rad_target;
+  a.rad = b->rad;
+  a.voxel = d->voxel;
+  a.face_ids = b->face_ids;
+  a.face_counts = b->face_counts;
+  a.prev_scores = b->prev_scores;
+  a.neighbor_list = b->neighbor_list;
+  a.neighbor_count = b->neighbor_count;
+  a.used_count = b->used_count;
+  a.fsc = b->fsc;
+  a.fsc_ema = b->fsc_ema;
+  a.alpha = d->ema_alpha;
+  a.frame = frame;
+  a.warmstart_frames = d->warmstart_frames;
+  a.fsc_zero_rate = d->fsc_zero_rate;
+  a.f_low = d->f_low;
+  a.f_high = d->f_high;
+  a.growth_pct = d->growth_pct;
+  a.r_min = d->r_min;
+  a.r_max = d->r_max;
+  a.deadband = d->deadband;
+  a.gate = b->maxd + 1;
+  return a;
+}
+
 }  // namespace
 
 #define FOS_CALL(x)             \
@@ -25,10 +56,11 @@ bool jfa_runs(const fos_frame_desc *d, int frame) {
     if (_rc) return _rc;        \
   } while (0)
 
-extern "C" {
-
-int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
-                   fos_frame_state *s, void *const *pass_events, void *stream) {
+// One frame.  defer: the face step goes without its host read, which the
+// next frame makes once it has queued the work that does not read the faces
+// (the rebuild and the overlap maximum).
+static int frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
+                      fos_frame_state *s, void *const *pass_events, void *stream, bool defer) {
   FOS_REQUIRE(ctx && d && b && s && d->n >= 1 && d->n <= ctx->max_n && d->res >= 1 &&
                   d->res <= ctx->max_res && d->G >= 3 &&
                   (int64_t)d->G * d->G * d->G <= ctx->max_cells && b->maxd_host &&
@@ -51,6 +83,9 @@ int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *
                             stream));
   FOS_CALL(fos_max_overlap(pos, b->rad, n, G, L, b->cell_start, b->cell_count, b->cell_indices,
                            b->lmd, b->maxd, stream));
+  // the last frame's deferred face step is finished before anything reads
+  // its faces, FSC, targets or gate
+  if (ctx->spec_pending) FOS_CALL(fos::face_step_finish(ctx, face_step(d, b, f - 1), st));
   // the overlap maximum (and the last gated frame's gate) go to the host
   // behind an event; settle, pressure and noise need neither and are queued
   // before the host waits
@@ -140,11 +175,14 @@ int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *
     FOS_CALL(fos_majority(d->res, b->grid, b->filtered, stream));
     // face rows, finalize and the FSC EMA, then 9: the warm-start gate and
     // the gated targets on the device
-    FOS_CALL(fos_jfa_faces_controller(
-        ctx, d->res, n, b->filtered, b->rad_target, b->rad, d->voxel, b->face_ids,
-        b->face_counts, b->prev_scores, b->neighbor_list, b->neighbor_count, b->used_count,
-        b->fsc, b->fsc_ema, d->ema_alpha, f, d->warmstart_frames, d->fsc_zero_rate, d->f_low,
-        d->f_high, d->growth_pct, d->r_min, d->r_max, d->deadband, b->maxd + 1, stream));
+    if (defer)
+      FOS_CALL(fos::face_step_defer(ctx, face_step(d, b, f), st));
+    else
+      FOS_CALL(fos_jfa_faces_controller(
+          ctx, d->res, n, b->filtered, b->rad_target, b->rad, d->voxel, b->face_ids,
+          b->face_counts, b->prev_scores, b->neighbor_list, b->neighbor_count, b->used_count,
+          b->fsc, b->fsc_ema, d->ema_alpha, f, d->warmstart_frames, d->fsc_zero_rate, d->f_low,
+          d->f_high, d->growth_pct, d->r_min, d->r_max, d->deadband, b->maxd + 1, stream));
     s->pending = 1;
     s->settle_gate = 1;
   } else {
@@ -156,6 +194,24 @@ int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *
   s->max_depth = max_depth;
   s->jfa_ran = ran ? 1 : 0;
   s->frame = f + 1;
+  return 0;
+}
+
+extern "C" {
+
+int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
+                   fos_frame_state *s, void *const *pass_events, void *stream) {
+  return frame_step(ctx, d, b, s, pass_events, stream, false);
+}
+
+int fos_frame_run(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
+                  fos_frame_state *s, int frames, void *const *const *pass_events,
+                  void *stream) {
+  FOS_REQUIRE(ctx && frames >= 0 && s, "fos_frame_run: bad arguments (frames=%d)", frames);
+  // every frame but the last leaves its face step's host read to the next
+  for (int k = 0; k < frames; ++k)
+    FOS_CALL(frame_step(ctx, d, b, s, pass_events ? pass_events[k] : nullptr, stream,
+                        k + 1 < frames));
   return 0;
 }
 

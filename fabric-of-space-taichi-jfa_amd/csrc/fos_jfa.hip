@@ -8,10 +8,13 @@
 namespace fos {
 int fill_u32(unsigned *p, int64_t n, unsigned v, hipStream_t st);
 int alloc_tables(fos_ctx *ctx, int cap);
+// hold_a / hold_b (may be NULL): device words; when either is set the gate
+// and the targets are left alone (a deferred face step is not final yet).
 int gate_targets(const float *rad, float *rad_target, const int32_t *fsc, const float *fsc_ema,
                  int n, int frame, int warmstart, double zero_rate, const int *zero_cnt,
                  int f_low, int f_high, float growth_pct, float r_min, float r_max,
-                 float deadband, float *gate, hipStream_t st);
+                 float deadband, float *gate, hipStream_t st, const int *hold_a = nullptr,
+                 const int *hold_b = nullptr);
 }  // namespace fos
 
 using namespace fos;
@@ -1065,7 +1068,11 @@ __global__ void __launch_bounds__(256)
                       const int *__restrict__ in_sorted, float *__restrict__ prev,
                       int32_t *__restrict__ nl, int32_t *__restrict__ nc,
                       int32_t *__restrict__ used_count, int32_t *__restrict__ fsc,
-                      float *__restrict__ ema, float alpha, int *__restrict__ zero_cnt) {
+                      float *__restrict__ ema, float alpha, int *__restrict__ zero_cnt,
+                      const int *__restrict__ hold_a, const int *__restrict__ hold_b) {
+  // a deferred face step that the host must still finish: the whole block
+  // leaves before any barrier
+  if ((hold_a && *hold_a) || (hold_b && *hold_b)) return;
   __shared__ int s_zero;
   __shared__ int s_time[8][FOS_MAX_NEIGHBORS];
   __shared__ float s_val[8][FOS_MAX_NEIGHBORS];
@@ -1349,6 +1356,9 @@ int fos_jfa_init(fos_ctx *ctx, const float *pos, const float *rad, int n,
 
 static int g_emit_tx = 8;
 static int g_emit_zw = 32;
+// fos_set_param "fin_first_burst": rounds of the finalize's first burst
+// (0, default: what the last finalize needed, 2 to 16)
+static int g_fin_first = 0;
 extern int g_grid_block;  // fos_grid.hip
 extern int g_grid_cpw;    // fos_grid.hip
 
@@ -1527,6 +1537,9 @@ int fos_set_param(const char *name, int value) {
     return pbd_lanes_set(value);
   } else if (!strcmp(name, "pressure_flow")) {
     return pressure_flow_set(value);
+  } else if (!strcmp(name, "fin_first_burst")) {
+    FOS_REQUIRE(value == 0 || (value >= 2 && value <= 16), "fin_first_burst must be 0 or in [2, 16]");
+    g_fin_first = value;
   } else if (!strcmp(name, "grid_range_cells")) {
     FOS_REQUIRE(value == 0 || (value >= 1 && value <= FOS_GR_CELLS),
                 "grid_range_cells must be 0 or in [1, %d]", FOS_GR_CELLS);
@@ -1551,6 +1564,8 @@ int fos_get_param(const char *name, int *value) {
     *value = pbd_lanes_get();
   else if (!strcmp(name, "pressure_flow"))
     *value = pressure_flow_get();
+  else if (!strcmp(name, "fin_first_burst"))
+    *value = g_fin_first;
   else if (!strcmp(name, "grid_range_cells"))
     *value = g_grid_cpw;
   else
@@ -1823,14 +1838,17 @@ static int fin_ring_clear(fos_ctx *ctx, int keep, hipStream_t st) {
 
 // ema (may be NULL): also the FSC EMA update with alpha; zero_cnt (may be
 // NULL, zero before): also counts the rows whose FSC is 0.
+// hold_a / hold_b (may be NULL): device words; when either is set the
+// kernel does nothing (a deferred face step the host must still finish).
 static int fin_materialize(fos_ctx *ctx, int n, const int32_t *face_ids, float *prev_scores,
                            int32_t *neighbor_list, int32_t *neighbor_count, int32_t *used_count,
                            int32_t *fsc, hipStream_t st, float *ema = nullptr, float alpha = 0.0f,
-                           int *zero_cnt = nullptr) {
+                           int *zero_cnt = nullptr, const int *hold_a = nullptr,
+                           const int *hold_b = nullptr) {
   hipLaunchKernelGGL(k_fin_materialize, dim3((unsigned)((n + 7) / 8)), dim3(256), 0, st, n,
                      face_ids, ctx->score, ctx->cls, (const unsigned *)ctx->dec, ctx->in_start,
                      ctx->in_sorted, prev_scores, neighbor_list, neighbor_count, used_count, fsc,
-                     ema, alpha, zero_cnt);
+                     ema, alpha, zero_cnt, hold_a, hold_b);
   FOS_CHECK_LAUNCH("k_fin_materialize");
   return 0;
 }
@@ -1857,6 +1875,7 @@ static int fin_needed(const int *f, int last) {
 // The first burst's size: what the last finalize needed (consecutive frames
 // need about the same), 2 to 16 rounds.
 static int fin_first_burst(const fos_ctx *ctx) {
+  if (g_fin_first) return g_fin_first;
   const int b = ctx->fin_burst > 0 ? ctx->fin_burst : FIN_BURST;
   return b < 2 ? 2 : b > 16 ? 16 : b;
 }
@@ -1998,6 +2017,79 @@ int fos_jfa_faces_controller(fos_ctx *ctx, int res, int n, const int32_t *filter
 }
 
 }  // extern "C"
+
+namespace fos {
+
+int face_step_defer(fos_ctx *ctx, const FaceStep &a, hipStream_t st) {
+  FOS_REQUIRE(!ctx->spec_pending, "face_step_defer: the previous deferred face step is open");
+  int *zero_cnt = ctx->flags + 6;
+  int rc = face_emit(ctx, a.res, a.n, a.filtered, st);
+  if (!rc) rc = face_compact(ctx, a.n, a.face_ids, a.face_counts, st, true, zero_cnt);
+  if (!rc)
+    rc = fin_prepare(ctx, a.n, a.rad, a.voxel, nullptr, a.face_ids, a.face_counts, a.prev_scores,
+                     st, true);
+  const int first = fin_first_burst(ctx);
+  if (!rc) rc = fin_rounds(ctx, a.n, 0, first, false, a.face_ids, st);
+  if (rc) return rc;
+  if (!ctx->spec_event) {
+    hipEvent_t ev;
+    FOS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    ctx->spec_event = (void *)ev;
+  }
+  FOS_HIP(hipMemcpyAsync(ctx->host_i, ctx->flags, 64 * sizeof(int), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipEventRecord((hipEvent_t)ctx->spec_event, st));
+  // held while the tables overflowed or the burst's last round changed a row
+  const int *ovf = ctx->flags, *more = fin_chg(ctx) + (first - 1) % FIN_RING;
+  rc = fin_materialize(ctx, a.n, a.face_ids, a.prev_scores, a.neighbor_list, a.neighbor_count,
+                       a.used_count, a.fsc, st, a.fsc_ema, a.alpha, zero_cnt, ovf, more);
+  if (!rc)
+    rc = gate_targets(a.rad, a.rad_target, a.fsc, a.fsc_ema, a.n, a.frame, a.warmstart_frames,
+                      a.fsc_zero_rate, zero_cnt, a.f_low, a.f_high, a.growth_pct, a.r_min,
+                      a.r_max, a.deadband, a.gate, st, ovf, more);
+  if (rc) return rc;
+  ctx->spec_pending = 1;
+  ctx->spec_first = first;
+  return 0;
+}
+
+int face_step_finish(fos_ctx *ctx, const FaceStep &a, hipStream_t st) {
+  if (!ctx->spec_pending) return 0;
+  ctx->spec_pending = 0;
+  FOS_HIP(hipEventSynchronize((hipEvent_t)ctx->spec_event));
+  int f[64];
+  const volatile int *h = (const volatile int *)ctx->host_i;
+  for (int k = 0; k < 64; ++k) f[k] = h[k];
+  FOS_REQUIRE(!f[FOS_FLAG_FLOW],
+              "pressure: a wait of the one-launch iterations timed out (results since are "
+              "wrong); set pressure_flow to 0");
+  const int first = ctx->spec_first;
+  int rc = 0;
+  if (f[0]) {
+    // row-table overflow: regrow and run the whole step (materialize and the
+    // gate did nothing, so its inputs are as they were)
+    rc = face_grow(ctx, st);
+    if (!rc)
+      rc = jfa_faces_impl(ctx, a.res, a.n, a.filtered, a.rad, a.voxel, a.face_ids, a.face_counts,
+                          a.prev_scores, a.neighbor_list, a.neighbor_count, a.used_count, a.fsc,
+                          a.fsc_ema, a.alpha, st);
+  } else if (f[16 + (first - 1) % FIN_RING]) {
+    // short of the fixed point: the rounds continue from the burst
+    rc = fin_rounds_host(ctx, a.n, first, FIN_BURST, f, a.face_ids, st);
+    if (!rc)
+      rc = fin_materialize(ctx, a.n, a.face_ids, a.prev_scores, a.neighbor_list,
+                           a.neighbor_count, a.used_count, a.fsc, st, a.fsc_ema, a.alpha,
+                           ctx->flags + 6);
+  } else {
+    ctx->fin_burst = fin_needed(f, first - 1);
+    return 0;
+  }
+  if (rc) return rc;
+  return gate_targets(a.rad, a.rad_target, a.fsc, a.fsc_ema, a.n, a.frame, a.warmstart_frames,
+                      a.fsc_zero_rate, ctx->flags + 6, a.f_low, a.f_high, a.growth_pct, a.r_min,
+                      a.r_max, a.deadband, a.gate, st);
+}
+
+}  // namespace fos
 
 extern "C" {
 

@@ -325,6 +325,50 @@ class FoamSim:
         self.frame += 1
         return self.last
 
+    def run(self, frames):
+        """`frames` step()s; returns the last frame's record.  On the native
+        path (no predictor, hook or telemetry) one fos_frame_run call issues
+        them all from C++ -- the same calls in the same order -- so the host
+        queues the next frame while the GPU finishes the last; only the last
+        frame's record is kept."""
+        if (frames <= 1 or not self.native or self.predictor is not None
+                or self.on_jfa is not None or self.telemetry):
+            last = None
+            for _ in range(frames):
+                last = self.step()
+            return last
+        # the growth of the last gated frame, read now (the run reads the
+        # following ones itself)
+        self.resolve()
+        s = self._fstate
+        if self._fdesc is None:
+            self._fdesc, self._fbufs = self._frame_desc(), self._frame_bufs()
+        f0 = self.frame
+        s.frame, s.pos_cur, s.sidx = f0, self._cur, self.sidx
+        s.settle_force, s.settle_gate = int(self.settle_force), int(self.settle_gate)
+        s.need_rebuild = int(self.cfg.L != 1.0 or self.grid_key != self.grid_token())
+        s.pending = 0
+        evs = [self.pass_events.pop(0) if self.pass_events else None for _ in range(frames)]
+        arr = (ctypes.c_void_p * frames)(
+            *[None if e is None else ctypes.cast(e, ctypes.c_void_p).value for e in evs])
+        fl.call("fos_frame_run", self.ctx.handle, ctypes.byref(self._fdesc),
+                ctypes.byref(self._fbufs), ctypes.byref(s), frames, arr, fl.stream())
+        if s.gate_read:
+            self.growth = float(s.gate) > 0.5
+        self._cur, self.sidx = s.pos_cur, s.sidx
+        self.settle_force, self.settle_gate = bool(s.settle_force), bool(s.settle_gate)
+        self.grid_key = self.grid_token()
+        ran = bool(s.jfa_ran)
+        self.last = FrameInfo(self, frame=f0 + frames - 1, pbd_passes=int(s.pbd_passes),
+                              max_depth=float(s.max_depth), jfa=ran,
+                              growth=None if ran else True)
+        if ran:
+            self.pending = self.last
+        else:
+            self.growth = True
+        self.frame = f0 + frames
+        return self.last
+
     def step(self):
         if self.native and self.predictor is None and self.on_jfa is None:
             return self._step_native()

@@ -114,6 +114,7 @@ SIGNATURES = {
     "fos_mlp_param_count": [ctypes.POINTER(I)],
     "fos_mlp_train_step": [P, P, P, P, I, F, P, P, P, I, F, P, P, P],
     "fos_frame_step": [P, P, P, P, P, P],
+    "fos_frame_run": [P, P, P, P, I, P, P],
     "fos_frame_release": [P],
 }
 

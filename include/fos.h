@@ -59,8 +59,11 @@ const char *fos_last_error(void);
  *                      fos_batch_pbd_pass); 0 (default): 32 below 131,072
  *                      particles, else 8
  *   "pressure_flow"    0,1  1 (default): fos_pressure_run runs its iterations
- *                      in one cooperative launch where it can; 0: one launch
- *                      per iteration
+ *                      in one launch where it can; 0: one launch per
+ *                      iteration
+ *   "fin_first_burst"  0,2..16  finalize rounds queued before the first host
+ *                      read of the fixed-point flags (0, default: as many as
+ *                      the context's last finalize needed)
  * They change speed only, never results.  No reference counterpart (Taichi
  * picks launch shapes itself). */
 int fos_set_param(const char *name, int value);
@@ -321,12 +324,12 @@ int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
  * iteration's deltas and stats (device float[2], optional) its
  * {max|dr|, changed count as int32 bits}.  The per-frame loop of run.py:755-771
  * repeated `iters` times.  With 2 <= iters <= 32, max_nei <= 16,
- * n <= min(max_n, 32768), every group co-resident and the stream not
- * capturing, all iterations run in one cooperative launch in which each
- * particle waits only on its own partners' previous iteration (tunable
- * "pressure_flow"); otherwise one kernel per iteration over a ping-pong
- * buffer.  Both give the same bits.  Calls on one context must not
- * overlap. */
+ * n <= min(max_n, 32768), a grid the device holds at once and the stream
+ * not capturing, all iterations run in one launch in which each particle
+ * waits only on its own partners' previous iteration (tunable
+ * "pressure_flow"; the stream must run nothing concurrently with it);
+ * otherwise one kernel per iteration over a ping-pong buffer.  Both give
+ * the same bits.  Calls on one context must not overlap. */
 int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
                      float *delta_r, const int32_t *face_ids, const int32_t *fsc,
                      float k, float p_exp, float pair_cap, int max_nei, float r_min,
@@ -431,6 +434,14 @@ typedef struct fos_frame_state {
 } fos_frame_state;
 int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
                    fos_frame_state *s, void *const *pass_events, void *stream);
+/* `frames` calls of fos_frame_step in a loop, frame s->frame onward, with
+ * pass_events[k] (pass_events or its entries may be NULL) for frame k of the
+ * run.  The same calls in the same order, so the same bits; the host issues
+ * the next frame while the GPU still runs the end of this one.  State after
+ * the run as after its last step.  No reference counterpart. */
+int fos_frame_run(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
+                  fos_frame_state *s, int frames, void *const *const *pass_events,
+                  void *stream);
 /* Frees what fos_frame_step created in the state (its host-read event). */
 int fos_frame_release(fos_frame_state *s);
 

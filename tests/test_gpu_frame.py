@@ -282,6 +282,84 @@ def test_runs_are_deterministic(gpu):
                 assert np.array_equal(sa[k], sb[k]), (run.__name__, k)
 
 
+def shell_foam(n=300, big=0.15, small=0.012):
+    """One large particle at the centre and n - 1 small ones on a shell that
+    touches it (Fibonacci points): row 0 meets far more face partners than
+    a fresh context's 64 row-table slots."""
+    k = np.arange(n - 1, dtype=np.float64) + 0.5
+    phi = np.arccos(1.0 - 2.0 * k / (n - 1))
+    th = np.pi * (1.0 + 5.0 ** 0.5) * k
+    u = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
+    pos = np.concatenate([np.zeros((1, 3)), (big + small) * u]).astype(F)
+    rad = np.full(n, small, F)
+    rad[0] = big
+    return fi.wrap(pos), rad
+
+
+@pytest.mark.parametrize("variant", ["c2", "noise_cadence", "short_burst", "table_growth"])
+def test_frame_run_equals_steps(gpu, variant):
+    """FoamSim.run(k) (one fos_frame_run call, each frame's face step left
+    to the next frame to finish) against k step() calls: every state array
+    and the last record bit for bit, across runs of several lengths with the
+    warm-start gate opening inside one, then step() after a run.
+    short_burst: two finalize rounds before the flags are read, so the
+    deferred steps finish their rounds on the host; table_growth: a fresh
+    context's face-row tables overflow on the first frame, which a deferred
+    step redoes after growing them."""
+    from foam import FoamSim
+    if variant == "c2":
+        cfg = fc.preset("c2")
+        cfg.warmstart_frames = 4
+        pos, rad = fi.for_config(cfg)
+    elif variant == "table_growth":
+        cfg = small_cfg(n=300, res=64)
+        cfg.r_min, cfg.r_max = 0.005, 0.2
+        pos, rad = shell_foam()
+    else:
+        cfg = small_cfg(brownian=True, brownian_mode=1, brownian_strength=1e-3, ou_jitter=True,
+                        jfa_every=3, jfa_warm=2)
+        cfg.r_min, cfg.r_max = 0.02, 0.07
+        pos, rad = fi.uniform(cfg.n, 0.03, 0.06, seed=34)
+    if variant == "short_burst":
+        gpu.call("fos_set_param", b"fin_first_burst", 2)
+    try:
+        a, b = FoamSim(cfg, pos, rad), FoamSim(cfg, pos, rad)
+        run_and_compare(a, b, variant)
+    finally:
+        gpu.call("fos_set_param", b"fin_first_burst", 0)
+    if variant == "table_growth":
+        # the first frame's grid gives particle 0 more than 64 distinct face
+        # partners, the row-table slots of a fresh context
+        c = FoamSim(cfg, pos, rad)
+        c.step()
+        g = c.filtered.cpu().numpy().reshape(cfg.res, cfg.res, cfg.res)
+        partners = set()
+        for ax in range(3):
+            h = np.roll(g, -1, axis=ax)
+            partners |= set(h[g == 0].tolist()) | set(g[h == 0].tolist())
+        partners.discard(0)
+        partners.discard(-1)
+        assert len(partners) > 64, len(partners)
+
+
+def run_and_compare(a, b, variant):
+    for k in (3, 1, 5, 2):
+        print(f"[frame run] {variant} run of {k} from frame {a.frame}", flush=True)
+        ra = a.run(k)
+        for _ in range(k):
+            rb = b.step()
+        for key in ("frame", "pbd_passes", "max_depth", "jfa", "growth"):
+            assert ra[key] == rb[key], (k, key, ra[key], rb[key])
+        sa, sb = a.snapshot(), b.snapshot()
+        for key in sa:
+            assert np.array_equal(sa[key], sb[key]), (k, key)
+    ra, rb = a.step(), b.step()
+    assert ra["growth"] == rb["growth"] and ra["pbd_passes"] == rb["pbd_passes"]
+    sa, sb = a.snapshot(), b.snapshot()
+    for key in sa:
+        assert np.array_equal(sa[key], sb[key]), key
+
+
 def test_integration_snippet_runs(gpu, orc, keep_config):
     """The drop-in snippet of INTEGRATION.md section 2, executed as written
     (REPO set to this checkout): its run_jfa FSC equals the oracle's on the

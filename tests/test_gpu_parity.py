@@ -767,7 +767,7 @@ def test_pressure_run(gpu, orc, n, iters, max_nei):
 
 
 def test_pressure_one_launch_equals_oracle(gpu, orc):
-    """The one-launch pressure iterations (cooperative, per-partner waits)
+    """The one-launch pressure iterations (per-partner waits, no barrier)
     give the oracle's radii bit for bit, call after call on one context:
     a repeated call, a call with other radii and frame, a smaller n with a
     ragged last block, two iterations, and the per-launch path beside it
