@@ -33,11 +33,15 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, default=None, help="default: the preset's frames")
     ap.add_argument("--predictor", default=None,
                     help="weights .npz of the learned face predictor (presets with judge_every)")
+    ap.add_argument("--judge-on", choices=("per_particle", "mean"), default=None,
+                    help="what the predictor's judge bounds (default: the preset's judge_on)")
     ap.add_argument("--telemetry", action="store_true")
     ap.add_argument("--csv", default=None, help="write ID,X,Y,Z,Radius,FSC at the end")
     ap.add_argument("--save", default=None, help="write the state snapshot (.npz) at the end")
     args = ap.parse_args(argv)
     cfg = fc.preset(args.config)
+    if args.judge_on:
+        cfg.judge_on = args.judge_on
     frames = cfg.frames if args.frames is None else args.frames
     pos, rad = fi.for_config(cfg)
     sim = FoamSim(cfg, pos, rad, telemetry=args.telemetry)
@@ -48,11 +52,13 @@ def main(argv=None):
         sim.set_predictor(w["params"], json.loads(str(w["meta"]))["mult"])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(frames):
-        sim.step()
-        if args.telemetry:
+    if args.telemetry:
+        for _ in range(frames):
+            sim.step()
             for line in sim.log_lines():
                 print(line, flush=True)
+    else:
+        sim.run(frames)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     if args.csv:
