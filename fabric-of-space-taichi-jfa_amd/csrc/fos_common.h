@@ -210,7 +210,8 @@ struct fos_ctx {
   unsigned long long *flow;
   int flow_n;             // particles the slots hold: min(max_n, FOS_FLOW_MAX_N)
   unsigned flow_tag;      // tag of the last such launch (0: none; slots start at 0)
-  int flow_blocks;        // co-resident blocks of that launch (0: not asked yet, -1: none)
+  int flow_blocks;        // co-resident blocks of the 16-lane form (0: not asked yet, -1: none)
+  int flow_blocks_lane;   // the same for the one-lane form
   // a face step queued without its host read (fos::face_step_defer): the
   // flags' copy event, and the first burst's round count
   void *spec_event;
@@ -241,7 +242,7 @@ struct fos_inst {
 
 #define FOS_BATCH_PASSES 16  // pass counter slots per foam of a batch
 #define FOS_FLOW_MAX_ITERS 32  // pressure iterations one launch can run
-#define FOS_FLOW_MAX_N 32768   // particles it can hold (all groups co-resident)
+#define FOS_FLOW_MAX_N 131072  // particles its slots hold
 #define FOS_FLAG_FLOW 8        // ctx->flags slot: a pressure dependency wait timed out
 namespace fos {
 // The arguments of fos_jfa_faces_controller.

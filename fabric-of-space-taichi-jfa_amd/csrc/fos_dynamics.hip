@@ -678,6 +678,82 @@ __global__ void __launch_bounds__(256)
   rad[i] = ri;
 }
 
+// pressure_group's term for a particle of radius ri and a partner of
+// radius rj (the same operations in the same order).
+__device__ __forceinline__ float pressure_term(float ri, float rj, float k, float p_exp,
+                                               float inv_p, float cap) {
+  const float vi = vol_f32(ri, p_exp);
+  const float vj = vol_f32(rj, p_exp);
+  const float raw = k * (vi - vj);
+  const float vmin = fminf(vi, vj);
+  const float dv = fmaxf(-cap * vmin, fminf(cap * vmin, raw));
+  return pow_f32(vi - dv, inv_p) - ri;
+}
+
+// One thread per particle (needs max_nei <= FLOW_MAXB), for foams too large
+// for the 16-lane form to be resident at once: the budgeted
+// partners are fixed for the call, so their slots are read together, their
+// terms evaluated independently and summed in rotation order (the order of
+// pressure_group's lanes; a missing partner adds +0, which leaves the sum's
+// bits alone: the sum starts at +0 and is never -0).
+#define FLOW_MAXB 16
+__global__ void __launch_bounds__(256)
+    k_pressure_flow_lane(int n, int frame, int iters, unsigned tag, float *__restrict__ rad,
+                    unsigned long long *__restrict__ slot, float *__restrict__ delta_r,
+                    const int32_t *__restrict__ face_ids, const int32_t *__restrict__ fsc,
+                    float k, float p_exp, float inv_p, float cap, int max_nei, float r_min,
+                    float r_max, float *stats, int *err) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  // the budget of pressure_group: partners row[(h + t) % fc], t < an
+  const int fc = fsc[i];
+  const int an = fc < max_nei ? fc : max_nei;
+  const unsigned h = fc > 0 ? (unsigned)lcg_hash(i, frame) % (unsigned)fc : 0u;
+  const int32_t *row = face_ids + (int64_t)i * FOS_MAX_NEIGHBORS;
+  int pj[FLOW_MAXB];
+#pragma unroll
+  for (int t = 0; t < FLOW_MAXB; ++t) {
+    int p = -1;
+    if (t < an) {
+      const unsigned ix = (h + (unsigned)t) % (unsigned)fc;
+      p = ix < FOS_MAX_NEIGHBORS ? row[ix] : -1;
+      if (p >= n || p == i) p = -1;
+    }
+    pj[t] = p;
+  }
+  float ri = rad[i];
+  flow_put(slot + i, tag, ri);
+  float d = 0.0f;
+  for (int it = 0; it < iters; ++it) {
+    const unsigned long long *sk = slot + (int64_t)it * n;
+    unsigned long long v[FLOW_MAXB];
+#pragma unroll
+    for (int t = 0; t < FLOW_MAXB; ++t) v[t] = pj[t] >= 0 ? flow_peek(sk + pj[t]) : 0ull;
+    float term[FLOW_MAXB];
+#pragma unroll
+    for (int t = 0; t < FLOW_MAXB; ++t) {
+      term[t] = 0.0f;
+      if (pj[t] >= 0)
+        term[t] = pressure_term(ri, flow_get(v[t], sk + pj[t], tag, err), k, p_exp, inv_p, cap);
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < FLOW_MAXB; ++t) acc = acc + term[t];
+    d = acc;
+    ri = fmaxf(r_min, fminf(r_max, ri + d));
+    if (it + 1 < iters) flow_put(slot + (int64_t)(it + 1) * n + i, tag, ri);
+  }
+  delta_r[i] = d;
+  if (stats) {
+    const float a = fabsf(d);
+    if (a > 1e-12f) {
+      atomicAdd((int *)&stats[1], 1);
+      atomicMax((int *)&stats[0], __float_as_int(a));
+    }
+  }
+  rad[i] = ri;
+}
+
 namespace fos {
 static int g_pressure_flow = 1;
 int pressure_flow_set(int on) {
@@ -1123,21 +1199,20 @@ int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
   return 0;
 }
 
-// Blocks of k_pressure_flow the device holds at once (asked once per
-// context; -1 when the query fails).
-static int flow_capacity(fos_ctx *ctx) {
-  if (ctx->flow_blocks == 0) {
+// Blocks of a flow kernel the device holds at once (asked once per context;
+// -1 when the query fails).
+static int flow_capacity(fos_ctx *ctx, int &cached, const void *kernel) {
+  if (cached == 0) {
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
             hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pressure_flow, 256, 0) !=
-            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
         per_cu < 1)
-      ctx->flow_blocks = -1;
+      cached = -1;
     else
-      ctx->flow_blocks = per_cu * cus;
+      cached = per_cu * cus;
   }
-  return ctx->flow_blocks;
+  return cached;
 }
 
 int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
@@ -1155,12 +1230,19 @@ int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
   if (n == 0 || iters == 0) return 0;
   // all iterations in one launch when every group fits on the chip at once
   // (not under graph capture: the tag is a host counter)
+  // (16 lanes per particle where the grid fits the device at once, else one
+  // lane per particle: measured faster respectively at N = 10,000 and 50,000)
   hipStreamCaptureStatus cap_st = hipStreamCaptureStatusNone;
   FOS_HIP(hipStreamIsCapturing(st, &cap_st));
-  const unsigned flow_grid = (unsigned)((n + 15) / 16);
-  if (g_pressure_flow && iters >= 2 && iters <= FOS_FLOW_MAX_ITERS && max_nei <= 16 &&
-      n <= ctx->flow_n && cap_st == hipStreamCaptureStatusNone &&
-      (int)flow_grid <= flow_capacity(ctx)) {
+  const unsigned grid16 = (unsigned)((n + 15) / 16), grid1 = (unsigned)((n + 255) / 256);
+  const bool flow_ok = g_pressure_flow && iters >= 2 && iters <= FOS_FLOW_MAX_ITERS &&
+                       max_nei <= 16 && n <= ctx->flow_n && cap_st == hipStreamCaptureStatusNone;
+  const bool use16 =
+      flow_ok && (int)grid16 <= flow_capacity(ctx, ctx->flow_blocks, (const void *)k_pressure_flow);
+  const bool use1 =
+      flow_ok && !use16 &&
+      (int)grid1 <= flow_capacity(ctx, ctx->flow_blocks_lane, (const void *)k_pressure_flow_lane);
+  if (use16 || use1) {
     unsigned tag = ++ctx->flow_tag;
     if (tag == 0) tag = ++ctx->flow_tag;
     float inv_p = 1.0f / p_exp;
@@ -1168,9 +1250,14 @@ int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
     const int32_t *fids = face_ids, *fs = fsc;
     int *err = ctx->flags + FOS_FLAG_FLOW;
     unsigned long long *slots = ctx->flow;
-    hipLaunchKernelGGL(k_pressure_flow, dim3(flow_grid), dim3(256), 0, st, n, frame, iters, tag,
-                       rad_a, slots, delta_r, fids, fs, k, p_exp, inv_p, pair_cap, max_nei, r_min,
-                       r_max, stats, err);
+    if (use16)
+      hipLaunchKernelGGL(k_pressure_flow, dim3(grid16), dim3(256), 0, st, n, frame, iters, tag,
+                         rad_a, slots, delta_r, fids, fs, k, p_exp, inv_p, pair_cap, max_nei,
+                         r_min, r_max, stats, err);
+    else
+      hipLaunchKernelGGL(k_pressure_flow_lane, dim3(grid1), dim3(256), 0, st, n, frame, iters,
+                         tag, rad_a, slots, delta_r, fids, fs, k, p_exp, inv_p, pair_cap, max_nei,
+                         r_min, r_max, stats, err);
     FOS_CHECK_LAUNCH("k_pressure_flow");
     return 0;
   }

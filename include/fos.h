@@ -324,11 +324,12 @@ int fos_pressure_iter(int n, int frame, float *rad, float *delta_r,
  * iteration's deltas and stats (device float[2], optional) its
  * {max|dr|, changed count as int32 bits}.  The per-frame loop of run.py:755-771
  * repeated `iters` times.  With 2 <= iters <= 32, max_nei <= 16,
- * n <= min(max_n, 32768), a grid the device holds at once and the stream
+ * n <= min(max_n, 131072), a grid the device holds at once and the stream
  * not capturing, all iterations run in one launch in which each particle
- * waits only on its own partners' previous iteration (tunable
- * "pressure_flow"; the stream must run nothing concurrently with it);
- * otherwise one kernel per iteration over a ping-pong buffer.  Both give
+ * waits only on its own partners' previous iteration (16 lanes per particle,
+ * or one for foams too large for that; tunable "pressure_flow"; the stream
+ * must run nothing concurrently with it); otherwise one kernel per iteration
+ * over a ping-pong buffer.  Both give
  * the same bits.  Calls on one context must not overlap. */
 int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
                      float *delta_r, const int32_t *face_ids, const int32_t *fsc,

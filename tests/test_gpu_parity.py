@@ -817,6 +817,46 @@ def test_pressure_one_launch_equals_oracle(gpu, orc):
     del ctx
 
 
+def test_pressure_one_launch_large_foam(gpu, orc):
+    """A foam above what the 16-lane one-launch form holds at once (N =
+    40,000: one lane per particle) against the oracle and the per-launch
+    path, on synthetic face rows with skipped entries (-1, self) and
+    budgets from 0 to the 32-entry row."""
+    n = 40000
+    rng = np.random.default_rng(21)
+    fs = rng.integers(0, 33, n).astype(np.int32)
+    fs[::97] = 0
+    ids = np.full((n, 32), -1, np.int32)
+    for i in range(n):
+        c = int(fs[i])
+        ids[i, :c] = rng.choice(n, c, replace=False)
+    ids[::13, 0] = np.arange(0, n, 13)      # a self entry
+    ids[5::17, 1] = -1                      # a hole inside a row
+    rad = rng.uniform(0.02, 0.04, n).astype(F)
+    want = rad.copy()
+    for _ in range(20):
+        want, dr, mx, ch = orc.pressure(n, 3, want, ids, fs, fc.f32(0.1), fc.f32(3.0),
+                                        fc.f32(0.02), 10, fc.f32(0.02), fc.f32(0.04))
+    ctx = gpu.Context(n, 1, 27)
+    try:
+        for flow in (1, 0):
+            gpu.call("fos_set_param", b"pressure_flow", flow)
+            tr = dev(rad)
+            tdr = torch.empty(n, dtype=torch.float32, device="cuda")
+            st = torch.zeros(2, dtype=torch.float32, device="cuda")
+            print(f"[pressure_flow large] flow={flow}", flush=True)
+            gpu.call("fos_pressure_run", ctx.handle, n, 3, 20, gpu.ptr(tr), gpu.ptr(tdr),
+                     gpu.ptr(dev(ids)), gpu.ptr(dev(fs)), 0.1, 3.0, 0.02, 10, 0.02, 0.04,
+                     gpu.ptr(st), gpu.stream())
+            assert np.array_equal(host(tr), want), flow
+            assert np.array_equal(host(tdr), dr), flow
+            s = host(st)
+            assert s.view(np.int32)[1] == ch and s[0] == pytest.approx(mx, rel=1e-6)
+    finally:
+        gpu.call("fos_set_param", b"pressure_flow", 1)
+    del ctx
+
+
 def test_noise_kernels(gpu, orc):
     n = 10000
     pos, rad = fi.uniform(n, 0.012, 0.024, seed=0)
