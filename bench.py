@@ -525,8 +525,9 @@ def main():
                      "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)],
                      "timed_frames_with_events": len(pass_ms),
                      "copy_measured_gbs": copy_bandwidth(),
-                     "limiter": "instruction issue: VALU 60-80% and texture-address 63-68% busy "
-                                "(profiles/k_jfa_pass_list_c2_pmc.md)"},
+                     "limiter": "instruction issue and latency at full occupancy: VALU 56-70% and "
+                                "texture-address 63-65% busy, waves waiting on a dependency 35-42% "
+                                "of their cycles (profiles/k_jfa_pass_list_c2_pmc.md)"},
         "gvox_pass_per_s": round(gvox, 2),
         "jfa_ms_per_frame": round(float(flat.sum(axis=1).mean()), 4),
     }
