@@ -142,27 +142,6 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-// Rebuild for small systems (n <= 16256, cells <= 16384) in two launches: the
-// multi-kernel path is launch-latency bound there (five dependent launches
-// of a few microseconds each).  k_grid_cells computes every particle's cell
-// across the chip; one workgroup then does the rest, keeping everything
-// between reading the cells and writing the three outputs in LDS as 16-bit
-// values: cell counters packed two per word (a 32-bit atomic add of
-// 1 << 16*(c&1) bumps one half; halves never carry since counts and offsets
-// stay <= n < 2^16) and the cell-ordered particle list.  Phases: count,
-// exclusive scan, scatter in arbitrary order within a cell, then each
-// particle writes itself at its cell's start plus its rank by index.
-#define GRID_BLOCK_C 16384
-#define GRID_BLOCK_N 16256
-#define GRID_BLOCK_PER ((GRID_BLOCK_N + 1023) / 1024)
-
-__device__ __forceinline__ int half_get(const unsigned *w, int c) {
-  return (int)((w[c >> 1] >> ((c & 1) * 16)) & 0xFFFFu);
-}
-__device__ __forceinline__ void half_set(unsigned *w, int c, int v) {
-  ((unsigned short *)w)[c] = (unsigned short)v;
-}
-
 // Cell of every particle, wrapped positions written back (grid.py:202-222 and
 // the wrap of grid.py:285-309), spread over many workgroups.
 __global__ void k_grid_cells(float *__restrict__ pos, int n, int G, float L, float half_l,
@@ -177,91 +156,6 @@ __global__ void k_grid_cells(float *__restrict__ pos, int n, int G, float L, flo
   cell_of[i] = c;
 }
 
-__global__ void __launch_bounds__(1024)
-    k_grid_build_block(const int32_t *__restrict__ cell_of, int n, int C,
-                       int32_t *__restrict__ count, int32_t *__restrict__ start,
-                       int32_t *__restrict__ idx) {
-  __shared__ unsigned s_cell[GRID_BLOCK_C / 2];
-  __shared__ unsigned short s_list[GRID_BLOCK_N];
-  __shared__ int s_wave[16];
-  const int t = (int)threadIdx.x, lane = t & 63, wid = t >> 6;
-  for (int k = t; k < (C + 1) / 2; k += 1024) s_cell[k] = 0u;
-  int cell[GRID_BLOCK_PER];
-#pragma unroll
-  for (int j = 0; j < GRID_BLOCK_PER; ++j) cell[j] = cell_of[min(j * 1024 + t, n - 1)];
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < GRID_BLOCK_PER; ++j) {
-    if (j * 1024 + t >= n) cell[j] = -1;
-    const int c = cell[j];
-    if (c >= 0) atomicAdd(&s_cell[c >> 1], 1u << ((c & 1) * 16));
-  }
-  __syncthreads();
-  for (int k = t; k < C; k += 1024) count[k] = half_get(s_cell, k);
-  // exclusive scan over contiguous runs of at most GRID_BLOCK_C / 1024 cells
-  const int per = (C + 1023) >> 10;
-  const int b = t * per;
-  int v[GRID_BLOCK_C / 1024];
-  int sum = 0;
-#pragma unroll
-  for (int j = 0; j < GRID_BLOCK_C / 1024; ++j) {
-    const int k = b + j;
-    const int raw = half_get(s_cell, min(k, C - 1));
-    v[j] = (j < per && k < C) ? raw : 0;
-    sum += v[j];
-  }
-  int x = sum;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) s_wave[wid] = x;
-  __syncthreads();
-  if (wid == 0) {
-    int w = lane < 16 ? s_wave[lane] : 0;
-    for (int o = 1; o < 16; o <<= 1) {
-      const int y = __shfl_up(w, o, 64);
-      if (lane >= o) w += y;
-    }
-    if (lane < 16) s_wave[lane] = w;
-  }
-  __syncthreads();
-  int ex = (wid ? s_wave[wid - 1] : 0) + x - sum;
-#pragma unroll
-  for (int j = 0; j < GRID_BLOCK_C / 1024; ++j) {
-    const int k = b + j;
-    if (j < per && k < C) half_set(s_cell, k, ex);
-    ex += v[j];
-  }
-  __syncthreads();
-  for (int k = t; k < C; k += 1024) start[k] = half_get(s_cell, k);
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < GRID_BLOCK_PER; ++j) {
-    const int c = cell[j];
-    if (c >= 0) {
-      const unsigned old = atomicAdd(&s_cell[c >> 1], 1u << ((c & 1) * 16));
-      s_list[(old >> ((c & 1) * 16)) & 0xFFFFu] = (unsigned short)(j * 1024 + t);
-    }
-  }
-  __syncthreads();
-  // Each particle's final slot: its cell's first slot plus the number of
-  // smaller indices in the cell (after the scatter each half holds its
-  // cell's end; cells are contiguous, so the start is the previous end).
-#pragma unroll
-  for (int j = 0; j < GRID_BLOCK_PER; ++j) {
-    const int c = cell[j];
-    if (c >= 0) {
-      const int i = j * 1024 + t;
-      const int lo = c ? half_get(s_cell, c - 1) : 0, hi = half_get(s_cell, c);
-      int r = lo;
-      for (int q = lo; q < hi; ++q) r += s_list[q] < i ? 1 : 0;
-      idx[r] = i;
-    }
-  }
-}
-
-// Single-workgroup rebuild on or off ("grid_block" in fos_set_param).
 // Rebuild by cell ranges: workgroup w owns cells [w*cpw, (w+1)*cpw) (at
 // most GR_CELLS).  It reads every particle's cell, counts its own cells and
 // the particles in earlier cells (its global offset), scans its counts,
@@ -467,7 +361,7 @@ __global__ void __launch_bounds__(1024)
 
 // 0: multi-kernel rebuild; 1: one workgroup where it fits (else 2);
 // 2: cell-range workgroups.
-int g_grid_block = 2;
+int g_grid_block = 1;  // fos_set_param "grid_block": 1 cell ranges, 0 the kernel chain
 // cells per range workgroup of a single-foam rebuild (0: fos_range_cells)
 int g_grid_cpw = 0;
 
@@ -534,13 +428,6 @@ int fos_grid_build_from_cells(fos_ctx *ctx, const float *pos, const float *rad, 
               "fos_grid_build_from_cells: bad arguments (n=%d G=%d)", n, G);
   const int C = G * G * G;
   hipStream_t st = (hipStream_t)stream;
-  if (n >= 1 && n <= GRID_BLOCK_N && C <= GRID_BLOCK_C && g_grid_block == 1) {
-    hipLaunchKernelGGL(k_grid_build_block, dim3(1), dim3(1024), 0, st,
-                       (const int32_t *)ctx->cell_of, n, C, cell_count, cell_start,
-                       cell_indices);
-    FOS_CHECK_LAUNCH("k_grid_build_block");
-    return grid_records(pos, rad, n, cell_indices, cell_pr, st);
-  }
   const int cpw = range_cells(C);
   const int wgs = (C + cpw - 1) / cpw;
   hipLaunchKernelGGL(k_grid_build_ranges, dim3((unsigned)wgs), dim3(1024), 0, st,
@@ -556,7 +443,7 @@ int fos_grid_build_pr(fos_ctx *ctx, float *pos, const float *rad, int n, int G, 
                       float *cell_pr, void *stream) {
   FOS_REQUIRE(!cell_pr || n == 0 || rad, "fos_grid_build_pr: NULL rad");
   const int C = G * G * G;
-  if (cell_pr && n > 0 && g_grid_block == 2 && ctx && G >= 3 &&
+  if (cell_pr && n > 0 && g_grid_block == 1 && ctx && G >= 3 &&
       (int64_t)C <= ctx->max_cells && n <= ctx->max_n && pos && cell_indices && cell_count &&
       cell_start) {
     // the range form writes the records as it places the slots
@@ -580,8 +467,7 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
               "fos_grid_build: bad arguments (n=%d G=%d)", n, G);
   const int C = G * G * G;
   hipStream_t st = (hipStream_t)stream;
-  const bool one_block = n >= 1 && n <= GRID_BLOCK_N && C <= GRID_BLOCK_C && g_grid_block == 1;
-  if (g_grid_block && !one_block) {
+  if (g_grid_block) {
     FOS_REQUIRE(n == 0 || (cell_indices && pos), "fos_grid_build: NULL pos or cell_indices");
     if (n > 0) {
       hipLaunchKernelGGL(k_grid_cells, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, n, G,
@@ -595,17 +481,6 @@ int fos_grid_build(fos_ctx *ctx, float *pos, int n, int G, float L,
                        cell_indices, (const float *)nullptr, (const float *)nullptr,
                        (float4 *)nullptr);
     FOS_CHECK_LAUNCH("k_grid_build_ranges");
-    return 0;
-  }
-  if (one_block) {
-    FOS_REQUIRE(cell_indices && pos, "fos_grid_build: NULL pos or cell_indices");
-    hipLaunchKernelGGL(k_grid_cells, dim3(blocks_for(n)), dim3(FOS_BLOCK), 0, st, pos, n, G, L,
-                       (float)(0.5 * (double)L), (float)(1.0 / (double)L), ctx->cell_of);
-    FOS_CHECK_LAUNCH("k_grid_cells");
-    hipLaunchKernelGGL(k_grid_build_block, dim3(1), dim3(1024), 0, st,
-                       (const int32_t *)ctx->cell_of, n, C, cell_count, cell_start,
-                       cell_indices);
-    FOS_CHECK_LAUNCH("k_grid_build_block");
     return 0;
   }
   int rc = fos_fill_i32(cell_count, C, 0, stream);

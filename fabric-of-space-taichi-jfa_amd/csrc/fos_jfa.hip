@@ -1531,7 +1531,7 @@ int fos_set_param(const char *name, int value) {
     FOS_REQUIRE(value == 16 || value == 32 || value == 64, "face_emit_zw must be 16, 32 or 64");
     g_emit_zw = value;
   } else if (!strcmp(name, "grid_block")) {
-    FOS_REQUIRE(value == 0 || value == 1 || value == 2, "grid_block must be 0, 1 or 2");
+    FOS_REQUIRE(value == 0 || value == 1, "grid_block must be 0 or 1");
     g_grid_block = value;
   } else if (!strcmp(name, "pbd_lanes")) {
     return pbd_lanes_set(value);

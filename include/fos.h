@@ -48,10 +48,9 @@ const char *fos_last_error(void);
  *                      many x-planes in LDS (default 8)
  *   "face_emit_zw"     16,32,64  z-extent of the emission tile (its y-extent
  *                      is 256 / zw rows; default 32)
- *   "grid_block"       0,1,2  hash-grid rebuild: 0 count/scan/scatter/rank
- *                      kernels; 1 one workgroup when n <= 16256 and cells
- *                      <= 16384, else as 2; 2 one workgroup per range of
- *                      cells, independent of each other (default)
+ *   "grid_block"       0,1  hash-grid rebuild: 1 (default) one workgroup
+ *                      per range of cells, independent of each other; 0 the
+ *                      count/scan/scatter/rank kernels
  *   "grid_range_cells" 0..1024  cells per workgroup of the range rebuild of
  *                      one foam (0, default: C/192 clamped to [64, 1024])
  *   "pbd_lanes"        0,8,16,32  lanes per particle of the slot-record PBD
