@@ -250,18 +250,42 @@ __device__ __forceinline__ int jfa_list_eval(const int lab[27], float cx, float 
   return best;
 }
 
+// Plane order of a pass of step `step`: the i-th x-plane processed is the
+// k-th of residue class r (x = r + k*step), classes in turn.  Consecutive
+// planes of a class share two of their three source planes (x - step, x,
+// x + step), so each source plane is read about once while it is in the
+// L2 of the XCD that processes the class, instead of again `step` planes
+// later.  (Class r < res % step holds res / step + 1 planes, the rest
+// res / step.)  The divisions by q = res / step >= 2 and q + 1 use the
+// host's magic numbers mq = ceil(2^32 / q), mq1 = ceil(2^32 / (q + 1)):
+// (i * m) >> 32 is the exact quotient for every i < 8192.
+__device__ __forceinline__ int jfa_plane(int i, int step, int qr, unsigned mq, unsigned mq1) {
+  const int q = qr >> 16, rem = qr & 0xffff;
+  const int big = rem * (q + 1);
+  if (i < big) {
+    const int r = (int)__umulhi((unsigned)i, mq1);
+    return r + (i - r * (q + 1)) * step;
+  }
+  const int j = i - big;
+  const int r = (int)__umulhi((unsigned)j, mq);
+  return rem + r + (j - r * q) * step;
+}
+
 // One tile (4 z-rows of 64 voxels at one x, this thread's wave one row) of a
 // list pass; returns this wave's change ballot (0 for an idle row).  The
-// caller has checked that the pass is active.
+// caller has checked that the pass is active.  CLASS: tiles take their
+// x-planes in jfa_plane order.
 // DM: 0 no distance buffers; 1 writes each voxel's winning power distance to
 // dist_out; 2 also starts from dist_in (the previous pass's dist_out).
-template <bool UNIT_L, int MODE, typename LT = int, int BATCH = JFA_BATCH, int DM = 0>
+template <bool UNIT_L, int MODE, typename LT = int, int BATCH = JFA_BATCH, int DM = 0,
+          bool CLASS = false>
 __device__ __forceinline__ unsigned long long jfa_list_tile(
     const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel, int step,
     const int32_t *__restrict__ in, int32_t *__restrict__ out, int tile, int nyb, int nzb,
     int ntiles, const unsigned long long *__restrict__ chg_prev,
     unsigned long long *__restrict__ chg_out, LT (*s_list)[256], int base = 0,
-    const float *__restrict__ dist_in = nullptr, float *__restrict__ dist_out = nullptr) {
+    const float *__restrict__ dist_in = nullptr, float *__restrict__ dist_out = nullptr,
+    int qr = 0, unsigned mq = 0, unsigned mq1 = 0) {
   const int t = (int)threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   int changed = 0;
@@ -271,7 +295,7 @@ __device__ __forceinline__ unsigned long long jfa_list_tile(
     zb = tile % nzb;
     const int t2 = tile / nzb;
     const int yb = t2 % nyb;
-    x = t2 / nyb;
+    x = CLASS ? jfa_plane(t2 / nyb, step, qr, mq, mq1) : t2 / nyb;
     y = yb * 4 + w;
     row_live = y < res;
     const int z = zb * 64 + lane;
@@ -380,7 +404,8 @@ __device__ __forceinline__ unsigned long long jfa_flat_tile(
   return __ballot(changed);
 }
 
-template <bool UNIT_L, int MODE, typename LT, int BATCH = JFA_BATCH, int DM = 0>
+template <bool UNIT_L, int MODE, typename LT, int BATCH = JFA_BATCH, int DM = 0,
+          bool CLASS = false>
 __global__ void __launch_bounds__(256)
     k_jfa_pass_list(const float4 *__restrict__ seeds, int res, float L, float half_l,
                     float voxel, int step, const int32_t *__restrict__ in,
@@ -388,13 +413,13 @@ __global__ void __launch_bounds__(256)
                     double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd,
                     const unsigned long long *__restrict__ chg_prev,
                     unsigned long long *__restrict__ chg_out, const float *__restrict__ dist_in,
-                    float *__restrict__ dist_out) {
+                    float *__restrict__ dist_out, int qr, unsigned mq, unsigned mq1) {
   __shared__ LT s_list[JFA_LIST_ROWS(BATCH)][256];
   if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
   const int tile = xcd_tile(blockIdx.x, per_xcd);
-  const unsigned long long bal = jfa_list_tile<UNIT_L, MODE, LT, BATCH, DM>(
+  const unsigned long long bal = jfa_list_tile<UNIT_L, MODE, LT, BATCH, DM, CLASS>(
       seeds, res, L, half_l, voxel, step, in, out, tile, nyb, nzb, ntiles, chg_prev, chg_out,
-      s_list, 0, dist_in, dist_out);
+      s_list, 0, dist_in, dist_out, qr, mq, mq1);
   block_count(bal, stripes, pass);
 }
 
@@ -1365,14 +1390,26 @@ extern int g_grid_cpw;    // fos_grid.hip
 // @jfa-pass-launch-begin
 // fos_set_param "jfa_variant": bit 0 evaluates four list entries per
 // iteration (two otherwise), bit 1 carries each voxel's winning power
-// distance into the next pass (the incumbent is not evaluated again); -1
-// (default) picks four entries for 32-bit lists (ids above 65,535: the
-// large grids, where it was measured faster) and the distance carry
-// whenever the context holds the distance buffers (grids up to 256^3).
+// distance into the next pass (the incumbent is not evaluated again), bit 2
+// takes the x-planes in residue-class order; -1 (default) picks four
+// entries for 32-bit lists (ids above 65,535: the large grids, where it was
+// measured faster), the distance carry whenever the context holds the
+// distance buffers (grids up to 256^3) and the class order per
+// jfa_class_order.
 static int g_jfa_variant = -1;
 static int jfa_variant(const fos_ctx *ctx) {
   if (g_jfa_variant >= 0) return g_jfa_variant;
   return (ctx->max_n > 65536 ? 1 : 0) | (ctx->dist[0] ? 2 : 0);
+}
+
+// Whether a pass takes its x-planes in residue-class order (jfa_plane).  By
+// default only where it was measured faster: grids of 256^3 and more, whose
+// x-slab per XCD exceeds its L2, at steps 4 .. res/16 (at larger steps the
+// classes are short and the planes they share are few; at steps 1-3 the
+// order is nearly the natural one).
+static bool jfa_class_order(const fos_ctx *ctx, int res, int step) {
+  if (g_jfa_variant >= 0) return (g_jfa_variant & 4) != 0;
+  return res >= 256 && step >= 4 && step * 16 <= res;
 }
 
 template <bool U, int MODE, typename LT, int BATCH, int DM>
@@ -1384,10 +1421,20 @@ static void launch_pass_list_v(fos_ctx *ctx, int res, float L, float half_l, flo
   const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
   const int ntiles = res * nyb * nzb;
   const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
-  hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, LT, BATCH, DM>), dim3(per * FOS_XCDS), dim3(256),
-                     0, st, ctx->seeds, res, L, half_l, voxel, step, in, out, ctx->stripes, pass,
-                     early, th, minp, nyb, nzb, ntiles, per, chg_prev, chg_out, dist_in,
-                     dist_out);
+  const unsigned q = (unsigned)(res / step);
+  const int qr = (int)((q << 16) | (unsigned)(res - (int)q * step));
+  const unsigned mq = (unsigned)(((1ull << 32) + q - 1) / q);
+  const unsigned mq1 = (unsigned)(((1ull << 32) + q) / (q + 1));
+  if (jfa_class_order(ctx, res, step))
+    hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, LT, BATCH, DM, true>), dim3(per * FOS_XCDS),
+                       dim3(256), 0, st, ctx->seeds, res, L, half_l, voxel, step, in, out,
+                       ctx->stripes, pass, early, th, minp, nyb, nzb, ntiles, per, chg_prev,
+                       chg_out, dist_in, dist_out, qr, mq, mq1);
+  else
+    hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, LT, BATCH, DM>), dim3(per * FOS_XCDS),
+                       dim3(256), 0, st, ctx->seeds, res, L, half_l, voxel, step, in, out,
+                       ctx->stripes, pass, early, th, minp, nyb, nzb, ntiles, per, chg_prev,
+                       chg_out, dist_in, dist_out, qr, mq, mq1);
 }
 
 // dm: 0 no distance buffers, 1 write dist_out, 2 read dist_in and write dist_out
@@ -1520,7 +1567,7 @@ extern "C" {
 int fos_set_param(const char *name, int value) {
   FOS_REQUIRE(name, "fos_set_param: name is NULL");
   if (!strcmp(name, "jfa_variant")) {
-    FOS_REQUIRE(value >= -1 && value <= 3, "jfa_variant must be in [-1, 3]");
+    FOS_REQUIRE(value >= -1 && value <= 7, "jfa_variant must be in [-1, 7]");
     g_jfa_variant = value;
   } else if (!strcmp(name, "face_emit_tx")) {
     FOS_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4 || value == 8 ||

@@ -146,8 +146,11 @@ def test_jfa_single_steps_match(gpu, orc):
 
 
 # fos_set_param "jfa_variant": bit 0 four list entries per iteration (two
-# otherwise), bit 1 the winning power distance carried from pass to pass
-PASS_VARIANTS = {"batch2": 0, "batch4": 1, "batch2_carry": 2, "batch4_carry": 3}
+# otherwise), bit 1 the winning power distance carried from pass to pass,
+# bit 2 the residue-class x-plane order
+PASS_VARIANTS = {"batch2": 0, "batch4": 1, "batch2_carry": 2, "batch4_carry": 3,
+                 "batch2_class": 4, "batch4_class": 5, "batch2_carry_class": 6,
+                 "batch4_carry_class": 7}
 
 
 @pytest.mark.parametrize("variant", sorted(PASS_VARIANTS))
