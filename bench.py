@@ -156,6 +156,7 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cadence", action="store_true")
+    ap.add_argument("--no-temporal", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=48)
     ap.add_argument("--no-ensemble", action="store_true")
@@ -214,6 +215,31 @@ def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
         pass_ms.append(list(ms))
         fl.call("fos_events_destroy", P + 1, arr)
     return t1 - t0, pass_ms, sim
+
+
+def temporal_bench(cfg, pos, rad, steps, warmup):
+    """C2 under the reference's temporal-skip scheduler (run.py:902-1018):
+    frames/s past the scheduler's warm start and why each timed frame ran
+    or skipped the JFA (one host read per frame for the decision)."""
+    import collections
+
+    import torch
+
+    from foam import FoamSim
+    sim = FoamSim(cfg, pos, rad)
+    sim.run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reasons = [sim.step()["jfa_reason"] for _ in range(steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": "same, JFA on the frames the temporal-skip scheduler picks "
+                        "(TH_RSIGMA 0.004, TH_MARGIN_SAFE 0.4, TH_FSC_DRIFT 0.05, "
+                        "MAX_SKIP_FRAMES 15, 30-frame warm start)",
+            "value": round(steps / el, 3), "unit": "frames/s",
+            "frames": [warmup, warmup + steps - 1],
+            "jfa_frames": sum(1 for r in reasons if r != "skip"),
+            "reasons": dict(collections.Counter(reasons))}
 
 
 C3_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_jfa_pass_c3_hbm_traffic.md")
@@ -546,6 +572,10 @@ def main():
         out["cadence"] = {"workload": "same, JFA every 5th frame after a 20-frame warm start",
                           "value": round(args.steps / el2, 3), "unit": "frames/s",
                           "frames": [w2, w2 + args.steps - 1], "jfa_frames": jfa_frames}
+    if not args.no_temporal:
+        c2t = fc.preset("c2t")
+        out["temporal"] = temporal_bench(c2t, pos, rad, args.steps,
+                                         max(args.warmup, c2t.skip_warmstart))
     if not args.no_c3:
         out["c3_jfa"] = c3_jfa_bench(args.c3_frames)
         out["c3_frames"] = c3_frames_bench(args.c3_frames)

@@ -162,6 +162,12 @@ __device__ __forceinline__ void block_add(int v, int *dst) {
 }  // namespace fos
 
 // Scratch owned by a context (fos_ctx_create); sized once, never per call.
+// scheduler statistics (fos_sched_stats): FOS_SCHED_BLOCKS block partials of
+// 8 doubles, the 8 first-stage totals, the 8 second-stage totals, then the 8
+// doubles of fos_jfa_schedule's result
+#define FOS_SCHED_BLOCKS 256
+#define FOS_SCHED_DOUBLES ((FOS_SCHED_BLOCKS + 3) * 8)
+
 struct fos_ctx {
   int device;
   int max_n;
@@ -217,6 +223,7 @@ struct fos_ctx {
   void *spec_event;
   int spec_pending;
   int spec_first;
+  double *sched;          // [FOS_SCHED_DOUBLES] scheduler statistics: block partials, totals
   float *fscratch;        // [16] small device floats (reductions)
   int *iscratch;          // [16] small device ints
   // host pinned mirror for small reads

@@ -203,7 +203,7 @@ int fos_ctx_destroy(fos_ctx *ctx) {
                   ctx->acc_in,   ctx->scan_blocks, ctx->cell_write, ctx->cell_of,
                   ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch,
                   ctx->chg[0],   ctx->chg[1],   ctx->grid_tmp, ctx->dist[0], ctx->dist[1],
-                  ctx->flow};
+                  ctx->flow,     ctx->sched};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->spec_event) (void)hipEventDestroy((hipEvent_t)ctx->spec_event);
@@ -303,6 +303,7 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
   ALLOC(c->flow, (size_t)FOS_FLOW_MAX_ITERS * c->flow_n * sizeof(unsigned long long));
   ALLOC(c->fscratch, 64 * sizeof(float));
   ALLOC(c->iscratch, 64 * sizeof(int));
+  ALLOC(c->sched, (size_t)FOS_SCHED_DOUBLES * sizeof(double));
 #undef ALLOC
   hipError_t e = hipHostMalloc((void **)&c->host_i, 64 * sizeof(int64_t));
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->host_f, 64 * sizeof(double));

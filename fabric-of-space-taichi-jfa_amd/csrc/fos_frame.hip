@@ -17,15 +17,21 @@ bool jfa_runs(const fos_frame_desc *d, int frame) {
   return (frame - d->jfa_warm) % every == 0;
 }
 
-// The face step's arguments for frame `frame` (fos_jfa_faces_controller).
-fos::FaceStep face_step(const fos_frame_desc *d, const fos_frame_bufs *b, int frame) {
+// jfa.py:224 pass count, ceil(log2 res) + 1
+int jfa_passes_for(int res) { return (int)ceil(log2((double)res)) + 1; }
+
+// The face step's arguments for frame `frame` (fos_jfa_faces_controller), at
+// the resolution that frame's JFA ran at (s->jfa_res; 0: the configured one).
+fos::FaceStep face_step(const fos_frame_desc *d, const fos_frame_bufs *b, int frame,
+                        const fos_frame_state *s) {
   fos::FaceStep a;
-  a.res = d->res;
+  const int res = s->jfa_res > 0 ? s->jfa_res : d->res;
+  a.res = res;
   a.n = d->n;
   a.filtered = b->filtered;
   a.rad_target = b->rad_target;
   a.rad = b->rad;
-  a.voxel = d->voxel;
+  a.voxel = res == d->res ? d->voxel : (float)(d->L_d / (double)res);
   a.face_ids = b->face_ids;
   a.face_counts = b->face_counts;
   a.prev_scores = b->prev_scores;
@@ -49,6 +55,117 @@ fos::FaceStep face_step(const fos_frame_desc *d, const fos_frame_bufs *b, int fr
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Temporal-skip scheduler statistics (run.py:902-1018), in a fixed order:
+// stage A, per block, sums |dr|, r, fsc, the out-of-band count and the
+// maximum squared displacement; stage B (one block) totals them; stage C,
+// per block, sums (|dr| - mean)^2 and moves the current values into the
+// previous-value buffers; stage D (one block) writes the result.
+// ---------------------------------------------------------------------------
+#define SCHED_T 256
+
+// Block sum of 8 doubles in a fixed tree order; thread 0 gets the result.
+__device__ __forceinline__ void sched_block_sum(double v[8], bool max5, double *dst) {
+  __shared__ double s_v[8][SCHED_T];
+  const int t = (int)threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s_v[k][t] = v[k];
+  __syncthreads();
+  for (int o = SCHED_T / 2; o > 0; o >>= 1) {
+    if (t < o) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        s_v[k][t] = (max5 && k == 5) ? fmax(s_v[k][t], s_v[k][t + o]) : s_v[k][t] + s_v[k][t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[k] = s_v[k][0];
+  }
+}
+
+// |pos - pos_prev| of particle i at the minimum image, as numpy computes it
+// in fp32: d = p - pp; |d| > L/2 -> d - sign(d) L; sqrt((dx^2 + dy^2) + dz^2).
+__device__ __forceinline__ float sched_disp(const float *__restrict__ pos,
+                                            const float *__restrict__ pp, int i, float L) {
+  float s = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float d = pos[3 * i + a] - pp[3 * i + a];
+    if (fabsf(d) > L * 0.5f) d = d - copysignf(L, d);
+    s = s + d * d;
+  }
+  return sqrtf(s);
+}
+
+__global__ void __launch_bounds__(SCHED_T)
+    k_sched_a(const float *__restrict__ pos, const float *__restrict__ rad,
+              const int32_t *__restrict__ fsc, int n, float L, int f_low, int f_high,
+              const float *__restrict__ pp, const float *__restrict__ rp, int have_prev,
+              double *__restrict__ part) {
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = blockIdx.x * SCHED_T + threadIdx.x; i < n; i += gridDim.x * SCHED_T) {
+    const float r = rad[i];
+    const int f = fsc[i];
+    if (have_prev) {
+      v[0] += (double)fabsf(r - rp[i]);
+      v[5] = fmax(v[5], (double)sched_disp(pos, pp, i, L));
+    }
+    v[1] += (double)r;
+    v[2] += (double)f;
+    v[3] += (f < f_low || f > f_high) ? 1.0 : 0.0;
+  }
+  sched_block_sum(v, true, part + 8 * blockIdx.x);
+}
+
+// Stage B and D: one block totals the partials of `blocks` blocks (slot 5
+// a maximum in stage B).
+__global__ void __launch_bounds__(SCHED_T)
+    k_sched_total(const double *__restrict__ part, int blocks, bool max5,
+                  double *__restrict__ dst) {
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int t = (int)threadIdx.x;
+  if (t < blocks) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[8 * t + k];
+  }
+  sched_block_sum(v, max5, dst);
+}
+
+__global__ void __launch_bounds__(SCHED_T)
+    k_sched_c(const float *__restrict__ pos, const float *__restrict__ rad, int n,
+              float *__restrict__ pp, float *__restrict__ rp, int have_prev,
+              const double *__restrict__ tot, double *__restrict__ part) {
+  const double mean = have_prev ? tot[0] / (double)n : 0.0;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = blockIdx.x * SCHED_T + threadIdx.x; i < n; i += gridDim.x * SCHED_T) {
+    const float r = rad[i];
+    if (have_prev) {
+      const double e = (double)fabsf(r - rp[i]) - mean;
+      v[0] += e * e;
+    }
+    rp[i] = r;
+    pp[3 * i] = pos[3 * i];
+    pp[3 * i + 1] = pos[3 * i + 1];
+    pp[3 * i + 2] = pos[3 * i + 2];
+  }
+  sched_block_sum(v, false, part + 8 * blockIdx.x);
+}
+
+__global__ void k_sched_out(const double *__restrict__ tot, const double *__restrict__ tot2,
+                            int n, double *__restrict__ out) {
+  const double dn = (double)n;
+  out[0] = tot[0] / dn;
+  out[1] = sqrt(tot2[0] / dn);
+  out[2] = tot[5];
+  out[3] = tot[1] / dn;
+  out[4] = tot[2] / dn;
+  out[5] = tot[3];
+  out[6] = 0.0;
+  out[7] = 0.0;
+}
 
 #define FOS_CALL(x)             \
   do {                          \
@@ -85,7 +202,7 @@ static int frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_buf
                            b->lmd, b->maxd, stream));
   // the last frame's deferred face step is finished before anything reads
   // its faces, FSC, targets or gate
-  if (ctx->spec_pending) FOS_CALL(fos::face_step_finish(ctx, face_step(d, b, f - 1), st));
+  if (ctx->spec_pending) FOS_CALL(fos::face_step_finish(ctx, face_step(d, b, f - 1, s), st));
   // the overlap maximum (and the last gated frame's gate) go to the host
   // behind an event; settle, pressure and noise need neither and are queued
   // before the host waits
@@ -164,22 +281,28 @@ static int frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_buf
     FOS_CALL(fos_grid_build(ctx, pos, n, G, L, b->cell_count, b->cell_start, b->cell_indices,
                             stream));
   s->need_rebuild = 0;
-  // 8 JFA on cadence frames (jfa.run_jfa without the host copies)
-  const bool ran = jfa_runs(d, f);
+  // 8 JFA on the frames the schedule picks (jfa.run_jfa without the host
+  // copies), at the resolution it picks
+  int run = 0, res = d->res;
+  FOS_CALL(fos_jfa_schedule(ctx, d, b, s, pos, &run, &res, stream));
+  const bool ran = run != 0;
   if (ran) {
-    FOS_CALL(fos_jfa_init(ctx, pos, b->rad, n, d->res, L, d->voxel, d->beta, b->grid, nullptr,
+    const float voxel = res == d->res ? d->voxel : (float)(d->L_d / (double)res);
+    const int passes = res == d->res ? d->jfa_passes : jfa_passes_for(res);
+    s->jfa_res = res;
+    FOS_CALL(fos_jfa_init(ctx, pos, b->rad, n, res, L, voxel, d->beta, b->grid, nullptr,
                           stream));
-    FOS_CALL(fos_jfa_passes_timed(ctx, d->res, L, d->voxel, d->jfa_passes, d->early_exit,
+    FOS_CALL(fos_jfa_passes_timed(ctx, res, L, voxel, passes, d->early_exit,
                                   d->th_pass_delta, d->min_passes, b->grid, b->temp, b->info,
                                   pass_events, stream));
-    FOS_CALL(fos_majority(d->res, b->grid, b->filtered, stream));
+    FOS_CALL(fos_majority(res, b->grid, b->filtered, stream));
     // face rows, finalize and the FSC EMA, then 9: the warm-start gate and
     // the gated targets on the device
     if (defer)
-      FOS_CALL(fos::face_step_defer(ctx, face_step(d, b, f), st));
+      FOS_CALL(fos::face_step_defer(ctx, face_step(d, b, f, s), st));
     else
       FOS_CALL(fos_jfa_faces_controller(
-          ctx, d->res, n, b->filtered, b->rad_target, b->rad, d->voxel, b->face_ids,
+          ctx, res, n, b->filtered, b->rad_target, b->rad, voxel, b->face_ids,
           b->face_counts, b->prev_scores, b->neighbor_list, b->neighbor_count, b->used_count,
           b->fsc, b->fsc_ema, d->ema_alpha, f, d->warmstart_frames, d->fsc_zero_rate, d->f_low,
           d->f_high, d->growth_pct, d->r_min, d->r_max, d->deadband, b->maxd + 1, stream));
@@ -198,6 +321,103 @@ static int frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_buf
 }
 
 extern "C" {
+
+int fos_sched_stats(fos_ctx *ctx, const float *pos, const float *rad, const int32_t *fsc, int n,
+                    float L, int f_low, int f_high, float *pos_prev, float *rad_prev,
+                    int have_prev, double *out, void *stream) {
+  FOS_REQUIRE(ctx && pos && rad && fsc && pos_prev && rad_prev && out && n >= 1 &&
+                  n <= ctx->max_n,
+              "fos_sched_stats: bad arguments (n=%d)", n);
+  hipStream_t st = (hipStream_t)stream;
+  int blocks = (n + SCHED_T - 1) / SCHED_T;
+  if (blocks > FOS_SCHED_BLOCKS) blocks = FOS_SCHED_BLOCKS;
+  double *part = ctx->sched, *tot = ctx->sched + 8 * FOS_SCHED_BLOCKS, *tot2 = tot + 8;
+  hipLaunchKernelGGL(k_sched_a, dim3(blocks), dim3(SCHED_T), 0, st, pos, rad, fsc, n, L, f_low,
+                     f_high, pos_prev, rad_prev, have_prev, part);
+  hipLaunchKernelGGL(k_sched_total, dim3(1), dim3(SCHED_T), 0, st, part, blocks, true, tot);
+  hipLaunchKernelGGL(k_sched_c, dim3(blocks), dim3(SCHED_T), 0, st, pos, rad, n, pos_prev,
+                     rad_prev, have_prev, tot, part);
+  hipLaunchKernelGGL(k_sched_total, dim3(1), dim3(SCHED_T), 0, st, part, blocks, false, tot2);
+  hipLaunchKernelGGL(k_sched_out, dim3(1), dim3(1), 0, st, tot, tot2, n, out);
+  FOS_CHECK_LAUNCH("k_sched");
+  return 0;
+}
+
+int fos_jfa_schedule(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
+                     fos_frame_state *s, const float *pos, int *run, int *res, void *stream) {
+  FOS_REQUIRE(ctx && d && b && s && pos && run && res, "fos_jfa_schedule: bad arguments");
+  *res = d->res;
+  const bool temporal = d->schedule == 1;
+  if (!temporal) {
+    *run = jfa_runs(d, s->frame) ? 1 : 0;
+    s->reason = *run ? FOS_JFA_CADENCE : FOS_JFA_SKIP;
+  }
+  if (!temporal && !(d->adaptive_res && *run)) return 0;
+  FOS_REQUIRE(b->pos_prev && b->rad_prev && b->sched_host,
+              "fos_jfa_schedule: the scheduler needs pos_prev, rad_prev and sched_host");
+  hipStream_t st = (hipStream_t)stream;
+  double *out = ctx->sched + 8 * (FOS_SCHED_BLOCKS + 2);
+  FOS_CALL(fos_sched_stats(ctx, pos, b->rad, b->fsc, d->n, d->L, d->f_low, d->f_high,
+                           b->pos_prev, b->rad_prev, temporal ? s->have_prev : 0, out, stream));
+  FOS_HIP(hipMemcpyAsync(b->sched_host, out, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+  FOS_HIP(hipStreamSynchronize(st));
+  const double *o = b->sched_host;
+  const double r_mean = o[3];
+  if (temporal) {
+    // run.py:906-1018
+    s->sched_frames += 1;
+    s->since += 1;
+    double sig = 999.0, vmax = 999.0;
+    if (s->have_prev) {
+      sig = o[1] / (r_mean > 1e-9 ? r_mean : 1e-9);
+      vmax = o[2];
+    }
+    s->have_prev = 1;
+    // the FSC mean after the last JFA: the FSC has not changed since (only a
+    // JFA writes it), so it is the mean measured now
+    if (s->fsc_mu_pending) {
+      s->last_fsc_mu = o[4];
+      s->fsc_mu_set = 1;
+      s->fsc_mu_pending = 0;
+    }
+    const double dmu = s->fsc_mu_set ? fabs(o[4] - s->last_fsc_mu) : 999.0;
+    const double th_margin = d->th_margin * r_mean;
+    if (s->lockout > 0) s->lockout -= 1;
+    const bool warm = s->sched_frames <= d->skip_warmstart;
+    const bool watchdog = s->since >= d->max_skip;
+    const bool band = s->lockout > 0;
+    const bool activity = sig > d->th_rsigma || vmax > th_margin || dmu > d->th_fsc_drift;
+    const double oob = o[5] / (double)(d->n > 1 ? d->n : 1);
+    const bool out_of_band = s->since > 1 && oob > d->oob_frac;
+    s->signals[0] = sig;
+    s->signals[1] = vmax;
+    s->signals[2] = dmu;
+    s->signals[3] = oob;
+    s->signals[4] = r_mean;
+    *run = (warm || watchdog || band || activity || out_of_band) ? 1 : 0;
+    s->reason = warm ? FOS_JFA_WARMSTART
+                : band ? FOS_JFA_BAND_CHANGE
+                : watchdog ? FOS_JFA_WATCHDOG
+                : out_of_band ? FOS_JFA_OUT_OF_BAND
+                : activity ? FOS_JFA_ACTIVITY
+                           : FOS_JFA_SKIP;
+    if (*run) {
+      s->since = 0;
+      s->fsc_mu_pending = 1;
+    }
+  }
+  if (*run && d->adaptive_res) {
+    // run.py:1054-1061: round() of a Python float (ties to even)
+    int r = (int)nearbyint(d->L_d * d->voxels_per_diameter / (2.0 * r_mean));
+    r = r < d->res_min ? d->res_min : r;
+    r = r > d->res_max ? d->res_max : r;
+    FOS_REQUIRE(r >= 1 && r <= ctx->max_res,
+                "fos_jfa_schedule: adaptive resolution %d outside the context's 1..%d", r,
+                ctx->max_res);
+    *res = r;
+  }
+  return 0;
+}
 
 int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
                    fos_frame_state *s, void *const *pass_events, void *stream) {

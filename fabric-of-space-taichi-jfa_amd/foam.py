@@ -85,7 +85,15 @@ class FoamSim:
         self.cell_pr = torch.zeros(max(n, 1), 4, dtype=torch.float32, device=dev)
         self.slot_records = True
         self.res = int(cfg.res)
-        V = self.res ** 3
+        if cfg.jfa_schedule not in ("cadence", "temporal"):
+            raise ValueError(f"jfa_schedule must be 'cadence' or 'temporal', not {cfg.jfa_schedule!r}")
+        if cfg.jfa_schedule == "temporal" and cfg.judge_every > 0:
+            raise ValueError("the temporal JFA schedule and the predictor's judge schedule "
+                             "are exclusive")
+        # the label buffers hold the largest grid a JFA frame may use
+        # (adaptive resolution: res_max)
+        self.max_res = int(cfg.max_jfa_res())
+        V = self.max_res ** 3
         self.grid = torch.full((V,), -1, dtype=torch.int32, device=dev)
         self.temp = torch.full((V,), -1, dtype=torch.int32, device=dev)
         self.filtered = torch.full((V,), -1, dtype=torch.int32, device=dev)
@@ -102,7 +110,12 @@ class FoamSim:
         # [0] overlap maximum of the frame, [1] the warm-start gate (1.0 =
         # growth) of the last frame that decided it on the device
         self.maxd = torch.zeros(2, dtype=torch.float32, device=dev)
-        self.ctx = fl.context(n, self.res, C)
+        self.ctx = fl.context(n, self.max_res, C)
+        # scheduler (temporal skip, adaptive resolution): positions and
+        # radii at the last decision, and the statistics' host copy
+        self.pos_prev = torch.zeros(n, 3, dtype=torch.float32, device=dev)
+        self.rad_prev = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.sched_host = torch.zeros(8, dtype=torch.float64).pin_memory()
         self.frame = 0
         # settle counter (run.py:744-751, 1231) on the device, ping-pong:
         # slot sidx enters the next frame.  settle_force: the last frame set
@@ -137,6 +150,8 @@ class FoamSim:
         self.native = True
         self._fdesc = self._fbufs = None
         self._fstate = fl.FrameState()
+        self._sched_since = 0
+        self.jfa_res = self.res
 
     @property
     def pos(self):
@@ -216,18 +231,22 @@ class FoamSim:
                 c.pressure_k, c.pressure_exp, c.pressure_cap, c.pressure_max_nei, c.r_min,
                 c.r_max, None, fl.stream())
 
-    def run_jfa(self):
-        """jfa.run_jfa without the host copies (jfa.py:1078-1132)."""
+    def run_jfa(self, res=None):
+        """jfa.run_jfa without the host copies (jfa.py:1078-1132), at the
+        configured resolution or `res` (adaptive resolution)."""
         c, ctx, st = self.cfg, self.ctx.handle, fl.stream()
-        voxel = c.voxel()
-        fl.call("fos_jfa_init", ctx, fl.ptr(self.pos), fl.ptr(self.rad), self.n, self.res, c.L,
+        res = self.res if res is None else int(res)
+        voxel = c.voxel() if res == c.res else fc.f32(c.L / res)
+        passes = c.passes() if res == c.res else int(math.ceil(math.log2(res))) + 1
+        self.jfa_res = res
+        fl.call("fos_jfa_init", ctx, fl.ptr(self.pos), fl.ptr(self.rad), self.n, res, c.L,
                 voxel, c.beta, fl.ptr(self.grid), None, st)
         ev = self.pass_events.pop(0) if self.pass_events else None
-        fl.call("fos_jfa_passes_timed", ctx, self.res, c.L, voxel, c.passes(),
+        fl.call("fos_jfa_passes_timed", ctx, res, c.L, voxel, passes,
                 int(c.early_exit), c.th_pass_delta, c.min_passes, fl.ptr(self.grid),
                 fl.ptr(self.temp), fl.ptr(self.info), ev, st)
-        fl.call("fos_majority", self.res, fl.ptr(self.grid), fl.ptr(self.filtered), st)
-        fl.call("fos_jfa_faces", ctx, self.res, self.n, fl.ptr(self.filtered),
+        fl.call("fos_majority", res, fl.ptr(self.grid), fl.ptr(self.filtered), st)
+        fl.call("fos_jfa_faces", ctx, res, self.n, fl.ptr(self.filtered),
                 fl.ptr(self.rad), voxel, fl.ptr(self.face_ids), fl.ptr(self.face_counts),
                 fl.ptr(self.prev_scores), fl.ptr(self.neighbor_list),
                 fl.ptr(self.neighbor_count), fl.ptr(self.used_count), fl.ptr(self.fsc), st)
@@ -281,7 +300,12 @@ class FoamSim:
             brownian_strength=c.brownian_strength, seed=c.seed, ou_jitter=int(c.ou_jitter),
             dt=fc.DT, jitter_rms=fc.JITTER_RMS, jitter_tau=fc.JITTER_TAU,
             drift_cap_frac=fc.f32(fc.MAX_DRIFT_FRACTION * c.gap_fraction),
-            jfa_every=c.jfa_every, jfa_warm=c.jfa_warm, slot_records=int(self.slot_records))
+            jfa_every=c.jfa_every, jfa_warm=c.jfa_warm, slot_records=int(self.slot_records),
+            schedule=1 if c.jfa_schedule == "temporal" else 0, skip_warmstart=c.skip_warmstart,
+            max_skip=c.max_skip, th_rsigma=float(c.th_rsigma), th_margin=float(c.th_margin),
+            th_fsc_drift=float(c.th_fsc_drift), oob_frac=float(c.oob_frac),
+            adaptive_res=int(c.adaptive_res), res_min=c.res_min, res_max=c.res_max,
+            voxels_per_diameter=float(c.voxels_per_diameter), L_d=float(c.L))
 
     def _frame_bufs(self):
         b = fl.FrameBufs()
@@ -289,17 +313,63 @@ class FoamSim:
         for k in ("rad", "rad_target", "lmd", "delta_r", "v_jit", "cell_count", "cell_start",
                   "cell_indices", "cell_pr", "grid", "temp", "filtered", "face_ids",
                   "face_counts", "fsc", "fsc_ema", "neighbor_list", "neighbor_count",
-                  "used_count", "prev_scores", "info", "maxd", "maxd_host", "settle_dev"):
+                  "used_count", "prev_scores", "info", "maxd", "maxd_host", "settle_dev",
+                  "pos_prev", "rad_prev", "sched_host"):
             setattr(b, k, getattr(self, k).data_ptr())
         return b
+
+    def _frame_objs(self):
+        if self._fdesc is None:
+            self._fdesc, self._fbufs = self._frame_desc(), self._frame_bufs()
+        return self._fdesc, self._fbufs
+
+    def schedule_jfa(self):
+        """The JFA decision of this frame (fos_jfa_schedule: the cadence, or
+        the temporal skip of run.py:902-1018) and the resolution a JFA frame
+        runs at (run.py:1049-1068 when adaptive).  Returns (run, res)."""
+        d, b = self._frame_objs()
+        s = self._fstate
+        s.frame = self.frame
+        run, res = ctypes.c_int(), ctypes.c_int()
+        since = s.since
+        fl.call("fos_jfa_schedule", self.ctx.handle, ctypes.byref(d), ctypes.byref(b),
+                ctypes.byref(s), fl.ptr(self.pos), ctypes.byref(run), ctypes.byref(res),
+                fl.stream())
+        self._sched_since = since + 1
+        return bool(run.value), int(res.value)
+
+    def schedule_record(self):
+        """The last scheduling decision: reason, frames since the last JFA as
+        the reference prints it, and (temporal skip) the signals."""
+        s = self._fstate
+        rec = {"jfa_reason": fl.JFA_REASONS.get(int(s.reason), "?")}
+        if self.cfg.jfa_schedule == "temporal":
+            rec["jfa_since"] = int(s.since) if int(s.reason) == 0 else self._sched_since
+            rec["jfa_signals"] = {"sigma_dr_over_r": float(s.signals[0]),
+                                  "vmax_dt": float(s.signals[1]),
+                                  "delta_fsc_mu": float(s.signals[2]),
+                                  "out_of_band": float(s.signals[3]),
+                                  "r_mean": float(s.signals[4])}
+        return rec
+
+    def set_band(self, f_low, f_high):
+        """Changes the FSC band between frames; with the temporal skip a change
+        forces the JFA for the lockout (run.py:1451-1453)."""
+        c = self.cfg
+        if (int(f_low), int(f_high)) == (c.f_low, c.f_high):
+            return
+        c.f_low, c.f_high = int(f_low), int(f_high)
+        if c.jfa_schedule == "temporal":
+            self._fstate.lockout = int(c.band_lockout)
+        self._fdesc = None
 
     def _step_native(self):
         """step() as one fos_frame_step call (the same calls in the same
         order, issued from C++)."""
         c, f = self.cfg, self.frame
         s = self._fstate
-        if self._fdesc is None:
-            self._fdesc, self._fbufs = self._frame_desc(), self._frame_bufs()
+        self._frame_objs()
+        since = s.since
         s.frame, s.pos_cur, s.sidx = f, self._cur, self.sidx
         s.settle_force, s.settle_gate = int(self.settle_force), int(self.settle_gate)
         s.need_rebuild = int(self.cfg.L != 1.0 or self.grid_key != self.grid_token())
@@ -313,10 +383,15 @@ class FoamSim:
         self.settle_force, self.settle_gate = bool(s.settle_force), bool(s.settle_gate)
         self.grid_key = self.grid_token()
         ran = bool(s.jfa_ran)
+        self._sched_since = since + 1
+        if ran:
+            self.jfa_res = int(s.jfa_res) or self.res
         self.last = FrameInfo(self, frame=f, pbd_passes=int(s.pbd_passes),
                               max_depth=float(s.max_depth), jfa=ran,
                               growth=None if ran else True)
+        self.last.update(self.schedule_record())
         if ran:
+            self.last["jfa_res"] = self.jfa_res
             self.pending = self.last
         else:
             self.growth = True
@@ -341,8 +416,7 @@ class FoamSim:
         # following ones itself)
         self.resolve()
         s = self._fstate
-        if self._fdesc is None:
-            self._fdesc, self._fbufs = self._frame_desc(), self._frame_bufs()
+        self._frame_objs()
         f0 = self.frame
         s.frame, s.pos_cur, s.sidx = f0, self._cur, self.sidx
         s.settle_force, s.settle_gate = int(self.settle_force), int(self.settle_gate)
@@ -359,9 +433,12 @@ class FoamSim:
         self.settle_force, self.settle_gate = bool(s.settle_force), bool(s.settle_gate)
         self.grid_key = self.grid_token()
         ran = bool(s.jfa_ran)
+        if ran:
+            self.jfa_res = int(s.jfa_res) or self.res
         self.last = FrameInfo(self, frame=f0 + frames - 1, pbd_passes=int(s.pbd_passes),
                               max_depth=float(s.max_depth), jfa=ran,
                               growth=None if ran else True)
+        self.last["jfa_reason"] = fl.JFA_REASONS.get(int(s.reason), "?")
         if ran:
             self.pending = self.last
         else:
@@ -464,15 +541,19 @@ class FoamSim:
                         fl.ptr(self.fsc_ema), st)
             gated = True
         else:
-            ran = c.jfa_runs(f)
+            ran, res = self.schedule_jfa()
             if ran:
-                self.run_jfa()
+                self.run_jfa(res)
                 if self.on_jfa is not None:
                     self.on_jfa(self)
             gated = ran
         # 9 controller: after a JFA the gate is decided on the device
         self.last = FrameInfo(self, frame=f, pbd_passes=passes, max_depth=max_depth, jfa=ran,
                               growth=None if gated else True)
+        if self.predictor is None or c.judge_every <= 0:
+            self.last.update(self.schedule_record())
+            if ran:
+                self.last["jfa_res"] = self.jfa_res
         if gated:
             self.gate_growth(f)
             self.pending = self.last
@@ -493,7 +574,8 @@ class FoamSim:
         c = self.cfg
         oi = (fl.I64 * 5)()
         of = (fl.D * 4)()
-        fl.call("fos_jfa_stats", self.ctx.handle, self.n, fl.ptr(self.rad), c.voxel(),
+        fl.call("fos_jfa_stats", self.ctx.handle, self.n, fl.ptr(self.rad),
+                c.voxel() if self.jfa_res == c.res else fc.f32(c.L / self.jfa_res),
                 fl.ptr(self.fsc), fl.ptr(self.fsc_ema), fl.ptr(self.used_count),
                 fl.ptr(self.face_ids), fl.ptr(self.face_counts), oi, of, fl.stream())
         vo = (fl.I64 * 3)()
@@ -501,7 +583,7 @@ class FoamSim:
                 fl.ptr(self.neighbor_count), vo, fl.stream())
         info = self.info.cpu().numpy()
         passes = int(info[0])
-        V = float(self.res) ** 3
+        V = float(self.jfa_res) ** 3
         return {"num_passes": passes, "early_exit": bool(info[1]),
                 "delta_history": [float(c) / V for c in info[2:2 + passes]],
                 "p_dir": int(oi[0]), "p_und_accepted": int(oi[0]) // 2,
@@ -517,10 +599,13 @@ class FoamSim:
         the frame summary (run.py:1303-1318) every 100th frame."""
         import telemetry as tm
         c, last, out = self.cfg, self.last, []
+        if c.jfa_schedule == "temporal" and "jfa_signals" in last:
+            out += tm.jfa_schedule_lines(last, int(self._fstate.sched_frames), c.skip_warmstart,
+                                         c.th_rsigma, c.th_margin, c.th_fsc_drift, c.max_skip)
         if self.telemetry and last.get("jfa") and "num_passes" in last:
             val = {"passed": last["validation_overflow"] == 0 and last["self_loops"] == 0,
                    "asymmetric_pairs": last["asymmetric_pairs"]}
-            out += tm.jfa_step4_lines(last, val, 0.0, self.res, self.n)
+            out += tm.jfa_step4_lines(last, val, 0.0, self.jfa_res, self.n)
         if self.frame % 100 == 0:
             out += tm.frame_lines(self.frame, self.n, last.get("pbd_passes", 0),
                                   last.get("max_depth", 0.0), self.fsc.cpu().numpy(),
@@ -537,6 +622,12 @@ class FoamSim:
         out = {k: getattr(self, k).detach().cpu().numpy().copy() for k in self.STATE}
         out["frame"] = np.int64(self.frame)
         out["settle"] = np.int64(self.settle)
+        s = self._fstate
+        out["pos_prev"] = self.pos_prev.cpu().numpy().copy()
+        out["rad_prev"] = self.rad_prev.cpu().numpy().copy()
+        out["sched"] = np.array([s.sched_frames, s.since, s.have_prev, s.fsc_mu_set,
+                                 s.fsc_mu_pending, s.lockout], np.int64)
+        out["sched_fsc_mu"] = np.float64(s.last_fsc_mu)
         return out
 
     @property
@@ -558,6 +649,17 @@ class FoamSim:
         self.settle_force = self.settle_gate = False
         self.pending = None
         self.grid_key = None
+        s = self._fstate
+        if "sched" in snap:
+            self.pos_prev.copy_(torch.from_numpy(np.ascontiguousarray(snap["pos_prev"])))
+            self.rad_prev.copy_(torch.from_numpy(np.ascontiguousarray(snap["rad_prev"])))
+            (s.sched_frames, s.since, s.have_prev, s.fsc_mu_set, s.fsc_mu_pending,
+             s.lockout) = [int(v) for v in snap["sched"]]
+            s.last_fsc_mu = float(snap["sched_fsc_mu"])
+        else:
+            s.sched_frames = s.since = s.have_prev = s.fsc_mu_set = s.fsc_mu_pending = 0
+            s.lockout = 0
+            s.last_fsc_mu = 0.0
 
     def save(self, path):
         np.savez(path, **self.snapshot())

@@ -11,6 +11,7 @@ the benchmark workloads:
   c2   N=10,000, 128^3, 200 frames, JFA every frame, Gaussian Brownian noise
        and the OU jitter (headline)
   c2c  as c2 with JFA on a 1/5 cadence after a 20-frame warm start
+  c2t  as c2 with the reference's temporal-skip JFA scheduler
   c3   N=200,000 clustered, 384^3, 50 frames
   c5   N=50,000, 256^3, radii U[0.005, 0.011] (learned face predictor workload)
   c5s  as c5 with the predictor between JFA judge frames (every 8th)
@@ -42,11 +43,21 @@ MIN_FACE_VOXELS = 2            # config.py:151
 JFA_RES_MIN = 192              # config.py:156
 JFA_RES_MAX = 320              # config.py:157
 JFA_VOXEL_SCALE = 2.8          # config.py:158
+JFA_ADAPTIVE_ENABLED = True    # config.py:163
 JFA_VOXELS_PER_DIAMETER = 12.0 # config.py:164
+JFA_CADENCE = 5                # config.py:172
+JFA_WARMSTART_FRAMES = 30      # config.py:173
 JFA_TILE_SIZE = 16             # config.py:182
 JFA_DIRTY_HALO = 1             # config.py:183
 JFA_DIRTY_POS_THRESHOLD = 0.5  # config.py:189
 JFA_DIRTY_RAD_THRESHOLD = 0.25 # config.py:190
+JFA_TEMPORAL_SKIP_ENABLED = True  # config.py:200
+TH_RSIGMA = 0.004              # config.py:201
+TH_MARGIN_SAFE = 0.4           # config.py:202
+TH_FSC_DRIFT = 0.05            # config.py:203
+MAX_SKIP_FRAMES = 15           # config.py:204
+SKIP_BAND_CHANGE_LOCKOUT = 2   # config.py:205
+OUT_OF_BAND_FRAC = 0.10        # run.py:976 (the out-of-band guard)
 JFA_EARLY_EXIT_ENABLED = True  # config.py:208
 TH_PASS_DELTA = 0.006          # config.py:209
 MIN_JFA_PASSES = 3             # config.py:210
@@ -136,9 +147,26 @@ class FoamConfig:
     brownian_strength: float = BROWNIAN_STRENGTH
     ou_jitter: bool = False
     seed: int = RNG_SEED
-    # JFA cadence: every `jfa_every` frames once frame >= jfa_warm
+    # which frames run the JFA: "cadence" (every `jfa_every` frames once
+    # frame >= jfa_warm) or "temporal" (the reference's temporal skip,
+    # run.py:902-1018: warm start, watchdog, band-change lockout, activity
+    # and out-of-band triggers)
+    jfa_schedule: str = "cadence"
     jfa_every: int = 1
     jfa_warm: int = 0
+    skip_warmstart: int = JFA_WARMSTART_FRAMES
+    max_skip: int = MAX_SKIP_FRAMES
+    th_rsigma: float = TH_RSIGMA
+    th_margin: float = TH_MARGIN_SAFE
+    th_fsc_drift: float = TH_FSC_DRIFT
+    oob_frac: float = OUT_OF_BAND_FRAC
+    band_lockout: int = SKIP_BAND_CHANGE_LOCKOUT
+    # adaptive JFA resolution on JFA frames (run.py:1049-1068): round(L *
+    # voxels_per_diameter / (2 r_mean)) clamped to [res_min, res_max]
+    adaptive_res: bool = False
+    voxels_per_diameter: float = JFA_VOXELS_PER_DIAMETER
+    res_min: int = JFA_RES_MIN
+    res_max: int = JFA_RES_MAX
     frames: int = 20
     # learned face predictor: JFA (the judge) every `judge_every` frames, the
     # predictor in between (0 = plain JFA); fall back to the JFA on every
@@ -177,6 +205,10 @@ class FoamConfig:
             return True
         return (frame - self.jfa_warm) % max(1, self.jfa_every) == 0
 
+    def max_jfa_res(self):
+        """The largest resolution a JFA frame may run at."""
+        return max(self.res, self.res_max) if self.adaptive_res else self.res
+
 
 def preset(name):
     """Benchmark workloads (uniform unit box unless stated)."""
@@ -184,7 +216,7 @@ def preset(name):
         return FoamConfig(name="c1", n=2000, res=64, r_min=0.01, r_max=0.04,
                           pbd_base=8, pbd_max=8, f_low=10, f_high=16,
                           pressure_iters=10, frames=20, warmstart_frames=5)
-    if name in ("c2", "c2c"):
+    if name in ("c2", "c2c", "c2t"):
         # the reference frame also runs the OU jitter every frame (run.py:857-867)
         cfg = FoamConfig(name=name, n=10000, res=128, r_min=0.006, r_max=0.024,
                          pbd_base=PBD_BASE_PASSES, pbd_max=12, f_low=12, f_high=16,
@@ -192,6 +224,8 @@ def preset(name):
                          brownian_strength=1e-4, ou_jitter=True, frames=200)
         if name == "c2c":
             cfg.jfa_every, cfg.jfa_warm = 5, 20
+        if name == "c2t":
+            cfg.jfa_schedule = "temporal"
         return cfg
     if name == "c3":
         return FoamConfig(name="c3", n=200000, res=384, r_min=0.0015, r_max=0.015,

@@ -42,7 +42,7 @@ def clustered(n, seed=0, centres=64, sigma=0.05, median=0.006, sigma_log=0.3,
 def for_config(cfg):
     if cfg.name in ("c1",):
         return uniform(cfg.n, 0.02, 0.04, seed=0)
-    if cfg.name in ("c2", "c2c"):
+    if cfg.name in ("c2", "c2c", "c2t"):
         return uniform(cfg.n, 0.012, 0.024, seed=0)
     if cfg.name == "c3":
         return clustered(cfg.n, seed=0)

@@ -116,6 +116,8 @@ SIGNATURES = {
     "fos_frame_step": [P, P, P, P, P, P],
     "fos_frame_run": [P, P, P, P, I, P, P],
     "fos_frame_release": [P],
+    "fos_sched_stats": [P, P, P, P, I, F, I, I, P, P, I, P, P],
+    "fos_jfa_schedule": [P, P, P, P, P, P, P, P],
 }
 
 
@@ -136,7 +138,11 @@ class FrameDesc(ctypes.Structure):
                 ("brownian", I), ("brownian_mode", I), ("brownian_strength", F),
                 ("seed", U64), ("ou_jitter", I),
                 ("dt", F), ("jitter_rms", F), ("jitter_tau", F), ("drift_cap_frac", F),
-                ("jfa_every", I), ("jfa_warm", I), ("slot_records", I)]
+                ("jfa_every", I), ("jfa_warm", I), ("slot_records", I),
+                ("schedule", I), ("skip_warmstart", I), ("max_skip", I),
+                ("th_rsigma", D), ("th_margin", D), ("th_fsc_drift", D), ("oob_frac", D),
+                ("adaptive_res", I), ("res_min", I), ("res_max", I),
+                ("voxels_per_diameter", D), ("L_d", D)]
 
 
 class FrameBufs(ctypes.Structure):
@@ -145,7 +151,7 @@ class FrameBufs(ctypes.Structure):
         "rad", "rad_target", "lmd", "delta_r", "v_jit", "cell_count", "cell_start",
         "cell_indices", "cell_pr", "grid", "temp", "filtered", "face_ids", "face_counts", "fsc",
         "fsc_ema", "neighbor_list", "neighbor_count", "used_count", "prev_scores", "info", "maxd",
-        "maxd_host", "settle_dev")]
+        "maxd_host", "settle_dev", "pos_prev", "rad_prev", "sched_host")]
 
 
 class FrameState(ctypes.Structure):
@@ -153,7 +159,14 @@ class FrameState(ctypes.Structure):
     _fields_ = [("frame", I), ("pos_cur", I), ("sidx", I), ("settle_force", I),
                 ("settle_gate", I), ("need_rebuild", I), ("pending", I),
                 ("pbd_passes", I), ("jfa_ran", I), ("gate_read", I),
-                ("max_depth", F), ("gate", F), ("event", P)]
+                ("max_depth", F), ("gate", F), ("event", P),
+                ("sched_frames", I), ("since", I), ("have_prev", I), ("fsc_mu_set", I),
+                ("fsc_mu_pending", I), ("lockout", I), ("reason", I), ("jfa_res", I),
+                ("last_fsc_mu", D), ("signals", D * 5)]
+
+# reasons of a JFA scheduling decision (FOS_JFA_* in include/fos.h)
+JFA_REASONS = {0: "skip", 1: "warm-start", 2: "band-change", 3: "watchdog",
+               4: "out-of-band-guard", 5: "activity", 6: "cadence"}
 
 _lib = None
 

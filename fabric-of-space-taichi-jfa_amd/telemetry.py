@@ -54,6 +54,31 @@ def jfa_step4_lines(stats, validation, t_jfa, res, n):
     return lines
 
 
+def jfa_schedule_lines(rec, sched_frames, warmstart_frames, th_rsigma, th_margin_frac,
+                       th_fsc_drift, max_skip):
+    """run.py:988-1015: the temporal-skip decision lines.
+
+    rec: a frame record with jfa_reason, jfa_since and jfa_signals (FoamSim's
+    schedule_record()); sched_frames: the scheduler's frame count (1 on the
+    first decision); the rest the configuration's thresholds.
+    """
+    reason, since, sig = rec["jfa_reason"], rec["jfa_since"], rec["jfa_signals"]
+    if reason != "skip":
+        lines = []
+        if reason == "warm-start" and sched_frames == 1:
+            lines.append(f"[JFA Skip] Warm-start: running every frame for first "
+                         f"{warmstart_frames} frames")
+        if reason == "out-of-band-guard":
+            reason = f"out-of-band-guard ({sig['out_of_band'] * 100:.1f}%)"
+        lines.append(f"[JFA Run ] reason={reason} | since={since}")
+        return lines
+    th_margin = th_margin_frac * sig["r_mean"]
+    return [f"[JFA Skip] σ(Δr)/r={sig['sigma_dr_over_r']:.4f} ≤ {th_rsigma:.4f} | "
+            f"vmax*dt={sig['vmax_dt']:.6f} ≤ {th_margin:.6f} | "
+            f"ΔμFSC={sig['delta_fsc_mu']:.3f} ≤ {th_fsc_drift:.3f} | "
+            f"since={since}/{max_skip}"]
+
+
 def dirty_tiles_line(dirty, total, halo, pos_thr, rad_thr, res, L=1.0):
     """run.py:1156-1163: the dirty-tile instrumentation line."""
     pct = 100.0 * dirty / max(1, total)

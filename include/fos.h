@@ -24,7 +24,7 @@
 
 #define FOS_OK 0
 #define FOS_EINVAL (-22)
-#define FOS_ABI_VERSION 1
+#define FOS_ABI_VERSION 2
 
 #ifdef __cplusplus
 extern "C" {
@@ -412,6 +412,16 @@ typedef struct fos_frame_desc {
   int ou_jitter;
   float dt, jitter_rms, jitter_tau, drift_cap_frac;
   int jfa_every, jfa_warm, slot_records;
+  /* which frames run the JFA: 0 the cadence above (jfa_every, jfa_warm),
+   * 1 the reference's temporal skip (fos_jfa_schedule, run.py:902-1018) */
+  int schedule;
+  int skip_warmstart, max_skip;       /* JFA_WARMSTART_FRAMES, MAX_SKIP_FRAMES */
+  double th_rsigma, th_margin, th_fsc_drift, oob_frac;  /* TH_*, out-of-band guard */
+  /* adaptive resolution on JFA frames (run.py:1049-1068): res =
+   * round(L_d * voxels_per_diameter / (2 r_mean)) clamped to [res_min,
+   * res_max] (res_max <= the context's max_res) */
+  int adaptive_res, res_min, res_max;
+  double voxels_per_diameter, L_d;
 } fos_frame_desc;
 typedef struct fos_frame_bufs {
   float *pos[2];
@@ -425,13 +435,35 @@ typedef struct fos_frame_bufs {
   int32_t *info;
   float *maxd, *maxd_host;
   int32_t *settle_dev;
+  /* scheduler (schedule 1 or adaptive_res): positions and radii at the last
+   * decision, and pinned host memory of 8 doubles for the statistics */
+  float *pos_prev, *rad_prev;
+  double *sched_host;
 } fos_frame_bufs;
 typedef struct fos_frame_state {
   int frame, pos_cur, sidx, settle_force, settle_gate, need_rebuild, pending;
   int pbd_passes, jfa_ran, gate_read;
   float max_depth, gate;
   void *event;
+  /* scheduler state (run.py:165-172, 904-1018): decisions made, frames since
+   * the last JFA, whether pos_prev/rad_prev hold a decision's values,
+   * whether last_fsc_mu is set (and whether the next decision sets it: the
+   * FSC mean after a JFA), the band-change lockout, the last decision's
+   * reason (FOS_JFA_* below) and signals (sigma(|dr|)/r_mean, max |dpos|,
+   * |d mean FSC|, out-of-band share, r_mean), and the resolution the last
+   * JFA frame ran at (0: desc->res) */
+  int sched_frames, since, have_prev, fsc_mu_set, fsc_mu_pending, lockout, reason, jfa_res;
+  double last_fsc_mu;
+  double signals[5];
 } fos_frame_state;
+/* reasons of a scheduling decision, in the reference's priority order */
+#define FOS_JFA_SKIP 0
+#define FOS_JFA_WARMSTART 1
+#define FOS_JFA_BAND_CHANGE 2
+#define FOS_JFA_WATCHDOG 3
+#define FOS_JFA_OUT_OF_BAND 4
+#define FOS_JFA_ACTIVITY 5
+#define FOS_JFA_CADENCE 6
 int fos_frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
                    fos_frame_state *s, void *const *pass_events, void *stream);
 /* `frames` calls of fos_frame_step in a loop, frame s->frame onward, with
@@ -444,6 +476,27 @@ int fos_frame_run(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b
                   void *stream);
 /* Frees what fos_frame_step created in the state (its host-read event). */
 int fos_frame_release(fos_frame_state *s);
+/* run.py:902-1018 signal measurement of the temporal-skip scheduler, on the
+ * device, with rad_prev/pos_prev then set to rad/pos (the reference's
+ * rad_prev_cached and pos_prev): out[0] mean |rad - rad_prev|, out[1] its
+ * population standard deviation, out[2] max |pos - pos_prev| at the minimum
+ * image (|d| > L/2 -> d - sign(d) L; fp32 as numpy computes it), out[3]
+ * mean radius, out[4] mean FSC, out[5] particles with fsc < f_low or fsc >
+ * f_high.  Sums in fp64 in a fixed order (deterministic).  With have_prev 0
+ * out[0..2] are 0.  out: device memory of 8 doubles. */
+int fos_sched_stats(fos_ctx *ctx, const float *pos, const float *rad, const int32_t *fsc, int n,
+                    float L, int f_low, int f_high, float *pos_prev, float *rad_prev,
+                    int have_prev, double *out, void *stream);
+/* The JFA decision of frame s->frame, made where the reference makes it
+ * (after the dynamics, run.py:902-1018): with desc->schedule 1 the
+ * temporal-skip triggers (warm start, watchdog, band-change lockout,
+ * activity, out-of-band guard) from fos_sched_stats and one host read, else
+ * the cadence; with desc->adaptive_res the resolution of a JFA frame from
+ * the mean radius (run.py:1049-1068).  Sets *run, *res and the state's
+ * scheduler fields; pos is the current positions.  No reference counterpart
+ * as an entry point (the reference decides in its Python loop). */
+int fos_jfa_schedule(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_bufs *b,
+                     fos_frame_state *s, const float *pos, int *run, int *res, void *stream);
 
 /* ---- topology.py ------------------------------------------------------- */
 

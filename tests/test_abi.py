@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
     lib.fos_abi_version.restype = ctypes.c_int
-    assert lib.fos_abi_version() == 1
+    assert lib.fos_abi_version() == 2
 
 
 def test_ctypes_table_covers_header():
