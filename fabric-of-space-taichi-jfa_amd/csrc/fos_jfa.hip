@@ -1404,12 +1404,12 @@ static int jfa_variant(const fos_ctx *ctx) {
 
 // Whether a pass takes its x-planes in residue-class order (jfa_plane).  By
 // default only where it was measured faster: grids of 256^3 and more, whose
-// x-slab per XCD exceeds its L2, at steps 4 .. res/16 (at larger steps the
-// classes are short and the planes they share are few; at steps 1-3 the
-// order is nearly the natural one).
+// x-slab per XCD exceeds its L2, at steps 2 .. res/16 (at larger steps the
+// classes are short and the planes they share are few; at step 1 the order
+// is the natural one).
 static bool jfa_class_order(const fos_ctx *ctx, int res, int step) {
   if (g_jfa_variant >= 0) return (g_jfa_variant & 4) != 0;
-  return res >= 256 && step >= 4 && step * 16 <= res;
+  return res >= 256 && step >= 2 && step * 16 <= res;
 }
 
 template <bool U, int MODE, typename LT, int BATCH, int DM>
