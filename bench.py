@@ -97,6 +97,34 @@ def traffic_fields(path):
                              if traffic is not None else "no counter profile")}
 
 
+PMC_FILE = os.path.join(ROOT, "profiles", "k_jfa_pass_list_c2_pmc.md")
+# wave64 VALU instructions the chip can issue per second: 256 CUs x 4 SIMDs,
+# one instruction per SIMD every 4 cycles (16 lanes), 2.4 GHz
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+
+def valu_fields(avg_launch_s, path=PMC_FILE):
+    """The pass kernel against its instruction-issue ceiling: the profile's
+    mean VALU instructions per launch over the bench's own mean launch time,
+    when the profile was taken on this kernel source."""
+    insts = sig = None
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("valu_wave_insts_per_launch:"):
+                    insts = float(line.split(":", 1)[1])
+                elif line.startswith("kernel_signature:"):
+                    sig = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    if insts is None or sig != kernel_signature():
+        return {"valu_issue": None}
+    rate = insts / avg_launch_s
+    return {"valu_issue": {"achieved": round(rate / 1e9, 1), "peak": round(VALU_PEAK_WAVE_INSTS / 1e9, 1),
+                           "unit": "G wave64 VALU inst/s", "frac": round(rate / VALU_PEAK_WAVE_INSTS, 3),
+                           "source": os.path.relpath(path, ROOT)}}
+
+
 def copy_calibration():
     """Three launches of a plain int32 copy of COPY_CAL_RES^3 elements, for
     scaling the FETCH_SIZE / WRITE_SIZE counters (run under rocprofv3 --pmc)."""
@@ -550,6 +578,7 @@ def main():
                      "avg_launch_us": round(avg_pass_s * 1e6, 3),
                      "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)],
                      "timed_frames_with_events": len(pass_ms),
+                     **valu_fields(avg_pass_s),
                      "copy_measured_gbs": copy_bandwidth(),
                      "limiter": "instruction issue and latency at full occupancy: VALU 56-70% and "
                                 "texture-address 63-65% busy, waves waiting on a dependency 35-42% "
