@@ -423,6 +423,80 @@ __global__ void __launch_bounds__(256)
   block_count(bal, stripes, pass);
 }
 
+// The pass at step = res / 2 (jfa.py:524-623 with 2 * step == res).  Then
+// -step and +step are the same voxel, so voxel v's 27 candidates sit on the
+// 8 voxels of its orbit {v + step * e : e in {0,1}^3}, and every voxel of an
+// orbit has the same 8 candidates.  One thread takes one orbit: it loads the
+// orbit's 8 labels once and decides its 8 voxels.  For the voxel at orbit
+// element e, candidate (a, b, c) of the scan sits at element e ^ m with m =
+// (a != 1, b != 1, c != 1), so the candidates' first occurrences in scan
+// order are e ^ 7, e ^ 6, ..., e ^ 1, e ^ 0 = the voxel itself: the
+// incumbent, then those seven in that order, strict '<' (a repeat never
+// wins).  Right after the seeding most orbits hold no owner at all, so the
+// pass costs about its memory traffic.  DM: 1 also writes each voxel's
+// winning power distance (1e10 when no owner).
+template <bool UNIT_L, int DM>
+__global__ void __launch_bounds__(256)
+    k_jfa_half(const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel,
+               const int32_t *__restrict__ in, int32_t *__restrict__ out,
+               int *__restrict__ stripes, int pass, int early, double th, int min_passes,
+               float *__restrict__ dist_out) {
+  if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
+  const int s = res >> 1;
+  const int64_t orbits = (int64_t)s * s * s;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int changed = 0;
+  if (o < orbits) {
+    const int oz = (int)(o % s), oy = (int)((o / s) % s), ox = (int)(o / ((int64_t)s * s));
+    int lab[8];
+    size_t vi[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int x = ox + ((e >> 2) & 1) * s, y = oy + ((e >> 1) & 1) * s, z = oz + (e & 1) * s;
+      vi[e] = ((size_t)x * res + y) * res + z;
+      lab[e] = in[vi[e]];
+    }
+    int any = 0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) any |= lab[e] >= 0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      int best = lab[e];
+      float bd = 1e10f;
+      if (any) {
+        const int x = ox + ((e >> 2) & 1) * s, y = oy + ((e >> 1) & 1) * s, z = oz + (e & 1) * s;
+        const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
+                    cz = vcenter(z, voxel, half_l);
+        if (best >= 0) bd = power_dist<UNIT_L>(cx, cy, cz, seeds[best], L);
+#pragma unroll
+        for (int m = 7; m >= 1; --m) {
+          const int c = lab[e ^ m];
+          if (c >= 0 && c != best) {
+            const float pd = power_dist<UNIT_L>(cx, cy, cz, seeds[c], L);
+            if (pd < bd) {
+              bd = pd;
+              best = c;
+            }
+          }
+        }
+      }
+      out[vi[e]] = best;
+      if (DM) dist_out[vi[e]] = bd;
+      changed += best != lab[e];
+    }
+  }
+  // the block's changed voxels, one atomic per block (block_count's sum)
+  __shared__ int s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  int w = changed;
+  for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+  if ((threadIdx.x & 63) == 0 && w) atomicAdd(&s_cnt, w);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_cnt)
+    atomicAdd(stripes + (pass * FOS_STRIPES + blockIdx.x % FOS_STRIPES) * FOS_STRIPE_INTS, s_cnt);
+}
+
 // @jfa-pass-kernel-end
 
 // Ends the pass loop: records passes executed and the exit flag, and copies
@@ -1397,6 +1471,14 @@ extern int g_grid_cpw;    // fos_grid.hip
 // distance buffers (grids up to 256^3) and the class order per
 // jfa_class_order.
 static int g_jfa_variant = -1;
+// fos_set_param "jfa_half_orbits": 1 runs the step = res / 2 pass by orbits
+// (k_jfa_half), 0 with the list kernel's JFA_HALF flavour, -1 (default) by
+// orbits on grids of 256^3 and more.  One thread per orbit loads each label
+// once where the list kernel loads it eight times, which pays where the
+// labels come from HBM (C3: 418 -> 167 us); on a grid that fits in the L2s
+// the list kernel's eight times as many waves hide latency better (C2: 18.9
+// against 23.2 us).
+static int g_jfa_half_orbits = -1;
 static int jfa_variant(const fos_ctx *ctx) {
   if (g_jfa_variant >= 0) return g_jfa_variant;
   return (ctx->max_n > 65536 ? 1 : 0) | (ctx->dist[0] ? 2 : 0);
@@ -1480,6 +1562,24 @@ static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
                        const float *dist_in = nullptr, float *dist_out = nullptr) {
   const float half_l = (float)(0.5 * (double)L);
   const bool u = unit_l(L);
+  const bool orbits = g_jfa_half_orbits < 0 ? res >= 256 : g_jfa_half_orbits != 0;
+  if (2 * step == res && orbits && dm != 2 && !chg_out) {
+    // the step = res / 2 pass by orbits (k_jfa_half; it writes no change
+    // masks, which only a third pass in a row at one step reads)
+    const int s = res / 2;
+    const unsigned b = blocks_for((int64_t)s * s * s);
+#define FOS_HALF(U, D)                                                                           \
+  hipLaunchKernelGGL((k_jfa_half<U, D>), dim3(b), dim3(256), 0, st, ctx->seeds, res, L, half_l, \
+                     voxel, in, out, ctx->stripes, pass, early, th, min_passes, dist_out)
+    if (u) {
+      if (dm) FOS_HALF(true, 1); else FOS_HALF(true, 0);
+    } else {
+      if (dm) FOS_HALF(false, 1); else FOS_HALF(false, 0);
+    }
+#undef FOS_HALF
+    FOS_CHECK_LAUNCH("k_jfa_half");
+    return 0;
+  }
   const int mode = 2 * step == res ? JFA_HALF
                    : settled       ? (chg_prev ? JFA_INCR : JFA_SETTLED)
                                    : JFA_PLAIN;
@@ -1566,7 +1666,10 @@ extern "C" {
 
 int fos_set_param(const char *name, int value) {
   FOS_REQUIRE(name, "fos_set_param: name is NULL");
-  if (!strcmp(name, "jfa_variant")) {
+  if (!strcmp(name, "jfa_half_orbits")) {
+    FOS_REQUIRE(value >= -1 && value <= 1, "jfa_half_orbits must be -1, 0 or 1");
+    g_jfa_half_orbits = value;
+  } else if (!strcmp(name, "jfa_variant")) {
     FOS_REQUIRE(value >= -1 && value <= 7, "jfa_variant must be in [-1, 7]");
     g_jfa_variant = value;
   } else if (!strcmp(name, "face_emit_tx")) {
@@ -1601,6 +1704,8 @@ int fos_get_param(const char *name, int *value) {
   FOS_REQUIRE(name && value, "fos_get_param: bad arguments");
   if (!strcmp(name, "jfa_variant"))
     *value = g_jfa_variant;
+  else if (!strcmp(name, "jfa_half_orbits"))
+    *value = g_jfa_half_orbits;
   else if (!strcmp(name, "grid_block"))
     *value = g_grid_block;
   else if (!strcmp(name, "face_emit_tx"))

@@ -185,6 +185,48 @@ def test_jfa_pass_kernel_variants(gpu, orc, variant):
         gpu.call("fos_set_param", b"jfa_variant", v0.value)
 
 
+@pytest.mark.parametrize("orbits", [1, 0])
+def test_jfa_half_pass_by_orbits(gpu, orc, orbits):
+    """The step = res/2 pass by orbits (k_jfa_half, default) and by the list
+    kernel give the oracle's labels and change counts: dense seeds (several
+    owners per orbit), coincident particles (equal-distance ties between
+    different ids), res 66 (step 33), and the distance carry (C2-size
+    context) as well as 32-bit lists (ids above 65,535)."""
+    import ctypes
+    v0 = ctypes.c_int()
+    gpu.call("fos_get_param", b"jfa_half_orbits", ctypes.byref(v0))
+    try:
+        gpu.call("fos_set_param", b"jfa_half_orbits", orbits)
+        for (n, res, seed, dup) in ((20000, 32, 3, False), (3000, 16, 2, True), (900, 66, 5, False),
+                                    (10000, 128, 0, False), (70000, 96, 4, False)):
+            pos, rad = fi.uniform(n, 0.005, 0.03, seed=seed)
+            if dup:
+                pos[1::2] = pos[0::2]
+                rad[1::2] = rad[0::2]
+            voxel = fc.f32(1.0 / res)
+            P = int(math.ceil(math.log2(res))) + 1
+            print(f"[jfa half] orbits={orbits} n={n} res={res}", flush=True)
+            lab, passes, ex, hist = orc.jfa_run(pos, rad, n, res, 1.0, voxel, P)
+            glab, info = run_jfa_gpu(gpu, pos, rad, n, res, P)
+            assert (glab == lab).all()
+            assert info[0] == passes and (info[2:2 + passes] == hist).all()
+            # the first pass alone, through the single-step entry point
+            out, ch = orc.jfa_step(pos, rad, res, 1.0, voxel, res // 2,
+                                   orc.jfa_init(pos, rad, n, res, 1.0, voxel)[0])
+            g = G(gpu, n, res)
+            tp, tr = dev(pos), dev(rad)
+            tin = torch.empty(res ** 3, dtype=torch.int32, device="cuda")
+            tout = torch.empty_like(tin)
+            g.call("fos_jfa_init", g.h, g.ptr(tp), g.ptr(tr), n, res, 1.0, voxel, 1.0,
+                   g.ptr(tin), None, gpu.stream())
+            chg = torch.zeros(1, dtype=torch.int32, device="cuda")
+            g.call("fos_jfa_step", g.h, res, 1.0, voxel, res // 2, g.ptr(tin), g.ptr(tout),
+                   g.ptr(chg), gpu.stream())
+            assert (host(tout) == out).all() and int(host(chg)[0]) == ch
+    finally:
+        gpu.call("fos_set_param", b"jfa_half_orbits", v0.value)
+
+
 @pytest.mark.parametrize("res,extra", [(66, 3), (130, 2), (64, 4)])
 def test_jfa_repeated_final_passes(gpu, orc, res, extra):
     """Several step-1 passes in a row: from the third on, a pass recomputes
