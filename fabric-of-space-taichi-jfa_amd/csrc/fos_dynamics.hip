@@ -1573,15 +1573,41 @@ __global__ void k_count_zero_b(const int32_t *__restrict__ fsc, const int32_t *_
 // loads are contiguous and independent of the id load.  Lanes first count
 // their cell's corrections, a prefix over the 27 lanes gives each its place
 // in the serial (cell, occupant) order, and the lanes then write the
-// corrections there for lane 0 to add in that order; more than PBD_HITS
-// corrections send the particle through the serial loop.
+// corrections there for lane 0 to add in that order, PBD_HITS at a time.
 #define PBD_HITS 48
+
+// pbd_hit's contact test alone (dist < ts with dist = sqrtf(d.d + 1e-12)),
+// rejecting first, without the square root, every pair whose squared
+// distance s exceeds fl(fl(ts*ts) * 1.0001f) >= 1.0000999 ts^2: then sqrt(s) >
+// 1.00004 ts, and the correctly rounded sqrtf(s) is still above ts, so the
+// exact test would reject it too.
+// gwrap; with U (L == 1) the products by L and 1/L are exact, so d -
+// floorf(d + 0.5f) gives the same bits in two fewer operations.
+template <bool U>
+__device__ __forceinline__ float gw(float d, float L, float inv_l) {
+  return U ? d - floorf(d + 0.5f) : gwrap(d, L, inv_l);
+}
+
+template <bool U>
+__device__ __forceinline__ bool pbd_contact(float qx, float qy, float qz, float rj, float px,
+                                            float py, float pz, float ri, float gap, float L,
+                                            float inv_l) {
+  const float dx = gw<U>(qx - px, L, inv_l);
+  const float dy = gw<U>(qy - py, L, inv_l);
+  const float dz = gw<U>(qz - pz, L, inv_l);
+  const float s = dot3(dx, dy, dz) + 1e-12f;
+  const float ts = (ri + rj) * gap;
+  if (s > ts * ts * 1.0001f) return false;
+  return sqrtf(s) < ts;
+}
+
+template <bool U = false>
 __device__ __forceinline__ bool pbd_hit(float qx, float qy, float qz, float rj, float px,
                                         float py, float pz, float ri, float maxd, float gap,
                                         float L, float inv_l, float &vx, float &vy, float &vz) {
-  const float dx = gwrap(qx - px, L, inv_l);
-  const float dy = gwrap(qy - py, L, inv_l);
-  const float dz = gwrap(qz - pz, L, inv_l);
+  const float dx = gw<U>(qx - px, L, inv_l);
+  const float dy = gw<U>(qy - py, L, inv_l);
+  const float dz = gw<U>(qz - pz, L, inv_l);
   const float dist = sqrtf(dot3(dx, dy, dz) + 1e-12f);
   const float ts = (ri + rj) * gap;
   if (!(dist < ts)) return false;
@@ -1596,8 +1622,12 @@ __device__ __forceinline__ bool pbd_hit(float qx, float qy, float qz, float rj, 
 
 // LPP lanes per particle (8, 16 or 32; 256 / LPP particles per block): lane
 // l walks stencil cells [l*CPL, l*CPL + CPL) in order, so the hits, placed by
-// an exclusive prefix over the lanes, stay in stencil-then-slot order.
-template <bool BATCH, int LPP>
+// an exclusive prefix over the lanes, stay in stencil-then-slot order.  The
+// walk records a lane's first PBD_HK hit slots in LDS, so a lane with at
+// most that many hits writes its corrections from those slots instead of
+// walking its cells again.  U: L == 1.
+#define PBD_HK 4
+template <bool BATCH, int LPP, bool U = false>
 __global__ void __launch_bounds__(256)
     k_pbd_pr(const float *__restrict__ pin, float *__restrict__ pout,
              const float4 *__restrict__ pr, const float *__restrict__ rad, int n, int G0, float L,
@@ -1608,6 +1638,8 @@ __global__ void __launch_bounds__(256)
              int per_xcd, int32_t *__restrict__ cell_of) {
   constexpr int NPB = 256 / LPP, CPL = (27 + LPP - 1) / LPP;
   __shared__ float s_v[NPB][PBD_HITS][3];
+  __shared__ int s_hk[PBD_HK][256];
+  const int t = (int)threadIdx.x;
   const int grp = (int)(threadIdx.x / LPP), l = (int)(threadIdx.x % LPP);
   // particles in cell order (slot k of the grid holds particle ci[k]): a
   // workgroup's particles and an XCD's run of workgroups read the same
@@ -1652,15 +1684,17 @@ __global__ void __launch_bounds__(256)
       }
     }
   }
+  // particle i sits in slot kk, so "slot k is not i" is k != kk
   int h = 0;
   float vx, vy, vz;
 #pragma unroll
   for (int u = 0; u < CPL; ++u)
     for (int k = st[u]; k < st[u] + ct[u]; ++k) {
       const float4 q = pr[k];
-      if (ci[k] != i &&
-          pbd_hit(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz))
+      if (k != kk && pbd_contact<U>(q.x, q.y, q.z, q.w, px, py, pz, ri, gap, L, inv_l)) {
+        if (h < PBD_HK) s_hk[h][t] = k;
         ++h;
+      }
     }
   int x = h;  // inclusive prefix over the particle's lanes
   for (int o = 1; o < LPP; o <<= 1) {
@@ -1668,43 +1702,62 @@ __global__ void __launch_bounds__(256)
     if (l >= o) x += y;
   }
   const int H = __shfl(x, LPP - 1, LPP);
-  const bool fits = H <= PBD_HITS;
-  if (live && fits && h > 0) {
-    int m = x - h;
-#pragma unroll
-    for (int u = 0; u < CPL; ++u)
-      for (int k = st[u]; k < st[u] + ct[u]; ++k) {
-        const float4 q = pr[k];
-        if (ci[k] != i &&
-            pbd_hit(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz)) {
-          s_v[grp][m][0] = vx;
-          s_v[grp][m][1] = vy;
-          s_v[grp][m][2] = vz;
-          ++m;
-        }
-      }
-  }
+  // The hits go through LDS in windows of PBD_HITS, lane 0 adding each
+  // window in order, so a particle with more hits than one window (clustered
+  // foams) keeps its lanes instead of walking the 27 cells alone; every
+  // group of the block runs the block's largest window count (one in the
+  // usual case).
+  __shared__ int s_rounds;
+  if (threadIdx.x == 0) s_rounds = 1;
   __syncthreads();
-  if (!live || l != 0) return;
+  if (live && l == 0 && H > PBD_HITS) atomicMax(&s_rounds, (H + PBD_HITS - 1) / PBD_HITS);
+  __syncthreads();
+  const int rounds = s_rounds;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  if (fits) {
-    for (int q = 0; q < H; ++q) {
-      ax = ax + s_v[grp][q][0];
-      ay = ay + s_v[grp][q][1];
-      az = az + s_v[grp][q][2];
-    }
-  } else {
-    for_stencil(cx, cy, cz, G, cs, cc, ci, [&](int j) {
-      bool hit;
-      float wx, wy, wz;
-      pbd_pair(j, i, px, py, pz, ri, maxd, gap, L, inv_l, pin, rad, hit, wx, wy, wz);
-      if (hit) {
-        ax = ax + wx;
-        ay = ay + wy;
-        az = az + wz;
+  for (int r = 0; r < rounds; ++r) {
+    const int lo = r * PBD_HITS, hi = lo + PBD_HITS;
+    if (live && h > 0 && x > lo && x - h < hi) {
+      int m = x - h;
+      if (h <= PBD_HK) {
+        for (int j = 0; j < h; ++j, ++m) {
+          const float4 q = pr[s_hk[j][t]];
+          pbd_hit<U>(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy, vz);
+          if (m >= lo && m < hi) {
+            s_v[grp][m - lo][0] = vx;
+            s_v[grp][m - lo][1] = vy;
+            s_v[grp][m - lo][2] = vz;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < CPL; ++u)
+          for (int k = st[u]; k < st[u] + ct[u]; ++k) {
+            const float4 q = pr[k];
+            if (k != kk && pbd_contact<U>(q.x, q.y, q.z, q.w, px, py, pz, ri, gap, L, inv_l) &&
+                pbd_hit<U>(q.x, q.y, q.z, q.w, px, py, pz, ri, maxd, gap, L, inv_l, vx, vy,
+                           vz)) {
+              if (m >= lo && m < hi) {
+                s_v[grp][m - lo][0] = vx;
+                s_v[grp][m - lo][1] = vy;
+                s_v[grp][m - lo][2] = vz;
+              }
+              ++m;
+            }
+          }
       }
-    });
+    }
+    __syncthreads();
+    if (live && l == 0) {
+      const int e = H - lo < PBD_HITS ? H - lo : PBD_HITS;
+      for (int q = 0; q < e; ++q) {
+        ax = ax + s_v[grp][q][0];
+        ay = ay + s_v[grp][q][1];
+        az = az + s_v[grp][q][2];
+      }
+    }
+    if (r + 1 < rounds) __syncthreads();
   }
+  if (!live || l != 0) return;
   const float maxt = maxd * dmul;
   const float cm = sqrtf(dot3(ax, ay, az));
   if (cm > maxt) {
@@ -1729,8 +1782,9 @@ __global__ void __launch_bounds__(256)
 }
 
 // Lanes per particle of the slot-record PBD pass (a speed parameter; 0:
-// by size -- small foams are latency-bound and gain from 32 lanes each,
-// batches of millions of particles from 8 lanes doing 4 cells each).
+// 32 lanes (one stencil cell each) for single foams, 8 (four cells each)
+// for batches of more than 131,072 particles, where it measured 3% faster;
+// a single clustered foam of 200,000 is 1.6% faster with 32).
 static int g_pbd_lanes = 0;
 
 // Launches k_pbd_pr with g_pbd_lanes lanes per particle.
@@ -1741,18 +1795,20 @@ static void launch_pbd_pr(hipStream_t st, int n, const float *pin, float *pout, 
                           const int32_t *pinst, const fos_inst *inst, const int32_t *npass,
                           int pass, int32_t *cell_of) {
   const float half_l = (float)(0.5 * (double)L), inv_l = (float)(1.0 / (double)L);
-  const int lpp = g_pbd_lanes ? g_pbd_lanes : (n >= 131072 ? 8 : 32), npb = 256 / lpp;
+  const int lpp = g_pbd_lanes ? g_pbd_lanes : (BATCH && n >= 131072 ? 8 : 32), npb = 256 / lpp;
   const int per = ((n + npb - 1) / npb + FOS_XCDS - 1) / FOS_XCDS;
-#define PBD_PR(LP)                                                                               \
-  hipLaunchKernelGGL((k_pbd_pr<BATCH, LP>), dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0, st, \
-                     pin, pout, (const float4 *)pr, rad, n, G, L, half_l, inv_l, gap, mdf, dmul, \
-                     cs, cc, ci, pinst, inst, npass, pass, per, cell_of)
-  if (lpp == 8)
-    PBD_PR(8);
-  else if (lpp == 16)
-    PBD_PR(16);
-  else
-    PBD_PR(32);
+#define PBD_PR(LP, U)                                                                            \
+  hipLaunchKernelGGL((k_pbd_pr<BATCH, LP, U>), dim3((unsigned)(per * FOS_XCDS)), dim3(256), 0,  \
+                     st, pin, pout, (const float4 *)pr, rad, n, G, L, half_l, inv_l, gap, mdf,    \
+                     dmul, cs, cc, ci, pinst, inst, npass, pass, per, cell_of)
+  const bool u = L == 1.0f;
+  if (lpp == 8) {
+    if (u) PBD_PR(8, true); else PBD_PR(8, false);
+  } else if (lpp == 16) {
+    if (u) PBD_PR(16, true); else PBD_PR(16, false);
+  } else {
+    if (u) PBD_PR(32, true); else PBD_PR(32, false);
+  }
 #undef PBD_PR
 }
 

@@ -654,6 +654,47 @@ def test_pbd_pass_cells_equals_pass_then_rebuild(gpu, orc, block, pr):
         assert np.array_equal(rec[:, :3], want[0][ci]) and np.array_equal(rec[:, 3], rad[ci])
 
 
+@pytest.mark.parametrize("lanes", [8, 16, 32])
+def test_pbd_crowded_hits_take_several_windows(gpu, orc, lanes):
+    """A crowded cluster: particles with more corrections than one LDS window
+    (48) add them in several windows; the pass (slot records, any lanes per
+    particle) equals the oracle's serial Jacobi pass bit for bit."""
+    n, G_ = 3000, 10
+    rng = np.random.default_rng(11)
+    pos, rad = fi.uniform(n, 0.02, 0.04, seed=7)
+    pos[:600] = (rng.normal(0.0, 0.03, (600, 3))).astype(F)      # a dense ball
+    rad[:600] = F(0.045)
+    p, cc, cs, ci = orc.grid_build(pos, n, G_, 1.0)
+    gap = fc.f32(1.015)
+    moved = orc.pbd_jacobi(p, rad, n, G_, 1.0, gap, 0.2, 2.0, cs, cc, ci)
+    want = orc.grid_build(moved, n, G_, 1.0)
+    # the cluster really overflows one window (count the hits on the CPU)
+    d = p[:600, None, :] - p[None, :600, :]
+    d -= np.round(d)
+    dist = np.sqrt((d * d).sum(-1) + F(1e-12))
+    hits = ((dist < (rad[:600, None] + rad[None, :600]) * gap).sum(1) - 1).max()
+    assert hits > 2 * 48, hits
+
+    def run():
+        g = G(gpu, n, 1, G_ ** 3)
+        tp, tr = dev(pos), dev(rad)
+        tcc = torch.empty(G_ ** 3, dtype=torch.int32, device="cuda")
+        tcs = torch.empty_like(tcc)
+        tci = torch.empty(n, dtype=torch.int32, device="cuda")
+        rec = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+        g.call("fos_grid_build_pr", g.h, g.ptr(tp), g.ptr(tr), n, G_, 1.0, g.ptr(tcc),
+               g.ptr(tcs), g.ptr(tci), g.ptr(rec), gpu.stream())
+        out = torch.empty_like(tp)
+        g.call("fos_pbd_pass_cells", g.h, g.ptr(tp), g.ptr(out), g.ptr(tr), n, G_, 1.0,
+               gap, 0.2, 2.0, g.ptr(tcs), g.ptr(tcc), g.ptr(tci), g.ptr(rec), gpu.stream())
+        g.call("fos_grid_build_from_cells", g.h, g.ptr(out), g.ptr(tr), n, G_, g.ptr(tcc),
+               g.ptr(tcs), g.ptr(tci), None, gpu.stream())
+        return host(out), host(tcc), host(tcs), host(tci)
+    got = with_param(gpu, "pbd_lanes", lanes, run)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("case", ["c1", "overflow"])
 def test_jfa_faces_equals_accumulate_then_finalize(gpu, orc, case):
     """fos_jfa_faces (emission, compaction and finalize behind one host read)
