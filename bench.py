@@ -207,9 +207,12 @@ EVENT_EVERY = 8   # timed frames between two with HIP events around every JFA pa
 
 
 def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
-    """Wall time of `steps` frames after `warmup`; with_events records HIP
-    events around each JFA pass on every EVENT_EVERY-th timed frame (event
-    records cost the stream time, so not on every frame)."""
+    """Wall time of `steps` frames after `warmup`.  with_events: on every
+    EVENT_EVERY-th timed frame HIP events bracket each JFA pass (the per-pass
+    times), and halfway between those frames two events bracket the whole pass
+    schedule (the mean launch time without a record between the passes; each
+    record costs the stream a few microseconds).  Returns (seconds, per-pass
+    ms per instrumented frame, whole-schedule ms per bracketed frame, sim)."""
     import torch
 
     import fos_lib as fl
@@ -220,7 +223,7 @@ def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
     run_frames(sim, warmup)
     torch.cuda.synchronize()
     P = cfg.passes()
-    ev_arrays = []
+    ev_arrays, ends = [], []
     if with_events:
         slots = []
         for k in range(steps):
@@ -229,6 +232,12 @@ def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
                 arr = (ctypes.c_void_p * (P + 1))()
                 fl.call("fos_events_create", P + 1, arr)
                 ev_arrays.append(arr)
+            elif k % EVENT_EVERY == EVENT_EVERY // 2:
+                two = (ctypes.c_void_p * 2)()
+                fl.call("fos_events_create", 2, two)
+                arr = (ctypes.c_void_p * (P + 1))()
+                arr[0], arr[P] = two[0], two[1]
+                ends.append(two)
             slots.append(arr)
         sim.pass_events = slots
     torch.cuda.synchronize()
@@ -236,13 +245,18 @@ def timed(cfg, pos, rad, steps, warmup, with_events, setup=None):
     run_frames(sim, steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    pass_ms = []
+    pass_ms, sched_ms = [], []
     for arr in ev_arrays:
         ms = (ctypes.c_float * P)()
         fl.call("fos_events_elapsed", P + 1, arr, ms)
         pass_ms.append(list(ms))
         fl.call("fos_events_destroy", P + 1, arr)
-    return t1 - t0, pass_ms, sim
+    for two in ends:
+        ms = (ctypes.c_float * 1)()
+        fl.call("fos_events_elapsed", 2, two, ms)
+        sched_ms.append(float(ms[0]))
+        fl.call("fos_events_destroy", 2, two)
+    return t1 - t0, pass_ms, sched_ms, sim
 
 
 def temporal_bench(cfg, pos, rad, steps, warmup):
@@ -326,7 +340,7 @@ def c3_frames_bench(frames, warmup=2):
     import foam_inputs as fi
     cfg = fc.preset("c3")
     pos, rad = fi.for_config(cfg)
-    el, _, sim = timed(cfg, pos, rad, frames, warmup, False)
+    el, _, _, sim = timed(cfg, pos, rad, frames, warmup, False)
     return {"workload": (f"C3 frames: N={cfg.n:,} clustered, radii lognormal clipped to "
                          f"[0.003, 0.015], JFA {cfg.res}^3 every frame, PBD {cfg.pbd_base}.."
                          f"{cfg.pbd_max}, hash {cfg.hash_res()}^3, pressure {cfg.pressure_iters} "
@@ -371,7 +385,7 @@ def surrogate_bench(steps, warmup):
     params, meta = w["params"], _json.loads(str(w["meta"]))
     plain = fc.preset("c5")
     pos, rad = fi.for_config(plain)
-    el_p, _, psim = timed(plain, pos, rad, steps, warmup, False)
+    el_p, _, _, psim = timed(plain, pos, rad, steps, warmup, False)
     fp_ = steps / el_p
     cover = {str(m): candidate_coverage(psim, m) for m in (1.6, meta["mult"])}
     out = {"workload": "N=50000, radii U[0.005,0.011], JFA 256^3; predictor path: JFA every "
@@ -398,7 +412,7 @@ def surrogate_bench(steps, warmup):
     for judge_on in ("per_particle", "mean"):
         surr = fc.preset("c5s")
         surr.judge_on = judge_on
-        el_s, _, sim = timed(surr, pos, rad, steps, warmup, False,
+        el_s, _, _, sim = timed(surr, pos, rad, steps, warmup, False,
                              setup=lambda s: s.set_predictor(params, meta["mult"]))
         judged = [j for j in sim.judged if j[0] >= warmup]
         k = 1 if judge_on == "per_particle" else 2
@@ -547,10 +561,12 @@ def main():
     pos, rad = fi.for_config(cfg)
     V = cfg.res ** 3
     P = cfg.passes()
-    elapsed, pass_ms, sim = timed(cfg, pos, rad, args.steps, args.warmup, True)
+    elapsed, pass_ms, sched_ms, sim = timed(cfg, pos, rad, args.steps, args.warmup, True)
     fps = args.steps / elapsed
     flat = np.array(pass_ms, dtype=np.float64)       # [frames, passes] in ms
-    avg_pass_s = float(flat.mean()) / 1e3
+    # mean launch time from the frames bracketed as a whole (no record
+    # between the passes); the per-pass frames when the run is too short
+    avg_pass_s = (float(np.mean(sched_ms)) / P if sched_ms else float(flat.mean())) / 1e3
     achieved = 8.0 * V / avg_pass_s / 1e9            # GB/s
     gvox = P * V / (float(flat.sum(axis=1).mean()) / 1e3) / 1e9
     out = {
@@ -578,6 +594,7 @@ def main():
                      "avg_launch_us": round(avg_pass_s * 1e6, 3),
                      "per_pass_us": [round(x * 1e3, 2) for x in flat.mean(axis=0)],
                      "timed_frames_with_events": len(pass_ms),
+                     "timed_frames_bracketed": len(sched_ms),
                      **valu_fields(avg_pass_s),
                      "copy_measured_gbs": copy_bandwidth(),
                      "limiter": "instruction issue and latency at full occupancy: VALU 56-70% and "
@@ -596,7 +613,7 @@ def main():
         c2c = fc.preset("c2c")
         # the window starts past the JFA warm start whatever --warmup is
         w2 = max(args.warmup, c2c.jfa_warm)
-        el2, _, _ = timed(c2c, pos, rad, args.steps, w2, False)
+        el2, _, _, _ = timed(c2c, pos, rad, args.steps, w2, False)
         jfa_frames = sum(1 for f in range(w2, w2 + args.steps) if c2c.jfa_runs(f))
         out["cadence"] = {"workload": "same, JFA every 5th frame after a 20-frame warm start",
                           "value": round(args.steps / el2, 3), "unit": "frames/s",

@@ -1782,7 +1782,7 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
     if (step < 1) step = 1;
     const int32_t *in = (p & 1) ? temp : grid;
     int32_t *out = (p & 1) ? grid : temp;
-    if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[p], st));
+    if (events && events[p]) FOS_HIP(hipEventRecord((hipEvent_t)events[p], st));
     // masks are read by the next pass only when it repeats this step a
     // third time in a row (step == prev there)
     const int next = step / 2 < 1 ? 1 : step / 2;
@@ -1802,7 +1802,7 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
     prev = step;
     step /= 2;
   }
-  if (events) FOS_HIP(hipEventRecord((hipEvent_t)events[num_passes], st));
+  if (events && events[num_passes]) FOS_HIP(hipEventRecord((hipEvent_t)events[num_passes], st));
   const int64_t V = (int64_t)res * res * res;
   unsigned b = blocks_for(V);
   if (b > 2048) b = 2048;

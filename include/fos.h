@@ -132,7 +132,9 @@ int fos_jfa_passes(fos_ctx *ctx, int res, float L, float voxel,
                    int32_t *info, void *stream);
 /* fos_jfa_passes that also records events[p] (hipEvent_t) on the stream
  * before pass p and events[num_passes] after the last pass, so each pass
- * kernel can be timed on the stream it runs on.  No reference counterpart:
+ * kernel can be timed on the stream it runs on; NULL entries record nothing
+ * (events[0] and events[num_passes] alone time the whole schedule without a
+ * record between the passes).  No reference counterpart:
  * per-pass timing for the benchmark (the reference times JFA on the host,
  * run.py:1071-1080). */
 int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
