@@ -438,10 +438,78 @@ __device__ __forceinline__ int lcg_hash(int i, int frame) {
   return (int)((1103515245u * t + 12345u) & 0x7fffffffu);
 }
 
+// x^y for y = fp32(1/3) = 1/3 + d, d = 2^-25 / 3, and x a positive normal
+// float, when it can be decided cheaply that (float)pow((double)x, y) is the
+// float returned: x^y = x^(1/3) * x^d with
+//   x^(1/3) = x * u^2, u = x^(-1/3): an fp32 estimate exp2(-log2(x) / 3)
+//     (relative error ~2^-17) refined by three Newton steps in double
+//     u <- u + u (1 - x u^3) / 3 (quadratic: 2^-17 -> 2^-33 -> 2^-65; the
+//     third covers a far worse estimate), leaving the double rounding of the
+//     steps, < 2^-50 for x^(1/3);
+//   x^d = exp(t) = 1 + t + t^2 / 2 with t = d ln x = d ln2 log2(x),
+//     |t| < 9e-7 (t^3 / 6 < 2^-62); log2(x) = exponent + fp32 log2 of the
+//     mantissa in [0.5, 1), abs. error < 2^-22, so t is off by < 2^-49.
+// z is within 2^-48 of x^y.  When z (1 - 2^-44) and z (1 + 2^-44) round to
+// the same float, every real within 2^-44 of z rounds to it: x^y, and the
+// double pow (within a few units of 2^-53 of x^y) with it.  Otherwise
+// (about one argument in 2^19) the caller evaluates the double pow.
+__device__ __forceinline__ bool pow_third_fast(float x, float &out) {
+  if (!(x >= 1.17549435e-38f && x <= 3.40282347e38f)) return false;
+  const double xd = (double)x;
+  double u = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * -0.333333343f);
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const double e = 1.0 - xd * (u * u * u);
+    u = u + u * (e * (1.0 / 3.0));
+  }
+  const double c = xd * (u * u);
+  int ex;
+  const float m = frexpf(x, &ex);
+  const double lg = (double)ex + (double)__builtin_amdgcn_logf(m);
+  const double t = lg * (0x1.5555555555555p-27 * 0x1.62e42fefa39efp-1);
+  const double z = c + c * (t + 0.5 * (t * t));
+  const float lo = (float)(z * (1.0 - 0x1p-44));
+  const float hi = (float)(z * (1.0 + 0x1p-44));
+  out = lo;
+  return lo == hi;
+}
+
 // fp32 power evaluated in double and rounded once: the correctly rounded fp32
 // result except in ~2^-29 of cases, independent of the fp32 pow library.
+// For y = fp32(1/3) (the pressure's inverse volume exponent at P = 3) the
+// same float usually comes from pow_third_fast.
 __device__ __forceinline__ float pow_f32(float x, float y) {
+  float r;
+  if (y == 0.333333343f && pow_third_fast(x, r)) return r;
   return (float)pow((double)x, (double)y);
+}
+
+// Self-check of pow_third_fast: for the positive floats with bit patterns
+// first + k * stride, k < count, counts the arguments where the fast path
+// answers with a float other than (float)pow((double)x, fp32(1/3)) (cnt[0])
+// and those it leaves to the double pow (cnt[1]).
+__global__ void __launch_bounds__(256)
+    k_pow_third_check(unsigned first, unsigned count, unsigned stride,
+                      unsigned long long *__restrict__ cnt) {
+  unsigned long long bad = 0, slow = 0;
+  for (unsigned k = blockIdx.x * 256u + threadIdx.x; k < count; k += gridDim.x * 256u) {
+    const float x = __uint_as_float(first + k * stride);
+    float r;
+    if (pow_third_fast(x, r)) {
+      const float want = (float)pow((double)x, (double)0.333333343f);
+      bad += __float_as_uint(r) != __float_as_uint(want);
+    } else {
+      ++slow;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    bad += __shfl_xor(bad, o);
+    slow += __shfl_xor(slow, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (bad) atomicAdd(&cnt[0], bad);
+    if (slow) atomicAdd(&cnt[1], slow);
+  }
 }
 
 // r^p for the volume: with p == 3 the product (r*r)*r in double is the
@@ -1298,6 +1366,21 @@ int fos_pressure_stats(fos_ctx *ctx, const float *rad, int n, float *out,
     const int b = key >= 0 ? key : (int)(0x80000000u - (unsigned)key);
     memcpy(&out[q], &b, sizeof(float));
   }
+  return 0;
+}
+
+int fos_pow_third_check(unsigned first, unsigned count, unsigned stride,
+                        unsigned long long *counts, void *stream) {
+  FOS_REQUIRE(counts && stride >= 1 && (unsigned long long)first +
+                                               (unsigned long long)(count ? count - 1 : 0) * stride <
+                                           0x80000000ull,
+              "fos_pow_third_check: the bit patterns must stay below 0x80000000");
+  if (count == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned blocks = count / 256 >= 8192 ? 8192u : (count + 255) / 256;
+  hipLaunchKernelGGL(k_pow_third_check, dim3(blocks), dim3(256), 0, st, first, count, stride,
+                     counts);
+  FOS_CHECK_LAUNCH("k_pow_third_check");
   return 0;
 }
 

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libfos.so")
 
 P = ctypes.c_void_p
 I = ctypes.c_int
+U = ctypes.c_uint
 I64 = ctypes.c_int64
 U64 = ctypes.c_uint64
 F = ctypes.c_float
@@ -61,6 +62,7 @@ SIGNATURES = {
     "fos_pressure_iter": [I, I, P, P, P, P, F, F, F, I, F, F, P, P],
     "fos_pressure_run": [P, I, I, I, P, P, P, P, F, F, F, I, F, F, P, P],
     "fos_pressure_stats": [P, P, I, P, P],
+    "fos_pow_third_check": [U, U, U, P, P],
     "fos_brownian": [P, I, F, F, U64, I, I, P],
     "fos_brownian_cells": [P, P, I, F, F, U64, I, I, I, P],
     "fos_settle_nudge": [P, P, P, I, F, F, F, F, I, F, D, P, P, P, I, I, I, P],

@@ -339,6 +339,15 @@ int fos_pressure_run(fos_ctx *ctx, int n, int frame, int iters, float *rad,
 /* dynamics.py:1560-1589 compute_pressure_stats: host out[2] = {rmin, rmax}. */
 int fos_pressure_stats(fos_ctx *ctx, const float *rad, int n, float *out,
                        void *stream);
+/* No counterpart in the reference: a self-check of the fast path that the
+ * pressure kernels take for x^(1/P) at P = 3 (the float the double pow gives,
+ * decided from a Newton cube root when its rounding is unambiguous).  For the
+ * positive floats with bit patterns first + k * stride, k < count (all below
+ * 0x80000000), adds to the device counters counts[0] the arguments where the
+ * fast path returns another float than (float)pow((double)x, fp32(1/3)) and
+ * to counts[1] those it leaves to the double pow.  The caller zeroes counts. */
+int fos_pow_third_check(unsigned first, unsigned count, unsigned stride,
+                        unsigned long long *counts, void *stream);
 /* dynamics.py:1596-1634 apply_brownian_motion with Philox4x32-10 noise keyed
  * (seed, frame, particle).  mode 0: uniform (u-0.5)*2*strength (reference
  * form); mode 1: Gaussian with sigma = strength. */
