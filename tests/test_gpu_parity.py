@@ -869,30 +869,33 @@ def test_pressure_one_launch_equals_oracle(gpu, orc):
     assert v.value == 1
     rng = np.random.default_rng(4)
     other = (rad * rng.uniform(0.9, 1.1, n)).astype(F)
-    cases = [(n, 5, 20, rad), (n, 5, 20, rad), (n, 9, 20, other), (1237, 2, 13, rad),
-             (n, 3, 2, rad), (n, 4, 32, rad)]
+    # (n, frame, iterations, radii, budget): budgets 16, 10, 7, 1 and 0 (a
+    # group's lanes past the budget add +0)
+    cases = [(n, 5, 20, rad, 10), (n, 5, 20, rad, 10), (n, 9, 20, other, 10),
+             (1237, 2, 13, rad, 10), (n, 3, 2, rad, 10), (n, 4, 32, rad, 10),
+             (n, 6, 6, other, 16), (1237, 8, 5, rad, 7), (n, 7, 3, rad, 1), (n, 2, 3, rad, 0)]
     try:
         for flow in (1, 0):
             gpu.call("fos_set_param", b"pressure_flow", flow)
-            for m, frame, iters, r0 in cases:
+            for m, frame, iters, r0, budget in cases:
                 # rows of the first m particles that point past m are skipped
                 # the same way by the oracle and the kernels (j >= n)
                 r = np.clip(r0[:m], fc.f32(0.02), fc.f32(0.04)).astype(F)
                 want = r.copy()
                 for _ in range(iters):
                     want, dr, mx, ch = orc.pressure(m, frame, want, ids[:m], fs[:m],
-                                                    fc.f32(0.1), fc.f32(3.0), fc.f32(0.02), 10,
+                                                    fc.f32(0.1), fc.f32(3.0), fc.f32(0.02), budget,
                                                     fc.f32(0.02), fc.f32(0.04))
                 tr = dev(r)
                 tdr = torch.empty(m, dtype=torch.float32, device="cuda")
                 st = torch.zeros(2, dtype=torch.float32, device="cuda")
-                print(f"[pressure_flow] flow={flow} n={m} frame={frame} iters={iters}",
-                      flush=True)
+                print(f"[pressure_flow] flow={flow} n={m} frame={frame} iters={iters} "
+                      f"budget={budget}", flush=True)
                 gpu.call("fos_pressure_run", ctx.handle, m, frame, iters, gpu.ptr(tr),
                          gpu.ptr(tdr), gpu.ptr(dev(ids[:m])), gpu.ptr(dev(fs[:m])), 0.1, 3.0,
-                         0.02, 10, 0.02, 0.04, gpu.ptr(st), gpu.stream())
-                assert np.array_equal(host(tr), want), (flow, m, frame, iters)
-                assert np.array_equal(host(tdr), dr), (flow, m, frame, iters)
+                         0.02, budget, 0.02, 0.04, gpu.ptr(st), gpu.stream())
+                assert np.array_equal(host(tr), want), (flow, m, frame, iters, budget)
+                assert np.array_equal(host(tdr), dr), (flow, m, frame, iters, budget)
                 s = host(st)
                 assert s.view(np.int32)[1] == ch and s[0] == pytest.approx(mx, rel=1e-6)
     finally:
