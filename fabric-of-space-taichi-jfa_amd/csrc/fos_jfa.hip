@@ -791,14 +791,27 @@ __device__ __forceinline__ void face_emit_tile(int tile, int res, int n,
     const int zb = tile % nzb, t2 = tile / nzb;
     const int yb = t2 % nyb, x0 = (t2 / nyb) * TX;
     const int z = zb * ZW + (t % ZW), y = yb * (256 / ZW) + t / ZW;
+    const bool in_yz = y < res && z < res;
+    // the next plane's seven labels are loaded before this plane's pairs go
+    // into the hash, so their latency overlaps the LDS work
+    int a_nx = -1, nb_nx[6];
+    if (in_yz && x0 < res) {
+      a_nx = f[(x0 * res + y) * res + z];
+      face_neighbours(f, res, x0, y, z, nb_nx);
+    }
     for (int xt = 0; xt < TX; ++xt) {
       const int x = x0 + xt;
-      if (x >= res || y >= res || z >= res) break;
+      if (x >= res || !in_yz) break;
       const int v = (x * res + y) * res + z;
-      const int a = f[v];
-      if (a < 0 || a >= n) continue;
+      const int a = a_nx;
       int nb[6];
-      face_neighbours(f, res, x, y, z, nb);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) nb[k] = nb_nx[k];
+      if (xt + 1 < TX && x + 1 < res) {
+        a_nx = f[v + res * res];
+        face_neighbours(f, res, x + 1, y, z, nb_nx);
+      }
+      if (a < 0 || a >= n) continue;
       const unsigned kbase = 6u * (unsigned)v;
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
