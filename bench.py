@@ -185,6 +185,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cadence", action="store_true")
     ap.add_argument("--no-temporal", action="store_true")
+    ap.add_argument("--no-early-exit", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=48)
     ap.add_argument("--no-ensemble", action="store_true")
@@ -282,6 +283,38 @@ def temporal_bench(cfg, pos, rad, steps, warmup):
             "frames": [warmup, warmup + steps - 1],
             "jfa_frames": sum(1 for r in reasons if r != "skip"),
             "reasons": dict(collections.Counter(reasons))}
+
+
+def early_exit_bench(cfg, pos, rad, steps, warmup, sample=20):
+    """C2 with the reference's default early exit inside the pass schedule
+    (config.py:208-210: stop once fewer than 0.6% of the voxels changed in a
+    pass, after at least 3 passes; the device skips the remaining passes):
+    frames/s over the timed window, then the passes the JFA ran on `sample`
+    further frames, read back one frame at a time outside the timed window."""
+    import collections
+    import dataclasses
+
+    import torch
+
+    from foam import FoamSim
+    c = dataclasses.replace(cfg, early_exit=True)
+    sim = FoamSim(c, pos, rad)
+    sim.run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    passes = []
+    for _ in range(sample):
+        sim.step()
+        passes.append(int(sim.info[0].item()))
+    return {"workload": f"same, early exit on (threshold {c.th_pass_delta}, at least "
+                        f"{c.min_passes} passes; the reference default, config.py:208-210)",
+            "value": round(steps / el, 3), "unit": "frames/s",
+            "frames": [warmup, warmup + steps - 1],
+            "passes_run": {str(k): v for k, v in sorted(collections.Counter(passes).items())},
+            "passes_sample": f"frames {warmup + steps}..{warmup + steps + sample - 1}"}
 
 
 C3_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_jfa_pass_c3_hbm_traffic.md")
@@ -618,6 +651,8 @@ def main():
         out["cadence"] = {"workload": "same, JFA every 5th frame after a 20-frame warm start",
                           "value": round(args.steps / el2, 3), "unit": "frames/s",
                           "frames": [w2, w2 + args.steps - 1], "jfa_frames": jfa_frames}
+    if not args.no_early_exit:
+        out["early_exit"] = early_exit_bench(cfg, pos, rad, args.steps, args.warmup)
     if not args.no_temporal:
         c2t = fc.preset("c2t")
         out["temporal"] = temporal_bench(c2t, pos, rad, args.steps,

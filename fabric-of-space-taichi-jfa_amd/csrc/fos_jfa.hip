@@ -88,15 +88,16 @@ __global__ void k_seed_fused(const float *__restrict__ pos, const float *__restr
 
 // Whether pass p runs under the device-side early exit (jfa.py:1104-1112):
 // it runs unless an earlier pass q >= min_passes-1 changed fewer than
-// th * V voxels.  Every thread derives the same answer from the history.
-// Must be reached by whole waves (the striped totals use cross-lane sums).
+// th * V voxels.  A pass that did not run counted no change (its counter is
+// cleared before the schedule), and no change is below any positive th, so
+// once one pass stops the schedule every later pass sees its predecessor
+// stopped: the previous pass's count alone decides (one load, not p).
+// Every thread derives the same answer.  Must be reached by whole waves (the
+// striped totals use cross-lane sums).
 __device__ __forceinline__ bool pass_active(const int *stripes, int p, int early,
                                             double th, int min_passes, double V) {
-  if (!early) return true;
-  for (int q = 0; q < p; ++q) {
-    if (q >= min_passes - 1 && (double)stripes_total(stripes, q) / V < th) return false;
-  }
-  return true;
+  if (!early || p < 1 || p - 1 < min_passes - 1) return true;
+  return !((double)stripes_total(stripes, p - 1) / V < th);
 }
 
 // pass_active evaluated by wave 0 and shared with the block (all threads of

@@ -46,11 +46,17 @@ def small_cfg(**kw):
     return cfg
 
 
-@pytest.mark.parametrize("variant", ["plain", "noise", "cadence", "global_back"])
+@pytest.mark.parametrize("variant", ["plain", "noise", "cadence", "global_back", "early_exit"])
 def test_frames_match_oracle(gpu, variant):
+    """Whole frames against the oracle's.  'early_exit' runs the reference's
+    default early exit (config.py:208-210) with a threshold of 3% so that it
+    fires on this small foam: the passes each frame ran and their change
+    history must match too."""
     from foam import FoamSim
     from oracle.frame import OracleFoam
     kw = {}
+    if variant == "early_exit":
+        kw = dict(early_exit=True, th_pass_delta=0.03, min_passes=3)
     if variant == "noise":
         kw = dict(brownian=True, brownian_mode=1, brownian_strength=1e-3, ou_jitter=True)
     if variant == "cadence":
@@ -62,6 +68,7 @@ def test_frames_match_oracle(gpu, variant):
     cfg.r_min, cfg.r_max = 0.02, 0.07
     sim = FoamSim(cfg, pos, rad)
     ref = OracleFoam(cfg, pos, rad)
+    ran = []
     for f in range(cfg.frames):
         print(f"[frame parity] variant={variant} frame={f}", flush=True)
         a = sim.step()
@@ -72,6 +79,14 @@ def test_frames_match_oracle(gpu, variant):
         compare(sa, sb, f)
         assert sa["settle"] == sb["settle"], f
         print(f"[frame parity] variant={variant} frame={f} settle={int(sa['settle'])}", flush=True)
+        if a["jfa"]:
+            info = sim.info.cpu().numpy()
+            passes, early, hist = ref.jfa_info
+            assert int(info[0]) == passes and bool(info[1]) == bool(early), f
+            assert [int(x) for x in info[2:2 + passes]] == [int(x) for x in hist[:passes]], f
+            ran.append(passes)
+    if variant == "early_exit":
+        assert min(ran) < cfg.passes(), ran
 
 
 def test_c1_first_frames_bit_exact(gpu):
