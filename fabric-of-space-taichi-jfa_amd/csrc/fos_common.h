@@ -177,8 +177,6 @@ struct fos_ctx {
   int stripes_clear;  // 1: fos_jfa_init has queued the clear of pass slots 0..FOS_PASS_SLOTS-2
   // seed table (x, y, z, (beta*r)^2)
   float4 *seeds;
-  // per-wave change masks of the last two JFA passes, [max_res^2][ceil(max_res/64)]
-  unsigned long long *chg[2];
   // each voxel's winning power distance in the last two JFA passes [max_res^3]
   // (NULL: the passes evaluate every incumbent)
   float *dist[2];

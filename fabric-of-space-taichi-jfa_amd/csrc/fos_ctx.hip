@@ -202,7 +202,7 @@ int fos_ctx_destroy(fos_ctx *ctx) {
                   ctx->in_sorted,
                   ctx->acc_in,   ctx->scan_blocks, ctx->cell_write, ctx->cell_of,
                   ctx->pos_tmp,  ctx->rad_tmp,  ctx->fscratch, ctx->iscratch,
-                  ctx->chg[0],   ctx->chg[1],   ctx->grid_tmp, ctx->dist[0], ctx->dist[1],
+                  ctx->grid_tmp, ctx->dist[0], ctx->dist[1],
                   ctx->flow,     ctx->sched};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -268,9 +268,6 @@ int fos_ctx_create(fos_ctx **out, int max_n, int max_res, int max_cells) {
   } while (0)
   ALLOC(c->seeds, (size_t)max_n * sizeof(float4));
   {
-    const size_t words = (size_t)max_res * max_res * ((max_res + 63) / 64);
-    ALLOC(c->chg[0], words * sizeof(unsigned long long));
-    ALLOC(c->chg[1], words * sizeof(unsigned long long));
     // per-voxel power distances carried between JFA passes, for grids up to
     // 256^3 (there the carry was measured faster; beyond, the extra label-
     // side traffic costs more than the evaluation it saves)

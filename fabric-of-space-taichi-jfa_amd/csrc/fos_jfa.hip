@@ -168,33 +168,7 @@ __device__ __forceinline__ bool jfa_skip(const int lab[27], int q, int inc) {
 // Pass flavours of the list kernel.
 #define JFA_PLAIN 0    // mirror skip rule
 #define JFA_SETTLED 1  // also skip repeats of the -1 neighbours
-#define JFA_INCR 2     // settled, and recompute only next to last pass's changes
 #define JFA_HALF 3     // 2*step == res: the -step and +step candidates coincide
-
-// Per-wave change masks: bit l of word (x*res + y)*nzb + zb is set when voxel
-// (x, y, 64*zb + l) changed in the pass.  In a pass whose step equals the
-// previous pass's, a voxel whose 3x3x3 neighbourhood did not change in the
-// previous pass sees the same 27 labels that pass saw, so its result is its
-// current label; only voxels next to a change are recomputed.
-__device__ __forceinline__ unsigned long long incr_wave_mask(
-    const unsigned long long *__restrict__ chg, int res, int nzb, int x, int y, int zb) {
-  const int zl = zb > 0 ? zb - 1 : nzb - 1;
-  const int posl = zb > 0 ? 63 : (res - 1) & 63;
-  const int last = min(63, res - 1 - 64 * zb);
-  const int zr = (64 * zb + last + 1 < res) ? zb + 1 : 0;
-  unsigned long long w = 0;
-#pragma unroll
-  for (int a = -1; a <= 1; ++a)
-#pragma unroll
-    for (int b = -1; b <= 1; ++b) {
-      const size_t base = ((size_t)wrap1(x + a, res) * res + wrap1(y + b, res)) * nzb;
-      const unsigned long long m = chg[base + zb];
-      const unsigned long long l = (chg[base + zl] >> posl) & 1ull;
-      const unsigned long long r = chg[base + zr] & 1ull;
-      w |= m | (m << 1) | (m >> 1) | l | (r << last);
-    }
-  return w;
-}
 
 // The owner of one voxel from its 27 candidate labels (lab[13] = current
 // owner): the kept candidates go to this thread's LDS list column, then the
@@ -283,33 +257,21 @@ template <bool UNIT_L, int MODE, typename LT = int, int BATCH = JFA_BATCH, int D
 __device__ __forceinline__ unsigned long long jfa_list_tile(
     const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel, int step,
     const int32_t *__restrict__ in, int32_t *__restrict__ out, int tile, int nyb, int nzb,
-    int ntiles, const unsigned long long *__restrict__ chg_prev,
-    unsigned long long *__restrict__ chg_out, LT (*s_list)[256], int base = 0,
+    int ntiles, LT (*s_list)[256], int base = 0,
     const float *__restrict__ dist_in = nullptr, float *__restrict__ dist_out = nullptr,
     int qr = 0, unsigned mq = 0, unsigned mq1 = 0) {
   const int t = (int)threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   int changed = 0;
-  bool row_live = false;
-  int x = 0, y = 0, zb = 0;
   if (tile < ntiles) {
-    zb = tile % nzb;
+    const int zb = tile % nzb;
     const int t2 = tile / nzb;
     const int yb = t2 % nyb;
-    x = CLASS ? jfa_plane(t2 / nyb, step, qr, mq, mq1) : t2 / nyb;
-    y = yb * 4 + w;
-    row_live = y < res;
+    const int x = CLASS ? jfa_plane(t2 / nyb, step, qr, mq, mq1) : t2 / nyb;
+    const int y = yb * 4 + w;
     const int z = zb * 64 + lane;
-    bool work = row_live && z < res;
+    const bool work = y < res && z < res;
     const size_t v = ((size_t)x * res + y) * res + z;
-    if (MODE == JFA_INCR && row_live) {
-      const unsigned long long m = incr_wave_mask(chg_prev, res, nzb, x, y, zb);
-      if (work && !((m >> lane) & 1ull)) {
-        out[v] = in[v];
-        if (DM == 2) dist_out[v] = dist_in[v];
-        work = false;
-      }
-    }
     if (work) {
       const float cx = vcenter(x, voxel, half_l), cy = vcenter(y, voxel, half_l),
                   cz = vcenter(z, voxel, half_l);
@@ -348,9 +310,7 @@ __device__ __forceinline__ unsigned long long jfa_list_tile(
       changed = (best != lab[13]);
     }
   }
-  const unsigned long long bal = __ballot(changed);
-  if (chg_out && row_live && lane == 0) chg_out[((size_t)x * res + y) * nzb + zb] = bal;
-  return bal;
+  return __ballot(changed);
 }
 
 // One flat tile of a list pass: 256 consecutive voxels in (y, z) order of
@@ -412,15 +372,14 @@ __global__ void __launch_bounds__(256)
                     float voxel, int step, const int32_t *__restrict__ in,
                     int32_t *__restrict__ out, int *__restrict__ stripes, int pass, int early,
                     double th, int min_passes, int nyb, int nzb, int ntiles, int per_xcd,
-                    const unsigned long long *__restrict__ chg_prev,
-                    unsigned long long *__restrict__ chg_out, const float *__restrict__ dist_in,
-                    float *__restrict__ dist_out, int qr, unsigned mq, unsigned mq1) {
+                    const float *__restrict__ dist_in, float *__restrict__ dist_out, int qr,
+                    unsigned mq, unsigned mq1) {
   __shared__ LT s_list[JFA_LIST_ROWS(BATCH)][256];
   if (!block_pass_active(stripes, pass, early, th, min_passes, (double)res * res * res)) return;
   const int tile = xcd_tile(blockIdx.x, per_xcd);
   const unsigned long long bal = jfa_list_tile<UNIT_L, MODE, LT, BATCH, DM, CLASS>(
-      seeds, res, L, half_l, voxel, step, in, out, tile, nyb, nzb, ntiles, chg_prev, chg_out,
-      s_list, 0, dist_in, dist_out, qr, mq, mq1);
+      seeds, res, L, half_l, voxel, step, in, out, tile, nyb, nzb, ntiles, s_list, 0, dist_in,
+      dist_out, qr, mq, mq1);
   block_count(bal, stripes, pass);
 }
 
@@ -1511,9 +1470,8 @@ static bool jfa_class_order(const fos_ctx *ctx, int res, int step) {
 template <bool U, int MODE, typename LT, int BATCH, int DM>
 static void launch_pass_list_v(fos_ctx *ctx, int res, float L, float half_l, float voxel,
                                int step, const int32_t *in, int32_t *out, int pass, int early,
-                               double th, int minp, const unsigned long long *chg_prev,
-                               unsigned long long *chg_out, const float *dist_in,
-                               float *dist_out, hipStream_t st) {
+                               double th, int minp, const float *dist_in, float *dist_out,
+                               hipStream_t st) {
   const int nyb = (res + 3) / 4, nzb = (res + 63) / 64;
   const int ntiles = res * nyb * nzb;
   const int per = (ntiles + FOS_XCDS - 1) / FOS_XCDS;
@@ -1524,27 +1482,25 @@ static void launch_pass_list_v(fos_ctx *ctx, int res, float L, float half_l, flo
   if (jfa_class_order(ctx, res, step))
     hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, LT, BATCH, DM, true>), dim3(per * FOS_XCDS),
                        dim3(256), 0, st, ctx->seeds, res, L, half_l, voxel, step, in, out,
-                       ctx->stripes, pass, early, th, minp, nyb, nzb, ntiles, per, chg_prev,
-                       chg_out, dist_in, dist_out, qr, mq, mq1);
+                       ctx->stripes, pass, early, th, minp, nyb, nzb, ntiles, per, dist_in,
+                       dist_out, qr, mq, mq1);
   else
     hipLaunchKernelGGL((k_jfa_pass_list<U, MODE, LT, BATCH, DM>), dim3(per * FOS_XCDS),
                        dim3(256), 0, st, ctx->seeds, res, L, half_l, voxel, step, in, out,
-                       ctx->stripes, pass, early, th, minp, nyb, nzb, ntiles, per, chg_prev,
-                       chg_out, dist_in, dist_out, qr, mq, mq1);
+                       ctx->stripes, pass, early, th, minp, nyb, nzb, ntiles, per, dist_in,
+                       dist_out, qr, mq, mq1);
 }
 
 // dm: 0 no distance buffers, 1 write dist_out, 2 read dist_in and write dist_out
 template <bool U, int MODE>
 static void launch_pass_list(fos_ctx *ctx, int res, float L, float half_l, float voxel,
                              int step, const int32_t *in, int32_t *out, int pass, int early,
-                             double th, int minp, const unsigned long long *chg_prev,
-                             unsigned long long *chg_out, int dm, const float *dist_in,
-                             float *dist_out, hipStream_t st) {
+                             double th, int minp, int dm, const float *dist_in, float *dist_out,
+                             hipStream_t st) {
   const bool b4 = (jfa_variant(ctx) & 1) != 0;
 #define FOS_LV(LT, B, D)                                                                     \
   launch_pass_list_v<U, MODE, LT, B, D>(ctx, res, L, half_l, voxel, step, in, out, pass,     \
-                                        early, th, minp, chg_prev, chg_out, dist_in, dist_out, \
-                                        st)
+                                        early, th, minp, dist_in, dist_out, st)
 #define FOS_LD(LT, B)                                                                        \
   do {                                                                                       \
     if (dm == 2) FOS_LV(LT, B, 2);                                                           \
@@ -1565,21 +1521,17 @@ static void launch_pass_list(fos_ctx *ctx, int res, float L, float half_l, float
 // Launches pass `pass` (its changed-voxel count accumulates in stripe slot
 // `pass`; the caller zeroes the slot first).  `settled` marks a pass whose
 // step equals the previous pass's (the field is close to converged): the
-// list kernel then also skips owners repeated by -1 neighbours, and with the
-// previous pass's change masks (chg_prev) recomputes only voxels next to a
-// change.  chg_out (may be NULL) receives this pass's change masks.  dm and
-// the distance buffers as for launch_pass_list.
+// list kernel then also skips owners repeated by -1 neighbours.  dm and the
+// distance buffers as for launch_pass_list.
 static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
                        const int32_t *in, int32_t *out, int pass, int early, double th,
-                       int min_passes, bool settled, const unsigned long long *chg_prev,
-                       unsigned long long *chg_out, hipStream_t st, int dm = 0,
+                       int min_passes, bool settled, hipStream_t st, int dm = 0,
                        const float *dist_in = nullptr, float *dist_out = nullptr) {
   const float half_l = (float)(0.5 * (double)L);
   const bool u = unit_l(L);
   const bool orbits = g_jfa_half_orbits < 0 ? res >= 256 : g_jfa_half_orbits != 0;
-  if (2 * step == res && orbits && dm != 2 && !chg_out) {
-    // the step = res / 2 pass by orbits (k_jfa_half; it writes no change
-    // masks, which only a third pass in a row at one step reads)
+  if (2 * step == res && orbits && dm != 2) {
+    // the step = res / 2 pass by orbits (k_jfa_half)
     const int s = res / 2;
     const unsigned b = blocks_for((int64_t)s * s * s);
 #define FOS_HALF(U, D)                                                                           \
@@ -1594,20 +1546,16 @@ static int launch_step(fos_ctx *ctx, int res, float L, float voxel, int step,
     FOS_CHECK_LAUNCH("k_jfa_half");
     return 0;
   }
-  const int mode = 2 * step == res ? JFA_HALF
-                   : settled       ? (chg_prev ? JFA_INCR : JFA_SETTLED)
-                                   : JFA_PLAIN;
+  const int mode = 2 * step == res ? JFA_HALF : settled ? JFA_SETTLED : JFA_PLAIN;
 #define FOS_LIST(U, M)                                                                   \
   launch_pass_list<U, M>(ctx, res, L, half_l, voxel, step, in, out, pass, early, th,    \
-                         min_passes, chg_prev, chg_out, dm, dist_in, dist_out, st)
+                         min_passes, dm, dist_in, dist_out, st)
   if (u) {
     if (mode == JFA_HALF) FOS_LIST(true, JFA_HALF);
-    else if (mode == JFA_INCR) FOS_LIST(true, JFA_INCR);
     else if (mode == JFA_SETTLED) FOS_LIST(true, JFA_SETTLED);
     else FOS_LIST(true, JFA_PLAIN);
   } else {
     if (mode == JFA_HALF) FOS_LIST(false, JFA_HALF);
-    else if (mode == JFA_INCR) FOS_LIST(false, JFA_INCR);
     else if (mode == JFA_SETTLED) FOS_LIST(false, JFA_SETTLED);
     else FOS_LIST(false, JFA_PLAIN);
   }
@@ -1750,8 +1698,7 @@ int fos_jfa_step(fos_ctx *ctx, int res, float L, float voxel, int step_size,
   int *sl = ctx->stripes + slot * FOS_STRIPE_PASS_INTS;
   int rc = fos_fill_i32(sl, FOS_STRIPE_PASS_INTS, 0, stream);
   if (rc) return rc;
-  rc = launch_step(ctx, res, L, voxel, step_size, in, out, slot, 0, 0.0, 0, step_size == 1,
-                   nullptr, nullptr, st);
+  rc = launch_step(ctx, res, L, voxel, step_size, in, out, slot, 0, 0.0, 0, step_size == 1, st);
   if (rc || !changes) return rc;
   hipLaunchKernelGGL(k_stripes_add, dim3(1), dim3(64), 0, st, (const int *)ctx->stripes, slot,
                      changes);
@@ -1787,32 +1734,21 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
     if (rc) return rc;
   }
   ctx->stripes_clear = 0;
-  // A pass is settled when its step equals the previous pass's; it recomputes
-  // only next to the previous pass's changes when that pass was settled too
-  // (after the first repeat the changes are few; the first step-1 pass after
-  // step 2 still changes most voxels).
-  int step = res / 2, prev = 0, prev2 = 0;
+  // A pass is settled when its step equals the previous pass's.
+  int step = res / 2, prev = 0;
   for (int p = 0; p < num_passes; ++p) {
     if (step < 1) step = 1;
     const int32_t *in = (p & 1) ? temp : grid;
     int32_t *out = (p & 1) ? grid : temp;
     if (events && events[p]) FOS_HIP(hipEventRecord((hipEvent_t)events[p], st));
-    // masks are read by the next pass only when it repeats this step a
-    // third time in a row (step == prev there)
-    const int next = step / 2 < 1 ? 1 : step / 2;
-    unsigned long long *chg_out =
-        (p + 1 < num_passes && next == step && step == prev) ? ctx->chg[p & 1] : nullptr;
-    const unsigned long long *chg_prev =
-        (step == prev && prev == prev2) ? ctx->chg[(p + 1) & 1] : nullptr;
     // winning power distances carried from pass to pass (the first pass
     // evaluates the seeded incumbents and writes them)
     const bool carry = ctx->dist[0] && (jfa_variant(ctx) & 2);
     rc = launch_step(ctx, res, L, voxel, step, in, out, p, early_exit, th_pass_delta,
-                     min_passes, step == prev, chg_prev, chg_out, st, carry ? (p ? 2 : 1) : 0,
+                     min_passes, step == prev, st, carry ? (p ? 2 : 1) : 0,
                      carry ? ctx->dist[(p + 1) & 1] : nullptr,
                      carry ? ctx->dist[p & 1] : nullptr);
     if (rc) return rc;
-    prev2 = prev;
     prev = step;
     step /= 2;
   }
@@ -2483,16 +2419,13 @@ __global__ void __launch_bounds__(256)
                                                  out + v0, tile, tpp, s_list, base);
   } else if (2 * step == res) {
     bal = jfa_list_tile<UNIT_L, JFA_HALF, LT>(seeds, res, L, half_l, voxel, step, in + v0,
-                                              out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
-                                              s_list, base);
+                                              out + v0, tile, nyb, nzb, ntiles, s_list, base);
   } else if (step == prev) {
     bal = jfa_list_tile<UNIT_L, JFA_SETTLED, LT>(seeds, res, L, half_l, voxel, step, in + v0,
-                                                 out + v0, tile, nyb, nzb, ntiles, nullptr,
-                                                 nullptr, s_list, base);
+                                                 out + v0, tile, nyb, nzb, ntiles, s_list, base);
   } else {
     bal = jfa_list_tile<UNIT_L, JFA_PLAIN, LT>(seeds, res, L, half_l, voxel, step, in + v0,
-                                               out + v0, tile, nyb, nzb, ntiles, nullptr, nullptr,
-                                               s_list, base);
+                                               out + v0, tile, nyb, nzb, ntiles, s_list, base);
   }
   block_count(bal, st, pass);
 }

@@ -6,7 +6,6 @@
 // The array `scratch` holds, at fixed offsets:
 //   seed table   float4 [FOS_STATIC_MAX_N]                       4 MiB
 //   counters     int [FOS_PASS_SLOTS][FOS_STRIPES][16]           256 KiB
-//   change masks u64 [2][max_res^2][ceil(max_res/64)]            13.5 MiB
 // Every call on one device shares that device's copy (the last call's
 // contents stay there); calls that overlap on one device race.
 #include <string.h>
@@ -19,9 +18,7 @@ namespace {
 
 constexpr int64_t kSeedBytes = (int64_t)FOS_STATIC_MAX_N * 16;
 constexpr int64_t kStripeBytes = (int64_t)FOS_PASS_SLOTS * FOS_STRIPE_PASS_INTS * 4;
-constexpr int64_t kMaskWords =
-    (int64_t)FOS_STATIC_MAX_RES * FOS_STATIC_MAX_RES * ((FOS_STATIC_MAX_RES + 63) / 64);
-constexpr int64_t kScratchBytes = kSeedBytes + kStripeBytes + 2 * kMaskWords * 8;
+constexpr int64_t kScratchBytes = kSeedBytes + kStripeBytes;
 
 __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -82,8 +79,6 @@ int fos_jfa_labels_noalloc(const float *pos, const float *rad, int n, int res, f
   c.max_cells = 27;
   c.seeds = (float4 *)b;
   c.stripes = (int *)(b + kSeedBytes);
-  c.chg[0] = (unsigned long long *)(b + kSeedBytes + kStripeBytes);
-  c.chg[1] = c.chg[0] + kMaskWords;
   rc = fos_jfa_init(&c, pos, rad, n, res, L, voxel, beta, grid, nullptr, stream);
   if (rc) return rc;
   return fos_jfa_passes(&c, res, L, voxel, num_passes, early_exit, th_pass_delta, min_passes,

@@ -149,9 +149,9 @@ int fos_jfa_passes_timed(fos_ctx *ctx, int res, float L, float voxel,
  * hipMalloc).  Computes what fos_jfa_init (grid only, temp untouched before
  * the passes) followed by fos_jfa_passes computes -- the seeding and the
  * whole pass schedule, the final labels in grid and info as fos_jfa_passes
- * writes it -- with no context: the seed table, pass counters and change
- * masks live in one static device array (about 17.8 MB per device, held
- * until the process ends).  Calls no hipMalloc/hipFree.  Rejects
+ * writes it -- with no context: the seed table and the pass counters live
+ * in one static device array (about 4.5 MB per device, held until the
+ * process ends).  Calls no hipMalloc/hipFree.  Rejects
  * n > FOS_STATIC_MAX_N or res > FOS_STATIC_MAX_RES with FOS_EINVAL.  Every
  * call on one device shares that device's array: calls that overlap on one
  * device race. */

@@ -157,7 +157,7 @@ PASS_VARIANTS = {"batch2": 0, "batch4": 1, "batch2_carry": 2, "batch4_carry": 3,
 def test_jfa_pass_kernel_variants(gpu, orc, variant):
     """Every pass-kernel variant gives the oracle's labels and change counts,
     including the collision-heavy dense case, a res that is not a multiple of
-    64, more step-1 passes than the schedule (the incremental passes), early
+    64, more step-1 passes than the schedule (settled passes in a row), early
     exit, and ids above 65,535 (32-bit candidate lists)."""
     import ctypes
     v0 = ctypes.c_int()
