@@ -722,8 +722,12 @@ __global__ void __launch_bounds__(256)
 // distinct pair once into the row tables.  Squarer tiles (smaller ZW) cross
 // fewer cell boundaries per voxel, so fewer pairs reach the row tables.  A pair that finds no
 // LDS slot within EMIT_PROBES probes goes straight to the row table, so the
-// LDS table can be smaller than the worst case.
-#define EMIT_HS 2048
+// LDS table can be smaller than the worst case.  512 slots (8 KB of LDS with
+// the keys and counts): a tile of 8 x 8 x 32 voxels holds a few dozen distinct
+// pairs, and the table's clear and flush loops and its LDS footprint shrink
+// with it (2048 slots measured 12% slower at C2, 55% at C5; 256 overflowed
+// into the row tables on the deeper 16-plane tiles).
+#define EMIT_HS 512
 #define EMIT_PROBES 32
 #define EMIT_EMPTY 0xffffffffffffffffull
 
