@@ -40,7 +40,7 @@ __device__ __forceinline__ float dot3f(float x, float y, float z) {
 // hash cell of at least mult * 2 * r_max.
 // ---------------------------------------------------------------------------
 #define PRED_CAP_MAX 128   // row capacity limit (LDS staging of the j < i block)
-#define PRED_NB_MAX 2048   // particles of a 27-cell neighbourhood staged in LDS
+#define PRED_NB_MAX 1024   // particles of a 27-cell neighbourhood staged in LDS
 
 // Row state of one particle while a wave scans its neighbourhood 64 hits at
 // a time: partners j > i go straight to the front of the row, partners
@@ -115,7 +115,7 @@ __device__ __forceinline__ bool cand_test(float qx, float qy, float qz, float rk
 // then each wave scans it for the cell's particles in turn.  A
 // neighbourhood of more than PRED_NB_MAX particles is left to
 // k_pred_cand_wave (its particles get ncand = -1 here).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
     k_pred_cand_cell(const float *__restrict__ pos, const float *__restrict__ rad, int G,
                      float L, float inv_l, float mult, int cap,
                      const int32_t *__restrict__ cs, const int32_t *__restrict__ cc,

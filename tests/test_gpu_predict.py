@@ -110,6 +110,34 @@ def test_candidates_truncated_rows(gpu):
     assert trunc.sum() > 0
 
 
+def test_candidates_dense_neighbourhood(gpu):
+    """A cluster of 1,300 particles inside one hash cell: their 27-cell
+    neighbourhoods exceed what the cell kernel stages in LDS (1,024), so
+    those particles go to the one-wave-per-particle kernel over global
+    memory; every row (truncated at the cap in scan order) equals the brute
+    force, on both sides of the split."""
+    rng = np.random.default_rng(17)
+    n, m = 2000, 1300
+    pos = np.empty((n, 3), F)
+    pos[:m] = (rng.random((m, 3)) * 0.05 - 0.3).astype(F)       # one 0.05 cube
+    pos[m:] = (rng.random((n - m, 3)) - 0.5).astype(F)
+    rad = rng.uniform(0.01, 0.04, n).astype(F)
+    G = fp.hash_res(1.0, 0.04)
+    assert 0.05 < 1.0 / G                                        # the cube fits a cell
+    pr = fp.FacePredictor(fp.init_params(), n, 1.0, 0.04)
+    tp, tr = dev(pos), dev(rad)
+    pr.candidates(tp, tr, n)
+    cand, nc, nu, trunc = host(pr.cand), host(pr.ncand), host(pr.nup), host(pr.trunc)
+    want = brute_candidates(pos, rad, G)
+    full = brute_candidates(pos, rad, G, cap=10 ** 6)
+    for i in range(n):
+        row, up = want[i]
+        assert nc[i] == len(row) and nu[i] == up, i
+        assert (cand[i, :nc[i]] == row).all(), i
+        assert trunc[i] == (len(full[i][0]) > fp.CAP), i
+    assert trunc[:m].all() and not trunc[m:].all()
+
+
 def face_area(d, ri, rj, ks, rks, kids, clip_r, near=8):
     """Area of the power-plane face of i (origin) and j (at d): a square of
     half-size clip_r*(ri+rj) around the plane point m, clipped by
