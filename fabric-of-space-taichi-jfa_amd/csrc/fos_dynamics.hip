@@ -1903,6 +1903,25 @@ int pbd_lanes_set(int lanes) {
   return 0;
 }
 int pbd_lanes_get() { return g_pbd_lanes; }
+
+// One PBD pass over the slot records of the rebuild before it, without the
+// next rebuild's cell step (the pass before a step that moves the particles
+// again, such as the OU jitter): what fos_pbd_pass computes, bit for bit.
+int pbd_pass_records(const float *pos_in, float *pos_out, const float *rad, int n, int G,
+                     float L, float gap_factor, float max_disp_frac, float disp_mult,
+                     const int32_t *cell_start, const int32_t *cell_count,
+                     const int32_t *cell_indices, const float *cell_pr, hipStream_t st) {
+  FOS_REQUIRE(n >= 0 && G >= 3 && L > 0.0f &&
+                  (n == 0 || (pos_in && pos_out && pos_in != pos_out && rad && cell_start &&
+                              cell_count && cell_indices && cell_pr)),
+              "pbd_pass_records: bad arguments (n=%d G=%d)", n, G);
+  if (n == 0) return 0;
+  launch_pbd_pr<false>(st, n, pos_in, pos_out, cell_pr, rad, G, L, gap_factor, max_disp_frac,
+                       disp_mult, cell_start, cell_count, cell_indices, nullptr, nullptr,
+                       nullptr, 0, nullptr);
+  FOS_CHECK_LAUNCH("k_pbd_pr");
+  return 0;
+}
 }  // namespace fos
 
 __global__ void __launch_bounds__(256)

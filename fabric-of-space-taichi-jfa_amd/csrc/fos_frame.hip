@@ -259,6 +259,10 @@ static int frame_step(fos_ctx *ctx, const fos_frame_desc *d, const fos_frame_buf
       FOS_CALL(fos_pbd_pass_cells(ctx, pos, pos_tmp, b->rad, n, G, L, d->gap_factor,
                                   d->max_disp_frac, d->disp_mult, b->cell_start, b->cell_count,
                                   b->cell_indices, pr, stream));
+    else if (pr)
+      FOS_CALL(fos::pbd_pass_records(pos, pos_tmp, b->rad, n, G, L, d->gap_factor,
+                                     d->max_disp_frac, d->disp_mult, b->cell_start,
+                                     b->cell_count, b->cell_indices, pr, st));
     else
       FOS_CALL(fos_pbd_pass(pos, pos_tmp, b->rad, n, G, L, d->gap_factor, d->max_disp_frac,
                             d->disp_mult, b->cell_start, b->cell_count, b->cell_indices,
