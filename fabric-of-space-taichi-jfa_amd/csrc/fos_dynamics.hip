@@ -1656,7 +1656,10 @@ __global__ void k_count_zero_b(const int32_t *__restrict__ fsc, const int32_t *_
 // loads are contiguous and independent of the id load.  Lanes first count
 // their cell's corrections, a prefix over the 27 lanes gives each its place
 // in the serial (cell, occupant) order, and the lanes then write the
-// corrections there for lane 0 to add in that order, PBD_HITS at a time.
+// corrections there for lane 0 to add in that order, PBD_HITS at a time
+// (32 in the 8-lane form, whose 32 particles per block would otherwise hold
+// 18 KB of windows: with 16 KB of LDS and at most 64 VGPRs it keeps eight
+// waves per SIMD; the batched C4 frame +0.5%).
 #define PBD_HITS 48
 
 // pbd_hit's contact test alone (dist < ts with dist = sqrtf(d.d + 1e-12)),
@@ -1711,7 +1714,7 @@ __device__ __forceinline__ bool pbd_hit(float qx, float qy, float qz, float rj, 
 // walking its cells again.  U: L == 1.
 #define PBD_HK 4
 template <bool BATCH, int LPP, bool U = false>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
     k_pbd_pr(const float *__restrict__ pin, float *__restrict__ pout,
              const float4 *__restrict__ pr, const float *__restrict__ rad, int n, int G0, float L,
              float half_l, float inv_l, float gap, float mdf, float dmul,
@@ -1720,7 +1723,8 @@ __global__ void __launch_bounds__(256)
              const fos_inst *__restrict__ inst, const int32_t *__restrict__ npass, int pass,
              int per_xcd, int32_t *__restrict__ cell_of) {
   constexpr int NPB = 256 / LPP, CPL = (27 + LPP - 1) / LPP;
-  __shared__ float s_v[NPB][PBD_HITS][3];
+  constexpr int HITS = LPP == 8 ? 32 : PBD_HITS;
+  __shared__ float s_v[NPB][HITS][3];
   __shared__ int s_hk[PBD_HK][256];
   const int t = (int)threadIdx.x;
   const int grp = (int)(threadIdx.x / LPP), l = (int)(threadIdx.x % LPP);
@@ -1793,12 +1797,12 @@ __global__ void __launch_bounds__(256)
   __shared__ int s_rounds;
   if (threadIdx.x == 0) s_rounds = 1;
   __syncthreads();
-  if (live && l == 0 && H > PBD_HITS) atomicMax(&s_rounds, (H + PBD_HITS - 1) / PBD_HITS);
+  if (live && l == 0 && H > HITS) atomicMax(&s_rounds, (H + HITS - 1) / HITS);
   __syncthreads();
   const int rounds = s_rounds;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   for (int r = 0; r < rounds; ++r) {
-    const int lo = r * PBD_HITS, hi = lo + PBD_HITS;
+    const int lo = r * HITS, hi = lo + HITS;
     if (live && h > 0 && x > lo && x - h < hi) {
       int m = x - h;
       if (h <= PBD_HK) {
@@ -1831,7 +1835,7 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
     if (live && l == 0) {
-      const int e = H - lo < PBD_HITS ? H - lo : PBD_HITS;
+      const int e = H - lo < HITS ? H - lo : HITS;
       for (int q = 0; q < e; ++q) {
         ax = ax + s_v[grp][q][0];
         ay = ay + s_v[grp][q][1];
