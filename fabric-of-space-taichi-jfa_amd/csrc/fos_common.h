@@ -281,10 +281,6 @@ int pressure_flow_get();
 // lanes per particle of the slot-record PBD pass (fos_set_param "pbd_lanes")
 int pbd_lanes_set(int lanes);
 int pbd_lanes_get();
-int pbd_pass_records(const float *pos_in, float *pos_out, const float *rad, int n, int G,
-                     float L, float gap_factor, float max_disp_frac, float disp_mult,
-                     const int32_t *cell_start, const int32_t *cell_count,
-                     const int32_t *cell_indices, const float *cell_pr, hipStream_t st);
 }  // namespace fos
 
 // Tile -> block mapping that keeps each XCD on a contiguous range of tiles

@@ -7,6 +7,15 @@
 // flags inside the face step), so a frame gives the same bits either way.
 #include "fos_common.h"
 
+namespace fos {
+// fos_dynamics.hip: a PBD pass over the slot records without the next
+// rebuild's cell step
+int pbd_pass_records(const float *pos_in, float *pos_out, const float *rad, int n, int G,
+                     float L, float gap_factor, float max_disp_frac, float disp_mult,
+                     const int32_t *cell_start, const int32_t *cell_count,
+                     const int32_t *cell_indices, const float *cell_pr, hipStream_t st);
+}  // namespace fos
+
 namespace {
 
 // run.py:1071-1078 cadence: every frame before jfa_warm, then every

@@ -468,35 +468,33 @@ __global__ void k_jfa_finish(int res, int num_passes, int early, double th,
                              int32_t *__restrict__ info, const int32_t *__restrict__ temp,
                              int32_t *__restrict__ grid) {
   const double V = (double)res * res * res;
-  __shared__ int s_exec, s_fired;
-  if (threadIdx.x < 64) {
-    int executed = 0;
-    for (int p = 0; p < num_passes; ++p) {
-      if (!pass_active(stripes, p, early, th, min_passes, V)) break;
-      executed = p + 1;
-    }
-    const int fired = (early && executed > 0 &&
-                       !pass_active(stripes, executed, early, th, min_passes, V)) ? 1 : 0;
-    if (threadIdx.x == 0) {
-      s_exec = executed;
-      s_fired = fired;
-    }
+  __shared__ int s_tot[FOS_PASS_SLOTS];
+  __shared__ int s_exec;
+  // thread p sums pass p's stripes (independent loads, not one dependent
+  // chain per pass); thread 0 then applies pass_active's rule to the totals
+  for (int p = (int)threadIdx.x; p < num_passes; p += (int)blockDim.x) {
+    int c = 0;
+    for (int q = 0; q < FOS_STRIPES; ++q) c += stripes[(p * FOS_STRIPES + q) * FOS_STRIPE_INTS];
+    s_tot[p] = c;
   }
   __syncthreads();
-  const int executed = s_exec, fired = s_fired;
+  if (threadIdx.x == 0) {
+    auto active = [&](int p) {
+      return !early || p < 1 || p - 1 < min_passes - 1 || !((double)s_tot[p - 1] / V < th);
+    };
+    int executed = 0;
+    while (executed < num_passes && active(executed)) ++executed;
+    if (blockIdx.x == 0) {
+      for (int p = 0; p < num_passes; ++p) info[2 + p] = p < executed ? s_tot[p] : 0;
+      info[0] = executed;
+      info[1] = (early && executed > 0 && !active(executed)) ? 1 : 0;
+    }
+    s_exec = executed;
+  }
+  __syncthreads();
+  const int executed = s_exec;
   const int64_t total = (int64_t)res * res * res;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x < 64) {
-    for (int p = 0; p < executed; ++p) {
-      const int c = stripes_total(stripes, p);
-      if (threadIdx.x == 0) info[2 + p] = c;
-    }
-    if (threadIdx.x == 0) {
-      for (int p = executed; p < num_passes; ++p) info[2 + p] = 0;
-      info[0] = executed;
-      info[1] = fired;
-    }
-  }
   // The last executed pass also decides the exit when it is the final one.
   if (executed & 1) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
