@@ -559,20 +559,31 @@ __device__ __forceinline__ void majority_vox(int res, const int32_t *__restrict_
   // Tally in first-seen order (centre, then the neighbours in order, -1
   // skipped); a label's count is its number of occurrences among the seven
   // (the centre counts once).  The first label with the strictly largest
-  // count wins.  Static indices only, so the tally stays in registers.
+  // count wins.  Each of the 21 pairs is compared once: an equal pair adds
+  // to both counts and marks the later entry as a repeat.  Static indices
+  // only, so the tally stays in registers.
+  int c[7];
+  bool rep[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    c[k] = 1;
+    rep[k] = t[k] < 0;
+  }
+#pragma unroll
+  for (int k = 1; k < 7; ++k)
+#pragma unroll
+    for (int j = 0; j < k; ++j) {
+      const bool e = t[j] == t[k];
+      c[j] += e ? 1 : 0;
+      c[k] += e ? 1 : 0;
+      rep[k] |= e;
+    }
   int maxc = 0, best = my;
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
-    const int id = t[k];
-    bool first = id >= 0;
-#pragma unroll
-    for (int j = 0; j < k; ++j) first &= (t[j] != id);
-    int c = 0;
-#pragma unroll
-    for (int j = 0; j < 7; ++j) c += (t[j] == id) ? 1 : 0;
-    if (first && c > maxc) {
-      maxc = c;
-      best = id;
+    if (!rep[k] && c[k] > maxc) {
+      maxc = c[k];
+      best = t[k];
     }
   }
   f[v] = best;
