@@ -250,8 +250,9 @@ __device__ __forceinline__ int jfa_plane(int i, int step, int qr, unsigned mq, u
 // list pass; returns this wave's change ballot (0 for an idle row).  The
 // caller has checked that the pass is active.  CLASS: tiles take their
 // x-planes in jfa_plane order.
-// DM: 0 no distance buffers; 1 writes each voxel's winning power distance to
-// dist_out; 2 also starts from dist_in (the previous pass's dist_out).
+// DM: 0 no distance buffers; 1 writes each owned voxel's winning power
+// distance to dist_out; 2 also starts from dist_in (the previous pass's
+// dist_out, read only where that pass left an owner).
 template <bool UNIT_L, int MODE, typename LT = int, int BATCH = JFA_BATCH, int DM = 0,
           bool CLASS = false>
 __device__ __forceinline__ unsigned long long jfa_list_tile(
@@ -306,7 +307,9 @@ __device__ __forceinline__ unsigned long long jfa_list_tile(
       const int best = jfa_list_eval<UNIT_L, MODE, LT, BATCH, DM == 2>(lab, cx, cy, cz, seeds, L,
                                                                          s_list, base, bd);
       out[v] = best;
-      if (DM) dist_out[v] = bd;
+      // a voxel with no owner has no distance to carry: the next pass reads
+      // dist_in only under an owned incumbent
+      if (DM && best >= 0) dist_out[v] = bd;
       changed = (best != lab[13]);
     }
   }
@@ -393,8 +396,8 @@ __global__ void __launch_bounds__(256)
 // order are e ^ 7, e ^ 6, ..., e ^ 1, e ^ 0 = the voxel itself: the
 // incumbent, then those seven in that order, strict '<' (a repeat never
 // wins).  Right after the seeding most orbits hold no owner at all, so the
-// pass costs about its memory traffic.  DM: 1 also writes each voxel's
-// winning power distance (1e10 when no owner).
+// pass costs about its memory traffic.  DM: 1 also writes each owned
+// voxel's winning power distance.
 template <bool UNIT_L, int DM>
 __global__ void __launch_bounds__(256)
     k_jfa_half(const float4 *__restrict__ seeds, int res, float L, float half_l, float voxel,
@@ -441,7 +444,7 @@ __global__ void __launch_bounds__(256)
         }
       }
       out[vi[e]] = best;
-      if (DM) dist_out[vi[e]] = bd;
+      if (DM && best >= 0) dist_out[vi[e]] = bd;
       changed += best != lab[e];
     }
   }
