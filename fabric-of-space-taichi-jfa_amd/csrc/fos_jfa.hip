@@ -183,7 +183,13 @@ template <bool UNIT_L, int MODE, typename LT = int, int BATCH = JFA_BATCH, bool 
 __device__ __forceinline__ int jfa_list_eval(const int lab[27], float cx, float cy, float cz,
                                              const float4 *__restrict__ seeds, float L,
                                              LT (*s_list)[256], int base, float &bd) {
-  const int t = (int)threadIdx.x;
+  // This thread's list column.  Each thread owns its column, so any
+  // one-to-one column map is correct.  With 16-bit entries two columns share
+  // a dword; lanes l and l + 32 of a wave take the two halves of one dword
+  // (lane l's column 2l, lane l + 32's 2l + 1), so lanes that append at
+  // different rows in the same half-wave never meet in one bank.
+  const int tid = (int)threadIdx.x;
+  const int t = sizeof(LT) == 2 ? (tid & ~63) | ((tid & 31) << 1) | ((tid >> 5) & 1) : tid;
   const int inc = lab[13];
   const bool have = DIN && inc >= 0;
   // Branch-free list build: every candidate is written at the current
